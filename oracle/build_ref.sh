@@ -1,0 +1,66 @@
+#!/bin/bash
+# Builds the reference CPU solver (VAC 4.52) from its Fortran sources under
+# /root/reference/src into oracle/_ref/, one executable per configuration.
+#
+#   oracle/build_ref.sh NAME NDIMNDIR PHYS G1[,G2[,G3]]
+#   e.g. oracle/build_ref.sh vac_mhd22_132 22 mhd 132,132
+#
+# The recipe is this script, not the reference's setvac/Makefile:
+#  * the needed .t sources are copied to a throw-away directory under /tmp,
+#  * the physics module is chosen by copying vacpar.t.PHYS etc. to vacpar.t,
+#  * the compile-time grid (ixGhi) line of vacdef.t is rewritten,
+#  * two portability edits are applied (Perl >= 5.22 rejects defined(%hash)
+#    in vacpp.pl:322; vacio.t's flushunit uses the NAG-only F90_UNIX_IO module),
+#  * every .t file is expanded by the reference's own preprocessor
+#    (perl vacpp.pl -d=NDIMNDIR), whose module switches already default to
+#    cd,tvdlf,tvd on and mc,fct,impl,poisson,ct,gencoord,resist,rk,mpi off,
+#  * the .f files are compiled with flang, FMA contraction off, so the
+#    arithmetic is the plain IEEE sequence the source spells out.
+# Only the final executable is written into oracle/_ref/; the scratch copy is
+# deleted. Nothing from /root/reference is copied into the repository.
+set -euo pipefail
+NAME=$1; D=$2; PHYS=$3; G=$4
+REF=${VAC_REF_SRC:-/root/reference/src}
+HERE=$(cd "$(dirname "$0")" && pwd)
+OUT=$HERE/_ref
+FC=${FC:-/opt/rocm/lib/llvm/bin/flang}
+FFLAGS=${FFLAGS:--O2 -ffp-contract=off}
+[ -d "$REF" ] || { echo "reference sources not found at $REF" >&2; exit 2; }
+mkdir -p "$OUT"
+TMP=$(mktemp -d /tmp/vacref.XXXXXX)
+trap 'rm -rf "$TMP"' EXIT
+cd "$TMP"
+for f in vac.t vacio.t vacgrid.t vacgrid.setnozzle.t vacdef.t vaccd.t vactvdlf.t vactvd.t \
+         vacusr.t.nul vacusrpar.t.nul vacnul.specialbound.t vacnul.specialini.t \
+         vacnul.specialsource.t vacnul.specialio.t vacnul.addsource.t vacnul.process.t \
+         vacpar.t.$PHYS vacphys0.t.$PHYS vacphys.t.$PHYS vacpp.pl; do
+  cp "$REF/$f" .
+done
+for f in "$REF"/vacphys.*.t "$REF"/vacproc.*.t; do cp "$f" .; done
+chmod u+w ./*
+cp vacpar.t.$PHYS vacpar.t; cp vacphys0.t.$PHYS vacphys0.t; cp vacphys.t.$PHYS vacphys.t
+cp vacusr.t.nul vacusr.t; cp vacusrpar.t.nul vacusrpar.t
+# portability edit 1: Perl >= 5.22
+sed -i 's/defined(%shift0)/%shift0/' vacpp.pl
+# portability edit 2: F2003 flush statement instead of the NAG module
+perl -0pi -e 's/use F90_UNIX_IO,only : flush[^\n]*\n//; s/call flush\(qunit,ierror\)[^\n]*/flush(qunit)/' vacio.t
+# compile-time grid
+IFS=, read -r -a GA <<< "$G"
+line="INTEGER,PARAMETER:: "; gmin=${GA[0]}; gmax=${GA[0]}
+for i in "${!GA[@]}"; do
+  line+="ixGhi$((i+1))=${GA[$i]},"
+  [ "${GA[$i]}" -lt "$gmin" ] && gmin=${GA[$i]}
+  [ "${GA[$i]}" -gt "$gmax" ] && gmax=${GA[$i]}
+done
+line+="ixGhimin=$gmin,ixGhimax=$gmax"
+perl -pi -e "s/^INTEGER,PARAMETER:: ixGhi1=.*\$/$line/" vacdef.t
+for m in vacdef vacpar vacusrpar vac vacio vacgrid vacphys0 vacphys vacusr vaccd vactvdlf vactvd; do
+  perl vacpp.pl -d=$D $m.t > $m.f
+done
+objs=""
+for m in vac vacio vacgrid vacphys0 vacphys vacusr vaccd vactvdlf vactvd; do
+  $FC $FFLAGS -ffree-form -c $m.f -o $m.o 2> $m.log || { cat $m.log >&2; exit 1; }
+  objs+=" $m.o"
+done
+$FC $FFLAGS -o "$OUT/$NAME" $objs
+echo "built $OUT/$NAME"

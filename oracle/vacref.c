@@ -1,0 +1,1022 @@
+/*
+ * oracle/vacref.c -- CPU restatement of VAC 4.52's explicit advect path
+ * (TVDLF with the Hancock predictor, and one-step TVD with the Roe solver)
+ * for ideal MHD and HD on slab Cartesian grids.
+ *
+ * TEST INFRASTRUCTURE ONLY.  It is the checker the parity tests hold the
+ * HIP path against; the product library never links or calls it.
+ *
+ * It is written as VAC is: every routine is a sequence of passes over index
+ * regions of full-grid arrays, and every arithmetic expression keeps the
+ * left-to-right evaluation order of the Fortran statement it restates, so
+ * that with FMA contraction off (-ffp-contract=off on both sides) it agrees
+ * bitwise with a VAC build on the same input.  Each function names the
+ * reference lines it follows.  The equalities that pin it are listed in
+ * tests/test_oracle.py (reference VAC fixtures, roetest identity, limiter
+ * closed forms, invariants).
+ *
+ * Index convention: 0-based; ixG = [0, nx+3], mesh ixM = [2, nx+1].
+ */
+#include "vacref.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define BIGD 1.0e99     /* bigdouble, src/vacdef.t:51 */
+#define SMALLD 1.0e-99  /* smalldouble, src/vacdef.t:51 */
+
+typedef struct {
+  const vacref_params* p;
+  vacref_state* s;
+  int ndim, ndir, nw, mhd;
+  int nG[3];
+  long st[3], nc;
+  const double* dx[3];
+  int mlo[3], mhi[3];  /* mesh */
+  int glo[3], ghi[3];  /* grid */
+  int istep, nstep;
+  int firstsweep, lastsweep;
+  /* work arrays (nw planes unless noted) */
+  double *w1, *wold, *wLC, *wRC, *wroe, *tmpw;
+  double *f, *fC, *sc1, *sc2, *sc3, *sc4, *sc5; /* 1 plane each */
+  double *jump, *adtdx, *phi, *rphi, *ljump;    /* 1 plane each */
+  /* Roe common block (src/vacphys.mhdroe.t:17-18) and saved arrays */
+  double *cfast, *cslow, *afast, *aslow, *csound2, *dp, *rhodv;
+  double *bdv, *bdb, *bv, *v2a2;
+} ctx_t;
+
+/* variable indices (0-based), src/vacpar.t.mhd:9-11, src/vacpar.t.hd:9-10 */
+#define RHO 0
+#define M(c) (1 + (c))
+#define E_(x) (1 + (x)->ndir)
+#define B(x, c) (2 + (x)->ndir + (c))
+
+#define PL(x, w, iw) ((w) + (long)(iw) * (x)->nc)
+
+#define LOOP(lo, hi)                               \
+  for (int k = (lo)[2]; k <= (hi)[2]; ++k)         \
+    for (int j = (lo)[1]; j <= (hi)[1]; ++j)       \
+      for (int i = (lo)[0]; i <= (hi)[0]; ++i)
+#define IDX(x) ((long)i + (long)j * (x)->st[1] + (long)k * (x)->st[2])
+/* coordinate of the current cell along dimension d */
+#define CRD(d) ((d) == 0 ? i : ((d) == 1 ? j : k))
+
+static inline double dmax(double a, double b) { return a < b ? b : a; }
+static inline double dmin(double a, double b) { return b < a ? b : a; }
+/* Fortran SIGN(a,b) */
+static inline double fsign(double a, double b) { return copysign(fabs(a), b); }
+
+static inline double dvol(const ctx_t* x, long idx) {
+  /* dvolume = dx1*dx2[*dx3] (src/vacgrid.t:574-575) */
+  long r = idx;
+  int c[3] = {0, 0, 0};
+  for (int d = x->ndim - 1; d >= 0; --d) { c[d] = (int)(r / x->st[d]); r %= x->st[d]; }
+  double v = x->dx[0][c[0]];
+  for (int d = 1; d < x->ndim; ++d) v = v * x->dx[d][c[d]];
+  return v;
+}
+
+static inline double dxat(const ctx_t* x, int d, long idx) {
+  long r = idx;
+  for (int dd = x->ndim - 1; dd > d; --dd) r %= x->st[dd];
+  return x->dx[d][(int)(r / x->st[d])];
+}
+
+/* ---------------------------------------------------------------- limiter */
+/* dwlimiter2 (src/vacgrid.t:938-996) on a = dwC(i), b = dwC(i-1) */
+double vacref_limit(int type, double a, double b) {
+  double s = copysign(1.0, a);  /* tmp=sign(one,dwC) */
+  switch (type) {
+    case VACREF_MINMOD:  /* :968-971 */
+      return s * dmax(0.0, dmin(fabs(a), s * b));
+    case VACREF_WOODWARD: /* :972-976 */
+      return 2.0 * s * dmax(0.0, dmin(dmin(fabs(a), s * b), s * 0.25 * (b + a)));
+    case VACREF_SUPERBEE: /* :977-981 */
+      return s * dmax(dmax(0.0, dmin(2.0 * fabs(a), s * b)), dmin(fabs(a), 2.0 * s * b));
+  }
+  return 0.0;
+}
+
+/* ------------------------------------------------------------ boundaries */
+/* getboundary for 'periodic' and 'cont' (src/vacgrid.t:195-413): regions
+ * x-low, x-high, y-low, ... each spanning the full transverse ixG range. */
+static void getboundary(ctx_t* x, double* w) {
+  for (int d = 0; d < x->ndim; ++d) {
+    for (int up = 0; up < 2; ++up) {
+      int iB = 2 * d + up;
+      int lo[3], hi[3];
+      for (int dd = 0; dd < 3; ++dd) { lo[dd] = x->glo[dd]; hi[dd] = x->ghi[dd]; }
+      if (up) { lo[d] = x->mhi[d] + 1; hi[d] = x->ghi[d]; }
+      else { lo[d] = x->glo[d]; hi[d] = x->mlo[d] - 1; }
+      int n = x->p->nx[d];
+      for (int iw = 0; iw < x->nw; ++iw) {
+        double* pw = PL(x, w, iw);
+        int bc = x->p->bc[iB][iw];
+        LOOP(lo, hi) {
+          int c = CRD(d), src;
+          if (bc == VACREF_PERIODIC) src = up ? c - n : c + n;   /* :311-313, pairs :107-120 */
+          else src = up ? x->mhi[d] : x->mlo[d];                  /* 'cont' :251-273 */
+          long idx = IDX(x);
+          long sidx = idx + (long)(src - c) * x->st[d];
+          pw[idx] = pw[sidx];
+        }
+      }
+    }
+  }
+}
+
+/* ------------------------------------------------------------- physics */
+/* getpthermal (MHD src/vacphys0.t.mhd:165-222, HD src/vacphys0.t.hd:103-134) */
+static void getpthermal(ctx_t* x, int clip, const double* w, const int* lo,
+                        const int* hi, double* p) {
+  const double g = x->p->gamma;
+  const double* rho = PL(x, w, RHO);
+  const double* e = PL(x, w, E_(x));
+  LOOP(lo, hi) {
+    long idx = IDX(x);
+    double msq = 0.0;
+    for (int c = 0; c < x->ndir; ++c) {
+      double m = PL(x, w, M(c))[idx];
+      msq = (c == 0) ? m * m : msq + m * m;
+    }
+    double pv;
+    if (x->mhd) {
+      pv = msq / rho[idx];
+      for (int c = 0; c < x->ndir; ++c) {
+        double b = PL(x, w, B(x, c))[idx];
+        pv = pv + b * b;
+      }
+      pv = (g - 1.0) * (e[idx] - 0.5 * pv);
+    } else {
+      pv = (g - 1.0) * (e[idx] - 0.5 * msq / rho[idx]);
+    }
+    p[idx] = pv;
+  }
+  if (x->s->smallp < 0.0) {
+    double mn = BIGD, mx = -BIGD;
+    LOOP(lo, hi) { long idx = IDX(x); mn = dmin(mn, p[idx]); mx = dmax(mx, p[idx]); }
+    x->s->smallp = mn * x->p->smallpcoeff;
+    if (x->mhd && x->s->smallp < 0.0) x->s->smallp = mx * x->p->smallpcoeff;
+  }
+  if (clip) {
+    const double sp = x->s->smallp;
+    LOOP(lo, hi) { long idx = IDX(x); p[idx] = dmax(p[idx], sp); }
+  }
+}
+
+/* getcmax (MHD src/vacphys0.t.mhd:79-116 with getcsound2 :140-162;
+ * HD src/vacphys0.t.hd:72-100) */
+static void getcmax(ctx_t* x, const double* w, const int* lo, const int* hi,
+                    int idim, double* cmax) {
+  const double g = x->p->gamma;
+  const double* rho = PL(x, w, RHO);
+  double* cs2 = x->sc5;
+  getpthermal(x, 1, w, lo, hi, cs2);
+  if (x->mhd) {
+    LOOP(lo, hi) {
+      long idx = IDX(x);
+      double c2 = g * cs2[idx] / rho[idx];
+      double bsq = 0.0;
+      for (int c = 0; c < x->ndir; ++c) {
+        double b = PL(x, w, B(x, c))[idx];
+        bsq = (c == 0) ? b * b : bsq + b * b;
+      }
+      double cf2 = bsq / rho[idx] + c2;
+      double bi = PL(x, w, B(x, idim))[idx];
+      double mi = PL(x, w, M(idim))[idx];
+      cmax[idx] = sqrt(0.5 * (cf2 + sqrt(cf2 * cf2 - 4.0 * c2 * (bi * bi) / rho[idx])))
+                  + fabs(mi / rho[idx]);
+    }
+  } else {
+    LOOP(lo, hi) {
+      long idx = IDX(x);
+      double cs = sqrt(g * cs2[idx] / rho[idx]);
+      cmax[idx] = cs + fabs(PL(x, w, M(idim))[idx] / rho[idx]);
+    }
+  }
+}
+
+/* getflux + transport flux for one variable over a region.
+ * MHD src/vacphys.mhd0.t:149-205 with getptotal src/vacphys0.t.mhd:225-257;
+ * HD src/vacphys.t.hd:49-75.  Transport added as in e.g.
+ * src/vactvdlf.t:67-70: f = f + v*w(iw) with v = m_idim/rho (getv). */
+static void getflux_full(ctx_t* x, const double* w, const int* lo, const int* hi,
+                         int iw, int idim, double* f) {
+  const double g = x->p->gamma;
+  const double* rho = PL(x, w, RHO);
+  const double* e = PL(x, w, E_(x));
+  int transport = 1;
+  if (!x->mhd) {
+    if (iw == M(idim) || iw == E_(x)) getpthermal(x, 1, w, lo, hi, f);
+    LOOP(lo, hi) {
+      long idx = IDX(x);
+      if (iw == M(idim)) { /* f = p */ }
+      else if (iw == E_(x)) f[idx] = PL(x, w, M(idim))[idx] / rho[idx] * f[idx];
+      else f[idx] = 0.0;
+    }
+  } else {
+    int bdim = B(x, idim);
+    LOOP(lo, hi) {
+      long idx = IDX(x);
+      double r = rho[idx];
+      double val;
+      if (iw == RHO) {
+        val = 0.0;
+      } else if (iw >= M(0) && iw < M(x->ndir)) {
+        int c = iw - M(0);
+        double bc = PL(x, w, B(x, c))[idx], bi = PL(x, w, bdim)[idx];
+        if (c == idim) {
+          /* getptotal */
+          double bsq = 0.0, msq = 0.0;
+          for (int cc = 0; cc < x->ndir; ++cc) {
+            double b = PL(x, w, B(x, cc))[idx], m = PL(x, w, M(cc))[idx];
+            bsq = (cc == 0) ? b * b : bsq + b * b;
+            msq = (cc == 0) ? m * m : msq + m * m;
+          }
+          double pt = (1.0 - 0.5 * g) * bsq;
+          pt = pt + (g - 1.0) * e[idx];
+          pt = pt - 0.5 * (g - 1.0) * msq / r;
+          val = pt - bc * bi;
+        } else {
+          val = -(bc * bi);
+        }
+      } else if (iw == E_(x)) {
+        double bsq = 0.0, msq = 0.0, bm = 0.0;
+        for (int cc = 0; cc < x->ndir; ++cc) {
+          double b = PL(x, w, B(x, cc))[idx], m = PL(x, w, M(cc))[idx];
+          bsq = (cc == 0) ? b * b : bsq + b * b;
+          msq = (cc == 0) ? m * m : msq + m * m;
+          bm = (cc == 0) ? b * m : bm + b * m;
+        }
+        double pt = (1.0 - 0.5 * g) * bsq;
+        pt = pt + (g - 1.0) * e[idx];
+        pt = pt - 0.5 * (g - 1.0) * msq / r;
+        val = (PL(x, w, M(idim))[idx] * pt - PL(x, w, bdim)[idx] * bm) / r;
+      } else {
+        int c = iw - B(x, 0);
+        if (c == idim) { val = 0.0; }
+        else val = -(PL(x, w, M(c))[idx] / r * PL(x, w, bdim)[idx]);
+      }
+      f[idx] = val;
+    }
+    if (iw == bdim) transport = 0;
+  }
+  if (transport) {
+    const double* wi = PL(x, w, iw);
+    const double* mi = PL(x, w, M(idim));
+    LOOP(lo, hi) {
+      long idx = IDX(x);
+      double v = mi[idx] / rho[idx];
+      f[idx] = f[idx] + v * wi[idx];
+    }
+  }
+}
+
+/* addsource_divb (src/vacphys.mhd0.t:246-280) with getdivb (:106-146) and
+ * the slab gradient (src/vacgrid.t:816-823).  ensurebound (src/vacgrid.t:163-192)
+ * refills the ghost cells of wCT when ixI does not reach ixO +- 1. */
+static void addsource(ctx_t* x, double qdt, int ixI_full, const double* wCTc,
+                      double* wnew) {
+  if (!x->mhd) return;                  /* HD: vacnul.addsource.t */
+  if (!x->p->divbfix) return;
+  double* wCT = (double*)wCTc;
+  if (!ixI_full) getboundary(x, wCT);
+  const int* lo = x->mlo;
+  const int* hi = x->mhi;
+  double* divb = x->sc4;
+  LOOP(lo, hi) {
+    long idx = IDX(x);
+    double dv = 0.0;
+    for (int d = 0; d < x->ndim; ++d) {
+      const double* b = PL(x, wCT, B(x, d));
+      double gq = 0.5 * (b[idx + x->st[d]] - b[idx - x->st[d]]) / x->dx[d][CRD(d)];
+      dv = dv + gq;
+    }
+    divb[idx] = qdt * dv;
+  }
+  const double* rho = PL(x, wCT, RHO);
+  for (int iw = 0; iw < x->nw; ++iw) {
+    double* wn = PL(x, wnew, iw);
+    if (iw >= M(0) && iw < M(x->ndir)) {
+      const double* b = PL(x, wCT, B(x, iw - M(0)));
+      LOOP(lo, hi) { long idx = IDX(x); wn[idx] = wn[idx] - b[idx] * divb[idx]; }
+    } else if (iw >= B(x, 0) && iw < B(x, x->ndir)) {
+      const double* m = PL(x, wCT, M(iw - B(x, 0)));
+      LOOP(lo, hi) { long idx = IDX(x); wn[idx] = wn[idx] - m[idx] / rho[idx] * divb[idx]; }
+    } else if (iw == E_(x)) {
+      LOOP(lo, hi) {
+        long idx = IDX(x);
+        double mb = 0.0;
+        for (int c = 0; c < x->ndir; ++c) {
+          double t = PL(x, wCT, M(c))[idx] * PL(x, wCT, B(x, c))[idx];
+          mb = (c == 0) ? t : mb + t;
+        }
+        wn[idx] = wn[idx] - mb / rho[idx] * divb[idx];
+      }
+    }
+  }
+}
+
+/* ---------------------------------------------------------------- TVDLF */
+/* upwindLR (src/vactvdlf.t:302-367) for the common case ixR = ixL - shift
+ * (hancock) and ixR = ixL (tvdmusclf): the limited slope ldw(i) is
+ * dwlimiter2 of dwC(i) = w(i+1)-w(i) and dwC(i-1). */
+static double ldw_at(ctx_t* x, const double* w, int iw, int d, long idx) {
+  const double* pw = PL(x, w, iw);
+  long s = x->st[d];
+  double a = pw[idx + s] - pw[idx];
+  double b = pw[idx] - pw[idx - s];
+  return vacref_limit(x->p->limiter[iw], a, b);
+}
+
+/* hancock (src/vactvdlf.t:5-89): predictor wnew += -qdt/dx*(F(wL)-F(wR)) */
+static void hancock(ctx_t* x, double qdt, int d, const double* wCT, double* wnew) {
+  const int* lo = x->mlo;
+  const int* hi = x->mhi;
+  double* wL = x->wLC;  /* wLC(ixO) */
+  double* wR = x->wRC;  /* wRC(hxO) stored at the cell index (ixO) */
+  for (int iw = 0; iw < x->nw; ++iw) {
+    const double* pc = PL(x, wCT, iw);
+    double* pl = PL(x, wL, iw);
+    double* pr = PL(x, wR, iw);
+    LOOP(lo, hi) {
+      long idx = IDX(x);
+      double l = ldw_at(x, wCT, iw, d, idx);
+      pr[idx] = pc[idx] - 0.5 * l;
+      pl[idx] = pc[idx] + 0.5 * l;
+    }
+  }
+  double* fL = x->f;
+  double* fR = x->fC;
+  for (int iw = 0; iw < x->nw; ++iw) {
+    getflux_full(x, wR, lo, hi, iw, d, fR);
+    getflux_full(x, wL, lo, hi, iw, d, fL);
+    double* wn = PL(x, wnew, iw);
+    LOOP(lo, hi) {  /* addflux (src/vacgrid.t:676-677) */
+      long idx = IDX(x);
+      wn[idx] = wn[idx] - qdt / x->dx[d][CRD(d)] * (fL[idx] - fR[idx]);
+    }
+  }
+  if (x->p->sourceunsplit) addsource(x, qdt * 1.0 / x->ndim, 1, wCT, wnew);
+}
+
+/* tvdmusclf for method='tvdlf' (src/vactvdlf.t:92-299); wnew is the base
+ * of the limiter (typelimited='original') and receives the update. */
+static void tvdlf_corrector(ctx_t* x, double qdt, int d, const double* wCT, double* wnew) {
+  int lo[3], hi[3];
+  for (int dd = 0; dd < 3; ++dd) { lo[dd] = x->mlo[dd]; hi[dd] = x->mhi[dd]; }
+  lo[d] -= 1;  /* ixC = [ixOmin-1, ixOmax] */
+  long s = x->st[d];
+  double* wL = x->wLC;
+  double* wR = x->wRC;
+  for (int iw = 0; iw < x->nw; ++iw) {
+    const double* pc = PL(x, wCT, iw);
+    double* pl = PL(x, wL, iw);
+    double* pr = PL(x, wR, iw);
+    LOOP(lo, hi) {
+      long idx = IDX(x);
+      double lj = ldw_at(x, wnew, iw, d, idx);
+      double lj1 = ldw_at(x, wnew, iw, d, idx + s);
+      pr[idx] = pc[idx + s] - 0.5 * lj1;
+      pl[idx] = pc[idx] + 0.5 * lj;
+    }
+  }
+  /* mean state in wLC, cmax of it */
+  for (int iw = 0; iw < x->nw; ++iw) {
+    double* pl = PL(x, wL, iw);
+    const double* pr = PL(x, wR, iw);
+    LOOP(lo, hi) { long idx = IDX(x); pl[idx] = 0.5 * (pl[idx] + pr[idx]); }
+  }
+  double* cmaxC = x->sc1;
+  getcmax(x, wL, lo, hi, d, cmaxC);
+  if (x->p->courantpar > 0.0 && x->istep == x->nstep) {
+    double cm = -BIGD;
+    LOOP(lo, hi) {
+      long idx = IDX(x);
+      int c = CRD(d);
+      double v = cmaxC[idx] * 2.0 / (x->dx[d][c] + x->dx[d][c + 1]);
+      cm = dmax(cm, v);
+    }
+    if (qdt * cm > 1.0) x->s->nerror[2]++;
+    if (cm > SMALLD) x->s->dtcourant[d] = dmin(x->s->dtcourant[d], x->p->courantpar / cm);
+  }
+  for (int iw = 0; iw < x->nw; ++iw) {
+    double* pl = PL(x, wL, iw);
+    const double* pr = PL(x, wR, iw);
+    LOOP(lo, hi) { long idx = IDX(x); pl[idx] = 2.0 * pl[idx] - pr[idx]; }
+  }
+  double* fL = x->f;
+  double* fR = x->fC;
+  int olo[3], ohi[3];
+  for (int dd = 0; dd < 3; ++dd) { olo[dd] = x->mlo[dd]; ohi[dd] = x->mhi[dd]; }
+  for (int iw = 0; iw < x->nw; ++iw) {
+    getflux_full(x, wL, lo, hi, iw, d, fL);
+    getflux_full(x, wR, lo, hi, iw, d, fR);
+    const double* pl = PL(x, wL, iw);
+    const double* pr = PL(x, wR, iw);
+    LOOP(lo, hi) {
+      long idx = IDX(x);
+      double fm = 0.5 * (fL[idx] + fR[idx]);
+      double diss = -(cmaxC[idx] * 0.5 * (pr[idx] - pl[idx]));
+      fL[idx] = fm + diss;
+    }
+    double* wn = PL(x, wnew, iw);
+    LOOP(olo, ohi) {
+      long idx = IDX(x);
+      wn[idx] = wn[idx] - qdt / x->dx[d][CRD(d)] * (fL[idx] - fL[idx - s]);
+    }
+  }
+  /* ixI = ixO expanded by 2 in idim only, so ensurebound refills wCT ghosts */
+  if (x->p->sourceunsplit) addsource(x, qdt * 1.0 / x->ndim, x->ndim == 1, wCT, wnew);
+}
+
+/* ------------------------------------------------------------ MHD Roe */
+/* average (src/vacphys.mhdroe.t:4-122) over an interface region; wroe
+ * receives sqrt(rho), v, p, B_idim (clamped) and beta_idir[,beta_jdir]. */
+static void roe_average(ctx_t* x, const double* wL, const double* wR,
+                        const int* lo, const int* hi, int idim, double* wroe) {
+  const double g = x->p->gamma;
+  const int nd = x->ndir, eI = E_(x);
+  LOOP(lo, hi) {
+    long idx = IDX(x);
+    PL(x, wroe, RHO)[idx] = 0.5 * (PL(x, wL, RHO)[idx] + PL(x, wR, RHO)[idx]);
+    for (int c = 0; c < nd; ++c) {
+      PL(x, wroe, M(c))[idx] = 0.5 * (PL(x, wL, M(c))[idx] / PL(x, wL, RHO)[idx] +
+                                      PL(x, wR, M(c))[idx] / PL(x, wR, RHO)[idx]);
+      PL(x, wroe, B(x, c))[idx] = 0.5 * (PL(x, wL, B(x, c))[idx] + PL(x, wR, B(x, c))[idx]);
+    }
+  }
+  getpthermal(x, 1, wL, lo, hi, x->afast);
+  getpthermal(x, 1, wR, lo, hi, x->aslow);
+  LOOP(lo, hi) {
+    long idx = IDX(x);
+    double* r = PL(x, wroe, 0);
+    (void)r;
+    PL(x, wroe, eI)[idx] = 0.5 * (x->afast[idx] + x->aslow[idx]);
+    double s1 = 0.0, s2 = 0.0, s3 = 0.0;
+    for (int c = 0; c < nd; ++c) {
+      double v = PL(x, wroe, M(c))[idx];
+      double t = v * (PL(x, wR, M(c))[idx] - PL(x, wL, M(c))[idx]);
+      s1 = (c == 0) ? t : s1 + t;
+      s2 = (c == 0) ? v * v : s2 + v * v;
+      double bR = PL(x, wR, B(x, c))[idx], bL = PL(x, wL, B(x, c))[idx];
+      s3 = (c == 0) ? bR * bR - bL * bL : s3 + bR * bR - bL * bL;
+    }
+    double drho = PL(x, wR, RHO)[idx] - PL(x, wL, RHO)[idx];
+    double t = PL(x, wR, eI)[idx] - PL(x, wL, eI)[idx] - s1 + 0.5 * s2 * drho - 0.5 * s3;
+    x->dp[idx] = (g - 1.0) * t;
+    x->rhodv[idx] = PL(x, wR, M(idim))[idx] - PL(x, wL, M(idim))[idx] -
+                    PL(x, wroe, M(idim))[idx] * drho;
+    /* getcsound2prim (src/vacphys0.t.mhd:119-137) */
+    double c2 = g * PL(x, wroe, eI)[idx] / PL(x, wroe, RHO)[idx];
+    x->csound2[idx] = c2;
+    double bsq = 0.0;
+    for (int c = 0; c < nd; ++c) {
+      double b = PL(x, wroe, B(x, c))[idx];
+      bsq = (c == 0) ? b * b : bsq + b * b;
+    }
+    double rr = PL(x, wroe, RHO)[idx];
+    double bi = PL(x, wroe, B(x, idim))[idx];
+    double cf = bsq / rr + c2;
+    double cs = dmax(0.0, 0.5 * (cf - sqrt(cf * cf - 4.0 * c2 * (bi * bi) / rr)));
+    cf = cf - cs;
+    double af = (c2 - cs) / (cf - cs);
+    af = dmin(1.0, dmax(af, 0.0));
+    x->aslow[idx] = sqrt(1.0 - af);
+    x->afast[idx] = sqrt(af);
+    x->cfast[idx] = sqrt(cf);
+    x->cslow[idx] = sqrt(cs);
+    PL(x, wroe, RHO)[idx] = sqrt(rr);
+    if (fabs(bi) < SMALLD) PL(x, wroe, B(x, idim))[idx] = SMALLD;
+    int idir = (idim + 1) % nd;
+    if (nd == 2) {
+      double* bb = PL(x, wroe, B(x, idir));
+      bb[idx] = (bb[idx] >= 0.0) ? 1.0 : -1.0;
+    } else {
+      int jdir = (idir + 1) % nd;
+      double bI = PL(x, wroe, B(x, idir))[idx], bJ = PL(x, wroe, B(x, jdir))[idx];
+      double tt = sqrt(bI * bI + bJ * bJ);
+      if (tt > SMALLD) {
+        PL(x, wroe, B(x, idir))[idx] = bI / tt;
+        PL(x, wroe, B(x, jdir))[idx] = bJ / tt;
+      } else {
+        PL(x, wroe, B(x, idir))[idx] = sqrt(0.5);
+        PL(x, wroe, B(x, jdir))[idx] = sqrt(0.5);
+      }
+    }
+  }
+}
+
+/* geteigenjump (src/vacphys.mhdroe.t:125-284), entropy fix 'nul' */
+static void roe_eigenjump(ctx_t* x, const double* wL, const double* wR,
+                          const double* wroe, const int* lo, const int* hi,
+                          int il, int idim, double* a, double* jump) {
+  const int nd = x->ndir;
+  int idir = (idim + 1) % nd, jdir = (idir + 1) % nd;
+  const double* sq = PL(x, wroe, RHO);
+  const double* vd = PL(x, wroe, M(idim));
+  const double* bd = PL(x, wroe, B(x, idim));
+  if (il == 0) {  /* fastRW_ */
+    LOOP(lo, hi) {
+      long idx = IDX(x);
+      double t = PL(x, wroe, B(x, idir))[idx] *
+                 (PL(x, wR, M(idir))[idx] / PL(x, wR, RHO)[idx] -
+                  PL(x, wL, M(idir))[idx] / PL(x, wL, RHO)[idx]);
+      if (nd == 3)
+        t = t + PL(x, wroe, B(x, jdir))[idx] *
+                    (PL(x, wR, M(jdir))[idx] / PL(x, wR, RHO)[idx] -
+                     PL(x, wL, M(jdir))[idx] / PL(x, wL, RHO)[idx]);
+      x->bdv[idx] = t * fsign(sq[idx] * sq[idx], bd[idx]);
+      double u = PL(x, wroe, B(x, idir))[idx] *
+                 (PL(x, wR, B(x, idir))[idx] - PL(x, wL, B(x, idir))[idx]);
+      if (nd == 3)
+        u = u + PL(x, wroe, B(x, jdir))[idx] *
+                    (PL(x, wR, B(x, jdir))[idx] - PL(x, wL, B(x, jdir))[idx]);
+      x->bdb[idx] = u * sqrt(x->csound2[idx]) * sq[idx];
+    }
+  }
+  if (il == 6) {  /* alfvRW_ */
+    LOOP(lo, hi) {
+      long idx = IDX(x);
+      double bI = PL(x, wroe, B(x, idir))[idx], bJ = PL(x, wroe, B(x, jdir))[idx];
+      double t = bJ * (PL(x, wR, M(idir))[idx] / PL(x, wR, RHO)[idx] -
+                       PL(x, wL, M(idir))[idx] / PL(x, wL, RHO)[idx]) -
+                 bI * (PL(x, wR, M(jdir))[idx] / PL(x, wR, RHO)[idx] -
+                       PL(x, wL, M(jdir))[idx] / PL(x, wL, RHO)[idx]);
+      double u = bJ * (PL(x, wR, B(x, idir))[idx] - PL(x, wL, B(x, idir))[idx]) -
+                 bI * (PL(x, wR, B(x, jdir))[idx] - PL(x, wL, B(x, jdir))[idx]);
+      x->bdv[idx] = t * 0.5 * (sq[idx] * sq[idx]);
+      x->bdb[idx] = u * 0.5 * fsign(sq[idx], bd[idx]);
+    }
+  }
+  LOOP(lo, hi) {
+    long idx = IDX(x);
+    double cf = x->cfast[idx], cs = x->cslow[idx], af = x->afast[idx], as = x->aslow[idx];
+    double c2 = x->csound2[idx], dpv = x->dp[idx], rdv = x->rhodv[idx];
+    double bdv = x->bdv[idx], bdb = x->bdb[idx];
+    switch (il) {
+      case 0:
+        a[idx] = vd[idx] + cf;
+        jump[idx] = 0.5 / c2 * (af * (+cf * rdv + dpv) + as * (-cs * bdv + bdb));
+        break;
+      case 1:
+        a[idx] = vd[idx] - cf;
+        jump[idx] = 0.5 / c2 * (af * (-cf * rdv + dpv) + as * (+cs * bdv + bdb));
+        break;
+      case 2:
+        a[idx] = vd[idx] + cs;
+        jump[idx] = 0.5 / c2 * (as * (+cs * rdv + dpv) + af * (+cf * bdv - bdb));
+        break;
+      case 3:
+        a[idx] = vd[idx] - cs;
+        jump[idx] = 0.5 / c2 * (as * (-cs * rdv + dpv) + af * (-cf * bdv - bdb));
+        break;
+      case 4:
+        a[idx] = vd[idx];
+        jump[idx] = PL(x, wR, RHO)[idx] - PL(x, wL, RHO)[idx] - dpv / c2;
+        break;
+      case 5:
+        if (x->p->divbwave) {
+          a[idx] = vd[idx];
+          jump[idx] = PL(x, wR, B(x, idim))[idx] - PL(x, wL, B(x, idim))[idx];
+        } else {
+          a[idx] = 0.0;
+          jump[idx] = 0.0;
+        }
+        break;
+      case 6:
+        a[idx] = vd[idx] + fabs(bd[idx]) / sq[idx];
+        jump[idx] = +bdv - bdb;
+        break;
+      case 7:
+        a[idx] = vd[idx] - fabs(bd[idx]) / sq[idx];
+        jump[idx] = -bdv - bdb;
+        break;
+    }
+  }
+}
+
+/* rtimes (src/vacphys.mhdroe.t:287-450): rq = R(iw,il) * q */
+static void roe_rtimes(ctx_t* x, const double* q, const double* wroe, const int* lo,
+                       const int* hi, int iw, int il, int idim, double* rq) {
+  const int nd = x->ndir;
+  const double g = x->p->gamma;
+  int idir = (idim + 1) % nd, jdir = (idir + 1) % nd;
+  const double* sq = PL(x, wroe, RHO);
+  const double* vd = PL(x, wroe, M(idim));
+  const double* bd = PL(x, wroe, B(x, idim));
+  const int eI = E_(x);
+  if (iw == eI) {
+    if (il == 0) {
+      LOOP(lo, hi) {
+        long idx = IDX(x);
+        double vs = 0.0;
+        for (int c = 0; c < nd; ++c) {
+          double v = PL(x, wroe, M(c))[idx];
+          vs = (c == 0) ? v * v : vs + v * v;
+        }
+        x->v2a2[idx] = 0.5 * vs + (2.0 - g) / (g - 1.0) * x->csound2[idx];
+        double t = PL(x, wroe, B(x, idir))[idx] * PL(x, wroe, M(idir))[idx];
+        if (nd == 3) t = t + PL(x, wroe, B(x, jdir))[idx] * PL(x, wroe, M(jdir))[idx];
+        x->bv[idx] = t * copysign(1.0, bd[idx]);
+      }
+    } else if (il == 6) {
+      LOOP(lo, hi) {
+        long idx = IDX(x);
+        x->bv[idx] = PL(x, wroe, B(x, jdir))[idx] * PL(x, wroe, M(idir))[idx] -
+                     PL(x, wroe, B(x, idir))[idx] * PL(x, wroe, M(jdir))[idx];
+      }
+    }
+  }
+  LOOP(lo, hi) {
+    long idx = IDX(x);
+    double qq = q[idx];
+    double cf = x->cfast[idx], cs = x->cslow[idx], af = x->afast[idx], as = x->aslow[idx];
+    double r = 0.0;
+    if (iw == RHO) {
+      switch (il) {
+        case 0: case 1: r = qq * af; break;
+        case 2: case 3: r = qq * as; break;
+        case 4: r = qq; break;
+        default: r = 0.0;
+      }
+    } else if (iw == eI) {
+      double bv = x->bv[idx], v2a2 = x->v2a2[idx];
+      switch (il) {
+        case 0: r = qq * (-as * cs * bv + af * (v2a2 + cf * (cf + vd[idx]))); break;
+        case 1: r = qq * (+as * cs * bv + af * (v2a2 + cf * (cf - vd[idx]))); break;
+        case 2: r = qq * (+af * cf * bv + as * (v2a2 + cs * (cs + vd[idx]))); break;
+        case 3: r = qq * (-af * cf * bv + as * (v2a2 + cs * (cs - vd[idx]))); break;
+        case 4: {
+          double vs = 0.0;
+          for (int c = 0; c < nd; ++c) {
+            double v = PL(x, wroe, M(c))[idx];
+            vs = (c == 0) ? v * v : vs + v * v;
+          }
+          r = qq * 0.5 * vs;
+          break;
+        }
+        case 5: r = x->p->divbwave ? qq * bd[idx] : 0.0; break;
+        case 6: r = +qq * bv; break;
+        case 7: r = -qq * bv; break;
+      }
+    } else if (iw >= M(0) && iw < M(nd)) {
+      int c = iw - M(0);
+      double vw = PL(x, wroe, iw)[idx];
+      if (c == idim) {
+        switch (il) {
+          case 0: r = qq * af * (vw + cf); break;
+          case 1: r = qq * af * (vw - cf); break;
+          case 2: r = qq * as * (vw + cs); break;
+          case 3: r = qq * as * (vw - cs); break;
+          case 4: r = qq * vw; break;
+          default: r = 0.0;
+        }
+      } else {
+        double beta = PL(x, wroe, B(x, c))[idx];
+        double sg = copysign(1.0, bd[idx]);
+        switch (il) {
+          case 0: r = qq * (af * vw - as * cs * beta * sg); break;
+          case 1: r = qq * (af * vw + as * cs * beta * sg); break;
+          case 2: r = qq * (as * vw + af * cf * beta * sg); break;
+          case 3: r = qq * (as * vw - af * cf * beta * sg); break;
+          case 4: r = qq * vw; break;
+          case 5: r = 0.0; break;
+          case 6:
+            r = (c == idir) ? +qq * PL(x, wroe, B(x, jdir))[idx]
+                            : -qq * PL(x, wroe, B(x, idir))[idx];
+            break;
+          case 7:
+            r = (c == idir) ? -qq * PL(x, wroe, B(x, jdir))[idx]
+                            : +qq * PL(x, wroe, B(x, idir))[idx];
+            break;
+        }
+      }
+    } else {
+      int c = iw - B(x, 0);
+      if (c == idim) {
+        r = (il == 5 && x->p->divbwave) ? qq : 0.0;
+      } else {
+        double beta = PL(x, wroe, iw)[idx];
+        switch (il) {
+          case 0: case 1: r = +qq * as * sqrt(x->csound2[idx]) * beta / sq[idx]; break;
+          case 2: case 3: r = -(qq * af * sqrt(x->csound2[idx]) * beta / sq[idx]); break;
+          case 4: case 5: r = 0.0; break;
+          case 6: case 7:
+            if (c == idir) r = -(qq * PL(x, wroe, B(x, jdir))[idx] / fsign(sq[idx], bd[idx]));
+            else r = +qq * PL(x, wroe, B(x, idir))[idx] / fsign(sq[idx], bd[idx]);
+            break;
+        }
+      }
+    }
+    rq[idx] = r;
+  }
+}
+
+/* tvdlimit2 (src/vactvd.t:86-206) with getphi for typetvd='roe' (:209-302),
+ * typeentropy='nul', acmcoef<0, acmexpo=0. */
+static void tvdlimit2(ctx_t* x, double qdt, int d, const double* wL, const double* wR,
+                      double* wnew) {
+  int icl[3], ich[3], cl[3], ch[3];
+  for (int dd = 0; dd < 3; ++dd) {
+    icl[dd] = cl[dd] = x->mlo[dd];
+    ich[dd] = ch[dd] = x->mhi[dd];
+  }
+  icl[d] = x->mlo[d] - 2; ich[d] = x->mhi[d] + 1;  /* ixIC */
+  cl[d] = x->mlo[d] - 1;                            /* ixC */
+  long s = x->st[d];
+  roe_average(x, wL, wR, icl, ich, d, x->wroe);
+  for (int il = 0; il < x->nw; ++il) {
+    roe_eigenjump(x, wL, wR, x->wroe, icl, ich, il, d, x->adtdx, x->jump);
+    LOOP(icl, ich) {
+      long idx = IDX(x);
+      int c = CRD(d);
+      x->adtdx[idx] = x->adtdx[idx] * qdt * 2.0 / (x->dx[d][c] + x->dx[d][c + 1]);
+    }
+    if ((il == 0 || il == 1) && x->p->courantpar > 0.0 && x->istep == x->nstep) {
+      double cm = 0.0;
+      int first = 1;
+      LOOP(icl, ich) {
+        long idx = IDX(x);
+        double v = fabs(x->adtdx[idx]);
+        cm = first ? v : dmax(cm, v);
+        first = 0;
+      }
+      if (cm > 1.0) x->s->nerror[2]++;
+      if (cm > SMALLD) x->s->dtcourant[d] = dmin(x->s->dtcourant[d], qdt * x->p->courantpar / cm);
+    }
+    /* getphi, method 'tvd' */
+    LOOP(icl, ich) {
+      long idx = IDX(x);
+      double av = x->adtdx[idx];
+      x->phi[idx] = fabs(av) - av * av;
+    }
+    int ll[3];
+    for (int dd = 0; dd < 3; ++dd) ll[dd] = icl[dd];
+    ll[d] = icl[d] + 1;
+    LOOP(ll, ich) {
+      long idx = IDX(x);
+      x->ljump[idx] = vacref_limit(x->p->limiter[il], x->jump[idx], x->jump[idx - s]);
+    }
+    LOOP(cl, ch) {
+      long idx = IDX(x);
+      double av = x->adtdx[idx];
+      if (av <= 0.0) x->phi[idx] = x->phi[idx] * (x->jump[idx] - x->ljump[idx + s]);
+      else x->phi[idx] = x->phi[idx] * (x->jump[idx] - x->ljump[idx]);
+      x->phi[idx] = x->phi[idx] + av * av * x->jump[idx];
+    }
+    for (int iw = 0; iw < x->nw; ++iw) {
+      roe_rtimes(x, x->phi, x->wroe, cl, ch, iw, il, d, x->rphi);
+      LOOP(cl, ch) {
+        long idx = IDX(x);
+        x->rphi[idx] = x->rphi[idx] * 0.25 * (dvol(x, idx + s) + dvol(x, idx));
+      }
+      double* wn = PL(x, wnew, iw);
+      LOOP(x->mlo, x->mhi) {  /* addflux with qdt<0 (src/vacgrid.t:716) */
+        long idx = IDX(x);
+        wn[idx] = wn[idx] + (x->rphi[idx] - x->rphi[idx - s]) / dvol(x, idx);
+      }
+    }
+  }
+}
+
+/* centdiff (src/vaccd.t:5-147), slab, the TVD pre-processing branch */
+static void centdiff(ctx_t* x, double qdt, int d, double* wCT, double* w) {
+  int tvdpreproc = (x->p->typefull == VACREF_TVD) && x->p->sourceunsplit;
+  if (tvdpreproc && x->firstsweep) {
+    if (x->p->sourceunsplit) addsource(x, qdt / 2, 1, w, wCT);
+    getboundary(x, wCT);
+  }
+  int lo[3], hi[3];
+  for (int dd = 0; dd < 3; ++dd) { lo[dd] = x->mlo[dd]; hi[dd] = x->mhi[dd]; }
+  lo[d] -= 1; hi[d] += 1;  /* ix = ixO +- 1 */
+  long s = x->st[d];
+  for (int iw = 0; iw < x->nw; ++iw) {
+    getflux_full(x, wCT, lo, hi, iw, d, x->f);
+    int cl[3], ch[3];
+    for (int dd = 0; dd < 3; ++dd) { cl[dd] = x->mlo[dd]; ch[dd] = x->mhi[dd]; }
+    cl[d] -= 1;
+    LOOP(cl, ch) {
+      long idx = IDX(x);
+      x->fC[idx] = 0.5 * (x->f[idx] + x->f[idx + s]);
+    }
+    double* wn = PL(x, w, iw);
+    LOOP(x->mlo, x->mhi) {
+      long idx = IDX(x);
+      wn[idx] = wn[idx] - qdt / x->dx[d][CRD(d)] * (x->fC[idx] - x->fC[idx - s]);
+    }
+  }
+}
+
+/* tvdlimit (src/vactvd.t:5-83) including the Ryu source step */
+static void tvdlimit(ctx_t* x, double qdt, int d, double* wCT, double* wnew) {
+  /* wL = wCT, wR = wCT shifted by one cell along d */
+  long s = x->st[d];
+  for (int iw = 0; iw < x->nw; ++iw) {
+    const double* pc = PL(x, wCT, iw);
+    double* pr = PL(x, x->wRC, iw);
+    for (long idx = 0; idx + s < x->nc; ++idx) pr[idx] = pc[idx + s];
+  }
+  tvdlimit2(x, qdt, d, wCT, x->wRC, wnew);
+  if (x->lastsweep && x->p->sourceunsplit) {
+    /* Ryu et al. (ApJ 452, 364): whalf=(wold+whydro+S)/2, wnew=whydro+S(whalf) */
+    double* wt = wCT;  /* VAC overwrites its w argument (the wCT array) */
+    for (int iw = 0; iw < x->nw; ++iw) {
+      double* a = PL(x, wt, iw);
+      const double* b = PL(x, wnew, iw);
+      LOOP(x->mlo, x->mhi) { long idx = IDX(x); a[idx] = b[idx]; }
+    }
+    addsource(x, qdt, 1, x->wold, wt);
+    for (int iw = 0; iw < x->nw; ++iw) {
+      double* a = PL(x, wt, iw);
+      const double* o = PL(x, x->wold, iw);
+      LOOP(x->mlo, x->mhi) { long idx = IDX(x); a[idx] = 0.5 * (o[idx] + a[idx]); }
+    }
+    addsource(x, qdt, 0, wt, wnew);
+  }
+}
+
+/* -------------------------------------------------------- time stepping */
+/* advect1 (src/vac.t:489-572) */
+static void advect1(ctx_t* x, int method, double qdt, int d, double* wCT, double* w) {
+  x->istep += 1;
+  switch (method) {
+    case 0: hancock(x, qdt, d, wCT, w); break;           /* 'hancock' */
+    case 1: tvdlf_corrector(x, qdt, d, wCT, w); break;   /* 'tvdlf' */
+    case 2: centdiff(x, qdt, d, wCT, w); tvdlimit(x, qdt, d, wCT, w); break;  /* 'tvd' */
+  }
+  getboundary(x, w);
+}
+
+/* advect (src/vac.t:363-486) */
+static void advect(ctx_t* x, int d, double* w) {
+  memcpy(x->w1, w, sizeof(double) * x->nc * x->nw);
+  x->istep = 0;
+  double dt = x->s->dt;
+  if (x->p->typefull == VACREF_TVDLF) {
+    advect1(x, 0, dt / 2, d, w, x->w1);
+    advect1(x, 1, dt, d, x->w1, w);
+  } else {
+    advect1(x, 2, dt, d, x->w1, w);
+  }
+  x->firstsweep = 0;
+}
+
+/* advance_expl with dimsplit (src/vac.t:283-360) */
+static void advance(ctx_t* x, double* w) {
+  x->firstsweep = 1;
+  int it = x->s->it;
+  if ((it / 2) * 2 == it || x->p->dimsplit_xy) {
+    for (int d = 0; d < x->ndim; ++d) { x->lastsweep = (d == x->ndim - 1); advect(x, d, w); }
+  } else {
+    for (int d = x->ndim - 1; d >= 0; --d) { x->lastsweep = (d == 0); advect(x, d, w); }
+  }
+}
+
+/* getdt_courant (src/vac.t:664-722) */
+static void getdt_courant(ctx_t* x, const double* w) {
+  double dt = BIGD, cm = 0.0;
+  for (int d = 0; d < x->ndim; ++d) {
+    if (x->s->dtcourant[d] < BIGD) {
+      dt = dmin(dt, x->s->dtcourant[d]);
+    } else {
+      getcmax(x, w, x->mlo, x->mhi, d, x->sc1);
+      double mx = -BIGD;
+      const int* lo = x->mlo;
+      const int* hi = x->mhi;
+      LOOP(lo, hi) {
+        long idx = IDX(x);
+        mx = dmax(mx, x->sc1[idx] / x->dx[d][CRD(d)]);
+      }
+      cm = dmax(cm, mx);
+    }
+  }
+  for (int d = 0; d < x->ndim; ++d) x->s->dtcourant[d] = BIGD;
+  if (cm > SMALLD) dt = dmin(dt, x->p->courantpar / cm);
+  x->s->dt = dt;
+}
+
+static int ctx_init(ctx_t* x, const vacref_params* p, vacref_state* s,
+                    const double* dx0, const double* dx1, const double* dx2) {
+  memset(x, 0, sizeof(*x));
+  if (p->ndim < 1 || p->ndim > 3) return -1;
+  x->p = p;
+  x->s = s;
+  x->ndim = p->ndim;
+  x->ndir = p->ndim;
+  x->mhd = (p->phys == VACREF_MHD);
+  x->nw = x->mhd ? 2 + 2 * x->ndir : 2 + x->ndim;
+  if (!x->mhd && p->typefull == VACREF_TVD) return -2;  /* HD Roe solver not restated */
+  const double* dxs[3] = {dx0, dx1, dx2};
+  long st = 1;
+  for (int d = 0; d < 3; ++d) {
+    if (d < x->ndim) {
+      if (p->nx[d] < 2) return -1;
+      x->nG[d] = p->nx[d] + 4;
+      x->mlo[d] = 2; x->mhi[d] = p->nx[d] + 1;
+      x->glo[d] = 0; x->ghi[d] = p->nx[d] + 3;
+      x->dx[d] = dxs[d];
+      x->st[d] = st;
+      st *= x->nG[d];
+    } else {
+      x->nG[d] = 1; x->mlo[d] = x->mhi[d] = x->glo[d] = x->ghi[d] = 0;
+      x->st[d] = 0;
+    }
+  }
+  x->nc = st;
+  x->nstep = (p->typefull == VACREF_TVDLF) ? 2 : 1;
+  size_t pl = sizeof(double) * (size_t)x->nc;
+  x->w1 = calloc(x->nw, pl);
+  x->wold = calloc(x->nw, pl);
+  x->wLC = calloc(x->nw, pl);
+  x->wRC = calloc(x->nw, pl);
+  x->wroe = calloc(x->nw, pl);
+  x->tmpw = calloc(x->nw, pl);
+  double** one[] = {&x->f, &x->fC, &x->sc1, &x->sc2, &x->sc3, &x->sc4, &x->sc5,
+                    &x->jump, &x->adtdx, &x->phi, &x->rphi, &x->ljump,
+                    &x->cfast, &x->cslow, &x->afast, &x->aslow, &x->csound2, &x->dp,
+                    &x->rhodv, &x->bdv, &x->bdb, &x->bv, &x->v2a2};
+  for (size_t q = 0; q < sizeof(one) / sizeof(one[0]); ++q) *one[q] = calloc(1, pl);
+  return 0;
+}
+
+static void ctx_free(ctx_t* x) {
+  double* all[] = {x->w1, x->wold, x->wLC, x->wRC, x->wroe, x->tmpw, x->f, x->fC,
+                   x->sc1, x->sc2, x->sc3, x->sc4, x->sc5, x->jump, x->adtdx, x->phi,
+                   x->rphi, x->ljump, x->cfast, x->cslow, x->afast, x->aslow,
+                   x->csound2, x->dp, x->rhodv, x->bdv, x->bdb, x->bv, x->v2a2};
+  for (size_t q = 0; q < sizeof(all) / sizeof(all[0]); ++q) free(all[q]);
+}
+
+int vacref_getdt(const vacref_params* p, vacref_state* s, const double* w,
+                 const double* dx0, const double* dx1, const double* dx2) {
+  ctx_t x;
+  int rc = ctx_init(&x, p, s, dx0, dx1, dx2);
+  if (rc) return rc;
+  getdt_courant(&x, w);
+  ctx_free(&x);
+  return 0;
+}
+
+/* main loop (src/vac.t:54-106) */
+int vacref_run(const vacref_params* p, vacref_state* s, double* w,
+               const double* dx0, const double* dx1, const double* dx2,
+               int nsteps, double* t_hist) {
+  ctx_t x;
+  int rc = ctx_init(&x, p, s, dx0, dx1, dx2);
+  if (rc) return rc;
+  if (p->dtpar > 0.0 && s->smallp < 0.0) { ctx_free(&x); return -3; }
+  getboundary(&x, w);
+  int done = 0;
+  for (;;) {
+    if (t_hist) t_hist[done] = s->t;
+    if (p->dtpar > 0.0) {
+      s->dt = p->dtpar;
+    } else if (p->courantpar > 0.0) {
+      getdt_courant(&x, w);
+    }
+    if (s->it >= p->itmax || s->t >= p->tmax || s->dt < 1.0e-99 || done >= nsteps) break;
+    if (p->tmaxexact) s->dt = dmin(s->dt, p->tmax - s->t + SMALLD);
+    memcpy(x.wold, w, sizeof(double) * x.nc * x.nw);
+    advance(&x, w);
+    s->it += 1;
+    s->t = s->t + s->dt;
+    done += 1;
+  }
+  ctx_free(&x);
+  return done;
+}
+
+/* roetest (src/roetest.t:38-57) on a single interface */
+int vacref_roe_identity(int ndim, double gamma, int idim, int divbwave, double smallp,
+                        const double* wL, const double* wR, double* out, double* a) {
+  vacref_params p;
+  memset(&p, 0, sizeof p);
+  p.ndim = ndim; p.phys = VACREF_MHD; p.gamma = gamma; p.typefull = VACREF_TVD;
+  p.divbwave = divbwave; p.smallpcoeff = 1e-6;
+  for (int d = 0; d < ndim; ++d) p.nx[d] = 2;
+  vacref_state s;
+  memset(&s, 0, sizeof s);
+  s.smallp = smallp;
+  double dxa[6] = {1, 1, 1, 1, 1, 1};
+  ctx_t x;
+  if (ctx_init(&x, &p, &s, dxa, dxa, dxa)) return -1;
+  /* one interface at cell 0 */
+  double* L = x.tmpw;
+  double* R = x.wRC;
+  for (int iw = 0; iw < x.nw; ++iw) { PL(&x, L, iw)[0] = wL[iw]; PL(&x, R, iw)[0] = wR[iw]; }
+  int lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
+  roe_average(&x, L, R, lo, hi, idim, x.wroe);
+  for (int iw = 0; iw < x.nw; ++iw) out[iw] = 0.0;
+  for (int il = 0; il < x.nw; ++il) {
+    roe_eigenjump(&x, L, R, x.wroe, lo, hi, il, idim, x.adtdx, x.jump);
+    a[il] = x.adtdx[0];
+    for (int iw = 0; iw < x.nw; ++iw) {
+      roe_rtimes(&x, x.jump, x.wroe, lo, hi, iw, il, idim, x.rphi);
+      out[iw] += x.rphi[0];
+    }
+  }
+  ctx_free(&x);
+  return 0;
+}

@@ -1,0 +1,76 @@
+/*
+ * oracle/vacref.h -- CPU restatement of VAC's explicit TVDLF / TVD (Roe)
+ * advect path for ideal MHD and HD.  TEST INFRASTRUCTURE ONLY: tests, the
+ * smoke check and the CPU-baseline leg of bench.py use it as the checker;
+ * the product (libvacgpu) never links or calls it.
+ *
+ * The state layout is VAC's w(ixG^T, nw) with the leading dimensions equal
+ * to the actual grid: nw planes of prod(nx[d]+4) doubles, x fastest.
+ * Mesh cells are indices [2, nx[d]+1] (0-based); two ghost layers per side.
+ */
+#ifndef VACREF_H
+#define VACREF_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum { VACREF_HD = 0, VACREF_MHD = 1 };
+enum { VACREF_TVDLF = 0, VACREF_TVD = 1 };
+enum { VACREF_MINMOD = 0, VACREF_WOODWARD = 1, VACREF_SUPERBEE = 2 };
+enum { VACREF_CONT = 0, VACREF_PERIODIC = 1 };
+
+typedef struct {
+  int ndim;              /* 1..3; ndir == ndim (setvac -d=11/22/33) */
+  int phys;              /* VACREF_HD / VACREF_MHD */
+  int nx[3];             /* mesh cells per dimension */
+  double gamma;          /* eqpar(gamma_) */
+  int typefull;          /* VACREF_TVDLF (twostep, hancock) / VACREF_TVD (onestep) */
+  int limiter[8];        /* typelimiter(1:nw) */
+  int bc[6][8];          /* typeB(iw, iB), iB = 2*d + upper */
+  int divbfix, divbwave, sourceunsplit;
+  int dimsplit_xy;       /* typedimsplit='xy' (else 'xyyx') */
+  double courantpar;
+  double smallpcoeff;
+  double dtpar;          /* > 0: fixed time step */
+  double tmax;
+  int tmaxexact;
+  int itmax;
+} vacref_params;
+
+typedef struct {
+  int it, itmin;
+  double t, dt;
+  double smallp;         /* < 0: set on the first pressure evaluation */
+  double dtcourant[3];
+  int nerror[4];
+} vacref_state;
+
+/* Main loop of src/vac.t:55-106 for up to nsteps steps (and the itmax/tmax
+ * stop conditions).  w holds the full ixG state (ghost cells are filled on
+ * entry, as VAC does at src/vac.t:54).  dx[d] are the nx[d]+4 cell widths.
+ * t_hist (optional, nsteps+1 doubles) receives t before each step and after
+ * the last one.  Returns the number of steps taken, or -1 on bad input. */
+int vacref_run(const vacref_params* p, vacref_state* s, double* w,
+               const double* dx0, const double* dx1, const double* dx2,
+               int nsteps, double* t_hist);
+
+/* One pass of VAC's getdt_courant (src/vac.t:664-722): sets s->dt. */
+int vacref_getdt(const vacref_params* p, vacref_state* s, const double* w,
+                 const double* dx0, const double* dx1, const double* dx2);
+
+/* dwlimiter2 on one pair (src/vacgrid.t:938-996): a = dw(i), b = dw(i-1). */
+double vacref_limit(int type, double a, double b);
+
+/* roetest identity (src/roetest.t:38-57): out[iw] = sum_il R(iw,il) *
+ * jump(il) for the MHD Roe matrices at the arithmetic average of wL, wR
+ * (src/vacphys.mhdroe.t).  ndim in {2,3} selects nw = 6 / 8.  Also returns
+ * the wave speeds a[il].  smallp is the pressure floor used by the average. */
+int vacref_roe_identity(int ndim, double gamma, int idim, int divbwave,
+                        double smallp, const double* wL, const double* wR,
+                        double* out, double* a);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

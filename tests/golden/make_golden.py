@@ -1,0 +1,100 @@
+"""Writes the golden fixtures under tests/golden/ from the reference VAC.
+
+Each fixture is one run of a reference VAC executable (built from the
+reference's Fortran sources by oracle/build_ref.sh) on an initial state made
+by vac_amd.initial:  the initial mesh state, the parameters, the time t after
+every step (from the per-step snapshots VAC writes) and the final mesh state.
+
+    python tests/golden/make_golden.py            # needs /root/reference
+
+The config-1 run (Orszag-Tang 128^2, TVDLF + minmod, 100 steps) is recorded
+as SHA-256 digests of its initial and final states plus its t sequence,
+which pins the full-size run without storing 0.8 MB arrays.
+"""
+from __future__ import annotations
+
+import hashlib
+import json
+import os
+import subprocess
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+sys.path.insert(0, os.path.join(ROOT, "oracle"))
+
+from vac_amd.initial import orszag_tang, random_fourier  # noqa: E402
+from run_ref import have_ref, make_ini, problem_par, run_ref  # noqa: E402
+
+BINARIES = {
+    "vac_mhd22_132": ("22", "mhd", "132,132"),
+    "vac_hd22_132": ("22", "hd", "132,132"),
+    "vac_mhd33_36": ("33", "mhd", "36,36,36"),
+}
+
+NAMES = {6: "rho m1 m2 e b1 b2", 4: "rho m1 m2 e", 8: "rho m1 m2 m3 e b1 b2 b3"}
+
+# name: (binary, physics, builder, gamma, steps, typefull, limiter, bc, divbfix)
+CASES = {
+    "mhd22_rf32x24_tvdlf_minmod": ("vac_mhd22_132", "mhd", lambda g: random_fourier((32, 24), 101, g), 5 / 3, 20, "tvdlf", "minmod", "periodic", True),
+    "mhd22_rf32x24_tvd_woodward": ("vac_mhd22_132", "mhd", lambda g: random_fourier((32, 24), 102, g), 5 / 3, 20, "tvd", "woodward", "periodic", True),
+    "mhd22_rf24x32_tvdlf_superbee_cont": ("vac_mhd22_132", "mhd", lambda g: random_fourier((24, 32), 103, g), 1.4, 15, "tvdlf", "superbee", "cont", True),
+    "mhd22_rf24x24_tvd_minmod_nodivb": ("vac_mhd22_132", "mhd", lambda g: random_fourier((24, 24), 104, g), 5 / 3, 15, "tvd", "minmod", "periodic", False),
+    "mhd22_ot32_tvdlf_woodward": ("vac_mhd22_132", "mhd", lambda g: orszag_tang(32, g), 5 / 3, 30, "tvdlf", "woodward", "periodic", True),
+    "hd22_rf32x24_tvdlf_minmod": ("vac_hd22_132", "hd", lambda g: random_fourier((32, 24), 105, g, physics="hd"), 1.4, 25, "tvdlf", "minmod", "periodic", True),
+    "hd22_rf24x24_tvdlf_woodward_cont": ("vac_hd22_132", "hd", lambda g: random_fourier((24, 24), 106, g, physics="hd"), 5 / 3, 20, "tvdlf", "woodward", "cont", True),
+    "mhd33_rf12x10x8_tvdlf_minmod": ("vac_mhd33_36", "mhd", lambda g: random_fourier((12, 10, 8), 107, g), 5 / 3, 10, "tvdlf", "minmod", "periodic", True),
+    "mhd33_rf8x10x12_tvd_woodward": ("vac_mhd33_36", "mhd", lambda g: random_fourier((8, 10, 12), 108, g), 5 / 3, 10, "tvd", "woodward", "periodic", True),
+}
+
+
+def ensure_binary(name: str) -> None:
+    if have_ref(name):
+        return
+    d, phys, g = BINARIES[name]
+    subprocess.run([os.path.join(ROOT, "oracle", "build_ref.sh"), name, d, phys, g], check=True)
+
+
+def run_case(binary, physics, w, coords, gamma, steps, typefull, limiter, bc, divbfix):
+    nw = w.shape[0]
+    ini = make_ini(w, coords, physics, gamma, name="golden")
+    par = problem_par(nw, NAMES[nw], steps, typefull=typefull, limiter=limiter,
+                      typeB=[bc] * (2 * len(coords)), divbfix=divbfix, every_step=True)
+    r = run_ref(binary, ini, par)
+    t = np.array([s.t for s in r.snapshots])
+    return r.snapshots[-1].w, t
+
+
+def sha(a: np.ndarray) -> str:
+    return hashlib.sha256(np.ascontiguousarray(a, dtype="<f8").tobytes()).hexdigest()
+
+
+def main() -> None:
+    for b in BINARIES:
+        ensure_binary(b)
+    for name, (binary, physics, build, gamma, steps, typefull, limiter, bc, divbfix) in CASES.items():
+        w0, coords = build(gamma)
+        wf, t = run_case(binary, physics, w0, coords, gamma, steps, typefull, limiter, bc, divbfix)
+        np.savez_compressed(
+            os.path.join(HERE, name + ".npz"), w0=w0, wf=wf, t=t,
+            nx=np.array([len(c) for c in coords]), gamma=gamma, steps=steps,
+            physics=physics, typefull=typefull, limiter=limiter, bc=bc,
+            divbfix=divbfix)
+        print("wrote", name, w0.shape, "steps", len(t) - 1)
+    # config 1 at full size: digests only
+    g = 5.0 / 3.0
+    w0, coords = orszag_tang(128, g)
+    wf, t = run_case("vac_mhd22_132", "mhd", w0, coords, g, 100, "tvdlf", "minmod", "periodic", True)
+    rec = {"case": "Orszag-Tang 128x128, gamma=5/3, TVDLF+minmod, twostep, courantpar=0.8, 100 steps, periodic, divbfix=T",
+           "w0_sha256": sha(w0), "wf_sha256": sha(wf), "t": [float(v) for v in t],
+           "wf_sum_abs": [float(np.abs(wf[i]).sum()) for i in range(wf.shape[0])]}
+    with open(os.path.join(HERE, "ot128_tvdlf_minmod_100.json"), "w") as f:
+        json.dump(rec, f, indent=1)
+    print("wrote ot128_tvdlf_minmod_100.json")
+
+
+if __name__ == "__main__":
+    main()
