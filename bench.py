@@ -1,0 +1,155 @@
+"""Benchmark of the MI355X advect path on its headline workload.
+
+Workload: 2D ideal MHD on a 2048x2048 periodic grid, FP64, gamma=5/3, TVD
+(Roe-type) with the woodward limiter, courantpar=0.8, 200 steps, from the
+seeded random-Fourier initial state (rho = 1 + 0.2*sum of 8 modes, p = 1,
+v and B = 0.1*sum of 8 modes each, B from a vector potential).
+
+A step is one full VAC time step (Courant dt, then one dimensionally split
+sweep per direction, each with the Roe dissipation and the Powell / Ryu
+source terms).  Metric: Mcell-updates/s = nx*ny*steps / seconds / 1e6.
+
+    python bench.py [--gpus 1] [--steps 200] [--warmup 5] [--no-cpu] [--precision fp64]
+
+Prints one JSON line.  The state is resident in HBM before the timed region
+starts; the timed region is the device-side time loop (vacgpu_run) between
+two stream synchronisations.  The TVD sweep kernel's average duration is
+measured with HIP events on the library's own stream in a separate pass.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+NX = NY = 2048
+STEPS = 200
+SEED = 20251203
+GAMMA = 5.0 / 3.0
+NW = 6
+# algorithmic bytes per cell for one TVD sweep launch: centdiff (read wCT,
+# read w, write w) + tvdlimit (read wCT, read w, write w) = 6*nw*8
+B_ALG_TVD_SWEEP = 6 * NW * 8
+HBM_PEAK_GBS = 8000.0
+
+
+def cpu_baseline(w0, coords, seconds_budget=30.0):
+    """The reference VAC executable on one host core, on a one-step sample of
+    the same 2048^2 workload; falls back to the C restatement if the
+    reference executable is not present."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    from run_ref import have_ref, make_ini, problem_par, run_ref
+
+    cells = NX * NY
+    if have_ref("vac_mhd22_2052"):
+        ini = make_ini(w0, coords, "mhd", GAMMA, name="bench")
+        par = problem_par(6, "rho m1 m2 e b1 b2", 1, typefull="tvd", limiter="woodward",
+                          courantpar=0.8)
+        r = run_ref("vac_mhd22_2052", ini, par, timeout=600)
+        return {"value": cells * 1 / r.advance_seconds / 1e6, "unit": "Mcell-updates/s",
+                "cores": 1, "kind": "reference",
+                "sample": "VAC 4.52 (flang -O2, fp-contract off), 2048x2048 TVD+woodward, "
+                          "1 of 200 steps, advance time only"}
+    import vacref
+    from vac_amd.grid import widths_1d
+
+    n = 512
+    p = vacref.make_params(2, "mhd", [n, n], GAMMA, typefull="tvd", limiter="woodward")
+    sub = np.ascontiguousarray(w0[:, :n, :n])
+    t0 = time.perf_counter()
+    vacref.run(p, sub, [widths_1d(c[:n]) for c in coords], 2)
+    dt = time.perf_counter() - t0
+    return {"value": n * n * 2 / dt / 1e6, "unit": "Mcell-updates/s", "cores": 1,
+            "kind": "port", "sample": "C restatement, 512x512 TVD+woodward, 2 steps"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=STEPS)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32bulk"])
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU only: this build has no multi-GPU path "
+              "(independent grids shard by instance; see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    from vac_amd import vacgpu
+    from vac_amd.grid import widths_1d
+    from vac_amd.initial import random_fourier
+
+    w0, coords = random_fourier((NX, NY), SEED, GAMMA)
+    dxs = [widths_1d(c) for c in coords]
+    method = vacgpu.MethodSpec(typefull="tvd", limiter="woodward", courantpar=0.8)
+    grid = vacgpu.Grid([NX, NY], "mhd", GAMMA, dxs, precision=args.precision)
+
+    # warm-up (first launches, clocks), then reset to the initial state
+    grid.put_mesh(w0)
+    if args.warmup > 0:
+        grid.run(method, args.warmup)
+    grid.close()
+    grid = vacgpu.Grid([NX, NY], "mhd", GAMMA, dxs, precision=args.precision)
+    grid.put_mesh(w0)
+
+    t0 = time.perf_counter()
+    n, _ = grid.run(method, args.steps)  # returns after a stream synchronisation
+    elapsed = time.perf_counter() - t0
+    if n != args.steps:
+        raise RuntimeError(f"ran {n} of {args.steps} steps")
+    sc = grid.scalars()
+    it, t_end = grid.get_time()
+    ms_per_step = elapsed / args.steps * 1e3
+    value = NX * NY * args.steps / elapsed / 1e6
+
+    # dominant-kernel timing with HIP events on the library stream
+    grid.put_mesh(w0)
+    grid.set_time(0, 0.0)
+    grid.profile(True)
+    grid.run(method, min(args.steps, 20))
+    k_ms, k_cnt = grid.kernel_time("tvd_sweep")
+    other = {k: grid.kernel_time(k) for k in ("ryu", "bound", "reduce")}
+    grid.profile(False)
+    grid.close()
+    achieved = B_ALG_TVD_SWEEP * NX * NY / (k_ms * 1e-3) / 1e9 if k_ms > 0 else 0.0
+
+    out = {
+        "metric": "Mcell-updates/s per step (2D MHD, TVD Roe + woodward, 2048^2)",
+        "value": round(value, 3),
+        "unit": "Mcell-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if args.precision == "fp64" else "f32-bulk/f64-accumulate",
+        "data": "synthetic (seeded random-Fourier MHD state)",
+        "config": {"workload": "2D ideal MHD 2048x2048 periodic, TVD (Roe) + woodward, "
+                               "courantpar 0.8, 200 steps, gamma 5/3",
+                   "nx": NX, "ny": NY, "seed": SEED, "precision": args.precision},
+        "t_end": t_end,
+        "nerror": [int(v) for v in sc["nerror"]],
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": None, "kernel": "k_tvd_sweep",
+                     "kernel_avg_ms": round(k_ms, 5), "kernel_launches": k_cnt,
+                     "b_alg_per_cell": B_ALG_TVD_SWEEP},
+        "other_kernels_avg_ms": {k: round(v[0], 5) for k, v in other.items()},
+    }
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(w0, coords)
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,127 @@
+/*
+ * vacgpu.h -- C ABI of the MI355X (gfx950) advect path for VAC.
+ *
+ * One handle holds one grid whose state stays resident in HBM between calls.
+ * Each entry point below replaces a piece of the reference's host loop for
+ * the TVDLF / TVD (Roe) step of the ideal MHD and HD modules; the citations
+ * are to the reference sources (VAC 4.52, src/).
+ *
+ * All functions return 0 on success and a negative VACGPU_E* code on
+ * failure; vacgpu_last_error() then returns a message, which a Fortran host
+ * passes to its die() (src/vacio.t:1108-1116).  Nothing falls back to a CPU
+ * path: an unsupported selector is an error.
+ *
+ * Arrays are passed as plain pointers in VAC's layout: w(ixG^T, nw) column
+ * major, x fastest, with caller-chosen leading dimensions ld[] (VAC's
+ * compile-time ixGhi^D).  Index 0 of each dimension is VAC's ixGlo = 1.
+ */
+#ifndef VACGPU_H
+#define VACGPU_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  VACGPU_OK = 0,
+  VACGPU_E_ARG = -1,         /* bad argument or unsupported selector */
+  VACGPU_E_HIP = -2,         /* HIP runtime error */
+  VACGPU_E_STATE = -3,       /* call out of order (e.g. advect before put_state) */
+  VACGPU_E_NOMEM = -4
+};
+
+enum { VACGPU_HD = 0, VACGPU_MHD = 1 };                 /* setvac -p= */
+enum { VACGPU_TVDLF = 0, VACGPU_TVD = 1 };              /* typefull */
+enum { VACGPU_MINMOD = 0, VACGPU_WOODWARD = 1, VACGPU_SUPERBEE = 2 };  /* typelimiter */
+enum { VACGPU_CONT = 0, VACGPU_PERIODIC = 1 };          /* typeB */
+enum { VACGPU_FP64 = 0, VACGPU_FP32BULK = 1 };          /* arithmetic tier */
+
+typedef struct vacgpu_grid vacgpu_grid;
+
+/* Grid and physics: what VAC fixes at setvac time and in boundsetup /
+ * gridsetup1 (src/vacgrid.t:9-160, :457-593). */
+typedef struct {
+  int ndim;             /* 2 or 3; ndir == ndim (setvac -d=22 / -d=33) */
+  int physics;          /* VACGPU_MHD or VACGPU_HD */
+  int nx[3];            /* mesh cells per dimension (the ixM extent) */
+  double eqpar[2];      /* eqpar(gamma_), eqpar(eta_); eta must be 0 */
+  int typeB[6][8];      /* typeB(iw, iB) with iB = 2*(idim-1) + upper */
+  int precision;        /* VACGPU_FP64 or VACGPU_FP32BULK */
+} vacgpu_config;
+
+/* Method selectors read from the par file (&methodlist, &paramlist). */
+typedef struct {
+  int typefull;         /* VACGPU_TVDLF (twostep, hancock) or VACGPU_TVD (onestep) */
+  int typelimiter[8];   /* per variable (TVDLF) or per characteristic wave (TVD) */
+  int divbfix;          /* Powell source terms (MHD) */
+  int divbwave;         /* divergence wave in the Roe solver (MHD) */
+  int sourceunsplit;
+  int dimsplit_xy;      /* 1: typedimsplit='xy'; 0: 'xyyx' (sweep order flips with it) */
+  double courantpar;
+  double smallpcoeff;
+} vacgpu_method;
+
+/* Replaces the geometry hand-off after gridsetup1 (src/vacgrid.t:457-593):
+ * dxd[i] are the cell widths along dimension d over ixG (nx[d]+4 values;
+ * VAC's dx(:, ..., d) restricted to one line -- the grid is slab Cartesian). */
+int vacgpu_create(const vacgpu_config* cfg, const double* dx0, const double* dx1,
+                  const double* dx2, vacgpu_grid** out);
+
+/* Optional: run on a caller-provided hipStream_t (default: a private stream). */
+int vacgpu_set_stream(vacgpu_grid* g, void* hip_stream);
+
+/* Copy the full ixG state in / out (src/vacio.t readfileini / savefile hand-off).
+ * ld[d] is the leading dimension of the caller's array along d (VAC: ixGhi^D). */
+int vacgpu_put_state(vacgpu_grid* g, const double* w, const int* ld);
+int vacgpu_get_state(vacgpu_grid* g, double* w, const int* ld);
+
+/* getdt_courant (src/vac.t:664-722).  dtcourant (ndim, in/out) and smallp
+ * (in/out, <0 means "set from the initial pressure minimum" as in
+ * src/vacphys0.t.mhd:202-214) may be NULL to use the device-resident values.
+ * On return *dt holds the Courant time step. */
+int vacgpu_getdt_courant(vacgpu_grid* g, const vacgpu_method* m, double* dtcourant,
+                         double* smallp, double* dt);
+
+/* advect (src/vac.t:363-486) for one dimensional sweep idimmin == idimmax
+ * (1-based, dimsplit), including advect1's getboundary (src/vac.t:567).
+ * firstsweep / lastsweep are VAC's common/first/ flags (src/vac.t:298-299).
+ * The first sweep of a step also records the state as wold (src/vac.t:85).
+ * dtcourant (ndim) and nerror (4) are min-accumulated / incremented as in
+ * src/vactvdlf.t:184-201 and src/vactvd.t:145-157; NULL keeps them on the
+ * device (no host synchronisation). */
+int vacgpu_advect(vacgpu_grid* g, const vacgpu_method* m, int idimmin, int idimmax,
+                  double dt, int firstsweep, int lastsweep, double* dtcourant, int* nerror);
+
+/* The time loop of src/vac.t:55-106 on the device for up to nsteps steps:
+ * dt from getdt_courant (or dtpar > 0), tmaxexact, advance with it-parity
+ * sweep order, it += 1, t += dt.  Stops at itmax or t >= tmax.  t_hist
+ * (optional, nsteps+1) receives t before every step and after the last.
+ * Synchronises with the host only when tmax is finite or t_hist is given. */
+int vacgpu_run(vacgpu_grid* g, const vacgpu_method* m, int nsteps, int itmax, double tmax,
+               int tmaxexact, double dtpar, double* t_hist);
+
+/* Time and step counter used by vacgpu_run (VAC's t, it). */
+int vacgpu_set_time(vacgpu_grid* g, int it, double t);
+int vacgpu_get_time(vacgpu_grid* g, int* it, double* t);
+
+/* Device-resident scalars: dtcourant[ndim], nerror[4], smallp, dt. Any may be NULL. */
+int vacgpu_get_scalars(vacgpu_grid* g, double* dtcourant, int* nerror, double* smallp,
+                       double* dt);
+
+/* Per-kernel timing with HIP events on the grid's stream (0 = off). */
+int vacgpu_profile(vacgpu_grid* g, int enable);
+/* Average duration (ms) and launch count of kernel family k since enabling:
+ * k = 0 TVD sweep, 1 TVDLF predictor, 2 TVDLF corrector, 3 Ryu source,
+ * 4 ghost cells, 5 reductions. */
+int vacgpu_kernel_time(vacgpu_grid* g, int k, double* avg_ms, long* count);
+
+/* Start-up check of a selector set: 0 if supported, else VACGPU_E_ARG. */
+int vacgpu_supported(const vacgpu_config* cfg, const vacgpu_method* m);
+
+const char* vacgpu_last_error(const vacgpu_grid* g);
+void vacgpu_destroy(vacgpu_grid* g);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,395 @@
+// Device-side ideal MHD / HD physics for the advect path, one cell or one
+// interface at a time.  Every expression keeps the evaluation order of the
+// VAC statement it restates (cited per function), so that the FP64 build
+// (compiled with -ffp-contract=off) reproduces VAC bit for bit.
+//
+// R is the arithmetic type of the bulk flux / limiter / eigen work (double,
+// or float for the FP32-bulk tier).  Thermal and total pressure recovery,
+// source terms and every update of a conserved variable stay in double.
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace vacgpu {
+
+constexpr double kBig = 1.0e99;     // bigdouble (src/vacdef.t:51)
+constexpr double kSmall = 1.0e-99;  // smalldouble (src/vacdef.t:51)
+
+enum { LIM_MINMOD = 0, LIM_WOODWARD = 1, LIM_SUPERBEE = 2 };
+
+template <int ND, bool MHD>
+struct Layout {
+  static constexpr int nd = ND;           // ndim == ndir
+  static constexpr int nw = MHD ? 2 + 2 * ND : 2 + ND;
+  static constexpr int rho = 0;
+  static constexpr int e = 1 + ND;
+  __host__ __device__ static constexpr int m(int c) { return 1 + c; }
+  __host__ __device__ static constexpr int b(int c) { return 2 + ND + c; }
+};
+
+template <class R>
+__device__ __forceinline__ R rmax(R a, R b) { return a < b ? b : a; }
+template <class R>
+__device__ __forceinline__ R rmin(R a, R b) { return b < a ? b : a; }
+// Fortran SIGN(a, b)
+template <class R>
+__device__ __forceinline__ R fsign(R a, R b) { return copysign(fabs(a), b); }
+
+// dwlimiter2 (src/vacgrid.t:938-996): a = dwC(i), b = dwC(i-1)
+template <class R>
+__device__ __forceinline__ R limiter(int type, R a, R b) {
+  const R one = R(1), zero = R(0);
+  R s = copysign(one, a);
+  if (type == LIM_MINMOD) return s * rmax(zero, rmin(R(fabs(a)), s * b));
+  if (type == LIM_WOODWARD)
+    return R(2) * s * rmax(zero, rmin(rmin(R(fabs(a)), s * b), s * R(0.25) * (b + a)));
+  return s * rmax(rmax(zero, rmin(R(2) * R(fabs(a)), s * b)), rmin(R(fabs(a)), R(2) * s * b));
+}
+
+// Unclipped thermal pressure (MHD src/vacphys0.t.mhd:191-199,
+// HD src/vacphys0.t.hd:118-119), always in double.
+template <int ND, bool MHD>
+__device__ __forceinline__ double pthermal_raw(const double* u, double g) {
+  using L = Layout<ND, MHD>;
+  double msq = u[L::m(0)] * u[L::m(0)];
+#pragma unroll
+  for (int c = 1; c < ND; ++c) msq = msq + u[L::m(c)] * u[L::m(c)];
+  if constexpr (MHD) {
+    double p = msq / u[L::rho];
+#pragma unroll
+    for (int c = 0; c < ND; ++c) p = p + u[L::b(c)] * u[L::b(c)];
+    return (g - 1.0) * (u[L::e] - 0.5 * p);
+  } else {
+    return (g - 1.0) * (u[L::e] - 0.5 * msq / u[L::rho]);
+  }
+}
+
+// Clipped thermal pressure (getpthermal with clipping=.true.)
+template <int ND, bool MHD>
+__device__ __forceinline__ double pthermal(const double* u, double g, double smallp) {
+  double p = pthermal_raw<ND, MHD>(u, g);
+  return p < smallp ? smallp : p;
+}
+
+// Total pressure (src/vacphys0.t.mhd:244-252), unclipped, in double.
+template <int ND>
+__device__ __forceinline__ double ptotal(const double* u, double g) {
+  using L = Layout<ND, true>;
+  double bsq = u[L::b(0)] * u[L::b(0)], msq = u[L::m(0)] * u[L::m(0)];
+#pragma unroll
+  for (int c = 1; c < ND; ++c) {
+    bsq = bsq + u[L::b(c)] * u[L::b(c)];
+    msq = msq + u[L::m(c)] * u[L::m(c)];
+  }
+  double p = (1.0 - 0.5 * g) * bsq;
+  p = p + (g - 1.0) * u[L::e];
+  p = p - 0.5 * (g - 1.0) * msq / u[L::rho];
+  return p;
+}
+
+// Physical flux plus transport flux of every variable along d
+// (MHD getflux src/vacphys.mhd0.t:149-205, HD src/vacphys.t.hd:49-75;
+// transport f + v*w with v = m_d/rho as in src/vactvdlf.t:67-70).
+template <int ND, bool MHD, class R>
+__device__ __forceinline__ void fluxes(const double* u, int d, double g, double smallp, R* f) {
+  using L = Layout<ND, MHD>;
+  const R rho = R(u[L::rho]);
+  const R v = R(u[L::m(d)]) / rho;
+  if constexpr (MHD) {
+    const R pt = R(ptotal<ND>(u, g));
+    const R bd = R(u[L::b(d)]);
+    f[L::rho] = R(0) + v * rho;
+#pragma unroll
+    for (int c = 0; c < ND; ++c) {
+      const R bc = R(u[L::b(c)]);
+      R t = (c == d) ? pt - bc * bd : -(bc * bd);
+      f[L::m(c)] = t + v * R(u[L::m(c)]);
+    }
+    R bm = R(u[L::b(0)]) * R(u[L::m(0)]);
+#pragma unroll
+    for (int c = 1; c < ND; ++c) bm = bm + R(u[L::b(c)]) * R(u[L::m(c)]);
+    f[L::e] = (R(u[L::m(d)]) * pt - bd * bm) / rho + v * R(u[L::e]);
+#pragma unroll
+    for (int c = 0; c < ND; ++c) {
+      if (c == d) f[L::b(c)] = R(0);
+      else f[L::b(c)] = -(R(u[L::m(c)]) / rho * bd) + v * R(u[L::b(c)]);
+    }
+  } else {
+    const R p = R(pthermal<ND, false>(u, g, smallp));
+    f[L::rho] = R(0) + v * rho;
+#pragma unroll
+    for (int c = 0; c < ND; ++c) {
+      R t = (c == d) ? p : R(0);
+      f[L::m(c)] = t + v * R(u[L::m(c)]);
+    }
+    f[L::e] = R(u[L::m(d)]) / rho * p + v * R(u[L::e]);
+  }
+}
+
+// cmax_d = fast (or sound) speed + |v_d| (MHD src/vacphys0.t.mhd:79-116
+// with getcsound2 :140-162; HD src/vacphys0.t.hd:72-100).
+template <int ND, bool MHD, class R>
+__device__ __forceinline__ R cmax(const double* u, int d, double g, double smallp) {
+  using L = Layout<ND, MHD>;
+  const double p = pthermal<ND, MHD>(u, g, smallp);
+  const R rho = R(u[L::rho]);
+  if constexpr (MHD) {
+    const R c2 = R(g) * R(p) / rho;
+    R bsq = R(u[L::b(0)]) * R(u[L::b(0)]);
+#pragma unroll
+    for (int c = 1; c < ND; ++c) bsq = bsq + R(u[L::b(c)]) * R(u[L::b(c)]);
+    const R cf2 = bsq / rho + c2;
+    const R bi = R(u[L::b(d)]);
+    return R(sqrt(R(0.5) * (cf2 + R(sqrt(cf2 * cf2 - R(4) * c2 * (bi * bi) / rho))))) +
+           R(fabs(R(u[L::m(d)]) / rho));
+  } else {
+    const R cs = R(sqrt(R(g) * R(p) / rho));
+    return cs + R(fabs(R(u[L::m(d)]) / rho));
+  }
+}
+
+// Powell divB source (src/vacphys.mhd0.t:246-280): subtracts qdt*divB*(...)
+// from out, with divB from the slab centred gradient (src/vacgrid.t:816-823)
+// of the state at the cell (uc) and its +-1 neighbours along each dim.
+template <int ND>
+__device__ __forceinline__ void powell(const double* uc, double divb_raw, double qdt, double* out) {
+  using L = Layout<ND, true>;
+  const double divb = qdt * divb_raw;
+#pragma unroll
+  for (int c = 0; c < ND; ++c) out[L::m(c)] = out[L::m(c)] - uc[L::b(c)] * divb;
+#pragma unroll
+  for (int c = 0; c < ND; ++c)
+    out[L::b(c)] = out[L::b(c)] - uc[L::m(c)] / uc[L::rho] * divb;
+  double mb = uc[L::m(0)] * uc[L::b(0)];
+#pragma unroll
+  for (int c = 1; c < ND; ++c) mb = mb + uc[L::m(c)] * uc[L::b(c)];
+  out[L::e] = out[L::e] - mb / uc[L::rho] * divb;
+}
+
+// ---------------------------------------------------------------- Roe (MHD)
+// Interface state of the eight-wave MHD Roe solver (src/vacphys.mhdroe.t).
+template <int ND, class R>
+struct Roe {
+  R sq;          // sqrt of the arithmetic mean density
+  R v[ND];       // mean velocity
+  R bw[ND];      // B_d (clamped) at slot d, beta at the transverse slots
+  R cs2, cf, cs, af, as;
+  R dp, rhodv;
+  R bdvf, bdbf;  // fast/slow-wave projections (il == fastRW)
+  R bdva, bdba;  // Alfven-wave projections (ND == 3)
+  R v2a2, bvf, bva;  // rtimes helpers for the energy row
+};
+
+// average + the parts of geteigenjump that do not depend on il
+// (src/vacphys.mhdroe.t:4-122 and :156-186), plus the rtimes helpers
+// (:320-345).  uL, uR are the conserved states left and right of the face.
+template <int ND, class R>
+__device__ __forceinline__ void roe_setup(const double* uL, const double* uR, int d, double g,
+                                          double smallp, Roe<ND, R>& r) {
+  using L = Layout<ND, true>;
+  const R rL = R(uL[L::rho]), rR = R(uR[L::rho]);
+  const R rho = R(0.5) * (rL + rR);
+  R B[ND];
+#pragma unroll
+  for (int c = 0; c < ND; ++c) {
+    r.v[c] = R(0.5) * (R(uL[L::m(c)]) / rL + R(uR[L::m(c)]) / rR);
+    B[c] = R(0.5) * (R(uL[L::b(c)]) + R(uR[L::b(c)]));
+  }
+  const double pL = pthermal<ND, true>(uL, g, smallp);
+  const double pR = pthermal<ND, true>(uR, g, smallp);
+  const R pbar = R(0.5 * (pL + pR));
+  R s1 = r.v[0] * (R(uR[L::m(0)]) - R(uL[L::m(0)]));
+  R s2 = r.v[0] * r.v[0];
+  R s3 = R(uR[L::b(0)]) * R(uR[L::b(0)]) - R(uL[L::b(0)]) * R(uL[L::b(0)]);
+#pragma unroll
+  for (int c = 1; c < ND; ++c) {
+    s1 = s1 + r.v[c] * (R(uR[L::m(c)]) - R(uL[L::m(c)]));
+    s2 = s2 + r.v[c] * r.v[c];
+    s3 = s3 + R(uR[L::b(c)]) * R(uR[L::b(c)]) - R(uL[L::b(c)]) * R(uL[L::b(c)]);
+  }
+  const R drho = rR - rL;
+  r.dp = (R(g) - R(1)) * (R(uR[L::e]) - R(uL[L::e]) - s1 + R(0.5) * s2 * drho - R(0.5) * s3);
+  r.rhodv = R(uR[L::m(d)]) - R(uL[L::m(d)]) - r.v[d] * drho;
+  const R c2 = R(g) * pbar / rho;
+  r.cs2 = c2;
+  R bsq = B[0] * B[0];
+#pragma unroll
+  for (int c = 1; c < ND; ++c) bsq = bsq + B[c] * B[c];
+  const R bi = B[d];
+  R cf = bsq / rho + c2;
+  R cs = rmax(R(0), R(0.5) * (cf - R(sqrt(cf * cf - R(4) * c2 * (bi * bi) / rho))));
+  cf = cf - cs;
+  R af = (c2 - cs) / (cf - cs);
+  af = rmin(R(1), rmax(af, R(0)));
+  r.as = R(sqrt(R(1) - af));
+  r.af = R(sqrt(af));
+  r.cf = R(sqrt(cf));
+  r.cs = R(sqrt(cs));
+  r.sq = R(sqrt(rho));
+  r.bw[d] = (R(fabs(bi)) < R(kSmall)) ? R(kSmall) : bi;
+  const int idir = (d + 1) % ND;
+  const int jdir = (idir + 1) % ND;
+  if constexpr (ND == 2) {
+    r.bw[idir] = (B[idir] >= R(0)) ? R(1) : R(-1);
+  } else {
+    const R t = R(sqrt(B[idir] * B[idir] + B[jdir] * B[jdir]));
+    if (t > R(kSmall)) {
+      r.bw[idir] = B[idir] / t;
+      r.bw[jdir] = B[jdir] / t;
+    } else {
+      r.bw[idir] = R(sqrt(R(0.5)));
+      r.bw[jdir] = R(sqrt(R(0.5)));
+    }
+  }
+  // fast/slow projections (il == fastRW_, :159-168)
+  {
+    R t = r.bw[idir] * (R(uR[L::m(idir)]) / rR - R(uL[L::m(idir)]) / rL);
+    if constexpr (ND == 3) t = t + r.bw[jdir] * (R(uR[L::m(jdir)]) / rR - R(uL[L::m(jdir)]) / rL);
+    r.bdvf = t * fsign(r.sq * r.sq, r.bw[d]);
+    R u = r.bw[idir] * (R(uR[L::b(idir)]) - R(uL[L::b(idir)]));
+    if constexpr (ND == 3) u = u + r.bw[jdir] * (R(uR[L::b(jdir)]) - R(uL[L::b(jdir)]));
+    r.bdbf = u * R(sqrt(r.cs2)) * r.sq;
+  }
+  if constexpr (ND == 3) {  // Alfven projections (il == alfvRW_, :176-183)
+    const R bI = r.bw[idir], bJ = r.bw[jdir];
+    R t = bJ * (R(uR[L::m(idir)]) / rR - R(uL[L::m(idir)]) / rL) -
+          bI * (R(uR[L::m(jdir)]) / rR - R(uL[L::m(jdir)]) / rL);
+    R u = bJ * (R(uR[L::b(idir)]) - R(uL[L::b(idir)])) - bI * (R(uR[L::b(jdir)]) - R(uL[L::b(jdir)]));
+    r.bdva = t * R(0.5) * (r.sq * r.sq);
+    r.bdba = u * R(0.5) * fsign(r.sq, r.bw[d]);
+    r.bva = bJ * r.v[idir] - bI * r.v[jdir];
+  }
+  // rtimes helpers for the energy row (:324-336)
+  R vs = r.v[0] * r.v[0];
+#pragma unroll
+  for (int c = 1; c < ND; ++c) vs = vs + r.v[c] * r.v[c];
+  r.v2a2 = R(0.5) * vs + (R(2) - R(g)) / (R(g) - R(1)) * r.cs2;
+  {
+    R t = r.bw[idir] * r.v[idir];
+    if constexpr (ND == 3) t = t + r.bw[jdir] * r.v[jdir];
+    r.bvf = t * copysign(R(1), r.bw[d]);
+  }
+}
+
+// Wave speed a and characteristic jump of wave il (geteigenjump
+// src/vacphys.mhdroe.t:188-226, entropy fix 'nul').
+template <int ND, class R>
+__device__ __forceinline__ void roe_wave(const Roe<ND, R>& r, const double* uL, const double* uR,
+                                         int il, int d, int divbwave, R& a, R& jump) {
+  using L = Layout<ND, true>;
+  const R vd = r.v[d];
+  switch (il) {
+    case 0:
+      a = vd + r.cf;
+      jump = R(0.5) / r.cs2 * (r.af * (+r.cf * r.rhodv + r.dp) + r.as * (-r.cs * r.bdvf + r.bdbf));
+      break;
+    case 1:
+      a = vd - r.cf;
+      jump = R(0.5) / r.cs2 * (r.af * (-r.cf * r.rhodv + r.dp) + r.as * (+r.cs * r.bdvf + r.bdbf));
+      break;
+    case 2:
+      a = vd + r.cs;
+      jump = R(0.5) / r.cs2 * (r.as * (+r.cs * r.rhodv + r.dp) + r.af * (+r.cf * r.bdvf - r.bdbf));
+      break;
+    case 3:
+      a = vd - r.cs;
+      jump = R(0.5) / r.cs2 * (r.as * (-r.cs * r.rhodv + r.dp) + r.af * (-r.cf * r.bdvf - r.bdbf));
+      break;
+    case 4:
+      a = vd;
+      jump = R(uR[L::rho]) - R(uL[L::rho]) - r.dp / r.cs2;
+      break;
+    case 5:
+      if (divbwave) {
+        a = vd;
+        jump = R(uR[L::b(d)]) - R(uL[L::b(d)]);
+      } else {
+        a = R(0);
+        jump = R(0);
+      }
+      break;
+    case 6:
+      a = vd + R(fabs(r.bw[d])) / r.sq;
+      jump = +r.bdva - r.bdba;
+      break;
+    default:
+      a = vd - R(fabs(r.bw[d])) / r.sq;
+      jump = -r.bdva - r.bdba;
+      break;
+  }
+}
+
+// R(iw, il) * q (rtimes, src/vacphys.mhdroe.t:287-450)
+template <int ND, class R>
+__device__ __forceinline__ R roe_rtimes(const Roe<ND, R>& r, R q, int iw, int il, int d, int divbwave) {
+  using L = Layout<ND, true>;
+  const int idir = (d + 1) % ND;
+  const int jdir = (idir + 1) % ND;
+  const R cf = r.cf, cs = r.cs, af = r.af, as = r.as;
+  if (iw == L::rho) {
+    if (il <= 1) return q * af;
+    if (il <= 3) return q * as;
+    if (il == 4) return q;
+    return R(0);
+  }
+  if (iw == L::e) {
+    const R vd = r.v[d];
+    switch (il) {
+      case 0: return q * (-as * cs * r.bvf + af * (r.v2a2 + cf * (cf + vd)));
+      case 1: return q * (+as * cs * r.bvf + af * (r.v2a2 + cf * (cf - vd)));
+      case 2: return q * (+af * cf * r.bvf + as * (r.v2a2 + cs * (cs + vd)));
+      case 3: return q * (-af * cf * r.bvf + as * (r.v2a2 + cs * (cs - vd)));
+      case 4: {
+        R vs = r.v[0] * r.v[0];
+#pragma unroll
+        for (int c = 1; c < ND; ++c) vs = vs + r.v[c] * r.v[c];
+        return q * R(0.5) * vs;
+      }
+      case 5: return divbwave ? q * r.bw[d] : R(0);
+      case 6: return +q * r.bva;
+      default: return -q * r.bva;
+    }
+  }
+  if (iw < L::e) {  // momentum
+    const int c = iw - 1;
+    const R vw = r.v[c];
+    if (c == d) {
+      switch (il) {
+        case 0: return q * af * (vw + cf);
+        case 1: return q * af * (vw - cf);
+        case 2: return q * as * (vw + cs);
+        case 3: return q * as * (vw - cs);
+        case 4: return q * vw;
+        default: return R(0);
+      }
+    }
+    const R beta = r.bw[c];
+    const R sg = copysign(R(1), r.bw[d]);
+    switch (il) {
+      case 0: return q * (af * vw - as * cs * beta * sg);
+      case 1: return q * (af * vw + as * cs * beta * sg);
+      case 2: return q * (as * vw + af * cf * beta * sg);
+      case 3: return q * (as * vw - af * cf * beta * sg);
+      case 4: return q * vw;
+      case 5: return R(0);
+      case 6: return (c == idir) ? +q * r.bw[jdir] : -q * r.bw[idir];
+      default: return (c == idir) ? -q * r.bw[jdir] : +q * r.bw[idir];
+    }
+  }
+  // magnetic field
+  const int c = iw - (2 + ND);
+  if (c == d) return (il == 5 && divbwave) ? q : R(0);
+  const R beta = r.bw[c];
+  switch (il) {
+    case 0:
+    case 1: return +q * as * R(sqrt(r.cs2)) * beta / r.sq;
+    case 2:
+    case 3: return -(q * af * R(sqrt(r.cs2)) * beta / r.sq);
+    case 4:
+    case 5: return R(0);
+    default:
+      if (c == idir) return -(q * r.bw[jdir] / fsign(r.sq, r.bw[d]));
+      return +q * r.bw[idir] / fsign(r.sq, r.bw[d]);
+  }
+}
+
+}  // namespace vacgpu

@@ -1,0 +1,271 @@
+"""Python host side of the MI355X advect path, over the C ABI in include/vacgpu.h.
+
+This mirrors the pieces of VAC's host loop that call the path:
+``Grid.getdt_courant`` (src/vac.t:664-722), ``Grid.advect`` (src/vac.t:363-486)
+and ``Grid.run`` (the time loop of src/vac.t:55-106).  The state stays on the
+device between calls; ``put_state`` / ``get_state`` move the full ixG array.
+
+The shared library is built in-tree by ``python -m vac_amd.build``; importing
+this module on a machine without it raises, there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass, field
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libvacgpu.so")
+
+HD, MHD = 0, 1
+TVDLF, TVD = 0, 1
+LIMITERS = {"minmod": 0, "woodward": 1, "superbee": 2}
+CONT, PERIODIC = 0, 1
+FP64, FP32BULK = 0, 1
+KERNELS = {"tvd_sweep": 0, "tvdlf_pred": 1, "tvdlf_corr": 2, "ryu": 3, "bound": 4, "reduce": 5}
+
+
+class Config(C.Structure):
+    _fields_ = [("ndim", C.c_int), ("physics", C.c_int), ("nx", C.c_int * 3),
+                ("eqpar", C.c_double * 2), ("typeB", (C.c_int * 8) * 6),
+                ("precision", C.c_int)]
+
+
+class Method(C.Structure):
+    _fields_ = [("typefull", C.c_int), ("typelimiter", C.c_int * 8), ("divbfix", C.c_int),
+                ("divbwave", C.c_int), ("sourceunsplit", C.c_int), ("dimsplit_xy", C.c_int),
+                ("courantpar", C.c_double), ("smallpcoeff", C.c_double)]
+
+
+_lib = None
+_DP = C.POINTER(C.c_double)
+_IP = C.POINTER(C.c_int)
+
+# every entry point declared in include/vacgpu.h with its ctypes signature
+SIGNATURES = {
+    "vacgpu_create": ([C.POINTER(Config), _DP, _DP, _DP, C.POINTER(C.c_void_p)], C.c_int),
+    "vacgpu_set_stream": ([C.c_void_p, C.c_void_p], C.c_int),
+    "vacgpu_put_state": ([C.c_void_p, _DP, _IP], C.c_int),
+    "vacgpu_get_state": ([C.c_void_p, _DP, _IP], C.c_int),
+    "vacgpu_getdt_courant": ([C.c_void_p, C.POINTER(Method), _DP, _DP, _DP], C.c_int),
+    "vacgpu_advect": ([C.c_void_p, C.POINTER(Method), C.c_int, C.c_int, C.c_double, C.c_int,
+                       C.c_int, _DP, _IP], C.c_int),
+    "vacgpu_run": ([C.c_void_p, C.POINTER(Method), C.c_int, C.c_int, C.c_double, C.c_int,
+                    C.c_double, _DP], C.c_int),
+    "vacgpu_set_time": ([C.c_void_p, C.c_int, C.c_double], C.c_int),
+    "vacgpu_get_time": ([C.c_void_p, _IP, _DP], C.c_int),
+    "vacgpu_get_scalars": ([C.c_void_p, _DP, _IP, _DP, _DP], C.c_int),
+    "vacgpu_profile": ([C.c_void_p, C.c_int], C.c_int),
+    "vacgpu_kernel_time": ([C.c_void_p, C.c_int, _DP, C.POINTER(C.c_long)], C.c_int),
+    "vacgpu_supported": ([C.POINTER(Config), C.POINTER(Method)], C.c_int),
+    "vacgpu_last_error": ([C.c_void_p], C.c_char_p),
+    "vacgpu_destroy": ([C.c_void_p], None),
+}
+
+
+def load_library(path: str = LIB_PATH):
+    """Load libvacgpu.so and declare its signatures (no device call is made)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise RuntimeError(f"{path} is missing: build it with `python -m vac_amd.build` "
+                           "(the advect path has no CPU fallback)")
+    L = C.CDLL(path)
+    for name, (args, res) in SIGNATURES.items():
+        fn = getattr(L, name)
+        fn.argtypes = args
+        fn.restype = res
+    _lib = L
+    return L
+
+
+def _ptr(a):
+    return a.ctypes.data_as(_DP) if a is not None else None
+
+
+class VacGpuError(RuntimeError):
+    pass
+
+
+@dataclass
+class MethodSpec:
+    typefull: str = "tvdlf"
+    limiter: object = "minmod"          # one name, or a list per variable / wave
+    divbfix: bool = True
+    divbwave: bool = True
+    sourceunsplit: bool = True
+    dimsplit_xy: bool = False
+    courantpar: float = 0.8
+    smallpcoeff: float = 1e-6
+
+    def to_c(self) -> Method:
+        m = Method()
+        m.typefull = TVDLF if self.typefull == "tvdlf" else TVD
+        lims = [self.limiter] * 8 if isinstance(self.limiter, str) else list(self.limiter) + ["minmod"] * 8
+        for i in range(8):
+            m.typelimiter[i] = LIMITERS[lims[i]]
+        m.divbfix = int(self.divbfix)
+        m.divbwave = int(self.divbwave)
+        m.sourceunsplit = int(self.sourceunsplit)
+        m.dimsplit_xy = int(self.dimsplit_xy)
+        m.courantpar = self.courantpar
+        m.smallpcoeff = self.smallpcoeff
+        return m
+
+
+def nvars(physics: str, ndim: int) -> int:
+    return 2 + 2 * ndim if physics == "mhd" else 2 + ndim
+
+
+class Grid:
+    """One grid resident on the GPU (a vacgpu_grid handle)."""
+
+    def __init__(self, nx, physics: str, gamma: float, dxs, bc="periodic",
+                 precision: str = "fp64"):
+        self.lib = load_library()
+        self.nx = [int(n) for n in nx]
+        self.ndim = len(self.nx)
+        self.physics = physics
+        self.nw = nvars(physics, self.ndim)
+        cfg = Config()
+        cfg.ndim = self.ndim
+        cfg.physics = MHD if physics == "mhd" else HD
+        for d in range(self.ndim):
+            cfg.nx[d] = self.nx[d]
+        cfg.eqpar[0] = gamma
+        cfg.eqpar[1] = 0.0
+        bcs = [bc] * (2 * self.ndim) if isinstance(bc, str) else list(bc)
+        for f in range(2 * self.ndim):
+            for iw in range(8):
+                cfg.typeB[f][iw] = PERIODIC if bcs[f] == "periodic" else CONT
+        cfg.precision = FP32BULK if precision == "fp32bulk" else FP64
+        self.cfg = cfg
+        self._dx = [np.ascontiguousarray(d, dtype=np.float64) for d in dxs]
+        for d in range(self.ndim):
+            if self._dx[d].shape != (self.nx[d] + 4,):
+                raise ValueError("dx arrays must have nx+4 entries (ixG)")
+        h = C.c_void_p()
+        dx = self._dx + [None] * (3 - self.ndim)
+        rc = self.lib.vacgpu_create(C.byref(cfg), _ptr(dx[0]), _ptr(dx[1]), _ptr(dx[2]), C.byref(h))
+        if rc != 0:
+            raise VacGpuError(f"vacgpu_create failed ({rc})")
+        self.h = h
+
+    # ------------------------------------------------------------------
+    def _check(self, rc, what):
+        if rc < 0:
+            msg = self.lib.vacgpu_last_error(self.h)
+            raise VacGpuError(f"{what}: {msg.decode() if msg else rc}")
+        return rc
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.vacgpu_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def shape_full(self):
+        return (self.nw,) + tuple(n + 4 for n in reversed(self.nx))
+
+    def put_state(self, w_full: np.ndarray):
+        w = np.ascontiguousarray(w_full, dtype=np.float64)
+        if w.shape != self.shape_full:
+            raise ValueError(f"state shape {w.shape} != {self.shape_full}")
+        self._check(self.lib.vacgpu_put_state(self.h, _ptr(w), None), "put_state")
+
+    def put_mesh(self, w_mesh: np.ndarray):
+        """Put a mesh-only state; ghost cells are filled by the first call."""
+        pad = [(0, 0)] + [(2, 2)] * self.ndim
+        self.put_state(np.pad(np.asarray(w_mesh, dtype=np.float64), pad))
+
+    def get_state(self) -> np.ndarray:
+        w = np.empty(self.shape_full)
+        self._check(self.lib.vacgpu_get_state(self.h, _ptr(w), None), "get_state")
+        return w
+
+    def get_mesh(self) -> np.ndarray:
+        w = self.get_state()
+        return np.ascontiguousarray(w[(slice(None),) + (slice(2, -2),) * self.ndim])
+
+    def set_time(self, it: int, t: float):
+        self._check(self.lib.vacgpu_set_time(self.h, it, t), "set_time")
+
+    def get_time(self):
+        it = C.c_int()
+        t = C.c_double()
+        self._check(self.lib.vacgpu_get_time(self.h, C.byref(it), C.byref(t)), "get_time")
+        return it.value, t.value
+
+    def scalars(self):
+        dtc = np.zeros(3)
+        ne = np.zeros(4, dtype=np.int32)
+        sp = C.c_double()
+        dt = C.c_double()
+        self._check(self.lib.vacgpu_get_scalars(self.h, _ptr(dtc), ne.ctypes.data_as(_IP),
+                                                C.byref(sp), C.byref(dt)), "get_scalars")
+        return {"dtcourant": dtc[: self.ndim], "nerror": ne, "smallp": sp.value, "dt": dt.value}
+
+    def getdt_courant(self, method: MethodSpec, dtcourant=None, smallp=None):
+        m = method.to_c()
+        dt = C.c_double()
+        dtc = None if dtcourant is None else np.ascontiguousarray(dtcourant, dtype=np.float64)
+        sp = None if smallp is None else C.c_double(smallp)
+        self._check(self.lib.vacgpu_getdt_courant(self.h, C.byref(m), _ptr(dtc),
+                                                  C.byref(sp) if sp is not None else None,
+                                                  C.byref(dt)), "getdt_courant")
+        return dt.value, dtc, (sp.value if sp is not None else None)
+
+    def advect(self, method: MethodSpec, idim: int, dt: float, firstsweep: bool,
+               lastsweep: bool, dtcourant=None, nerror=None):
+        m = method.to_c()
+        ne = None if nerror is None else nerror.ctypes.data_as(_IP)
+        self._check(self.lib.vacgpu_advect(self.h, C.byref(m), idim, idim, dt, int(firstsweep),
+                                           int(lastsweep), _ptr(dtcourant), ne), "advect")
+
+    def run(self, method: MethodSpec, nsteps: int, itmax: int = 1 << 30, tmax: float = 1e99,
+            tmaxexact: bool = False, dtpar: float = -1.0, record_t: bool = False):
+        m = method.to_c()
+        hist = np.zeros(nsteps + 1) if record_t else None
+        n = self._check(self.lib.vacgpu_run(self.h, C.byref(m), nsteps, itmax, tmax,
+                                            int(tmaxexact), dtpar, _ptr(hist)), "run")
+        return n, (hist[: n + 1] if record_t else None)
+
+    def profile(self, enable: bool):
+        self._check(self.lib.vacgpu_profile(self.h, int(enable)), "profile")
+
+    def kernel_time(self, name: str):
+        ms = C.c_double()
+        cnt = C.c_long()
+        self._check(self.lib.vacgpu_kernel_time(self.h, KERNELS[name], C.byref(ms), C.byref(cnt)),
+                    "kernel_time")
+        return ms.value, cnt.value
+
+
+def host_loop(grid: Grid, method: MethodSpec, nsteps: int, it0: int = 0, t0: float = 0.0):
+    """VAC's time loop (src/vac.t:55-106) driven from the host through
+    getdt_courant and advect, with dtcourant / nerror kept on the host as
+    VAC's COMMON variables are.  Returns (t history, nerror)."""
+    ndim = grid.ndim
+    dtcourant = np.full(ndim, 1e99)
+    nerror = np.zeros(4, dtype=np.int32)
+    smallp = -1.0
+    it, t = it0, t0
+    hist = [t]
+    for _ in range(nsteps):
+        dt, dtcourant, smallp = grid.getdt_courant(method, dtcourant, smallp)
+        order = list(range(ndim)) if (it % 2 == 0 or method.dimsplit_xy) else list(range(ndim - 1, -1, -1))
+        for k, d in enumerate(order):
+            grid.advect(method, d + 1, dt, k == 0, k == ndim - 1, dtcourant, nerror)
+        it += 1
+        t = t + dt
+        hist.append(t)
+    return np.array(hist), nerror
