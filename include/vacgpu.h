@@ -88,9 +88,11 @@ int vacgpu_getdt_courant(vacgpu_grid* g, const vacgpu_method* m, double* dtcoura
  * The first sweep of a step also records the state as wold (src/vac.t:85).
  * dtcourant (ndim) and nerror (4) are min-accumulated / incremented as in
  * src/vactvdlf.t:184-201 and src/vactvd.t:145-157; NULL keeps them on the
- * device (no host synchronisation). */
+ * device (no host synchronisation).  smallp (in/out, may be NULL) is VAC's
+ * pressure floor: a host that ran getdt_courant itself passes its value. */
 int vacgpu_advect(vacgpu_grid* g, const vacgpu_method* m, int idimmin, int idimmax,
-                  double dt, int firstsweep, int lastsweep, double* dtcourant, int* nerror);
+                  double dt, int firstsweep, int lastsweep, double* dtcourant, int* nerror,
+                  double* smallp);
 
 /* The time loop of src/vac.t:55-106 on the device for up to nsteps steps:
  * dt from getdt_courant (or dtpar > 0), tmaxexact, advance with it-parity

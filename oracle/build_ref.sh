@@ -2,8 +2,14 @@
 # Builds the reference CPU solver (VAC 4.52) from its Fortran sources under
 # /root/reference/src into oracle/_ref/, one executable per configuration.
 #
-#   oracle/build_ref.sh NAME NDIMNDIR PHYS G1[,G2[,G3]]
+#   oracle/build_ref.sh NAME NDIMNDIR PHYS G1[,G2[,G3]] [gpu]
 #   e.g. oracle/build_ref.sh vac_mhd22_132 22 mhd 132,132
+#
+# With the fifth argument "gpu" the same VAC host is linked with the GPU
+# drop-in instead of its own advect (vac_amd/fortran/vacgpu_advect.t, which
+# calls libvacgpu through vac_amd/fortran/vacgpu_mod.f90): the CPU advect of
+# vac.t is renamed advect_cpu and left unused.  This integration executable
+# is what the drop-in test runs against the plain reference build.
 #
 # The recipe is this script, not the reference's setvac/Makefile:
 #  * the needed .t sources are copied to a throw-away directory under /tmp,
@@ -19,7 +25,7 @@
 # Only the final executable is written into oracle/_ref/; the scratch copy is
 # deleted. Nothing from /root/reference is copied into the repository.
 set -euo pipefail
-NAME=$1; D=$2; PHYS=$3; G=$4
+NAME=$1; D=$2; PHYS=$3; G=$4; MODE=${5:-cpu}
 REF=${VAC_REF_SRC:-/root/reference/src}
 HERE=$(cd "$(dirname "$0")" && pwd)
 OUT=$HERE/_ref
@@ -54,13 +60,23 @@ for i in "${!GA[@]}"; do
 done
 line+="ixGhimin=$gmin,ixGhimax=$gmax"
 perl -pi -e "s/^INTEGER,PARAMETER:: ixGhi1=.*\$/$line/" vacdef.t
-for m in vacdef vacpar vacusrpar vac vacio vacgrid vacphys0 vacphys vacusr vaccd vactvdlf vactvd; do
+MODS="vac vacio vacgrid vacphys0 vacphys vacusr vaccd vactvdlf vactvd"
+LINK=""
+if [ "$MODE" = gpu ]; then
+  REPO=$(cd "$HERE/.." && pwd)
+  cp "$REPO/vac_amd/fortran/vacgpu_advect.t" "$REPO/vac_amd/fortran/vacgpu_mod.f90" .
+  perl -pi -e 's/^subroutine advect\(method/subroutine advect_cpu(method/' vac.t
+  $FC $FFLAGS -c vacgpu_mod.f90 -o vacgpu_mod.o
+  MODS="$MODS vacgpu_advect"
+  LINK="vacgpu_mod.o -L$REPO/vac_amd -lvacgpu -Wl,-rpath,\$ORIGIN/../../vac_amd -L/opt/rocm/lib -Wl,-rpath,/opt/rocm/lib"
+fi
+for m in vacdef vacpar vacusrpar $MODS; do
   perl vacpp.pl -d=$D $m.t > $m.f
 done
 objs=""
-for m in vac vacio vacgrid vacphys0 vacphys vacusr vaccd vactvdlf vactvd; do
+for m in $MODS; do
   $FC $FFLAGS -ffree-form -c $m.f -o $m.o 2> $m.log || { cat $m.log >&2; exit 1; }
   objs+=" $m.o"
 done
-$FC $FFLAGS -o "$OUT/$NAME" $objs
+$FC $FFLAGS -o "$OUT/$NAME" $objs $LINK
 echo "built $OUT/$NAME"

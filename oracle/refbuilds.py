@@ -1,0 +1,17 @@
+"""The reference executables oracle/build_ref.sh makes (test infrastructure).
+
+BINARIES: plain VAC 4.52 builds (the CPU oracle and baseline).
+HOOKED:   the same VAC host with the GPU drop-in linked in place of advect.
+Each entry: name -> (setvac -d value, physics, compile-time grid ixGhi).
+"""
+BINARIES = {
+    "vac_mhd22_132": ("22", "mhd", "132,132"),
+    "vac_hd22_132": ("22", "hd", "132,132"),
+    "vac_mhd33_36": ("33", "mhd", "36,36,36"),
+    "vac_mhd22_2052": ("22", "mhd", "2052,2052"),
+}
+HOOKED = {
+    "vacgpu_mhd22_132": ("22", "mhd", "132,132"),
+    "vacgpu_hd22_132": ("22", "hd", "132,132"),
+    "vacgpu_mhd33_36": ("33", "mhd", "36,36,36"),
+}

@@ -29,11 +29,7 @@ sys.path.insert(0, os.path.join(ROOT, "oracle"))
 from vac_amd.initial import orszag_tang, random_fourier  # noqa: E402
 from run_ref import have_ref, make_ini, problem_par, run_ref  # noqa: E402
 
-BINARIES = {
-    "vac_mhd22_132": ("22", "mhd", "132,132"),
-    "vac_hd22_132": ("22", "hd", "132,132"),
-    "vac_mhd33_36": ("33", "mhd", "36,36,36"),
-}
+from refbuilds import BINARIES  # noqa: E402
 
 NAMES = {6: "rho m1 m2 e b1 b2", 4: "rho m1 m2 e", 8: "rho m1 m2 m3 e b1 b2 b3"}
 
@@ -73,7 +69,7 @@ def sha(a: np.ndarray) -> str:
 
 
 def main() -> None:
-    for b in BINARIES:
+    for b in ("vac_mhd22_132", "vac_hd22_132", "vac_mhd33_36"):
         ensure_binary(b)
     for name, (binary, physics, build, gamma, steps, typefull, limiter, bc, divbfix) in CASES.items():
         w0, coords = build(gamma)

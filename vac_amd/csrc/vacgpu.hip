@@ -1117,7 +1117,7 @@ int vacgpu_getdt_courant(vacgpu_grid* g, const vacgpu_method* m, double* dtcoura
 }
 
 int vacgpu_advect(vacgpu_grid* g, const vacgpu_method* m, int idimmin, int idimmax, double dt,
-                  int firstsweep, int lastsweep, double* dtcourant, int* nerror) {
+                  int firstsweep, int lastsweep, double* dtcourant, int* nerror, double* smallp) {
   if (!g) return VACGPU_E_ARG;
   std::string why;
   if (check_method(&g->cfg, m, why)) return fail(g, VACGPU_E_ARG, why);
@@ -1125,23 +1125,30 @@ int vacgpu_advect(vacgpu_grid* g, const vacgpu_method* m, int idimmin, int idimm
   if (idimmin != idimmax) return fail(g, VACGPU_E_ARG, "only dimensionally split sweeps (idimmin == idimmax) are on the GPU path");
   if (idimmin < 1 || idimmin > g->cfg.ndim) return fail(g, VACGPU_E_ARG, "idim out of range");
   int rc;
-  if (dtcourant || nerror) {
+  if (dtcourant || nerror || smallp) {
     if ((rc = read_scalars(g))) return rc;
     if (dtcourant)
       for (int d = 0; d < g->cfg.ndim; ++d) g->sc_host->dtcourant[d] = dtcourant[d];
     if (nerror)
       for (int i = 0; i < 4; ++i) g->sc_host->nerror[i] = nerror[i];
+    if (smallp && *smallp >= 0.0) g->sc_host->smallp = *smallp;
     if ((rc = write_scalars(g))) return rc;
   }
+  if (!smallp) {
+    if ((rc = read_scalars(g))) return rc;
+  }
+  if (g->sc_host->smallp < 0.0)
+    return fail(g, VACGPU_E_STATE, "smallp is not set: call vacgpu_getdt_courant first or pass smallp >= 0");
   hipLaunchKernelGGL(k_set_dt, dim3(1), dim3(1), 0, g->stream, g->sc, dt);
   if ((rc = launch_check(g, "k_set_dt"))) return rc;
   if ((rc = sweep_device(g, m, idimmin - 1, firstsweep, lastsweep))) return rc;
-  if (dtcourant || nerror) {
+  if (dtcourant || nerror || smallp) {
     if ((rc = read_scalars(g))) return rc;
     if (dtcourant)
       for (int d = 0; d < g->cfg.ndim; ++d) dtcourant[d] = g->sc_host->dtcourant[d];
     if (nerror)
       for (int i = 0; i < 4; ++i) nerror[i] = g->sc_host->nerror[i];
+    if (smallp) *smallp = g->sc_host->smallp;
   }
   return 0;
 }

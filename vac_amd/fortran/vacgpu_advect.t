@@ -1,0 +1,182 @@
+!##############################################################################
+! module vacgpu_advect -- MI355X drop-in for VAC's advect (src/vac.t:363-486)
+!
+! Linked into VAC in place of the advect subroutine of vac.t.  Every call
+! advances one dimensionally split sweep on the GPU through libvacgpu (the C
+! ABI of include/vacgpu.h, bound in vacgpu_mod.f90).  The selectors come from
+! VAC's own common block, so the par file keeps choosing typefull,
+! typelimiter, courantpar, divbfix, typeB, ...  A selector the GPU path does
+! not implement stops the run through die(); nothing falls back to the CPU.
+!
+! State transfer: w is copied to the device at the first sweep of a time step
+! and back at the last one, so the rest of VAC (saves, the first step's
+! getdt_courant, residuals, filters, user hooks) always sees a current w.
+! dtcourant, nerror and smallp are exchanged with the common block on every
+! call, as the CPU advect would update them.
+!=============================================================================
+subroutine advect(method,ix^L,iws,idim^LIM,w1,w)
+
+use iso_c_binding
+use vacgpu_mod
+
+include 'vacdef.f'
+
+character*^LENTYPE :: method
+integer:: ix^L,iws(niw_),idim^LIM
+double precision:: w1(ixG^T,nw),w(ixG^T,nw)
+
+logical:: firstsweep,lastsweep
+common/first/firstsweep,lastsweep
+
+type(c_ptr), save :: grid
+logical, save :: created
+data created /.false./
+type(vacgpu_config), save :: cfg
+type(vacgpu_method) :: m
+integer(c_int) :: rc
+integer(c_int), target :: ld(3), ne(4)
+integer:: iw, iB, code, ngx(3), mlo(3), mhi(3)
+double precision, save, target :: dxl(ixGhimax,3)
+real(c_double), target :: dtc(3), sp
+!-----------------------------------------------------------------------------
+
+if(ix^L/=ixG^L|.or.|.or.) call die('vacgpu advect: subgrids are not supported')
+if(idimmin/=idimmax) call die('vacgpu advect: needs dimsplit=T')
+if(nproc(1)/=0 .or. nproc(2)/=0) call die('vacgpu advect: nproc is not on the GPU path')
+if(typeaxial/='slab' .or. gencoord) call die('vacgpu advect: slab Cartesian grids only')
+if(useprimitive) call die('vacgpu advect: useprimitive=T is not on the GPU path')
+if(typelimited/='original') call die('vacgpu advect: typelimited must be original')
+if(smallfix .or. extraB) call die('vacgpu advect: smallfix/extraB are not on the GPU path')
+if(acmexpo>zero) call die('vacgpu advect: acmexpo>0 is not on the GPU path')
+if(ndir/=ndim) call die('vacgpu advect: ndir must equal ndim')
+do iw=1,nw
+   if(acmcoef(iw)>=zero) call die('vacgpu advect: acmcoef is not on the GPU path')
+   if(typeentropy(iw)/='nul') call die('vacgpu advect: typeentropy must be nul')
+end do
+select case(method)
+case('tvdlf')
+   if(typeadvance/='twostep' .or. typepred1/='hancock') &
+      call die('vacgpu advect: tvdlf needs typeadvance=twostep, typepred=hancock')
+   m%typefull=VACGPU_TVDLF
+case('tvd')
+   if(typeadvance/='onestep') call die('vacgpu advect: tvd needs typeadvance=onestep')
+   if(typetvd/='roe') call die('vacgpu advect: typetvd must be roe')
+   m%typefull=VACGPU_TVD
+case default
+   call die('vacgpu advect: typefull='//method//' is not on the GPU path')
+end select
+
+^D&ld(^D)=ixGhi^D;
+if(ndim<3) ld(3)=1
+if(ndim<2) ld(2)=1
+
+if(.not.created)then
+   cfg%ndim=ndim
+   select case(typephys)
+   case('mhd')
+      cfg%physics=VACGPU_MHD
+   case('hd')
+      cfg%physics=VACGPU_HD
+   case default
+      call die('vacgpu advect: physics '//typephys//' is not on the GPU path')
+   end select
+   cfg%nx=1
+   ^D&cfg%nx(^D)=ixMmax^D-ixMmin^D+1;
+   cfg%eqpar=zero
+   cfg%eqpar(1:min(2,neqpar))=eqpar(1:min(2,neqpar))
+   if(nB/=2*ndim) call die('vacgpu advect: only the default boundary regions are supported')
+   cfg%typeB=VACGPU_CONT
+   do iB=1,nB
+      do iw=1,nw
+         code=vacgpu_bc_code(typeB(iw,iB))
+         if(code<0) call die('vacgpu advect: typeB='//typeB(iw,iB)//' is not on the GPU path')
+         cfg%typeB(iw,iB)=code
+      end do
+   end do
+   cfg%precision=VACGPU_FP64
+   ngx=1
+   mlo=1
+   mhi=1
+   ^D&ngx(^D)=ixGmax^D;
+   ^D&mlo(^D)=ixMmin^D;
+   ^D&mhi(^D)=ixMmax^D;
+   call vacgpu_dxlines(dx,ld,ndim,ngx,mlo,mhi,dxl,ixGhimax)
+   rc=vacgpu_create(cfg,c_loc(dxl(1,1)),c_loc(dxl(1,2)),c_loc(dxl(1,3)),grid)
+   if(rc/=0) call die('vacgpu advect: vacgpu_create failed (GPU missing or unsupported grid)')
+   created=.true.
+endif
+
+m%typelimiter=VACGPU_MINMOD
+do iw=1,nw
+   code=vacgpu_limiter_code(typelimiter(iw))
+   if(code<0) call die('vacgpu advect: typelimiter='//typelimiter(iw)//' is not on the GPU path')
+   m%typelimiter(iw)=code
+end do
+m%divbfix=merge(1,0,divbfix)
+m%divbwave=merge(1,0,divbwave)
+m%sourceunsplit=merge(1,0,sourceunsplit)
+m%dimsplit_xy=merge(1,0,typedimsplit=='xy')
+m%courantpar=courantpar
+if(implpar>=zero) m%courantpar=-one
+m%smallpcoeff=smallpcoeff
+
+if(firstsweep)then
+   rc=vacgpu_put_state(grid,w,ld)
+   if(rc/=0) call die('vacgpu advect: '//vacgpu_errmsg(grid))
+endif
+
+dtc=bigdouble
+dtc(1:ndim)=dtcourant(1:ndim)
+ne=nerror(1:4)
+sp=smallp
+rc=vacgpu_advect(grid,m,idimmin,idimmax,dt,merge(1,0,firstsweep),merge(1,0,lastsweep), &
+   c_loc(dtc),c_loc(ne),c_loc(sp))
+if(rc/=0) call die('vacgpu advect: '//vacgpu_errmsg(grid))
+dtcourant(1:ndim)=dtc(1:ndim)
+nerror(1:4)=ne
+smallp=sp
+
+if(lastsweep)then
+   rc=vacgpu_get_state(grid,w,ld)
+   if(rc/=0) call die('vacgpu advect: '//vacgpu_errmsg(grid))
+endif
+
+firstsweep=.false.
+
+return
+end
+
+!=============================================================================
+subroutine vacgpu_dxlines(dxa,ld,nd,ngx,mlo,mhi,dxl,nmax)
+
+! Cell widths along each dimension over ixG, taken on the line through the
+! first mesh cell, after checking that dx(:,...,d) does not vary across the
+! mesh (a slab Cartesian grid, src/vacgrid.t:527-530).
+
+integer:: ld(3),nd,ngx(3),mlo(3),mhi(3),nmax
+double precision:: dxa(*),dxl(nmax,3)
+integer:: d,i,j,k,ic(3),lin
+dxl=1.d0
+do d=1,nd
+   do i=1,ngx(d)
+      ic=mlo
+      if(nd<3) ic(3)=1
+      if(nd<2) ic(2)=1
+      ic(d)=i
+      lin=ic(1)+(ic(2)-1)*ld(1)+(ic(3)-1)*ld(1)*ld(2)+(d-1)*ld(1)*ld(2)*ld(3)
+      dxl(i,d)=dxa(lin)
+   end do
+   do k=mlo(3),mhi(3)
+      do j=mlo(2),mhi(2)
+         do i=mlo(1),mhi(1)
+            ic(1)=i; ic(2)=j; ic(3)=k
+            if(nd<3) ic(3)=1
+            lin=ic(1)+(ic(2)-1)*ld(1)+(ic(3)-1)*ld(1)*ld(2)+(d-1)*ld(1)*ld(2)*ld(3)
+            if(dxa(lin)/=dxl(ic(d),d)) call die('vacgpu advect: dx varies across the mesh')
+         end do
+      end do
+   end do
+end do
+
+return
+end

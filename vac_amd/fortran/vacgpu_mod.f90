@@ -1,0 +1,205 @@
+!==============================================================================
+! vacgpu_mod -- Fortran 2003 ISO_C_BINDING interface to libvacgpu (the MI355X
+! advect path), for a Fortran 90 host such as VAC.
+!
+! The derived types are interoperable with the structs of include/vacgpu.h.
+! C's  int typeB[6][8]  is  typeB(8,6)  here, i.e. typeB(iw,iB) exactly as
+! VAC's own typeB array (src/vacdef.t:110-112).  Pointers that C allows to be
+! NULL are passed as type(c_ptr), value (c_loc(x) or c_null_ptr).
+!==============================================================================
+module vacgpu_mod
+  use iso_c_binding
+  implicit none
+
+  integer(c_int), parameter :: VACGPU_OK = 0, VACGPU_E_ARG = -1, VACGPU_E_HIP = -2, &
+                               VACGPU_E_STATE = -3, VACGPU_E_NOMEM = -4
+  integer(c_int), parameter :: VACGPU_HD = 0, VACGPU_MHD = 1
+  integer(c_int), parameter :: VACGPU_TVDLF = 0, VACGPU_TVD = 1
+  integer(c_int), parameter :: VACGPU_MINMOD = 0, VACGPU_WOODWARD = 1, VACGPU_SUPERBEE = 2
+  integer(c_int), parameter :: VACGPU_CONT = 0, VACGPU_PERIODIC = 1
+  integer(c_int), parameter :: VACGPU_FP64 = 0, VACGPU_FP32BULK = 1
+
+  type, bind(C) :: vacgpu_config
+     integer(c_int) :: ndim
+     integer(c_int) :: physics
+     integer(c_int) :: nx(3)
+     real(c_double) :: eqpar(2)
+     integer(c_int) :: typeB(8, 6)
+     integer(c_int) :: precision
+  end type vacgpu_config
+
+  type, bind(C) :: vacgpu_method
+     integer(c_int) :: typefull
+     integer(c_int) :: typelimiter(8)
+     integer(c_int) :: divbfix
+     integer(c_int) :: divbwave
+     integer(c_int) :: sourceunsplit
+     integer(c_int) :: dimsplit_xy
+     real(c_double) :: courantpar
+     real(c_double) :: smallpcoeff
+  end type vacgpu_method
+
+  interface
+     integer(c_int) function vacgpu_create(cfg, dx0, dx1, dx2, grid) bind(C, name='vacgpu_create')
+       import :: c_int, c_ptr, vacgpu_config
+       type(vacgpu_config), intent(in) :: cfg
+       type(c_ptr), value :: dx0, dx1, dx2
+       type(c_ptr), intent(out) :: grid
+     end function vacgpu_create
+
+     integer(c_int) function vacgpu_set_stream(grid, stream) bind(C, name='vacgpu_set_stream')
+       import :: c_int, c_ptr
+       type(c_ptr), value :: grid, stream
+     end function vacgpu_set_stream
+
+     integer(c_int) function vacgpu_put_state(grid, w, ld) bind(C, name='vacgpu_put_state')
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: grid
+       real(c_double), intent(in) :: w(*)
+       integer(c_int), intent(in) :: ld(*)
+     end function vacgpu_put_state
+
+     integer(c_int) function vacgpu_get_state(grid, w, ld) bind(C, name='vacgpu_get_state')
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: grid
+       real(c_double), intent(inout) :: w(*)
+       integer(c_int), intent(in) :: ld(*)
+     end function vacgpu_get_state
+
+     integer(c_int) function vacgpu_getdt_courant(grid, m, dtcourant, smallp, dt) &
+          bind(C, name='vacgpu_getdt_courant')
+       import :: c_int, c_ptr, c_double, vacgpu_method
+       type(c_ptr), value :: grid
+       type(vacgpu_method), intent(in) :: m
+       type(c_ptr), value :: dtcourant, smallp
+       real(c_double), intent(out) :: dt
+     end function vacgpu_getdt_courant
+
+     integer(c_int) function vacgpu_advect(grid, m, idimmin, idimmax, dt, firstsweep, lastsweep, &
+          dtcourant, nerror, smallp) bind(C, name='vacgpu_advect')
+       import :: c_int, c_ptr, c_double, vacgpu_method
+       type(c_ptr), value :: grid
+       type(vacgpu_method), intent(in) :: m
+       integer(c_int), value :: idimmin, idimmax, firstsweep, lastsweep
+       real(c_double), value :: dt
+       type(c_ptr), value :: dtcourant, nerror, smallp
+     end function vacgpu_advect
+
+     integer(c_int) function vacgpu_run(grid, m, nsteps, itmax, tmax, tmaxexact, dtpar, t_hist) &
+          bind(C, name='vacgpu_run')
+       import :: c_int, c_ptr, c_double, vacgpu_method
+       type(c_ptr), value :: grid
+       type(vacgpu_method), intent(in) :: m
+       integer(c_int), value :: nsteps, itmax, tmaxexact
+       real(c_double), value :: tmax, dtpar
+       type(c_ptr), value :: t_hist
+     end function vacgpu_run
+
+     integer(c_int) function vacgpu_set_time(grid, it, t) bind(C, name='vacgpu_set_time')
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: grid
+       integer(c_int), value :: it
+       real(c_double), value :: t
+     end function vacgpu_set_time
+
+     integer(c_int) function vacgpu_get_time(grid, it, t) bind(C, name='vacgpu_get_time')
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: grid
+       integer(c_int), intent(out) :: it
+       real(c_double), intent(out) :: t
+     end function vacgpu_get_time
+
+     integer(c_int) function vacgpu_get_scalars(grid, dtcourant, nerror, smallp, dt) &
+          bind(C, name='vacgpu_get_scalars')
+       import :: c_int, c_ptr
+       type(c_ptr), value :: grid, dtcourant, nerror, smallp, dt
+     end function vacgpu_get_scalars
+
+     integer(c_int) function vacgpu_profile(grid, enable) bind(C, name='vacgpu_profile')
+       import :: c_int, c_ptr
+       type(c_ptr), value :: grid
+       integer(c_int), value :: enable
+     end function vacgpu_profile
+
+     integer(c_int) function vacgpu_kernel_time(grid, k, avg_ms, count) &
+          bind(C, name='vacgpu_kernel_time')
+       import :: c_int, c_ptr, c_double, c_long
+       type(c_ptr), value :: grid
+       integer(c_int), value :: k
+       real(c_double), intent(out) :: avg_ms
+       integer(c_long), intent(out) :: count
+     end function vacgpu_kernel_time
+
+     integer(c_int) function vacgpu_supported(cfg, m) bind(C, name='vacgpu_supported')
+       import :: c_int, vacgpu_config, vacgpu_method
+       type(vacgpu_config), intent(in) :: cfg
+       type(vacgpu_method), intent(in) :: m
+     end function vacgpu_supported
+
+     type(c_ptr) function vacgpu_last_error(grid) bind(C, name='vacgpu_last_error')
+       import :: c_ptr
+       type(c_ptr), value :: grid
+     end function vacgpu_last_error
+
+     subroutine vacgpu_destroy(grid) bind(C, name='vacgpu_destroy')
+       import :: c_ptr
+       type(c_ptr), value :: grid
+     end subroutine vacgpu_destroy
+
+     integer(c_size_t) function c_strlen(s) bind(C, name='strlen')
+       import :: c_size_t, c_ptr
+       type(c_ptr), value :: s
+     end function c_strlen
+  end interface
+
+contains
+
+  ! The message of the last failed call on grid, as a Fortran string.
+  function vacgpu_errmsg(grid) result(msg)
+    type(c_ptr), intent(in) :: grid
+    character(len=:), allocatable :: msg
+    type(c_ptr) :: p
+    character(kind=c_char), pointer :: chars(:)
+    integer :: n, i
+    p = vacgpu_last_error(grid)
+    if (.not. c_associated(p)) then
+       msg = 'unknown error'
+       return
+    end if
+    n = int(c_strlen(p))
+    call c_f_pointer(p, chars, [n])
+    allocate(character(len=n) :: msg)
+    do i = 1, n
+       msg(i:i) = chars(i)
+    end do
+  end function vacgpu_errmsg
+
+  ! VAC limiter name -> code (src/vacgrid.t:938-996); -1 if not on the GPU path
+  integer(c_int) function vacgpu_limiter_code(name)
+    character(len=*), intent(in) :: name
+    select case (trim(name))
+    case ('minmod')
+       vacgpu_limiter_code = VACGPU_MINMOD
+    case ('woodward')
+       vacgpu_limiter_code = VACGPU_WOODWARD
+    case ('superbee')
+       vacgpu_limiter_code = VACGPU_SUPERBEE
+    case default
+       vacgpu_limiter_code = -1
+    end select
+  end function vacgpu_limiter_code
+
+  ! VAC boundary type -> code (src/vacgrid.t:195-413); -1 if not on the GPU path
+  integer(c_int) function vacgpu_bc_code(name)
+    character(len=*), intent(in) :: name
+    select case (trim(name))
+    case ('cont')
+       vacgpu_bc_code = VACGPU_CONT
+    case ('periodic')
+       vacgpu_bc_code = VACGPU_PERIODIC
+    case default
+       vacgpu_bc_code = -1
+    end select
+  end function vacgpu_bc_code
+
+end module vacgpu_mod

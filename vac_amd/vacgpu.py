@@ -51,7 +51,7 @@ SIGNATURES = {
     "vacgpu_get_state": ([C.c_void_p, _DP, _IP], C.c_int),
     "vacgpu_getdt_courant": ([C.c_void_p, C.POINTER(Method), _DP, _DP, _DP], C.c_int),
     "vacgpu_advect": ([C.c_void_p, C.POINTER(Method), C.c_int, C.c_int, C.c_double, C.c_int,
-                       C.c_int, _DP, _IP], C.c_int),
+                       C.c_int, _DP, _IP, _DP], C.c_int),
     "vacgpu_run": ([C.c_void_p, C.POINTER(Method), C.c_int, C.c_int, C.c_double, C.c_int,
                     C.c_double, _DP], C.c_int),
     "vacgpu_set_time": ([C.c_void_p, C.c_int, C.c_double], C.c_int),
@@ -225,11 +225,14 @@ class Grid:
         return dt.value, dtc, (sp.value if sp is not None else None)
 
     def advect(self, method: MethodSpec, idim: int, dt: float, firstsweep: bool,
-               lastsweep: bool, dtcourant=None, nerror=None):
+               lastsweep: bool, dtcourant=None, nerror=None, smallp=None):
         m = method.to_c()
         ne = None if nerror is None else nerror.ctypes.data_as(_IP)
+        sp = None if smallp is None else C.c_double(smallp)
         self._check(self.lib.vacgpu_advect(self.h, C.byref(m), idim, idim, dt, int(firstsweep),
-                                           int(lastsweep), _ptr(dtcourant), ne), "advect")
+                                           int(lastsweep), _ptr(dtcourant), ne,
+                                           C.byref(sp) if sp is not None else None), "advect")
+        return sp.value if sp is not None else None
 
     def run(self, method: MethodSpec, nsteps: int, itmax: int = 1 << 30, tmax: float = 1e99,
             tmaxexact: bool = False, dtpar: float = -1.0, record_t: bool = False):
@@ -264,7 +267,7 @@ def host_loop(grid: Grid, method: MethodSpec, nsteps: int, it0: int = 0, t0: flo
         dt, dtcourant, smallp = grid.getdt_courant(method, dtcourant, smallp)
         order = list(range(ndim)) if (it % 2 == 0 or method.dimsplit_xy) else list(range(ndim - 1, -1, -1))
         for k, d in enumerate(order):
-            grid.advect(method, d + 1, dt, k == 0, k == ndim - 1, dtcourant, nerror)
+            grid.advect(method, d + 1, dt, k == 0, k == ndim - 1, dtcourant, nerror, smallp)
         it += 1
         t = t + dt
         hist.append(t)
