@@ -1,0 +1,53 @@
+"""FP32-bulk tier against the FP64 oracle.
+
+In the FP32-bulk tier the flux, limiter, wave-speed and Roe eigen-system
+arithmetic runs in FP32, while every update of a conserved variable, the
+thermal / total pressure recovery p = (gamma-1)(e - kinetic - magnetic), the
+source terms and the dt reduction stay in FP64.
+
+Tolerance (stated by the project): relative L1 error <= 1e-5 per conserved
+variable against the FP64 oracle after the stated number of steps.  The L1
+and Linf errors are printed for every case.
+"""
+import numpy as np
+import pytest
+
+import vacref
+from vac_amd import vacgpu
+from vac_amd.grid import widths_1d
+from vac_amd.initial import orszag_tang, random_fourier
+
+pytestmark = pytest.mark.gpu
+
+TOL_REL_L1 = 1e-5
+
+CASES = [
+    ("ot128-tvdlf-minmod", lambda g: orszag_tang(128, g), "mhd", 5 / 3, "tvdlf", "minmod", 100),
+    ("rf256-tvd-woodward", lambda g: random_fourier((256, 256), 2, g), "mhd", 5 / 3, "tvd", "woodward", 50),
+    ("rf3d32-tvdlf-minmod", lambda g: random_fourier((32, 32, 32), 3, g), "mhd", 5 / 3, "tvdlf", "minmod", 20),
+    ("hd-rf192-tvdlf-woodward", lambda g: random_fourier((192, 160), 4, g, physics="hd"), "hd", 1.4, "tvdlf", "woodward", 60),
+]
+
+
+@pytest.mark.parametrize("case", CASES, ids=[c[0] for c in CASES])
+def test_fp32bulk_error_against_fp64_oracle(case):
+    name, build, physics, gamma, typefull, limiter, steps = case
+    w0, coords = build(gamma)
+    nx = [len(c) for c in coords]
+    dxs = [widths_1d(c) for c in coords]
+    method = vacgpu.MethodSpec(typefull=typefull, limiter=limiter)
+    print(f"case {name}", flush=True)
+    g = vacgpu.Grid(nx, physics, gamma, dxs, precision="fp32bulk")
+    g.put_mesh(w0)
+    n, t = g.run(method, steps, record_t=True)
+    w = g.get_mesh()
+    g.close()
+    p = vacref.make_params(len(nx), physics, nx, gamma, typefull=typefull, limiter=limiter)
+    wr, tr, st = vacref.run(p, w0, dxs, steps)
+    assert n == steps
+    for iw in range(w.shape[0]):
+        l1 = np.abs(w[iw] - wr[iw]).sum() / np.abs(wr[iw]).sum()
+        linf = np.abs(w[iw] - wr[iw]).max() / np.abs(wr[iw]).max()
+        print(f"  {name} iw={iw} rel L1 {l1:.3e} rel Linf {linf:.3e}")
+        assert l1 <= TOL_REL_L1, f"variable {iw}: rel L1 {l1:.3e} > {TOL_REL_L1}"
+    assert abs(t[-1] - tr[-1]) <= 1e-5 * tr[-1]

@@ -3,7 +3,9 @@
     python -m vac_amd.build
 
 FP64 parity needs the arithmetic exactly as written: FMA contraction is off
-(-ffp-contract=off) and packed-FP32 instruction selection is disabled.
+(-ffp-contract=off) and packed-FP32 instruction selection is disabled.  FP32
+divide and square root (used only by the FP32-bulk tier) run at hardware
+rate instead of the correctly rounded software sequence.
 """
 from __future__ import annotations
 
@@ -22,6 +24,9 @@ FLAGS = [
     "-O3", "-std=c++17", "-fPIC", "-shared",
     f"--offload-arch={ARCH}",
     "-ffp-contract=off",
+    # FP32-bulk tier only: hardware-rate FP32 divide and sqrt (about 1 ulp);
+    # FP64 arithmetic is IEEE correctly rounded either way
+    "-fno-hip-fp32-correctly-rounded-divide-sqrt",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-I", os.path.join(ROOT, "include"),
     "-Wall", "-Wno-unused-function",

@@ -176,7 +176,19 @@ struct Roe {
   R dp, rhodv;
   R bdvf, bdbf;  // fast/slow-wave projections (il == fastRW)
   R bdva, bdba;  // Alfven-wave projections (ND == 3)
-  R v2a2, bvf, bva;  // rtimes helpers for the energy row
+  R v2a2, bvf, bva = R(0);  // rtimes helpers for the energy row
+};
+
+// The part of the interface state that rtimes needs (sqcs2 = sqrt(cs2) as
+// computed inline at src/vacphys.mhdroe.t:430-434).  All members have type R,
+// so a RoeRt can be moved between lanes member by member.
+template <int ND, class R>
+struct RoeRt {
+  R sq;
+  R v[ND];
+  R bw[ND];
+  R cs2, sqcs2, cf, cs, af, as;
+  R v2a2, bvf, bva;
 };
 
 // average + the parts of geteigenjump that do not depend on il
@@ -270,6 +282,27 @@ __device__ __forceinline__ void roe_setup(const double* uL, const double* uR, in
   }
 }
 
+template <int ND, class R>
+__device__ __forceinline__ RoeRt<ND, R> roe_rt(const Roe<ND, R>& r) {
+  RoeRt<ND, R> t;
+  t.sq = r.sq;
+#pragma unroll
+  for (int c = 0; c < ND; ++c) {
+    t.v[c] = r.v[c];
+    t.bw[c] = r.bw[c];
+  }
+  t.cs2 = r.cs2;
+  t.sqcs2 = R(sqrt(r.cs2));
+  t.cf = r.cf;
+  t.cs = r.cs;
+  t.af = r.af;
+  t.as = r.as;
+  t.v2a2 = r.v2a2;
+  t.bvf = r.bvf;
+  t.bva = (ND == 3) ? r.bva : R(0);
+  return t;
+}
+
 // Wave speed a and characteristic jump of wave il (geteigenjump
 // src/vacphys.mhdroe.t:188-226, entropy fix 'nul').
 template <int ND, class R>
@@ -320,7 +353,7 @@ __device__ __forceinline__ void roe_wave(const Roe<ND, R>& r, const double* uL, 
 
 // R(iw, il) * q (rtimes, src/vacphys.mhdroe.t:287-450)
 template <int ND, class R>
-__device__ __forceinline__ R roe_rtimes(const Roe<ND, R>& r, R q, int iw, int il, int d, int divbwave) {
+__device__ __forceinline__ R roe_rtimes(const RoeRt<ND, R>& r, R q, int iw, int il, int d, int divbwave) {
   using L = Layout<ND, true>;
   const int idir = (d + 1) % ND;
   const int jdir = (idir + 1) % ND;
@@ -381,15 +414,28 @@ __device__ __forceinline__ R roe_rtimes(const Roe<ND, R>& r, R q, int iw, int il
   const R beta = r.bw[c];
   switch (il) {
     case 0:
-    case 1: return +q * as * R(sqrt(r.cs2)) * beta / r.sq;
+    case 1: return +q * as * r.sqcs2 * beta / r.sq;
     case 2:
-    case 3: return -(q * af * R(sqrt(r.cs2)) * beta / r.sq);
+    case 3: return -(q * af * r.sqcs2 * beta / r.sq);
     case 4:
     case 5: return R(0);
     default:
       if (c == idir) return -(q * r.bw[jdir] / fsign(r.sq, r.bw[d]));
       return +q * r.bw[idir] / fsign(r.sq, r.bw[d]);
   }
+}
+
+// getphi for method 'tvd', typetvd 'roe', entropy fix 'nul'
+// (src/vactvd.t:246-247, :294-302): jm, j0, jp are the jumps of one wave at
+// faces J-1, J, J+1 and av = a*dt*2/(dx_J+dx_J+1) at face J.
+template <class R>
+__device__ __forceinline__ R tvd_phi(int lim, R jm, R j0, R jp, R av) {
+  R ph = R(fabs(av)) - av * av;
+  const R ljl = limiter<R>(lim, j0, jm);  // ljump(J)
+  const R ljr = limiter<R>(lim, jp, j0);  // ljump(J+1)
+  if (av <= R(0)) ph = ph * (j0 - ljr);
+  else ph = ph * (j0 - ljl);
+  return ph + av * av * j0;
 }
 
 }  // namespace vacgpu

@@ -179,10 +179,11 @@ __global__ __launch_bounds__(256) void k_bound(Geo g, BC bc, double* __restrict_
 // ------------------------------------------------------------ TVDLF
 // Hancock predictor (src/vactvdlf.t:5-89) with qdt = dt/2:
 // w1 = w - qdt/dx*(F(w+l/2) - F(w-l/2)) - (qdt/ndim)*S_Powell(w)
-template <int ND, bool MHD, class R>
+template <int ND, bool MHD, class R, int D>
 __global__ __launch_bounds__(256) void k_tvdlf_pred(Geo g, Sel s, const double* __restrict__ w,
                                                    double* __restrict__ w1,
-                                                   const Scal* __restrict__ sc, int d) {
+                                                   const Scal* __restrict__ sc) {
+  constexpr int d = D;
   using L = Layout<ND, MHD>;
   constexpr int NW = L::nw;
   const CellIx cl = cell_of(g);
@@ -221,13 +222,13 @@ __global__ __launch_bounds__(256) void k_tvdlf_pred(Geo g, Sel s, const double* 
 // (typelimited='original'), local Lax-Friedrichs flux with cmax of the mean
 // state, Powell source of w1 with dt/ndim.  Writes the partial max of
 // 2*cmax/(dx_j+dx_j+1) (src/vactvdlf.t:190) to part[SLOT_C0].
-template <int ND, bool MHD, class R>
+template <int ND, bool MHD, class R, int D>
 __global__ __launch_bounds__(256) void k_tvdlf_corr(Geo g, Sel s, const double* __restrict__ w,
                                                    const double* __restrict__ w1,
                                                    double* __restrict__ wout,
                                                    const Scal* __restrict__ sc,
-                                                   double* __restrict__ part, long long pstride,
-                                                   int d) {
+                                                   double* __restrict__ part, long long pstride) {
+  constexpr int d = D;
   using L = Layout<ND, MHD>;
   constexpr int NW = L::nw;
   __shared__ double sm[8];
@@ -319,117 +320,254 @@ __global__ __launch_bounds__(256) void k_tvd_halfsrc(Geo g, const double* __rest
   for (int iw = 0; iw < NW; ++iw) w1[iw * g.ps + cl.idx] = out[iw];
 }
 
-// One TVD sweep for cell i: the centred flux difference of wCT (centdiff,
+// ---- TVD sweep.  For cell i: the centred flux difference of wCT (centdiff,
 // src/vaccd.t:78-136) followed by the Roe-type dissipation of tvdlimit2
-// (src/vactvd.t:86-206, getphi 'roe' :244-302) over the faces i-1/2, i+1/2,
-// whose limiters need the characteristic jumps at i-3/2 and i+3/2 as well.
-// Writes the partial maxima of |a dt 2/(dx_j+dx_j+1)| of the two fast waves
-// over the faces j in [i-2, i+1] (covering ixIC) to part[SLOT_C0/C1].
+// (src/vactvd.t:86-206, getphi 'roe' :244-302) through the faces i-1/2 and
+// i+1/2, whose limiters also need the jumps at faces i-3/2 and i+3/2.  Face
+// quantities are computed once per face: along x each lane of a wavefront
+// owns one face and neighbours are read with cross-lane shuffles; along y
+// and z each thread marches through a segment of cells, keeping a sliding
+// window of faces in registers.  Both kernels write the partial maxima of
+// |a dt 2/(dx_J+dx_J+1)| of the two fast waves over the faces they compute
+// (covering ixIC, src/vactvd.t:42,145-157) to part[SLOT_C0/C1].
+
+// Everything one face contributes to the update of its two cells.
 template <int ND, class R>
-__global__ __launch_bounds__(256) void k_tvd_sweep(Geo g, Sel s, const double* __restrict__ wct,
-                                                  const double* __restrict__ win,
-                                                  double* __restrict__ wout,
-                                                  const Scal* __restrict__ sc,
-                                                  double* __restrict__ part, long long pstride,
-                                                  int d) {
+struct TvdFace {
+  R jump[Layout<ND, true>::nw];
+  R adt[Layout<ND, true>::nw];
+  RoeRt<ND, R> rt;
+};
+
+template <int ND, class R>
+__device__ __forceinline__ void tvd_face(const double* uL, const double* uR, int d, const Sel& s,
+                                         double smallp, double dt, double dxsum, TvdFace<ND, R>& f,
+                                         double& c0, double& c1) {
+  constexpr int NW = Layout<ND, true>::nw;
+  Roe<ND, R> r;
+  roe_setup<ND, R>(uL, uR, d, s.gamma, smallp, r);
+  const R sdx = R(dxsum);
+#pragma unroll
+  for (int il = 0; il < NW; ++il) {
+    R a, jp;
+    roe_wave<ND, R>(r, uL, uR, il, d, s.divbwave, a, jp);
+    const R ad = a * R(dt) * R(2) / sdx;
+    f.jump[il] = jp;
+    f.adt[il] = ad;
+  }
+  c0 = fabs(double(f.adt[0]));
+  c1 = fabs(double(f.adt[1]));
+  f.rt = roe_rt<ND, R>(r);
+}
+
+// out[iw] += sum over il (in order) of (R.phi(J) q(J) - R.phi(J-1) q(J-1)) / dV(i)
+template <int ND, class R>
+__device__ __forceinline__ void tvd_dissipate(const RoeRt<ND, R>& rm, const R* phim,
+                                              const RoeRt<ND, R>& r0, const R* phi0, int d,
+                                              const Sel& s, R dvm, R dv0, double dvi, double* out) {
+  constexpr int NW = Layout<ND, true>::nw;
+#pragma unroll
+  for (int il = 0; il < NW; ++il) {
+#pragma unroll
+    for (int iw = 0; iw < NW; ++iw) {
+      const R a0 = roe_rtimes<ND, R>(rm, phim[il], iw, il, d, s.divbwave) * R(0.25) * dvm;
+      const R a1 = roe_rtimes<ND, R>(r0, phi0[il], iw, il, d, s.divbwave) * R(0.25) * dv0;
+      out[iw] = out[iw] + double(a1 - a0) / dvi;
+    }
+  }
+}
+
+template <class T>
+__device__ __forceinline__ T shfl_up1(T v) { return __shfl_up(v, 1, 64); }
+template <class T>
+__device__ __forceinline__ T shfl_down1(T v) { return __shfl_down(v, 1, 64); }
+
+template <class S, class R>
+__device__ __forceinline__ S shfl_up_struct(const S& v) {
+  static_assert(sizeof(S) % sizeof(R) == 0, "struct of R");
+  S o;
+  const R* a = reinterpret_cast<const R*>(&v);
+  R* b = reinterpret_cast<R*>(&o);
+#pragma unroll
+  for (int k = 0; k < (int)(sizeof(S) / sizeof(R)); ++k) b[k] = shfl_up1(a[k]);
+  return o;
+}
+
+constexpr int TX_STRIDE = 61;  // cells updated per 64-lane wavefront along x
+
+// x sweep: lane L of a wavefront owns face J = x0 - 2 + L (between cells J and
+// J+1) and updates cell J for L in [2, 62].
+template <int ND, class R>
+__global__ __launch_bounds__(256) void k_tvd_x(Geo g, Sel s, const double* __restrict__ wct,
+                                              const double* __restrict__ win,
+                                              double* __restrict__ wout,
+                                              const Scal* __restrict__ sc,
+                                              double* __restrict__ part, long long pstride) {
   using L = Layout<ND, true>;
   constexpr int NW = L::nw;
   __shared__ double sm[8];
-  const CellIx cl = cell_of(g);
-  double c0 = -INFINITY, c1 = -INFINITY;
-  if (cl.ok) {
-    const double dt = sc->dt;
-    const double smallp = sc->smallp;
-    const long long sd = g.s[d];
-    const int ci = cl.c[d];
+  const double dt = sc->dt, smallp = sc->smallp;
+  const int lane = threadIdx.x;
+  const int x0 = 2 + (int)blockIdx.x * TX_STRIDE;
+  const int J = x0 - 2 + lane;
+  const int y = 2 + (int)(blockIdx.y * BY + threadIdx.y);
+  const int z = (g.nd == 3) ? 2 + (int)blockIdx.z : 0;
+  const bool rowok = y <= g.n[1] + 1;
+  const int yc = rowok ? y : g.n[1] + 1;
+  const int Jc = J < g.n[0] + 2 ? J : g.n[0] + 2;
+  const long long base = g.off + (long long)yc * g.s[1] + (long long)z * g.s[2];
+  double uL[NW], uR[NW];
+  load<NW>(wct, g.ps, base + Jc, uL);
+  load<NW>(wct, g.ps, base + Jc + 1, uR);
+  TvdFace<ND, R> f;
+  double c0, c1;
+  tvd_face<ND, R>(uL, uR, 0, s, smallp, dt, g.dx[0][Jc] + g.dx[0][Jc + 1], f, c0, c1);
+  const bool faceok = rowok && J <= g.n[0] + 2;
+  if (!faceok) { c0 = -INFINITY; c1 = -INFINITY; }
+  // centdiff: flux at cell J, face average fC(J) and fC(J-1)
+  R F[NW], fC[NW], fCm[NW];
+  fluxes<ND, true, R>(uL, 0, s.gamma, smallp, F);
+#pragma unroll
+  for (int iw = 0; iw < NW; ++iw) fC[iw] = R(0.5) * (F[iw] + shfl_down1(F[iw]));
+#pragma unroll
+  for (int iw = 0; iw < NW; ++iw) fCm[iw] = shfl_up1(fC[iw]);
+  // phi at face J from the jumps at J-1, J, J+1
+  R phi[NW];
+#pragma unroll
+  for (int il = 0; il < NW; ++il) {
+    const R jm = shfl_up1(f.jump[il]), jp = shfl_down1(f.jump[il]);
+    phi[il] = tvd_phi<R>(s.lim[il], jm, f.jump[il], jp, f.adt[il]);
+  }
+  R phim[NW];
+#pragma unroll
+  for (int il = 0; il < NW; ++il) phim[il] = shfl_up1(phi[il]);
+  const RoeRt<ND, R> rtm = shfl_up_struct<RoeRt<ND, R>, R>(f.rt);
+  const int i = J;
+  if (rowok && lane >= 2 && lane <= 62 && i <= g.n[0] + 1) {
+    const long long idx = base + i;
     double out[NW];
-    // centdiff: fC(j) = (F(j) + F(j+1))/2 of wCT
-    {
-      double u[NW];
-      R fm[NW], f0[NW], fp[NW];
-      load<NW>(wct, g.ps, cl.idx - sd, u);
-      fluxes<ND, true, R>(u, d, s.gamma, smallp, fm);
-      load<NW>(wct, g.ps, cl.idx, u);
-      fluxes<ND, true, R>(u, d, s.gamma, smallp, f0);
-      load<NW>(wct, g.ps, cl.idx + sd, u);
-      fluxes<ND, true, R>(u, d, s.gamma, smallp, fp);
-      const double qdx = dt / g.dx[d][ci];
+    const double qdx = dt / g.dx[0][i];
 #pragma unroll
-      for (int iw = 0; iw < NW; ++iw) {
-        const R fcR = R(0.5) * (f0[iw] + fp[iw]);
-        const R fcL = R(0.5) * (fm[iw] + f0[iw]);
-        out[iw] = win[iw * g.ps + cl.idx] - qdx * double(fcR - fcL);
-      }
+    for (int iw = 0; iw < NW; ++iw) out[iw] = win[iw * g.ps + idx] - qdx * double(fC[iw] - fCm[iw]);
+    // dvolume = (dx1*dx2)*dx3 evaluated left to right
+    auto dvol = [&](int xi) {
+      double v = g.dx[0][xi];
+      v = v * g.dx[1][yc];
+      if (ND == 3) v = v * g.dx[2][z];
+      return v;
+    };
+    const double dm = dvol(i - 1), d0 = dvol(i), dp = dvol(i + 1);
+    tvd_dissipate<ND, R>(rtm, phim, f.rt, phi, 0, s, R(d0 + dm), R(dp + d0), d0, out);
+#pragma unroll
+    for (int iw = 0; iw < NW; ++iw) wout[iw * g.ps + idx] = out[iw];
+  }
+  const double b0 = block_reduce<true>(c0, sm);
+  if (threadIdx.x == 0 && threadIdx.y == 0) part[SLOT_C0 * pstride + block_linear()] = b0;
+  const double b1 = block_reduce<true>(c1, sm);
+  if (threadIdx.x == 0 && threadIdx.y == 0) part[SLOT_C1 * pstride + block_linear()] = b1;
+}
+
+// y / z sweep: thread (x, segment) marches through cells s0..s1 along d.
+template <int ND, class R, int D>
+__global__ __launch_bounds__(256) void k_tvd_m(Geo g, Sel s, const double* __restrict__ wct,
+                                              const double* __restrict__ win,
+                                              double* __restrict__ wout,
+                                              const Scal* __restrict__ sc,
+                                              double* __restrict__ part, long long pstride,
+                                              int seglen) {
+  constexpr int d = D;
+  using L = Layout<ND, true>;
+  constexpr int NW = L::nw;
+  __shared__ double sm[8];
+  const double dt = sc->dt, smallp = sc->smallp;
+  const int x = 2 + (int)(blockIdx.x * BX + threadIdx.x);
+  const int seg = (int)(blockIdx.y * BY + threadIdx.y);
+  const int s0 = 2 + seg * seglen;
+  // the transverse coordinate that is neither x nor d (3D only)
+  const int t = (g.nd == 3) ? 2 + (int)blockIdx.z : 0;
+  const int nd_ = g.n[d];
+  const bool active = x <= g.n[0] + 1 && s0 <= nd_ + 1;
+  double c0 = -INFINITY, c1 = -INFINITY;
+  if (active) {
+    const int s1 = (s0 + seglen - 1 < nd_ + 1) ? s0 + seglen - 1 : nd_ + 1;
+    const int other = (d == 1) ? 2 : 1;  // dimension of t in 3D
+    const long long sd = g.s[d];
+    const long long base = g.off + x + (ND == 3 ? (long long)t * g.s[other] : 0);
+    const double* dxd = g.dx[d];
+    // volume of cell c along d: (dx1*dx2)*dx3 in dimension order
+    const double dxt = (ND == 3) ? g.dx[other][t] : 1.0;
+    auto dvol = [&](int c) {
+      double v = g.dx[0][x];
+      if (ND == 2) return v * dxd[c];
+      // d == 1: (dx0*dx1[c])*dx2[t];  d == 2: (dx0*dx1[t])*dx2[c]
+      if (d == 1) return (v * dxd[c]) * dxt;
+      return (v * dxt) * dxd[c];
+    };
+    double uL[NW], uR[NW];
+    TvdFace<ND, R> fm, f0, fp;  // faces i-1, i, i+1
+    R phim[NW], phi0[NW];
+    R Fm[NW], F0[NW], fCm[NW];
+    double a, b;
+    // prologue: faces s0-2, s0-1, s0; centred fluxes at cells s0-1, s0
+    load<NW>(wct, g.ps, base + (long long)(s0 - 2) * sd, uL);
+    load<NW>(wct, g.ps, base + (long long)(s0 - 1) * sd, uR);
+    tvd_face<ND, R>(uL, uR, d, s, smallp, dt, dxd[s0 - 2] + dxd[s0 - 1], fm, a, b);
+    c0 = c0 < a ? a : c0; c1 = c1 < b ? b : c1;
+    fluxes<ND, true, R>(uR, d, s.gamma, smallp, Fm);
+#pragma unroll
+    for (int iw = 0; iw < NW; ++iw) uL[iw] = uR[iw];
+    load<NW>(wct, g.ps, base + (long long)s0 * sd, uR);
+    tvd_face<ND, R>(uL, uR, d, s, smallp, dt, dxd[s0 - 1] + dxd[s0], f0, a, b);
+    c0 = c0 < a ? a : c0; c1 = c1 < b ? b : c1;
+    fluxes<ND, true, R>(uR, d, s.gamma, smallp, F0);
+#pragma unroll
+    for (int iw = 0; iw < NW; ++iw) fCm[iw] = R(0.5) * (Fm[iw] + F0[iw]);  // face s0-1
+#pragma unroll
+    for (int iw = 0; iw < NW; ++iw) uL[iw] = uR[iw];
+    load<NW>(wct, g.ps, base + (long long)(s0 + 1) * sd, uR);
+    tvd_face<ND, R>(uL, uR, d, s, smallp, dt, dxd[s0] + dxd[s0 + 1], fp, a, b);
+    c0 = c0 < a ? a : c0; c1 = c1 < b ? b : c1;
+    // phi at face s0-1 from faces s0-2, s0-1, s0
+#pragma unroll
+    for (int il = 0; il < NW; ++il) phim[il] = tvd_phi<R>(s.lim[il], fm.jump[il], f0.jump[il], fp.jump[il], f0.adt[il]);
+    // window now: fm <- face s0-1, f0 <- face s0 (fp holds face s0)
+    fm = f0;
+    f0 = fp;
+    for (int i = s0;; ++i) {
+      // f0 = face i, fm = face i-1 (with phim), uL = w(i), uR = w(i+1)
+      fluxes<ND, true, R>(uR, d, s.gamma, smallp, Fm);  // F(i+1), reuse Fm as scratch
+      R fC[NW];
+#pragma unroll
+      for (int iw = 0; iw < NW; ++iw) fC[iw] = R(0.5) * (F0[iw] + Fm[iw]);  // face i
+#pragma unroll
+      for (int iw = 0; iw < NW; ++iw) F0[iw] = Fm[iw];
+      // face i+1 from w(i+1), w(i+2)
+#pragma unroll
+      for (int iw = 0; iw < NW; ++iw) uL[iw] = uR[iw];
+      load<NW>(wct, g.ps, base + (long long)(i + 2) * sd, uR);
+      tvd_face<ND, R>(uL, uR, d, s, smallp, dt, dxd[i + 1] + dxd[i + 2], fp, a, b);
+      c0 = c0 < a ? a : c0; c1 = c1 < b ? b : c1;
+#pragma unroll
+      for (int il = 0; il < NW; ++il) phi0[il] = tvd_phi<R>(s.lim[il], fm.jump[il], f0.jump[il], fp.jump[il], f0.adt[il]);
+      // update cell i
+      const long long idx = base + (long long)i * sd;
+      double out[NW];
+      const double qdx = dt / dxd[i];
+#pragma unroll
+      for (int iw = 0; iw < NW; ++iw) out[iw] = win[iw * g.ps + idx] - qdx * double(fC[iw] - fCm[iw]);
+      const double dm = dvol(i - 1), d0 = dvol(i), dp = dvol(i + 1);
+      tvd_dissipate<ND, R>(fm.rt, phim, f0.rt, phi0, d, s, R(d0 + dm), R(dp + d0), d0, out);
+#pragma unroll
+      for (int iw = 0; iw < NW; ++iw) wout[iw * g.ps + idx] = out[iw];
+      if (i == s1) break;
+      // shift the window
+#pragma unroll
+      for (int iw = 0; iw < NW; ++iw) fCm[iw] = fC[iw];
+#pragma unroll
+      for (int il = 0; il < NW; ++il) phim[il] = phi0[il];
+      fm = f0;
+      f0 = fp;
     }
-    // characteristic decomposition at faces j = i-2 .. i+1 (k = 0..3)
-    R jump[4][NW], adt[2][NW];
-    Roe<ND, R> roe[2];
-#pragma unroll
-    for (int k = 0; k < 4; ++k) {
-      double uL[NW], uR[NW];
-      const long long jl = cl.idx + (k - 2) * sd;
-      load<NW>(wct, g.ps, jl, uL);
-      load<NW>(wct, g.ps, jl + sd, uR);
-      Roe<ND, R> r;
-      roe_setup<ND, R>(uL, uR, d, s.gamma, smallp, r);
-      const int J = ci + k - 2;
-      const R sdx = R(g.dx[d][J] + g.dx[d][J + 1]);
-#pragma unroll
-      for (int il = 0; il < NW; ++il) {
-        R a, jp;
-        roe_wave<ND, R>(r, uL, uR, il, d, s.divbwave, a, jp);
-        const R ad = a * R(dt) * R(2) / sdx;
-        jump[k][il] = jp;
-        if (k == 1 || k == 2) adt[k - 1][il] = ad;
-        if (il == 0) { double v = fabs(double(ad)); c0 = c0 < v ? v : c0; }
-        if (il == 1) { double v = fabs(double(ad)); c1 = c1 < v ? v : c1; }
-      }
-      if (k == 1) roe[0] = r;
-      if (k == 2) roe[1] = r;
-    }
-    // phi at faces i-1/2 (f = 0) and i+1/2 (f = 1), then R*phi into the cell
-    R dvf[2];
-    {
-      // dvolume of the two cells of each face, in dimension order
-      double dvc[3];
-#pragma unroll
-      for (int k = 0; k < 3; ++k) {
-        int cc[3] = {cl.c[0], cl.c[1], cl.c[2]};
-        cc[d] = ci + k - 1;
-        double v = g.dx[0][cc[0]];
-#pragma unroll
-        for (int dd = 1; dd < ND; ++dd) v = v * g.dx[dd][cc[dd]];
-        dvc[k] = v;
-      }
-      dvf[0] = R(dvc[1] + dvc[0]);
-      dvf[1] = R(dvc[2] + dvc[1]);
-      // dvolume of the cell itself
-      const double dvi = dvc[1];
-#pragma unroll
-      for (int il = 0; il < NW; ++il) {
-        R phi[2];
-#pragma unroll
-        for (int f = 0; f < 2; ++f) {
-          const R jm = jump[f][il], j0 = jump[f + 1][il], jp = jump[f + 2][il];
-          const R av = adt[f][il];
-          R ph = R(fabs(av)) - av * av;
-          const R ljl = limiter<R>(s.lim[il], j0, jm);   // ljump(J)
-          const R ljr = limiter<R>(s.lim[il], jp, j0);   // ljump(J+1)
-          if (av <= R(0)) ph = ph * (j0 - ljr);
-          else ph = ph * (j0 - ljl);
-          ph = ph + av * av * j0;
-          phi[f] = ph;
-        }
-#pragma unroll
-        for (int iw = 0; iw < NW; ++iw) {
-          const R r0 = roe_rtimes<ND, R>(roe[0], phi[0], iw, il, d, s.divbwave) * R(0.25) * dvf[0];
-          const R r1 = roe_rtimes<ND, R>(roe[1], phi[1], iw, il, d, s.divbwave) * R(0.25) * dvf[1];
-          out[iw] = out[iw] + double(r1 - r0) / dvi;
-        }
-      }
-    }
-#pragma unroll
-    for (int iw = 0; iw < NW; ++iw) wout[iw * g.ps + cl.idx] = out[iw];
   }
   const double b0 = block_reduce<true>(c0, sm);
   if (threadIdx.x == 0 && threadIdx.y == 0) part[SLOT_C0 * pstride + block_linear()] = b0;
@@ -528,7 +666,7 @@ __device__ double fold(const double* p, long long n, bool is_max, double* sm) {
 
 __global__ void k_fin_smallp(Scal* sc, const double* part, long long pstride, long long nb,
                              double coeff, int mhd) {
-  __shared__ double sm[8];
+  __shared__ double sm[16];
   if (sc->smallp >= 0.0) return;
   const double mn = fold(part + SLOT_PMIN * pstride, nb, false, sm);
   __syncthreads();
@@ -544,7 +682,7 @@ __global__ void k_fin_smallp(Scal* sc, const double* part, long long pstride, lo
 // clip (src/vac.t:83); dtpar > 0 replaces the Courant step.
 __global__ void k_fin_getdt(Scal* sc, const double* part, long long pstride, long long nb, int nd,
                             double courantpar, double dtpar, int tmaxexact) {
-  __shared__ double sm[8];
+  __shared__ double sm[16];
   double dtc[3] = {kBig, kBig, kBig};
   for (int d = 0; d < nd; ++d) dtc[d] = sc->dtcourant[d];
   double cmd[3] = {0.0, 0.0, 0.0};
@@ -580,7 +718,7 @@ __global__ void k_fin_getdt(Scal* sc, const double* part, long long pstride, lon
 // dtcourant update of the TVDLF corrector (src/vactvdlf.t:184-201)
 __global__ void k_fin_courant_tvdlf(Scal* sc, const double* part, long long pstride, long long nb,
                                     int d, double courantpar) {
-  __shared__ double sm[8];
+  __shared__ double sm[16];
   const double cm = fold(part + SLOT_C0 * pstride, nb, true, sm);
   if (threadIdx.x == 0) {
     if (sc->dt * cm > 1.0) sc->nerror[2] += 1;
@@ -594,7 +732,7 @@ __global__ void k_fin_courant_tvdlf(Scal* sc, const double* part, long long pstr
 // dtcourant update of tvdlimit2 for the two extreme waves (src/vactvd.t:145-157)
 __global__ void k_fin_courant_tvd(Scal* sc, const double* part, long long pstride, long long nb,
                                   int d, double courantpar) {
-  __shared__ double sm[8];
+  __shared__ double sm[16];
   for (int slot = SLOT_C0; slot <= SLOT_C1; ++slot) {
     const double cm = fold(part + slot * pstride, nb, true, sm);
     if (threadIdx.x == 0) {
@@ -633,6 +771,9 @@ struct vacgpu_grid {
   double* part = nullptr;
   long long pstride = 0;
   dim3 grid, block;
+  dim3 grid_tx;              // k_tvd_x
+  dim3 grid_tm[3];           // k_tvd_m per sweep direction
+  int seglen = 16;
   long long nbound = 0, slab[3] = {0, 0, 0};
   int cur = 0, old = 0;
   bool have_state = false;
@@ -800,14 +941,14 @@ int getdt_device(vacgpu_grid* g, const vacgpu_method* m, double dtpar, int tmaxe
       (void)sizeof(R);
       hipLaunchKernelGGL((k_pminmax<ND, MHD>), g->grid, g->block, 0, g->stream, g->geo, w,
                          g->cfg.eqpar[0], g->sc, g->part, g->pstride);
-      hipLaunchKernelGGL(k_fin_smallp, dim3(1), dim3(256), 0, g->stream, g->sc, g->part,
+      hipLaunchKernelGGL(k_fin_smallp, dim3(1), dim3(1024), 0, g->stream, g->sc, g->part,
                          g->pstride, nb, m->smallpcoeff, (int)mhd);
       if (dtpar <= 0.0)
         for (int d = 0; d < ND; ++d)
           hipLaunchKernelGGL((k_cmaxdx<ND, MHD>), g->grid, g->block, 0, g->stream, g->geo, w,
                              g->cfg.eqpar[0], g->sc, g->part, g->pstride, d);
     });
-    hipLaunchKernelGGL(k_fin_getdt, dim3(1), dim3(256), 0, g->stream, g->sc, g->part, g->pstride,
+    hipLaunchKernelGGL(k_fin_getdt, dim3(1), dim3(1024), 0, g->stream, g->sc, g->part, g->pstride,
                        nb, g->cfg.ndim, m->courantpar, dtpar, tmaxexact);
   }
   return launch_check(g, "getdt");
@@ -830,8 +971,9 @@ int sweep_device(vacgpu_grid* g, const vacgpu_method* m, int d, int firstsweep, 
     {
       Timed t(g, KF_PRED);
       DISPATCH(g, {
-        hipLaunchKernelGGL((k_tvdlf_pred<ND, MHD, R>), g->grid, g->block, 0, g->stream, g->geo, s,
-                           win, g->w1, g->sc, d);
+        if (d == 0) hipLaunchKernelGGL((k_tvdlf_pred<ND, MHD, R, 0>), g->grid, g->block, 0, g->stream, g->geo, s, win, g->w1, g->sc);
+        else if (d == 1) hipLaunchKernelGGL((k_tvdlf_pred<ND, MHD, R, 1>), g->grid, g->block, 0, g->stream, g->geo, s, win, g->w1, g->sc);
+        else hipLaunchKernelGGL((k_tvdlf_pred<ND, MHD, R, (ND == 3 ? 2 : 1)>), g->grid, g->block, 0, g->stream, g->geo, s, win, g->w1, g->sc);
       });
       if ((rc = launch_check(g, "k_tvdlf_pred"))) return rc;
     }
@@ -839,14 +981,15 @@ int sweep_device(vacgpu_grid* g, const vacgpu_method* m, int d, int firstsweep, 
     {
       Timed t(g, KF_CORR);
       DISPATCH(g, {
-        hipLaunchKernelGGL((k_tvdlf_corr<ND, MHD, R>), g->grid, g->block, 0, g->stream, g->geo, s,
-                           win, g->w1, wout, g->sc, g->part, g->pstride, d);
+        if (d == 0) hipLaunchKernelGGL((k_tvdlf_corr<ND, MHD, R, 0>), g->grid, g->block, 0, g->stream, g->geo, s, win, g->w1, wout, g->sc, g->part, g->pstride);
+        else if (d == 1) hipLaunchKernelGGL((k_tvdlf_corr<ND, MHD, R, 1>), g->grid, g->block, 0, g->stream, g->geo, s, win, g->w1, wout, g->sc, g->part, g->pstride);
+        else hipLaunchKernelGGL((k_tvdlf_corr<ND, MHD, R, (ND == 3 ? 2 : 1)>), g->grid, g->block, 0, g->stream, g->geo, s, win, g->w1, wout, g->sc, g->part, g->pstride);
       });
       if ((rc = launch_check(g, "k_tvdlf_corr"))) return rc;
     }
     if (courant) {
       Timed t(g, KF_RED);
-      hipLaunchKernelGGL(k_fin_courant_tvdlf, dim3(1), dim3(256), 0, g->stream, g->sc, g->part,
+      hipLaunchKernelGGL(k_fin_courant_tvdlf, dim3(1), dim3(1024), 0, g->stream, g->sc, g->part,
                          g->pstride, nb, d, m->courantpar);
       if ((rc = launch_check(g, "k_fin_courant_tvdlf"))) return rc;
     }
@@ -864,22 +1007,32 @@ int sweep_device(vacgpu_grid* g, const vacgpu_method* m, int d, int firstsweep, 
       if ((rc = bound(g, g->w1))) return rc;
       wct = g->w1;
     }
+    long long nbt;
     {
       Timed t(g, KF_TVD);
       const bool f32 = g->cfg.precision == VACGPU_FP32BULK;
-      if (g->cfg.ndim == 2) {
-        if (f32) hipLaunchKernelGGL((k_tvd_sweep<2, float>), g->grid, g->block, 0, g->stream, g->geo, s, wct, win, wout, g->sc, g->part, g->pstride, d);
-        else hipLaunchKernelGGL((k_tvd_sweep<2, double>), g->grid, g->block, 0, g->stream, g->geo, s, wct, win, wout, g->sc, g->part, g->pstride, d);
+      if (d == 0) {
+        const dim3 gr = g->grid_tx;
+        nbt = (long long)gr.x * gr.y * gr.z;
+#define TVDX(NDV, RT) hipLaunchKernelGGL((k_tvd_x<NDV, RT>), gr, g->block, 0, g->stream, g->geo, s, wct, win, wout, g->sc, g->part, g->pstride)
+        if (g->cfg.ndim == 2) { if (f32) TVDX(2, float); else TVDX(2, double); }
+        else { if (f32) TVDX(3, float); else TVDX(3, double); }
+#undef TVDX
       } else {
-        if (f32) hipLaunchKernelGGL((k_tvd_sweep<3, float>), g->grid, g->block, 0, g->stream, g->geo, s, wct, win, wout, g->sc, g->part, g->pstride, d);
-        else hipLaunchKernelGGL((k_tvd_sweep<3, double>), g->grid, g->block, 0, g->stream, g->geo, s, wct, win, wout, g->sc, g->part, g->pstride, d);
+        const dim3 gr = g->grid_tm[d];
+        nbt = (long long)gr.x * gr.y * gr.z;
+#define TVDM(NDV, RT, DV) hipLaunchKernelGGL((k_tvd_m<NDV, RT, DV>), gr, g->block, 0, g->stream, g->geo, s, wct, win, wout, g->sc, g->part, g->pstride, g->seglen)
+        if (g->cfg.ndim == 2) { if (f32) TVDM(2, float, 1); else TVDM(2, double, 1); }
+        else if (d == 1) { if (f32) TVDM(3, float, 1); else TVDM(3, double, 1); }
+        else { if (f32) TVDM(3, float, 2); else TVDM(3, double, 2); }
+#undef TVDM
       }
       if ((rc = launch_check(g, "k_tvd_sweep"))) return rc;
     }
     if (courant) {
       Timed t(g, KF_RED);
-      hipLaunchKernelGGL(k_fin_courant_tvd, dim3(1), dim3(256), 0, g->stream, g->sc, g->part,
-                         g->pstride, nb, d, m->courantpar);
+      hipLaunchKernelGGL(k_fin_courant_tvd, dim3(1), dim3(1024), 0, g->stream, g->sc, g->part,
+                         g->pstride, nbt, d, m->courantpar);
       if ((rc = launch_check(g, "k_fin_courant_tvd"))) return rc;
     }
     if (lastsweep && src) {
@@ -990,7 +1143,23 @@ int vacgpu_create(const vacgpu_config* cfg, const double* dx0, const double* dx1
   g->nbound = tot;
   g->block = dim3(BX, BY, 1);
   g->grid = dim3((geo.n[0] + BX - 1) / BX, (geo.n[1] + BY - 1) / BY, nd == 3 ? geo.n[2] : 1);
-  g->pstride = (long long)g->grid.x * g->grid.y * g->grid.z;
+  g->grid_tx = dim3((geo.n[0] + TX_STRIDE - 1) / TX_STRIDE, g->grid.y, g->grid.z);
+  {
+    // segment length of the marching sweeps: enough threads to fill the chip
+    long long cells = (long long)geo.n[0] * geo.n[1] * (nd == 3 ? geo.n[2] : 1);
+    long long sl = cells / 262144;
+    g->seglen = (int)(sl < 4 ? 4 : (sl > 32 ? 32 : sl));
+  }
+  for (int d = 1; d < nd; ++d) {
+    const int nseg = (geo.n[d] + g->seglen - 1) / g->seglen;
+    const int other = (d == 1) ? 2 : 1;
+    g->grid_tm[d] = dim3(g->grid.x, (nseg + BY - 1) / BY, nd == 3 ? geo.n[other] : 1);
+  }
+  auto count = [](dim3 v) { return (long long)v.x * v.y * v.z; };
+  g->pstride = count(g->grid);
+  if (count(g->grid_tx) > g->pstride) g->pstride = count(g->grid_tx);
+  for (int d = 1; d < nd; ++d)
+    if (count(g->grid_tm[d]) > g->pstride) g->pstride = count(g->grid_tm[d]);
   const size_t bytes = sizeof(double) * (size_t)geo.ps * g->nw;
   hipError_t e = hipSuccess;
   for (int i = 0; i < 3 && e == hipSuccess; ++i) e = hipMalloc(&g->buf[i], bytes);
