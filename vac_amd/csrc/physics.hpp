@@ -89,39 +89,49 @@ __device__ __forceinline__ double ptotal(const double* u, double g) {
 // Physical flux plus transport flux of every variable along d
 // (MHD getflux src/vacphys.mhd0.t:149-205, HD src/vacphys.t.hd:49-75;
 // transport f + v*w with v = m_d/rho as in src/vactvdlf.t:67-70).
+// Results are stored in double.  The flux of the normal momentum m_d, which
+// carries the total (MHD) or thermal (HD) pressure, is evaluated in double
+// for every R; the other fluxes in R.  For R = double every expression is
+// VAC's.
 template <int ND, bool MHD, class R>
-__device__ __forceinline__ void fluxes(const double* u, int d, double g, double smallp, R* f) {
+__device__ __forceinline__ void fluxes(const double* u, int d, double g, double smallp, double* f) {
   using L = Layout<ND, MHD>;
   const R rho = R(u[L::rho]);
   const R v = R(u[L::m(d)]) / rho;
+  const double vD = u[L::m(d)] / u[L::rho];
   if constexpr (MHD) {
-    const R pt = R(ptotal<ND>(u, g));
+    const double ptD = ptotal<ND>(u, g);
+    const R pt = R(ptD);
     const R bd = R(u[L::b(d)]);
-    f[L::rho] = R(0) + v * rho;
+    f[L::rho] = double(R(0) + v * rho);
 #pragma unroll
     for (int c = 0; c < ND; ++c) {
-      const R bc = R(u[L::b(c)]);
-      R t = (c == d) ? pt - bc * bd : -(bc * bd);
-      f[L::m(c)] = t + v * R(u[L::m(c)]);
+      if (c == d) {
+        f[L::m(c)] = ptD - u[L::b(c)] * u[L::b(d)] + vD * u[L::m(c)];
+      } else {
+        const R bc = R(u[L::b(c)]);
+        f[L::m(c)] = double(-(bc * bd) + v * R(u[L::m(c)]));
+      }
     }
     R bm = R(u[L::b(0)]) * R(u[L::m(0)]);
 #pragma unroll
     for (int c = 1; c < ND; ++c) bm = bm + R(u[L::b(c)]) * R(u[L::m(c)]);
-    f[L::e] = (R(u[L::m(d)]) * pt - bd * bm) / rho + v * R(u[L::e]);
+    f[L::e] = double((R(u[L::m(d)]) * pt - bd * bm) / rho + v * R(u[L::e]));
 #pragma unroll
     for (int c = 0; c < ND; ++c) {
-      if (c == d) f[L::b(c)] = R(0);
-      else f[L::b(c)] = -(R(u[L::m(c)]) / rho * bd) + v * R(u[L::b(c)]);
+      if (c == d) f[L::b(c)] = 0.0;
+      else f[L::b(c)] = double(-(R(u[L::m(c)]) / rho * bd) + v * R(u[L::b(c)]));
     }
   } else {
-    const R p = R(pthermal<ND, false>(u, g, smallp));
-    f[L::rho] = R(0) + v * rho;
+    const double pD = pthermal<ND, false>(u, g, smallp);
+    const R p = R(pD);
+    f[L::rho] = double(R(0) + v * rho);
 #pragma unroll
     for (int c = 0; c < ND; ++c) {
-      R t = (c == d) ? p : R(0);
-      f[L::m(c)] = t + v * R(u[L::m(c)]);
+      if (c == d) f[L::m(c)] = pD + vD * u[L::m(c)];
+      else f[L::m(c)] = double(R(0) + v * R(u[L::m(c)]));
     }
-    f[L::e] = R(u[L::m(d)]) / rho * p + v * R(u[L::e]);
+    f[L::e] = double(R(u[L::m(d)]) / rho * p + v * R(u[L::e]));
   }
 }
 

@@ -16,6 +16,7 @@
 // partial per block and a one-block kernel folds them (no atomics).
 #include "../../include/vacgpu.h"
 #include "physics.hpp"
+#include "sweep.hpp"
 
 #include <hip/hip_runtime.h>
 
@@ -27,732 +28,6 @@
 
 using namespace vacgpu;
 
-namespace {
-
-constexpr int BX = 64, BY = 4;      // cell kernels: 64 lanes along x, 4 rows
-constexpr int NSLOT = 8;            // partial-reduction slots
-enum { SLOT_C0 = 0, SLOT_C1 = 1, SLOT_PMIN = 2, SLOT_PMAX = 3, SLOT_DX = 4 };
-enum { KF_TVD = 0, KF_PRED = 1, KF_CORR = 2, KF_RYU = 3, KF_BOUND = 4, KF_RED = 5, KF_N = 6 };
-
-struct Geo {
-  int nd;
-  int n[3], ng[3];
-  long long s[3];  // element strides per dimension
-  long long ps;    // plane stride
-  long long off;   // element offset of ixG cell (0,0,0)
-  const double* dx[3];
-};
-
-struct Sel {
-  int lim[8];
-  int divbfix, divbwave, sourceunsplit;
-  double courantpar, gamma;
-};
-
-struct BC {
-  unsigned char t[6][8];
-};
-
-// device-resident scalars
-struct Scal {
-  double dt, t, smallp;
-  double dtcourant[3];
-  double tmax;
-  int nerror[4];
-  int it, tmaxexact;
-};
-
-struct CellIx {
-  int c[3];
-  long long idx;
-  bool ok;
-};
-
-__device__ __forceinline__ CellIx cell_of(const Geo& g) {
-  CellIx r;
-  r.c[0] = 2 + (int)(blockIdx.x * BX + threadIdx.x);
-  r.c[1] = 2 + (int)(blockIdx.y * BY + threadIdx.y);
-  r.c[2] = (g.nd == 3) ? 2 + (int)blockIdx.z : 0;
-  r.ok = (r.c[0] <= g.n[0] + 1) && (r.c[1] <= g.n[1] + 1);
-  r.idx = g.off + r.c[0] + (long long)r.c[1] * g.s[1] + (long long)r.c[2] * g.s[2];
-  return r;
-}
-
-__device__ __forceinline__ long long block_linear() {
-  return blockIdx.x + (long long)gridDim.x * (blockIdx.y + (long long)gridDim.y * blockIdx.z);
-}
-
-template <int NW>
-__device__ __forceinline__ void load(const double* __restrict__ w, long long ps, long long idx,
-                                     double* u) {
-#pragma unroll
-  for (int iw = 0; iw < NW; ++iw) u[iw] = w[iw * ps + idx];
-}
-
-// block-wide max / min of one value per thread (all threads must call)
-__device__ __forceinline__ double wave_max(double v) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) {
-    double u = __shfl_xor(v, o, 64);
-    v = v < u ? u : v;
-  }
-  return v;
-}
-__device__ __forceinline__ double wave_min(double v) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) {
-    double u = __shfl_xor(v, o, 64);
-    v = u < v ? u : v;
-  }
-  return v;
-}
-
-template <bool IsMax>
-__device__ __forceinline__ double block_reduce(double v, double* sm) {
-  const int tid = threadIdx.x + blockDim.x * (threadIdx.y + blockDim.y * threadIdx.z);
-  const int nthr = blockDim.x * blockDim.y * blockDim.z;
-  v = IsMax ? wave_max(v) : wave_min(v);
-  __syncthreads();
-  if ((tid & 63) == 0) sm[tid >> 6] = v;
-  __syncthreads();
-  if (tid < 64) {
-    double u = (tid < (nthr + 63) / 64) ? sm[tid] : (IsMax ? -INFINITY : INFINITY);
-    v = IsMax ? wave_max(u) : wave_min(u);
-  }
-  return v;  // valid in thread 0
-}
-
-// divB at a cell (src/vacphys.mhd0.t:119-133, src/vacgrid.t:822): sum over
-// dims of 0.5*(B_d(i+1) - B_d(i-1))/dx_d(i), accumulated from 0.
-template <int ND>
-__device__ __forceinline__ double divb_at(const Geo& g, const double* __restrict__ w, long long idx,
-                                          const int* c) {
-  using L = Layout<ND, true>;
-  double dv = 0.0;
-#pragma unroll
-  for (int d = 0; d < ND; ++d) {
-    const double* b = w + L::b(d) * g.ps;
-    double gq = 0.5 * (b[idx + g.s[d]] - b[idx - g.s[d]]) / g.dx[d][c[d]];
-    dv = dv + gq;
-  }
-  return dv;
-}
-
-// ------------------------------------------------------------ ghost cells
-// getboundary for 'periodic' and 'cont' (src/vacgrid.t:195-413).  VAC fills
-// face regions in the order x-low, x-high, y-low, ... over the full
-// transverse ixG range; since both types copy, every ghost cell ends up
-// equal to the mesh cell obtained by mapping each coordinate on its own,
-// which is what this kernel writes (one thread per ghost cell).
-template <int NW>
-__global__ __launch_bounds__(256) void k_bound(Geo g, BC bc, double* __restrict__ w,
-                                              long long total, long long slab0, long long slab1) {
-  long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= total) return;
-  int d;
-  if (t < slab0) d = 0;
-  else if (t < slab0 + slab1) { d = 1; t -= slab0; }
-  else { d = 2; t -= slab0 + slab1; }
-  int c[3] = {0, 0, 0};
-  for (int k = 0; k < g.nd; ++k) {
-    int ext = (k < d) ? g.n[k] : (k == d ? 4 : g.ng[k]);
-    int v = (int)(t % ext);
-    t /= ext;
-    if (k < d) c[k] = 2 + v;
-    else if (k == d) c[k] = (v < 2) ? v : g.n[k] + v;
-    else c[k] = v;
-  }
-  const long long dst = g.off + c[0] + c[1] * g.s[1] + c[2] * g.s[2];
-#pragma unroll
-  for (int iw = 0; iw < NW; ++iw) {
-    long long src = g.off;
-    for (int k = 0; k < g.nd; ++k) {
-      int s = c[k];
-      if (c[k] < 2) s = bc.t[2 * k][iw] ? c[k] + g.n[k] : 2;
-      else if (c[k] > g.n[k] + 1) s = bc.t[2 * k + 1][iw] ? c[k] - g.n[k] : g.n[k] + 1;
-      src += s * g.s[k];
-    }
-    w[iw * g.ps + dst] = w[iw * g.ps + src];
-  }
-}
-
-// ------------------------------------------------------------ TVDLF
-// Hancock predictor (src/vactvdlf.t:5-89) with qdt = dt/2:
-// w1 = w - qdt/dx*(F(w+l/2) - F(w-l/2)) - (qdt/ndim)*S_Powell(w)
-template <int ND, bool MHD, class R, int D>
-__global__ __launch_bounds__(256) void k_tvdlf_pred(Geo g, Sel s, const double* __restrict__ w,
-                                                   double* __restrict__ w1,
-                                                   const Scal* __restrict__ sc) {
-  constexpr int d = D;
-  using L = Layout<ND, MHD>;
-  constexpr int NW = L::nw;
-  const CellIx cl = cell_of(g);
-  if (!cl.ok) return;
-  const double qdt = sc->dt / 2;
-  const double smallp = sc->smallp;
-  const long long sd = g.s[d];
-  double uc[NW], wl[NW], wr[NW];
-  load<NW>(w, g.ps, cl.idx, uc);
-#pragma unroll
-  for (int iw = 0; iw < NW; ++iw) {
-    const double um = w[iw * g.ps + cl.idx - sd], up = w[iw * g.ps + cl.idx + sd];
-    const R l = limiter<R>(s.lim[iw], R(up - uc[iw]), R(uc[iw] - um));
-    wr[iw] = uc[iw] - 0.5 * double(l);
-    wl[iw] = uc[iw] + 0.5 * double(l);
-  }
-  R fL[NW], fR[NW];
-  fluxes<ND, MHD, R>(wl, d, s.gamma, smallp, fL);
-  fluxes<ND, MHD, R>(wr, d, s.gamma, smallp, fR);
-  const double qdx = qdt / g.dx[d][cl.c[d]];
-  double out[NW];
-#pragma unroll
-  for (int iw = 0; iw < NW; ++iw) out[iw] = uc[iw] - qdx * double(fL[iw] - fR[iw]);
-  if constexpr (MHD) {
-    if (s.sourceunsplit && s.divbfix) {
-      const double dv = divb_at<ND>(g, w, cl.idx, cl.c);
-      powell<ND>(uc, dv, qdt * 1.0 / ND, out);
-    }
-  }
-#pragma unroll
-  for (int iw = 0; iw < NW; ++iw) w1[iw * g.ps + cl.idx] = out[iw];
-}
-
-// TVDLF corrector (tvdmusclf, src/vactvdlf.t:92-299) for the faces i-1/2 and
-// i+1/2 of cell i: MUSCL states of w1 with limited slopes of w
-// (typelimited='original'), local Lax-Friedrichs flux with cmax of the mean
-// state, Powell source of w1 with dt/ndim.  Writes the partial max of
-// 2*cmax/(dx_j+dx_j+1) (src/vactvdlf.t:190) to part[SLOT_C0].
-template <int ND, bool MHD, class R, int D>
-__global__ __launch_bounds__(256) void k_tvdlf_corr(Geo g, Sel s, const double* __restrict__ w,
-                                                   const double* __restrict__ w1,
-                                                   double* __restrict__ wout,
-                                                   const Scal* __restrict__ sc,
-                                                   double* __restrict__ part, long long pstride) {
-  constexpr int d = D;
-  using L = Layout<ND, MHD>;
-  constexpr int NW = L::nw;
-  __shared__ double sm[8];
-  const CellIx cl = cell_of(g);
-  double cmx = -INFINITY;
-  if (cl.ok) {
-    const double dt = sc->dt;
-    const double smallp = sc->smallp;
-    const long long sd = g.s[d];
-    const int ci = cl.c[d];
-    // limited slopes l(i-1), l(i), l(i+1) of w
-    double lm[NW], l0[NW], lp[NW];
-#pragma unroll
-    for (int iw = 0; iw < NW; ++iw) {
-      const double* p = w + iw * g.ps + cl.idx;
-      const double a = p[-2 * sd], b = p[-sd], c = p[0], e = p[sd], f = p[2 * sd];
-      lm[iw] = double(limiter<R>(s.lim[iw], R(c - b), R(b - a)));
-      l0[iw] = double(limiter<R>(s.lim[iw], R(e - c), R(c - b)));
-      lp[iw] = double(limiter<R>(s.lim[iw], R(f - e), R(e - c)));
-    }
-    R F[2][NW];
-#pragma unroll
-    for (int side = 0; side < 2; ++side) {
-      // face J+1/2 with J = i-1 (side 0) or J = i (side 1)
-      const long long jl = cl.idx + (side - 1) * sd;
-      double wlf[NW], wrf[NW];
-#pragma unroll
-      for (int iw = 0; iw < NW; ++iw) {
-        const double* p = w1 + iw * g.ps + jl;
-        wrf[iw] = p[sd] - 0.5 * (side ? lp[iw] : l0[iw]);
-        wlf[iw] = p[0] + 0.5 * (side ? l0[iw] : lm[iw]);
-      }
-      double mean[NW];
-#pragma unroll
-      for (int iw = 0; iw < NW; ++iw) mean[iw] = 0.5 * (wlf[iw] + wrf[iw]);
-      const R cm = cmax<ND, MHD, R>(mean, d, s.gamma, smallp);
-      const int J = ci + side - 1;
-      const double cv = double(cm) * 2.0 / (g.dx[d][J] + g.dx[d][J + 1]);
-      cmx = cmx < cv ? cv : cmx;
-#pragma unroll
-      for (int iw = 0; iw < NW; ++iw) wlf[iw] = 2.0 * mean[iw] - wrf[iw];
-      R fL[NW], fR[NW];
-      fluxes<ND, MHD, R>(wlf, d, s.gamma, smallp, fL);
-      fluxes<ND, MHD, R>(wrf, d, s.gamma, smallp, fR);
-#pragma unroll
-      for (int iw = 0; iw < NW; ++iw) {
-        const R fm = R(0.5) * (fL[iw] + fR[iw]);
-        const R diss = -(cm * R(0.5) * R(wrf[iw] - wlf[iw]));
-        F[side][iw] = fm + diss;
-      }
-    }
-    double out[NW];
-    const double qdx = dt / g.dx[d][ci];
-#pragma unroll
-    for (int iw = 0; iw < NW; ++iw)
-      out[iw] = w[iw * g.ps + cl.idx] - qdx * double(F[1][iw] - F[0][iw]);
-    if constexpr (MHD) {
-      if (s.sourceunsplit && s.divbfix) {
-        double u1[NW];
-        load<NW>(w1, g.ps, cl.idx, u1);
-        const double dv = divb_at<ND>(g, w1, cl.idx, cl.c);
-        powell<ND>(u1, dv, dt * 1.0 / ND, out);
-      }
-    }
-#pragma unroll
-    for (int iw = 0; iw < NW; ++iw) wout[iw * g.ps + cl.idx] = out[iw];
-  }
-  const double bm = block_reduce<true>(cmx, sm);
-  if (threadIdx.x == 0 && threadIdx.y == 0) part[SLOT_C0 * pstride + block_linear()] = bm;
-}
-
-// ------------------------------------------------------------ TVD (Roe)
-// First-sweep half source of the TVD pre-processing (src/vaccd.t:44-68):
-// w1 = w - (dt/2)*S_Powell(w) on the mesh.
-template <int ND>
-__global__ __launch_bounds__(256) void k_tvd_halfsrc(Geo g, const double* __restrict__ w,
-                                                    double* __restrict__ w1,
-                                                    const Scal* __restrict__ sc) {
-  using L = Layout<ND, true>;
-  constexpr int NW = L::nw;
-  const CellIx cl = cell_of(g);
-  if (!cl.ok) return;
-  double u[NW], out[NW];
-  load<NW>(w, g.ps, cl.idx, u);
-#pragma unroll
-  for (int iw = 0; iw < NW; ++iw) out[iw] = u[iw];
-  powell<ND>(u, divb_at<ND>(g, w, cl.idx, cl.c), sc->dt / 2, out);
-#pragma unroll
-  for (int iw = 0; iw < NW; ++iw) w1[iw * g.ps + cl.idx] = out[iw];
-}
-
-// ---- TVD sweep.  For cell i: the centred flux difference of wCT (centdiff,
-// src/vaccd.t:78-136) followed by the Roe-type dissipation of tvdlimit2
-// (src/vactvd.t:86-206, getphi 'roe' :244-302) through the faces i-1/2 and
-// i+1/2, whose limiters also need the jumps at faces i-3/2 and i+3/2.  Face
-// quantities are computed once per face: along x each lane of a wavefront
-// owns one face and neighbours are read with cross-lane shuffles; along y
-// and z each thread marches through a segment of cells, keeping a sliding
-// window of faces in registers.  Both kernels write the partial maxima of
-// |a dt 2/(dx_J+dx_J+1)| of the two fast waves over the faces they compute
-// (covering ixIC, src/vactvd.t:42,145-157) to part[SLOT_C0/C1].
-
-// Everything one face contributes to the update of its two cells.
-template <int ND, class R>
-struct TvdFace {
-  R jump[Layout<ND, true>::nw];
-  R adt[Layout<ND, true>::nw];
-  RoeRt<ND, R> rt;
-};
-
-template <int ND, class R>
-__device__ __forceinline__ void tvd_face(const double* uL, const double* uR, int d, const Sel& s,
-                                         double smallp, double dt, double dxsum, TvdFace<ND, R>& f,
-                                         double& c0, double& c1) {
-  constexpr int NW = Layout<ND, true>::nw;
-  Roe<ND, R> r;
-  roe_setup<ND, R>(uL, uR, d, s.gamma, smallp, r);
-  const R sdx = R(dxsum);
-#pragma unroll
-  for (int il = 0; il < NW; ++il) {
-    R a, jp;
-    roe_wave<ND, R>(r, uL, uR, il, d, s.divbwave, a, jp);
-    const R ad = a * R(dt) * R(2) / sdx;
-    f.jump[il] = jp;
-    f.adt[il] = ad;
-  }
-  c0 = fabs(double(f.adt[0]));
-  c1 = fabs(double(f.adt[1]));
-  f.rt = roe_rt<ND, R>(r);
-}
-
-// out[iw] += sum over il (in order) of (R.phi(J) q(J) - R.phi(J-1) q(J-1)) / dV(i)
-template <int ND, class R>
-__device__ __forceinline__ void tvd_dissipate(const RoeRt<ND, R>& rm, const R* phim,
-                                              const RoeRt<ND, R>& r0, const R* phi0, int d,
-                                              const Sel& s, R dvm, R dv0, double dvi, double* out) {
-  constexpr int NW = Layout<ND, true>::nw;
-#pragma unroll
-  for (int il = 0; il < NW; ++il) {
-#pragma unroll
-    for (int iw = 0; iw < NW; ++iw) {
-      const R a0 = roe_rtimes<ND, R>(rm, phim[il], iw, il, d, s.divbwave) * R(0.25) * dvm;
-      const R a1 = roe_rtimes<ND, R>(r0, phi0[il], iw, il, d, s.divbwave) * R(0.25) * dv0;
-      out[iw] = out[iw] + double(a1 - a0) / dvi;
-    }
-  }
-}
-
-template <class T>
-__device__ __forceinline__ T shfl_up1(T v) { return __shfl_up(v, 1, 64); }
-template <class T>
-__device__ __forceinline__ T shfl_down1(T v) { return __shfl_down(v, 1, 64); }
-
-template <class S, class R>
-__device__ __forceinline__ S shfl_up_struct(const S& v) {
-  static_assert(sizeof(S) % sizeof(R) == 0, "struct of R");
-  S o;
-  const R* a = reinterpret_cast<const R*>(&v);
-  R* b = reinterpret_cast<R*>(&o);
-#pragma unroll
-  for (int k = 0; k < (int)(sizeof(S) / sizeof(R)); ++k) b[k] = shfl_up1(a[k]);
-  return o;
-}
-
-constexpr int TX_STRIDE = 61;  // cells updated per 64-lane wavefront along x
-
-// x sweep: lane L of a wavefront owns face J = x0 - 2 + L (between cells J and
-// J+1) and updates cell J for L in [2, 62].
-template <int ND, class R>
-__global__ __launch_bounds__(256) void k_tvd_x(Geo g, Sel s, const double* __restrict__ wct,
-                                              const double* __restrict__ win,
-                                              double* __restrict__ wout,
-                                              const Scal* __restrict__ sc,
-                                              double* __restrict__ part, long long pstride) {
-  using L = Layout<ND, true>;
-  constexpr int NW = L::nw;
-  __shared__ double sm[8];
-  const double dt = sc->dt, smallp = sc->smallp;
-  const int lane = threadIdx.x;
-  const int x0 = 2 + (int)blockIdx.x * TX_STRIDE;
-  const int J = x0 - 2 + lane;
-  const int y = 2 + (int)(blockIdx.y * BY + threadIdx.y);
-  const int z = (g.nd == 3) ? 2 + (int)blockIdx.z : 0;
-  const bool rowok = y <= g.n[1] + 1;
-  const int yc = rowok ? y : g.n[1] + 1;
-  const int Jc = J < g.n[0] + 2 ? J : g.n[0] + 2;
-  const long long base = g.off + (long long)yc * g.s[1] + (long long)z * g.s[2];
-  double uL[NW], uR[NW];
-  load<NW>(wct, g.ps, base + Jc, uL);
-  load<NW>(wct, g.ps, base + Jc + 1, uR);
-  TvdFace<ND, R> f;
-  double c0, c1;
-  tvd_face<ND, R>(uL, uR, 0, s, smallp, dt, g.dx[0][Jc] + g.dx[0][Jc + 1], f, c0, c1);
-  const bool faceok = rowok && J <= g.n[0] + 2;
-  if (!faceok) { c0 = -INFINITY; c1 = -INFINITY; }
-  // centdiff: flux at cell J, face average fC(J) and fC(J-1)
-  R F[NW], fC[NW], fCm[NW];
-  fluxes<ND, true, R>(uL, 0, s.gamma, smallp, F);
-#pragma unroll
-  for (int iw = 0; iw < NW; ++iw) fC[iw] = R(0.5) * (F[iw] + shfl_down1(F[iw]));
-#pragma unroll
-  for (int iw = 0; iw < NW; ++iw) fCm[iw] = shfl_up1(fC[iw]);
-  // phi at face J from the jumps at J-1, J, J+1
-  R phi[NW];
-#pragma unroll
-  for (int il = 0; il < NW; ++il) {
-    const R jm = shfl_up1(f.jump[il]), jp = shfl_down1(f.jump[il]);
-    phi[il] = tvd_phi<R>(s.lim[il], jm, f.jump[il], jp, f.adt[il]);
-  }
-  R phim[NW];
-#pragma unroll
-  for (int il = 0; il < NW; ++il) phim[il] = shfl_up1(phi[il]);
-  const RoeRt<ND, R> rtm = shfl_up_struct<RoeRt<ND, R>, R>(f.rt);
-  const int i = J;
-  if (rowok && lane >= 2 && lane <= 62 && i <= g.n[0] + 1) {
-    const long long idx = base + i;
-    double out[NW];
-    const double qdx = dt / g.dx[0][i];
-#pragma unroll
-    for (int iw = 0; iw < NW; ++iw) out[iw] = win[iw * g.ps + idx] - qdx * double(fC[iw] - fCm[iw]);
-    // dvolume = (dx1*dx2)*dx3 evaluated left to right
-    auto dvol = [&](int xi) {
-      double v = g.dx[0][xi];
-      v = v * g.dx[1][yc];
-      if (ND == 3) v = v * g.dx[2][z];
-      return v;
-    };
-    const double dm = dvol(i - 1), d0 = dvol(i), dp = dvol(i + 1);
-    tvd_dissipate<ND, R>(rtm, phim, f.rt, phi, 0, s, R(d0 + dm), R(dp + d0), d0, out);
-#pragma unroll
-    for (int iw = 0; iw < NW; ++iw) wout[iw * g.ps + idx] = out[iw];
-  }
-  const double b0 = block_reduce<true>(c0, sm);
-  if (threadIdx.x == 0 && threadIdx.y == 0) part[SLOT_C0 * pstride + block_linear()] = b0;
-  const double b1 = block_reduce<true>(c1, sm);
-  if (threadIdx.x == 0 && threadIdx.y == 0) part[SLOT_C1 * pstride + block_linear()] = b1;
-}
-
-// y / z sweep: thread (x, segment) marches through cells s0..s1 along d.
-template <int ND, class R, int D>
-__global__ __launch_bounds__(256) void k_tvd_m(Geo g, Sel s, const double* __restrict__ wct,
-                                              const double* __restrict__ win,
-                                              double* __restrict__ wout,
-                                              const Scal* __restrict__ sc,
-                                              double* __restrict__ part, long long pstride,
-                                              int seglen) {
-  constexpr int d = D;
-  using L = Layout<ND, true>;
-  constexpr int NW = L::nw;
-  __shared__ double sm[8];
-  const double dt = sc->dt, smallp = sc->smallp;
-  const int x = 2 + (int)(blockIdx.x * BX + threadIdx.x);
-  const int seg = (int)(blockIdx.y * BY + threadIdx.y);
-  const int s0 = 2 + seg * seglen;
-  // the transverse coordinate that is neither x nor d (3D only)
-  const int t = (g.nd == 3) ? 2 + (int)blockIdx.z : 0;
-  const int nd_ = g.n[d];
-  const bool active = x <= g.n[0] + 1 && s0 <= nd_ + 1;
-  double c0 = -INFINITY, c1 = -INFINITY;
-  if (active) {
-    const int s1 = (s0 + seglen - 1 < nd_ + 1) ? s0 + seglen - 1 : nd_ + 1;
-    const int other = (d == 1) ? 2 : 1;  // dimension of t in 3D
-    const long long sd = g.s[d];
-    const long long base = g.off + x + (ND == 3 ? (long long)t * g.s[other] : 0);
-    const double* dxd = g.dx[d];
-    // volume of cell c along d: (dx1*dx2)*dx3 in dimension order
-    const double dxt = (ND == 3) ? g.dx[other][t] : 1.0;
-    auto dvol = [&](int c) {
-      double v = g.dx[0][x];
-      if (ND == 2) return v * dxd[c];
-      // d == 1: (dx0*dx1[c])*dx2[t];  d == 2: (dx0*dx1[t])*dx2[c]
-      if (d == 1) return (v * dxd[c]) * dxt;
-      return (v * dxt) * dxd[c];
-    };
-    double uL[NW], uR[NW];
-    TvdFace<ND, R> fm, f0, fp;  // faces i-1, i, i+1
-    R phim[NW], phi0[NW];
-    R Fm[NW], F0[NW], fCm[NW];
-    double a, b;
-    // prologue: faces s0-2, s0-1, s0; centred fluxes at cells s0-1, s0
-    load<NW>(wct, g.ps, base + (long long)(s0 - 2) * sd, uL);
-    load<NW>(wct, g.ps, base + (long long)(s0 - 1) * sd, uR);
-    tvd_face<ND, R>(uL, uR, d, s, smallp, dt, dxd[s0 - 2] + dxd[s0 - 1], fm, a, b);
-    c0 = c0 < a ? a : c0; c1 = c1 < b ? b : c1;
-    fluxes<ND, true, R>(uR, d, s.gamma, smallp, Fm);
-#pragma unroll
-    for (int iw = 0; iw < NW; ++iw) uL[iw] = uR[iw];
-    load<NW>(wct, g.ps, base + (long long)s0 * sd, uR);
-    tvd_face<ND, R>(uL, uR, d, s, smallp, dt, dxd[s0 - 1] + dxd[s0], f0, a, b);
-    c0 = c0 < a ? a : c0; c1 = c1 < b ? b : c1;
-    fluxes<ND, true, R>(uR, d, s.gamma, smallp, F0);
-#pragma unroll
-    for (int iw = 0; iw < NW; ++iw) fCm[iw] = R(0.5) * (Fm[iw] + F0[iw]);  // face s0-1
-#pragma unroll
-    for (int iw = 0; iw < NW; ++iw) uL[iw] = uR[iw];
-    load<NW>(wct, g.ps, base + (long long)(s0 + 1) * sd, uR);
-    tvd_face<ND, R>(uL, uR, d, s, smallp, dt, dxd[s0] + dxd[s0 + 1], fp, a, b);
-    c0 = c0 < a ? a : c0; c1 = c1 < b ? b : c1;
-    // phi at face s0-1 from faces s0-2, s0-1, s0
-#pragma unroll
-    for (int il = 0; il < NW; ++il) phim[il] = tvd_phi<R>(s.lim[il], fm.jump[il], f0.jump[il], fp.jump[il], f0.adt[il]);
-    // window now: fm <- face s0-1, f0 <- face s0 (fp holds face s0)
-    fm = f0;
-    f0 = fp;
-    for (int i = s0;; ++i) {
-      // f0 = face i, fm = face i-1 (with phim), uL = w(i), uR = w(i+1)
-      fluxes<ND, true, R>(uR, d, s.gamma, smallp, Fm);  // F(i+1), reuse Fm as scratch
-      R fC[NW];
-#pragma unroll
-      for (int iw = 0; iw < NW; ++iw) fC[iw] = R(0.5) * (F0[iw] + Fm[iw]);  // face i
-#pragma unroll
-      for (int iw = 0; iw < NW; ++iw) F0[iw] = Fm[iw];
-      // face i+1 from w(i+1), w(i+2)
-#pragma unroll
-      for (int iw = 0; iw < NW; ++iw) uL[iw] = uR[iw];
-      load<NW>(wct, g.ps, base + (long long)(i + 2) * sd, uR);
-      tvd_face<ND, R>(uL, uR, d, s, smallp, dt, dxd[i + 1] + dxd[i + 2], fp, a, b);
-      c0 = c0 < a ? a : c0; c1 = c1 < b ? b : c1;
-#pragma unroll
-      for (int il = 0; il < NW; ++il) phi0[il] = tvd_phi<R>(s.lim[il], fm.jump[il], f0.jump[il], fp.jump[il], f0.adt[il]);
-      // update cell i
-      const long long idx = base + (long long)i * sd;
-      double out[NW];
-      const double qdx = dt / dxd[i];
-#pragma unroll
-      for (int iw = 0; iw < NW; ++iw) out[iw] = win[iw * g.ps + idx] - qdx * double(fC[iw] - fCm[iw]);
-      const double dm = dvol(i - 1), d0 = dvol(i), dp = dvol(i + 1);
-      tvd_dissipate<ND, R>(fm.rt, phim, f0.rt, phi0, d, s, R(d0 + dm), R(dp + d0), d0, out);
-#pragma unroll
-      for (int iw = 0; iw < NW; ++iw) wout[iw * g.ps + idx] = out[iw];
-      if (i == s1) break;
-      // shift the window
-#pragma unroll
-      for (int iw = 0; iw < NW; ++iw) fCm[iw] = fC[iw];
-#pragma unroll
-      for (int il = 0; il < NW; ++il) phim[il] = phi0[il];
-      fm = f0;
-      f0 = fp;
-    }
-  }
-  const double b0 = block_reduce<true>(c0, sm);
-  if (threadIdx.x == 0 && threadIdx.y == 0) part[SLOT_C0 * pstride + block_linear()] = b0;
-  const double b1 = block_reduce<true>(c1, sm);
-  if (threadIdx.x == 0 && threadIdx.y == 0) part[SLOT_C1 * pstride + block_linear()] = b1;
-}
-
-// Ryu source step on the last sweep (src/vactvd.t:60-80):
-// wt = (wold + wnew - dt*S(wold))/2 on the mesh (ghosts refreshed after) ...
-template <int ND>
-__global__ __launch_bounds__(256) void k_ryu_a(Geo g, const double* __restrict__ wold,
-                                              const double* __restrict__ wnew,
-                                              double* __restrict__ wt,
-                                              const Scal* __restrict__ sc) {
-  using L = Layout<ND, true>;
-  constexpr int NW = L::nw;
-  const CellIx cl = cell_of(g);
-  if (!cl.ok) return;
-  double uo[NW], t[NW];
-  load<NW>(wold, g.ps, cl.idx, uo);
-  load<NW>(wnew, g.ps, cl.idx, t);
-  powell<ND>(uo, divb_at<ND>(g, wold, cl.idx, cl.c), sc->dt, t);
-#pragma unroll
-  for (int iw = 0; iw < NW; ++iw) wt[iw * g.ps + cl.idx] = 0.5 * (uo[iw] + t[iw]);
-}
-
-// ... then wnew = wnew - dt*S(wt).
-template <int ND>
-__global__ __launch_bounds__(256) void k_ryu_b(Geo g, const double* __restrict__ wt,
-                                              double* __restrict__ wnew,
-                                              const Scal* __restrict__ sc) {
-  using L = Layout<ND, true>;
-  constexpr int NW = L::nw;
-  const CellIx cl = cell_of(g);
-  if (!cl.ok) return;
-  double ut[NW], o[NW];
-  load<NW>(wt, g.ps, cl.idx, ut);
-  load<NW>(wnew, g.ps, cl.idx, o);
-  powell<ND>(ut, divb_at<ND>(g, wt, cl.idx, cl.c), sc->dt, o);
-#pragma unroll
-  for (int iw = 0; iw < NW; ++iw) wnew[iw * g.ps + cl.idx] = o[iw];
-}
-
-// ------------------------------------------------------------ time step
-// Partial min / max of the unclipped pressure over the mesh, for the
-// pressure floor set on the first getpthermal call (src/vacphys0.t.mhd:202-214).
-template <int ND, bool MHD>
-__global__ __launch_bounds__(256) void k_pminmax(Geo g, const double* __restrict__ w, double gamma,
-                                                const Scal* __restrict__ sc,
-                                                double* __restrict__ part, long long pstride) {
-  using L = Layout<ND, MHD>;
-  __shared__ double sm[8];
-  if (sc->smallp >= 0.0) return;  // uniform: already set
-  const CellIx cl = cell_of(g);
-  double mn = INFINITY, mx = -INFINITY;
-  if (cl.ok) {
-    double u[L::nw];
-    load<L::nw>(w, g.ps, cl.idx, u);
-    mn = mx = pthermal_raw<ND, MHD>(u, gamma);
-  }
-  const double bmn = block_reduce<false>(mn, sm);
-  if (threadIdx.x == 0 && threadIdx.y == 0) part[SLOT_PMIN * pstride + block_linear()] = bmn;
-  const double bmx = block_reduce<true>(mx, sm);
-  if (threadIdx.x == 0 && threadIdx.y == 0) part[SLOT_PMAX * pstride + block_linear()] = bmx;
-}
-
-// Partial max of cmax/dx at cell centres for dimension d (src/vac.t:697-701),
-// skipped when a lagged dtcourant(d) exists.
-template <int ND, bool MHD>
-__global__ __launch_bounds__(256) void k_cmaxdx(Geo g, const double* __restrict__ w, double gamma,
-                                               const Scal* __restrict__ sc,
-                                               double* __restrict__ part, long long pstride, int d) {
-  using L = Layout<ND, MHD>;
-  __shared__ double sm[8];
-  if (sc->dtcourant[d] < kBig) return;  // uniform
-  const double smallp = sc->smallp;
-  const CellIx cl = cell_of(g);
-  double v = -INFINITY;
-  if (cl.ok) {
-    double u[L::nw];
-    load<L::nw>(w, g.ps, cl.idx, u);
-    v = cmax<ND, MHD, double>(u, d, gamma, smallp) / g.dx[d][cl.c[d]];
-  }
-  const double b = block_reduce<true>(v, sm);
-  if (threadIdx.x == 0 && threadIdx.y == 0) part[(SLOT_DX + d) * pstride + block_linear()] = b;
-}
-
-__device__ double fold(const double* p, long long n, bool is_max, double* sm) {
-  double v = is_max ? -INFINITY : INFINITY;
-  for (long long i = threadIdx.x; i < n; i += blockDim.x) {
-    double u = p[i];
-    v = is_max ? (v < u ? u : v) : (u < v ? u : v);
-  }
-  return is_max ? block_reduce<true>(v, sm) : block_reduce<false>(v, sm);
-}
-
-__global__ void k_fin_smallp(Scal* sc, const double* part, long long pstride, long long nb,
-                             double coeff, int mhd) {
-  __shared__ double sm[16];
-  if (sc->smallp >= 0.0) return;
-  const double mn = fold(part + SLOT_PMIN * pstride, nb, false, sm);
-  __syncthreads();
-  const double mx = fold(part + SLOT_PMAX * pstride, nb, true, sm);
-  if (threadIdx.x == 0) {
-    double sp = mn * coeff;
-    if (mhd && sp < 0.0) sp = mx * coeff;
-    sc->smallp = sp;
-  }
-}
-
-// getdt_courant's fold (src/vac.t:690-719) and the main loop's tmaxexact
-// clip (src/vac.t:83); dtpar > 0 replaces the Courant step.
-__global__ void k_fin_getdt(Scal* sc, const double* part, long long pstride, long long nb, int nd,
-                            double courantpar, double dtpar, int tmaxexact) {
-  __shared__ double sm[16];
-  double dtc[3] = {kBig, kBig, kBig};
-  for (int d = 0; d < nd; ++d) dtc[d] = sc->dtcourant[d];
-  double cmd[3] = {0.0, 0.0, 0.0};
-  for (int d = 0; d < nd; ++d) {
-    if (dtc[d] >= kBig && dtpar <= 0.0) {
-      cmd[d] = fold(part + (SLOT_DX + d) * pstride, nb, true, sm);
-      __syncthreads();
-    }
-  }
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    double dt;
-    if (dtpar > 0.0) {
-      dt = dtpar;
-    } else {
-      dt = kBig;
-      double cm = 0.0;
-      for (int d = 0; d < nd; ++d) {
-        if (dtc[d] < kBig) dt = dtc[d] < dt ? dtc[d] : dt;
-        else cm = cm < cmd[d] ? cmd[d] : cm;
-      }
-      for (int d = 0; d < nd; ++d) sc->dtcourant[d] = kBig;
-      if (cm > kSmall) { double c = courantpar / cm; dt = c < dt ? c : dt; }
-    }
-    if (tmaxexact) {
-      double c = sc->tmax - sc->t + kSmall;
-      dt = c < dt ? c : dt;
-    }
-    sc->dt = dt;
-  }
-}
-
-// dtcourant update of the TVDLF corrector (src/vactvdlf.t:184-201)
-__global__ void k_fin_courant_tvdlf(Scal* sc, const double* part, long long pstride, long long nb,
-                                    int d, double courantpar) {
-  __shared__ double sm[16];
-  const double cm = fold(part + SLOT_C0 * pstride, nb, true, sm);
-  if (threadIdx.x == 0) {
-    if (sc->dt * cm > 1.0) sc->nerror[2] += 1;
-    if (cm > kSmall) {
-      double c = courantpar / cm;
-      sc->dtcourant[d] = c < sc->dtcourant[d] ? c : sc->dtcourant[d];
-    }
-  }
-}
-
-// dtcourant update of tvdlimit2 for the two extreme waves (src/vactvd.t:145-157)
-__global__ void k_fin_courant_tvd(Scal* sc, const double* part, long long pstride, long long nb,
-                                  int d, double courantpar) {
-  __shared__ double sm[16];
-  for (int slot = SLOT_C0; slot <= SLOT_C1; ++slot) {
-    const double cm = fold(part + slot * pstride, nb, true, sm);
-    if (threadIdx.x == 0) {
-      if (cm > 1.0) sc->nerror[2] += 1;
-      if (cm > kSmall) {
-        double c = sc->dt * courantpar / cm;
-        sc->dtcourant[d] = c < sc->dtcourant[d] ? c : sc->dtcourant[d];
-      }
-    }
-    __syncthreads();
-  }
-}
-
-__global__ void k_set_dt(Scal* sc, double dt) { sc->dt = dt; }
-__global__ void k_step_end(Scal* sc) {
-  sc->it += 1;
-  sc->t = sc->t + sc->dt;
-}
-
-}  // namespace
 
 // ======================================================================
 // host side
