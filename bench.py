@@ -10,8 +10,14 @@ sweep per direction, each with the Roe dissipation and the Powell / Ryu
 source terms).  Metric: Mcell-updates/s = nx*ny*steps / seconds / 1e6.
 
     python bench.py [--gpus 1] [--steps 200] [--warmup 5] [--no-cpu] [--precision fp64]
+                    [--no-batched] [--no-serving]
 
-Prints one JSON line.  The state is resident in HBM before the timed region
+Prints one JSON line.  Beside the headline it reports the batched workload
+(512 independent 2D MHD grids, sides log-uniform in [64, 512], TVDLF +
+minmod, 100 steps each, FP64 and FP32-bulk) and the served workload (open-loop
+Poisson arrivals at 50, 200 and 1000 requests/s of 2D HD or MHD solves, side
+log-uniform in [32, 1024], end time uniform in [0.05, 0.5], FP64 or FP32-bulk)
+with p50/p99 latency.  The state is resident in HBM before the timed region
 starts; the timed region is the device-side time loop (vacgpu_run) between
 two stream synchronisations.  The TVD sweep kernel's average duration is
 measured with HIP events on the library's own stream in a separate pass.
@@ -77,6 +83,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32bulk"])
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--no-serving", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
@@ -146,9 +154,42 @@ def main():
                      "b_alg_per_cell": B_ALG_TVD_SWEEP},
         "other_kernels_avg_ms": {k: round(v[0], 5) for k, v in other.items()},
     }
+    if not args.no_batched:
+        out["batched"] = batched()
+    if not args.no_serving:
+        out["serving"] = serving()
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(w0, coords)
     print(json.dumps(out))
+
+
+def batched():
+    """512 uneven 2D MHD grids, TVDLF + minmod, 100 steps, FP64 and FP32-bulk."""
+    from vac_amd import serve as sv
+
+    insts = [i.prepare() for i in sv.batched_instances(512, seed=4)]
+    res = {}
+    for prec in ("fp64", "fp32bulk"):
+        r = sv.run_batched(insts, steps=100, precision=prec)
+        res[prec] = {k: (round(v, 4) if isinstance(v, float) else v) for k, v in r.items()}
+    res["workload"] = ("512 x 2D MHD, sides log-uniform [64,512] (non-square), gamma in {5/3,1.4}, "
+                       "TVDLF+minmod, 100 steps each, one batch, hipGraph replay, autotuned tiles")
+    return res
+
+
+def serving():
+    """Open-loop Poisson arrivals at 50, 200 and 1000 requests/s."""
+    from vac_amd import serve as sv
+
+    res = {}
+    for rate, count in ((50, 100), (200, 200), (1000, 400)):
+        reqs = [r.prepare() for r in sv.make_requests(rate, count, seed=rate)]
+        r = sv.serve(reqs, budget=32 << 30)
+        res[f"{rate}_per_s"] = {k: (round(v, 3) if isinstance(v, float) else v) for k, v in r.items()}
+    res["workload"] = ("2D HD or MHD (50/50), side log-uniform [32,1024], t_end U[0.05,0.5], "
+                       "FP64 or FP32-bulk (50/50), TVDLF+minmod, admission by HBM footprint "
+                       "(32 GiB budget), continuous batching")
+    return res
 
 
 if __name__ == "__main__":

@@ -123,6 +123,50 @@ int vacgpu_supported(const vacgpu_config* cfg, const vacgpu_method* m);
 const char* vacgpu_last_error(const vacgpu_grid* g);
 void vacgpu_destroy(vacgpu_grid* g);
 
+/* ---------------------------------------------------------------- batches
+ * Many independent 2D TVDLF grids advanced together (no counterpart in the
+ * reference, which runs one grid per process: each instance follows VAC's
+ * time loop, src/vac.t:55-106, with its own dt, t, it, dtcourant, smallp and
+ * stop condition).  An instance in a batch computes exactly the same bits as
+ * the same grid advanced alone. */
+typedef struct vacgpu_batch vacgpu_batch;
+
+/* Device bytes one instance of cfg occupies (admission control). */
+long long vacgpu_batch_footprint(const vacgpu_config* cfg);
+
+/* A batch that admits instances while their footprints fit hbm_budget_bytes,
+ * with at most capacity instances at a time. */
+int vacgpu_batch_create(long long hbm_budget_bytes, int capacity, vacgpu_batch** out);
+
+/* Admit one 2D TVDLF instance with its initial ixG state w (leading dims ld,
+ * NULL = tight), step counter it, time t and stop condition (itmax, tmax,
+ * tmaxexact, src/vac.t:73-83).  VACGPU_E_NOMEM (nothing changed) when the
+ * budget or the slots are full.  *slot receives the instance id. */
+int vacgpu_batch_add(vacgpu_batch* b, const vacgpu_config* cfg, const vacgpu_method* m,
+                     const double* dx0, const double* dx1, const double* w, const int* ld,
+                     int it, double t, int itmax, double tmax, int tmaxexact, int* slot);
+int vacgpu_batch_remove(vacgpu_batch* b, int slot);
+
+/* Enqueue nsteps time steps of every instance that has not reached its stop
+ * condition (asynchronous; vacgpu_batch_sync waits and refreshes status). */
+int vacgpu_batch_step(vacgpu_batch* b, int nsteps);
+int vacgpu_batch_sync(vacgpu_batch* b);
+/* Status as of the last vacgpu_batch_sync.  Any output may be NULL. */
+int vacgpu_batch_status(vacgpu_batch* b, int slot, int* it, double* t, double* dt, int* done,
+                        int* nerror);
+int vacgpu_batch_get_state(vacgpu_batch* b, int slot, double* w, const int* ld);
+
+/* Replay each step from a captured hipGraph (re-captured after membership changes). */
+int vacgpu_batch_graph(vacgpu_batch* b, int enable);
+/* Autotuner: time candidate tile shapes (64 x rows, rows in {1,2,4,8}) per
+ * size class of the current instances and keep the fastest; rows_out[4]
+ * (may be NULL) receives the choice for cells < 128^2, < 256^2, < 512^2, larger. */
+int vacgpu_batch_tune(vacgpu_batch* b, int* rows_out);
+int vacgpu_batch_set_tiles(vacgpu_batch* b, const int* rows);
+long long vacgpu_batch_bytes_used(const vacgpu_batch* b);
+const char* vacgpu_batch_last_error(const vacgpu_batch* b);
+void vacgpu_batch_destroy(vacgpu_batch* b);
+
 #ifdef __cplusplus
 }
 #endif

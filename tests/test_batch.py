@@ -1,0 +1,172 @@
+"""Batched and continuously batched grids (vacgpu_batch) on the GPU.
+
+An instance advanced in a batch must be bitwise identical to the same grid
+advanced alone through the single-grid path -- whatever the mix of sizes,
+physics, gamma and arithmetic tier around it, whether the step runs eagerly
+or from a replayed hipGraph, and whenever it was admitted.
+"""
+import numpy as np
+import pytest
+
+from vac_amd import vacgpu
+from vac_amd.grid import cell_centres, widths_1d
+from vac_amd.initial import random_fourier
+
+pytestmark = pytest.mark.gpu
+
+METHOD = vacgpu.MethodSpec(typefull="tvdlf", limiter="minmod", courantpar=0.8)
+
+MIX = [
+    # (nx, ny, physics, gamma, precision, seed)
+    (64, 48, "mhd", 5 / 3, "fp64", 1),
+    (40, 72, "hd", 1.4, "fp64", 2),
+    (96, 96, "mhd", 1.4, "fp64", 3),
+    (128, 64, "hd", 5 / 3, "fp64", 4),
+    (33, 65, "mhd", 5 / 3, "fp32bulk", 5),
+    (200, 150, "mhd", 5 / 3, "fp64", 6),
+    (72, 40, "hd", 1.4, "fp32bulk", 7),
+    (260, 300, "hd", 1.4, "fp64", 8),
+]
+
+
+def state(nx, ny, physics, gamma, seed):
+    w, coords = random_fourier((nx, ny), seed, gamma, physics=physics)
+    return w, [widths_1d(c) for c in coords]
+
+
+def alone(w, dxs, nx, ny, physics, gamma, precision, steps=None, tmax=1e99, tmaxexact=False):
+    g = vacgpu.Grid([nx, ny], physics, gamma, dxs, precision=precision)
+    g.put_mesh(w)
+    n, t = g.run(METHOD, steps if steps is not None else 1 << 20, tmax=tmax, tmaxexact=tmaxexact,
+                 record_t=False)
+    out = g.get_mesh()
+    it, tt = g.get_time()
+    g.close()
+    return out, it, tt
+
+
+def run_batch(batch, entries, max_steps):
+    for _ in range(max_steps):
+        batch.step(1)
+    batch.sync()
+    return {s: (batch.get_mesh(s), batch.status(s)) for s in entries}
+
+
+def test_batch_matches_each_grid_alone():
+    b = vacgpu.Batch(8 << 30, 64)
+    entries = {}
+    for (nx, ny, ph, g, prec, seed) in MIX:
+        w, dxs = state(nx, ny, ph, g, seed)
+        s = b.add(w, [nx, ny], ph, g, dxs, METHOD, precision=prec, itmax=12)
+        entries[s] = (w, dxs, nx, ny, ph, g, prec)
+    print("case batch mix of", len(entries), flush=True)
+    res = run_batch(b, entries, 14)
+    for s, (w, dxs, nx, ny, ph, g, prec) in entries.items():
+        wb, st = res[s]
+        assert st["done"] and st["it"] == 12
+        wa, it, t = alone(w, dxs, nx, ny, ph, g, prec, steps=12)
+        assert it == 12 and t == st["t"], (s, t, st["t"])
+        assert np.array_equal(wa, wb), f"instance {s} ({nx}x{ny} {ph} {prec}) differs"
+    b.close()
+
+
+def test_graph_replays_match_eager():
+    out = []
+    for use_graph in (False, True):
+        b = vacgpu.Batch(4 << 30, 16)
+        slots = []
+        for (nx, ny, ph, g, prec, seed) in MIX[:5]:
+            w, dxs = state(nx, ny, ph, g, seed)
+            slots.append(b.add(w, [nx, ny], ph, g, dxs, METHOD, precision=prec, itmax=9))
+        b.graph(use_graph)
+        print(f"case graph={use_graph}", flush=True)
+        snaps = []
+        for rep in range(4):  # several replays, with synchronisation between them
+            b.step(2)
+            b.sync()
+            snaps.append([b.get_mesh(s) for s in slots])
+        out.append(snaps)
+        b.close()
+    for rep in range(4):
+        for a, c in zip(out[0][rep], out[1][rep]):
+            assert np.array_equal(a, c), f"replay {rep} differs from the eager step"
+
+
+def test_continuous_admission_matches_alone():
+    b = vacgpu.Batch(4 << 30, 16)
+    pending = list(MIX[:6])
+    live = {}
+    steps = 0
+    print("case continuous admission", flush=True)
+    while pending or live:
+        # admit one instance every third step (so admissions land on both parities)
+        if pending and steps % 3 == 0:
+            nx, ny, ph, g, prec, seed = pending.pop(0)
+            w, dxs = state(nx, ny, ph, g, seed)
+            itmax = 6 + seed
+            s = b.add(w, [nx, ny], ph, g, dxs, METHOD, precision=prec, itmax=itmax)
+            live[s] = (w, dxs, nx, ny, ph, g, prec, itmax)
+        b.step(1)
+        steps += 1
+        b.sync()
+        for s in list(live):
+            st = b.status(s)
+            if st["done"]:
+                w, dxs, nx, ny, ph, g, prec, itmax = live.pop(s)
+                wb = b.get_mesh(s)
+                b.remove(s)
+                wa, it, t = alone(w, dxs, nx, ny, ph, g, prec, steps=itmax)
+                assert st["it"] == itmax and st["t"] == t
+                assert np.array_equal(wa, wb), f"instance admitted late differs ({nx}x{ny})"
+        assert steps < 200
+    b.close()
+
+
+def test_tmax_exact_per_instance():
+    b = vacgpu.Batch(2 << 30, 8)
+    cases = []
+    for (nx, ny, ph, g, prec, seed), tmax in zip(MIX[:3], (0.05, 0.08, 0.031)):
+        w, dxs = state(nx, ny, ph, g, seed)
+        s = b.add(w, [nx, ny], ph, g, dxs, METHOD, precision=prec, tmax=tmax, tmaxexact=True)
+        cases.append((s, w, dxs, nx, ny, ph, g, prec, tmax))
+    print("case tmaxexact", flush=True)
+    for _ in range(200):
+        b.step(1)
+    b.sync()
+    for (s, w, dxs, nx, ny, ph, g, prec, tmax) in cases:
+        st = b.status(s)
+        assert st["done"]
+        assert abs(st["t"] - tmax) <= 1e-12
+        wa, it, t = alone(w, dxs, nx, ny, ph, g, prec, tmax=tmax, tmaxexact=True)
+        assert it == st["it"] and t == st["t"]
+        assert np.array_equal(wa, b.get_mesh(s))
+    b.close()
+
+
+def test_budget_admission():
+    fp = vacgpu.Batch.footprint([128, 128], "mhd")
+    b = vacgpu.Batch(int(fp * 2.5), 8)
+    w, dxs = state(128, 128, "mhd", 5 / 3, 3)
+    b.add(w, [128, 128], "mhd", 5 / 3, dxs, METHOD)
+    b.add(w, [128, 128], "mhd", 5 / 3, dxs, METHOD)
+    with pytest.raises(vacgpu.Batch.BudgetFull):
+        b.add(w, [128, 128], "mhd", 5 / 3, dxs, METHOD)
+    assert b.bytes_used() == 2 * fp
+    b.close()
+
+
+def test_autotune_keeps_results():
+    b = vacgpu.Batch(4 << 30, 16)
+    entries = {}
+    for (nx, ny, ph, g, prec, seed) in MIX:
+        w, dxs = state(nx, ny, ph, g, seed)
+        s = b.add(w, [nx, ny], ph, g, dxs, METHOD, precision=prec, itmax=5)
+        entries[s] = (w, dxs, nx, ny, ph, g, prec)
+    rows = b.tune()
+    print("case autotune rows", rows, flush=True)
+    assert all(r in (1, 2, 4, 8) for r in rows)
+    res = run_batch(b, entries, 6)
+    for s, (w, dxs, nx, ny, ph, g, prec) in entries.items():
+        wa, it, t = alone(w, dxs, nx, ny, ph, g, prec, steps=5)
+        assert np.array_equal(wa, res[s][0])
+    b.close()

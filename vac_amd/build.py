@@ -15,7 +15,7 @@ import sys
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
-SRC = os.path.join(HERE, "csrc", "vacgpu.hip")
+SRCS = [os.path.join(HERE, "csrc", "vacgpu.hip"), os.path.join(HERE, "csrc", "batch.hip")]
 OUT = os.path.join(HERE, "libvacgpu.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("VACGPU_ARCH", "gfx950")
@@ -34,7 +34,7 @@ FLAGS = [
 
 
 def build(verbose: bool = False) -> str:
-    cmd = [HIPCC, *FLAGS, SRC, "-o", OUT + ".tmp"]
+    cmd = [HIPCC, *FLAGS, *SRCS, "-o", OUT + ".tmp"]
     if verbose:
         print(" ".join(cmd))
     r = subprocess.run(cmd, capture_output=True, text=True)

@@ -79,28 +79,42 @@ def _mode_sum(rng: SplitMix64, coords: list, lattice: list, amp: float,
               nmodes: int = 8, over_k: bool = False) -> np.ndarray:
     """amp * sum over nmodes of (1/nmodes) cos(k.x + phi) [/|k|].
 
-    cos(k.x + phi) is evaluated as the real part of a product of 1D complex
-    exponentials, one per dimension, so the cost is separable.
+    cos(k.x + phi) is split into a product of 1D factors: in 2D
+    cos(kx x + ky y + phi) = cos(kx x) cos(ky y + phi) - sin(kx x) sin(ky y + phi),
+    so the whole sum is one small matrix product (ny x 2M) @ (2M x nx); in
+    3D the z and y factors are combined first.
     """
     ndim = len(coords)
-    shape = tuple(len(c) for c in reversed(coords))
-    acc = np.zeros(shape)
+    modes = []
     for _ in range(nmodes):
         k = lattice[rng.below(len(lattice))]
         phi = 2.0 * math.pi * rng.uniform()
-        # e^{i(k.x + phi)} = e^{i phi} * prod_d e^{i k_d x_d}
-        field = np.full(shape, complex(math.cos(phi), math.sin(phi)))
-        for d in range(ndim):
-            ph = k[d] * coords[d]
-            e1 = np.cos(ph) + 1j * np.sin(ph)
-            bshape = [1] * ndim
-            bshape[ndim - 1 - d] = len(coords[d])
-            field = field * e1.reshape(bshape)
-        term = field.real / nmodes
+        wgt = 1.0 / nmodes
         if over_k:
-            term = term / math.sqrt(sum(kk * kk for kk in k))
-        acc += term
-    return amp * acc
+            wgt = wgt / math.sqrt(sum(kk * kk for kk in k))
+        modes.append((k, phi, wgt))
+    x = coords[0]
+    if ndim == 2:
+        y = coords[1]
+        A = np.empty((len(y), 2 * nmodes))
+        B = np.empty((2 * nmodes, len(x)))
+        for m, (k, phi, wgt) in enumerate(modes):
+            A[:, 2 * m] = wgt * np.cos(k[1] * y + phi)
+            A[:, 2 * m + 1] = -wgt * np.sin(k[1] * y + phi)
+            B[2 * m] = np.cos(k[0] * x)
+            B[2 * m + 1] = np.sin(k[0] * x)
+        return amp * (A @ B)
+    y, z = coords[1], coords[2]
+    ny, nz = len(y), len(z)
+    A = np.empty((nz, ny, 2 * nmodes))
+    B = np.empty((2 * nmodes, len(x)))
+    for m, (k, phi, wgt) in enumerate(modes):
+        ph = k[2] * z[:, None] + k[1] * y[None, :] + phi
+        A[:, :, 2 * m] = wgt * np.cos(ph)
+        A[:, :, 2 * m + 1] = -wgt * np.sin(ph)
+        B[2 * m] = np.cos(k[0] * x)
+        B[2 * m + 1] = np.sin(k[0] * x)
+    return amp * (A.reshape(nz * ny, -1) @ B).reshape(nz, ny, len(x))
 
 
 def _centred_diff(a: np.ndarray, d: int, dx_d: np.ndarray, ndim: int) -> np.ndarray:

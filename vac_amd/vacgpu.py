@@ -62,6 +62,21 @@ SIGNATURES = {
     "vacgpu_supported": ([C.POINTER(Config), C.POINTER(Method)], C.c_int),
     "vacgpu_last_error": ([C.c_void_p], C.c_char_p),
     "vacgpu_destroy": ([C.c_void_p], None),
+    "vacgpu_batch_footprint": ([C.POINTER(Config)], C.c_longlong),
+    "vacgpu_batch_create": ([C.c_longlong, C.c_int, C.POINTER(C.c_void_p)], C.c_int),
+    "vacgpu_batch_add": ([C.c_void_p, C.POINTER(Config), C.POINTER(Method), _DP, _DP, _DP, _IP,
+                          C.c_int, C.c_double, C.c_int, C.c_double, C.c_int, _IP], C.c_int),
+    "vacgpu_batch_remove": ([C.c_void_p, C.c_int], C.c_int),
+    "vacgpu_batch_step": ([C.c_void_p, C.c_int], C.c_int),
+    "vacgpu_batch_sync": ([C.c_void_p], C.c_int),
+    "vacgpu_batch_status": ([C.c_void_p, C.c_int, _IP, _DP, _DP, _IP, _IP], C.c_int),
+    "vacgpu_batch_get_state": ([C.c_void_p, C.c_int, _DP, _IP], C.c_int),
+    "vacgpu_batch_graph": ([C.c_void_p, C.c_int], C.c_int),
+    "vacgpu_batch_tune": ([C.c_void_p, _IP], C.c_int),
+    "vacgpu_batch_set_tiles": ([C.c_void_p, _IP], C.c_int),
+    "vacgpu_batch_bytes_used": ([C.c_void_p], C.c_longlong),
+    "vacgpu_batch_last_error": ([C.c_void_p], C.c_char_p),
+    "vacgpu_batch_destroy": ([C.c_void_p], None),
 }
 
 
@@ -118,6 +133,23 @@ class MethodSpec:
 
 def nvars(physics: str, ndim: int) -> int:
     return 2 + 2 * ndim if physics == "mhd" else 2 + ndim
+
+
+def make_config(nx, physics: str, gamma: float, bc="periodic", precision: str = "fp64") -> Config:
+    ndim = len(nx)
+    cfg = Config()
+    cfg.ndim = ndim
+    cfg.physics = MHD if physics == "mhd" else HD
+    for d in range(ndim):
+        cfg.nx[d] = int(nx[d])
+    cfg.eqpar[0] = gamma
+    cfg.eqpar[1] = 0.0
+    bcs = [bc] * (2 * ndim) if isinstance(bc, str) else list(bc)
+    for f in range(2 * ndim):
+        for iw in range(8):
+            cfg.typeB[f][iw] = PERIODIC if bcs[f] == "periodic" else CONT
+    cfg.precision = FP32BULK if precision == "fp32bulk" else FP64
+    return cfg
 
 
 class Grid:
@@ -272,3 +304,106 @@ def host_loop(grid: Grid, method: MethodSpec, nsteps: int, it0: int = 0, t0: flo
         t = t + dt
         hist.append(t)
     return np.array(hist), nerror
+
+
+class Batch:
+    """Many independent 2D TVDLF grids advanced together (vacgpu_batch).
+
+    Each instance follows VAC's time loop with its own dt, t, it and stop
+    condition; a finished instance stops on the device.  ``add`` raises
+    ``BudgetFull`` when the instance does not fit the HBM budget.
+    """
+
+    class BudgetFull(Exception):
+        pass
+
+    def __init__(self, hbm_budget_bytes: int, capacity: int = 1024):
+        self.lib = load_library()
+        h = C.c_void_p()
+        rc = self.lib.vacgpu_batch_create(int(hbm_budget_bytes), int(capacity), C.byref(h))
+        if rc != 0:
+            raise VacGpuError(f"vacgpu_batch_create failed ({rc})")
+        self.h = h
+        self.meta = {}
+
+    def _check(self, rc, what):
+        if rc < 0:
+            msg = self.lib.vacgpu_batch_last_error(self.h)
+            raise VacGpuError(f"{what}: {msg.decode() if msg else rc}")
+        return rc
+
+    @staticmethod
+    def footprint(nx, physics, gamma=5.0 / 3.0, precision="fp64") -> int:
+        lib = load_library()
+        cfg = make_config(nx, physics, gamma, precision=precision)
+        return int(lib.vacgpu_batch_footprint(C.byref(cfg)))
+
+    def add(self, w_mesh, nx, physics, gamma, dxs, method: MethodSpec, bc="periodic",
+            precision="fp64", it=0, t=0.0, itmax=1 << 30, tmax=1e99, tmaxexact=False) -> int:
+        cfg = make_config(nx, physics, gamma, bc=bc, precision=precision)
+        m = method.to_c()
+        w = np.ascontiguousarray(np.pad(np.asarray(w_mesh, dtype=np.float64),
+                                        [(0, 0), (2, 2), (2, 2)]))
+        dx0 = np.ascontiguousarray(dxs[0], dtype=np.float64)
+        dx1 = np.ascontiguousarray(dxs[1], dtype=np.float64)
+        slot = C.c_int()
+        rc = self.lib.vacgpu_batch_add(self.h, C.byref(cfg), C.byref(m), _ptr(dx0), _ptr(dx1),
+                                       _ptr(w), None, it, t, itmax, tmax, int(tmaxexact),
+                                       C.byref(slot))
+        if rc == -4:
+            raise Batch.BudgetFull(self.lib.vacgpu_batch_last_error(self.h).decode())
+        self._check(rc, "batch_add")
+        self.meta[slot.value] = (tuple(int(n) for n in nx), nvars(physics, 2))
+        return slot.value
+
+    def remove(self, slot: int):
+        self._check(self.lib.vacgpu_batch_remove(self.h, slot), "batch_remove")
+        self.meta.pop(slot, None)
+
+    def step(self, nsteps: int = 1):
+        self._check(self.lib.vacgpu_batch_step(self.h, nsteps), "batch_step")
+
+    def sync(self):
+        self._check(self.lib.vacgpu_batch_sync(self.h), "batch_sync")
+
+    def status(self, slot: int):
+        it = C.c_int()
+        t = C.c_double()
+        dt = C.c_double()
+        done = C.c_int()
+        ne = np.zeros(4, dtype=np.int32)
+        self._check(self.lib.vacgpu_batch_status(self.h, slot, C.byref(it), C.byref(t), C.byref(dt),
+                                                 C.byref(done), ne.ctypes.data_as(_IP)), "status")
+        return {"it": it.value, "t": t.value, "dt": dt.value, "done": bool(done.value), "nerror": ne}
+
+    def get_mesh(self, slot: int) -> np.ndarray:
+        nx, nw = self.meta[slot]
+        w = np.empty((nw, nx[1] + 4, nx[0] + 4))
+        self._check(self.lib.vacgpu_batch_get_state(self.h, slot, _ptr(w), None), "get_state")
+        return np.ascontiguousarray(w[:, 2:-2, 2:-2])
+
+    def graph(self, enable: bool):
+        self._check(self.lib.vacgpu_batch_graph(self.h, int(enable)), "batch_graph")
+
+    def tune(self):
+        rows = np.zeros(4, dtype=np.int32)
+        self._check(self.lib.vacgpu_batch_tune(self.h, rows.ctypes.data_as(_IP)), "batch_tune")
+        return [int(r) for r in rows]
+
+    def set_tiles(self, rows):
+        r = np.ascontiguousarray(rows, dtype=np.int32)
+        self._check(self.lib.vacgpu_batch_set_tiles(self.h, r.ctypes.data_as(_IP)), "set_tiles")
+
+    def bytes_used(self) -> int:
+        return int(self.lib.vacgpu_batch_bytes_used(self.h))
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.vacgpu_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
