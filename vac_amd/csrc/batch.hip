@@ -665,6 +665,8 @@ int vacgpu_batch_add(vacgpu_batch* b, const vacgpu_config* cfg, const vacgpu_met
   b->dirty = true;
   b->fresh.push_back(s);
   *slot = s;
+  // the caller may release its host arrays on return
+  BHIP(b, hipStreamSynchronize(b->stream));
   return 0;
 }
 
