@@ -83,6 +83,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32bulk"])
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-tvdlf", action="store_true")
+    ap.add_argument("--no-3d", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-serving", action="store_true")
     args = ap.parse_args()
@@ -154,6 +156,10 @@ def main():
                      "b_alg_per_cell": B_ALG_TVD_SWEEP},
         "other_kernels_avg_ms": {k: round(v[0], 5) for k, v in other.items()},
     }
+    if not args.no_tvdlf:
+        out["tvdlf_2048"] = tvdlf_2048(w0, dxs, min(args.steps, 100))
+    if not args.no_3d:
+        out["mhd3d_256"] = mhd3d_256()
     if not args.no_batched:
         out["batched"] = batched()
     if not args.no_serving:
@@ -161,6 +167,103 @@ def main():
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(w0, coords)
     print(json.dumps(out))
+
+
+def errors(a, ref):
+    """Per conserved variable: relative L1 = sum|a-ref|/sum|ref|, Linf = max|a-ref|."""
+    nw = ref.shape[0]
+    d = np.abs(a.reshape(nw, -1) - ref.reshape(nw, -1))
+    l1 = d.sum(axis=1) / np.abs(ref.reshape(nw, -1)).sum(axis=1)
+    return {"rel_l1": [float(f"{v:.3e}") for v in l1],
+            "linf": [float(f"{v:.3e}") for v in d.max(axis=1)]}
+
+
+def timed_run(shape, physics, dxs, w0, method, precision, steps, kernels):
+    """Warm up, reset, time `steps` full steps; then time the listed kernels."""
+    from vac_amd import vacgpu
+
+    g = vacgpu.Grid(shape, physics, GAMMA, dxs, precision=precision)
+    g.put_mesh(w0)
+    g.run(method, 3)
+    g.put_mesh(w0)
+    g.set_time(0, 0.0)
+    t0 = time.perf_counter()
+    n, _ = g.run(method, steps)
+    elapsed = time.perf_counter() - t0
+    if n != steps:
+        raise RuntimeError(f"ran {n} of {steps} steps")
+    w = g.get_mesh()
+    g.put_mesh(w0)
+    g.set_time(0, 0.0)
+    g.profile(True)
+    g.run(method, min(steps, 20))
+    kt = {k: round(g.kernel_time(k)[0], 5) for k in kernels}
+    g.profile(False)
+    g.close()
+    return elapsed, w, kt
+
+
+def tvdlf_2048(w0, dxs, steps):
+    """2D MHD 2048^2, TVDLF + minmod, FP64 and FP32-bulk.  Effective HBM
+    bandwidth = 480 B (VAC's dataflow: per sweep the predictor reads w and
+    writes w1, the corrector reads w1 and w and writes w; 5*ndim*nw*8 B) per
+    cell-update / time."""
+    from vac_amd import vacgpu
+
+    method = vacgpu.MethodSpec(typefull="tvdlf", limiter="minmod", courantpar=0.8)
+    res = {}
+    ref = None
+    for prec in ("fp64", "fp32bulk"):
+        el, w, kt = timed_run([NX, NY], "mhd", dxs, w0, method, prec, steps,
+                              ("tvdlf_pred", "tvdlf_corr", "bound", "reduce"))
+        rate = NX * NY * steps / el
+        r = {"mcell_updates_per_s": round(rate / 1e6, 2), "ms_per_step": round(el / steps * 1e3, 4),
+             "effective_hbm_gbs": round(480 * rate / 1e9, 1),
+             "effective_hbm_frac": round(480 * rate / 1e9 / HBM_PEAK_GBS, 4),
+             "kernel_avg_ms": kt}
+        if ref is None:
+            ref = w
+        else:
+            r["error_vs_fp64"] = errors(w, ref)
+        res[prec] = r
+    res["fp32bulk_over_fp64"] = round(res["fp32bulk"]["mcell_updates_per_s"]
+                                      / res["fp64"]["mcell_updates_per_s"], 3)
+    res["workload"] = (f"2D MHD 2048x2048 periodic random-Fourier, TVDLF+minmod, courantpar 0.8, "
+                       f"{steps} steps; B_alg 480 B/cell-update")
+    return res
+
+
+def mhd3d_256(steps=100):
+    """3D MHD 256^3, TVDLF + minmod, courantpar 0.6: FP64 and FP32-bulk,
+    with the FP32-bulk error per variable against the FP64 result (the FP64
+    path is bitwise equal to the oracle at every tested size)."""
+    from vac_amd import vacgpu
+    from vac_amd.grid import widths_1d
+    from vac_amd.initial import random_fourier
+
+    n = 256
+    w0, coords = random_fourier((n, n, n), SEED + 3, GAMMA)
+    dxs = [widths_1d(c) for c in coords]
+    method = vacgpu.MethodSpec(typefull="tvdlf", limiter="minmod", courantpar=0.6)
+    res = {}
+    ref = None
+    for prec in ("fp64", "fp32bulk"):
+        el, w, kt = timed_run([n, n, n], "mhd", dxs, w0, method, prec, steps,
+                              ("tvdlf_pred", "tvdlf_corr", "bound", "reduce"))
+        rate = n ** 3 * steps / el
+        r = {"mcell_updates_per_s": round(rate / 1e6, 2), "ms_per_step": round(el / steps * 1e3, 4),
+             "effective_hbm_frac": round(960 * rate / 1e9 / HBM_PEAK_GBS, 4),
+             "kernel_avg_ms": kt}
+        if ref is None:
+            ref = w
+        else:
+            r["error_vs_fp64"] = errors(w, ref)
+        res[prec] = r
+    res["fp32bulk_over_fp64"] = round(res["fp32bulk"]["mcell_updates_per_s"]
+                                      / res["fp64"]["mcell_updates_per_s"], 3)
+    res["workload"] = (f"3D MHD 256^3 periodic random-Fourier (B = curl A), TVDLF+minmod, "
+                       f"courantpar 0.6, {steps} steps; B_alg 960 B/cell-update")
+    return res
 
 
 def batched():

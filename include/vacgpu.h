@@ -114,8 +114,16 @@ int vacgpu_get_scalars(vacgpu_grid* g, double* dtcourant, int* nerror, double* s
 int vacgpu_profile(vacgpu_grid* g, int enable);
 /* Average duration (ms) and launch count of kernel family k since enabling:
  * k = 0 TVD sweep, 1 TVDLF predictor, 2 TVDLF corrector, 3 Ryu source,
- * 4 ghost cells, 5 reductions. */
+ * 4 ghost cells, 5 reductions, 6 fused TVDLF sweep. */
 int vacgpu_kernel_time(vacgpu_grid* g, int k, double* avg_ms, long* count);
+
+/* Kernel choice for 2D TVDLF sweeps (no counterpart in VAC; results are
+ * bitwise the same either way).  fused = 1 (the default) runs predictor and
+ * corrector of a sweep in one launch when every boundary face has one type
+ * for all variables; fused = 0 runs them as two launches with the predictor
+ * state stored in between.  seglen = rows per marching segment of the fused
+ * kernels (0 = automatic, else clamped to [4, 256]). */
+int vacgpu_set_kernels(vacgpu_grid* g, int fused, int seglen);
 
 /* Start-up check of a selector set: 0 if supported, else VACGPU_E_ARG. */
 int vacgpu_supported(const vacgpu_config* cfg, const vacgpu_method* m);

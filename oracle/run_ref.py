@@ -137,7 +137,9 @@ def problem_par(nw: int, wnames: str, steps: int, typefull: str = "tvdlf",
     ndim_faces = len(typeB) if typeB is not None else 4
     if typeB is None:
         typeB = ["periodic"] * ndim_faces
-    tb = ",".join(f"{nw}*'{t}'" for t in typeB)
+    # an entry is one type for all variables of a region, or a list per variable
+    tb = ",".join(f"{nw}*'{t}'" if isinstance(t, str) else ",".join(f"'{v}'" for v in t)
+                  for t in typeB)
     save = " itsave(1,2)=0\n ditsave(2)=1\n" if every_step else " itsave(1,2)=0\n"
     stop = f" itmax={steps}\n"
     if tmax is not None:

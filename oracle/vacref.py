@@ -87,11 +87,14 @@ def make_params(ndim, physics, nx, gamma, typefull="tvdlf", limiter="minmod",
     lims = [limiter] * 8 if isinstance(limiter, str) else list(limiter) + ["minmod"] * 8
     for i in range(8):
         p.limiter[i] = LIMITERS[lims[i]]
+    # bc: one type for all faces, or one entry per face; an entry is a type
+    # or a list of types per variable
     bcs = [bc] * 6 if isinstance(bc, str) else list(bc)
     for f in range(6):
-        code = PERIODIC if (f < len(bcs) and bcs[f] == "periodic") else CONT
+        spec = bcs[f] if f < len(bcs) else "cont"
         for iw in range(8):
-            p.bc[f][iw] = code
+            name = spec if isinstance(spec, str) else spec[min(iw, len(spec) - 1)]
+            p.bc[f][iw] = PERIODIC if name == "periodic" else CONT
     p.divbfix = int(divbfix)
     p.divbwave = int(divbwave)
     p.sourceunsplit = int(sourceunsplit)

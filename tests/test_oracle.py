@@ -168,3 +168,23 @@ def test_oracle_against_live_reference():
     out, t, st = _run_state(w, [40, 30], "mhd", g, 12, typefull="tvd", limiter="superbee")
     assert np.array_equal(np.array([s.t for s in r.snapshots]), t)
     assert np.array_equal(r.snapshots[-1].w, out)
+
+
+def test_oracle_per_variable_boundaries_against_live_reference():
+    """typeB set per variable (density periodic, the rest 'cont' on the x
+    faces): the oracle's getboundary follows VAC's per-variable tables."""
+    from run_ref import have_ref, make_ini, problem_par, run_ref
+    if not have_ref("vac_hd22_132"):
+        pytest.skip("reference executable not built (oracle/build_ref.sh)")
+    g = 5 / 3
+    w, c = random_fourier((40, 30), 5, g, physics="hd")
+    mixed = ["periodic", "cont", "cont", "cont"]
+    bc = [mixed, mixed, "periodic", "periodic"]
+    ini = make_ini(w, c, "hd", g)
+    par = problem_par(4, "rho m1 m2 e", 10, typefull="tvdlf", limiter="minmod", typeB=bc,
+                      every_step=True)
+    r = run_ref("vac_hd22_132", ini, par)
+    p = vacref.make_params(2, "hd", [40, 30], g, typefull="tvdlf", limiter="minmod", bc=bc)
+    out, t, st = vacref.run(p, w, [widths_1d(cc) for cc in c], 10)
+    assert np.array_equal(np.array([s.t for s in r.snapshots]), t)
+    assert np.array_equal(r.snapshots[-1].w, out)

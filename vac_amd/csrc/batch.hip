@@ -265,6 +265,9 @@ struct vacgpu_batch {
   long long budget = 0, used_bytes = 0;
   std::vector<HostInst> inst;
   InstDev* d_inst = nullptr;
+  InstDev* h_inst = nullptr;  // pinned mirror, uploaded whole by rebuild
+  double* stage = nullptr;    // pinned staging for state uploads / downloads
+  size_t stage_bytes = 0;
   BScal* d_bs = nullptr;
   BScal* h_bs = nullptr;  // pinned mirror
   int cap = 0;
@@ -305,6 +308,17 @@ int bfail(vacgpu_batch* b, int code, const std::string& m) {
     if (e_ != hipSuccess)                                                             \
       return bfail(b, VACGPU_E_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
   } while (0)
+
+// Device blocks are kept in size classes (64 KiB steps up to 1 MiB, then
+// four classes per power of two) so that freed blocks serve later instances
+// of similar size; the class size is what an instance occupies.
+size_t pool_round(size_t bytes) {
+  const size_t k64 = size_t(1) << 16;
+  if (bytes <= (size_t(1) << 20)) return (bytes + k64 - 1) / k64 * k64;
+  int lg = 63 - __builtin_clzll((unsigned long long)bytes);
+  const size_t q = size_t(1) << (lg - 2);
+  return (bytes + q - 1) / q * q;
+}
 
 double* pool_get(vacgpu_batch* b, size_t bytes) {
   auto it = b->pool.find(bytes);
@@ -347,12 +361,23 @@ Geo make_geo(const vacgpu_config& c) {
   return geo;
 }
 
+size_t plane_bytes_of(const Geo& g, int nw) { return pool_round(sizeof(double) * (size_t)g.ps * nw); }
+size_t dx_bytes_of(const Geo& g) { return pool_round(sizeof(double) * (size_t)(g.ng[0] + g.ng[1])); }
+
 long long footprint(const vacgpu_config& c) {
   const Geo g = make_geo(c);
   const int nw = c.physics == VACGPU_MHD ? 2 + 2 * c.ndim : 2 + c.ndim;
-  long long b = 3LL * nw * g.ps * (long long)sizeof(double);
-  b += (long long)sizeof(double) * (g.ng[0] + g.ng[1]);
-  return b;
+  return (long long)(3 * plane_bytes_of(g, nw) + dx_bytes_of(g));
+}
+
+int ensure_stage(vacgpu_batch* b, size_t bytes) {
+  if (bytes <= b->stage_bytes) return 0;
+  if (b->stage) BHIP(b, hipHostFree(b->stage));
+  b->stage = nullptr;
+  b->stage_bytes = 0;
+  BHIP(b, hipHostMalloc(&b->stage, bytes, hipHostMallocDefault));
+  b->stage_bytes = bytes;
+  return 0;
 }
 
 int ensure_launch(vacgpu_batch* b, Launch& L, const std::vector<Work>& v) {
@@ -397,8 +422,10 @@ int rebuild(vacgpu_batch* b) {
       (k.mhd ? g6 : g4).push_back(gw);
     }
     ids.push_back(i);
-    BHIP(b, hipMemcpyAsync(b->d_inst + i, &h.dev, sizeof(InstDev), hipMemcpyHostToDevice, b->stream));
+    b->h_inst[i] = h.dev;
   }
+  if (!ids.empty())
+    BHIP(b, hipMemcpyAsync(b->d_inst, b->h_inst, sizeof(InstDev) * (ids.back() + 1), hipMemcpyHostToDevice, b->stream));
   // partial arrays
   if (pbase > b->pstride) {
     if (b->part) BHIP(b, hipFree(b->part));
@@ -562,6 +589,7 @@ int vacgpu_batch_create(long long hbm_budget_bytes, int capacity, vacgpu_batch**
   b->cap = capacity;
   b->inst.resize(capacity);
   hipError_t e = hipMalloc(&b->d_inst, sizeof(InstDev) * capacity);
+  if (e == hipSuccess) e = hipHostMalloc(&b->h_inst, sizeof(InstDev) * capacity, hipHostMallocDefault);
   if (e == hipSuccess) e = hipMalloc(&b->d_bs, sizeof(BScal) * capacity);
   if (e == hipSuccess) e = hipHostMalloc(&b->h_bs, sizeof(BScal) * capacity, hipHostMallocDefault);
   if (e == hipSuccess) e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking);
@@ -618,8 +646,8 @@ int vacgpu_batch_add(vacgpu_batch* b, const vacgpu_config* cfg, const vacgpu_met
   I.nbound = ghost_slabs(I.geo, slab);
   I.slab0 = slab[0];
   I.slab1 = slab[1];
-  const size_t plane_bytes = sizeof(double) * (size_t)I.geo.ps * nw;
-  const size_t dx_bytes = sizeof(double) * (I.geo.ng[0] + I.geo.ng[1]);
+  const size_t plane_bytes = plane_bytes_of(I.geo, nw);
+  const size_t dx_bytes = dx_bytes_of(I.geo);
   I.w = pool_get(b, plane_bytes);
   I.b = pool_get(b, plane_bytes);
   I.w1 = pool_get(b, plane_bytes);
@@ -636,11 +664,23 @@ int vacgpu_batch_add(vacgpu_batch* b, const vacgpu_config* cfg, const vacgpu_met
   I.geo.dx[2] = dxd;
   BHIP(b, hipMemcpyAsync(dxd, dx0, sizeof(double) * I.geo.ng[0], hipMemcpyHostToDevice, b->stream));
   BHIP(b, hipMemcpyAsync(dxd + I.geo.ng[0], dx1, sizeof(double) * I.geo.ng[1], hipMemcpyHostToDevice, b->stream));
-  // state (full ixG with caller leading dimensions)
+  // state (full ixG with caller leading dimensions), through pinned staging:
+  // one host copy, then pitched DMA per variable
   const int l0 = ld ? ld[0] : I.geo.ng[0], l1 = ld ? ld[1] : I.geo.ng[1];
+  const size_t hbytes = sizeof(double) * (size_t)nw * l0 * l1;
+  int rc;
+  if ((rc = ensure_stage(b, hbytes))) {
+    pool_put(b, plane_bytes, I.w);
+    pool_put(b, plane_bytes, I.b);
+    pool_put(b, plane_bytes, I.w1);
+    pool_put(b, dx_bytes, dxd);
+    return rc;
+  }
+  BHIP(b, hipStreamSynchronize(b->stream));  // the stage may still feed an earlier copy
+  memcpy(b->stage, w, hbytes);
   for (int iw = 0; iw < nw; ++iw)
     BHIP(b, hipMemcpy2DAsync(I.w + iw * I.geo.ps + I.geo.off, sizeof(double) * I.geo.s[1],
-                             w + (size_t)iw * l0 * l1, sizeof(double) * l0,
+                             b->stage + (size_t)iw * l0 * l1, sizeof(double) * l0,
                              sizeof(double) * I.geo.ng[0], I.geo.ng[1], hipMemcpyHostToDevice, b->stream));
   BHIP(b, hipMemsetAsync(I.b, 0, plane_bytes, b->stream));
   BHIP(b, hipMemsetAsync(I.w1, 0, plane_bytes, b->stream));
@@ -676,8 +716,8 @@ int vacgpu_batch_remove(vacgpu_batch* b, int slot) {
   BHIP(b, hipStreamSynchronize(b->stream));
   HostInst& h = b->inst[slot];
   const int nw = h.dev.mhd ? 6 : 4;
-  const size_t plane_bytes = sizeof(double) * (size_t)h.dev.geo.ps * nw;
-  const size_t dx_bytes = sizeof(double) * (h.dev.geo.ng[0] + h.dev.geo.ng[1]);
+  const size_t plane_bytes = plane_bytes_of(h.dev.geo, nw);
+  const size_t dx_bytes = dx_bytes_of(h.dev.geo);
   pool_put(b, plane_bytes, h.dev.w);
   pool_put(b, plane_bytes, h.dev.b);
   pool_put(b, plane_bytes, h.dev.w1);
@@ -742,11 +782,20 @@ int vacgpu_batch_get_state(vacgpu_batch* b, int slot, double* w, const int* ld) 
   const InstDev& I = h.dev;
   const int nw = I.mhd ? 6 : 4;
   const int l0 = ld ? ld[0] : I.geo.ng[0], l1 = ld ? ld[1] : I.geo.ng[1];
+  const size_t hbytes = sizeof(double) * (size_t)nw * l0 * l1;
+  int rc;
+  BHIP(b, hipStreamSynchronize(b->stream));
+  if ((rc = ensure_stage(b, hbytes))) return rc;
   for (int iw = 0; iw < nw; ++iw)
-    BHIP(b, hipMemcpy2DAsync(w + (size_t)iw * l0 * l1, sizeof(double) * l0,
+    BHIP(b, hipMemcpy2DAsync(b->stage + (size_t)iw * l0 * l1, sizeof(double) * l0,
                              I.w + iw * I.geo.ps + I.geo.off, sizeof(double) * I.geo.s[1],
                              sizeof(double) * I.geo.ng[0], I.geo.ng[1], hipMemcpyDeviceToHost, b->stream));
   BHIP(b, hipStreamSynchronize(b->stream));
+  // rows beyond ixG in the caller's layout are left as the caller had them
+  for (int iw = 0; iw < nw; ++iw)
+    for (int y = 0; y < I.geo.ng[1]; ++y)
+      memcpy(w + (size_t)iw * l0 * l1 + (size_t)y * l0, b->stage + (size_t)iw * l0 * l1 + (size_t)y * l0,
+             sizeof(double) * I.geo.ng[0]);
   return 0;
 }
 
@@ -881,6 +930,8 @@ void vacgpu_batch_destroy(vacgpu_batch* b) {
   if (b->d_inst) (void)hipFree(b->d_inst);
   if (b->d_bs) (void)hipFree(b->d_bs);
   if (b->h_bs) (void)hipHostFree(b->h_bs);
+  if (b->h_inst) (void)hipHostFree(b->h_inst);
+  if (b->stage) (void)hipHostFree(b->stage);
   if (b->stream) (void)hipStreamDestroy(b->stream);
   delete b;
 }

@@ -18,7 +18,7 @@ namespace {
 constexpr int BX = 64, BY = 4;      // cell kernels: 64 lanes along x, 4 rows
 constexpr int NSLOT = 8;            // partial-reduction slots
 enum { SLOT_C0 = 0, SLOT_C1 = 1, SLOT_PMIN = 2, SLOT_PMAX = 3, SLOT_DX = 4 };
-enum { KF_TVD = 0, KF_PRED = 1, KF_CORR = 2, KF_RYU = 3, KF_BOUND = 4, KF_RED = 5, KF_N = 6 };
+enum { KF_TVD = 0, KF_PRED = 1, KF_CORR = 2, KF_RYU = 3, KF_BOUND = 4, KF_RED = 5, KF_FUSED = 6, KF_N = 7 };
 
 struct Geo {
   int nd;
@@ -190,25 +190,23 @@ __global__ __launch_bounds__(256) void k_bound(Geo g, BC bc, double* __restrict_
 }
 
 // ------------------------------------------------------------ TVDLF
-// Hancock predictor (src/vactvdlf.t:5-89) with qdt = dt/2:
-// w1 = w - qdt/dx*(F(w+l/2) - F(w-l/2)) - (qdt/ndim)*S_Powell(w)
+// Hancock predictor (src/vactvdlf.t:5-89) at one cell with qdt = dt/2:
+// w1 = w - qdt/dx*(F(w+l/2) - F(w-l/2)) - (qdt/ndim)*S_Powell(w).
+// idx / c address the cell; reads w at the cell and its neighbours.
 template <int ND, bool MHD, class R, int D>
-__device__ __forceinline__ void tvdlf_pred_body(const Geo& g, const Sel& s,
-                                                const double* __restrict__ w,
-                                                double* __restrict__ w1, double dt, double smallp,
-                                                int bx, int by, int bz) {
+__device__ __forceinline__ void pred_point(const Geo& g, const Sel& s, const double* __restrict__ w,
+                                           long long idx, const int* c, double dt, double smallp,
+                                           double* out) {
   constexpr int d = D;
   using L = Layout<ND, MHD>;
   constexpr int NW = L::nw;
-  const CellIx cl = cell_at(g, bx, by, bz);
-  if (!cl.ok) return;
   const double qdt = dt / 2;
   const long long sd = g.s[d];
   double uc[NW], wl[NW], wr[NW];
-  load<NW>(w, g.ps, cl.idx, uc);
+  load<NW>(w, g.ps, idx, uc);
 #pragma unroll
   for (int iw = 0; iw < NW; ++iw) {
-    const double um = w[iw * g.ps + cl.idx - sd], up = w[iw * g.ps + cl.idx + sd];
+    const double um = w[iw * g.ps + idx - sd], up = w[iw * g.ps + idx + sd];
     const R l = limiter<R>(s.lim[iw], R(up - uc[iw]), R(uc[iw] - um));
     wr[iw] = uc[iw] - 0.5 * double(l);
     wl[iw] = uc[iw] + 0.5 * double(l);
@@ -216,16 +214,27 @@ __device__ __forceinline__ void tvdlf_pred_body(const Geo& g, const Sel& s,
   double fL[NW], fR[NW];
   fluxes<ND, MHD, R>(wl, d, s.gamma, smallp, fL);
   fluxes<ND, MHD, R>(wr, d, s.gamma, smallp, fR);
-  const double qdx = qdt / g.dx[d][cl.c[d]];
-  double out[NW];
+  const double qdx = qdt / g.dx[d][c[d]];
 #pragma unroll
   for (int iw = 0; iw < NW; ++iw) out[iw] = uc[iw] - qdx * (fL[iw] - fR[iw]);
   if constexpr (MHD) {
     if (s.sourceunsplit && s.divbfix) {
-      const double dv = divb_at<ND>(g, w, cl.idx, cl.c);
+      const double dv = divb_at<ND>(g, w, idx, c);
       powell<ND>(uc, dv, qdt * 1.0 / ND, out);
     }
   }
+}
+
+template <int ND, bool MHD, class R, int D>
+__device__ __forceinline__ void tvdlf_pred_body(const Geo& g, const Sel& s,
+                                                const double* __restrict__ w,
+                                                double* __restrict__ w1, double dt, double smallp,
+                                                int bx, int by, int bz) {
+  constexpr int NW = Layout<ND, MHD>::nw;
+  const CellIx cl = cell_at(g, bx, by, bz);
+  if (!cl.ok) return;
+  double out[NW];
+  pred_point<ND, MHD, R, D>(g, s, w, cl.idx, cl.c, dt, smallp, out);
 #pragma unroll
   for (int iw = 0; iw < NW; ++iw) w1[iw * g.ps + cl.idx] = out[iw];
 }
@@ -238,11 +247,50 @@ __global__ __launch_bounds__(256) void k_tvdlf_pred(Geo g, Sel s, const double* 
                                  (int)blockIdx.z);
 }
 
-// TVDLF corrector (tvdmusclf, src/vactvdlf.t:92-299) for the faces i-1/2 and
-// i+1/2 of cell i: MUSCL states of w1 with limited slopes of w
-// (typelimited='original'), local Lax-Friedrichs flux with cmax of the mean
-// state, Powell source of w1 with dt/ndim.  Writes the partial max of
-// 2*cmax/(dx_j+dx_j+1) (src/vactvdlf.t:190) to part[SLOT_C0].
+// TVDLF corrector flux at the face J+1/2 (src/vactvdlf.t:147-254): MUSCL
+// states wR = w1(J+1) - l(J+1)/2, wL = w1(J) + l(J)/2 from the predictor
+// states pl = w1(J), pr = w1(J+1) and the limited slopes ll, lr of the
+// start-of-sweep w (typelimited='original'); cmax of the mean state; the
+// wLC = 2*mean - wRC restore (:205); F = (F(wL) + F(wR))/2 - cmax*(wR - wL)/2.
+// cv = 2*cmax/(dx(J) + dx(J+1)) is the face's Courant value (:190).
+template <int ND, bool MHD, class R, int D>
+__device__ __forceinline__ void corr_face(const Sel& s, const double* pl, const double* pr,
+                                          const double* ll, const double* lr, double dxsum,
+                                          double smallp, double* F, double& cv) {
+  constexpr int d = D;
+  constexpr int NW = Layout<ND, MHD>::nw;
+  double wlf[NW], wrf[NW], mean[NW];
+#pragma unroll
+  for (int iw = 0; iw < NW; ++iw) {
+    wrf[iw] = pr[iw] - 0.5 * lr[iw];
+    wlf[iw] = pl[iw] + 0.5 * ll[iw];
+  }
+#pragma unroll
+  for (int iw = 0; iw < NW; ++iw) mean[iw] = 0.5 * (wlf[iw] + wrf[iw]);
+  const R cm = cmax<ND, MHD, R>(mean, d, s.gamma, smallp);
+  cv = double(cm) * 2.0 / dxsum;
+#pragma unroll
+  for (int iw = 0; iw < NW; ++iw) wlf[iw] = 2.0 * mean[iw] - wrf[iw];
+  double fL[NW], fR[NW];
+  fluxes<ND, MHD, R>(wlf, d, s.gamma, smallp, fL);
+  fluxes<ND, MHD, R>(wrf, d, s.gamma, smallp, fR);
+#pragma unroll
+  for (int iw = 0; iw < NW; ++iw) {
+    const double fm = 0.5 * (fL[iw] + fR[iw]);
+    const R diss = -(cm * R(0.5) * R(wrf[iw] - wlf[iw]));
+    F[iw] = fm + double(diss);
+  }
+}
+
+// limited slope of w at a cell along d from w(i-1), w(i), w(i+1)
+template <class R>
+__device__ __forceinline__ double slope(int lim, double a, double b, double e) {
+  return double(limiter<R>(lim, R(e - b), R(b - a)));
+}
+
+// TVDLF corrector (tvdmusclf, src/vactvdlf.t:92-299) for the cell i from its
+// faces i-1/2 and i+1/2, then the Powell source of w1 with dt/ndim.
+// Returns the max face Courant value (-inf outside the mesh).
 template <int ND, bool MHD, class R, int D>
 __device__ __forceinline__ double tvdlf_corr_body(const Geo& g, const Sel& s,
                                                   const double* __restrict__ w,
@@ -263,52 +311,28 @@ __device__ __forceinline__ double tvdlf_corr_body(const Geo& g, const Sel& s,
     for (int iw = 0; iw < NW; ++iw) {
       const double* p = w + iw * g.ps + cl.idx;
       const double a = p[-2 * sd], b = p[-sd], c = p[0], e = p[sd], f = p[2 * sd];
-      lm[iw] = double(limiter<R>(s.lim[iw], R(c - b), R(b - a)));
-      l0[iw] = double(limiter<R>(s.lim[iw], R(e - c), R(c - b)));
-      lp[iw] = double(limiter<R>(s.lim[iw], R(f - e), R(e - c)));
+      lm[iw] = slope<R>(s.lim[iw], a, b, c);
+      l0[iw] = slope<R>(s.lim[iw], b, c, e);
+      lp[iw] = slope<R>(s.lim[iw], c, e, f);
     }
-    double F[2][NW];
-#pragma unroll
-    for (int side = 0; side < 2; ++side) {
-      // face J+1/2 with J = i-1 (side 0) or J = i (side 1)
-      const long long jl = cl.idx + (side - 1) * sd;
-      double wlf[NW], wrf[NW];
-#pragma unroll
-      for (int iw = 0; iw < NW; ++iw) {
-        const double* p = w1 + iw * g.ps + jl;
-        wrf[iw] = p[sd] - 0.5 * (side ? lp[iw] : l0[iw]);
-        wlf[iw] = p[0] + 0.5 * (side ? l0[iw] : lm[iw]);
-      }
-      double mean[NW];
-#pragma unroll
-      for (int iw = 0; iw < NW; ++iw) mean[iw] = 0.5 * (wlf[iw] + wrf[iw]);
-      const R cm = cmax<ND, MHD, R>(mean, d, s.gamma, smallp);
-      const int J = ci + side - 1;
-      const double cv = double(cm) * 2.0 / (g.dx[d][J] + g.dx[d][J + 1]);
-      cmx = cmx < cv ? cv : cmx;
-#pragma unroll
-      for (int iw = 0; iw < NW; ++iw) wlf[iw] = 2.0 * mean[iw] - wrf[iw];
-      double fL[NW], fR[NW];
-      fluxes<ND, MHD, R>(wlf, d, s.gamma, smallp, fL);
-      fluxes<ND, MHD, R>(wrf, d, s.gamma, smallp, fR);
-#pragma unroll
-      for (int iw = 0; iw < NW; ++iw) {
-        const double fm = 0.5 * (fL[iw] + fR[iw]);
-        const R diss = -(cm * R(0.5) * R(wrf[iw] - wlf[iw]));
-        F[side][iw] = fm + double(diss);
-      }
-    }
+    double um[NW], u0[NW], up[NW];
+    load<NW>(w1, g.ps, cl.idx - sd, um);
+    load<NW>(w1, g.ps, cl.idx, u0);
+    load<NW>(w1, g.ps, cl.idx + sd, up);
+    double F0[NW], F1[NW], cv;
+    corr_face<ND, MHD, R, D>(s, um, u0, lm, l0, g.dx[d][ci - 1] + g.dx[d][ci], smallp, F0, cv);
+    cmx = cmx < cv ? cv : cmx;
+    corr_face<ND, MHD, R, D>(s, u0, up, l0, lp, g.dx[d][ci] + g.dx[d][ci + 1], smallp, F1, cv);
+    cmx = cmx < cv ? cv : cmx;
     double out[NW];
     const double qdx = dt / g.dx[d][ci];
 #pragma unroll
     for (int iw = 0; iw < NW; ++iw)
-      out[iw] = w[iw * g.ps + cl.idx] - qdx * (F[1][iw] - F[0][iw]);
+      out[iw] = w[iw * g.ps + cl.idx] - qdx * (F1[iw] - F0[iw]);
     if constexpr (MHD) {
       if (s.sourceunsplit && s.divbfix) {
-        double u1[NW];
-        load<NW>(w1, g.ps, cl.idx, u1);
         const double dv = divb_at<ND>(g, w1, cl.idx, cl.c);
-        powell<ND>(u1, dv, dt * 1.0 / ND, out);
+        powell<ND>(u0, dv, dt * 1.0 / ND, out);
       }
     }
 #pragma unroll
@@ -327,6 +351,212 @@ __global__ __launch_bounds__(256) void k_tvdlf_corr(Geo g, Sel s, const double* 
   const double cmx = tvdlf_corr_body<ND, MHD, R, D>(g, s, w, w1, wout, sc->dt, sc->smallp,
                                                     (int)blockIdx.x, (int)blockIdx.y,
                                                     (int)blockIdx.z);
+  const double bm = block_reduce<true>(cmx, sm);
+  if (threadIdx.x == 0 && threadIdx.y == 0) part[SLOT_C0 * pstride + block_linear()] = bm;
+}
+
+// ------------------------------------------------------------ fused TVDLF (2D)
+// One launch per sweep: predictor and corrector together, without the w1
+// array.  A wave covers 64 consecutive x cells (cell 1 + 62*strip + lane;
+// lanes 1..62 produce output) and marches along y through a segment of rows,
+// keeping the predictor states of rows j-1, j, j+1 in registers.  Each face
+// is evaluated once: along x by the lane left of it (the neighbour's values
+// arrive by lane shuffles), along y once per marching step.
+//
+// The predictor state of a ghost cell is the predictor state of the mesh cell
+// getboundary copies into it (src/vacgrid.t:234-273), computed there
+// directly; this needs one boundary type per face for all variables, which
+// the host checks before choosing these kernels.
+constexpr int FS_STRIDE = 62;
+
+__device__ __forceinline__ int ghost_image(int c, int n, bool per_lo, bool per_hi) {
+  if (c < 2) return per_lo ? c + n : 2;
+  if (c > n + 1) return per_hi ? c - n : n + 1;
+  return c;
+}
+
+template <class T>
+__device__ __forceinline__ T lane_next(T v) { return __shfl_down(v, 1, 64); }
+template <class T>
+__device__ __forceinline__ T lane_prev(T v) { return __shfl_up(v, 1, 64); }
+
+template <bool MHD, class R, int D>
+__device__ __forceinline__ void pred_at(const Geo& g, const Sel& s, const double* __restrict__ w,
+                                        int px, int py, double dt, double smallp, double* out) {
+  const int c[3] = {px, py, 0};
+  pred_point<2, MHD, R, D>(g, s, w, g.off + px + (long long)py * g.s[1], c, dt, smallp, out);
+}
+
+// x sweep
+template <bool MHD, class R>
+__device__ __forceinline__ double tvdlf_fx_body(const Geo& g, const Sel& s, const BC& bc,
+                                                const double* __restrict__ w,
+                                                double* __restrict__ wout, double dt,
+                                                double smallp, int strip, int seg, int seglen) {
+  constexpr int ND = 2;
+  using L = Layout<ND, MHD>;
+  constexpr int NW = L::nw;
+  const int lane = (int)threadIdx.x;
+  const int n0 = g.n[0], n1 = g.n[1];
+  int c = 1 + strip * FS_STRIDE + lane;
+  const bool useful = lane >= 1 && lane <= FS_STRIDE && c <= n0 + 1;
+  const bool face_ok = lane <= FS_STRIDE && c <= n0 + 1;  // face c+1/2 with c in [1, n0+1]
+  c = c < n0 + 2 ? c : n0 + 2;                              // idle lanes: in-bounds, unused
+  const int px = ghost_image(c, n0, bc.t[0][0], bc.t[1][0]);
+  const bool plo = bc.t[2][0], phi = bc.t[3][0];
+  const int s0 = 2 + seg * seglen;
+  double cmx = -INFINITY;
+  if (s0 > n1 + 1) return cmx;  // wave-uniform
+  const int s1 = (s0 + seglen - 1 < n1 + 1) ? s0 + seglen - 1 : n1 + 1;
+  const double* dx0 = g.dx[0];
+  const double* dx1 = g.dx[1];
+  double Pc[NW], Pn[NW];
+  double byp = 0.0;  // B_y of the predictor state one row below
+  pred_at<MHD, R, 0>(g, s, w, px, ghost_image(s0 - 1, n1, plo, phi), dt, smallp, Pn);
+  if constexpr (MHD) byp = Pn[L::b(1)];
+  pred_at<MHD, R, 0>(g, s, w, px, s0, dt, smallp, Pc);
+  const double dxs = dx0[c] + dx0[c + 1];
+  const double qdx = dt / dx0[c];
+  for (int j = s0; j <= s1; ++j) {
+    pred_at<MHD, R, 0>(g, s, w, px, ghost_image(j + 1, n1, plo, phi), dt, smallp, Pn);
+    const long long idx = g.off + c + (long long)j * g.s[1];
+    double lw[NW], u[NW];
+#pragma unroll
+    for (int iw = 0; iw < NW; ++iw) {
+      const double* p = w + iw * g.ps + idx;
+      u[iw] = p[0];
+      lw[iw] = slope<R>(s.lim[iw], p[-1], u[iw], p[1]);
+    }
+    double pr[NW], lr[NW];
+#pragma unroll
+    for (int iw = 0; iw < NW; ++iw) {
+      pr[iw] = lane_next(Pc[iw]);
+      lr[iw] = lane_next(lw[iw]);
+    }
+    double F[NW], cv;
+    corr_face<ND, MHD, R, 0>(s, Pc, pr, lw, lr, dxs, smallp, F, cv);
+    if (face_ok) cmx = cmx < cv ? cv : cmx;
+    double out[NW];
+#pragma unroll
+    for (int iw = 0; iw < NW; ++iw) {
+      const double Fm = lane_prev(F[iw]);
+      out[iw] = u[iw] - qdx * (F[iw] - Fm);
+    }
+    if constexpr (MHD) {
+      const double bxp = lane_next(Pc[L::b(0)]), bxm = lane_prev(Pc[L::b(0)]);
+      if (s.sourceunsplit && s.divbfix) {
+        double dv = 0.0;
+        const double g0 = 0.5 * (bxp - bxm) / dx0[c];
+        dv = dv + g0;
+        const double g1 = 0.5 * (Pn[L::b(1)] - byp) / dx1[j];
+        dv = dv + g1;
+        powell<ND>(Pc, dv, dt * 1.0 / ND, out);
+      }
+      byp = Pc[L::b(1)];
+    }
+    if (useful) {
+#pragma unroll
+      for (int iw = 0; iw < NW; ++iw) wout[iw * g.ps + idx] = out[iw];
+    }
+#pragma unroll
+    for (int iw = 0; iw < NW; ++iw) Pc[iw] = Pn[iw];
+  }
+  return cmx;
+}
+
+// y sweep
+template <bool MHD, class R>
+__device__ __forceinline__ double tvdlf_fy_body(const Geo& g, const Sel& s, const BC& bc,
+                                                const double* __restrict__ w,
+                                                double* __restrict__ wout, double dt,
+                                                double smallp, int strip, int seg, int seglen) {
+  constexpr int ND = 2;
+  using L = Layout<ND, MHD>;
+  constexpr int NW = L::nw;
+  const int lane = (int)threadIdx.x;
+  const int n0 = g.n[0], n1 = g.n[1];
+  int c = 1 + strip * FS_STRIDE + lane;
+  const bool useful = lane >= 1 && lane <= FS_STRIDE && c <= n0 + 1;
+  c = c < n0 + 2 ? c : n0 + 2;
+  const int px = ghost_image(c, n0, bc.t[0][0], bc.t[1][0]);
+  const bool plo = bc.t[2][0], phi = bc.t[3][0];
+  const int s0 = 2 + seg * seglen;
+  double cmx = -INFINITY;
+  if (s0 > n1 + 1) return cmx;  // wave-uniform
+  const int s1 = (s0 + seglen - 1 < n1 + 1) ? s0 + seglen - 1 : n1 + 1;
+  const double* dx0 = g.dx[0];
+  const double* dx1 = g.dx[1];
+  const long long sy = g.s[1];
+  const long long col = g.off + c;
+  auto slopes_at = [&](int r, double* l) {
+#pragma unroll
+    for (int iw = 0; iw < NW; ++iw) {
+      const double* p = w + iw * g.ps + col + (long long)r * sy;
+      l[iw] = slope<R>(s.lim[iw], p[-sy], p[0], p[sy]);
+    }
+  };
+  double Pc[NW], Pn[NW], lc[NW], ln[NW], Fp[NW], Fn[NW], cv;
+  double byp = 0.0;
+  // face s0-1/2
+  pred_at<MHD, R, 1>(g, s, w, px, ghost_image(s0 - 1, n1, plo, phi), dt, smallp, Pn);
+  slopes_at(s0 - 1, ln);
+  pred_at<MHD, R, 1>(g, s, w, px, s0, dt, smallp, Pc);
+  slopes_at(s0, lc);
+  corr_face<ND, MHD, R, 1>(s, Pn, Pc, ln, lc, dx1[s0 - 1] + dx1[s0], smallp, Fp, cv);
+  if (useful) cmx = cmx < cv ? cv : cmx;
+  if constexpr (MHD) byp = Pn[L::b(1)];
+  for (int j = s0; j <= s1; ++j) {
+    pred_at<MHD, R, 1>(g, s, w, px, ghost_image(j + 1, n1, plo, phi), dt, smallp, Pn);
+    slopes_at(j + 1, ln);
+    corr_face<ND, MHD, R, 1>(s, Pc, Pn, lc, ln, dx1[j] + dx1[j + 1], smallp, Fn, cv);
+    if (useful) cmx = cmx < cv ? cv : cmx;
+    const long long idx = col + (long long)j * sy;
+    const double qdx = dt / dx1[j];
+    double out[NW];
+#pragma unroll
+    for (int iw = 0; iw < NW; ++iw) out[iw] = w[iw * g.ps + idx] - qdx * (Fn[iw] - Fp[iw]);
+    if constexpr (MHD) {
+      const double bxp = lane_next(Pc[L::b(0)]), bxm = lane_prev(Pc[L::b(0)]);
+      if (s.sourceunsplit && s.divbfix) {
+        double dv = 0.0;
+        const double g0 = 0.5 * (bxp - bxm) / dx0[c];
+        dv = dv + g0;
+        const double g1 = 0.5 * (Pn[L::b(1)] - byp) / dx1[j];
+        dv = dv + g1;
+        powell<ND>(Pc, dv, dt * 1.0 / ND, out);
+      }
+      byp = Pc[L::b(1)];
+    }
+    if (useful) {
+#pragma unroll
+      for (int iw = 0; iw < NW; ++iw) wout[iw * g.ps + idx] = out[iw];
+    }
+#pragma unroll
+    for (int iw = 0; iw < NW; ++iw) {
+      Pc[iw] = Pn[iw];
+      lc[iw] = ln[iw];
+      Fp[iw] = Fn[iw];
+    }
+  }
+  return cmx;
+}
+
+// blockDim = (64, 4): wave threadIdx.y takes segment blockIdx.y*4 + y of
+// strip blockIdx.x
+template <bool MHD, class R, int D>
+__global__ __launch_bounds__(256) void k_tvdlf_fused(Geo g, Sel s, BC bc,
+                                                    const double* __restrict__ w,
+                                                    double* __restrict__ wout,
+                                                    const Scal* __restrict__ sc,
+                                                    double* __restrict__ part, long long pstride,
+                                                    int seglen) {
+  __shared__ double sm[8];
+  const int seg = (int)(blockIdx.y * blockDim.y + threadIdx.y);
+  double cmx;
+  if constexpr (D == 0)
+    cmx = tvdlf_fx_body<MHD, R>(g, s, bc, w, wout, sc->dt, sc->smallp, (int)blockIdx.x, seg, seglen);
+  else
+    cmx = tvdlf_fy_body<MHD, R>(g, s, bc, w, wout, sc->dt, sc->smallp, (int)blockIdx.x, seg, seglen);
   const double bm = block_reduce<true>(cmx, sm);
   if (threadIdx.x == 0 && threadIdx.y == 0) part[SLOT_C0 * pstride + block_linear()] = bm;
 }

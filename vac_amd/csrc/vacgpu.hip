@@ -49,6 +49,13 @@ struct vacgpu_grid {
   dim3 grid_tx;              // k_tvd_x
   dim3 grid_tm[3];           // k_tvd_m per sweep direction
   int seglen = 16;
+  // fused 2D TVDLF sweeps: eligible when every face has one boundary type
+  // for all variables; use_fused selects them (default on)
+  bool fused_ok = false, use_fused = true;
+  int fs_seglen = 16;
+  dim3 grid_fs, block_fs;
+  double* stage = nullptr;  // pinned staging of put_state / get_state
+  size_t stage_bytes = 0;
   long long nbound = 0, slab[3] = {0, 0, 0};
   int cur = 0, old = 0;
   bool have_state = false;
@@ -59,8 +66,8 @@ struct vacgpu_grid {
   bool prof = false;
   struct Ev { int k; hipEvent_t a, b; };
   std::vector<Ev> evs;
-  double tsum[KF_N] = {0, 0, 0, 0, 0, 0};
-  long tcnt[KF_N] = {0, 0, 0, 0, 0, 0};
+  double tsum[KF_N] = {};
+  long tcnt[KF_N] = {};
 };
 
 namespace {
@@ -230,6 +237,22 @@ int getdt_device(vacgpu_grid* g, const vacgpu_method* m, double dtpar, int tmaxe
 }
 
 // one dimensional sweep (advect, src/vac.t:363-486), dt already on device
+// launch shape of the fused TVDLF sweeps: strips of 62 x cells, waves of
+// one segment of seglen rows each, four waves per block.  seglen 0 picks a
+// length that gives about 4096 waves.
+void set_fused_shape(vacgpu_grid* g, int seglen) {
+  const int strips = (g->geo.n[0] + FS_STRIDE - 1) / FS_STRIDE;
+  if (seglen <= 0) {
+    const int want = (4096 + strips - 1) / strips;
+    seglen = (g->geo.n[1] + want - 1) / want;
+  }
+  seglen = seglen < 4 ? 4 : (seglen > 256 ? 256 : seglen);
+  g->fs_seglen = seglen;
+  const int segs = (g->geo.n[1] + seglen - 1) / seglen;
+  g->block_fs = dim3(64, 4, 1);
+  g->grid_fs = dim3(strips, (segs + 3) / 4, 1);
+}
+
 int sweep_device(vacgpu_grid* g, const vacgpu_method* m, int d, int firstsweep, int lastsweep) {
   if (firstsweep) g->old = g->cur;
   const int outi = other_buffer(g);
@@ -242,7 +265,29 @@ int sweep_device(vacgpu_grid* g, const vacgpu_method* m, int d, int firstsweep, 
   const bool src = mhd && m->divbfix && m->sourceunsplit;
   const bool courant = m->courantpar > 0.0;
   int rc = 0;
-  if (m->typefull == VACGPU_TVDLF) {
+  if (m->typefull == VACGPU_TVDLF && g->cfg.ndim == 2 && g->fused_ok && g->use_fused) {
+    const long long nbf = (long long)g->grid_fs.x * g->grid_fs.y;
+    {
+      Timed t(g, KF_FUSED);
+      const bool f32 = g->cfg.precision == VACGPU_FP32BULK;
+#define FUSED(MV, RT, DV) hipLaunchKernelGGL((k_tvdlf_fused<MV, RT, DV>), g->grid_fs, g->block_fs, 0, g->stream, g->geo, s, g->bc, win, wout, g->sc, g->part, g->pstride, g->fs_seglen)
+      if (mhd) {
+        if (f32) { if (d == 0) FUSED(true, float, 0); else FUSED(true, float, 1); }
+        else { if (d == 0) FUSED(true, double, 0); else FUSED(true, double, 1); }
+      } else {
+        if (f32) { if (d == 0) FUSED(false, float, 0); else FUSED(false, float, 1); }
+        else { if (d == 0) FUSED(false, double, 0); else FUSED(false, double, 1); }
+      }
+#undef FUSED
+      if ((rc = launch_check(g, "k_tvdlf_fused"))) return rc;
+    }
+    if (courant) {
+      Timed t(g, KF_RED);
+      hipLaunchKernelGGL(k_fin_courant_tvdlf, dim3(1), dim3(1024), 0, g->stream, g->sc, g->part,
+                         g->pstride, nbf, d, m->courantpar);
+      if ((rc = launch_check(g, "k_fin_courant_tvdlf"))) return rc;
+    }
+  } else if (m->typefull == VACGPU_TVDLF) {
     {
       Timed t(g, KF_PRED);
       DISPATCH(g, {
@@ -430,8 +475,21 @@ int vacgpu_create(const vacgpu_config* cfg, const double* dx0, const double* dx1
     const int other = (d == 1) ? 2 : 1;
     g->grid_tm[d] = dim3(g->grid.x, (nseg + BY - 1) / BY, nd == 3 ? geo.n[other] : 1);
   }
+  // fused TVDLF: one boundary type per face for all variables
+  g->fused_ok = nd == 2;
+  for (int f = 0; f < 2 * nd; ++f)
+    for (int iw = 1; iw < g->nw; ++iw)
+      if (g->bc.t[f][iw] != g->bc.t[f][0]) g->fused_ok = false;
+  set_fused_shape(g, 0);
   auto count = [](dim3 v) { return (long long)v.x * v.y * v.z; };
   g->pstride = count(g->grid);
+  {
+    // partials of the fused sweeps at the shortest segment length
+    const long long strips = (geo.n[0] + FS_STRIDE - 1) / FS_STRIDE;
+    const long long segs = (geo.n[1] + 3) / 4;
+    const long long nbmax = strips * ((segs + 3) / 4);
+    if (nbmax > g->pstride) g->pstride = nbmax;
+  }
   if (count(g->grid_tx) > g->pstride) g->pstride = count(g->grid_tx);
   for (int d = 1; d < nd; ++d)
     if (count(g->grid_tm[d]) > g->pstride) g->pstride = count(g->grid_tm[d]);
@@ -505,9 +563,23 @@ static int copy_state(vacgpu_grid* g, double* host, const int* ld, bool put) {
   const size_t width = sizeof(double) * geo.ng[0];
   const int nz = nd == 3 ? geo.ng[2] : 1;
   double* dev = g->buf[g->cur];
+  // through a pinned staging copy of the caller's array: one host copy, then
+  // pitched DMA per plane
+  const size_t plane = (size_t)l[0] * l[1];
+  const size_t hbytes = sizeof(double) * plane * (nd == 3 ? l[2] : 1) * g->nw;
+  HIPCHK(g, hipStreamSynchronize(g->stream));
+  if (hbytes > g->stage_bytes) {
+    if (g->stage) HIPCHK(g, hipHostFree(g->stage));
+    g->stage = nullptr;
+    g->stage_bytes = 0;
+    HIPCHK(g, hipHostMalloc(&g->stage, hbytes, hipHostMallocDefault));
+    g->stage_bytes = hbytes;
+  }
+  if (put) memcpy(g->stage, host, hbytes);
   for (int iw = 0; iw < g->nw; ++iw)
     for (int z = 0; z < nz; ++z) {
-      double* h = host + (size_t)iw * l[0] * l[1] * (nd == 3 ? l[2] : 1) + (size_t)z * l[0] * l[1];
+      const size_t ho = (size_t)iw * plane * (nd == 3 ? l[2] : 1) + (size_t)z * plane;
+      double* h = g->stage + ho;
       double* dv = dev + iw * geo.ps + geo.off + z * geo.s[2];
       if (put)
         HIPCHK(g, hipMemcpy2DAsync(dv, dpitch, h, hpitch, width, geo.ng[1], hipMemcpyHostToDevice, g->stream));
@@ -515,6 +587,13 @@ static int copy_state(vacgpu_grid* g, double* host, const int* ld, bool put) {
         HIPCHK(g, hipMemcpy2DAsync(h, hpitch, dv, dpitch, width, geo.ng[1], hipMemcpyDeviceToHost, g->stream));
     }
   HIPCHK(g, hipStreamSynchronize(g->stream));
+  if (!put)  // only the ixG part of each row: the caller's padding stays as it was
+    for (int iw = 0; iw < g->nw; ++iw)
+      for (int z = 0; z < nz; ++z)
+        for (int y = 0; y < geo.ng[1]; ++y) {
+          const size_t ho = (size_t)iw * plane * (nd == 3 ? l[2] : 1) + (size_t)z * plane + (size_t)y * l[0];
+          memcpy(host + ho, g->stage + ho, width);
+        }
   return 0;
 }
 
@@ -679,6 +758,14 @@ int vacgpu_profile(vacgpu_grid* g, int enable) {
   return 0;
 }
 
+int vacgpu_set_kernels(vacgpu_grid* g, int fused, int seglen) {
+  if (!g) return VACGPU_E_ARG;
+  HIPCHK(g, hipStreamSynchronize(g->stream));
+  g->use_fused = fused != 0;
+  set_fused_shape(g, seglen);
+  return 0;
+}
+
 int vacgpu_kernel_time(vacgpu_grid* g, int k, double* avg_ms, long* count) {
   if (!g || k < 0 || k >= KF_N) return VACGPU_E_ARG;
   HIPCHK(g, hipStreamSynchronize(g->stream));
@@ -703,6 +790,7 @@ void vacgpu_destroy(vacgpu_grid* g) {
   if (g->dxdev) (void)hipFree(g->dxdev);
   if (g->sc) (void)hipFree(g->sc);
   if (g->sc_host) (void)hipHostFree(g->sc_host);
+  if (g->stage) (void)hipHostFree(g->stage);
   if (g->part) (void)hipFree(g->part);
   if (g->own_stream && g->stream) (void)hipStreamDestroy(g->stream);
   delete g;

@@ -123,12 +123,15 @@ def serve(requests, budget=32 << 30, capacity=1024, steps_per_poll=2, keep_state
     cell_updates = 0
     peak_active = 0
     nxt = 0
+    t_admit = t_step = t_poll = 0.0
+    polls = 0
     start = time.perf_counter()
     while nxt < len(requests) or queue or active:
         now = time.perf_counter() - start
         while nxt < len(requests) and requests[nxt].arrival <= now:
             queue.append(requests[nxt])
             nxt += 1
+        ta = time.perf_counter()
         while queue:
             r = queue[0]
             try:
@@ -143,9 +146,14 @@ def serve(requests, budget=32 << 30, capacity=1024, steps_per_poll=2, keep_state
             if nxt < len(requests):
                 time.sleep(max(0.0, min(0.001, requests[nxt].arrival - now)))
             continue
+        ts = time.perf_counter()
+        t_admit += ts - ta
         b.step(steps_per_poll)
         b.sync()
-        done_at = time.perf_counter() - start
+        tp = time.perf_counter()
+        t_step += tp - ts
+        polls += 1
+        done_at = tp - start
         for slot in list(active):
             st = b.status(slot)
             if st["done"]:
@@ -155,6 +163,7 @@ def serve(requests, budget=32 << 30, capacity=1024, steps_per_poll=2, keep_state
                     results[id(r)] = (b.get_mesh(slot), st)
                 b.remove(slot)
                 latencies.append(done_at - r.arrival)
+        t_poll += time.perf_counter() - tp
     total = time.perf_counter() - start
     b.close()
     lat = np.array(latencies)
@@ -163,7 +172,9 @@ def serve(requests, budget=32 << 30, capacity=1024, steps_per_poll=2, keep_state
            "p99_ms": float(np.percentile(lat, 99) * 1e3),
            "mean_ms": float(lat.mean() * 1e3),
            "mcell_updates_per_s": cell_updates / total / 1e6,
-           "peak_resident": peak_active}
+           "peak_resident": peak_active, "polls": polls,
+           "host_s": {"admit": round(t_admit, 4), "step_and_sync": round(t_step, 4),
+                      "poll_and_remove": round(t_poll, 4)}}
     if keep_states:
         out["results"] = results
     return out

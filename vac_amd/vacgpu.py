@@ -24,7 +24,8 @@ TVDLF, TVD = 0, 1
 LIMITERS = {"minmod": 0, "woodward": 1, "superbee": 2}
 CONT, PERIODIC = 0, 1
 FP64, FP32BULK = 0, 1
-KERNELS = {"tvd_sweep": 0, "tvdlf_pred": 1, "tvdlf_corr": 2, "ryu": 3, "bound": 4, "reduce": 5}
+KERNELS = {"tvd_sweep": 0, "tvdlf_pred": 1, "tvdlf_corr": 2, "ryu": 3, "bound": 4, "reduce": 5,
+           "tvdlf_fused": 6}
 
 
 class Config(C.Structure):
@@ -59,6 +60,7 @@ SIGNATURES = {
     "vacgpu_get_scalars": ([C.c_void_p, _DP, _IP, _DP, _DP], C.c_int),
     "vacgpu_profile": ([C.c_void_p, C.c_int], C.c_int),
     "vacgpu_kernel_time": ([C.c_void_p, C.c_int, _DP, C.POINTER(C.c_long)], C.c_int),
+    "vacgpu_set_kernels": ([C.c_void_p, C.c_int, C.c_int], C.c_int),
     "vacgpu_supported": ([C.POINTER(Config), C.POINTER(Method)], C.c_int),
     "vacgpu_last_error": ([C.c_void_p], C.c_char_p),
     "vacgpu_destroy": ([C.c_void_p], None),
@@ -144,10 +146,14 @@ def make_config(nx, physics: str, gamma: float, bc="periodic", precision: str = 
         cfg.nx[d] = int(nx[d])
     cfg.eqpar[0] = gamma
     cfg.eqpar[1] = 0.0
+    # bc: one type for all faces, or one entry per face (x-low, x-high,
+    # y-low, ...); an entry is a type or a list of types per variable
     bcs = [bc] * (2 * ndim) if isinstance(bc, str) else list(bc)
     for f in range(2 * ndim):
+        spec = bcs[f]
         for iw in range(8):
-            cfg.typeB[f][iw] = PERIODIC if bcs[f] == "periodic" else CONT
+            name = spec if isinstance(spec, str) else spec[min(iw, len(spec) - 1)]
+            cfg.typeB[f][iw] = PERIODIC if name == "periodic" else CONT
     cfg.precision = FP32BULK if precision == "fp32bulk" else FP64
     return cfg
 
@@ -162,18 +168,7 @@ class Grid:
         self.ndim = len(self.nx)
         self.physics = physics
         self.nw = nvars(physics, self.ndim)
-        cfg = Config()
-        cfg.ndim = self.ndim
-        cfg.physics = MHD if physics == "mhd" else HD
-        for d in range(self.ndim):
-            cfg.nx[d] = self.nx[d]
-        cfg.eqpar[0] = gamma
-        cfg.eqpar[1] = 0.0
-        bcs = [bc] * (2 * self.ndim) if isinstance(bc, str) else list(bc)
-        for f in range(2 * self.ndim):
-            for iw in range(8):
-                cfg.typeB[f][iw] = PERIODIC if bcs[f] == "periodic" else CONT
-        cfg.precision = FP32BULK if precision == "fp32bulk" else FP64
+        cfg = make_config(self.nx, physics, gamma, bc=bc, precision=precision)
         self.cfg = cfg
         self._dx = [np.ascontiguousarray(d, dtype=np.float64) for d in dxs]
         for d in range(self.ndim):
@@ -276,6 +271,11 @@ class Grid:
 
     def profile(self, enable: bool):
         self._check(self.lib.vacgpu_profile(self.h, int(enable)), "profile")
+
+    def set_kernels(self, fused: bool = True, seglen: int = 0):
+        """2D TVDLF sweeps as one fused launch (default) or predictor +
+        corrector launches; seglen = rows per marching segment (0 = auto)."""
+        self._check(self.lib.vacgpu_set_kernels(self.h, int(fused), int(seglen)), "set_kernels")
 
     def kernel_time(self, name: str):
         ms = C.c_double()
