@@ -83,6 +83,10 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32bulk"])
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--probe", action="store_true",
+                    help="run only a short workload for hardware-counter passes: a 1 GiB "
+                         "device copy (calibration), then 3 steps each of the 2048^2 TVD "
+                         "and TVDLF workloads; prints no JSON")
     ap.add_argument("--no-tvdlf", action="store_true")
     ap.add_argument("--no-3d", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
@@ -98,6 +102,9 @@ def main():
     from vac_amd.initial import random_fourier
 
     w0, coords = random_fourier((NX, NY), SEED, GAMMA)
+    if args.probe:
+        probe(w0, [widths_1d(c) for c in coords], args.precision)
+        return
     dxs = [widths_1d(c) for c in coords]
     method = vacgpu.MethodSpec(typefull="tvd", limiter="woodward", courantpar=0.8)
     grid = vacgpu.Grid([NX, NY], "mhd", GAMMA, dxs, precision=args.precision)
@@ -167,6 +174,28 @@ def main():
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(w0, coords)
     print(json.dumps(out))
+
+
+def probe(w0, dxs, precision):
+    """Short run for counter passes (rocprofv3 --pmc): the copy kernel moves
+    exactly 1 GiB in and 1 GiB out, which calibrates FETCH_SIZE and
+    WRITE_SIZE; then the two 2048^2 workloads for 3 steps each."""
+    import torch
+
+    from vac_amd import vacgpu
+
+    a = torch.empty(1 << 27, dtype=torch.float64, device="cuda")
+    b = torch.empty_like(a)
+    a.fill_(1.0)
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    for typefull, lim in (("tvd", "woodward"), ("tvdlf", "minmod")):
+        g = vacgpu.Grid([NX, NY], "mhd", GAMMA, dxs, precision=precision)
+        g.put_mesh(w0)
+        g.run(vacgpu.MethodSpec(typefull=typefull, limiter=lim, courantpar=0.8), 3)
+        g.close()
+    print("probe done", flush=True)
 
 
 def errors(a, ref):

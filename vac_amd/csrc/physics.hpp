@@ -34,15 +34,22 @@ __device__ __forceinline__ R rmin(R a, R b) { return b < a ? b : a; }
 template <class R>
 __device__ __forceinline__ R fsign(R a, R b) { return copysign(fabs(a), b); }
 
-// dwlimiter2 (src/vacgrid.t:938-996): a = dwC(i), b = dwC(i-1)
-template <class R>
-__device__ __forceinline__ R limiter(int type, R a, R b) {
+// dwlimiter2 (src/vacgrid.t:938-996): a = dwC(i), b = dwC(i-1).  LIM >= 0
+// fixes the type at compile time (no branches); LIM < 0 takes it from type.
+template <int LIM, class R>
+__device__ __forceinline__ R limiter_t(int type, R a, R b) {
+  const int t = LIM >= 0 ? LIM : type;
   const R one = R(1), zero = R(0);
   R s = copysign(one, a);
-  if (type == LIM_MINMOD) return s * rmax(zero, rmin(R(fabs(a)), s * b));
-  if (type == LIM_WOODWARD)
+  if (t == LIM_MINMOD) return s * rmax(zero, rmin(R(fabs(a)), s * b));
+  if (t == LIM_WOODWARD)
     return R(2) * s * rmax(zero, rmin(rmin(R(fabs(a)), s * b), s * R(0.25) * (b + a)));
   return s * rmax(rmax(zero, rmin(R(2) * R(fabs(a)), s * b)), rmin(R(fabs(a)), R(2) * s * b));
+}
+
+template <class R>
+__device__ __forceinline__ R limiter(int type, R a, R b) {
+  return limiter_t<-1, R>(type, a, b);
 }
 
 // Unclipped thermal pressure (MHD src/vacphys0.t.mhd:191-199,

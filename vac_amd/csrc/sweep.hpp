@@ -193,7 +193,9 @@ __global__ __launch_bounds__(256) void k_bound(Geo g, BC bc, double* __restrict_
 // Hancock predictor (src/vactvdlf.t:5-89) at one cell with qdt = dt/2:
 // w1 = w - qdt/dx*(F(w+l/2) - F(w-l/2)) - (qdt/ndim)*S_Powell(w).
 // idx / c address the cell; reads w at the cell and its neighbours.
-template <int ND, bool MHD, class R, int D>
+// LIM: limiter type fixed at compile time (>= 0) or per variable from s
+// (< 0); PWM: Powell source on (1), off (0) or as s selects (2).
+template <int ND, bool MHD, class R, int D, int LIM = -1, int PWM = 2>
 __device__ __forceinline__ void pred_point(const Geo& g, const Sel& s, const double* __restrict__ w,
                                            long long idx, const int* c, double dt, double smallp,
                                            double* out) {
@@ -207,7 +209,7 @@ __device__ __forceinline__ void pred_point(const Geo& g, const Sel& s, const dou
 #pragma unroll
   for (int iw = 0; iw < NW; ++iw) {
     const double um = w[iw * g.ps + idx - sd], up = w[iw * g.ps + idx + sd];
-    const R l = limiter<R>(s.lim[iw], R(up - uc[iw]), R(uc[iw] - um));
+    const R l = limiter_t<LIM, R>(s.lim[iw], R(up - uc[iw]), R(uc[iw] - um));
     wr[iw] = uc[iw] - 0.5 * double(l);
     wl[iw] = uc[iw] + 0.5 * double(l);
   }
@@ -217,8 +219,8 @@ __device__ __forceinline__ void pred_point(const Geo& g, const Sel& s, const dou
   const double qdx = qdt / g.dx[d][c[d]];
 #pragma unroll
   for (int iw = 0; iw < NW; ++iw) out[iw] = uc[iw] - qdx * (fL[iw] - fR[iw]);
-  if constexpr (MHD) {
-    if (s.sourceunsplit && s.divbfix) {
+  if constexpr (MHD && PWM != 0) {
+    if (PWM == 1 || (s.sourceunsplit && s.divbfix)) {
       const double dv = divb_at<ND>(g, w, idx, c);
       powell<ND>(uc, dv, qdt * 1.0 / ND, out);
     }
@@ -283,9 +285,9 @@ __device__ __forceinline__ void corr_face(const Sel& s, const double* pl, const 
 }
 
 // limited slope of w at a cell along d from w(i-1), w(i), w(i+1)
-template <class R>
+template <class R, int LIM = -1>
 __device__ __forceinline__ double slope(int lim, double a, double b, double e) {
-  return double(limiter<R>(lim, R(e - b), R(b - a)));
+  return double(limiter_t<LIM, R>(lim, R(e - b), R(b - a)));
 }
 
 // TVDLF corrector (tvdmusclf, src/vactvdlf.t:92-299) for the cell i from its
@@ -370,9 +372,8 @@ __global__ __launch_bounds__(256) void k_tvdlf_corr(Geo g, Sel s, const double* 
 constexpr int FS_STRIDE = 62;
 
 __device__ __forceinline__ int ghost_image(int c, int n, bool per_lo, bool per_hi) {
-  if (c < 2) return per_lo ? c + n : 2;
-  if (c > n + 1) return per_hi ? c - n : n + 1;
-  return c;
+  const int lo = per_lo ? c + n : 2, hi = per_hi ? c - n : n + 1;
+  return c < 2 ? lo : (c > n + 1 ? hi : c);
 }
 
 template <class T>
@@ -380,15 +381,18 @@ __device__ __forceinline__ T lane_next(T v) { return __shfl_down(v, 1, 64); }
 template <class T>
 __device__ __forceinline__ T lane_prev(T v) { return __shfl_up(v, 1, 64); }
 
-template <bool MHD, class R, int D>
+// fused kernels run with the Powell source on for MHD (divbfix and
+// sourceunsplit, VAC's defaults) and one limiter type LIM for all variables
+template <bool MHD, class R, int D, int LIM>
 __device__ __forceinline__ void pred_at(const Geo& g, const Sel& s, const double* __restrict__ w,
                                         int px, int py, double dt, double smallp, double* out) {
   const int c[3] = {px, py, 0};
-  pred_point<2, MHD, R, D>(g, s, w, g.off + px + (long long)py * g.s[1], c, dt, smallp, out);
+  pred_point<2, MHD, R, D, LIM, 1>(g, s, w, g.off + px + (long long)py * g.s[1], c, dt, smallp,
+                                   out);
 }
 
 // x sweep
-template <bool MHD, class R>
+template <bool MHD, class R, int LIM>
 __device__ __forceinline__ double tvdlf_fx_body(const Geo& g, const Sel& s, const BC& bc,
                                                 const double* __restrict__ w,
                                                 double* __restrict__ wout, double dt,
@@ -412,20 +416,20 @@ __device__ __forceinline__ double tvdlf_fx_body(const Geo& g, const Sel& s, cons
   const double* dx1 = g.dx[1];
   double Pc[NW], Pn[NW];
   double byp = 0.0;  // B_y of the predictor state one row below
-  pred_at<MHD, R, 0>(g, s, w, px, ghost_image(s0 - 1, n1, plo, phi), dt, smallp, Pn);
+  pred_at<MHD, R, 0, LIM>(g, s, w, px, ghost_image(s0 - 1, n1, plo, phi), dt, smallp, Pn);
   if constexpr (MHD) byp = Pn[L::b(1)];
-  pred_at<MHD, R, 0>(g, s, w, px, s0, dt, smallp, Pc);
+  pred_at<MHD, R, 0, LIM>(g, s, w, px, s0, dt, smallp, Pc);
   const double dxs = dx0[c] + dx0[c + 1];
   const double qdx = dt / dx0[c];
   for (int j = s0; j <= s1; ++j) {
-    pred_at<MHD, R, 0>(g, s, w, px, ghost_image(j + 1, n1, plo, phi), dt, smallp, Pn);
+    pred_at<MHD, R, 0, LIM>(g, s, w, px, ghost_image(j + 1, n1, plo, phi), dt, smallp, Pn);
     const long long idx = g.off + c + (long long)j * g.s[1];
     double lw[NW], u[NW];
 #pragma unroll
     for (int iw = 0; iw < NW; ++iw) {
       const double* p = w + iw * g.ps + idx;
       u[iw] = p[0];
-      lw[iw] = slope<R>(s.lim[iw], p[-1], u[iw], p[1]);
+      lw[iw] = slope<R, LIM>(s.lim[iw], p[-1], u[iw], p[1]);
     }
     double pr[NW], lr[NW];
 #pragma unroll
@@ -444,7 +448,7 @@ __device__ __forceinline__ double tvdlf_fx_body(const Geo& g, const Sel& s, cons
     }
     if constexpr (MHD) {
       const double bxp = lane_next(Pc[L::b(0)]), bxm = lane_prev(Pc[L::b(0)]);
-      if (s.sourceunsplit && s.divbfix) {
+      {
         double dv = 0.0;
         const double g0 = 0.5 * (bxp - bxm) / dx0[c];
         dv = dv + g0;
@@ -465,7 +469,7 @@ __device__ __forceinline__ double tvdlf_fx_body(const Geo& g, const Sel& s, cons
 }
 
 // y sweep
-template <bool MHD, class R>
+template <bool MHD, class R, int LIM>
 __device__ __forceinline__ double tvdlf_fy_body(const Geo& g, const Sel& s, const BC& bc,
                                                 const double* __restrict__ w,
                                                 double* __restrict__ wout, double dt,
@@ -492,21 +496,21 @@ __device__ __forceinline__ double tvdlf_fy_body(const Geo& g, const Sel& s, cons
 #pragma unroll
     for (int iw = 0; iw < NW; ++iw) {
       const double* p = w + iw * g.ps + col + (long long)r * sy;
-      l[iw] = slope<R>(s.lim[iw], p[-sy], p[0], p[sy]);
+      l[iw] = slope<R, LIM>(s.lim[iw], p[-sy], p[0], p[sy]);
     }
   };
   double Pc[NW], Pn[NW], lc[NW], ln[NW], Fp[NW], Fn[NW], cv;
   double byp = 0.0;
   // face s0-1/2
-  pred_at<MHD, R, 1>(g, s, w, px, ghost_image(s0 - 1, n1, plo, phi), dt, smallp, Pn);
+  pred_at<MHD, R, 1, LIM>(g, s, w, px, ghost_image(s0 - 1, n1, plo, phi), dt, smallp, Pn);
   slopes_at(s0 - 1, ln);
-  pred_at<MHD, R, 1>(g, s, w, px, s0, dt, smallp, Pc);
+  pred_at<MHD, R, 1, LIM>(g, s, w, px, s0, dt, smallp, Pc);
   slopes_at(s0, lc);
   corr_face<ND, MHD, R, 1>(s, Pn, Pc, ln, lc, dx1[s0 - 1] + dx1[s0], smallp, Fp, cv);
   if (useful) cmx = cmx < cv ? cv : cmx;
   if constexpr (MHD) byp = Pn[L::b(1)];
   for (int j = s0; j <= s1; ++j) {
-    pred_at<MHD, R, 1>(g, s, w, px, ghost_image(j + 1, n1, plo, phi), dt, smallp, Pn);
+    pred_at<MHD, R, 1, LIM>(g, s, w, px, ghost_image(j + 1, n1, plo, phi), dt, smallp, Pn);
     slopes_at(j + 1, ln);
     corr_face<ND, MHD, R, 1>(s, Pc, Pn, lc, ln, dx1[j] + dx1[j + 1], smallp, Fn, cv);
     if (useful) cmx = cmx < cv ? cv : cmx;
@@ -517,7 +521,7 @@ __device__ __forceinline__ double tvdlf_fy_body(const Geo& g, const Sel& s, cons
     for (int iw = 0; iw < NW; ++iw) out[iw] = w[iw * g.ps + idx] - qdx * (Fn[iw] - Fp[iw]);
     if constexpr (MHD) {
       const double bxp = lane_next(Pc[L::b(0)]), bxm = lane_prev(Pc[L::b(0)]);
-      if (s.sourceunsplit && s.divbfix) {
+      {
         double dv = 0.0;
         const double g0 = 0.5 * (bxp - bxm) / dx0[c];
         dv = dv + g0;
@@ -543,8 +547,11 @@ __device__ __forceinline__ double tvdlf_fy_body(const Geo& g, const Sel& s, cons
 
 // blockDim = (64, 4): wave threadIdx.y takes segment blockIdx.y*4 + y of
 // strip blockIdx.x
-template <bool MHD, class R, int D>
-__global__ __launch_bounds__(256) void k_tvdlf_fused(Geo g, Sel s, BC bc,
+#ifndef VACGPU_FUSED_WAVES
+#define VACGPU_FUSED_WAVES 1
+#endif
+template <bool MHD, class R, int D, int LIM>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(VACGPU_FUSED_WAVES))) void k_tvdlf_fused(Geo g, Sel s, BC bc,
                                                     const double* __restrict__ w,
                                                     double* __restrict__ wout,
                                                     const Scal* __restrict__ sc,
@@ -554,9 +561,9 @@ __global__ __launch_bounds__(256) void k_tvdlf_fused(Geo g, Sel s, BC bc,
   const int seg = (int)(blockIdx.y * blockDim.y + threadIdx.y);
   double cmx;
   if constexpr (D == 0)
-    cmx = tvdlf_fx_body<MHD, R>(g, s, bc, w, wout, sc->dt, sc->smallp, (int)blockIdx.x, seg, seglen);
+    cmx = tvdlf_fx_body<MHD, R, LIM>(g, s, bc, w, wout, sc->dt, sc->smallp, (int)blockIdx.x, seg, seglen);
   else
-    cmx = tvdlf_fy_body<MHD, R>(g, s, bc, w, wout, sc->dt, sc->smallp, (int)blockIdx.x, seg, seglen);
+    cmx = tvdlf_fy_body<MHD, R, LIM>(g, s, bc, w, wout, sc->dt, sc->smallp, (int)blockIdx.x, seg, seglen);
   const double bm = block_reduce<true>(cmx, sm);
   if (threadIdx.x == 0 && threadIdx.y == 0) part[SLOT_C0 * pstride + block_linear()] = bm;
 }

@@ -265,12 +265,20 @@ int sweep_device(vacgpu_grid* g, const vacgpu_method* m, int d, int firstsweep, 
   const bool src = mhd && m->divbfix && m->sourceunsplit;
   const bool courant = m->courantpar > 0.0;
   int rc = 0;
-  if (m->typefull == VACGPU_TVDLF && g->cfg.ndim == 2 && g->fused_ok && g->use_fused) {
+  // fused sweeps: 2D, one boundary type per face, one limiter for all
+  // variables, and for MHD the Powell source on (VAC's defaults)
+  bool one_lim = true;
+  for (int iw = 1; iw < g->nw; ++iw) one_lim &= m->typelimiter[iw] == m->typelimiter[0];
+  const bool fused = m->typefull == VACGPU_TVDLF && g->cfg.ndim == 2 && g->fused_ok &&
+                     g->use_fused && one_lim && (!mhd || src);
+  if (fused) {
     const long long nbf = (long long)g->grid_fs.x * g->grid_fs.y;
     {
       Timed t(g, KF_FUSED);
       const bool f32 = g->cfg.precision == VACGPU_FP32BULK;
-#define FUSED(MV, RT, DV) hipLaunchKernelGGL((k_tvdlf_fused<MV, RT, DV>), g->grid_fs, g->block_fs, 0, g->stream, g->geo, s, g->bc, win, wout, g->sc, g->part, g->pstride, g->fs_seglen)
+      const int lim = m->typelimiter[0];
+#define FUSED_L(MV, RT, DV, LV) hipLaunchKernelGGL((k_tvdlf_fused<MV, RT, DV, LV>), g->grid_fs, g->block_fs, 0, g->stream, g->geo, s, g->bc, win, wout, g->sc, g->part, g->pstride, g->fs_seglen)
+#define FUSED(MV, RT, DV) do { if (lim == LIM_MINMOD) FUSED_L(MV, RT, DV, LIM_MINMOD); else if (lim == LIM_WOODWARD) FUSED_L(MV, RT, DV, LIM_WOODWARD); else FUSED_L(MV, RT, DV, LIM_SUPERBEE); } while (0)
       if (mhd) {
         if (f32) { if (d == 0) FUSED(true, float, 0); else FUSED(true, float, 1); }
         else { if (d == 0) FUSED(true, double, 0); else FUSED(true, double, 1); }
@@ -279,6 +287,7 @@ int sweep_device(vacgpu_grid* g, const vacgpu_method* m, int d, int firstsweep, 
         else { if (d == 0) FUSED(false, double, 0); else FUSED(false, double, 1); }
       }
 #undef FUSED
+#undef FUSED_L
       if ((rc = launch_check(g, "k_tvdlf_fused"))) return rc;
     }
     if (courant) {
