@@ -93,3 +93,24 @@ def test_mixed_variable_boundaries_take_two_launch_path():
     wr, tr, _ = vacref.run(p, w0, dxs, 10)
     assert np.array_equal(t, tr)
     assert np.array_equal(w, wr)
+
+
+@pytest.mark.parametrize("bc", ["periodic", "cont", ["periodic", "periodic", "cont", "cont"]],
+                         ids=["periodic", "cont", "mixed"])
+def test_fused_fills_ghost_cells(bc):
+    # the fused sweeps write the ghost cells themselves; the whole ixG array
+    # (ghosts included) must equal the two-launch path's, which fills them
+    # with the separate getboundary kernel
+    nx, ny = 70, 45
+    w0, coords = random_fourier((nx, ny), 9, 5 / 3)
+    dxs = [widths_1d(c) for c in coords]
+    out = []
+    for fused in (True, False):
+        g = vacgpu.Grid([nx, ny], "mhd", 5 / 3, dxs, bc=bc)
+        g.set_kernels(fused, 5)
+        g.put_mesh(w0)
+        g.run(METHOD, 7)
+        out.append(g.get_state())
+        g.close()
+    print(f"case fused ghosts {bc}", flush=True)
+    assert np.array_equal(out[0], out[1])

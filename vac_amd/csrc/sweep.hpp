@@ -190,41 +190,80 @@ __global__ __launch_bounds__(256) void k_bound(Geo g, BC bc, double* __restrict_
 }
 
 // ------------------------------------------------------------ TVDLF
-// Hancock predictor (src/vactvdlf.t:5-89) at one cell with qdt = dt/2:
+// Hancock predictor (src/vactvdlf.t:5-89) with qdt = dt/2 at one cell, from
+// its gathered inputs: uc = w(i), um / up = w(i -/+ e_D) for all variables,
+// bn[k] = {B_k(i - e_k), B_k(i + e_k)} for the divergence along k != D (along
+// D the B values come from um / up) and dxk[k] = dx_k(i):
 // w1 = w - qdt/dx*(F(w+l/2) - F(w-l/2)) - (qdt/ndim)*S_Powell(w).
-// idx / c address the cell; reads w at the cell and its neighbours.
+// The limited slopes l along D are returned in lout when it is not null.
 // LIM: limiter type fixed at compile time (>= 0) or per variable from s
 // (< 0); PWM: Powell source on (1), off (0) or as s selects (2).
-template <int ND, bool MHD, class R, int D, int LIM = -1, int PWM = 2>
-__device__ __forceinline__ void pred_point(const Geo& g, const Sel& s, const double* __restrict__ w,
-                                           long long idx, const int* c, double dt, double smallp,
-                                           double* out) {
+template <int ND, bool MHD, class R, int D, int LIM, int PWM>
+__device__ __forceinline__ void pred_compute(const Sel& s, const double* uc, const double* um,
+                                             const double* up, const double (*bn)[2],
+                                             const double* dxk, double dt, double smallp,
+                                             double* out, double* lout) {
   constexpr int d = D;
   using L = Layout<ND, MHD>;
   constexpr int NW = L::nw;
   const double qdt = dt / 2;
-  const long long sd = g.s[d];
-  double uc[NW], wl[NW], wr[NW];
-  load<NW>(w, g.ps, idx, uc);
+  double wl[NW], wr[NW];
 #pragma unroll
   for (int iw = 0; iw < NW; ++iw) {
-    const double um = w[iw * g.ps + idx - sd], up = w[iw * g.ps + idx + sd];
-    const R l = limiter_t<LIM, R>(s.lim[iw], R(up - uc[iw]), R(uc[iw] - um));
+    const R l = limiter_t<LIM, R>(s.lim[iw], R(up[iw] - uc[iw]), R(uc[iw] - um[iw]));
     wr[iw] = uc[iw] - 0.5 * double(l);
     wl[iw] = uc[iw] + 0.5 * double(l);
+    if (lout) lout[iw] = double(l);
   }
   double fL[NW], fR[NW];
   fluxes<ND, MHD, R>(wl, d, s.gamma, smallp, fL);
   fluxes<ND, MHD, R>(wr, d, s.gamma, smallp, fR);
-  const double qdx = qdt / g.dx[d][c[d]];
+  const double qdx = qdt / dxk[d];
 #pragma unroll
   for (int iw = 0; iw < NW; ++iw) out[iw] = uc[iw] - qdx * (fL[iw] - fR[iw]);
   if constexpr (MHD && PWM != 0) {
     if (PWM == 1 || (s.sourceunsplit && s.divbfix)) {
-      const double dv = divb_at<ND>(g, w, idx, c);
+      // divB (src/vacphys.mhd0.t:119-133, src/vacgrid.t:822), summed from 0
+      double dv = 0.0;
+#pragma unroll
+      for (int k = 0; k < ND; ++k) {
+        const double bm = (k == d) ? um[L::b(k)] : bn[k][0];
+        const double bp = (k == d) ? up[L::b(k)] : bn[k][1];
+        const double gq = 0.5 * (bp - bm) / dxk[k];
+        dv = dv + gq;
+      }
       powell<ND>(uc, dv, qdt * 1.0 / ND, out);
     }
   }
+}
+
+// the predictor at cell idx / c of a grid in HBM
+template <int ND, bool MHD, class R, int D, int LIM = -1, int PWM = 2>
+__device__ __forceinline__ void pred_point(const Geo& g, const Sel& s, const double* __restrict__ w,
+                                           long long idx, const int* c, double dt, double smallp,
+                                           double* out) {
+  using L = Layout<ND, MHD>;
+  constexpr int NW = L::nw;
+  const long long sd = g.s[D];
+  double uc[NW], um[NW], up[NW], bn[ND][2], dxk[ND];
+#pragma unroll
+  for (int iw = 0; iw < NW; ++iw) {
+    uc[iw] = w[iw * g.ps + idx];
+    um[iw] = w[iw * g.ps + idx - sd];
+    up[iw] = w[iw * g.ps + idx + sd];
+  }
+#pragma unroll
+  for (int k = 0; k < ND; ++k) {
+    dxk[k] = g.dx[k][c[k]];
+    bn[k][0] = bn[k][1] = 0.0;
+    if constexpr (MHD && PWM != 0) {
+      if (k != D && (PWM == 1 || (s.sourceunsplit && s.divbfix))) {
+        bn[k][0] = w[L::b(k) * g.ps + idx - g.s[k]];
+        bn[k][1] = w[L::b(k) * g.ps + idx + g.s[k]];
+      }
+    }
+  }
+  pred_compute<ND, MHD, R, D, LIM, PWM>(s, uc, um, up, bn, dxk, dt, smallp, out, nullptr);
 }
 
 template <int ND, bool MHD, class R, int D>
@@ -363,12 +402,20 @@ __global__ __launch_bounds__(256) void k_tvdlf_corr(Geo g, Sel s, const double* 
 // lanes 1..62 produce output) and marches along y through a segment of rows,
 // keeping the predictor states of rows j-1, j, j+1 in registers.  Each face
 // is evaluated once: along x by the lane left of it (the neighbour's values
-// arrive by lane shuffles), along y once per marching step.
+// arrive by lane shuffles), along y once per marching step.  The predictor's
+// limited slopes are the corrector's (typelimited='original': both limit the
+// start-of-sweep w), so they are computed once.
 //
-// The predictor state of a ghost cell is the predictor state of the mesh cell
-// getboundary copies into it (src/vacgrid.t:234-273), computed there
-// directly; this needs one boundary type per face for all variables, which
-// the host checks before choosing these kernels.
+// The predictor state of a ghost cell is the one of the mesh cell that
+// getboundary copies into it (src/vacgrid.t:234-273), computed there directly.
+// The corrector slope at a ghost cell is that image's slope for 'periodic'
+// (the ghost values are copies of the image's neighbourhood) and exactly 0
+// for 'cont' (both differences vanish).  Output cells are also written to the
+// ghost cells that take their value, so the sweep leaves wout's ghost cells
+// filled.  The Courant maximum of the sweep is folded by the last block to
+// finish.  All of this needs one boundary type per face for all variables,
+// at least 4 cells per dimension and planes below 2^29 elements, which the
+// host checks before choosing these kernels.
 constexpr int FS_STRIDE = 62;
 
 __device__ __forceinline__ int ghost_image(int c, int n, bool per_lo, bool per_hi) {
@@ -376,19 +423,108 @@ __device__ __forceinline__ int ghost_image(int c, int n, bool per_lo, bool per_h
   return c < 2 ? lo : (c > n + 1 ? hi : c);
 }
 
+// ghost cells whose image is mesh cell c (n >= 4), written to t; returns the count
+__device__ __forceinline__ int ghost_preimages(int c, int n, bool per_lo, bool per_hi, int* t) {
+  int k = 0;
+  if (per_lo) { if (c >= n) t[k++] = c - n; }
+  else if (c == 2) { t[k++] = 0; t[k++] = 1; }
+  if (per_hi) { if (c <= 3) t[k++] = c + n; }
+  else if (c == n + 1) { t[k++] = n + 2; t[k++] = n + 3; }
+  return k;
+}
+
 template <class T>
 __device__ __forceinline__ T lane_next(T v) { return __shfl_down(v, 1, 64); }
 template <class T>
 __device__ __forceinline__ T lane_prev(T v) { return __shfl_up(v, 1, 64); }
 
-// fused kernels run with the Powell source on for MHD (divbfix and
-// sourceunsplit, VAC's defaults) and one limiter type LIM for all variables
+// plane iw at byte offset o (planes are addressed with 32-bit byte offsets)
+__device__ __forceinline__ double ldp(const double* __restrict__ w, long long ps, int iw, unsigned o) {
+  return *reinterpret_cast<const double*>(reinterpret_cast<const char*>(w + iw * ps) + o);
+}
+__device__ __forceinline__ void stp(double* __restrict__ w, long long ps, int iw, unsigned o, double v) {
+  *reinterpret_cast<double*>(reinterpret_cast<char*>(w + iw * ps) + o) = v;
+}
+
+struct FusedGeo {  // per-wave constants of a fused sweep
+  int n0, n1;
+  unsigned sy8;     // row stride in bytes
+  unsigned base8;   // byte offset of ixG cell (0, 0)
+  bool pxlo, pxhi, pylo, pyhi;
+  __device__ FusedGeo(const Geo& g, const BC& bc)
+      : n0(g.n[0]), n1(g.n[1]), sy8((unsigned)g.s[1] * 8u), base8((unsigned)g.off * 8u),
+        pxlo(bc.t[0][0]), pxhi(bc.t[1][0]), pylo(bc.t[2][0]), pyhi(bc.t[3][0]) {}
+  __device__ unsigned at(int x, int y) const { return base8 + (unsigned)x * 8u + (unsigned)y * sy8; }
+};
+
+// inputs of the predictor at (px, py) loaded from HBM: w at the cell and its
+// two neighbours along D, B across D, and the widths
+template <bool MHD>
+struct PredIn {
+  static constexpr int NW = Layout<2, MHD>::nw;
+  double uc[NW], um[NW], up[NW];
+  double bn[2][2];
+  double dxk[2];
+};
+
+template <bool MHD, int D>
+__device__ __forceinline__ void pred_load(const Geo& g, const FusedGeo& f,
+                                          const double* __restrict__ w, int px, int py,
+                                          PredIn<MHD>& in) {
+  using L = Layout<2, MHD>;
+  constexpr int NW = L::nw;
+  const unsigned o = f.at(px, py);
+  const unsigned sd = D == 0 ? 8u : f.sy8, st = D == 0 ? f.sy8 : 8u;
+#pragma unroll
+  for (int iw = 0; iw < NW; ++iw) {
+    in.uc[iw] = ldp(w, g.ps, iw, o);
+    in.um[iw] = ldp(w, g.ps, iw, o - sd);
+    in.up[iw] = ldp(w, g.ps, iw, o + sd);
+  }
+  in.bn[0][0] = in.bn[0][1] = in.bn[1][0] = in.bn[1][1] = 0.0;
+  if constexpr (MHD) {
+    in.bn[1 - D][0] = ldp(w, g.ps, L::b(1 - D), o - st);
+    in.bn[1 - D][1] = ldp(w, g.ps, L::b(1 - D), o + st);
+  }
+  in.dxk[0] = g.dx[0][px];
+  in.dxk[1] = g.dx[1][py];
+}
+
 template <bool MHD, class R, int D, int LIM>
-__device__ __forceinline__ void pred_at(const Geo& g, const Sel& s, const double* __restrict__ w,
-                                        int px, int py, double dt, double smallp, double* out) {
-  const int c[3] = {px, py, 0};
-  pred_point<2, MHD, R, D, LIM, 1>(g, s, w, g.off + px + (long long)py * g.s[1], c, dt, smallp,
-                                   out);
+__device__ __forceinline__ void pred_run(const Sel& s, const PredIn<MHD>& in, double dt,
+                                         double smallp, double* out, double* lout) {
+  pred_compute<2, MHD, R, D, LIM, MHD ? 1 : 0>(s, in.uc, in.um, in.up, in.bn, in.dxk, dt, smallp,
+                                               out, lout);
+}
+
+// predictor (and its slopes) at (px, py), every input loaded from HBM
+template <bool MHD, class R, int D, int LIM>
+__device__ __forceinline__ void pred_gather(const Geo& g, const FusedGeo& f, const Sel& s,
+                                            const double* __restrict__ w, int px, int py, double dt,
+                                            double smallp, double* out, double* lout, double* uout) {
+  PredIn<MHD> in;
+  pred_load<MHD, D>(g, f, w, px, py, in);
+  pred_run<MHD, R, D, LIM>(s, in, dt, smallp, out, lout);
+#pragma unroll
+  for (int iw = 0; iw < PredIn<MHD>::NW; ++iw) uout[iw] = in.uc[iw];
+}
+
+// store out at mesh cell (c, j) and at the ghost cells taking its value
+template <int NW>
+__device__ __forceinline__ void store_with_ghosts(const Geo& g, const FusedGeo& f,
+                                                  double* __restrict__ wout, int c, int j,
+                                                  const double* out) {
+  int tx[5], ty[5];
+  tx[0] = c;
+  ty[0] = j;
+  const int nx = 1 + ghost_preimages(c, f.n0, f.pxlo, f.pxhi, tx + 1);
+  const int ny = 1 + ghost_preimages(j, f.n1, f.pylo, f.pyhi, ty + 1);
+  for (int a = 0; a < nx; ++a)
+    for (int b = 0; b < ny; ++b) {
+      const unsigned o = f.at(tx[a], ty[b]);
+#pragma unroll
+      for (int iw = 0; iw < NW; ++iw) stp(wout, g.ps, iw, o, out[iw]);
+    }
 }
 
 // x sweep
@@ -400,40 +536,41 @@ __device__ __forceinline__ double tvdlf_fx_body(const Geo& g, const Sel& s, cons
   constexpr int ND = 2;
   using L = Layout<ND, MHD>;
   constexpr int NW = L::nw;
+  const FusedGeo f(g, bc);
   const int lane = (int)threadIdx.x;
-  const int n0 = g.n[0], n1 = g.n[1];
   int c = 1 + strip * FS_STRIDE + lane;
-  const bool useful = lane >= 1 && lane <= FS_STRIDE && c <= n0 + 1;
-  const bool face_ok = lane <= FS_STRIDE && c <= n0 + 1;  // face c+1/2 with c in [1, n0+1]
-  c = c < n0 + 2 ? c : n0 + 2;                              // idle lanes: in-bounds, unused
-  const int px = ghost_image(c, n0, bc.t[0][0], bc.t[1][0]);
-  const bool plo = bc.t[2][0], phi = bc.t[3][0];
+  const bool useful = lane >= 1 && lane <= FS_STRIDE && c <= f.n0 + 1;
+  const bool face_ok = lane <= FS_STRIDE && c <= f.n0 + 1;  // face c+1/2 with c in [1, n0+1]
+  c = c < f.n0 + 2 ? c : f.n0 + 2;                          // idle lanes: in-bounds, unused
+  const int px = ghost_image(c, f.n0, f.pxlo, f.pxhi);
+  const bool zero_slope = (c < 2 && !f.pxlo) || (c > f.n0 + 1 && !f.pxhi);
   const int s0 = 2 + seg * seglen;
   double cmx = -INFINITY;
-  if (s0 > n1 + 1) return cmx;  // wave-uniform
-  const int s1 = (s0 + seglen - 1 < n1 + 1) ? s0 + seglen - 1 : n1 + 1;
+  if (s0 > f.n1 + 1) return cmx;  // wave-uniform
+  const int s1 = (s0 + seglen - 1 < f.n1 + 1) ? s0 + seglen - 1 : f.n1 + 1;
   const double* dx0 = g.dx[0];
   const double* dx1 = g.dx[1];
-  double Pc[NW], Pn[NW];
+  double Pc[NW], Pn[NW], Lc[NW], Ln[NW], Uc[NW], Un[NW];
   double byp = 0.0;  // B_y of the predictor state one row below
-  pred_at<MHD, R, 0, LIM>(g, s, w, px, ghost_image(s0 - 1, n1, plo, phi), dt, smallp, Pn);
+  pred_gather<MHD, R, 0, LIM>(g, f, s, w, px, ghost_image(s0 - 1, f.n1, f.pylo, f.pyhi), dt,
+                              smallp, Pn, Ln, Un);
   if constexpr (MHD) byp = Pn[L::b(1)];
-  pred_at<MHD, R, 0, LIM>(g, s, w, px, s0, dt, smallp, Pc);
+  pred_gather<MHD, R, 0, LIM>(g, f, s, w, px, s0, dt, smallp, Pc, Lc, Uc);
   const double dxs = dx0[c] + dx0[c + 1];
   const double qdx = dt / dx0[c];
+  // inputs of the next row's predictor are loaded one marching step ahead
+  PredIn<MHD> in;
+  pred_load<MHD, 0>(g, f, w, px, ghost_image(s0 + 1, f.n1, f.pylo, f.pyhi), in);
+#pragma unroll 2
   for (int j = s0; j <= s1; ++j) {
-    pred_at<MHD, R, 0, LIM>(g, s, w, px, ghost_image(j + 1, n1, plo, phi), dt, smallp, Pn);
-    const long long idx = g.off + c + (long long)j * g.s[1];
-    double lw[NW], u[NW];
+    pred_run<MHD, R, 0, LIM>(s, in, dt, smallp, Pn, Ln);
+#pragma unroll
+    for (int iw = 0; iw < NW; ++iw) Un[iw] = in.uc[iw];
+    if (j < s1) pred_load<MHD, 0>(g, f, w, px, ghost_image(j + 2, f.n1, f.pylo, f.pyhi), in);
+    double lw[NW], pr[NW], lr[NW];
 #pragma unroll
     for (int iw = 0; iw < NW; ++iw) {
-      const double* p = w + iw * g.ps + idx;
-      u[iw] = p[0];
-      lw[iw] = slope<R, LIM>(s.lim[iw], p[-1], u[iw], p[1]);
-    }
-    double pr[NW], lr[NW];
-#pragma unroll
-    for (int iw = 0; iw < NW; ++iw) {
+      lw[iw] = zero_slope ? 0.0 : Lc[iw];
       pr[iw] = lane_next(Pc[iw]);
       lr[iw] = lane_next(lw[iw]);
     }
@@ -444,31 +581,31 @@ __device__ __forceinline__ double tvdlf_fx_body(const Geo& g, const Sel& s, cons
 #pragma unroll
     for (int iw = 0; iw < NW; ++iw) {
       const double Fm = lane_prev(F[iw]);
-      out[iw] = u[iw] - qdx * (F[iw] - Fm);
+      out[iw] = Uc[iw] - qdx * (F[iw] - Fm);
     }
     if constexpr (MHD) {
       const double bxp = lane_next(Pc[L::b(0)]), bxm = lane_prev(Pc[L::b(0)]);
-      {
-        double dv = 0.0;
-        const double g0 = 0.5 * (bxp - bxm) / dx0[c];
-        dv = dv + g0;
-        const double g1 = 0.5 * (Pn[L::b(1)] - byp) / dx1[j];
-        dv = dv + g1;
-        powell<ND>(Pc, dv, dt * 1.0 / ND, out);
-      }
+      double dv = 0.0;
+      const double g0 = 0.5 * (bxp - bxm) / dx0[c];
+      dv = dv + g0;
+      const double g1 = 0.5 * (Pn[L::b(1)] - byp) / dx1[j];
+      dv = dv + g1;
+      powell<ND>(Pc, dv, dt * 1.0 / ND, out);
       byp = Pc[L::b(1)];
     }
-    if (useful) {
+    if (useful) store_with_ghosts<NW>(g, f, wout, c, j, out);
 #pragma unroll
-      for (int iw = 0; iw < NW; ++iw) wout[iw * g.ps + idx] = out[iw];
+    for (int iw = 0; iw < NW; ++iw) {
+      Pc[iw] = Pn[iw];
+      Lc[iw] = Ln[iw];
+      Uc[iw] = Un[iw];
     }
-#pragma unroll
-    for (int iw = 0; iw < NW; ++iw) Pc[iw] = Pn[iw];
   }
   return cmx;
 }
 
-// y sweep
+// y sweep: raw rows j, j+1, j+2 of w at column px are kept in registers, so
+// a marching step loads one new row
 template <bool MHD, class R, int LIM>
 __device__ __forceinline__ double tvdlf_fy_body(const Geo& g, const Sel& s, const BC& bc,
                                                 const double* __restrict__ w,
@@ -477,95 +614,183 @@ __device__ __forceinline__ double tvdlf_fy_body(const Geo& g, const Sel& s, cons
   constexpr int ND = 2;
   using L = Layout<ND, MHD>;
   constexpr int NW = L::nw;
+  const FusedGeo f(g, bc);
   const int lane = (int)threadIdx.x;
-  const int n0 = g.n[0], n1 = g.n[1];
   int c = 1 + strip * FS_STRIDE + lane;
-  const bool useful = lane >= 1 && lane <= FS_STRIDE && c <= n0 + 1;
-  c = c < n0 + 2 ? c : n0 + 2;
-  const int px = ghost_image(c, n0, bc.t[0][0], bc.t[1][0]);
-  const bool plo = bc.t[2][0], phi = bc.t[3][0];
+  const bool useful = lane >= 1 && lane <= FS_STRIDE && c <= f.n0 + 1;
+  c = c < f.n0 + 2 ? c : f.n0 + 2;
+  const int px = ghost_image(c, f.n0, f.pxlo, f.pxhi);
   const int s0 = 2 + seg * seglen;
   double cmx = -INFINITY;
-  if (s0 > n1 + 1) return cmx;  // wave-uniform
-  const int s1 = (s0 + seglen - 1 < n1 + 1) ? s0 + seglen - 1 : n1 + 1;
+  if (s0 > f.n1 + 1) return cmx;  // wave-uniform
+  const int s1 = (s0 + seglen - 1 < f.n1 + 1) ? s0 + seglen - 1 : f.n1 + 1;
   const double* dx0 = g.dx[0];
   const double* dx1 = g.dx[1];
-  const long long sy = g.s[1];
-  const long long col = g.off + c;
-  auto slopes_at = [&](int r, double* l) {
-#pragma unroll
-    for (int iw = 0; iw < NW; ++iw) {
-      const double* p = w + iw * g.ps + col + (long long)r * sy;
-      l[iw] = slope<R, LIM>(s.lim[iw], p[-sy], p[0], p[sy]);
+  const double dxp = dx0[px];
+  // B_x at (px -/+ 1, r) for the predictor's divergence
+  auto load_bx = [&](int r, double* bx) {
+    bx[0] = bx[1] = 0.0;
+    if constexpr (MHD) {
+      const unsigned o = f.at(px, r);
+      bx[0] = ldp(w, g.ps, L::b(0), o - 8u);
+      bx[1] = ldp(w, g.ps, L::b(0), o + 8u);
     }
   };
-  double Pc[NW], Pn[NW], lc[NW], ln[NW], Fp[NW], Fn[NW], cv;
+  // predictor at mesh row r from raw rows r-1, r, r+1 and B_x in registers
+  auto pred_rowx = [&](int r, const double* wm, const double* w0, const double* wp,
+                       const double* bx, double* P, double* l) {
+    const double bn[2][2] = {{bx[0], bx[1]}, {0.0, 0.0}};
+    const double dxk[2] = {dxp, dx1[r]};
+    pred_compute<2, MHD, R, 1, LIM, MHD ? 1 : 0>(s, w0, wm, wp, bn, dxk, dt, smallp, P, l);
+  };
+  auto pred_row = [&](int r, const double* wm, const double* w0, const double* wp, double* P,
+                      double* l) {
+    double bx[2];
+    load_bx(r, bx);
+    pred_rowx(r, wm, w0, wp, bx, P, l);
+  };
+  // predictor at a ghost row: at its image, with that image's slopes ('periodic')
+  // or zero slopes ('cont')
+  auto pred_ghost = [&](int r, double* P, double* l) {
+    double u[NW];
+    pred_gather<MHD, R, 1, LIM>(g, f, s, w, px, ghost_image(r, f.n1, f.pylo, f.pyhi), dt, smallp,
+                                P, l, u);
+    const bool zs = (r < 2 && !f.pylo) || (r > f.n1 + 1 && !f.pyhi);
+#pragma unroll
+    for (int iw = 0; iw < NW; ++iw) l[iw] = zs ? 0.0 : l[iw];
+  };
+  auto load_row = [&](int r, double* u) {
+    const unsigned o = f.at(px, r);
+#pragma unroll
+    for (int iw = 0; iw < NW; ++iw) u[iw] = ldp(w, g.ps, iw, o);
+  };
+  double Pc[NW], Pn[NW], lc[NW], ln[NW], Fp[NW], Fn[NW], W0[NW], W1[NW], W2[NW], cv;
   double byp = 0.0;
-  // face s0-1/2
-  pred_at<MHD, R, 1, LIM>(g, s, w, px, ghost_image(s0 - 1, n1, plo, phi), dt, smallp, Pn);
-  slopes_at(s0 - 1, ln);
-  pred_at<MHD, R, 1, LIM>(g, s, w, px, s0, dt, smallp, Pc);
-  slopes_at(s0, lc);
+  // face s0-1/2 from rows s0-1 and s0
+  load_row(s0 - 1, W0);
+  load_row(s0, W1);
+  load_row(s0 + 1, W2);
+  if (s0 - 1 >= 2) {
+    double Wm[NW];
+    load_row(s0 - 2, Wm);
+    pred_row(s0 - 1, Wm, W0, W1, Pn, ln);
+  } else {
+    pred_ghost(s0 - 1, Pn, ln);
+  }
+  pred_row(s0, W0, W1, W2, Pc, lc);
   corr_face<ND, MHD, R, 1>(s, Pn, Pc, ln, lc, dx1[s0 - 1] + dx1[s0], smallp, Fp, cv);
   if (useful) cmx = cmx < cv ? cv : cmx;
   if constexpr (MHD) byp = Pn[L::b(1)];
+  // window: W0 = row j, W1 = row j+1, W2 = row j+2 and bx = B_x around
+  // row j+1, loaded one marching step ahead
+  double bx[2] = {0.0, 0.0}, W3[NW] = {}, bxn[2] = {0.0, 0.0};
+#pragma unroll
+  for (int iw = 0; iw < NW; ++iw) W0[iw] = W1[iw];
+#pragma unroll
+  for (int iw = 0; iw < NW; ++iw) W1[iw] = W2[iw];
+  if (s0 + 1 <= f.n1 + 1) {
+    load_row(s0 + 2, W2);
+    load_bx(s0 + 1, bx);
+  }
+#pragma unroll 2
   for (int j = s0; j <= s1; ++j) {
-    pred_at<MHD, R, 1, LIM>(g, s, w, px, ghost_image(j + 1, n1, plo, phi), dt, smallp, Pn);
-    slopes_at(j + 1, ln);
+    const bool ahead = j + 1 <= s1 && j + 2 <= f.n1 + 1;
+    if (ahead) {
+      load_row(j + 3, W3);
+      load_bx(j + 2, bxn);
+    }
+    if (j + 1 <= f.n1 + 1) {
+      pred_rowx(j + 1, W0, W1, W2, bx, Pn, ln);
+    } else {
+      pred_ghost(j + 1, Pn, ln);
+    }
     corr_face<ND, MHD, R, 1>(s, Pc, Pn, lc, ln, dx1[j] + dx1[j + 1], smallp, Fn, cv);
     if (useful) cmx = cmx < cv ? cv : cmx;
-    const long long idx = col + (long long)j * sy;
     const double qdx = dt / dx1[j];
     double out[NW];
 #pragma unroll
-    for (int iw = 0; iw < NW; ++iw) out[iw] = w[iw * g.ps + idx] - qdx * (Fn[iw] - Fp[iw]);
+    for (int iw = 0; iw < NW; ++iw) out[iw] = W0[iw] - qdx * (Fn[iw] - Fp[iw]);
     if constexpr (MHD) {
       const double bxp = lane_next(Pc[L::b(0)]), bxm = lane_prev(Pc[L::b(0)]);
-      {
-        double dv = 0.0;
-        const double g0 = 0.5 * (bxp - bxm) / dx0[c];
-        dv = dv + g0;
-        const double g1 = 0.5 * (Pn[L::b(1)] - byp) / dx1[j];
-        dv = dv + g1;
-        powell<ND>(Pc, dv, dt * 1.0 / ND, out);
-      }
+      double dv = 0.0;
+      const double g0 = 0.5 * (bxp - bxm) / dx0[c];
+      dv = dv + g0;
+      const double g1 = 0.5 * (Pn[L::b(1)] - byp) / dx1[j];
+      dv = dv + g1;
+      powell<ND>(Pc, dv, dt * 1.0 / ND, out);
       byp = Pc[L::b(1)];
     }
-    if (useful) {
-#pragma unroll
-      for (int iw = 0; iw < NW; ++iw) wout[iw * g.ps + idx] = out[iw];
-    }
+    if (useful) store_with_ghosts<NW>(g, f, wout, c, j, out);
 #pragma unroll
     for (int iw = 0; iw < NW; ++iw) {
       Pc[iw] = Pn[iw];
       lc[iw] = ln[iw];
       Fp[iw] = Fn[iw];
+      W0[iw] = W1[iw];
+      W1[iw] = W2[iw];
+      W2[iw] = W3[iw];
     }
+    bx[0] = bxn[0];
+    bx[1] = bxn[1];
   }
   return cmx;
 }
 
-// blockDim = (64, 4): wave threadIdx.y takes segment blockIdx.y*4 + y of
-// strip blockIdx.x
 #ifndef VACGPU_FUSED_WAVES
 #define VACGPU_FUSED_WAVES 1
 #endif
+// blockDim = (64, 4): wave threadIdx.y takes segment blockIdx.y*4 + y of
+// strip blockIdx.x.  The block's Courant maximum goes to part[SLOT_C0]; the
+// last block to finish folds them and applies the dtcourant update of the
+// TVDLF corrector (src/vactvdlf.t:184-201) when courantpar > 0, then resets
+// the counter for the next launch.
 template <bool MHD, class R, int D, int LIM>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(VACGPU_FUSED_WAVES))) void k_tvdlf_fused(Geo g, Sel s, BC bc,
-                                                    const double* __restrict__ w,
-                                                    double* __restrict__ wout,
-                                                    const Scal* __restrict__ sc,
-                                                    double* __restrict__ part, long long pstride,
-                                                    int seglen) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(VACGPU_FUSED_WAVES))) void k_tvdlf_fused(
+    Geo g, Sel s, BC bc, const double* __restrict__ w, double* __restrict__ wout,
+    Scal* __restrict__ sc, double* __restrict__ part, long long pstride, int seglen,
+    unsigned* __restrict__ ctr, double courantpar) {
   __shared__ double sm[8];
+  __shared__ int last;
   const int seg = (int)(blockIdx.y * blockDim.y + threadIdx.y);
+  const double dt = sc->dt, smallp = sc->smallp;
   double cmx;
   if constexpr (D == 0)
-    cmx = tvdlf_fx_body<MHD, R, LIM>(g, s, bc, w, wout, sc->dt, sc->smallp, (int)blockIdx.x, seg, seglen);
+    cmx = tvdlf_fx_body<MHD, R, LIM>(g, s, bc, w, wout, dt, smallp, (int)blockIdx.x, seg, seglen);
   else
-    cmx = tvdlf_fy_body<MHD, R, LIM>(g, s, bc, w, wout, sc->dt, sc->smallp, (int)blockIdx.x, seg, seglen);
+    cmx = tvdlf_fy_body<MHD, R, LIM>(g, s, bc, w, wout, dt, smallp, (int)blockIdx.x, seg, seglen);
   const double bm = block_reduce<true>(cmx, sm);
-  if (threadIdx.x == 0 && threadIdx.y == 0) part[SLOT_C0 * pstride + block_linear()] = bm;
+  const unsigned nb = gridDim.x * gridDim.y;
+  const int tid = threadIdx.x + blockDim.x * threadIdx.y;
+  if (tid == 0) {
+    __hip_atomic_store(reinterpret_cast<unsigned long long*>(part + SLOT_C0 * pstride) +
+                           block_linear(),
+                       (unsigned long long)__double_as_longlong(bm), __ATOMIC_RELAXED,
+                       __HIP_MEMORY_SCOPE_AGENT);
+    __threadfence();
+    last = atomicAdd(ctr, 1u) == nb - 1;
+  }
+  __syncthreads();
+  if (!last) return;  // block-uniform
+  __threadfence();
+  double v = -INFINITY;
+  for (unsigned i = tid; i < nb; i += blockDim.x * blockDim.y) {
+    const unsigned long long u = __hip_atomic_load(
+        reinterpret_cast<unsigned long long*>(part + SLOT_C0 * pstride) + i, __ATOMIC_RELAXED,
+        __HIP_MEMORY_SCOPE_AGENT);
+    const double x = __longlong_as_double((long long)u);
+    v = v < x ? x : v;
+  }
+  const double cm = block_reduce<true>(v, sm);
+  if (tid == 0) {
+    if (courantpar > 0.0) {
+      if (dt * cm > 1.0) sc->nerror[2] += 1;
+      if (cm > kSmall) {
+        const double cc = courantpar / cm;
+        sc->dtcourant[D] = cc < sc->dtcourant[D] ? cc : sc->dtcourant[D];
+      }
+    }
+    atomicExch(ctr, 0u);
+  }
 }
 
 // ------------------------------------------------------------ TVD (Roe)

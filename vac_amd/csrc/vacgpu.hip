@@ -55,6 +55,7 @@ struct vacgpu_grid {
   int fs_seglen = 16;
   dim3 grid_fs, block_fs;
   double* stage = nullptr;  // pinned staging of put_state / get_state
+  unsigned* ctr = nullptr;  // block counter of the fused sweeps (zero between launches)
   size_t stage_bytes = 0;
   long long nbound = 0, slab[3] = {0, 0, 0};
   int cur = 0, old = 0;
@@ -277,7 +278,7 @@ int sweep_device(vacgpu_grid* g, const vacgpu_method* m, int d, int firstsweep, 
       Timed t(g, KF_FUSED);
       const bool f32 = g->cfg.precision == VACGPU_FP32BULK;
       const int lim = m->typelimiter[0];
-#define FUSED_L(MV, RT, DV, LV) hipLaunchKernelGGL((k_tvdlf_fused<MV, RT, DV, LV>), g->grid_fs, g->block_fs, 0, g->stream, g->geo, s, g->bc, win, wout, g->sc, g->part, g->pstride, g->fs_seglen)
+#define FUSED_L(MV, RT, DV, LV) hipLaunchKernelGGL((k_tvdlf_fused<MV, RT, DV, LV>), g->grid_fs, g->block_fs, 0, g->stream, g->geo, s, g->bc, win, wout, g->sc, g->part, g->pstride, g->fs_seglen, g->ctr, m->courantpar)
 #define FUSED(MV, RT, DV) do { if (lim == LIM_MINMOD) FUSED_L(MV, RT, DV, LIM_MINMOD); else if (lim == LIM_WOODWARD) FUSED_L(MV, RT, DV, LIM_WOODWARD); else FUSED_L(MV, RT, DV, LIM_SUPERBEE); } while (0)
       if (mhd) {
         if (f32) { if (d == 0) FUSED(true, float, 0); else FUSED(true, float, 1); }
@@ -290,12 +291,10 @@ int sweep_device(vacgpu_grid* g, const vacgpu_method* m, int d, int firstsweep, 
 #undef FUSED_L
       if ((rc = launch_check(g, "k_tvdlf_fused"))) return rc;
     }
-    if (courant) {
-      Timed t(g, KF_RED);
-      hipLaunchKernelGGL(k_fin_courant_tvdlf, dim3(1), dim3(1024), 0, g->stream, g->sc, g->part,
-                         g->pstride, nbf, d, m->courantpar);
-      if ((rc = launch_check(g, "k_fin_courant_tvdlf"))) return rc;
-    }
+    (void)nbf;
+    // the fused kernel has folded the Courant maximum and filled wout's ghost cells
+    g->cur = outi;
+    return 0;
   } else if (m->typefull == VACGPU_TVDLF) {
     {
       Timed t(g, KF_PRED);
@@ -485,7 +484,7 @@ int vacgpu_create(const vacgpu_config* cfg, const double* dx0, const double* dx1
     g->grid_tm[d] = dim3(g->grid.x, (nseg + BY - 1) / BY, nd == 3 ? geo.n[other] : 1);
   }
   // fused TVDLF: one boundary type per face for all variables
-  g->fused_ok = nd == 2;
+  g->fused_ok = nd == 2 && geo.n[0] >= 4 && geo.n[1] >= 4 && (unsigned long long)geo.ps * 8ULL < (1ULL << 32);
   for (int f = 0; f < 2 * nd; ++f)
     for (int iw = 1; iw < g->nw; ++iw)
       if (g->bc.t[f][iw] != g->bc.t[f][0]) g->fused_ok = false;
@@ -512,6 +511,8 @@ int vacgpu_create(const vacgpu_config* cfg, const double* dx0, const double* dx1
   if (e == hipSuccess) e = hipMalloc(&g->sc, sizeof(Scal));
   if (e == hipSuccess) e = hipHostMalloc(&g->sc_host, sizeof(Scal), hipHostMallocDefault);
   if (e == hipSuccess) e = hipMalloc(&g->part, sizeof(double) * NSLOT * g->pstride);
+  if (e == hipSuccess) e = hipMalloc(&g->ctr, sizeof(unsigned));
+  if (e == hipSuccess) e = hipMemset(g->ctr, 0, sizeof(unsigned));
   if (e == hipSuccess) e = hipStreamCreateWithFlags(&g->stream, hipStreamNonBlocking);
   if (e != hipSuccess) {
     fprintf(stderr, "vacgpu_create: %s\n", hipGetErrorString(e));
@@ -801,6 +802,7 @@ void vacgpu_destroy(vacgpu_grid* g) {
   if (g->sc_host) (void)hipHostFree(g->sc_host);
   if (g->stage) (void)hipHostFree(g->stage);
   if (g->part) (void)hipFree(g->part);
+  if (g->ctr) (void)hipFree(g->ctr);
   if (g->own_stream && g->stream) (void)hipStreamDestroy(g->stream);
   delete g;
 }
