@@ -445,14 +445,19 @@ __device__ __forceinline__ R roe_rtimes(const RoeRt<ND, R>& r, R q, int iw, int 
 // getphi for method 'tvd', typetvd 'roe', entropy fix 'nul'
 // (src/vactvd.t:246-247, :294-302): jm, j0, jp are the jumps of one wave at
 // faces J-1, J, J+1 and av = a*dt*2/(dx_J+dx_J+1) at face J.
-template <class R>
-__device__ __forceinline__ R tvd_phi(int lim, R jm, R j0, R jp, R av) {
+template <int LIM, class R>
+__device__ __forceinline__ R tvd_phi_t(int lim, R jm, R j0, R jp, R av) {
   R ph = R(fabs(av)) - av * av;
-  const R ljl = limiter<R>(lim, j0, jm);  // ljump(J)
-  const R ljr = limiter<R>(lim, jp, j0);  // ljump(J+1)
+  const R ljl = limiter_t<LIM, R>(lim, j0, jm);  // ljump(J)
+  const R ljr = limiter_t<LIM, R>(lim, jp, j0);  // ljump(J+1)
   if (av <= R(0)) ph = ph * (j0 - ljr);
   else ph = ph * (j0 - ljl);
   return ph + av * av * j0;
+}
+
+template <class R>
+__device__ __forceinline__ R tvd_phi(int lim, R jm, R j0, R jp, R av) {
+  return tvd_phi_t<-1, R>(lim, jm, j0, jp, av);
 }
 
 }  // namespace vacgpu

@@ -890,7 +890,7 @@ constexpr int TX_STRIDE = 61;  // cells updated per 64-lane wavefront along x
 
 // x sweep: lane L of a wavefront owns face J = x0 - 2 + L (between cells J and
 // J+1) and updates cell J for L in [2, 62].
-template <int ND, class R>
+template <int ND, class R, int LIM = -1>
 __global__ __launch_bounds__(256) void k_tvd_x(Geo g, Sel s, const double* __restrict__ wct,
                                               const double* __restrict__ win,
                                               double* __restrict__ wout,
@@ -929,7 +929,7 @@ __global__ __launch_bounds__(256) void k_tvd_x(Geo g, Sel s, const double* __res
 #pragma unroll
   for (int il = 0; il < NW; ++il) {
     const R jm = shfl_up1(f.jump[il]), jp = shfl_down1(f.jump[il]);
-    phi[il] = tvd_phi<R>(s.lim[il], jm, f.jump[il], jp, f.adt[il]);
+    phi[il] = tvd_phi_t<LIM, R>(s.lim[il], jm, f.jump[il], jp, f.adt[il]);
   }
   R phim[NW];
 #pragma unroll
@@ -961,7 +961,7 @@ __global__ __launch_bounds__(256) void k_tvd_x(Geo g, Sel s, const double* __res
 }
 
 // y / z sweep: thread (x, segment) marches through cells s0..s1 along d.
-template <int ND, class R, int D>
+template <int ND, class R, int D, int LIM = -1>
 __global__ __launch_bounds__(256) void k_tvd_m(Geo g, Sel s, const double* __restrict__ wct,
                                               const double* __restrict__ win,
                                               double* __restrict__ wout,
@@ -1022,7 +1022,7 @@ __global__ __launch_bounds__(256) void k_tvd_m(Geo g, Sel s, const double* __res
     c0 = c0 < a ? a : c0; c1 = c1 < b ? b : c1;
     // phi at face s0-1 from faces s0-2, s0-1, s0
 #pragma unroll
-    for (int il = 0; il < NW; ++il) phim[il] = tvd_phi<R>(s.lim[il], fm.jump[il], f0.jump[il], fp.jump[il], f0.adt[il]);
+    for (int il = 0; il < NW; ++il) phim[il] = tvd_phi_t<LIM, R>(s.lim[il], fm.jump[il], f0.jump[il], fp.jump[il], f0.adt[il]);
     // window now: fm <- face s0-1, f0 <- face s0 (fp holds face s0)
     fm = f0;
     f0 = fp;
@@ -1041,7 +1041,7 @@ __global__ __launch_bounds__(256) void k_tvd_m(Geo g, Sel s, const double* __res
       tvd_face<ND, R>(uL, uR, d, s, smallp, dt, dxd[i + 1] + dxd[i + 2], fp, a, b);
       c0 = c0 < a ? a : c0; c1 = c1 < b ? b : c1;
 #pragma unroll
-      for (int il = 0; il < NW; ++il) phi0[il] = tvd_phi<R>(s.lim[il], fm.jump[il], f0.jump[il], fp.jump[il], f0.adt[il]);
+      for (int il = 0; il < NW; ++il) phi0[il] = tvd_phi_t<LIM, R>(s.lim[il], fm.jump[il], f0.jump[il], fp.jump[il], f0.adt[il]);
       // update cell i
       const long long idx = base + (long long)i * sd;
       double out[NW];

@@ -339,22 +339,48 @@ int sweep_device(vacgpu_grid* g, const vacgpu_method* m, int d, int firstsweep, 
     {
       Timed t(g, KF_TVD);
       const bool f32 = g->cfg.precision == VACGPU_FP32BULK;
+      // one limiter type for every characteristic wave: fixed at compile time
+      bool one = true;
+      for (int il = 1; il < g->nw; ++il) one &= m->typelimiter[il] == m->typelimiter[0];
+      const int lim = one ? m->typelimiter[0] : -1;
+#define LIMSW(X) do { if (lim == LIM_MINMOD) X(LIM_MINMOD); else if (lim == LIM_WOODWARD) X(LIM_WOODWARD); else if (lim == LIM_SUPERBEE) X(LIM_SUPERBEE); else X(-1); } while (0)
       if (d == 0) {
         const dim3 gr = g->grid_tx;
         nbt = (long long)gr.x * gr.y * gr.z;
-#define TVDX(NDV, RT) hipLaunchKernelGGL((k_tvd_x<NDV, RT>), gr, g->block, 0, g->stream, g->geo, s, wct, win, wout, g->sc, g->part, g->pstride)
-        if (g->cfg.ndim == 2) { if (f32) TVDX(2, float); else TVDX(2, double); }
-        else { if (f32) TVDX(3, float); else TVDX(3, double); }
-#undef TVDX
+#define TVDX_L(NDV, RT, LV) hipLaunchKernelGGL((k_tvd_x<NDV, RT, LV>), gr, g->block, 0, g->stream, g->geo, s, wct, win, wout, g->sc, g->part, g->pstride)
+#define X2D(LV) TVDX_L(2, double, LV)
+#define X2F(LV) TVDX_L(2, float, LV)
+#define X3D(LV) TVDX_L(3, double, LV)
+#define X3F(LV) TVDX_L(3, float, LV)
+        if (g->cfg.ndim == 2) { if (f32) LIMSW(X2F); else LIMSW(X2D); }
+        else { if (f32) LIMSW(X3F); else LIMSW(X3D); }
+#undef X2D
+#undef X2F
+#undef X3D
+#undef X3F
+#undef TVDX_L
       } else {
         const dim3 gr = g->grid_tm[d];
         nbt = (long long)gr.x * gr.y * gr.z;
-#define TVDM(NDV, RT, DV) hipLaunchKernelGGL((k_tvd_m<NDV, RT, DV>), gr, g->block, 0, g->stream, g->geo, s, wct, win, wout, g->sc, g->part, g->pstride, g->seglen)
-        if (g->cfg.ndim == 2) { if (f32) TVDM(2, float, 1); else TVDM(2, double, 1); }
-        else if (d == 1) { if (f32) TVDM(3, float, 1); else TVDM(3, double, 1); }
-        else { if (f32) TVDM(3, float, 2); else TVDM(3, double, 2); }
-#undef TVDM
+#define TVDM_L(NDV, RT, DV, LV) hipLaunchKernelGGL((k_tvd_m<NDV, RT, DV, LV>), gr, g->block, 0, g->stream, g->geo, s, wct, win, wout, g->sc, g->part, g->pstride, g->seglen)
+#define M21D(LV) TVDM_L(2, double, 1, LV)
+#define M21F(LV) TVDM_L(2, float, 1, LV)
+#define M31D(LV) TVDM_L(3, double, 1, LV)
+#define M31F(LV) TVDM_L(3, float, 1, LV)
+#define M32D(LV) TVDM_L(3, double, 2, LV)
+#define M32F(LV) TVDM_L(3, float, 2, LV)
+        if (g->cfg.ndim == 2) { if (f32) LIMSW(M21F); else LIMSW(M21D); }
+        else if (d == 1) { if (f32) LIMSW(M31F); else LIMSW(M31D); }
+        else { if (f32) LIMSW(M32F); else LIMSW(M32D); }
+#undef M21D
+#undef M21F
+#undef M31D
+#undef M31F
+#undef M32D
+#undef M32F
+#undef TVDM_L
       }
+#undef LIMSW
       if ((rc = launch_check(g, "k_tvd_sweep"))) return rc;
     }
     if (courant) {
