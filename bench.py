@@ -44,6 +44,17 @@ NW = 6
 # read w, write w) + tvdlimit (read wCT, read w, write w) = 6*nw*8
 B_ALG_TVD_SWEEP = 6 * NW * 8
 HBM_PEAK_GBS = 8000.0
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic.json")
+
+
+def measured_traffic(kernel):
+    """HBM bytes per launch of a kernel from the committed counter runs
+    (FETCH_SIZE and WRITE_SIZE, calibrated on a 1 GiB copy), or None."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            return json.load(f)[kernel]["bytes_per_launch"]
+    except (OSError, KeyError, ValueError):
+        return None
 
 
 def cpu_baseline(w0, coords, seconds_budget=30.0):
@@ -158,7 +169,7 @@ def main():
         "nerror": [int(v) for v in sc["nerror"]],
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": None, "kernel": "k_tvd_sweep",
+                     "traffic": measured_traffic("k_tvd_sweep"), "kernel": "k_tvd_sweep",
                      "kernel_avg_ms": round(k_ms, 5), "kernel_launches": k_cnt,
                      "b_alg_per_cell": B_ALG_TVD_SWEEP},
         "other_kernels_avg_ms": {k: round(v[0], 5) for k, v in other.items()},
@@ -244,12 +255,16 @@ def tvdlf_2048(w0, dxs, steps):
     ref = None
     for prec in ("fp64", "fp32bulk"):
         el, w, kt = timed_run([NX, NY], "mhd", dxs, w0, method, prec, steps,
-                              ("tvdlf_pred", "tvdlf_corr", "bound", "reduce"))
+                              ("tvdlf_fused", "reduce"))
         rate = NX * NY * steps / el
+        ksweep = kt["tvdlf_fused"]
         r = {"mcell_updates_per_s": round(rate / 1e6, 2), "ms_per_step": round(el / steps * 1e3, 4),
              "effective_hbm_gbs": round(480 * rate / 1e9, 1),
              "effective_hbm_frac": round(480 * rate / 1e9 / HBM_PEAK_GBS, 4),
-             "kernel_avg_ms": kt}
+             "kernel_avg_ms": kt,
+             "sweep_kernel_effective_gbs": round(240 * NX * NY / (ksweep * 1e-3) / 1e9, 1) if ksweep else None}
+        if prec == "fp64":
+            r["sweep_kernel_traffic_bytes"] = measured_traffic("k_tvdlf_fused")
         if ref is None:
             ref = w
         else:
