@@ -748,7 +748,7 @@ template <bool MHD, class R, int D, int LIM>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(VACGPU_FUSED_WAVES))) void k_tvdlf_fused(
     Geo g, Sel s, BC bc, const double* __restrict__ w, double* __restrict__ wout,
     Scal* __restrict__ sc, double* __restrict__ part, long long pstride, int seglen,
-    unsigned* __restrict__ ctr, double courantpar) {
+    unsigned* __restrict__ ctr, double courantpar, int step_end) {
   __shared__ double sm[8];
   __shared__ int last;
   const int seg = (int)(blockIdx.y * blockDim.y + threadIdx.y);
@@ -788,6 +788,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(VACGPU_FUSE
         const double cc = courantpar / cm;
         sc->dtcourant[D] = cc < sc->dtcourant[D] ? cc : sc->dtcourant[D];
       }
+    }
+    if (step_end) {  // the last sweep of a device-loop step: it and t advance (src/vac.t:97-98)
+      sc->it += 1;
+      sc->t = sc->t + dt;
     }
     atomicExch(ctr, 0u);
   }
@@ -1191,20 +1195,11 @@ __global__ void k_fin_smallp(Scal* sc, const double* part, long long pstride, lo
 
 // getdt_courant's fold (src/vac.t:690-719) and the main loop's tmaxexact
 // clip (src/vac.t:83); dtpar > 0 replaces the Courant step.
-__global__ void k_fin_getdt(Scal* sc, const double* part, long long pstride, long long nb, int nd,
-                            double courantpar, double dtpar, int tmaxexact) {
-  __shared__ double sm[16];
-  double dtc[3] = {kBig, kBig, kBig};
-  for (int d = 0; d < nd; ++d) dtc[d] = sc->dtcourant[d];
-  double cmd[3] = {0.0, 0.0, 0.0};
-  for (int d = 0; d < nd; ++d) {
-    if (dtc[d] >= kBig && dtpar <= 0.0) {
-      cmd[d] = fold(part + (SLOT_DX + d) * pstride, nb, true, sm);
-      __syncthreads();
-    }
-  }
-  __syncthreads();
-  if (threadIdx.x == 0) {
+// thread 0: dt from the lagged dtcourant and, where it is missing, the
+// cell-centre maxima cmd[] (src/vac.t:690-719), then the tmaxexact clip
+__device__ __forceinline__ void getdt_finish(Scal* sc, const double* dtc, const double* cmd, int nd,
+                                             double courantpar, double dtpar, int tmaxexact) {
+  {
     double dt;
     if (dtpar > 0.0) {
       dt = dtpar;
@@ -1224,6 +1219,82 @@ __global__ void k_fin_getdt(Scal* sc, const double* part, long long pstride, lon
     }
     sc->dt = dt;
   }
+}
+
+__global__ void k_fin_getdt(Scal* sc, const double* part, long long pstride, long long nb, int nd,
+                            double courantpar, double dtpar, int tmaxexact) {
+  __shared__ double sm[16];
+  double dtc[3] = {kBig, kBig, kBig};
+  for (int d = 0; d < nd; ++d) dtc[d] = sc->dtcourant[d];
+  double cmd[3] = {0.0, 0.0, 0.0};
+  for (int d = 0; d < nd; ++d) {
+    if (dtc[d] >= kBig && dtpar <= 0.0) {
+      cmd[d] = fold(part + (SLOT_DX + d) * pstride, nb, true, sm);
+      __syncthreads();
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) getdt_finish(sc, dtc, cmd, nd, courantpar, dtpar, tmaxexact);
+}
+
+// getdt_courant in one block for the steps after the first of a device loop,
+// where the pressure floor is set and every dimension has a lagged
+// dtcourant: then this is k_fin_getdt without partials.  Should either be
+// missing (a pressure floor that stays negative, a sweep whose Courant
+// maximum was below smalldouble), the block computes it itself over the mesh
+// -- slow, but exactly what the multi-block passes compute.
+template <int ND, bool MHD>
+__global__ __launch_bounds__(1024) void k_getdt_lagged(Geo g, const double* __restrict__ w, double gamma,
+                                                      Scal* sc, double smallpcoeff, double courantpar,
+                                                      double dtpar, int tmaxexact) {
+  using L = Layout<ND, MHD>;
+  __shared__ double sm[16];
+  const long long ncell = (long long)g.n[0] * g.n[1] * (ND == 3 ? g.n[2] : 1);
+  auto cell = [&](long long t, int* c) {
+    c[0] = 2 + (int)(t % g.n[0]);
+    c[1] = 2 + (int)((t / g.n[0]) % g.n[1]);
+    c[2] = ND == 3 ? 2 + (int)(t / ((long long)g.n[0] * g.n[1])) : 0;
+    return g.off + c[0] + c[1] * g.s[1] + c[2] * g.s[2];
+  };
+  if (sc->smallp < 0.0) {  // block-uniform
+    double mn = INFINITY, mx = -INFINITY;
+    for (long long t = threadIdx.x; t < ncell; t += blockDim.x) {
+      int c[3];
+      double u[L::nw];
+      load<L::nw>(w, g.ps, cell(t, c), u);
+      const double p = pthermal_raw<ND, MHD>(u, gamma);
+      mn = p < mn ? p : mn;
+      mx = mx < p ? p : mx;
+    }
+    mn = block_reduce<false>(mn, sm);
+    __syncthreads();
+    mx = block_reduce<true>(mx, sm);
+    if (threadIdx.x == 0) {
+      double sp = mn * smallpcoeff;
+      if (MHD && sp < 0.0) sp = mx * smallpcoeff;
+      sc->smallp = sp;
+    }
+    __syncthreads();
+  }
+  const double smallp = sc->smallp;
+  double dtc[3] = {kBig, kBig, kBig};
+  for (int d = 0; d < ND; ++d) dtc[d] = sc->dtcourant[d];
+  double cmd[3] = {0.0, 0.0, 0.0};
+  for (int d = 0; d < ND; ++d) {
+    if (dtc[d] >= kBig && dtpar <= 0.0) {  // block-uniform
+      double v = -INFINITY;
+      for (long long t = threadIdx.x; t < ncell; t += blockDim.x) {
+        int c[3];
+        double u[L::nw];
+        load<L::nw>(w, g.ps, cell(t, c), u);
+        const double q = cmax<ND, MHD, double>(u, d, gamma, smallp) / g.dx[d][c[d]];
+        v = v < q ? q : v;
+      }
+      cmd[d] = block_reduce<true>(v, sm);
+      __syncthreads();
+    }
+  }
+  if (threadIdx.x == 0) getdt_finish(sc, dtc, cmd, ND, courantpar, dtpar, tmaxexact);
 }
 
 // dtcourant update of the TVDLF corrector (src/vactvdlf.t:184-201)

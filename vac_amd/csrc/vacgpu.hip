@@ -56,6 +56,8 @@ struct vacgpu_grid {
   dim3 grid_fs, block_fs;
   double* stage = nullptr;  // pinned staging of put_state / get_state
   unsigned* ctr = nullptr;  // block counter of the fused sweeps (zero between launches)
+  // device loop: the last fused sweep of a step also advances it and t
+  bool fold_step_end = false, step_end_done = false;
   size_t stage_bytes = 0;
   long long nbound = 0, slab[3] = {0, 0, 0};
   int cur = 0, old = 0;
@@ -254,6 +256,19 @@ void set_fused_shape(vacgpu_grid* g, int seglen) {
   g->grid_fs = dim3(strips, (segs + 3) / 4, 1);
 }
 
+// getdt of the steps after the first in a device loop: one block, lagged
+// dtcourant (see k_getdt_lagged)
+int getdt_lagged(vacgpu_grid* g, const vacgpu_method* m, double dtpar, int tmaxexact) {
+  Timed t(g, KF_RED);
+  DISPATCH(g, {
+    (void)sizeof(R);
+    hipLaunchKernelGGL((k_getdt_lagged<ND, MHD>), dim3(1), dim3(1024), 0, g->stream, g->geo,
+                       g->buf[g->cur], g->cfg.eqpar[0], g->sc, m->smallpcoeff, m->courantpar, dtpar,
+                       tmaxexact);
+  });
+  return launch_check(g, "k_getdt_lagged");
+}
+
 int sweep_device(vacgpu_grid* g, const vacgpu_method* m, int d, int firstsweep, int lastsweep) {
   if (firstsweep) g->old = g->cur;
   const int outi = other_buffer(g);
@@ -278,7 +293,9 @@ int sweep_device(vacgpu_grid* g, const vacgpu_method* m, int d, int firstsweep, 
       Timed t(g, KF_FUSED);
       const bool f32 = g->cfg.precision == VACGPU_FP32BULK;
       const int lim = m->typelimiter[0];
-#define FUSED_L(MV, RT, DV, LV) hipLaunchKernelGGL((k_tvdlf_fused<MV, RT, DV, LV>), g->grid_fs, g->block_fs, 0, g->stream, g->geo, s, g->bc, win, wout, g->sc, g->part, g->pstride, g->fs_seglen, g->ctr, m->courantpar)
+      const int step_end = (g->fold_step_end && lastsweep) ? 1 : 0;
+      if (step_end) g->step_end_done = true;
+#define FUSED_L(MV, RT, DV, LV) hipLaunchKernelGGL((k_tvdlf_fused<MV, RT, DV, LV>), g->grid_fs, g->block_fs, 0, g->stream, g->geo, s, g->bc, win, wout, g->sc, g->part, g->pstride, g->fs_seglen, g->ctr, m->courantpar, step_end)
 #define FUSED(MV, RT, DV) do { if (lim == LIM_MINMOD) FUSED_L(MV, RT, DV, LIM_MINMOD); else if (lim == LIM_WOODWARD) FUSED_L(MV, RT, DV, LIM_WOODWARD); else FUSED_L(MV, RT, DV, LIM_SUPERBEE); } while (0)
       if (mhd) {
         if (f32) { if (d == 0) FUSED(true, float, 0); else FUSED(true, float, 1); }
@@ -750,16 +767,24 @@ int vacgpu_run(vacgpu_grid* g, const vacgpu_method* m, int nsteps, int itmax, do
   int done = 0;
   for (;;) {
     if (t_hist) t_hist[done] = t;
-    if ((rc = getdt_device(g, m, dtpar, tmaxexact))) return rc;
+    if (done == 0) rc = getdt_device(g, m, dtpar, tmaxexact);
+    else rc = getdt_lagged(g, m, dtpar, tmaxexact);
+    if (rc) return rc;
     double dt = 1.0;
     if (sync_each) {
       if ((rc = read_scalars(g))) return rc;
       dt = g->sc_host->dt;
     }
     if (it >= itmax || t >= tmax || dt < 1.0e-99 || done >= nsteps) break;
-    if ((rc = advance_device(g, m, it))) return rc;
-    hipLaunchKernelGGL(k_step_end, dim3(1), dim3(1), 0, g->stream, g->sc);
-    if ((rc = launch_check(g, "k_step_end"))) return rc;
+    g->fold_step_end = true;
+    g->step_end_done = false;
+    rc = advance_device(g, m, it);
+    g->fold_step_end = false;
+    if (rc) return rc;
+    if (!g->step_end_done) {
+      hipLaunchKernelGGL(k_step_end, dim3(1), dim3(1), 0, g->stream, g->sc);
+      if ((rc = launch_check(g, "k_step_end"))) return rc;
+    }
     it += 1;
     done += 1;
     if (sync_each) {
