@@ -40,9 +40,17 @@ STEPS = 200
 SEED = 20251203
 GAMMA = 5.0 / 3.0
 NW = 6
-# algorithmic bytes per cell for one TVD sweep launch: centdiff (read wCT,
-# read w, write w) + tvdlimit (read wCT, read w, write w) = 6*nw*8
-B_ALG_TVD_SWEEP = 6 * NW * 8
+# algorithmic bytes per cell of the TVD sweep launches, in VAC's dataflow
+# (every substep reads its inputs and writes its output once, FP64):
+# centdiff (read wCT, read w, write w) + tvdlimit (read wCT, read w, write w)
+# = 6*nw*8 per launch; the first sweep of a step also does the half-source
+# step (read w, write wCT: 2*nw*8) and an x sweep that ends a step (every
+# other step) the first half of the Ryu step (read wold, read w, write wt:
+# 3*nw*8), so per launch on average 6*nw*8 + 2*nw*8/2 + 3*nw*8/4
+B_ALG_TVD_SWEEP = 6 * NW * 8 + (2 * NW * 8) // 2 + (3 * NW * 8) // 4
+# a whole TVD step (SURVEY-style count: wold copy 2nw, half source 2nw,
+# 6nw per sweep, Ryu 6nw) = (6*ndim + 10)*nw*8
+B_ALG_TVD_STEP = (6 * 2 + 10) * NW * 8
 HBM_PEAK_GBS = 8000.0
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic.json")
 
@@ -173,6 +181,8 @@ def main():
                      "kernel_avg_ms": round(k_ms, 5), "kernel_launches": k_cnt,
                      "b_alg_per_cell": B_ALG_TVD_SWEEP},
         "other_kernels_avg_ms": {k: round(v[0], 5) for k, v in other.items()},
+        "step_effective_hbm_frac": round(B_ALG_TVD_STEP * NX * NY / (ms_per_step * 1e-3) / 1e9
+                                         / HBM_PEAK_GBS, 4),
     }
     if not args.no_tvdlf:
         out["tvdlf_2048"] = tvdlf_2048(w0, dxs, min(args.steps, 100))

@@ -114,3 +114,39 @@ def test_fused_fills_ghost_cells(bc):
         g.close()
     print(f"case fused ghosts {bc}", flush=True)
     assert np.array_equal(out[0], out[1])
+
+
+TVD_CASES = [
+    # (nx, ny, bc, limiter, precision, steps)
+    (61, 37, "periodic", "woodward", "fp64", 9),
+    (64, 40, "cont", "superbee", "fp64", 9),
+    (130, 67, ["periodic", "periodic", "cont", "cont"], "minmod", "fp64", 6),
+    (45, 90, "periodic", "woodward", "fp32bulk", 8),
+]
+
+
+@pytest.mark.parametrize("case", TVD_CASES, ids=[f"{c[0]}x{c[1]}-{c[3]}-{c[4]}" for c in TVD_CASES])
+def test_tvd_folded_sources_match(case):
+    # the TVD sweeps fold the first-sweep half source and the first half of
+    # the Ryu step; the whole ixG array must equal the separate-kernel path,
+    # and the FP64 result the oracle's
+    nx, ny, bc, limiter, precision, steps = case
+    w0, coords = random_fourier((nx, ny), 21 + nx, 5 / 3)
+    dxs = [widths_1d(c) for c in coords]
+    method = vacgpu.MethodSpec(typefull="tvd", limiter=limiter, courantpar=0.8)
+    print(f"case tvd fold {case}", flush=True)
+    out = []
+    for fused in (True, False):
+        g = vacgpu.Grid([nx, ny], "mhd", 5 / 3, dxs, bc=bc, precision=precision)
+        g.set_kernels(fused)
+        g.put_mesh(w0)
+        n, t = g.run(method, steps, record_t=True)
+        out.append((g.get_state(), t))
+        g.close()
+    assert np.array_equal(out[0][1], out[1][1])
+    assert np.array_equal(out[0][0], out[1][0])
+    if precision == "fp64":
+        p = vacref.make_params(2, "mhd", [nx, ny], 5 / 3, typefull="tvd", limiter=limiter, bc=bc)
+        wr, tr, _ = vacref.run(p, w0, dxs, steps)
+        assert np.array_equal(out[0][1], tr)
+        assert np.array_equal(out[0][0][:, 2:-2, 2:-2], wr)

@@ -817,6 +817,51 @@ __global__ __launch_bounds__(256) void k_tvd_halfsrc(Geo g, const double* __rest
   for (int iw = 0; iw < NW; ++iw) w1[iw * g.ps + cl.idx] = out[iw];
 }
 
+// Sources folded into the TVD sweeps (2D, one boundary type per face for
+// all variables).  SRC = 1, first sweep: the sweep reads wCT = w + (dt/2)
+// S_Powell(w) computed on the fly at the mesh cell whose value the ghost rule
+// copies into the cell (what k_tvd_halfsrc + getboundary produce).  SRC = 2,
+// last sweep: the sweep also writes the first half of Ryu's source step,
+// wt = (wold + wnew + dt S_Powell(wold))/2, at every cell it updates (what
+// k_ryu_a produces).
+__device__ __forceinline__ int image_of(const Geo& g, const BC& bc, int k, int c) {
+  const int n = g.n[k];
+  const int lo = bc.t[2 * k][0] ? c + n : 2, hi = bc.t[2 * k + 1][0] ? c - n : n + 1;
+  return c < 2 ? lo : (c > n + 1 ? hi : c);
+}
+
+template <int ND>
+__device__ __forceinline__ void wct_halfsrc(const Geo& g, const BC& bc, const double* __restrict__ w,
+                                            const int* craw, double dt, double* u) {
+  constexpr int NW = Layout<ND, true>::nw;
+  int c[3] = {0, 0, 0};
+  long long idx = g.off;
+#pragma unroll
+  for (int k = 0; k < ND; ++k) {
+    c[k] = image_of(g, bc, k, craw[k]);
+    idx += c[k] * g.s[k];
+  }
+  double uc[NW];
+  load<NW>(w, g.ps, idx, uc);
+#pragma unroll
+  for (int iw = 0; iw < NW; ++iw) u[iw] = uc[iw];
+  powell<ND>(uc, divb_at<ND>(g, w, idx, c), dt / 2, u);
+}
+
+template <int ND>
+__device__ __forceinline__ void ryu_first_half(const Geo& g, const double* __restrict__ wold,
+                                               double* __restrict__ wt, long long idx, const int* c,
+                                               double dt, const double* wnew) {
+  constexpr int NW = Layout<ND, true>::nw;
+  double uo[NW], t[NW];
+  load<NW>(wold, g.ps, idx, uo);
+#pragma unroll
+  for (int iw = 0; iw < NW; ++iw) t[iw] = wnew[iw];
+  powell<ND>(uo, divb_at<ND>(g, wold, idx, c), dt, t);
+#pragma unroll
+  for (int iw = 0; iw < NW; ++iw) wt[iw * g.ps + idx] = 0.5 * (uo[iw] + t[iw]);
+}
+
 // ---- TVD sweep.  For cell i: the centred flux difference of wCT (centdiff,
 // src/vaccd.t:78-136) followed by the Roe-type dissipation of tvdlimit2
 // (src/vactvd.t:86-206, getphi 'roe' :244-302) through the faces i-1/2 and
@@ -894,12 +939,14 @@ constexpr int TX_STRIDE = 61;  // cells updated per 64-lane wavefront along x
 
 // x sweep: lane L of a wavefront owns face J = x0 - 2 + L (between cells J and
 // J+1) and updates cell J for L in [2, 62].
-template <int ND, class R, int LIM = -1>
-__global__ __launch_bounds__(256) void k_tvd_x(Geo g, Sel s, const double* __restrict__ wct,
+template <int ND, class R, int LIM = -1, int SRC = 0>
+__global__ __launch_bounds__(256) void k_tvd_x(Geo g, Sel s, BC bc, const double* __restrict__ wct,
                                               const double* __restrict__ win,
                                               double* __restrict__ wout,
                                               const Scal* __restrict__ sc,
-                                              double* __restrict__ part, long long pstride) {
+                                              double* __restrict__ part, long long pstride,
+                                              const double* __restrict__ wold,
+                                              double* __restrict__ wt) {
   using L = Layout<ND, true>;
   constexpr int NW = L::nw;
   __shared__ double sm[8];
@@ -914,8 +961,14 @@ __global__ __launch_bounds__(256) void k_tvd_x(Geo g, Sel s, const double* __res
   const int Jc = J < g.n[0] + 2 ? J : g.n[0] + 2;
   const long long base = g.off + (long long)yc * g.s[1] + (long long)z * g.s[2];
   double uL[NW], uR[NW];
-  load<NW>(wct, g.ps, base + Jc, uL);
-  load<NW>(wct, g.ps, base + Jc + 1, uR);
+  if constexpr (SRC == 1) {
+    const int cl[3] = {Jc, yc, z}, cr[3] = {Jc + 1, yc, z};
+    wct_halfsrc<ND>(g, bc, win, cl, dt, uL);
+    wct_halfsrc<ND>(g, bc, win, cr, dt, uR);
+  } else {
+    load<NW>(wct, g.ps, base + Jc, uL);
+    load<NW>(wct, g.ps, base + Jc + 1, uR);
+  }
   TvdFace<ND, R> f;
   double c0, c1;
   tvd_face<ND, R>(uL, uR, 0, s, smallp, dt, g.dx[0][Jc] + g.dx[0][Jc + 1], f, c0, c1);
@@ -957,6 +1010,10 @@ __global__ __launch_bounds__(256) void k_tvd_x(Geo g, Sel s, const double* __res
     tvd_dissipate<ND, R>(rtm, phim, f.rt, phi, 0, s, R(d0 + dm), R(dp + d0), d0, out);
 #pragma unroll
     for (int iw = 0; iw < NW; ++iw) wout[iw * g.ps + idx] = out[iw];
+    if constexpr (SRC == 2) {
+      const int c[3] = {i, yc, z};
+      ryu_first_half<ND>(g, wold, wt, idx, c, dt, out);
+    }
   }
   const double b0 = block_reduce<true>(c0, sm);
   if (threadIdx.x == 0 && threadIdx.y == 0) part[SLOT_C0 * pstride + block_linear()] = b0;
@@ -965,13 +1022,14 @@ __global__ __launch_bounds__(256) void k_tvd_x(Geo g, Sel s, const double* __res
 }
 
 // y / z sweep: thread (x, segment) marches through cells s0..s1 along d.
-template <int ND, class R, int D, int LIM = -1>
-__global__ __launch_bounds__(256) void k_tvd_m(Geo g, Sel s, const double* __restrict__ wct,
+template <int ND, class R, int D, int LIM = -1, int SRC = 0>
+__global__ __launch_bounds__(256) void k_tvd_m(Geo g, Sel s, BC bc, const double* __restrict__ wct,
                                               const double* __restrict__ win,
                                               double* __restrict__ wout,
                                               const Scal* __restrict__ sc,
                                               double* __restrict__ part, long long pstride,
-                                              int seglen) {
+                                              int seglen, const double* __restrict__ wold,
+                                              double* __restrict__ wt) {
   constexpr int d = D;
   using L = Layout<ND, true>;
   constexpr int NW = L::nw;
@@ -1000,20 +1058,31 @@ __global__ __launch_bounds__(256) void k_tvd_m(Geo g, Sel s, const double* __res
       if (d == 1) return (v * dxd[c]) * dxt;
       return (v * dxt) * dxd[c];
     };
+    // wCT at cell c along d (through the half source on the first sweep)
+    auto get_ct = [&](int c, double* u) {
+      if constexpr (SRC == 1) {
+        int cc[3] = {x, 0, 0};
+        cc[d] = c;
+        if (ND == 3) cc[other] = t;
+        wct_halfsrc<ND>(g, bc, win, cc, dt, u);
+      } else {
+        load<NW>(wct, g.ps, base + (long long)c * sd, u);
+      }
+    };
     double uL[NW], uR[NW];
     TvdFace<ND, R> fm, f0, fp;  // faces i-1, i, i+1
     R phim[NW], phi0[NW];
     double Fm[NW], F0[NW], fCm[NW];
     double a, b;
     // prologue: faces s0-2, s0-1, s0; centred fluxes at cells s0-1, s0
-    load<NW>(wct, g.ps, base + (long long)(s0 - 2) * sd, uL);
-    load<NW>(wct, g.ps, base + (long long)(s0 - 1) * sd, uR);
+    get_ct(s0 - 2, uL);
+    get_ct(s0 - 1, uR);
     tvd_face<ND, R>(uL, uR, d, s, smallp, dt, dxd[s0 - 2] + dxd[s0 - 1], fm, a, b);
     c0 = c0 < a ? a : c0; c1 = c1 < b ? b : c1;
     fluxes<ND, true, R>(uR, d, s.gamma, smallp, Fm);
 #pragma unroll
     for (int iw = 0; iw < NW; ++iw) uL[iw] = uR[iw];
-    load<NW>(wct, g.ps, base + (long long)s0 * sd, uR);
+    get_ct(s0, uR);
     tvd_face<ND, R>(uL, uR, d, s, smallp, dt, dxd[s0 - 1] + dxd[s0], f0, a, b);
     c0 = c0 < a ? a : c0; c1 = c1 < b ? b : c1;
     fluxes<ND, true, R>(uR, d, s.gamma, smallp, F0);
@@ -1021,7 +1090,7 @@ __global__ __launch_bounds__(256) void k_tvd_m(Geo g, Sel s, const double* __res
     for (int iw = 0; iw < NW; ++iw) fCm[iw] = 0.5 * (Fm[iw] + F0[iw]);  // face s0-1
 #pragma unroll
     for (int iw = 0; iw < NW; ++iw) uL[iw] = uR[iw];
-    load<NW>(wct, g.ps, base + (long long)(s0 + 1) * sd, uR);
+    get_ct(s0 + 1, uR);
     tvd_face<ND, R>(uL, uR, d, s, smallp, dt, dxd[s0] + dxd[s0 + 1], fp, a, b);
     c0 = c0 < a ? a : c0; c1 = c1 < b ? b : c1;
     // phi at face s0-1 from faces s0-2, s0-1, s0
@@ -1041,7 +1110,7 @@ __global__ __launch_bounds__(256) void k_tvd_m(Geo g, Sel s, const double* __res
       // face i+1 from w(i+1), w(i+2)
 #pragma unroll
       for (int iw = 0; iw < NW; ++iw) uL[iw] = uR[iw];
-      load<NW>(wct, g.ps, base + (long long)(i + 2) * sd, uR);
+      get_ct(i + 2, uR);
       tvd_face<ND, R>(uL, uR, d, s, smallp, dt, dxd[i + 1] + dxd[i + 2], fp, a, b);
       c0 = c0 < a ? a : c0; c1 = c1 < b ? b : c1;
 #pragma unroll
@@ -1056,6 +1125,12 @@ __global__ __launch_bounds__(256) void k_tvd_m(Geo g, Sel s, const double* __res
       tvd_dissipate<ND, R>(fm.rt, phim, f0.rt, phi0, d, s, R(d0 + dm), R(dp + d0), d0, out);
 #pragma unroll
       for (int iw = 0; iw < NW; ++iw) wout[iw * g.ps + idx] = out[iw];
+      if constexpr (SRC == 2) {
+        int cc[3] = {x, 0, 0};
+        cc[d] = i;
+        if (ND == 3) cc[other] = t;
+        ryu_first_half<ND>(g, wold, wt, idx, cc, dt, out);
+      }
       if (i == s1) break;
       // shift the window
 #pragma unroll

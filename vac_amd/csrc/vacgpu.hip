@@ -52,6 +52,7 @@ struct vacgpu_grid {
   // fused 2D TVDLF sweeps: eligible when every face has one boundary type
   // for all variables; use_fused selects them (default on)
   bool fused_ok = false, use_fused = true;
+  bool bc_uniform = false;  // one boundary type per face for all variables, n >= 4
   int fs_seglen = 16;
   dim3 grid_fs, block_fs;
   double* stage = nullptr;  // pinned staging of put_state / get_state
@@ -269,6 +270,13 @@ int getdt_lagged(vacgpu_grid* g, const vacgpu_method* m, double dtpar, int tmaxe
   return launch_check(g, "k_getdt_lagged");
 }
 
+// the limiter type shared by every variable / characteristic wave, or -1
+int common_limiter(const vacgpu_grid* g, const vacgpu_method* m) {
+  for (int iw = 1; iw < g->nw; ++iw)
+    if (m->typelimiter[iw] != m->typelimiter[0]) return -1;
+  return m->typelimiter[0];
+}
+
 int sweep_device(vacgpu_grid* g, const vacgpu_method* m, int d, int firstsweep, int lastsweep) {
   if (firstsweep) g->old = g->cur;
   const int outi = other_buffer(g);
@@ -339,8 +347,17 @@ int sweep_device(vacgpu_grid* g, const vacgpu_method* m, int d, int firstsweep, 
       if ((rc = launch_check(g, "k_fin_courant_tvdlf"))) return rc;
     }
   } else {
+    // 2D grids with one boundary type per face fold the half source into the
+    // first sweep and the first half of the Ryu step into the last one
+    // (one limiter for all characteristic waves is fixed at compile time)
+    const int lim = common_limiter(g, m);
+    const bool fold = g->cfg.ndim == 2 && g->bc_uniform && src && g->use_fused && lim >= 0;
+    // (the Ryu half folds into the x sweep only: in the marching y sweep it
+    // would cost occupancy)
+    const int srcmode = (fold && firstsweep && !lastsweep) ? 1
+                        : ((fold && lastsweep && !firstsweep && d == 0) ? 2 : 0);
     const double* wct = win;
-    if (firstsweep && src) {
+    if (firstsweep && src && srcmode != 1) {
       {
         Timed t(g, KF_RYU);
         if (g->cfg.ndim == 2)
@@ -356,41 +373,56 @@ int sweep_device(vacgpu_grid* g, const vacgpu_method* m, int d, int firstsweep, 
     {
       Timed t(g, KF_TVD);
       const bool f32 = g->cfg.precision == VACGPU_FP32BULK;
-      // one limiter type for every characteristic wave: fixed at compile time
-      bool one = true;
-      for (int il = 1; il < g->nw; ++il) one &= m->typelimiter[il] == m->typelimiter[0];
-      const int lim = one ? m->typelimiter[0] : -1;
 #define LIMSW(X) do { if (lim == LIM_MINMOD) X(LIM_MINMOD); else if (lim == LIM_WOODWARD) X(LIM_WOODWARD); else if (lim == LIM_SUPERBEE) X(LIM_SUPERBEE); else X(-1); } while (0)
+#define LIMSW3(X) do { if (lim == LIM_MINMOD) X(LIM_MINMOD); else if (lim == LIM_WOODWARD) X(LIM_WOODWARD); else X(LIM_SUPERBEE); } while (0)
       if (d == 0) {
         const dim3 gr = g->grid_tx;
         nbt = (long long)gr.x * gr.y * gr.z;
-#define TVDX_L(NDV, RT, LV) hipLaunchKernelGGL((k_tvd_x<NDV, RT, LV>), gr, g->block, 0, g->stream, g->geo, s, wct, win, wout, g->sc, g->part, g->pstride)
-#define X2D(LV) TVDX_L(2, double, LV)
-#define X2F(LV) TVDX_L(2, float, LV)
-#define X3D(LV) TVDX_L(3, double, LV)
-#define X3F(LV) TVDX_L(3, float, LV)
-        if (g->cfg.ndim == 2) { if (f32) LIMSW(X2F); else LIMSW(X2D); }
-        else { if (f32) LIMSW(X3F); else LIMSW(X3D); }
+#define TVDX_L(NDV, RT, LV, SV) hipLaunchKernelGGL((k_tvd_x<NDV, RT, LV, SV>), gr, g->block, 0, g->stream, g->geo, s, g->bc, wct, win, wout, g->sc, g->part, g->pstride, wold, g->w1)
+#define X2D(LV) TVDX_L(2, double, LV, 0)
+#define X2F(LV) TVDX_L(2, float, LV, 0)
+#define X3D(LV) TVDX_L(3, double, LV, 0)
+#define X3F(LV) TVDX_L(3, float, LV, 0)
+#define X2D1(LV) TVDX_L(2, double, LV, 1)
+#define X2F1(LV) TVDX_L(2, float, LV, 1)
+#define X2D2(LV) TVDX_L(2, double, LV, 2)
+#define X2F2(LV) TVDX_L(2, float, LV, 2)
+        if (g->cfg.ndim == 2) {
+          if (srcmode == 1) { if (f32) LIMSW3(X2F1); else LIMSW3(X2D1); }
+          else if (srcmode == 2) { if (f32) LIMSW3(X2F2); else LIMSW3(X2D2); }
+          else { if (f32) LIMSW(X2F); else LIMSW(X2D); }
+        } else { if (f32) LIMSW(X3F); else LIMSW(X3D); }
 #undef X2D
 #undef X2F
 #undef X3D
 #undef X3F
+#undef X2D1
+#undef X2F1
+#undef X2D2
+#undef X2F2
 #undef TVDX_L
       } else {
         const dim3 gr = g->grid_tm[d];
         nbt = (long long)gr.x * gr.y * gr.z;
-#define TVDM_L(NDV, RT, DV, LV) hipLaunchKernelGGL((k_tvd_m<NDV, RT, DV, LV>), gr, g->block, 0, g->stream, g->geo, s, wct, win, wout, g->sc, g->part, g->pstride, g->seglen)
-#define M21D(LV) TVDM_L(2, double, 1, LV)
-#define M21F(LV) TVDM_L(2, float, 1, LV)
-#define M31D(LV) TVDM_L(3, double, 1, LV)
-#define M31F(LV) TVDM_L(3, float, 1, LV)
-#define M32D(LV) TVDM_L(3, double, 2, LV)
-#define M32F(LV) TVDM_L(3, float, 2, LV)
-        if (g->cfg.ndim == 2) { if (f32) LIMSW(M21F); else LIMSW(M21D); }
+#define TVDM_L(NDV, RT, DV, LV, SV) hipLaunchKernelGGL((k_tvd_m<NDV, RT, DV, LV, SV>), gr, g->block, 0, g->stream, g->geo, s, g->bc, wct, win, wout, g->sc, g->part, g->pstride, g->seglen, wold, g->w1)
+#define M21D(LV) TVDM_L(2, double, 1, LV, 0)
+#define M21F(LV) TVDM_L(2, float, 1, LV, 0)
+#define M21D1(LV) TVDM_L(2, double, 1, LV, 1)
+#define M21F1(LV) TVDM_L(2, float, 1, LV, 1)
+#define M31D(LV) TVDM_L(3, double, 1, LV, 0)
+#define M31F(LV) TVDM_L(3, float, 1, LV, 0)
+#define M32D(LV) TVDM_L(3, double, 2, LV, 0)
+#define M32F(LV) TVDM_L(3, float, 2, LV, 0)
+        if (g->cfg.ndim == 2) {
+          if (srcmode == 1) { if (f32) LIMSW3(M21F1); else LIMSW3(M21D1); }
+          else { if (f32) LIMSW(M21F); else LIMSW(M21D); }
+        }
         else if (d == 1) { if (f32) LIMSW(M31F); else LIMSW(M31D); }
         else { if (f32) LIMSW(M32F); else LIMSW(M32D); }
 #undef M21D
 #undef M21F
+#undef M21D1
+#undef M21F1
 #undef M31D
 #undef M31F
 #undef M32D
@@ -398,6 +430,7 @@ int sweep_device(vacgpu_grid* g, const vacgpu_method* m, int d, int firstsweep, 
 #undef TVDM_L
       }
 #undef LIMSW
+#undef LIMSW3
       if ((rc = launch_check(g, "k_tvd_sweep"))) return rc;
     }
     if (courant) {
@@ -406,8 +439,9 @@ int sweep_device(vacgpu_grid* g, const vacgpu_method* m, int d, int firstsweep, 
                          g->pstride, nbt, d, m->courantpar);
       if ((rc = launch_check(g, "k_fin_courant_tvd"))) return rc;
     }
+    const bool ryu_folded = srcmode == 2;
     if (lastsweep && src) {
-      {
+      if (!ryu_folded) {
         Timed t(g, KF_RYU);
         if (g->cfg.ndim == 2)
           hipLaunchKernelGGL(k_ryu_a<2>, g->grid, g->block, 0, g->stream, g->geo, wold, wout, g->w1, g->sc);
@@ -527,10 +561,12 @@ int vacgpu_create(const vacgpu_config* cfg, const double* dx0, const double* dx1
     g->grid_tm[d] = dim3(g->grid.x, (nseg + BY - 1) / BY, nd == 3 ? geo.n[other] : 1);
   }
   // fused TVDLF: one boundary type per face for all variables
-  g->fused_ok = nd == 2 && geo.n[0] >= 4 && geo.n[1] >= 4 && (unsigned long long)geo.ps * 8ULL < (1ULL << 32);
+  g->bc_uniform = true;
+  for (int d = 0; d < nd; ++d) g->bc_uniform &= geo.n[d] >= 4;
   for (int f = 0; f < 2 * nd; ++f)
     for (int iw = 1; iw < g->nw; ++iw)
-      if (g->bc.t[f][iw] != g->bc.t[f][0]) g->fused_ok = false;
+      if (g->bc.t[f][iw] != g->bc.t[f][0]) g->bc_uniform = false;
+  g->fused_ok = nd == 2 && g->bc_uniform && (unsigned long long)geo.ps * 8ULL < (1ULL << 32);
   set_fused_shape(g, 0);
   auto count = [](dim3 v) { return (long long)v.x * v.y * v.z; };
   g->pstride = count(g->grid);
