@@ -166,11 +166,24 @@ int vacgpu_batch_get_state(vacgpu_batch* b, int slot, double* w, const int* ld);
 
 /* Replay each step from a captured hipGraph (re-captured after membership changes). */
 int vacgpu_batch_graph(vacgpu_batch* b, int enable);
-/* Autotuner: time candidate tile shapes (64 x rows, rows in {1,2,4,8}) per
- * size class of the current instances and keep the fastest; rows_out[4]
- * (may be NULL) receives the choice for cells < 128^2, < 256^2, < 512^2, larger. */
+/* Autotuner: per size class of the current instances (cells < 128^2, < 256^2,
+ * < 512^2, larger) time the candidate tile shapes and keep the fastest:
+ * 64 x rows tiles (rows in {1,2,4,8}) for predictor + corrector instances,
+ * segment lengths {8,16,32,64} for fused-sweep instances.  rows_out[4] (may
+ * be NULL) receives the tile rows; vacgpu_batch_get_tiles returns both. */
 int vacgpu_batch_tune(vacgpu_batch* b, int* rows_out);
 int vacgpu_batch_set_tiles(vacgpu_batch* b, const int* rows);
+/* Sweep kernels of the batch: fused = 1 (the default) advances instances
+ * whose faces have one boundary type for all variables, that use one
+ * limiter for all variables and (MHD) the Powell source, with the fused
+ * predictor-corrector sweep (ghost cells written by the sweep); fused = 0
+ * uses predictor + corrector tiles for every instance.  Results are bitwise
+ * the same.  seglen[4] (may be NULL) sets the fused sweeps' rows per
+ * marching segment per size class, each in [4, 256]; vacgpu_batch_tune
+ * chooses them from {8, 16, 32, 64}. */
+int vacgpu_batch_set_kernels(vacgpu_batch* b, int fused, const int* seglen);
+/* Current tile rows and fused segment lengths per size class (either may be NULL). */
+int vacgpu_batch_get_tiles(const vacgpu_batch* b, int* rows, int* seglen);
 long long vacgpu_batch_bytes_used(const vacgpu_batch* b);
 const char* vacgpu_batch_last_error(const vacgpu_batch* b);
 void vacgpu_batch_destroy(vacgpu_batch* b);

@@ -170,3 +170,43 @@ def test_autotune_keeps_results():
         wa, it, t = alone(w, dxs, nx, ny, ph, g, prec, steps=5)
         assert np.array_equal(wa, res[s][0])
     b.close()
+
+
+def test_fused_and_tile_sweeps_agree():
+    # eligible instances run the fused predictor-corrector sweep; forcing the
+    # predictor + corrector tiles for every instance must give the same bits,
+    # and a 'cont' instance and a mixed-limiter one ride along in both
+    method_mixed = vacgpu.MethodSpec(typefull="tvdlf", limiter=["minmod", "woodward"] * 4,
+                                     courantpar=0.8)
+    res = []
+    for fused in (True, False):
+        b = vacgpu.Batch(4 << 30, 16)
+        b.set_kernels(fused, [8, 16, 5, 32])
+        slots = []
+        for j, (nx, ny, ph, g, prec, seed) in enumerate(MIX[:6]):
+            w, dxs = state(nx, ny, ph, g, seed)
+            bc = "cont" if j == 2 else "periodic"
+            m = method_mixed if j == 3 else METHOD
+            slots.append(b.add(w, [nx, ny], ph, g, dxs, m, bc=bc, precision=prec, itmax=7))
+        print(f"case batch fused={fused}", flush=True)
+        for _ in range(8):
+            b.step(1)
+        b.sync()
+        res.append([(b.get_mesh(s), b.status(s)["t"]) for s in slots])
+        b.close()
+    for (wa, ta), (wb, tb) in zip(res[0], res[1]):
+        assert ta == tb
+        assert np.array_equal(wa, wb)
+
+
+def test_tune_reports_segment_lengths():
+    b = vacgpu.Batch(4 << 30, 16)
+    for (nx, ny, ph, g, prec, seed) in MIX:
+        w, dxs = state(nx, ny, ph, g, seed)
+        b.add(w, [nx, ny], ph, g, dxs, METHOD, precision=prec, itmax=3)
+    b.tune()
+    rows, sl = b.get_tiles()
+    print("case tune tiles", rows, sl, flush=True)
+    assert all(r in (1, 2, 4, 8) for r in rows)
+    assert all(x in (8, 16, 32, 64) for x in sl)
+    b.close()

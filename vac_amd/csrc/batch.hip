@@ -49,8 +49,10 @@ struct InstDev {
   double* b;   // output of the first sweep
   double* w1;  // predictor
   long long nbound, slab0, slab1;
-  int pbase, ntiles;
+  int pbase, ntiles;    // partials of the reduction tiles (pressure floor, cell-centre dt)
+  int pbase2, ntiles2;  // partials of the sweep launches (Courant maximum)
   int mhd;
+  int seglen;           // fused sweeps: rows per marching segment
 };
 
 struct BScal {
@@ -120,6 +122,31 @@ __global__ __launch_bounds__(512) void kb_corr(const InstDev* __restrict__ inst,
   const double cm = tvdlf_corr_body<ND, MHD, R, D>(I.geo, I.sel, src, I.w1, dst, S.s.dt,
                                                    S.s.smallp, wk.bx, wk.by, wk.bz);
   const double b = block_reduce<true>(cm, sm);
+  if (threadIdx.x == 0 && threadIdx.y == 0) part[SLOT_C0 * pstride + wk.pidx] = b;
+}
+
+// fused 2D TVDLF sweep of instances whose faces have one boundary type for all
+// variables (the single-grid kernel body): work item (strip, block of four
+// segments); sweep k reads w and writes b (k = 0) or reads b and writes w
+// (k = 1); ghost cells of the output are written by the sweep itself
+template <bool MHD, class R, int D, int LIM>
+__global__ __launch_bounds__(256) void kb_fused(const InstDev* __restrict__ inst,
+                                               const BScal* __restrict__ bs,
+                                               const Work* __restrict__ work, int k,
+                                               double* __restrict__ part, long long pstride) {
+  __shared__ double sm[8];
+  BATCH_PROLOGUE
+  const double* src = (k == 0) ? I.w : I.b;
+  double* dst = (k == 0) ? I.b : I.w;
+  const int seg = wk.by * 4 + (int)threadIdx.y;
+  double cmx;
+  if constexpr (D == 0)
+    cmx = tvdlf_fx_body<MHD, R, LIM>(I.geo, I.sel, I.bc, src, dst, S.s.dt, S.s.smallp, wk.bx, seg,
+                                     I.seglen);
+  else
+    cmx = tvdlf_fy_body<MHD, R, LIM>(I.geo, I.sel, I.bc, src, dst, S.s.dt, S.s.smallp, wk.bx, seg,
+                                     I.seglen);
+  const double b = block_reduce<true>(cmx, sm);
   if (threadIdx.x == 0 && threadIdx.y == 0) part[SLOT_C0 * pstride + wk.pidx] = b;
 }
 
@@ -210,7 +237,7 @@ __global__ void kb_fin_courant(const InstDev* __restrict__ inst, BScal* __restri
   BScal& S = bs[i];
   if (S.done) return;
   const InstDev& I = inst[i];
-  const double cm = fold_range(part + SLOT_C0 * pstride + I.pbase, I.ntiles, true, sm);
+  const double cm = fold_range(part + SLOT_C0 * pstride + I.pbase2, I.ntiles2, true, sm);
   if (threadIdx.x == 0 && I.sel.courantpar > 0.0) {
     const int nd = I.geo.nd;
     const bool fwd = (S.s.it / 2) * 2 == S.s.it;
@@ -235,10 +262,17 @@ __global__ void kb_step_end(BScal* __restrict__ bs, const int* __restrict__ ids,
 // ---------------------------------------------------------------- host
 enum { NCLASS = 4 };
 const int kTileRows[] = {1, 2, 4, 8};
+const int kSegLens[] = {8, 16, 32, 64};
 
+// kernel instance of a work list: physics, tier, and for the fused sweeps
+// the limiter type (lim < 0: predictor / corrector tiles)
 struct Key {
-  int mhd, f32;
-  bool operator<(const Key& o) const { return mhd != o.mhd ? mhd < o.mhd : f32 < o.f32; }
+  int mhd, f32, lim;
+  bool operator<(const Key& o) const {
+    if (mhd != o.mhd) return mhd < o.mhd;
+    if (f32 != o.f32) return f32 < o.f32;
+    return lim < o.lim;
+  }
 };
 
 struct HostInst {
@@ -251,6 +285,8 @@ struct HostInst {
   int phase_class = 0;  // (it at admission - batch step at admission) mod 2
   int ntiles = 0;
   dim3 tiles;           // tile grid for the class tile shape
+  bool fused = false;   // sweeps by kb_fused (else predictor + corrector tiles)
+  int lim = -1;         // the common limiter type of a fused instance
 };
 
 struct Launch {          // one work list on the device
@@ -273,7 +309,9 @@ struct vacgpu_batch {
   int cap = 0;
   double* part = nullptr;
   long long pstride = 0;
-  int rows[NCLASS] = {4, 4, 4, 4};  // tile rows per size class (autotuned)
+  int rows[NCLASS] = {4, 4, 4, 4};       // tile rows per size class (autotuned)
+  int seglen[NCLASS] = {16, 16, 16, 16};  // fused sweeps: rows per segment per size class
+  bool use_fused = true;
   long long step_count = 0;
   bool dirty = true;
   // work lists per (physics/tier key, tile rows): every tile, and the tiles
@@ -397,29 +435,51 @@ int rebuild(vacgpu_batch* b) {
   std::map<std::pair<Key, int>, std::vector<Work>> all, ph0, ph1;
   std::vector<GWork> g4, g6;
   std::vector<int> ids;
-  int pbase = 0;
+  int pbase = 0, pbase2 = 0;
   for (int i = 0; i < (int)b->inst.size(); ++i) {
     HostInst& h = b->inst[i];
     if (!h.used) continue;
     const int rows = b->rows[h.klass];
     const Geo& g = h.dev.geo;
+    // reduction tiles (64 x rows), also the sweep tiles of unfused instances
     h.tiles = dim3((g.n[0] + BX - 1) / BX, (g.n[1] + rows - 1) / rows, 1);
     h.ntiles = (int)(h.tiles.x * h.tiles.y);
     h.dev.pbase = pbase;
     h.dev.ntiles = h.ntiles;
-    Key k{h.cfg.physics == VACGPU_MHD, h.cfg.precision == VACGPU_FP32BULK};
-    auto key = std::make_pair(k, rows);
+    const bool fused = h.fused && b->use_fused;
+    const Key kr{h.cfg.physics == VACGPU_MHD, h.cfg.precision == VACGPU_FP32BULK, -1};
+    auto key = std::make_pair(kr, rows);
+    auto& phl = h.phase_class ? ph1 : ph0;
     int t = 0;
     for (unsigned by = 0; by < h.tiles.y; ++by)
       for (unsigned bx = 0; bx < h.tiles.x; ++bx, ++t) {
-        Work w{i, (int)bx, (int)by, 0, pbase + t};
-        all[key].push_back(w);
-        (h.phase_class ? ph1 : ph0)[key].push_back(w);
+        all[key].push_back(Work{i, (int)bx, (int)by, 0, pbase + t});
+        if (!fused) phl[key].push_back(Work{i, (int)bx, (int)by, 0, pbase2 + t});
       }
     pbase += h.ntiles;
-    for (long long s = 0; s < h.dev.nbound; s += 256) {
-      GWork gw{i, (int)s};
-      (k.mhd ? g6 : g4).push_back(gw);
+    if (fused) {
+      // sweep blocks: strips of 62 x cells, blocks of four segments
+      const int sl = b->seglen[h.klass];
+      const int strips = (g.n[0] + FS_STRIDE - 1) / FS_STRIDE;
+      const int segs = (g.n[1] + sl - 1) / sl;
+      const int sblocks = (segs + 3) / 4;
+      h.dev.seglen = sl;
+      h.dev.pbase2 = pbase2;
+      h.dev.ntiles2 = strips * sblocks;
+      const Key kf{kr.mhd, kr.f32, h.lim};
+      auto keyf = std::make_pair(kf, sl);
+      all[keyf];  // the list exists (its reduction part stays empty)
+      int u = 0;
+      for (int sb = 0; sb < sblocks; ++sb)
+        for (int st = 0; st < strips; ++st, ++u) phl[keyf].push_back(Work{i, st, sb, 0, pbase2 + u});
+      pbase2 += h.dev.ntiles2;
+    } else {
+      h.dev.seglen = 0;
+      h.dev.pbase2 = pbase2;
+      h.dev.ntiles2 = h.ntiles;
+      pbase2 += h.ntiles;
+      // ghost cells are filled by separate launches only for unfused instances
+      for (long long q = 0; q < h.dev.nbound; q += 256) (kr.mhd ? g6 : g4).push_back(GWork{i, (int)q});
     }
     ids.push_back(i);
     b->h_inst[i] = h.dev;
@@ -427,9 +487,10 @@ int rebuild(vacgpu_batch* b) {
   if (!ids.empty())
     BHIP(b, hipMemcpyAsync(b->d_inst, b->h_inst, sizeof(InstDev) * (ids.back() + 1), hipMemcpyHostToDevice, b->stream));
   // partial arrays
-  if (pbase > b->pstride) {
+  const int pneed = pbase > pbase2 ? pbase : pbase2;
+  if (pneed > b->pstride) {
     if (b->part) BHIP(b, hipFree(b->part));
-    b->pstride = (long long)pbase * 2 + 64;
+    b->pstride = (long long)pneed * 2 + 64;
     BHIP(b, hipMalloc(&b->part, sizeof(double) * NSLOT * b->pstride));
   }
   for (auto& kv : b->lists) { kv.second.all.n = 0; kv.second.ph[0].n = 0; kv.second.ph[1].n = 0; }
@@ -543,11 +604,25 @@ int enqueue_step(vacgpu_batch* b, int parity) {
       for (auto& kv : b->lists) {
         const auto& L = kv.second;
         const dim3 blk(BX, kv.first.second);
+        const int lim = kv.first.first.lim;
         for (int c = 0; c < 2; ++c) {
           const Launch& W = L.ph[c];
           if (!W.n) continue;
           const bool fwd = ((c + parity) % 2) == 0;
           const int D = fwd ? k : ND - 1 - k;
+          if (lim >= 0) {  // fused sweep: one launch, in the corrector pass
+            if (pass == 1) {
+              const dim3 fb(64, 4);
+#define KBF(DV, LV) hipLaunchKernelGGL((kb_fused<MHD, R, DV, LV>), dim3(W.n), fb, 0, b->stream, b->d_inst, b->d_bs, W.d, k, b->part, b->pstride)
+#define KBFL(DV) do { if (lim == LIM_MINMOD) KBF(DV, LIM_MINMOD); else if (lim == LIM_WOODWARD) KBF(DV, LIM_WOODWARD); else KBF(DV, LIM_SUPERBEE); } while (0)
+              BDISPATCH(kv.first.first.mhd, kv.first.first.f32, {
+                if (D == 0) KBFL(0); else KBFL(1);
+              });
+#undef KBFL
+#undef KBF
+            }
+            continue;
+          }
           BDISPATCH(kv.first.first.mhd, kv.first.first.f32, {
             if (pass == 0) {
               if (D == 0) hipLaunchKernelGGL((kb_pred<ND, MHD, R, 0>), dim3(W.n), blk, 0, b->stream, b->d_inst, b->d_bs, W.d, k);
@@ -642,6 +717,17 @@ int vacgpu_batch_add(vacgpu_batch* b, const vacgpu_config* cfg, const vacgpu_met
   I.sel.sourceunsplit = m->sourceunsplit;
   I.sel.courantpar = m->courantpar;
   I.sel.gamma = cfg->eqpar[0];
+  // fused sweeps: one boundary type per face for all variables, one limiter,
+  // n >= 4, planes below 2^29 elements, MHD with the Powell source on
+  {
+    bool ok = cfg->nx[0] >= 4 && cfg->nx[1] >= 4 && (unsigned long long)I.geo.ps * 8ULL < (1ULL << 32);
+    for (int f = 0; f < 4; ++f)
+      for (int iw = 1; iw < nw; ++iw) ok &= I.bc.t[f][iw] == I.bc.t[f][0];
+    for (int iw = 1; iw < nw; ++iw) ok &= m->typelimiter[iw] == m->typelimiter[0];
+    if (I.mhd) ok &= m->divbfix && m->sourceunsplit;
+    h.fused = ok;
+    h.lim = ok ? m->typelimiter[0] : -1;
+  }
   long long slab[3];
   I.nbound = ghost_slabs(I.geo, slab);
   I.slab0 = slab[0];
@@ -824,7 +910,7 @@ int vacgpu_batch_tune(vacgpu_batch* b, int* rows_out) {
   for (int c = 0; c < NCLASS; ++c) {
     // tiles of class-c instances per physics/tier key, for each candidate shape
     bool present = false;
-    for (const auto& h : b->inst) present |= (h.used && h.klass == c);
+    for (const auto& h : b->inst) present |= (h.used && h.klass == c && !(h.fused && b->use_fused));
     if (!present) continue;
     float best = 1e30f;
     int best_rows = b->rows[c];
@@ -879,6 +965,66 @@ int vacgpu_batch_tune(vacgpu_batch* b, int* rows_out) {
     }
     b->rows[c] = best_rows;
   }
+  // fused instances: segment length per size class, timing both sweeps
+  for (int c = 0; c < NCLASS; ++c) {
+    bool present = false;
+    for (const auto& h : b->inst) present |= (h.used && h.klass == c && h.fused && b->use_fused);
+    if (!present) continue;
+    float best = 1e30f;
+    int best_sl = b->seglen[c];
+    for (int sl : kSegLens) {
+      std::map<Key, std::vector<Work>> lists;
+      for (int i = 0; i < (int)b->inst.size(); ++i) {
+        const HostInst& h = b->inst[i];
+        if (!h.used || h.klass != c || !h.fused) continue;
+        b->h_inst[i].seglen = sl;
+        const Geo& g = h.dev.geo;
+        const int strips = (g.n[0] + FS_STRIDE - 1) / FS_STRIDE;
+        const int sblocks = ((g.n[1] + sl - 1) / sl + 3) / 4;
+        Key k{h.cfg.physics == VACGPU_MHD, h.cfg.precision == VACGPU_FP32BULK, h.lim};
+        for (int sb = 0; sb < sblocks; ++sb)
+          for (int st = 0; st < strips; ++st) lists[k].push_back(Work{i, st, sb, 0, 0});
+      }
+      BHIP(b, hipMemcpyAsync(b->d_inst, b->h_inst, sizeof(InstDev) * b->inst.size(), hipMemcpyHostToDevice, b->stream));
+      size_t total = 0;
+      for (auto& kv : lists) total += kv.second.size();
+      if ((int)total > tmp_cap) {
+        if (tmp) BHIP(b, hipFree(tmp));
+        tmp_cap = (int)total;
+        BHIP(b, hipMalloc(&tmp, sizeof(Work) * tmp_cap));
+      }
+      size_t off = 0;
+      std::vector<std::pair<Key, std::pair<size_t, int>>> segs;
+      for (auto& kv : lists) {
+        BHIP(b, hipMemcpy(tmp + off, kv.second.data(), sizeof(Work) * kv.second.size(), hipMemcpyHostToDevice));
+        segs.push_back({kv.first, {off, (int)kv.second.size()}});
+        off += kv.second.size();
+      }
+      float tot = 0.f;
+      for (int rep = 0; rep < 3; ++rep) {
+        BHIP(b, hipEventRecord(e0, b->stream));
+        for (auto& sg : segs) {
+          const Work* W = tmp + sg.second.first;
+          const int n = sg.second.second;
+          const int lim = sg.first.lim;
+          const dim3 fb(64, 4);
+#define KBT(DV, LV) hipLaunchKernelGGL((kb_fused<MHD, R, DV, LV>), dim3(n), fb, 0, b->stream, b->d_inst, b->d_bs, W, 0, b->part, b->pstride)
+#define KBTL(DV) do { if (lim == LIM_MINMOD) KBT(DV, LIM_MINMOD); else if (lim == LIM_WOODWARD) KBT(DV, LIM_WOODWARD); else KBT(DV, LIM_SUPERBEE); } while (0)
+          BDISPATCH(sg.first.mhd, sg.first.f32, { KBTL(0); KBTL(1); });
+#undef KBTL
+#undef KBT
+        }
+        BHIP(b, hipEventRecord(e1, b->stream));
+        BHIP(b, hipEventSynchronize(e1));
+        float ms = 0.f;
+        BHIP(b, hipEventElapsedTime(&ms, e0, e1));
+        if (rep > 0) tot += ms;
+      }
+      if ((rc = launch_err(b, "tune"))) return rc;
+      if (tot < best) { best = tot; best_sl = sl; }
+    }
+    b->seglen[c] = best_sl;
+  }
   if (tmp) BHIP(b, hipFree(tmp));
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
@@ -898,6 +1044,28 @@ int vacgpu_batch_set_tiles(vacgpu_batch* b, const int* rows) {
     b->rows[c] = rows[c];
   }
   b->dirty = true;
+  return 0;
+}
+
+int vacgpu_batch_set_kernels(vacgpu_batch* b, int fused, const int* seglen) {
+  if (!b) return VACGPU_E_ARG;
+  if (seglen)
+    for (int c = 0; c < NCLASS; ++c)
+      if (seglen[c] < 4 || seglen[c] > 256) return bfail(b, VACGPU_E_ARG, "segment length must be in [4, 256]");
+  BHIP(b, hipStreamSynchronize(b->stream));
+  b->use_fused = fused != 0;
+  if (seglen)
+    for (int c = 0; c < NCLASS; ++c) b->seglen[c] = seglen[c];
+  b->dirty = true;
+  return 0;
+}
+
+int vacgpu_batch_get_tiles(const vacgpu_batch* b, int* rows, int* seglen) {
+  if (!b) return VACGPU_E_ARG;
+  for (int c = 0; c < NCLASS; ++c) {
+    if (rows) rows[c] = b->rows[c];
+    if (seglen) seglen[c] = b->seglen[c];
+  }
   return 0;
 }
 

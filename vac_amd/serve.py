@@ -75,7 +75,9 @@ def run_batched(instances, steps=100, precision="fp64", budget=64 << 30, graph=T
     for inst in instances:
         slots.append(b.add(inst.w, [inst.nx, inst.ny], inst.physics, inst.gamma, inst.dxs, METHOD,
                            precision=precision, itmax=steps))
-    rows = b.tune() if tune else None
+    if tune:
+        b.tune()
+    rows, seglen = b.get_tiles()
     b.graph(graph)
     # one untimed step would change the states; instead time from the admitted
     # states, after a synchronisation
@@ -92,7 +94,7 @@ def run_batched(instances, steps=100, precision="fp64", budget=64 << 30, graph=T
             "cell_updates": cells * steps, "seconds": elapsed,
             "mcell_updates_per_s": cells * steps / elapsed / 1e6,
             "all_reached_itmax": done == len(instances) and min(its) == steps,
-            "tile_rows_per_class": rows, "graph": graph}
+            "tile_rows_per_class": rows, "fused_seglen_per_class": seglen, "graph": graph}
 
 
 def make_requests(rate, count, seed=7, lo=32, hi=1024):

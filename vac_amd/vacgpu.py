@@ -76,6 +76,8 @@ SIGNATURES = {
     "vacgpu_batch_graph": ([C.c_void_p, C.c_int], C.c_int),
     "vacgpu_batch_tune": ([C.c_void_p, _IP], C.c_int),
     "vacgpu_batch_set_tiles": ([C.c_void_p, _IP], C.c_int),
+    "vacgpu_batch_set_kernels": ([C.c_void_p, C.c_int, _IP], C.c_int),
+    "vacgpu_batch_get_tiles": ([C.c_void_p, _IP, _IP], C.c_int),
     "vacgpu_batch_bytes_used": ([C.c_void_p], C.c_longlong),
     "vacgpu_batch_last_error": ([C.c_void_p], C.c_char_p),
     "vacgpu_batch_destroy": ([C.c_void_p], None),
@@ -393,6 +395,21 @@ class Batch:
     def set_tiles(self, rows):
         r = np.ascontiguousarray(rows, dtype=np.int32)
         self._check(self.lib.vacgpu_batch_set_tiles(self.h, r.ctypes.data_as(_IP)), "set_tiles")
+
+    def set_kernels(self, fused: bool = True, seglen=None):
+        """Fused sweeps for eligible instances (default) or predictor +
+        corrector tiles for all; seglen: rows per segment per size class."""
+        sl = None if seglen is None else np.ascontiguousarray(seglen, dtype=np.int32)
+        self._check(self.lib.vacgpu_batch_set_kernels(self.h, int(fused),
+                                                      None if sl is None else sl.ctypes.data_as(_IP)),
+                    "set_kernels")
+
+    def get_tiles(self):
+        rows = np.zeros(4, dtype=np.int32)
+        sl = np.zeros(4, dtype=np.int32)
+        self._check(self.lib.vacgpu_batch_get_tiles(self.h, rows.ctypes.data_as(_IP),
+                                                    sl.ctypes.data_as(_IP)), "get_tiles")
+        return [int(r) for r in rows], [int(x) for x in sl]
 
     def bytes_used(self) -> int:
         return int(self.lib.vacgpu_batch_bytes_used(self.h))
