@@ -830,6 +830,34 @@ __device__ __forceinline__ int image_of(const Geo& g, const BC& bc, int k, int c
   return c < 2 ? lo : (c > n + 1 ? hi : c);
 }
 
+// store out at mesh cell c (index idx) and, when ghosts is set, at every
+// ghost cell whose image it is (one boundary type per face for all
+// variables, n >= 4): the sweep then leaves the output's ghost cells filled
+template <int ND, int NW>
+__device__ __forceinline__ void store_cell(const Geo& g, const BC& bc, double* __restrict__ wout,
+                                           long long idx, const int* c, const double* out,
+                                           int ghosts) {
+#pragma unroll
+  for (int iw = 0; iw < NW; ++iw) wout[iw * g.ps + idx] = out[iw];
+  if (!ghosts) return;
+  int t[3][5], nt[3] = {1, 1, 1};
+#pragma unroll
+  for (int k = 0; k < ND; ++k) {
+    t[k][0] = c[k];
+    nt[k] = 1 + ghost_preimages(c[k], g.n[k], bc.t[2 * k][0], bc.t[2 * k + 1][0], &t[k][1]);
+  }
+  if (nt[0] + nt[1] + nt[2] == 3) return;  // an interior cell
+  for (int a = 0; a < nt[0]; ++a)
+    for (int b = 0; b < nt[1]; ++b)
+      for (int e = 0; e < (ND == 3 ? nt[2] : 1); ++e) {
+        if (a == 0 && b == 0 && e == 0) continue;
+        long long o = g.off + t[0][a] + t[1][b] * g.s[1];
+        if (ND == 3) o += t[2][e] * g.s[2];
+#pragma unroll
+        for (int iw = 0; iw < NW; ++iw) wout[iw * g.ps + o] = out[iw];
+      }
+}
+
 template <int ND>
 __device__ __forceinline__ void wct_halfsrc(const Geo& g, const BC& bc, const double* __restrict__ w,
                                             const int* craw, double dt, double* u) {
@@ -849,9 +877,10 @@ __device__ __forceinline__ void wct_halfsrc(const Geo& g, const BC& bc, const do
 }
 
 template <int ND>
-__device__ __forceinline__ void ryu_first_half(const Geo& g, const double* __restrict__ wold,
+__device__ __forceinline__ void ryu_first_half(const Geo& g, const BC& bc,
+                                               const double* __restrict__ wold,
                                                double* __restrict__ wt, long long idx, const int* c,
-                                               double dt, const double* wnew) {
+                                               double dt, const double* wnew, int ghosts) {
   constexpr int NW = Layout<ND, true>::nw;
   double uo[NW], t[NW];
   load<NW>(wold, g.ps, idx, uo);
@@ -859,7 +888,8 @@ __device__ __forceinline__ void ryu_first_half(const Geo& g, const double* __res
   for (int iw = 0; iw < NW; ++iw) t[iw] = wnew[iw];
   powell<ND>(uo, divb_at<ND>(g, wold, idx, c), dt, t);
 #pragma unroll
-  for (int iw = 0; iw < NW; ++iw) wt[iw * g.ps + idx] = 0.5 * (uo[iw] + t[iw]);
+  for (int iw = 0; iw < NW; ++iw) t[iw] = 0.5 * (uo[iw] + t[iw]);
+  store_cell<ND, NW>(g, bc, wt, idx, c, t, ghosts);
 }
 
 // ---- TVD sweep.  For cell i: the centred flux difference of wCT (centdiff,
@@ -946,7 +976,7 @@ __global__ __launch_bounds__(256) void k_tvd_x(Geo g, Sel s, BC bc, const double
                                               const Scal* __restrict__ sc,
                                               double* __restrict__ part, long long pstride,
                                               const double* __restrict__ wold,
-                                              double* __restrict__ wt) {
+                                              double* __restrict__ wt, int ghosts) {
   using L = Layout<ND, true>;
   constexpr int NW = L::nw;
   __shared__ double sm[8];
@@ -1008,12 +1038,10 @@ __global__ __launch_bounds__(256) void k_tvd_x(Geo g, Sel s, BC bc, const double
     };
     const double dm = dvol(i - 1), d0 = dvol(i), dp = dvol(i + 1);
     tvd_dissipate<ND, R>(rtm, phim, f.rt, phi, 0, s, R(d0 + dm), R(dp + d0), d0, out);
-#pragma unroll
-    for (int iw = 0; iw < NW; ++iw) wout[iw * g.ps + idx] = out[iw];
-    if constexpr (SRC == 2) {
-      const int c[3] = {i, yc, z};
-      ryu_first_half<ND>(g, wold, wt, idx, c, dt, out);
-    }
+    const int c[3] = {i, yc, z};
+    // (the last sweep's output is final only after the Ryu step, which fills its ghosts)
+    store_cell<ND, NW>(g, bc, wout, idx, c, out, SRC == 2 ? 0 : ghosts);
+    if constexpr (SRC == 2) ryu_first_half<ND>(g, bc, wold, wt, idx, c, dt, out, ghosts);
   }
   const double b0 = block_reduce<true>(c0, sm);
   if (threadIdx.x == 0 && threadIdx.y == 0) part[SLOT_C0 * pstride + block_linear()] = b0;
@@ -1029,7 +1057,11 @@ __global__ __launch_bounds__(256) void k_tvd_m(Geo g, Sel s, BC bc, const double
                                               const Scal* __restrict__ sc,
                                               double* __restrict__ part, long long pstride,
                                               int seglen, const double* __restrict__ wold,
-                                              double* __restrict__ wt) {
+                                              double* __restrict__ wt, int ghosts) {
+  // (ghost cells of the output are left to a boundary pass: writing them here
+  // costs the marching kernel occupancy; the Ryu half is folded into x sweeps only)
+  static_assert(SRC != 2, "the Ryu half is folded into x sweeps only");
+  (void)ghosts;
   constexpr int d = D;
   using L = Layout<ND, true>;
   constexpr int NW = L::nw;
@@ -1125,12 +1157,6 @@ __global__ __launch_bounds__(256) void k_tvd_m(Geo g, Sel s, BC bc, const double
       tvd_dissipate<ND, R>(fm.rt, phim, f0.rt, phi0, d, s, R(d0 + dm), R(dp + d0), d0, out);
 #pragma unroll
       for (int iw = 0; iw < NW; ++iw) wout[iw * g.ps + idx] = out[iw];
-      if constexpr (SRC == 2) {
-        int cc[3] = {x, 0, 0};
-        cc[d] = i;
-        if (ND == 3) cc[other] = t;
-        ryu_first_half<ND>(g, wold, wt, idx, cc, dt, out);
-      }
       if (i == s1) break;
       // shift the window
 #pragma unroll
@@ -1150,27 +1176,24 @@ __global__ __launch_bounds__(256) void k_tvd_m(Geo g, Sel s, BC bc, const double
 // Ryu source step on the last sweep (src/vactvd.t:60-80):
 // wt = (wold + wnew - dt*S(wold))/2 on the mesh (ghosts refreshed after) ...
 template <int ND>
-__global__ __launch_bounds__(256) void k_ryu_a(Geo g, const double* __restrict__ wold,
+__global__ __launch_bounds__(256) void k_ryu_a(Geo g, BC bc, const double* __restrict__ wold,
                                               const double* __restrict__ wnew,
                                               double* __restrict__ wt,
-                                              const Scal* __restrict__ sc) {
+                                              const Scal* __restrict__ sc, int ghosts) {
   using L = Layout<ND, true>;
   constexpr int NW = L::nw;
   const CellIx cl = cell_of(g);
   if (!cl.ok) return;
-  double uo[NW], t[NW];
-  load<NW>(wold, g.ps, cl.idx, uo);
-  load<NW>(wnew, g.ps, cl.idx, t);
-  powell<ND>(uo, divb_at<ND>(g, wold, cl.idx, cl.c), sc->dt, t);
-#pragma unroll
-  for (int iw = 0; iw < NW; ++iw) wt[iw * g.ps + cl.idx] = 0.5 * (uo[iw] + t[iw]);
+  double un[NW];
+  load<NW>(wnew, g.ps, cl.idx, un);
+  ryu_first_half<ND>(g, bc, wold, wt, cl.idx, cl.c, sc->dt, un, ghosts);
 }
 
 // ... then wnew = wnew - dt*S(wt).
 template <int ND>
-__global__ __launch_bounds__(256) void k_ryu_b(Geo g, const double* __restrict__ wt,
+__global__ __launch_bounds__(256) void k_ryu_b(Geo g, BC bc, const double* __restrict__ wt,
                                               double* __restrict__ wnew,
-                                              const Scal* __restrict__ sc) {
+                                              const Scal* __restrict__ sc, int ghosts) {
   using L = Layout<ND, true>;
   constexpr int NW = L::nw;
   const CellIx cl = cell_of(g);
@@ -1179,8 +1202,7 @@ __global__ __launch_bounds__(256) void k_ryu_b(Geo g, const double* __restrict__
   load<NW>(wt, g.ps, cl.idx, ut);
   load<NW>(wnew, g.ps, cl.idx, o);
   powell<ND>(ut, divb_at<ND>(g, wt, cl.idx, cl.c), sc->dt, o);
-#pragma unroll
-  for (int iw = 0; iw < NW; ++iw) wnew[iw * g.ps + cl.idx] = o[iw];
+  store_cell<ND, NW>(g, bc, wnew, cl.idx, cl.c, o, ghosts);
 }
 
 // ------------------------------------------------------------ time step

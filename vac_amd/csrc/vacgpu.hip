@@ -356,6 +356,9 @@ int sweep_device(vacgpu_grid* g, const vacgpu_method* m, int d, int firstsweep, 
     // would cost occupancy)
     const int srcmode = (fold && firstsweep && !lastsweep) ? 1
                         : ((fold && lastsweep && !firstsweep && d == 0) ? 2 : 0);
+    // with one boundary type per face the kernels write the ghost cells of
+    // what they produce, and no separate boundary pass runs
+    const int ghosts = (g->bc_uniform && g->use_fused) ? 1 : 0;
     const double* wct = win;
     if (firstsweep && src && srcmode != 1) {
       {
@@ -378,7 +381,7 @@ int sweep_device(vacgpu_grid* g, const vacgpu_method* m, int d, int firstsweep, 
       if (d == 0) {
         const dim3 gr = g->grid_tx;
         nbt = (long long)gr.x * gr.y * gr.z;
-#define TVDX_L(NDV, RT, LV, SV) hipLaunchKernelGGL((k_tvd_x<NDV, RT, LV, SV>), gr, g->block, 0, g->stream, g->geo, s, g->bc, wct, win, wout, g->sc, g->part, g->pstride, wold, g->w1)
+#define TVDX_L(NDV, RT, LV, SV) hipLaunchKernelGGL((k_tvd_x<NDV, RT, LV, SV>), gr, g->block, 0, g->stream, g->geo, s, g->bc, wct, win, wout, g->sc, g->part, g->pstride, wold, g->w1, ghosts)
 #define X2D(LV) TVDX_L(2, double, LV, 0)
 #define X2F(LV) TVDX_L(2, float, LV, 0)
 #define X3D(LV) TVDX_L(3, double, LV, 0)
@@ -404,7 +407,7 @@ int sweep_device(vacgpu_grid* g, const vacgpu_method* m, int d, int firstsweep, 
       } else {
         const dim3 gr = g->grid_tm[d];
         nbt = (long long)gr.x * gr.y * gr.z;
-#define TVDM_L(NDV, RT, DV, LV, SV) hipLaunchKernelGGL((k_tvd_m<NDV, RT, DV, LV, SV>), gr, g->block, 0, g->stream, g->geo, s, g->bc, wct, win, wout, g->sc, g->part, g->pstride, g->seglen, wold, g->w1)
+#define TVDM_L(NDV, RT, DV, LV, SV) hipLaunchKernelGGL((k_tvd_m<NDV, RT, DV, LV, SV>), gr, g->block, 0, g->stream, g->geo, s, g->bc, wct, win, wout, g->sc, g->part, g->pstride, g->seglen, wold, g->w1, ghosts)
 #define M21D(LV) TVDM_L(2, double, 1, LV, 0)
 #define M21F(LV) TVDM_L(2, float, 1, LV, 0)
 #define M21D1(LV) TVDM_L(2, double, 1, LV, 1)
@@ -444,20 +447,26 @@ int sweep_device(vacgpu_grid* g, const vacgpu_method* m, int d, int firstsweep, 
       if (!ryu_folded) {
         Timed t(g, KF_RYU);
         if (g->cfg.ndim == 2)
-          hipLaunchKernelGGL(k_ryu_a<2>, g->grid, g->block, 0, g->stream, g->geo, wold, wout, g->w1, g->sc);
+          hipLaunchKernelGGL(k_ryu_a<2>, g->grid, g->block, 0, g->stream, g->geo, g->bc, wold, wout, g->w1, g->sc, ghosts);
         else
-          hipLaunchKernelGGL(k_ryu_a<3>, g->grid, g->block, 0, g->stream, g->geo, wold, wout, g->w1, g->sc);
+          hipLaunchKernelGGL(k_ryu_a<3>, g->grid, g->block, 0, g->stream, g->geo, g->bc, wold, wout, g->w1, g->sc, ghosts);
         if ((rc = launch_check(g, "k_ryu_a"))) return rc;
       }
-      if ((rc = bound(g, g->w1))) return rc;
+      if (!ghosts && (rc = bound(g, g->w1))) return rc;
       {
         Timed t(g, KF_RYU);
         if (g->cfg.ndim == 2)
-          hipLaunchKernelGGL(k_ryu_b<2>, g->grid, g->block, 0, g->stream, g->geo, g->w1, wout, g->sc);
+          hipLaunchKernelGGL(k_ryu_b<2>, g->grid, g->block, 0, g->stream, g->geo, g->bc, g->w1, wout, g->sc, ghosts);
         else
-          hipLaunchKernelGGL(k_ryu_b<3>, g->grid, g->block, 0, g->stream, g->geo, g->w1, wout, g->sc);
+          hipLaunchKernelGGL(k_ryu_b<3>, g->grid, g->block, 0, g->stream, g->geo, g->bc, g->w1, wout, g->sc, ghosts);
         if ((rc = launch_check(g, "k_ryu_b"))) return rc;
       }
+    }
+    // ghost cells of wout: written by an x sweep, or by the Ryu step on the
+    // last sweep; a y / z sweep that does not end the step needs the pass
+    if (ghosts && (d == 0 || (lastsweep && src))) {
+      g->cur = outi;
+      return 0;
     }
   }
   if ((rc = bound(g, wout))) return rc;
