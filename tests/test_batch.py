@@ -179,24 +179,32 @@ def test_fused_and_tile_sweeps_agree():
     method_mixed = vacgpu.MethodSpec(typefull="tvdlf", limiter=["minmod", "woodward"] * 4,
                                      courantpar=0.8)
     res = []
+    precs = []
     for fused in (True, False):
         b = vacgpu.Batch(4 << 30, 16)
         b.set_kernels(fused, [8, 16, 5, 32])
         slots = []
+        precs = []
         for j, (nx, ny, ph, g, prec, seed) in enumerate(MIX[:6]):
             w, dxs = state(nx, ny, ph, g, seed)
             bc = "cont" if j == 2 else "periodic"
             m = method_mixed if j == 3 else METHOD
             slots.append(b.add(w, [nx, ny], ph, g, dxs, m, bc=bc, precision=prec, itmax=7))
+            precs.append(prec)
         print(f"case batch fused={fused}", flush=True)
         for _ in range(8):
             b.step(1)
         b.sync()
         res.append([(b.get_mesh(s), b.status(s)["t"]) for s in slots])
         b.close()
-    for (wa, ta), (wb, tb) in zip(res[0], res[1]):
-        assert ta == tb
-        assert np.array_equal(wa, wb)
+    for (wa, ta), (wb, tb), prec in zip(res[0], res[1], precs):
+        if prec == "fp64":
+            assert ta == tb
+            assert np.array_equal(wa, wb)
+        else:  # the fused FP32-bulk sweep has its own increment form: the tier's bar
+            assert abs(ta - tb) <= 1e-5 * tb
+            for iw in range(wa.shape[0]):
+                assert np.abs(wa[iw] - wb[iw]).sum() <= 1e-5 * np.abs(wb[iw]).sum()
 
 
 def test_tune_reports_segment_lengths():

@@ -140,7 +140,12 @@ __global__ __launch_bounds__(256) void kb_fused(const InstDev* __restrict__ inst
   double* dst = (k == 0) ? I.b : I.w;
   const int seg = wk.by * 4 + (int)threadIdx.y;
   double cmx;
-  if constexpr (D == 0)
+  if constexpr (sizeof(R) == 4)  // FP32-bulk
+    cmx = D == 0 ? bulk_fx_body<MHD, LIM>(I.geo, I.sel, I.bc, src, dst, S.s.dt, S.s.smallp, wk.bx,
+                                          seg, I.seglen)
+                 : bulk_fy_body<MHD, LIM>(I.geo, I.sel, I.bc, src, dst, S.s.dt, S.s.smallp, wk.bx,
+                                          seg, I.seglen);
+  else if constexpr (D == 0)
     cmx = tvdlf_fx_body<MHD, R, LIM>(I.geo, I.sel, I.bc, src, dst, S.s.dt, S.s.smallp, wk.bx, seg,
                                      I.seglen);
   else

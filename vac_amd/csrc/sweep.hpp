@@ -736,6 +736,8 @@ __device__ __forceinline__ double tvdlf_fy_body(const Geo& g, const Sel& s, cons
   return cmx;
 }
 
+#include "bulk32.hpp"
+
 #ifndef VACGPU_FUSED_WAVES
 #define VACGPU_FUSED_WAVES 1
 #endif
@@ -754,7 +756,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(VACGPU_FUSE
   const int seg = (int)(blockIdx.y * blockDim.y + threadIdx.y);
   const double dt = sc->dt, smallp = sc->smallp;
   double cmx;
-  if constexpr (D == 0)
+  if constexpr (sizeof(R) == 4)  // FP32-bulk: increments over FP64 bases
+    cmx = D == 0 ? bulk_fx_body<MHD, LIM>(g, s, bc, w, wout, dt, smallp, (int)blockIdx.x, seg, seglen)
+                 : bulk_fy_body<MHD, LIM>(g, s, bc, w, wout, dt, smallp, (int)blockIdx.x, seg, seglen);
+  else if constexpr (D == 0)
     cmx = tvdlf_fx_body<MHD, R, LIM>(g, s, bc, w, wout, dt, smallp, (int)blockIdx.x, seg, seglen);
   else
     cmx = tvdlf_fy_body<MHD, R, LIM>(g, s, bc, w, wout, dt, smallp, (int)blockIdx.x, seg, seglen);
