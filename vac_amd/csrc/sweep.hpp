@@ -438,6 +438,160 @@ __device__ __forceinline__ T lane_next(T v) { return __shfl_down(v, 1, 64); }
 template <class T>
 __device__ __forceinline__ T lane_prev(T v) { return __shfl_up(v, 1, 64); }
 
+// ---- face-parallel TVDLF corrector (two-launch form, 2D and 3D): each face
+// is evaluated once.  x sweep: lane L of a wave owns the face c+1/2 of cell
+// c = x0 - 1 + L and updates c for L in [1, 62] (the left face arrives by a
+// lane shuffle); y / z sweeps: a thread marches along the sweep direction,
+// carrying the face below and a window of w and w1.  The arithmetic of a
+// face and of a cell update is tvdlf_corr_body's.
+constexpr int CX_STRIDE = 62;
+
+template <int ND, bool MHD, class R, int LIM>
+__global__ __launch_bounds__(256) void k_tvdlf_corrx(Geo g, Sel s, const double* __restrict__ w,
+                                                    const double* __restrict__ w1,
+                                                    double* __restrict__ wout,
+                                                    const Scal* __restrict__ sc,
+                                                    double* __restrict__ part, long long pstride) {
+  constexpr int D = 0;
+  using L = Layout<ND, MHD>;
+  constexpr int NW = L::nw;
+  __shared__ double sm[8];
+  const double dt = sc->dt, smallp = sc->smallp;
+  const int lane = (int)threadIdx.x;
+  const int c = 1 + (int)blockIdx.x * CX_STRIDE + lane;
+  const int y = 2 + (int)(blockIdx.y * BY + threadIdx.y);
+  const int z = (ND == 3) ? 2 + (int)blockIdx.z : 0;
+  const bool rowok = y <= g.n[1] + 1;
+  const bool useful = rowok && lane >= 1 && lane <= CX_STRIDE && c <= g.n[0] + 1;
+  const bool face_ok = rowok && lane <= CX_STRIDE && c <= g.n[0] + 1;
+  const int cc = c < g.n[0] + 2 ? c : g.n[0] + 2;  // idle lanes: in-bounds, unused
+  const int yc = rowok ? y : g.n[1] + 1;
+  const long long idx = g.off + cc + (long long)yc * g.s[1] + (long long)z * g.s[2];
+  double l[NW], lr[NW], u0[NW], u1[NW];
+#pragma unroll
+  for (int iw = 0; iw < NW; ++iw) {
+    const double* p = w + iw * g.ps + idx;
+    l[iw] = slope<R, LIM>(s.lim[iw], p[-1], p[0], p[1]);
+    u0[iw] = w1[iw * g.ps + idx];
+    u1[iw] = w1[iw * g.ps + idx + 1];
+  }
+#pragma unroll
+  for (int iw = 0; iw < NW; ++iw) lr[iw] = lane_next(l[iw]);
+  double F[NW], cv;
+  corr_face<ND, MHD, R, D>(s, u0, u1, l, lr, g.dx[0][cc] + g.dx[0][cc + 1], smallp, F, cv);
+  double cmx = face_ok ? cv : -INFINITY;
+  double out[NW];
+  const double qdx = dt / g.dx[0][cc];
+#pragma unroll
+  for (int iw = 0; iw < NW; ++iw) {
+    const double Fm = lane_prev(F[iw]);
+    out[iw] = w[iw * g.ps + idx] - qdx * (F[iw] - Fm);
+  }
+  if constexpr (MHD) {
+    if (s.sourceunsplit && s.divbfix) {
+      const int c3[3] = {cc, yc, z};
+      const double dv = divb_at<ND>(g, w1, idx, c3);
+      powell<ND>(u0, dv, dt * 1.0 / ND, out);
+    }
+  }
+  if (useful) {
+#pragma unroll
+    for (int iw = 0; iw < NW; ++iw) wout[iw * g.ps + idx] = out[iw];
+  }
+  const double bm = block_reduce<true>(cmx, sm);
+  if (threadIdx.x == 0 && threadIdx.y == 0) part[SLOT_C0 * pstride + block_linear()] = bm;
+}
+
+template <int ND, bool MHD, class R, int D, int LIM>
+__global__ __launch_bounds__(256) void k_tvdlf_corrm(Geo g, Sel s, const double* __restrict__ w,
+                                                    const double* __restrict__ w1,
+                                                    double* __restrict__ wout,
+                                                    const Scal* __restrict__ sc,
+                                                    double* __restrict__ part, long long pstride,
+                                                    int seglen) {
+  using L = Layout<ND, MHD>;
+  constexpr int NW = L::nw;
+  __shared__ double sm[8];
+  const double dt = sc->dt, smallp = sc->smallp;
+  const int x = 2 + (int)(blockIdx.x * BX + threadIdx.x);
+  const int seg = (int)(blockIdx.y * BY + threadIdx.y);
+  const int s0 = 2 + seg * seglen;
+  const int t = (ND == 3) ? 2 + (int)blockIdx.z : 0;  // the coordinate that is neither x nor D
+  const int other = (D == 1) ? 2 : 1;
+  const int nd_ = g.n[D];
+  double cmx = -INFINITY;
+  if (x <= g.n[0] + 1 && s0 <= nd_ + 1) {
+    const int s1 = (s0 + seglen - 1 < nd_ + 1) ? s0 + seglen - 1 : nd_ + 1;
+    const long long sd = g.s[D];
+    const long long base = g.off + x + (ND == 3 ? (long long)t * g.s[other] : 0);
+    const double* dxd = g.dx[D];
+    int c3[3] = {x, 0, 0};
+    if (ND == 3) c3[other] = t;
+    // window: w at J-1, J, J+1 (for the slope at J) and w1 at J; slope at J
+    double wa[NW], wb[NW], wc[NW], u0[NW], lc[NW], Fp[NW];
+    load<NW>(w, g.ps, base + (long long)(s0 - 2) * sd, wa);
+    load<NW>(w, g.ps, base + (long long)(s0 - 1) * sd, wb);
+    load<NW>(w, g.ps, base + (long long)s0 * sd, wc);
+    load<NW>(w1, g.ps, base + (long long)(s0 - 1) * sd, u0);
+#pragma unroll
+    for (int iw = 0; iw < NW; ++iw) lc[iw] = slope<R, LIM>(s.lim[iw], wa[iw], wb[iw], wc[iw]);
+    // face s0-1/2
+    {
+      double u1[NW], ln[NW], wd[NW];
+      load<NW>(w1, g.ps, base + (long long)s0 * sd, u1);
+      load<NW>(w, g.ps, base + (long long)(s0 + 1) * sd, wd);
+#pragma unroll
+      for (int iw = 0; iw < NW; ++iw) ln[iw] = slope<R, LIM>(s.lim[iw], wb[iw], wc[iw], wd[iw]);
+      double cv;
+      corr_face<ND, MHD, R, D>(s, u0, u1, lc, ln, dxd[s0 - 1] + dxd[s0], smallp, Fp, cv);
+      cmx = cmx < cv ? cv : cmx;
+#pragma unroll
+      for (int iw = 0; iw < NW; ++iw) {
+        u0[iw] = u1[iw];
+        lc[iw] = ln[iw];
+        wa[iw] = wb[iw];
+        wb[iw] = wc[iw];
+        wc[iw] = wd[iw];
+      }
+    }
+    // now: u0 = w1(i), lc = slope(i), wb = w(i), wc = w(i+1)
+    for (int i = s0; i <= s1; ++i) {
+      double u1[NW], ln[NW], wd[NW], F[NW], cv;
+      load<NW>(w1, g.ps, base + (long long)(i + 1) * sd, u1);
+      load<NW>(w, g.ps, base + (long long)(i + 2) * sd, wd);
+#pragma unroll
+      for (int iw = 0; iw < NW; ++iw) ln[iw] = slope<R, LIM>(s.lim[iw], wb[iw], wc[iw], wd[iw]);
+      corr_face<ND, MHD, R, D>(s, u0, u1, lc, ln, dxd[i] + dxd[i + 1], smallp, F, cv);
+      cmx = cmx < cv ? cv : cmx;
+      const long long idx = base + (long long)i * sd;
+      const double qdx = dt / dxd[i];
+      double out[NW];
+#pragma unroll
+      for (int iw = 0; iw < NW; ++iw) out[iw] = wb[iw] - qdx * (F[iw] - Fp[iw]);
+      if constexpr (MHD) {
+        if (s.sourceunsplit && s.divbfix) {
+          c3[D] = i;
+          const double dv = divb_at<ND>(g, w1, idx, c3);
+          powell<ND>(u0, dv, dt * 1.0 / ND, out);
+        }
+      }
+#pragma unroll
+      for (int iw = 0; iw < NW; ++iw) wout[iw * g.ps + idx] = out[iw];
+#pragma unroll
+      for (int iw = 0; iw < NW; ++iw) {
+        Fp[iw] = F[iw];
+        u0[iw] = u1[iw];
+        lc[iw] = ln[iw];
+        wb[iw] = wc[iw];
+        wc[iw] = wd[iw];
+      }
+    }
+  }
+  const double bm = block_reduce<true>(cmx, sm);
+  if (threadIdx.x == 0 && threadIdx.y == 0) part[SLOT_C0 * pstride + block_linear()] = bm;
+}
+
+
 // plane iw at byte offset o (planes are addressed with 32-bit byte offsets)
 __device__ __forceinline__ double ldp(const double* __restrict__ w, long long ps, int iw, unsigned o) {
   return *reinterpret_cast<const double*>(reinterpret_cast<const char*>(w + iw * ps) + o);

@@ -321,6 +321,7 @@ int sweep_device(vacgpu_grid* g, const vacgpu_method* m, int d, int firstsweep, 
     g->cur = outi;
     return 0;
   } else if (m->typefull == VACGPU_TVDLF) {
+    long long corr_blocks = nb;
     {
       Timed t(g, KF_PRED);
       DISPATCH(g, {
@@ -332,18 +333,36 @@ int sweep_device(vacgpu_grid* g, const vacgpu_method* m, int d, int firstsweep, 
     }
     if ((rc = bound(g, g->w1))) return rc;
     {
+      // face-parallel corrector: x sweep one face per lane, y / z sweeps marching
       Timed t(g, KF_CORR);
-      DISPATCH(g, {
-        if (d == 0) hipLaunchKernelGGL((k_tvdlf_corr<ND, MHD, R, 0>), g->grid, g->block, 0, g->stream, g->geo, s, win, g->w1, wout, g->sc, g->part, g->pstride);
-        else if (d == 1) hipLaunchKernelGGL((k_tvdlf_corr<ND, MHD, R, 1>), g->grid, g->block, 0, g->stream, g->geo, s, win, g->w1, wout, g->sc, g->part, g->pstride);
-        else hipLaunchKernelGGL((k_tvdlf_corr<ND, MHD, R, (ND == 3 ? 2 : 1)>), g->grid, g->block, 0, g->stream, g->geo, s, win, g->w1, wout, g->sc, g->part, g->pstride);
-      });
+      const int lim = common_limiter(g, m);
+      long long nbc = 0;
+#define LIMSW(X) do { if (lim == LIM_MINMOD) X(LIM_MINMOD); else if (lim == LIM_WOODWARD) X(LIM_WOODWARD); else if (lim == LIM_SUPERBEE) X(LIM_SUPERBEE); else X(-1); } while (0)
+      if (d == 0) {
+        const dim3 gr((g->geo.n[0] + CX_STRIDE - 1) / CX_STRIDE, g->grid.y, g->grid.z);
+        nbc = (long long)gr.x * gr.y * gr.z;
+#define CX(LV) hipLaunchKernelGGL((k_tvdlf_corrx<ND, MHD, R, LV>), gr, g->block, 0, g->stream, g->geo, s, win, g->w1, wout, g->sc, g->part, g->pstride)
+        DISPATCH(g, { LIMSW(CX); });
+#undef CX
+      } else {
+        const dim3 gr = g->grid_tm[d];
+        nbc = (long long)gr.x * gr.y * gr.z;
+#define CM(DV, LV) hipLaunchKernelGGL((k_tvdlf_corrm<ND, MHD, R, DV, LV>), gr, g->block, 0, g->stream, g->geo, s, win, g->w1, wout, g->sc, g->part, g->pstride, g->seglen)
+#define CM1(LV) CM(1, LV)
+#define CM2(LV) CM((ND == 3 ? 2 : 1), LV)
+        DISPATCH(g, { if (d == 1) LIMSW(CM1); else LIMSW(CM2); });
+#undef CM2
+#undef CM1
+#undef CM
+      }
+#undef LIMSW
       if ((rc = launch_check(g, "k_tvdlf_corr"))) return rc;
+      corr_blocks = nbc;
     }
     if (courant) {
       Timed t(g, KF_RED);
       hipLaunchKernelGGL(k_fin_courant_tvdlf, dim3(1), dim3(1024), 0, g->stream, g->sc, g->part,
-                         g->pstride, nb, d, m->courantpar);
+                         g->pstride, corr_blocks, d, m->courantpar);
       if ((rc = launch_check(g, "k_fin_courant_tvdlf"))) return rc;
     }
   } else {
