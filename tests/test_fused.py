@@ -162,3 +162,43 @@ def test_tvd_folded_sources_match(case):
         wr, tr, _ = vacref.run(p, w0, dxs, steps)
         assert np.array_equal(out[0][1], tr)
         assert np.array_equal(out[0][0][:, 2:-2, 2:-2], wr)
+
+
+GHOST_CASES = [
+    # (physics, shape, bc, limiter, steps): TVDLF grids the fused 2D kernels
+    # do not take, swept by the two-launch kernels that write ghosts themselves
+    ("mhd", (20, 14, 11), "periodic", "minmod", 5),
+    ("mhd", (13, 18, 9), ["cont", "cont", "periodic", "periodic", "cont", "cont"], "woodward", 5),
+    ("hd", (24, 10, 19), "cont", "superbee", 6),
+    ("hd", (33, 21, 16), "periodic", "minmod", 6),
+]
+
+
+@pytest.mark.parametrize("case", GHOST_CASES,
+                         ids=[f"{c[0]}{'x'.join(map(str, c[1]))}-{c[3]}" for c in GHOST_CASES])
+def test_two_launch_tvdlf_writes_ghost_cells(case):
+    # with one boundary type per face the predictor and corrector kernels
+    # write the ghost cells of their outputs; the whole ixG array must equal
+    # the path that fills them with the getboundary kernel, and the mesh the
+    # oracle's
+    physics, shape, bc, limiter, steps = case
+    w0, coords = random_fourier(shape, 31 + shape[0], 5 / 3, physics=physics)
+    dxs = [widths_1d(c) for c in coords]
+    method = vacgpu.MethodSpec(typefull="tvdlf", limiter=limiter, courantpar=0.8)
+    print(f"case two-launch ghosts {case}", flush=True)
+    out = []
+    for fused in (True, False):
+        g = vacgpu.Grid(list(shape), physics, 5 / 3, dxs, bc=bc)
+        g.set_kernels(fused)
+        g.put_mesh(w0)
+        n, t = g.run(method, steps, record_t=True)
+        out.append((g.get_state(), t))
+        g.close()
+    assert np.array_equal(out[0][1], out[1][1])
+    assert np.array_equal(out[0][0], out[1][0])
+    p = vacref.make_params(len(shape), physics, list(shape), 5 / 3, typefull="tvdlf",
+                           limiter=limiter, bc=bc)
+    wr, tr, _ = vacref.run(p, w0, dxs, steps)
+    assert np.array_equal(out[0][1], tr)
+    mesh = (slice(None),) + (slice(2, -2),) * len(shape)
+    assert np.array_equal(out[0][0][mesh], wr)

@@ -280,13 +280,6 @@ __device__ __forceinline__ void tvdlf_pred_body(const Geo& g, const Sel& s,
   for (int iw = 0; iw < NW; ++iw) w1[iw * g.ps + cl.idx] = out[iw];
 }
 
-template <int ND, bool MHD, class R, int D>
-__global__ __launch_bounds__(256) void k_tvdlf_pred(Geo g, Sel s, const double* __restrict__ w,
-                                                   double* __restrict__ w1,
-                                                   const Scal* __restrict__ sc) {
-  tvdlf_pred_body<ND, MHD, R, D>(g, s, w, w1, sc->dt, sc->smallp, (int)blockIdx.x, (int)blockIdx.y,
-                                 (int)blockIdx.z);
-}
 
 // TVDLF corrector flux at the face J+1/2 (src/vactvdlf.t:147-254): MUSCL
 // states wR = w1(J+1) - l(J+1)/2, wL = w1(J) + l(J)/2 from the predictor
@@ -438,6 +431,49 @@ __device__ __forceinline__ T lane_next(T v) { return __shfl_down(v, 1, 64); }
 template <class T>
 __device__ __forceinline__ T lane_prev(T v) { return __shfl_up(v, 1, 64); }
 
+// store out at mesh cell c (index idx) and, when ghosts is set, at every
+// ghost cell whose image it is (one boundary type per face for all
+// variables, n >= 4): the sweep then leaves the output's ghost cells filled
+template <int ND, int NW>
+__device__ __forceinline__ void store_cell(const Geo& g, const BC& bc, double* __restrict__ wout,
+                                           long long idx, const int* c, const double* out,
+                                           int ghosts) {
+#pragma unroll
+  for (int iw = 0; iw < NW; ++iw) wout[iw * g.ps + idx] = out[iw];
+  if (!ghosts) return;
+  int t[3][5], nt[3] = {1, 1, 1};
+#pragma unroll
+  for (int k = 0; k < ND; ++k) {
+    t[k][0] = c[k];
+    nt[k] = 1 + ghost_preimages(c[k], g.n[k], bc.t[2 * k][0], bc.t[2 * k + 1][0], &t[k][1]);
+  }
+  if (nt[0] + nt[1] + nt[2] == 3) return;  // an interior cell
+  for (int a = 0; a < nt[0]; ++a)
+    for (int b = 0; b < nt[1]; ++b)
+      for (int e = 0; e < (ND == 3 ? nt[2] : 1); ++e) {
+        if (a == 0 && b == 0 && e == 0) continue;
+        long long o = g.off + t[0][a] + t[1][b] * g.s[1];
+        if (ND == 3) o += t[2][e] * g.s[2];
+#pragma unroll
+        for (int iw = 0; iw < NW; ++iw) wout[iw * g.ps + o] = out[iw];
+      }
+}
+
+// predictor launch: one cell per thread; with ghosts set the ghost cells of
+// w1 that take a cell's value are written too (one boundary type per face)
+template <int ND, bool MHD, class R, int D>
+__global__ __launch_bounds__(256) void k_tvdlf_pred(Geo g, Sel s, BC bc, const double* __restrict__ w,
+                                                   double* __restrict__ w1,
+                                                   const Scal* __restrict__ sc, int ghosts) {
+  constexpr int NW = Layout<ND, MHD>::nw;
+  const CellIx cl = cell_of(g);
+  if (!cl.ok) return;
+  double out[NW];
+  pred_point<ND, MHD, R, D>(g, s, w, cl.idx, cl.c, sc->dt, sc->smallp, out);
+  store_cell<ND, NW>(g, bc, w1, cl.idx, cl.c, out, ghosts);
+}
+
+
 // ---- face-parallel TVDLF corrector (two-launch form, 2D and 3D): each face
 // is evaluated once.  x sweep: lane L of a wave owns the face c+1/2 of cell
 // c = x0 - 1 + L and updates c for L in [1, 62] (the left face arrives by a
@@ -447,11 +483,12 @@ __device__ __forceinline__ T lane_prev(T v) { return __shfl_up(v, 1, 64); }
 constexpr int CX_STRIDE = 62;
 
 template <int ND, bool MHD, class R, int LIM>
-__global__ __launch_bounds__(256) void k_tvdlf_corrx(Geo g, Sel s, const double* __restrict__ w,
+__global__ __launch_bounds__(256) void k_tvdlf_corrx(Geo g, Sel s, BC bc, const double* __restrict__ w,
                                                     const double* __restrict__ w1,
                                                     double* __restrict__ wout,
                                                     const Scal* __restrict__ sc,
-                                                    double* __restrict__ part, long long pstride) {
+                                                    double* __restrict__ part, long long pstride,
+                                                    int ghosts) {
   constexpr int D = 0;
   using L = Layout<ND, MHD>;
   constexpr int NW = L::nw;
@@ -495,20 +532,20 @@ __global__ __launch_bounds__(256) void k_tvdlf_corrx(Geo g, Sel s, const double*
     }
   }
   if (useful) {
-#pragma unroll
-    for (int iw = 0; iw < NW; ++iw) wout[iw * g.ps + idx] = out[iw];
+    const int c3[3] = {cc, yc, z};
+    store_cell<ND, NW>(g, bc, wout, idx, c3, out, ghosts);
   }
   const double bm = block_reduce<true>(cmx, sm);
   if (threadIdx.x == 0 && threadIdx.y == 0) part[SLOT_C0 * pstride + block_linear()] = bm;
 }
 
 template <int ND, bool MHD, class R, int D, int LIM>
-__global__ __launch_bounds__(256) void k_tvdlf_corrm(Geo g, Sel s, const double* __restrict__ w,
+__global__ __launch_bounds__(256) void k_tvdlf_corrm(Geo g, Sel s, BC bc, const double* __restrict__ w,
                                                     const double* __restrict__ w1,
                                                     double* __restrict__ wout,
                                                     const Scal* __restrict__ sc,
                                                     double* __restrict__ part, long long pstride,
-                                                    int seglen) {
+                                                    int seglen, int ghosts) {
   using L = Layout<ND, MHD>;
   constexpr int NW = L::nw;
   __shared__ double sm[8];
@@ -575,8 +612,8 @@ __global__ __launch_bounds__(256) void k_tvdlf_corrm(Geo g, Sel s, const double*
           powell<ND>(u0, dv, dt * 1.0 / ND, out);
         }
       }
-#pragma unroll
-      for (int iw = 0; iw < NW; ++iw) wout[iw * g.ps + idx] = out[iw];
+      c3[D] = i;
+      store_cell<ND, NW>(g, bc, wout, idx, c3, out, ghosts);
 #pragma unroll
       for (int iw = 0; iw < NW; ++iw) {
         Fp[iw] = F[iw];
@@ -987,34 +1024,6 @@ __device__ __forceinline__ int image_of(const Geo& g, const BC& bc, int k, int c
   const int n = g.n[k];
   const int lo = bc.t[2 * k][0] ? c + n : 2, hi = bc.t[2 * k + 1][0] ? c - n : n + 1;
   return c < 2 ? lo : (c > n + 1 ? hi : c);
-}
-
-// store out at mesh cell c (index idx) and, when ghosts is set, at every
-// ghost cell whose image it is (one boundary type per face for all
-// variables, n >= 4): the sweep then leaves the output's ghost cells filled
-template <int ND, int NW>
-__device__ __forceinline__ void store_cell(const Geo& g, const BC& bc, double* __restrict__ wout,
-                                           long long idx, const int* c, const double* out,
-                                           int ghosts) {
-#pragma unroll
-  for (int iw = 0; iw < NW; ++iw) wout[iw * g.ps + idx] = out[iw];
-  if (!ghosts) return;
-  int t[3][5], nt[3] = {1, 1, 1};
-#pragma unroll
-  for (int k = 0; k < ND; ++k) {
-    t[k][0] = c[k];
-    nt[k] = 1 + ghost_preimages(c[k], g.n[k], bc.t[2 * k][0], bc.t[2 * k + 1][0], &t[k][1]);
-  }
-  if (nt[0] + nt[1] + nt[2] == 3) return;  // an interior cell
-  for (int a = 0; a < nt[0]; ++a)
-    for (int b = 0; b < nt[1]; ++b)
-      for (int e = 0; e < (ND == 3 ? nt[2] : 1); ++e) {
-        if (a == 0 && b == 0 && e == 0) continue;
-        long long o = g.off + t[0][a] + t[1][b] * g.s[1];
-        if (ND == 3) o += t[2][e] * g.s[2];
-#pragma unroll
-        for (int iw = 0; iw < NW; ++iw) wout[iw * g.ps + o] = out[iw];
-      }
 }
 
 template <int ND>

@@ -322,16 +322,18 @@ int sweep_device(vacgpu_grid* g, const vacgpu_method* m, int d, int firstsweep, 
     return 0;
   } else if (m->typefull == VACGPU_TVDLF) {
     long long corr_blocks = nb;
+    // with one boundary type per face the kernels write their outputs' ghost cells
+    const int ghosts = (g->bc_uniform && g->use_fused) ? 1 : 0;
     {
       Timed t(g, KF_PRED);
       DISPATCH(g, {
-        if (d == 0) hipLaunchKernelGGL((k_tvdlf_pred<ND, MHD, R, 0>), g->grid, g->block, 0, g->stream, g->geo, s, win, g->w1, g->sc);
-        else if (d == 1) hipLaunchKernelGGL((k_tvdlf_pred<ND, MHD, R, 1>), g->grid, g->block, 0, g->stream, g->geo, s, win, g->w1, g->sc);
-        else hipLaunchKernelGGL((k_tvdlf_pred<ND, MHD, R, (ND == 3 ? 2 : 1)>), g->grid, g->block, 0, g->stream, g->geo, s, win, g->w1, g->sc);
+        if (d == 0) hipLaunchKernelGGL((k_tvdlf_pred<ND, MHD, R, 0>), g->grid, g->block, 0, g->stream, g->geo, s, g->bc, win, g->w1, g->sc, ghosts);
+        else if (d == 1) hipLaunchKernelGGL((k_tvdlf_pred<ND, MHD, R, 1>), g->grid, g->block, 0, g->stream, g->geo, s, g->bc, win, g->w1, g->sc, ghosts);
+        else hipLaunchKernelGGL((k_tvdlf_pred<ND, MHD, R, (ND == 3 ? 2 : 1)>), g->grid, g->block, 0, g->stream, g->geo, s, g->bc, win, g->w1, g->sc, ghosts);
       });
       if ((rc = launch_check(g, "k_tvdlf_pred"))) return rc;
     }
-    if ((rc = bound(g, g->w1))) return rc;
+    if (!ghosts && (rc = bound(g, g->w1))) return rc;
     {
       // face-parallel corrector: x sweep one face per lane, y / z sweeps marching
       Timed t(g, KF_CORR);
@@ -341,13 +343,13 @@ int sweep_device(vacgpu_grid* g, const vacgpu_method* m, int d, int firstsweep, 
       if (d == 0) {
         const dim3 gr((g->geo.n[0] + CX_STRIDE - 1) / CX_STRIDE, g->grid.y, g->grid.z);
         nbc = (long long)gr.x * gr.y * gr.z;
-#define CX(LV) hipLaunchKernelGGL((k_tvdlf_corrx<ND, MHD, R, LV>), gr, g->block, 0, g->stream, g->geo, s, win, g->w1, wout, g->sc, g->part, g->pstride)
+#define CX(LV) hipLaunchKernelGGL((k_tvdlf_corrx<ND, MHD, R, LV>), gr, g->block, 0, g->stream, g->geo, s, g->bc, win, g->w1, wout, g->sc, g->part, g->pstride, ghosts)
         DISPATCH(g, { LIMSW(CX); });
 #undef CX
       } else {
         const dim3 gr = g->grid_tm[d];
         nbc = (long long)gr.x * gr.y * gr.z;
-#define CM(DV, LV) hipLaunchKernelGGL((k_tvdlf_corrm<ND, MHD, R, DV, LV>), gr, g->block, 0, g->stream, g->geo, s, win, g->w1, wout, g->sc, g->part, g->pstride, g->seglen)
+#define CM(DV, LV) hipLaunchKernelGGL((k_tvdlf_corrm<ND, MHD, R, DV, LV>), gr, g->block, 0, g->stream, g->geo, s, g->bc, win, g->w1, wout, g->sc, g->part, g->pstride, g->seglen, ghosts)
 #define CM1(LV) CM(1, LV)
 #define CM2(LV) CM((ND == 3 ? 2 : 1), LV)
         DISPATCH(g, { if (d == 1) LIMSW(CM1); else LIMSW(CM2); });
@@ -364,6 +366,10 @@ int sweep_device(vacgpu_grid* g, const vacgpu_method* m, int d, int firstsweep, 
       hipLaunchKernelGGL(k_fin_courant_tvdlf, dim3(1), dim3(1024), 0, g->stream, g->sc, g->part,
                          g->pstride, corr_blocks, d, m->courantpar);
       if ((rc = launch_check(g, "k_fin_courant_tvdlf"))) return rc;
+    }
+    if (ghosts) {
+      g->cur = outi;
+      return 0;
     }
   } else {
     // 2D grids with one boundary type per face fold the half source into the
