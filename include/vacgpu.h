@@ -125,6 +125,16 @@ int vacgpu_kernel_time(vacgpu_grid* g, int k, double* avg_ms, long* count);
  * kernels (0 = automatic, else clamped to [4, 256]). */
 int vacgpu_set_kernels(vacgpu_grid* g, int fused, int seglen);
 
+/* Self-test of the quotient the device kernels use for every division by a
+ * positive quantity (densities, cell widths, squared sound speeds; no
+ * counterpart in VAC, whose divisions are plain IEEE divisions): computes
+ * q_kernel[i] with that quotient and q_ieee[i] = a[i] / b[i] with the
+ * compiler's correctly rounded division, on the current device.  The two
+ * agree bit for bit for b > 0 with b and 1/b normal, a zero or of magnitude
+ * >= 2^-969, and a/b normal.  Host arrays of n doubles. */
+int vacgpu_check_quotients(const double* a, const double* b, long long n, double* q_kernel,
+                           double* q_ieee);
+
 /* Start-up check of a selector set: 0 if supported, else VACGPU_E_ARG. */
 int vacgpu_supported(const vacgpu_config* cfg, const vacgpu_method* m);
 

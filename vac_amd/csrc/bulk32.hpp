@@ -283,7 +283,7 @@ __device__ __forceinline__ double bulk_fx_body(const Geo& g, const Sel& s, const
   const double* dx0 = g.dx[0];
   const double* dx1 = g.dx[1];
   const double dxs = dx0[c] + dx0[c + 1];
-  const double qdx = dt / dx0[c];
+  const double qdx = pdiv(dt, dx0[c]);
   // the faces c+1/2 of a row: F (FP32, m_x part excluded) and Fmd (FP64)
   auto faces = [&](const BulkRow<MHD, 0, LIM>& R, float* F, double& Fmd, double& cv) {
     float dL[NW], dR[NW], b[NW];
@@ -298,7 +298,7 @@ __device__ __forceinline__ double bulk_fx_body(const Geo& g, const Sel& s, const
       dR[iw] = float(Ur - R.U[iw]) + dPr - 0.5f * lr;
     }
     const float cm = face32<MHD, 0>(s, b, dL, dR, R.ptot, R.pth, smallp, F, Fmd);
-    cv = double(cm) * 2.0 / dxs;
+    cv = pdiv(double(cm) * 2.0, dxs);
   };
   // carried from row j: base, predictor B_x increment, rounded predictor
   // state, face fluxes; and B_y (base, increment) of rows j-1, j
@@ -423,7 +423,7 @@ __device__ __forceinline__ double bulk_fy_body(const Geo& g, const Sel& s, const
       dR[iw] = float(R.U[iw] - U0[iw]) + R.dP[iw] - 0.5f * R.l[iw];
     }
     const float cm = face32<MHD, 1>(s, b, dL0, dR, pt0, ph0, smallp, F, Fmd);
-    return double(cm) * 2.0 / dxs;
+    return pdiv(double(cm) * 2.0, dxs);
   };
   {
     BulkRow<MHD, 1, LIM> R;
@@ -454,7 +454,7 @@ __device__ __forceinline__ double bulk_fy_body(const Geo& g, const Sel& s, const
       byU_m = U0[L::b(1)];
       byd_m = dPby0;
     }
-    const double qdx = dt / dx1[j];
+    const double qdx = pdiv(dt, dx1[j]);
     double out[NW];
 #pragma unroll
     for (int iw = 0; iw < NW; ++iw) out[iw] = U0[iw] - qdx * double(Fn[iw] - Fp[iw]) + double(inc[iw]);

@@ -34,6 +34,28 @@ __device__ __forceinline__ R rmin(R a, R b) { return b < a ? b : a; }
 template <class R>
 __device__ __forceinline__ R fsign(R a, R b) { return copysign(fabs(a), b); }
 
+// a / b for a denominator b > 0.
+//
+// double: the hardware reciprocal estimate of b refined by two Newton-Raphson
+// steps, then q0 = a*y and one Markstein residual correction
+// q = q0 + (a - b*q0)*y (P. Markstein, IBM J. Res. Dev. 34(1), 1990).  These
+// are the operations of the device's correctly rounded double division when
+// that division needs neither operand scaling nor a special-value fix-up,
+// i.e. for b and 1/b normal, a zero or of magnitude >= 2^-969 and a/b
+// normal, so there the result is the same double.  The reciprocal depends on
+// b alone, so quotients with one denominator share it (8 instructions for the
+// first, 3 for each further one, against 11 for every division).  The
+// residual is formed negated, which keeps the sign of a zero numerator.
+// float: the ordinary division.
+__device__ __forceinline__ double pdiv(double a, double b) {
+  const double y0 = __builtin_amdgcn_rcp(b);
+  const double y1 = __builtin_fma(y0, __builtin_fma(-b, y0, 1.0), y0);
+  const double y = __builtin_fma(y1, __builtin_fma(-b, y1, 1.0), y1);
+  const double q0 = a * y;
+  return __builtin_fma(-__builtin_fma(b, q0, -a), y, q0);
+}
+__device__ __forceinline__ float pdiv(float a, float b) { return a / b; }
+
 // dwlimiter2 (src/vacgrid.t:938-996): a = dwC(i), b = dwC(i-1).  LIM >= 0
 // fixes the type at compile time (no branches); LIM < 0 takes it from type.
 template <int LIM, class R>
@@ -61,12 +83,12 @@ __device__ __forceinline__ double pthermal_raw(const double* u, double g) {
 #pragma unroll
   for (int c = 1; c < ND; ++c) msq = msq + u[L::m(c)] * u[L::m(c)];
   if constexpr (MHD) {
-    double p = msq / u[L::rho];
+    double p = pdiv(msq, u[L::rho]);
 #pragma unroll
     for (int c = 0; c < ND; ++c) p = p + u[L::b(c)] * u[L::b(c)];
     return (g - 1.0) * (u[L::e] - 0.5 * p);
   } else {
-    return (g - 1.0) * (u[L::e] - 0.5 * msq / u[L::rho]);
+    return (g - 1.0) * (u[L::e] - pdiv(0.5 * msq, u[L::rho]));
   }
 }
 
@@ -89,7 +111,7 @@ __device__ __forceinline__ double ptotal(const double* u, double g) {
   }
   double p = (1.0 - 0.5 * g) * bsq;
   p = p + (g - 1.0) * u[L::e];
-  p = p - 0.5 * (g - 1.0) * msq / u[L::rho];
+  p = p - pdiv(0.5 * (g - 1.0) * msq, u[L::rho]);
   return p;
 }
 
@@ -104,8 +126,8 @@ template <int ND, bool MHD, class R>
 __device__ __forceinline__ void fluxes(const double* u, int d, double g, double smallp, double* f) {
   using L = Layout<ND, MHD>;
   const R rho = R(u[L::rho]);
-  const R v = R(u[L::m(d)]) / rho;
-  const double vD = u[L::m(d)] / u[L::rho];
+  const R v = pdiv(R(u[L::m(d)]), rho);
+  const double vD = pdiv(u[L::m(d)], u[L::rho]);
   if constexpr (MHD) {
     const double ptD = ptotal<ND>(u, g);
     const R pt = R(ptD);
@@ -123,11 +145,11 @@ __device__ __forceinline__ void fluxes(const double* u, int d, double g, double 
     R bm = R(u[L::b(0)]) * R(u[L::m(0)]);
 #pragma unroll
     for (int c = 1; c < ND; ++c) bm = bm + R(u[L::b(c)]) * R(u[L::m(c)]);
-    f[L::e] = double((R(u[L::m(d)]) * pt - bd * bm) / rho + v * R(u[L::e]));
+    f[L::e] = double(pdiv(R(u[L::m(d)]) * pt - bd * bm, rho) + v * R(u[L::e]));
 #pragma unroll
     for (int c = 0; c < ND; ++c) {
       if (c == d) f[L::b(c)] = 0.0;
-      else f[L::b(c)] = double(-(R(u[L::m(c)]) / rho * bd) + v * R(u[L::b(c)]));
+      else f[L::b(c)] = double(-(pdiv(R(u[L::m(c)]), rho) * bd) + v * R(u[L::b(c)]));
     }
   } else {
     const double pD = pthermal<ND, false>(u, g, smallp);
@@ -138,7 +160,7 @@ __device__ __forceinline__ void fluxes(const double* u, int d, double g, double 
       if (c == d) f[L::m(c)] = pD + vD * u[L::m(c)];
       else f[L::m(c)] = double(R(0) + v * R(u[L::m(c)]));
     }
-    f[L::e] = double(R(u[L::m(d)]) / rho * p + v * R(u[L::e]));
+    f[L::e] = double(pdiv(R(u[L::m(d)]), rho) * p + v * R(u[L::e]));
   }
 }
 
@@ -150,17 +172,17 @@ __device__ __forceinline__ R cmax(const double* u, int d, double g, double small
   const double p = pthermal<ND, MHD>(u, g, smallp);
   const R rho = R(u[L::rho]);
   if constexpr (MHD) {
-    const R c2 = R(g) * R(p) / rho;
+    const R c2 = pdiv(R(g) * R(p), rho);
     R bsq = R(u[L::b(0)]) * R(u[L::b(0)]);
 #pragma unroll
     for (int c = 1; c < ND; ++c) bsq = bsq + R(u[L::b(c)]) * R(u[L::b(c)]);
-    const R cf2 = bsq / rho + c2;
+    const R cf2 = pdiv(bsq, rho) + c2;
     const R bi = R(u[L::b(d)]);
-    return R(sqrt(R(0.5) * (cf2 + R(sqrt(cf2 * cf2 - R(4) * c2 * (bi * bi) / rho))))) +
-           R(fabs(R(u[L::m(d)]) / rho));
+    return R(sqrt(R(0.5) * (cf2 + R(sqrt(cf2 * cf2 - pdiv(R(4) * c2 * (bi * bi), rho)))))) +
+           R(fabs(pdiv(R(u[L::m(d)]), rho)));
   } else {
-    const R cs = R(sqrt(R(g) * R(p) / rho));
-    return cs + R(fabs(R(u[L::m(d)]) / rho));
+    const R cs = R(sqrt(pdiv(R(g) * R(p), rho)));
+    return cs + R(fabs(pdiv(R(u[L::m(d)]), rho)));
   }
 }
 
@@ -175,11 +197,11 @@ __device__ __forceinline__ void powell(const double* uc, double divb_raw, double
   for (int c = 0; c < ND; ++c) out[L::m(c)] = out[L::m(c)] - uc[L::b(c)] * divb;
 #pragma unroll
   for (int c = 0; c < ND; ++c)
-    out[L::b(c)] = out[L::b(c)] - uc[L::m(c)] / uc[L::rho] * divb;
+    out[L::b(c)] = out[L::b(c)] - pdiv(uc[L::m(c)], uc[L::rho]) * divb;
   double mb = uc[L::m(0)] * uc[L::b(0)];
 #pragma unroll
   for (int c = 1; c < ND; ++c) mb = mb + uc[L::m(c)] * uc[L::b(c)];
-  out[L::e] = out[L::e] - mb / uc[L::rho] * divb;
+  out[L::e] = out[L::e] - pdiv(mb, uc[L::rho]) * divb;
 }
 
 // ---------------------------------------------------------------- Roe (MHD)
@@ -220,7 +242,7 @@ __device__ __forceinline__ void roe_setup(const double* uL, const double* uR, in
   R B[ND];
 #pragma unroll
   for (int c = 0; c < ND; ++c) {
-    r.v[c] = R(0.5) * (R(uL[L::m(c)]) / rL + R(uR[L::m(c)]) / rR);
+    r.v[c] = R(0.5) * (pdiv(R(uL[L::m(c)]), rL) + pdiv(R(uR[L::m(c)]), rR));
     B[c] = R(0.5) * (R(uL[L::b(c)]) + R(uR[L::b(c)]));
   }
   const double pL = pthermal<ND, true>(uL, g, smallp);
@@ -238,14 +260,14 @@ __device__ __forceinline__ void roe_setup(const double* uL, const double* uR, in
   const R drho = rR - rL;
   r.dp = (R(g) - R(1)) * (R(uR[L::e]) - R(uL[L::e]) - s1 + R(0.5) * s2 * drho - R(0.5) * s3);
   r.rhodv = R(uR[L::m(d)]) - R(uL[L::m(d)]) - r.v[d] * drho;
-  const R c2 = R(g) * pbar / rho;
+  const R c2 = pdiv(R(g) * pbar, rho);
   r.cs2 = c2;
   R bsq = B[0] * B[0];
 #pragma unroll
   for (int c = 1; c < ND; ++c) bsq = bsq + B[c] * B[c];
   const R bi = B[d];
-  R cf = bsq / rho + c2;
-  R cs = rmax(R(0), R(0.5) * (cf - R(sqrt(cf * cf - R(4) * c2 * (bi * bi) / rho))));
+  R cf = pdiv(bsq, rho) + c2;
+  R cs = rmax(R(0), R(0.5) * (cf - R(sqrt(cf * cf - pdiv(R(4) * c2 * (bi * bi), rho)))));
   cf = cf - cs;
   R af = (c2 - cs) / (cf - cs);
   af = rmin(R(1), rmax(af, R(0)));
@@ -262,8 +284,8 @@ __device__ __forceinline__ void roe_setup(const double* uL, const double* uR, in
   } else {
     const R t = R(sqrt(B[idir] * B[idir] + B[jdir] * B[jdir]));
     if (t > R(kSmall)) {
-      r.bw[idir] = B[idir] / t;
-      r.bw[jdir] = B[jdir] / t;
+      r.bw[idir] = pdiv(B[idir], t);
+      r.bw[jdir] = pdiv(B[jdir], t);
     } else {
       r.bw[idir] = R(sqrt(R(0.5)));
       r.bw[jdir] = R(sqrt(R(0.5)));
@@ -271,8 +293,8 @@ __device__ __forceinline__ void roe_setup(const double* uL, const double* uR, in
   }
   // fast/slow projections (il == fastRW_, :159-168)
   {
-    R t = r.bw[idir] * (R(uR[L::m(idir)]) / rR - R(uL[L::m(idir)]) / rL);
-    if constexpr (ND == 3) t = t + r.bw[jdir] * (R(uR[L::m(jdir)]) / rR - R(uL[L::m(jdir)]) / rL);
+    R t = r.bw[idir] * (pdiv(R(uR[L::m(idir)]), rR) - pdiv(R(uL[L::m(idir)]), rL));
+    if constexpr (ND == 3) t = t + r.bw[jdir] * (pdiv(R(uR[L::m(jdir)]), rR) - pdiv(R(uL[L::m(jdir)]), rL));
     r.bdvf = t * fsign(r.sq * r.sq, r.bw[d]);
     R u = r.bw[idir] * (R(uR[L::b(idir)]) - R(uL[L::b(idir)]));
     if constexpr (ND == 3) u = u + r.bw[jdir] * (R(uR[L::b(jdir)]) - R(uL[L::b(jdir)]));
@@ -280,8 +302,8 @@ __device__ __forceinline__ void roe_setup(const double* uL, const double* uR, in
   }
   if constexpr (ND == 3) {  // Alfven projections (il == alfvRW_, :176-183)
     const R bI = r.bw[idir], bJ = r.bw[jdir];
-    R t = bJ * (R(uR[L::m(idir)]) / rR - R(uL[L::m(idir)]) / rL) -
-          bI * (R(uR[L::m(jdir)]) / rR - R(uL[L::m(jdir)]) / rL);
+    R t = bJ * (pdiv(R(uR[L::m(idir)]), rR) - pdiv(R(uL[L::m(idir)]), rL)) -
+          bI * (pdiv(R(uR[L::m(jdir)]), rR) - pdiv(R(uL[L::m(jdir)]), rL));
     R u = bJ * (R(uR[L::b(idir)]) - R(uL[L::b(idir)])) - bI * (R(uR[L::b(jdir)]) - R(uL[L::b(jdir)]));
     r.bdva = t * R(0.5) * (r.sq * r.sq);
     r.bdba = u * R(0.5) * fsign(r.sq, r.bw[d]);
@@ -291,7 +313,7 @@ __device__ __forceinline__ void roe_setup(const double* uL, const double* uR, in
   R vs = r.v[0] * r.v[0];
 #pragma unroll
   for (int c = 1; c < ND; ++c) vs = vs + r.v[c] * r.v[c];
-  r.v2a2 = R(0.5) * vs + (R(2) - R(g)) / (R(g) - R(1)) * r.cs2;
+  r.v2a2 = R(0.5) * vs + pdiv(R(2) - R(g), R(g) - R(1)) * r.cs2;
   {
     R t = r.bw[idir] * r.v[idir];
     if constexpr (ND == 3) t = t + r.bw[jdir] * r.v[jdir];
@@ -330,23 +352,23 @@ __device__ __forceinline__ void roe_wave(const Roe<ND, R>& r, const double* uL, 
   switch (il) {
     case 0:
       a = vd + r.cf;
-      jump = R(0.5) / r.cs2 * (r.af * (+r.cf * r.rhodv + r.dp) + r.as * (-r.cs * r.bdvf + r.bdbf));
+      jump = pdiv(R(0.5), r.cs2) * (r.af * (+r.cf * r.rhodv + r.dp) + r.as * (-r.cs * r.bdvf + r.bdbf));
       break;
     case 1:
       a = vd - r.cf;
-      jump = R(0.5) / r.cs2 * (r.af * (-r.cf * r.rhodv + r.dp) + r.as * (+r.cs * r.bdvf + r.bdbf));
+      jump = pdiv(R(0.5), r.cs2) * (r.af * (-r.cf * r.rhodv + r.dp) + r.as * (+r.cs * r.bdvf + r.bdbf));
       break;
     case 2:
       a = vd + r.cs;
-      jump = R(0.5) / r.cs2 * (r.as * (+r.cs * r.rhodv + r.dp) + r.af * (+r.cf * r.bdvf - r.bdbf));
+      jump = pdiv(R(0.5), r.cs2) * (r.as * (+r.cs * r.rhodv + r.dp) + r.af * (+r.cf * r.bdvf - r.bdbf));
       break;
     case 3:
       a = vd - r.cs;
-      jump = R(0.5) / r.cs2 * (r.as * (-r.cs * r.rhodv + r.dp) + r.af * (-r.cf * r.bdvf - r.bdbf));
+      jump = pdiv(R(0.5), r.cs2) * (r.as * (-r.cs * r.rhodv + r.dp) + r.af * (-r.cf * r.bdvf - r.bdbf));
       break;
     case 4:
       a = vd;
-      jump = R(uR[L::rho]) - R(uL[L::rho]) - r.dp / r.cs2;
+      jump = R(uR[L::rho]) - R(uL[L::rho]) - pdiv(r.dp, r.cs2);
       break;
     case 5:
       if (divbwave) {
@@ -358,11 +380,11 @@ __device__ __forceinline__ void roe_wave(const Roe<ND, R>& r, const double* uL, 
       }
       break;
     case 6:
-      a = vd + R(fabs(r.bw[d])) / r.sq;
+      a = vd + pdiv(R(fabs(r.bw[d])), r.sq);
       jump = +r.bdva - r.bdba;
       break;
     default:
-      a = vd - R(fabs(r.bw[d])) / r.sq;
+      a = vd - pdiv(R(fabs(r.bw[d])), r.sq);
       jump = -r.bdva - r.bdba;
       break;
   }
@@ -431,14 +453,15 @@ __device__ __forceinline__ R roe_rtimes(const RoeRt<ND, R>& r, R q, int iw, int 
   const R beta = r.bw[c];
   switch (il) {
     case 0:
-    case 1: return +q * as * r.sqcs2 * beta / r.sq;
+    case 1: return +pdiv(q * as * r.sqcs2 * beta, r.sq);
     case 2:
-    case 3: return -(q * af * r.sqcs2 * beta / r.sq);
+    case 3: return -pdiv(q * af * r.sqcs2 * beta, r.sq);
     case 4:
     case 5: return R(0);
     default:
-      if (c == idir) return -(q * r.bw[jdir] / fsign(r.sq, r.bw[d]));
-      return +q * r.bw[idir] / fsign(r.sq, r.bw[d]);
+      // x / SIGN(sq, bw_d) == +-(x / sq): IEEE division is odd in its divisor
+      if (c == idir) return -(signbit(r.bw[d]) ? -pdiv(q * r.bw[jdir], r.sq) : pdiv(q * r.bw[jdir], r.sq));
+      return signbit(r.bw[d]) ? -pdiv(q * r.bw[idir], r.sq) : pdiv(q * r.bw[idir], r.sq);
   }
 }
 

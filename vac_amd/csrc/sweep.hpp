@@ -122,7 +122,7 @@ __device__ __forceinline__ double divb_at(const Geo& g, const double* __restrict
 #pragma unroll
   for (int d = 0; d < ND; ++d) {
     const double* b = w + L::b(d) * g.ps;
-    double gq = 0.5 * (b[idx + g.s[d]] - b[idx - g.s[d]]) / g.dx[d][c[d]];
+    double gq = pdiv(0.5 * (b[idx + g.s[d]] - b[idx - g.s[d]]), g.dx[d][c[d]]);
     dv = dv + gq;
   }
   return dv;
@@ -218,7 +218,7 @@ __device__ __forceinline__ void pred_compute(const Sel& s, const double* uc, con
   double fL[NW], fR[NW];
   fluxes<ND, MHD, R>(wl, d, s.gamma, smallp, fL);
   fluxes<ND, MHD, R>(wr, d, s.gamma, smallp, fR);
-  const double qdx = qdt / dxk[d];
+  const double qdx = pdiv(qdt, dxk[d]);
 #pragma unroll
   for (int iw = 0; iw < NW; ++iw) out[iw] = uc[iw] - qdx * (fL[iw] - fR[iw]);
   if constexpr (MHD && PWM != 0) {
@@ -229,7 +229,7 @@ __device__ __forceinline__ void pred_compute(const Sel& s, const double* uc, con
       for (int k = 0; k < ND; ++k) {
         const double bm = (k == d) ? um[L::b(k)] : bn[k][0];
         const double bp = (k == d) ? up[L::b(k)] : bn[k][1];
-        const double gq = 0.5 * (bp - bm) / dxk[k];
+        const double gq = pdiv(0.5 * (bp - bm), dxk[k]);
         dv = dv + gq;
       }
       powell<ND>(uc, dv, qdt * 1.0 / ND, out);
@@ -302,7 +302,7 @@ __device__ __forceinline__ void corr_face(const Sel& s, const double* pl, const 
 #pragma unroll
   for (int iw = 0; iw < NW; ++iw) mean[iw] = 0.5 * (wlf[iw] + wrf[iw]);
   const R cm = cmax<ND, MHD, R>(mean, d, s.gamma, smallp);
-  cv = double(cm) * 2.0 / dxsum;
+  cv = pdiv(double(cm) * 2.0, dxsum);
 #pragma unroll
   for (int iw = 0; iw < NW; ++iw) wlf[iw] = 2.0 * mean[iw] - wrf[iw];
   double fL[NW], fR[NW];
@@ -359,7 +359,7 @@ __device__ __forceinline__ double tvdlf_corr_body(const Geo& g, const Sel& s,
     corr_face<ND, MHD, R, D>(s, u0, up, l0, lp, g.dx[d][ci] + g.dx[d][ci + 1], smallp, F1, cv);
     cmx = cmx < cv ? cv : cmx;
     double out[NW];
-    const double qdx = dt / g.dx[d][ci];
+    const double qdx = pdiv(dt, g.dx[d][ci]);
 #pragma unroll
     for (int iw = 0; iw < NW; ++iw)
       out[iw] = w[iw * g.ps + cl.idx] - qdx * (F1[iw] - F0[iw]);
@@ -426,10 +426,28 @@ __device__ __forceinline__ int ghost_preimages(int c, int n, bool per_lo, bool p
   return k;
 }
 
+// Whole-wave shifts by one lane through DPP (wave_shl:1 / wave_shr:1, one
+// v_mov_b32_dpp per 32-bit half, no LDS round trip).  lane_next gives lane i
+// the value of lane i+1, lane_prev that of lane i-1; the lane with no source
+// (63, resp. 0) keeps its own value.
+template <int CTRL, class T>
+__device__ __forceinline__ T dpp_shift(T v) {
+  static_assert(sizeof(T) == 4 || sizeof(T) == 8, "32- or 64-bit values");
+  if constexpr (sizeof(T) == 8) {
+    const unsigned long long b = __builtin_bit_cast(unsigned long long, v);
+    const int lo = (int)(unsigned)b, hi = (int)(unsigned)(b >> 32);
+    const unsigned rlo = (unsigned)__builtin_amdgcn_update_dpp(lo, lo, CTRL, 0xf, 0xf, false);
+    const unsigned rhi = (unsigned)__builtin_amdgcn_update_dpp(hi, hi, CTRL, 0xf, 0xf, false);
+    return __builtin_bit_cast(T, ((unsigned long long)rhi << 32) | rlo);
+  } else {
+    const int x = __builtin_bit_cast(int, v);
+    return __builtin_bit_cast(T, __builtin_amdgcn_update_dpp(x, x, CTRL, 0xf, 0xf, false));
+  }
+}
 template <class T>
-__device__ __forceinline__ T lane_next(T v) { return __shfl_down(v, 1, 64); }
+__device__ __forceinline__ T lane_next(T v) { return dpp_shift<0x130>(v); }
 template <class T>
-__device__ __forceinline__ T lane_prev(T v) { return __shfl_up(v, 1, 64); }
+__device__ __forceinline__ T lane_prev(T v) { return dpp_shift<0x138>(v); }
 
 // store out at mesh cell c (index idx) and, when ghosts is set, at every
 // ghost cell whose image it is (one boundary type per face for all
@@ -518,7 +536,7 @@ __global__ __launch_bounds__(256) void k_tvdlf_corrx(Geo g, Sel s, BC bc, const 
   corr_face<ND, MHD, R, D>(s, u0, u1, l, lr, g.dx[0][cc] + g.dx[0][cc + 1], smallp, F, cv);
   double cmx = face_ok ? cv : -INFINITY;
   double out[NW];
-  const double qdx = dt / g.dx[0][cc];
+  const double qdx = pdiv(dt, g.dx[0][cc]);
 #pragma unroll
   for (int iw = 0; iw < NW; ++iw) {
     const double Fm = lane_prev(F[iw]);
@@ -601,7 +619,7 @@ __global__ __launch_bounds__(256) void k_tvdlf_corrm(Geo g, Sel s, BC bc, const 
       corr_face<ND, MHD, R, D>(s, u0, u1, lc, ln, dxd[i] + dxd[i + 1], smallp, F, cv);
       cmx = cmx < cv ? cv : cmx;
       const long long idx = base + (long long)i * sd;
-      const double qdx = dt / dxd[i];
+      const double qdx = pdiv(dt, dxd[i]);
       double out[NW];
 #pragma unroll
       for (int iw = 0; iw < NW; ++iw) out[iw] = wb[iw] - qdx * (F[iw] - Fp[iw]);
@@ -748,7 +766,7 @@ __device__ __forceinline__ double tvdlf_fx_body(const Geo& g, const Sel& s, cons
   if constexpr (MHD) byp = Pn[L::b(1)];
   pred_gather<MHD, R, 0, LIM>(g, f, s, w, px, s0, dt, smallp, Pc, Lc, Uc);
   const double dxs = dx0[c] + dx0[c + 1];
-  const double qdx = dt / dx0[c];
+  const double qdx = pdiv(dt, dx0[c]);
   // inputs of the next row's predictor are loaded one marching step ahead
   PredIn<MHD> in;
   pred_load<MHD, 0>(g, f, w, px, ghost_image(s0 + 1, f.n1, f.pylo, f.pyhi), in);
@@ -777,9 +795,9 @@ __device__ __forceinline__ double tvdlf_fx_body(const Geo& g, const Sel& s, cons
     if constexpr (MHD) {
       const double bxp = lane_next(Pc[L::b(0)]), bxm = lane_prev(Pc[L::b(0)]);
       double dv = 0.0;
-      const double g0 = 0.5 * (bxp - bxm) / dx0[c];
+      const double g0 = pdiv(0.5 * (bxp - bxm), dx0[c]);
       dv = dv + g0;
-      const double g1 = 0.5 * (Pn[L::b(1)] - byp) / dx1[j];
+      const double g1 = pdiv(0.5 * (Pn[L::b(1)] - byp), dx1[j]);
       dv = dv + g1;
       powell<ND>(Pc, dv, dt * 1.0 / ND, out);
       byp = Pc[L::b(1)];
@@ -897,16 +915,16 @@ __device__ __forceinline__ double tvdlf_fy_body(const Geo& g, const Sel& s, cons
     }
     corr_face<ND, MHD, R, 1>(s, Pc, Pn, lc, ln, dx1[j] + dx1[j + 1], smallp, Fn, cv);
     if (useful) cmx = cmx < cv ? cv : cmx;
-    const double qdx = dt / dx1[j];
+    const double qdx = pdiv(dt, dx1[j]);
     double out[NW];
 #pragma unroll
     for (int iw = 0; iw < NW; ++iw) out[iw] = W0[iw] - qdx * (Fn[iw] - Fp[iw]);
     if constexpr (MHD) {
       const double bxp = lane_next(Pc[L::b(0)]), bxm = lane_prev(Pc[L::b(0)]);
       double dv = 0.0;
-      const double g0 = 0.5 * (bxp - bxm) / dx0[c];
+      const double g0 = pdiv(0.5 * (bxp - bxm), dx0[c]);
       dv = dv + g0;
-      const double g1 = 0.5 * (Pn[L::b(1)] - byp) / dx1[j];
+      const double g1 = pdiv(0.5 * (Pn[L::b(1)] - byp), dx1[j]);
       dv = dv + g1;
       powell<ND>(Pc, dv, dt * 1.0 / ND, out);
       byp = Pc[L::b(1)];
@@ -1091,7 +1109,7 @@ __device__ __forceinline__ void tvd_face(const double* uL, const double* uR, int
   for (int il = 0; il < NW; ++il) {
     R a, jp;
     roe_wave<ND, R>(r, uL, uR, il, d, s.divbwave, a, jp);
-    const R ad = a * R(dt) * R(2) / sdx;
+    const R ad = pdiv(a * R(dt) * R(2), sdx);
     f.jump[il] = jp;
     f.adt[il] = ad;
   }
@@ -1112,15 +1130,15 @@ __device__ __forceinline__ void tvd_dissipate(const RoeRt<ND, R>& rm, const R* p
     for (int iw = 0; iw < NW; ++iw) {
       const R a0 = roe_rtimes<ND, R>(rm, phim[il], iw, il, d, s.divbwave) * R(0.25) * dvm;
       const R a1 = roe_rtimes<ND, R>(r0, phi0[il], iw, il, d, s.divbwave) * R(0.25) * dv0;
-      out[iw] = out[iw] + (double(a1) - double(a0)) / dvi;
+      out[iw] = out[iw] + pdiv(double(a1) - double(a0), dvi);
     }
   }
 }
 
 template <class T>
-__device__ __forceinline__ T shfl_up1(T v) { return __shfl_up(v, 1, 64); }
+__device__ __forceinline__ T shfl_up1(T v) { return lane_prev(v); }
 template <class T>
-__device__ __forceinline__ T shfl_down1(T v) { return __shfl_down(v, 1, 64); }
+__device__ __forceinline__ T shfl_down1(T v) { return lane_next(v); }
 
 template <class S, class R>
 __device__ __forceinline__ S shfl_up_struct(const S& v) {
@@ -1194,7 +1212,7 @@ __global__ __launch_bounds__(256) void k_tvd_x(Geo g, Sel s, BC bc, const double
   if (rowok && lane >= 2 && lane <= 62 && i <= g.n[0] + 1) {
     const long long idx = base + i;
     double out[NW];
-    const double qdx = dt / g.dx[0][i];
+    const double qdx = pdiv(dt, g.dx[0][i]);
 #pragma unroll
     for (int iw = 0; iw < NW; ++iw) out[iw] = win[iw * g.ps + idx] - qdx * (fC[iw] - fCm[iw]);
     // dvolume = (dx1*dx2)*dx3 evaluated left to right
@@ -1318,7 +1336,7 @@ __global__ __launch_bounds__(256) void k_tvd_m(Geo g, Sel s, BC bc, const double
       // update cell i
       const long long idx = base + (long long)i * sd;
       double out[NW];
-      const double qdx = dt / dxd[i];
+      const double qdx = pdiv(dt, dxd[i]);
 #pragma unroll
       for (int iw = 0; iw < NW; ++iw) out[iw] = win[iw * g.ps + idx] - qdx * (fC[iw] - fCm[iw]);
       const double dm = dvol(i - 1), d0 = dvol(i), dp = dvol(i + 1);
@@ -1416,7 +1434,7 @@ __device__ __forceinline__ double cmaxdx_body(const Geo& g, const double* __rest
   if (cl.ok) {
     double u[L::nw];
     load<L::nw>(w, g.ps, cl.idx, u);
-    v = cmax<ND, MHD, double>(u, d, gamma, smallp) / g.dx[d][cl.c[d]];
+    v = pdiv(cmax<ND, MHD, double>(u, d, gamma, smallp), g.dx[d][cl.c[d]]);
   }
   return v;
 }
@@ -1552,7 +1570,7 @@ __global__ __launch_bounds__(1024) void k_getdt_lagged(Geo g, const double* __re
         int c[3];
         double u[L::nw];
         load<L::nw>(w, g.ps, cell(t, c), u);
-        const double q = cmax<ND, MHD, double>(u, d, gamma, smallp) / g.dx[d][c[d]];
+        const double q = pdiv(cmax<ND, MHD, double>(u, d, gamma, smallp), g.dx[d][c[d]]);
         v = v < q ? q : v;
       }
       cmd[d] = block_reduce<true>(v, sm);

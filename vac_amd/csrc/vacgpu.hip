@@ -240,15 +240,57 @@ int getdt_device(vacgpu_grid* g, const vacgpu_method* m, double dtpar, int tmaxe
   return launch_check(g, "getdt");
 }
 
-// one dimensional sweep (advect, src/vac.t:363-486), dt already on device
+// Blocks of the fused sweeps that the device holds at once: CUs times the
+// resident blocks per CU of the x and y sweep kernels (the fewer), or 0 when
+// the runtime cannot say.
+long long fused_slots(const vacgpu_grid* g) {
+  int dev = 0, cus = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return 0;
+  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return 0;
+  const void* fx = nullptr;
+  const void* fy = nullptr;
+  const bool mhd = g->cfg.physics == VACGPU_MHD, f32 = g->cfg.precision == VACGPU_FP32BULK;
+#define FSLOT(MV, RT)                                                     \
+  do {                                                                    \
+    fx = reinterpret_cast<const void*>(&k_tvdlf_fused<MV, RT, 0, 0>);     \
+    fy = reinterpret_cast<const void*>(&k_tvdlf_fused<MV, RT, 1, 0>);     \
+  } while (0)
+  if (mhd) { if (f32) FSLOT(true, float); else FSLOT(true, double); }
+  else { if (f32) FSLOT(false, float); else FSLOT(false, double); }
+#undef FSLOT
+  int a = 0, b = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&a, fx, 256, 0) != hipSuccess) return 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, fy, 256, 0) != hipSuccess) return 0;
+  const int per_cu = a < b ? a : b;
+  return per_cu > 0 && cus > 0 ? (long long)per_cu * cus : 0;
+}
+
 // launch shape of the fused TVDLF sweeps: strips of 62 x cells, waves of
-// one segment of seglen rows each, four waves per block.  seglen 0 picks a
-// length that gives about 4096 waves.
+// one segment of seglen rows each, four waves per block.  seglen 0 picks the
+// length with the least estimated time (below), or about 4096 waves when
+// the occupancy is unknown.
 void set_fused_shape(vacgpu_grid* g, int seglen) {
   const int strips = (g->geo.n[0] + FS_STRIDE - 1) / FS_STRIDE;
   if (seglen <= 0) {
-    const int want = (4096 + strips - 1) / strips;
-    seglen = (g->geo.n[1] + want - 1) / want;
+    const long long slots = fused_slots(g);
+    if (slots > 0) {
+      // Every wave marches seglen rows after a prologue worth about 1.5 rows
+      // (two predictor rows); the waves of a launch run in rounds of `slots`
+      // blocks, and a partly filled last round costs a whole wave time.
+      double best = 1e300;
+      for (int sl = 4; sl <= 256; ++sl) {
+        const long long segs = (g->geo.n[1] + sl - 1) / sl;
+        const long long blocks = (long long)strips * ((segs + 3) / 4);
+        const double cost = (double)((blocks + slots - 1) / slots) * (sl + 1.5);
+        if (cost < best) {
+          best = cost;
+          seglen = sl;
+        }
+      }
+    } else {
+      const int want = (4096 + strips - 1) / strips;
+      seglen = (g->geo.n[1] + want - 1) / want;
+    }
   }
   seglen = seglen < 4 ? 4 : (seglen > 256 ? 256 : seglen);
   g->fs_seglen = seglen;
@@ -277,6 +319,7 @@ int common_limiter(const vacgpu_grid* g, const vacgpu_method* m) {
   return m->typelimiter[0];
 }
 
+// one dimensional sweep (advect, src/vac.t:363-486), dt already on device
 int sweep_device(vacgpu_grid* g, const vacgpu_method* m, int d, int firstsweep, int lastsweep) {
   if (firstsweep) g->old = g->cur;
   const int outi = other_buffer(g);
@@ -525,6 +568,15 @@ int write_scalars(vacgpu_grid* g) {
 }
 
 }  // namespace
+
+// pdiv, the quotient of the sweep kernels, beside the compiler's division
+__global__ void k_check_quotients(const double* __restrict__ a, const double* __restrict__ b,
+                                  long long n, double* __restrict__ qk, double* __restrict__ qi) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  qk[i] = pdiv(a[i], b[i]);
+  qi[i] = a[i] / b[i];
+}
 
 extern "C" {
 
@@ -904,6 +956,26 @@ int vacgpu_kernel_time(vacgpu_grid* g, int k, double* avg_ms, long* count) {
   if (avg_ms) *avg_ms = g->tcnt[k] ? g->tsum[k] / g->tcnt[k] : 0.0;
   if (count) *count = g->tcnt[k];
   return 0;
+}
+
+int vacgpu_check_quotients(const double* a, const double* b, long long n, double* q_kernel,
+                           double* q_ieee) {
+  if (!a || !b || !q_kernel || !q_ieee || n < 0) return VACGPU_E_ARG;
+  if (n == 0) return 0;
+  const size_t bytes = sizeof(double) * (size_t)n;
+  double* d = nullptr;
+  if (hipMalloc(&d, 4 * bytes) != hipSuccess) return VACGPU_E_HIP;
+  hipError_t e = hipMemcpy(d, a, bytes, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(d + n, b, bytes, hipMemcpyHostToDevice);
+  if (e == hipSuccess) {
+    hipLaunchKernelGGL(k_check_quotients, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, nullptr,
+                       d, d + n, n, d + 2 * n, d + 3 * n);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipMemcpy(q_kernel, d + 2 * n, bytes, hipMemcpyDeviceToHost);
+  if (e == hipSuccess) e = hipMemcpy(q_ieee, d + 3 * n, bytes, hipMemcpyDeviceToHost);
+  (void)hipFree(d);
+  return e == hipSuccess ? 0 : VACGPU_E_HIP;
 }
 
 const char* vacgpu_last_error(const vacgpu_grid* g) {

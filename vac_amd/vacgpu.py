@@ -61,6 +61,7 @@ SIGNATURES = {
     "vacgpu_profile": ([C.c_void_p, C.c_int], C.c_int),
     "vacgpu_kernel_time": ([C.c_void_p, C.c_int, _DP, C.POINTER(C.c_long)], C.c_int),
     "vacgpu_set_kernels": ([C.c_void_p, C.c_int, C.c_int], C.c_int),
+    "vacgpu_check_quotients": ([_DP, _DP, C.c_longlong, _DP, _DP], C.c_int),
     "vacgpu_supported": ([C.POINTER(Config), C.POINTER(Method)], C.c_int),
     "vacgpu_last_error": ([C.c_void_p], C.c_char_p),
     "vacgpu_destroy": ([C.c_void_p], None),
