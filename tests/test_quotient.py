@@ -54,5 +54,10 @@ def test_quotient_matches_ieee_division():
     assert lib.vacgpu_check_quotients(p(a), p(b), n, p(qk), p(qi)) == 0
     # the compiler's division is correctly rounded: numpy's quotient
     assert np.array_equal(qi.view(np.uint64), (a / b).view(np.uint64))
+    # every operand here lies in the stated range
+    tiny = np.finfo(np.float64).tiny
+    q = np.abs(a / b)
+    assert np.all((b >= tiny) & (1.0 / b >= tiny) & ((a == 0) | (np.abs(a) >= 2.0**-969))
+                  & ((a == 0) | (q >= tiny)) & np.isfinite(q))
     bad = np.nonzero(qk.view(np.uint64) != qi.view(np.uint64))[0]
     assert bad.size == 0, [(a[i], b[i], qk[i], qi[i]) for i in bad[:5]]

@@ -32,7 +32,7 @@ __device__ __forceinline__ void dpress(const float* b, const float* q, float gm1
     m2 += b[L::m(c)] * b[L::m(c)];
   }
   const float rho = b[L::rho], rho1 = b[L::rho] + q[L::rho];
-  const float dkin = (rho * dm2 - q[L::rho] * m2) / (rho * rho1);  // d(m^2/rho)
+  const float dkin = pdiv(rho * dm2 - q[L::rho] * m2, rho * rho1);  // d(m^2/rho)
   if constexpr (MHD) {
     float db2 = (2.f * b[L::b(0)] + q[L::b(0)]) * q[L::b(0)];
 #pragma unroll
@@ -69,7 +69,7 @@ __device__ __forceinline__ void flux32(const float* s, double pmd, float pf, flo
   using L = Layout<ND, MHD>;
   constexpr int d = D;
   const float rho = s[L::rho];
-  const float v = s[L::m(d)] / rho;
+  const float v = pdiv(s[L::m(d)], rho);
   f[L::rho] = v * rho;
   if constexpr (MHD) {
     const float bd = s[L::b(d)];
@@ -78,9 +78,9 @@ __device__ __forceinline__ void flux32(const float* s, double pmd, float pf, flo
     float bm = s[L::b(0)] * s[L::m(0)];
 #pragma unroll
     for (int c = 1; c < ND; ++c) bm += s[L::b(c)] * s[L::m(c)];
-    f[L::e] = (s[L::m(d)] * pf - bd * bm) / rho + v * s[L::e];
+    f[L::e] = pdiv(s[L::m(d)] * pf - bd * bm, rho) + v * s[L::e];
 #pragma unroll
-    for (int c = 0; c < ND; ++c) f[L::b(c)] = (c == d) ? 0.f : -(s[L::m(c)] / rho * bd) + v * s[L::b(c)];
+    for (int c = 0; c < ND; ++c) f[L::b(c)] = (c == d) ? 0.f : -(pdiv(s[L::m(c)], rho) * bd) + v * s[L::b(c)];
   } else {
 #pragma unroll
     for (int c = 0; c < ND; ++c) f[L::m(c)] = v * s[L::m(c)];
@@ -127,19 +127,19 @@ __device__ __forceinline__ float cmax_of(const float* b, const float* q, double 
   double p = pth + double(dph);
   p = p < smallp ? smallp : p;
   const float rho = s[L::rho];
-  const float c2 = g * float(p) / rho;
+  const float c2 = pdiv(g * float(p), rho);
   if constexpr (MHD) {
     float bsq = s[L::b(0)] * s[L::b(0)];
 #pragma unroll
     for (int c = 1; c < ND; ++c) bsq += s[L::b(c)] * s[L::b(c)];
-    const float cf2 = bsq / rho + c2;
+    const float cf2 = pdiv(bsq, rho) + c2;
     const float bi = s[L::b(D)];
     // the discriminant is >= 0 in exact arithmetic; FP32 rounding can take it
     // a hair below 0 where slow and fast speeds meet
-    const float disc = fmaxf(cf2 * cf2 - 4.f * c2 * (bi * bi) / rho, 0.f);
-    return sqrtf(0.5f * (cf2 + sqrtf(disc))) + fabsf(s[L::m(D)] / rho);
+    const float disc = fmaxf(cf2 * cf2 - pdiv(4.f * c2 * (bi * bi), rho), 0.f);
+    return sqrtf(0.5f * (cf2 + sqrtf(disc))) + fabsf(pdiv(s[L::m(D)], rho));
   } else {
-    return sqrtf(c2) + fabsf(s[L::m(D)] / rho);
+    return sqrtf(c2) + fabsf(pdiv(s[L::m(D)], rho));
   }
 }
 
@@ -156,9 +156,9 @@ __device__ __forceinline__ void powell32(const float* s, float divb, float* inc)
 #pragma unroll
   for (int c = 0; c < ND; ++c) {
     inc[L::m(c)] -= s[L::b(c)] * divb;
-    inc[L::b(c)] -= s[L::m(c)] / rho * divb;
+    inc[L::b(c)] -= pdiv(s[L::m(c)], rho) * divb;
   }
-  inc[L::e] -= mb / rho * divb;
+  inc[L::e] -= pdiv(mb, rho) * divb;
 }
 
 // The predictor in increments: dP = P - U at a cell with base U (FP64) and
@@ -194,7 +194,7 @@ __device__ __forceinline__ void pred32(const Sel& s, const double* uc, const dou
 #pragma unroll
     for (int k = 0; k < 2; ++k) {
       const float bd = (k == D) ? float(up[L::b(k)] - um[L::b(k)]) : bdiff[k];
-      dv += 0.5f * bd / float(dxk[k]);
+      dv += pdiv(0.5f * bd, float(dxk[k]));
     }
     powell32<2>(b, dv * float(dt / 2 / 2), dP);
   }
@@ -336,8 +336,8 @@ __device__ __forceinline__ double bulk_fx_body(const Geo& g, const Sel& s, const
     if constexpr (MHD) {
       const double bxp = lane_next(U0[L::b(0)]), bxm = lane_prev(U0[L::b(0)]);
       const float dxp_ = lane_next(dPbx0), dxm_ = lane_prev(dPbx0);
-      const float gx = 0.5f * (float(bxp - bxm) + (dxp_ - dxm_)) / float(dx0[c]);
-      const float gy = 0.5f * (float(R.U[L::b(1)] - byU_m) + (R.dP[L::b(1)] - byd_m)) / float(dx1[j]);
+      const float gx = pdiv(0.5f * (float(bxp - bxm) + (dxp_ - dxm_)), float(dx0[c]));
+      const float gy = pdiv(0.5f * (float(R.U[L::b(1)] - byU_m) + (R.dP[L::b(1)] - byd_m)), float(dx1[j]));
       powell32<ND>(sP0, (gx + gy) * float(dt / ND), inc);
     }
     double out[NW];
@@ -448,8 +448,8 @@ __device__ __forceinline__ double bulk_fy_body(const Geo& g, const Sel& s, const
     if constexpr (MHD) {
       const double bxp = lane_next(U0[L::b(0)]), bxm = lane_prev(U0[L::b(0)]);
       const float dxp_ = lane_next(dPbx0), dxm_ = lane_prev(dPbx0);
-      const float gx = 0.5f * (float(bxp - bxm) + (dxp_ - dxm_)) / float(dx0[c]);
-      const float gy = 0.5f * (float(R.U[L::b(1)] - byU_m) + (R.dP[L::b(1)] - byd_m)) / float(dx1[j]);
+      const float gx = pdiv(0.5f * (float(bxp - bxm) + (dxp_ - dxm_)), float(dx0[c]));
+      const float gy = pdiv(0.5f * (float(R.U[L::b(1)] - byU_m) + (R.dP[L::b(1)] - byd_m)), float(dx1[j]));
       powell32<ND>(sP0, (gx + gy) * float(dt / ND), inc);
       byU_m = U0[L::b(1)];
       byd_m = dPby0;

@@ -46,7 +46,8 @@ __device__ __forceinline__ R fsign(R a, R b) { return copysign(fabs(a), b); }
 // b alone, so quotients with one denominator share it (8 instructions for the
 // first, 3 for each further one, against 11 for every division).  The
 // residual is formed negated, which keeps the sign of a zero numerator.
-// float: the ordinary division.
+// float (FP32-bulk tier only): a times the hardware reciprocal of b (about
+// 1 ulp), shared like the double one; the tier's 1e-5 error budget allows it.
 __device__ __forceinline__ double pdiv(double a, double b) {
   const double y0 = __builtin_amdgcn_rcp(b);
   const double y1 = __builtin_fma(y0, __builtin_fma(-b, y0, 1.0), y0);
@@ -54,7 +55,7 @@ __device__ __forceinline__ double pdiv(double a, double b) {
   const double q0 = a * y;
   return __builtin_fma(-__builtin_fma(b, q0, -a), y, q0);
 }
-__device__ __forceinline__ float pdiv(float a, float b) { return a / b; }
+__device__ __forceinline__ float pdiv(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
 
 // dwlimiter2 (src/vacgrid.t:938-996): a = dwC(i), b = dwC(i-1).  LIM >= 0
 // fixes the type at compile time (no branches); LIM < 0 takes it from type.
