@@ -130,6 +130,21 @@ module vacgpu_mod
        integer(c_long), intent(out) :: count
      end function vacgpu_kernel_time
 
+     integer(c_int) function vacgpu_set_kernels(grid, fused, seglen) &
+          bind(C, name='vacgpu_set_kernels')
+       import :: c_int, c_ptr
+       type(c_ptr), value :: grid
+       integer(c_int), value :: fused, seglen
+     end function vacgpu_set_kernels
+
+     integer(c_int) function vacgpu_check_quotients(a, b, n, q_kernel, q_ieee) &
+          bind(C, name='vacgpu_check_quotients')
+       import :: c_int, c_double, c_long_long
+       real(c_double), intent(in) :: a(*), b(*)
+       integer(c_long_long), value :: n
+       real(c_double), intent(out) :: q_kernel(*), q_ieee(*)
+     end function vacgpu_check_quotients
+
      integer(c_int) function vacgpu_supported(cfg, m) bind(C, name='vacgpu_supported')
        import :: c_int, vacgpu_config, vacgpu_method
        type(vacgpu_config), intent(in) :: cfg
