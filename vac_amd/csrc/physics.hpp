@@ -59,15 +59,21 @@ __device__ __forceinline__ float pdiv(float a, float b) { return a * __builtin_a
 
 // dwlimiter2 (src/vacgrid.t:938-996): a = dwC(i), b = dwC(i-1).  LIM >= 0
 // fixes the type at compile time (no branches); LIM < 0 takes it from type.
+// max / min are the hardware v_max / v_min (one instruction, against a compare
+// and two selects for rmax / rmin).  They differ from max(x, y) = x < y ? y : x
+// only in the sign of a zero result when the operands are +0 and -0, and every
+// such inner min feeds a max whose other operand is +0 (or a value >= +0 that
+// is itself such a max), which returns +0 either way: the limited slope is the
+// same double.  (NaN operands do not occur on a valid state.)
 template <int LIM, class R>
 __device__ __forceinline__ R limiter_t(int type, R a, R b) {
   const int t = LIM >= 0 ? LIM : type;
   const R one = R(1), zero = R(0);
   R s = copysign(one, a);
-  if (t == LIM_MINMOD) return s * rmax(zero, rmin(R(fabs(a)), s * b));
+  if (t == LIM_MINMOD) return s * fmax(zero, fmin(R(fabs(a)), s * b));
   if (t == LIM_WOODWARD)
-    return R(2) * s * rmax(zero, rmin(rmin(R(fabs(a)), s * b), s * R(0.25) * (b + a)));
-  return s * rmax(rmax(zero, rmin(R(2) * R(fabs(a)), s * b)), rmin(R(fabs(a)), R(2) * s * b));
+    return R(2) * s * fmax(zero, fmin(fmin(R(fabs(a)), s * b), s * R(0.25) * (b + a)));
+  return s * fmax(fmax(zero, fmin(R(2) * R(fabs(a)), s * b)), fmin(R(fabs(a)), R(2) * s * b));
 }
 
 template <class R>
