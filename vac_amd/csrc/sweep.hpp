@@ -429,19 +429,21 @@ __device__ __forceinline__ int ghost_preimages(int c, int n, bool per_lo, bool p
 // Whole-wave shifts by one lane through DPP (wave_shl:1 / wave_shr:1, one
 // v_mov_b32_dpp per 32-bit half, no LDS round trip).  lane_next gives lane i
 // the value of lane i+1, lane_prev that of lane i-1; the lane with no source
-// (63, resp. 0) keeps its own value.
+// (63, resp. 0) reads 0 (bound_ctrl), so no copy of the old value is needed.
+// Every caller uses lanes 0 and 63 as halo lanes whose shifted-in values
+// reach no output.
 template <int CTRL, class T>
 __device__ __forceinline__ T dpp_shift(T v) {
   static_assert(sizeof(T) == 4 || sizeof(T) == 8, "32- or 64-bit values");
   if constexpr (sizeof(T) == 8) {
     const unsigned long long b = __builtin_bit_cast(unsigned long long, v);
     const int lo = (int)(unsigned)b, hi = (int)(unsigned)(b >> 32);
-    const unsigned rlo = (unsigned)__builtin_amdgcn_update_dpp(lo, lo, CTRL, 0xf, 0xf, false);
-    const unsigned rhi = (unsigned)__builtin_amdgcn_update_dpp(hi, hi, CTRL, 0xf, 0xf, false);
+    const unsigned rlo = (unsigned)__builtin_amdgcn_update_dpp(0, lo, CTRL, 0xf, 0xf, true);
+    const unsigned rhi = (unsigned)__builtin_amdgcn_update_dpp(0, hi, CTRL, 0xf, 0xf, true);
     return __builtin_bit_cast(T, ((unsigned long long)rhi << 32) | rlo);
   } else {
     const int x = __builtin_bit_cast(int, v);
-    return __builtin_bit_cast(T, __builtin_amdgcn_update_dpp(x, x, CTRL, 0xf, 0xf, false));
+    return __builtin_bit_cast(T, __builtin_amdgcn_update_dpp(0, x, CTRL, 0xf, 0xf, true));
   }
 }
 template <class T>
