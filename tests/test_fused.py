@@ -202,3 +202,25 @@ def test_two_launch_tvdlf_writes_ghost_cells(case):
     assert np.array_equal(out[0][1], tr)
     mesh = (slice(None),) + (slice(2, -2),) * len(shape)
     assert np.array_equal(out[0][0][mesh], wr)
+
+
+@pytest.mark.parametrize("bc", ["periodic", "cont"])
+def test_tvd_3d_ghost_writes_and_ryu_pass(bc):
+    # 3D TVD: the x sweep writes its ghost cells and the last (z) sweep's Ryu
+    # step runs as one pass over wold and wnew; the whole ixG array must equal
+    # the path with separate boundary passes and Ryu halves
+    shape = (18, 12, 10)
+    w0, coords = random_fourier(shape, 41, 5 / 3)
+    dxs = [widths_1d(c) for c in coords]
+    method = vacgpu.MethodSpec(typefull="tvd", limiter="woodward", courantpar=0.8)
+    print(f"case tvd 3d ghosts {bc}", flush=True)
+    out = []
+    for fused in (True, False):
+        g = vacgpu.Grid(list(shape), "mhd", 5 / 3, dxs, bc=bc)
+        g.set_kernels(fused)
+        g.put_mesh(w0)
+        n, t = g.run(method, 4, record_t=True)
+        out.append((g.get_state(), t))
+        g.close()
+    assert np.array_equal(out[0][1], out[1][1])
+    assert np.array_equal(out[0][0], out[1][0])

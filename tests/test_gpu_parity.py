@@ -130,6 +130,7 @@ def test_host_loop_through_advect_matches_run(typefull, limiter):
     ("hd", (48, 40), "tvdlf", "minmod", "cont", 20),
     ("hd", (16, 20, 12), "tvdlf", "minmod", "periodic", 8),
     ("mhd", (64, 40), "tvd", "superbee", "cont", 15),
+    ("mhd", (14, 12, 10), "tvd", "minmod", "cont", 5),
 ])
 def test_gpu_matches_oracle(case):
     physics, nx, typefull, limiter, bc, steps = case
@@ -143,3 +144,24 @@ def test_gpu_matches_oracle(case):
     wr, tr, st = vacref.run(p, w0, [widths_1d(c) for c in coords], steps)
     assert np.array_equal(t, tr)
     assert_bitwise(w, wr, str(case))
+
+
+@pytest.mark.parametrize("typefull,limiter", [("tvd", "woodward"), ("tvd", "minmod"),
+                                              ("tvdlf", "minmod")])
+def test_courant_violation_counted_like_vac(typefull, limiter):
+    # courantpar 2: every sweep's Courant fold sees CFL > 1 for the extreme
+    # waves and must count it (nerror(couranterr_)) exactly as VAC does, with
+    # the same dt history and state
+    g = 5.0 / 3.0
+    nx = [52, 44]
+    w0, coords = random_fourier(tuple(nx), 77, g)
+    dxs = [widths_1d(c) for c in coords]
+    method = vacgpu.MethodSpec(typefull=typefull, limiter=limiter, courantpar=2.0)
+    p = vacref.make_params(2, "mhd", nx, g, typefull=typefull, limiter=limiter, courantpar=2.0)
+    wr, tr, st = vacref.run(p, w0, dxs, 4)
+    print(f"case courant violation {typefull} {limiter}", flush=True)
+    w, t, sc = gpu_run(w0, nx, "mhd", g, 4, method)
+    assert st.nerror[2] > 0
+    assert list(sc["nerror"]) == list(st.nerror)
+    assert np.array_equal(t, tr)
+    assert_bitwise(w, wr, "courant violation")

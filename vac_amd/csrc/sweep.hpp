@@ -1378,6 +1378,57 @@ __global__ __launch_bounds__(256) void k_ryu_a(Geo g, BC bc, const double* __res
 }
 
 // ... then wnew = wnew - dt*S(wt).
+// Both halves of Ryu's source step in one pass, for a last sweep that did
+// not fold the first half (one boundary type per face for all variables):
+// wnew - dt S(wt) with wt = (wold + wnew - dt S(wold))/2 evaluated where it
+// is needed, at the cell and, for the B component that divB(wt) takes, at
+// the images of its neighbours (the ghost values getboundary would give wt),
+// with the arithmetic of k_ryu_a and k_ryu_b.  wnew is read only; the result
+// goes to wout (a buffer the step no longer needs) with its ghost cells.
+template <int ND>
+__device__ __forceinline__ long long cell_index(const Geo& g, const int* c) {
+  return g.off + c[0] + (long long)c[1] * g.s[1] + (ND == 3 ? (long long)c[2] * g.s[2] : 0);
+}
+template <int ND>
+__global__ __launch_bounds__(256) void k_ryu_ab(Geo g, BC bc, const double* __restrict__ wold,
+                                               const double* __restrict__ wnew,
+                                               double* __restrict__ wout,
+                                               const Scal* __restrict__ sc) {
+  using L = Layout<ND, true>;
+  constexpr int NW = L::nw;
+  const CellIx cl = cell_of(g);
+  if (!cl.ok) return;
+  const double dt = sc->dt;
+  double ut[NW], o[NW];
+  {
+    double uo[NW];
+    load<NW>(wold, g.ps, cl.idx, uo);
+    load<NW>(wnew, g.ps, cl.idx, ut);
+    powell<ND>(uo, divb_at<ND>(g, wold, cl.idx, cl.c), dt, ut);
+#pragma unroll
+    for (int iw = 0; iw < NW; ++iw) ut[iw] = 0.5 * (uo[iw] + ut[iw]);
+  }
+  // B_d of wt at a mesh cell: the b(d) row of the block above
+  auto wt_b = [&](int d, const int* c) {
+    const long long k = cell_index<ND>(g, c);
+    const double divb = dt * divb_at<ND>(g, wold, k, c);
+    const double t = wnew[L::b(d) * g.ps + k] - pdiv(wold[L::m(d) * g.ps + k], wold[k]) * divb;
+    return 0.5 * (wold[L::b(d) * g.ps + k] + t);
+  };
+  double dv = 0.0;
+#pragma unroll
+  for (int d = 0; d < ND; ++d) {
+    int cm[3] = {cl.c[0], cl.c[1], cl.c[2]}, cp[3] = {cl.c[0], cl.c[1], cl.c[2]};
+    cm[d] = image_of(g, bc, d, cl.c[d] - 1);
+    cp[d] = image_of(g, bc, d, cl.c[d] + 1);
+    const double gq = pdiv(0.5 * (wt_b(d, cp) - wt_b(d, cm)), g.dx[d][cl.c[d]]);
+    dv = dv + gq;
+  }
+  load<NW>(wnew, g.ps, cl.idx, o);
+  powell<ND>(ut, dv, dt, o);
+  store_cell<ND, NW>(g, bc, wout, cl.idx, cl.c, o, 1);
+}
+
 template <int ND>
 __global__ __launch_bounds__(256) void k_ryu_b(Geo g, BC bc, const double* __restrict__ wt,
                                               double* __restrict__ wnew,

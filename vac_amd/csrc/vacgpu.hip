@@ -511,6 +511,16 @@ int sweep_device(vacgpu_grid* g, const vacgpu_method* m, int d, int firstsweep, 
       if ((rc = launch_check(g, "k_fin_courant_tvd"))) return rc;
     }
     const bool ryu_folded = srcmode == 2;
+    if (lastsweep && src && ghosts && !ryu_folded && !firstsweep) {
+      // both Ryu halves in one pass, into the buffer of this sweep's input
+      // (dead now); that buffer becomes the step's output
+      Timed t(g, KF_RYU);
+      if (g->cfg.ndim == 2)
+        hipLaunchKernelGGL(k_ryu_ab<2>, g->grid, g->block, 0, g->stream, g->geo, g->bc, wold, wout, win, g->sc);
+      else
+        hipLaunchKernelGGL(k_ryu_ab<3>, g->grid, g->block, 0, g->stream, g->geo, g->bc, wold, wout, win, g->sc);
+      return launch_check(g, "k_ryu_ab");
+    }
     if (lastsweep && src) {
       if (!ryu_folded) {
         Timed t(g, KF_RYU);
