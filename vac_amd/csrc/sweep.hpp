@@ -1361,6 +1361,97 @@ __global__ __launch_bounds__(256) void k_tvd_m(Geo g, Sel s, BC bc, const double
   if (threadIdx.x == 0 && threadIdx.y == 0) part[SLOT_C1 * pstride + block_linear()] = b1;
 }
 
+// k_ryu_ab on 2D grids as a march along y: lanes cover a strip of 62 x
+// cells (lanes 0 and 63 are the neighbours on either side), each marching
+// step evaluates wt once at one new row (at the image of a ghost row or
+// column), keeps B_y of the rows above and below in registers and takes B_x
+// of the x neighbours from the next / previous lane.  Per cell: wold and wnew
+// read once, the result written once.  Same arithmetic as k_ryu_ab.
+constexpr int RY_STRIDE = 62;
+__global__ __launch_bounds__(256) void k_ryu_ab_y(Geo g, BC bc, const double* __restrict__ wold,
+                                                 const double* __restrict__ wnew,
+                                                 double* __restrict__ wout,
+                                                 const Scal* __restrict__ sc, int seglen) {
+  using L = Layout<2, true>;
+  constexpr int NW = L::nw;
+  const int lane = (int)threadIdx.x;
+  const int n0 = g.n[0], n1 = g.n[1];
+  int c = 1 + (int)blockIdx.x * RY_STRIDE + lane;
+  const bool useful = lane >= 1 && lane <= RY_STRIDE && c <= n0 + 1;
+  c = c < n0 + 2 ? c : n0 + 2;  // idle lanes: in-bounds, unused
+  const int px = image_of(g, bc, 0, c);
+  const int s0 = 2 + ((int)blockIdx.y * (int)blockDim.y + (int)threadIdx.y) * seglen;
+  if (s0 > n1 + 1) return;  // wave-uniform
+  const int s1 = (s0 + seglen - 1 < n1 + 1) ? s0 + seglen - 1 : n1 + 1;
+  const double dt = sc->dt;
+  const long long sy = g.s[1];
+  const double dxp = g.dx[0][px];
+  // inputs of wt at (px, image of row r), loaded one marching step ahead
+  struct In {
+    double uo[NW], un[NW], bxm, bxp, bym, byp, dyp;
+  } in;
+  auto load_row = [&](int r) {
+    const int py = image_of(g, bc, 1, r);
+    const long long k = g.off + px + (long long)py * sy;
+    load<NW>(wold, g.ps, k, in.uo);
+    load<NW>(wnew, g.ps, k, in.un);
+    in.bxm = wold[L::b(0) * g.ps + k - 1];
+    in.bxp = wold[L::b(0) * g.ps + k + 1];
+    in.bym = wold[L::b(1) * g.ps + k - sy];
+    in.byp = wold[L::b(1) * g.ps + k + sy];
+    in.dyp = g.dx[1][py];
+  };
+  // wt = (wold + wnew - dt S(wold))/2 and wnew there (divb_at's arithmetic)
+  auto wt_of = [&](double* t, double* un) {
+    double dv = 0.0;
+    const double g0 = pdiv(0.5 * (in.bxp - in.bxm), dxp);
+    dv = dv + g0;
+    const double g1 = pdiv(0.5 * (in.byp - in.bym), in.dyp);
+    dv = dv + g1;
+#pragma unroll
+    for (int iw = 0; iw < NW; ++iw) t[iw] = in.un[iw];
+    powell<2>(in.uo, dv, dt, t);
+#pragma unroll
+    for (int iw = 0; iw < NW; ++iw) {
+      t[iw] = 0.5 * (in.uo[iw] + t[iw]);
+      un[iw] = in.un[iw];
+    }
+  };
+  double tc[NW], uc[NW], tn[NW], un[NW];
+  double bym;  // B_y of wt one row below
+  load_row(s0 - 1);
+  wt_of(tn, un);
+  bym = tn[L::b(1)];
+  load_row(s0);
+  wt_of(tc, uc);
+  load_row(s0 + 1);
+  const double dxc = g.dx[0][c];
+  for (int j = s0; j <= s1; ++j) {
+    wt_of(tn, un);  // row j + 1
+    if (j < s1) load_row(j + 2);
+    const double bxp = lane_next(tc[L::b(0)]), bxm = lane_prev(tc[L::b(0)]);
+    double dv = 0.0;
+    const double g0 = pdiv(0.5 * (bxp - bxm), dxc);
+    dv = dv + g0;
+    const double g1 = pdiv(0.5 * (tn[L::b(1)] - bym), g.dx[1][j]);
+    dv = dv + g1;
+    double o[NW];
+#pragma unroll
+    for (int iw = 0; iw < NW; ++iw) o[iw] = uc[iw];
+    powell<2>(tc, dv, dt, o);
+    if (useful) {
+      const int c3[3] = {c, j, 0};
+      store_cell<2, NW>(g, bc, wout, g.off + c + (long long)j * sy, c3, o, 1);
+    }
+    bym = tc[L::b(1)];
+#pragma unroll
+    for (int iw = 0; iw < NW; ++iw) {
+      tc[iw] = tn[iw];
+      uc[iw] = un[iw];
+    }
+  }
+}
+
 // Ryu source step on the last sweep (src/vactvd.t:60-80):
 // wt = (wold + wnew - dt*S(wold))/2 on the mesh (ghosts refreshed after) ...
 template <int ND>

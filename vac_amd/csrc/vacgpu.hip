@@ -55,6 +55,8 @@ struct vacgpu_grid {
   bool bc_uniform = false;  // one boundary type per face for all variables, n >= 4
   int fs_seglen = 16;
   dim3 grid_fs, block_fs;
+  int ry_seglen = 16;  // k_ryu_ab_y (2D one-pass Ryu step)
+  dim3 grid_ry;
   double* stage = nullptr;  // pinned staging of put_state / get_state
   unsigned* ctr = nullptr;  // block counter of the fused sweeps (zero between launches)
   // device loop: the last fused sweep of a step also advances it and t
@@ -240,53 +242,59 @@ int getdt_device(vacgpu_grid* g, const vacgpu_method* m, double dtpar, int tmaxe
   return launch_check(g, "getdt");
 }
 
-// Blocks of the fused sweeps that the device holds at once: CUs times the
-// resident blocks per CU of the x and y sweep kernels (the fewer), or 0 when
-// the runtime cannot say.
-long long fused_slots(const vacgpu_grid* g) {
-  int dev = 0, cus = 0;
+// Blocks of 256 threads of a kernel that the device holds at once: CUs
+// times the kernel's resident blocks per CU, or 0 when the runtime cannot say.
+long long resident_blocks(const void* kernel) {
+  int dev = 0, cus = 0, per_cu = 0;
   if (hipGetDevice(&dev) != hipSuccess) return 0;
   if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return 0;
-  const void* fx = nullptr;
-  const void* fy = nullptr;
-  const bool mhd = g->cfg.physics == VACGPU_MHD, f32 = g->cfg.precision == VACGPU_FP32BULK;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, 256, 0) != hipSuccess) return 0;
+  return per_cu > 0 && cus > 0 ? (long long)per_cu * cus : 0;
+}
+
+// Segment length for a marching launch of `strips` strips over `rows` rows,
+// four segments (waves) per block, `slots` resident blocks: every wave
+// marches seglen rows after a prologue worth `prologue` rows, the waves run in
+// rounds of `slots` blocks, and a partly filled last round costs a whole
+// wave time; the length with the least estimated time wins.
+int pick_seglen(long long strips, int rows, long long slots, double prologue) {
+  int best_sl = 4;
+  double best = 1e300;
+  for (int sl = 4; sl <= 256; ++sl) {
+    const long long segs = (rows + sl - 1) / sl;
+    const long long blocks = strips * ((segs + 3) / 4);
+    const double cost = (double)((blocks + slots - 1) / slots) * (sl + prologue);
+    if (cost < best) {
+      best = cost;
+      best_sl = sl;
+    }
+  }
+  return best_sl;
+}
+
+// launch shape of the fused TVDLF sweeps: strips of 62 x cells, waves of
+// one segment of seglen rows each, four waves per block.  seglen 0 picks the
+// length by pick_seglen (a prologue of two predictor rows, about 1.5 rows of
+// work) over the fewer resident blocks of the x and y kernels, or about 4096
+// waves when the occupancy is unknown.
+void set_fused_shape(vacgpu_grid* g, int seglen) {
+  const int strips = (g->geo.n[0] + FS_STRIDE - 1) / FS_STRIDE;
+  if (seglen <= 0) {
+    const void* fx = nullptr;
+    const void* fy = nullptr;
+    const bool mhd = g->cfg.physics == VACGPU_MHD, f32 = g->cfg.precision == VACGPU_FP32BULK;
 #define FSLOT(MV, RT)                                                     \
   do {                                                                    \
     fx = reinterpret_cast<const void*>(&k_tvdlf_fused<MV, RT, 0, 0>);     \
     fy = reinterpret_cast<const void*>(&k_tvdlf_fused<MV, RT, 1, 0>);     \
   } while (0)
-  if (mhd) { if (f32) FSLOT(true, float); else FSLOT(true, double); }
-  else { if (f32) FSLOT(false, float); else FSLOT(false, double); }
+    if (mhd) { if (f32) FSLOT(true, float); else FSLOT(true, double); }
+    else { if (f32) FSLOT(false, float); else FSLOT(false, double); }
 #undef FSLOT
-  int a = 0, b = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&a, fx, 256, 0) != hipSuccess) return 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, fy, 256, 0) != hipSuccess) return 0;
-  const int per_cu = a < b ? a : b;
-  return per_cu > 0 && cus > 0 ? (long long)per_cu * cus : 0;
-}
-
-// launch shape of the fused TVDLF sweeps: strips of 62 x cells, waves of
-// one segment of seglen rows each, four waves per block.  seglen 0 picks the
-// length with the least estimated time (below), or about 4096 waves when
-// the occupancy is unknown.
-void set_fused_shape(vacgpu_grid* g, int seglen) {
-  const int strips = (g->geo.n[0] + FS_STRIDE - 1) / FS_STRIDE;
-  if (seglen <= 0) {
-    const long long slots = fused_slots(g);
+    const long long a = resident_blocks(fx), b = resident_blocks(fy);
+    const long long slots = a < b ? a : b;
     if (slots > 0) {
-      // Every wave marches seglen rows after a prologue worth about 1.5 rows
-      // (two predictor rows); the waves of a launch run in rounds of `slots`
-      // blocks, and a partly filled last round costs a whole wave time.
-      double best = 1e300;
-      for (int sl = 4; sl <= 256; ++sl) {
-        const long long segs = (g->geo.n[1] + sl - 1) / sl;
-        const long long blocks = (long long)strips * ((segs + 3) / 4);
-        const double cost = (double)((blocks + slots - 1) / slots) * (sl + 1.5);
-        if (cost < best) {
-          best = cost;
-          seglen = sl;
-        }
-      }
+      seglen = pick_seglen(strips, g->geo.n[1], slots, 1.5);
     } else {
       const int want = (4096 + strips - 1) / strips;
       seglen = (g->geo.n[1] + want - 1) / want;
@@ -297,6 +305,16 @@ void set_fused_shape(vacgpu_grid* g, int seglen) {
   const int segs = (g->geo.n[1] + seglen - 1) / seglen;
   g->block_fs = dim3(64, 4, 1);
   g->grid_fs = dim3(strips, (segs + 3) / 4, 1);
+}
+
+// launch shape of k_ryu_ab_y (2D): strips of 62 x cells, four segments per
+// block; the prologue is two rows of wt, about 1.3 rows of work
+void set_ryu_shape(vacgpu_grid* g) {
+  const int strips = (g->geo.n[0] + RY_STRIDE - 1) / RY_STRIDE;
+  const long long slots = resident_blocks(reinterpret_cast<const void*>(&k_ryu_ab_y));
+  g->ry_seglen = slots > 0 ? pick_seglen(strips, g->geo.n[1], slots, 1.3) : 16;
+  const int segs = (g->geo.n[1] + g->ry_seglen - 1) / g->ry_seglen;
+  g->grid_ry = dim3(strips, (segs + 3) / 4, 1);
 }
 
 // getdt of the steps after the first in a device loop: one block, lagged
@@ -515,10 +533,12 @@ int sweep_device(vacgpu_grid* g, const vacgpu_method* m, int d, int firstsweep, 
       // both Ryu halves in one pass, into the buffer of this sweep's input
       // (dead now); that buffer becomes the step's output
       Timed t(g, KF_RYU);
-      if (g->cfg.ndim == 2)
-        hipLaunchKernelGGL(k_ryu_ab<2>, g->grid, g->block, 0, g->stream, g->geo, g->bc, wold, wout, win, g->sc);
-      else
+      if (g->cfg.ndim == 2) {
+        hipLaunchKernelGGL(k_ryu_ab_y, g->grid_ry, dim3(64, 4), 0, g->stream, g->geo, g->bc, wold,
+                           wout, win, g->sc, g->ry_seglen);
+      } else {
         hipLaunchKernelGGL(k_ryu_ab<3>, g->grid, g->block, 0, g->stream, g->geo, g->bc, wold, wout, win, g->sc);
+      }
       return launch_check(g, "k_ryu_ab");
     }
     if (lastsweep && src) {
@@ -664,6 +684,7 @@ int vacgpu_create(const vacgpu_config* cfg, const double* dx0, const double* dx1
       if (g->bc.t[f][iw] != g->bc.t[f][0]) g->bc_uniform = false;
   g->fused_ok = nd == 2 && g->bc_uniform && (unsigned long long)geo.ps * 8ULL < (1ULL << 32);
   set_fused_shape(g, 0);
+  if (nd == 2) set_ryu_shape(g);
   auto count = [](dim3 v) { return (long long)v.x * v.y * v.z; };
   g->pstride = count(g->grid);
   {
