@@ -649,6 +649,15 @@ __global__ __launch_bounds__(256) void k_tvdlf_corrm(Geo g, Sel s, BC bc, const 
 }
 
 
+// Wait for every vector memory load in flight (vmcnt(0); expcnt and lgkmcnt
+// left at their maxima).  The fused y sweep calls it after a step's
+// arithmetic and before its stores: its prefetch for the next step has had the whole step
+// to arrive, and the next step's first use of the prefetched registers then
+// needs no wait.  Without it that use must wait with a count that cannot
+// tell the step's stores (a data-dependent number) from the loads, i.e. for
+// the stores just issued as well, since memory operations complete in order.
+__device__ __forceinline__ void prefetch_done() { __builtin_amdgcn_s_waitcnt(0x0F70); }
+
 // plane iw at byte offset o (planes are addressed with 32-bit byte offsets)
 __device__ __forceinline__ double ldp(const double* __restrict__ w, long long ps, int iw, unsigned o) {
   return *reinterpret_cast<const double*>(reinterpret_cast<const char*>(w + iw * ps) + o);
@@ -769,14 +778,19 @@ __device__ __forceinline__ double tvdlf_fx_body(const Geo& g, const Sel& s, cons
   pred_gather<MHD, R, 0, LIM>(g, f, s, w, px, s0, dt, smallp, Pc, Lc, Uc);
   const double dxs = dx0[c] + dx0[c + 1];
   const double qdx = pdiv(dt, dx0[c]);
-  // inputs of the next row's predictor are loaded one marching step ahead
+  // inputs of the next row's predictor are loaded one marching step ahead;
+  // dy(j) comes from the previous step's inputs, so no load is consumed in
+  // the step that issues it (a fresh load waited for would also wait for the
+  // prefetch: loads complete in order)
   PredIn<MHD> in;
   pred_load<MHD, 0>(g, f, w, px, ghost_image(s0 + 1, f.n1, f.pylo, f.pyhi), in);
+  double dyj = dx1[s0];
 #pragma unroll 2
   for (int j = s0; j <= s1; ++j) {
     pred_run<MHD, R, 0, LIM>(s, in, dt, smallp, Pn, Ln);
 #pragma unroll
     for (int iw = 0; iw < NW; ++iw) Un[iw] = in.uc[iw];
+    const double dyn = in.dxk[1];  // dy(j + 1)
     if (j < s1) pred_load<MHD, 0>(g, f, w, px, ghost_image(j + 2, f.n1, f.pylo, f.pyhi), in);
     double lw[NW], pr[NW], lr[NW];
 #pragma unroll
@@ -799,7 +813,7 @@ __device__ __forceinline__ double tvdlf_fx_body(const Geo& g, const Sel& s, cons
       double dv = 0.0;
       const double g0 = pdiv(0.5 * (bxp - bxm), dx0[c]);
       dv = dv + g0;
-      const double g1 = pdiv(0.5 * (Pn[L::b(1)] - byp), dx1[j]);
+      const double g1 = pdiv(0.5 * (Pn[L::b(1)] - byp), dyj);
       dv = dv + g1;
       powell<ND>(Pc, dv, dt * 1.0 / ND, out);
       byp = Pc[L::b(1)];
@@ -811,6 +825,7 @@ __device__ __forceinline__ double tvdlf_fx_body(const Geo& g, const Sel& s, cons
       Lc[iw] = Ln[iw];
       Uc[iw] = Un[iw];
     }
+    dyj = dyn;
   }
   return cmx;
 }
@@ -848,17 +863,17 @@ __device__ __forceinline__ double tvdlf_fy_body(const Geo& g, const Sel& s, cons
     }
   };
   // predictor at mesh row r from raw rows r-1, r, r+1 and B_x in registers
-  auto pred_rowx = [&](int r, const double* wm, const double* w0, const double* wp,
+  auto pred_rowx = [&](double dyr, const double* wm, const double* w0, const double* wp,
                        const double* bx, double* P, double* l) {
     const double bn[2][2] = {{bx[0], bx[1]}, {0.0, 0.0}};
-    const double dxk[2] = {dxp, dx1[r]};
+    const double dxk[2] = {dxp, dyr};
     pred_compute<2, MHD, R, 1, LIM, MHD ? 1 : 0>(s, w0, wm, wp, bn, dxk, dt, smallp, P, l);
   };
   auto pred_row = [&](int r, const double* wm, const double* w0, const double* wp, double* P,
                       double* l) {
     double bx[2];
     load_bx(r, bx);
-    pred_rowx(r, wm, w0, wp, bx, P, l);
+    pred_rowx(dx1[r], wm, w0, wp, bx, P, l);
   };
   // predictor at a ghost row: at its image, with that image's slopes ('periodic')
   // or zero slopes ('cont')
@@ -892,9 +907,11 @@ __device__ __forceinline__ double tvdlf_fy_body(const Geo& g, const Sel& s, cons
   corr_face<ND, MHD, R, 1>(s, Pn, Pc, ln, lc, dx1[s0 - 1] + dx1[s0], smallp, Fp, cv);
   if (useful) cmx = cmx < cv ? cv : cmx;
   if constexpr (MHD) byp = Pn[L::b(1)];
-  // window: W0 = row j, W1 = row j+1, W2 = row j+2 and bx = B_x around
-  // row j+1, loaded one marching step ahead
+  // window: W0 = row j, W1 = row j+1, W2 = row j+2, bx = B_x around row j+1
+  // and the widths dy0 = dy(j), dy1 = dy(j+1), all loaded one marching step
+  // ahead
   double bx[2] = {0.0, 0.0}, W3[NW] = {}, bxn[2] = {0.0, 0.0};
+  double dy0 = dx1[s0], dy1 = dx1[s0 + 1], dy2 = 0.0;
 #pragma unroll
   for (int iw = 0; iw < NW; ++iw) W0[iw] = W1[iw];
 #pragma unroll
@@ -910,14 +927,15 @@ __device__ __forceinline__ double tvdlf_fy_body(const Geo& g, const Sel& s, cons
       load_row(j + 3, W3);
       load_bx(j + 2, bxn);
     }
+    dy2 = dx1[j + 2];
     if (j + 1 <= f.n1 + 1) {
-      pred_rowx(j + 1, W0, W1, W2, bx, Pn, ln);
+      pred_rowx(dy1, W0, W1, W2, bx, Pn, ln);
     } else {
       pred_ghost(j + 1, Pn, ln);
     }
-    corr_face<ND, MHD, R, 1>(s, Pc, Pn, lc, ln, dx1[j] + dx1[j + 1], smallp, Fn, cv);
+    corr_face<ND, MHD, R, 1>(s, Pc, Pn, lc, ln, dy0 + dy1, smallp, Fn, cv);
     if (useful) cmx = cmx < cv ? cv : cmx;
-    const double qdx = pdiv(dt, dx1[j]);
+    const double qdx = pdiv(dt, dy0);
     double out[NW];
 #pragma unroll
     for (int iw = 0; iw < NW; ++iw) out[iw] = W0[iw] - qdx * (Fn[iw] - Fp[iw]);
@@ -926,11 +944,12 @@ __device__ __forceinline__ double tvdlf_fy_body(const Geo& g, const Sel& s, cons
       double dv = 0.0;
       const double g0 = pdiv(0.5 * (bxp - bxm), dx0[c]);
       dv = dv + g0;
-      const double g1 = pdiv(0.5 * (Pn[L::b(1)] - byp), dx1[j]);
+      const double g1 = pdiv(0.5 * (Pn[L::b(1)] - byp), dy0);
       dv = dv + g1;
       powell<ND>(Pc, dv, dt * 1.0 / ND, out);
       byp = Pc[L::b(1)];
     }
+    prefetch_done();
     if (useful) store_with_ghosts<NW>(g, f, wout, c, j, out);
 #pragma unroll
     for (int iw = 0; iw < NW; ++iw) {
@@ -943,6 +962,8 @@ __device__ __forceinline__ double tvdlf_fy_body(const Geo& g, const Sel& s, cons
     }
     bx[0] = bxn[0];
     bx[1] = bxn[1];
+    dy0 = dy1;
+    dy1 = dy2;
   }
   return cmx;
 }
