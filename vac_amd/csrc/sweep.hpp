@@ -1158,6 +1158,25 @@ __device__ __forceinline__ void tvd_dissipate(const RoeRt<ND, R>& rm, const R* p
   }
 }
 
+// tvd_dissipate for the x sweep, where lane L owns face J and the face J-1
+// term of cell J is lane L-1's face term: a0(il, iw) of this lane is a1(il, iw)
+// of the previous lane (the same R.phi product, scaled by the same sum
+// dvol(J) + dvol(J-1)), so it is taken from there instead of recomputed.
+template <int ND, class R>
+__device__ __forceinline__ void tvd_dissipate_x(const RoeRt<ND, R>& r0, const R* phi0, const Sel& s,
+                                                R dv0, double dvi, double* out) {
+  constexpr int NW = Layout<ND, true>::nw;
+#pragma unroll
+  for (int il = 0; il < NW; ++il) {
+#pragma unroll
+    for (int iw = 0; iw < NW; ++iw) {
+      const R a1 = roe_rtimes<ND, R>(r0, phi0[il], iw, il, 0, s.divbwave) * R(0.25) * dv0;
+      const R a0 = lane_prev(a1);
+      out[iw] = out[iw] + pdiv(double(a1) - double(a0), dvi);
+    }
+  }
+}
+
 template <class T>
 __device__ __forceinline__ T shfl_up1(T v) { return lane_prev(v); }
 template <class T>
@@ -1227,15 +1246,16 @@ __global__ __launch_bounds__(256) void k_tvd_x(Geo g, Sel s, BC bc, const double
     const R jm = shfl_up1(f.jump[il]), jp = shfl_down1(f.jump[il]);
     phi[il] = tvd_phi_t<LIM, R>(s.lim[il], jm, f.jump[il], jp, f.adt[il]);
   }
-  R phim[NW];
-#pragma unroll
-  for (int il = 0; il < NW; ++il) phim[il] = shfl_up1(phi[il]);
-  const RoeRt<ND, R> rtm = shfl_up_struct<RoeRt<ND, R>, R>(f.rt);
   const int i = J;
-  if (rowok && lane >= 2 && lane <= 62 && i <= g.n[0] + 1) {
-    const long long idx = base + i;
-    double out[NW];
-    const double qdx = pdiv(dt, g.dx[0][i]);
+  // the dissipation needs every lane's face term (lane_prev inside), so it is
+  // evaluated by all lanes and only the owners of a cell store it
+  double out[NW];
+  {
+    // lanes 1..61 feed the next lane exactly (cells 1 .. n+1 as they are);
+    // the rest are clamped into ixG and their values go nowhere
+    const int ic = i < g.n[0] + 2 ? (i > 1 ? i : 1) : g.n[0] + 2;
+    const long long idx = base + ic;
+    const double qdx = pdiv(dt, g.dx[0][ic]);
 #pragma unroll
     for (int iw = 0; iw < NW; ++iw) out[iw] = win[iw * g.ps + idx] - qdx * (fC[iw] - fCm[iw]);
     // dvolume = (dx1*dx2)*dx3 evaluated left to right
@@ -1245,8 +1265,11 @@ __global__ __launch_bounds__(256) void k_tvd_x(Geo g, Sel s, BC bc, const double
       if (ND == 3) v = v * g.dx[2][z];
       return v;
     };
-    const double dm = dvol(i - 1), d0 = dvol(i), dp = dvol(i + 1);
-    tvd_dissipate<ND, R>(rtm, phim, f.rt, phi, 0, s, R(d0 + dm), R(dp + d0), d0, out);
+    const double d0 = dvol(ic), dp = dvol(ic + 1);
+    tvd_dissipate_x<ND, R>(f.rt, phi, s, R(dp + d0), d0, out);
+  }
+  if (rowok && lane >= 2 && lane <= 62 && i <= g.n[0] + 1) {
+    const long long idx = base + i;
     const int c[3] = {i, yc, z};
     // (the last sweep's output is final only after the Ryu step, which fills its ghosts)
     store_cell<ND, NW>(g, bc, wout, idx, c, out, SRC == 2 ? 0 : ghosts);
