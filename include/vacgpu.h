@@ -135,6 +135,21 @@ int vacgpu_set_kernels(vacgpu_grid* g, int fused, int seglen);
 int vacgpu_check_quotients(const double* a, const double* b, long long n, double* q_kernel,
                            double* q_ieee);
 
+/* Asynchronous snapshots (no counterpart in VAC, whose savefile,
+ * src/vacio.t:1008-1047, writes the host copy of w and so needs the state
+ * downloaded first).  vacgpu_snapshot_start copies the current state (ixG,
+ * every variable) and its it / t into a device buffer of slot `slot` in
+ * stream order, then to pinned host memory on a second stream, and returns
+ * at once: later steps may be enqueued and run while the copy to the host is
+ * in flight.  vacgpu_snapshot_fetch waits for that copy and writes the state
+ * into w in VAC's layout (as vacgpu_get_state does, leading dimensions ld or
+ * ixG) with the it and t it was taken at; a host then writes the .out record
+ * (vac_amd/vacfile.py) while the device keeps stepping.  Starting a slot
+ * again first waits for its previous copy. */
+#define VACGPU_SNAP_SLOTS 2
+int vacgpu_snapshot_start(vacgpu_grid* g, int slot);
+int vacgpu_snapshot_fetch(vacgpu_grid* g, int slot, double* w, const int* ld, int* it, double* t);
+
 /* Start-up check of a selector set: 0 if supported, else VACGPU_E_ARG. */
 int vacgpu_supported(const vacgpu_config* cfg, const vacgpu_method* m);
 

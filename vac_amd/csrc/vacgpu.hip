@@ -67,6 +67,17 @@ struct vacgpu_grid {
   bool have_state = false;
   hipStream_t stream = nullptr;
   bool own_stream = false;
+  // asynchronous snapshots: a device copy taken in stream order, then copied
+  // to pinned host memory on copy_stream while the steps go on
+  struct Snap {
+    double* dev = nullptr;
+    double* host = nullptr;
+    Scal* dsc = nullptr;
+    Scal* hsc = nullptr;
+    hipEvent_t copied = nullptr, done = nullptr;
+    bool pending = false, valid = false;
+  } snap[VACGPU_SNAP_SLOTS];
+  hipStream_t copy_stream = nullptr;
   std::string err;
   // profiling
   bool prof = false;
@@ -1009,6 +1020,59 @@ int vacgpu_check_quotients(const double* a, const double* b, long long n, double
   return e == hipSuccess ? 0 : VACGPU_E_HIP;
 }
 
+int vacgpu_snapshot_start(vacgpu_grid* g, int slot) {
+  if (!g || slot < 0 || slot >= VACGPU_SNAP_SLOTS) return VACGPU_E_ARG;
+  if (!g->have_state) return fail(g, VACGPU_E_STATE, "no state on the device (call vacgpu_put_state)");
+  auto& S = g->snap[slot];
+  const size_t bytes = sizeof(double) * (size_t)g->geo.ps * g->nw;
+  if (!g->copy_stream) HIPCHK(g, hipStreamCreateWithFlags(&g->copy_stream, hipStreamNonBlocking));
+  if (S.pending) HIPCHK(g, hipEventSynchronize(S.done));  // the slot's last copy is out
+  if (!S.dev) {
+    HIPCHK(g, hipMalloc(&S.dev, bytes));
+    HIPCHK(g, hipHostMalloc(&S.host, bytes, hipHostMallocDefault));
+    HIPCHK(g, hipMalloc(&S.dsc, sizeof(Scal)));
+    HIPCHK(g, hipHostMalloc(&S.hsc, sizeof(Scal), hipHostMallocDefault));
+    HIPCHK(g, hipEventCreateWithFlags(&S.copied, hipEventDisableTiming));
+    HIPCHK(g, hipEventCreateWithFlags(&S.done, hipEventDisableTiming));
+  }
+  HIPCHK(g, hipMemcpyAsync(S.dev, g->buf[g->cur], bytes, hipMemcpyDeviceToDevice, g->stream));
+  HIPCHK(g, hipMemcpyAsync(S.dsc, g->sc, sizeof(Scal), hipMemcpyDeviceToDevice, g->stream));
+  HIPCHK(g, hipEventRecord(S.copied, g->stream));
+  HIPCHK(g, hipStreamWaitEvent(g->copy_stream, S.copied, 0));
+  HIPCHK(g, hipMemcpyAsync(S.host, S.dev, bytes, hipMemcpyDeviceToHost, g->copy_stream));
+  HIPCHK(g, hipMemcpyAsync(S.hsc, S.dsc, sizeof(Scal), hipMemcpyDeviceToHost, g->copy_stream));
+  HIPCHK(g, hipEventRecord(S.done, g->copy_stream));
+  S.pending = S.valid = true;
+  return 0;
+}
+
+int vacgpu_snapshot_fetch(vacgpu_grid* g, int slot, double* w, const int* ld, int* it, double* t) {
+  if (!g || !w || slot < 0 || slot >= VACGPU_SNAP_SLOTS) return VACGPU_E_ARG;
+  auto& S = g->snap[slot];
+  if (!S.valid) return fail(g, VACGPU_E_STATE, "no snapshot in this slot (call vacgpu_snapshot_start)");
+  const Geo& geo = g->geo;
+  const int nd = g->cfg.ndim;
+  int l[3] = {geo.ng[0], geo.ng[1], nd == 3 ? geo.ng[2] : 1};
+  if (ld)
+    for (int d = 0; d < nd; ++d) {
+      if (ld[d] < geo.ng[d]) return fail(g, VACGPU_E_ARG, "leading dimension smaller than ixG");
+      l[d] = ld[d];
+    }
+  HIPCHK(g, hipEventSynchronize(S.done));
+  S.pending = false;
+  const int nz = nd == 3 ? geo.ng[2] : 1;
+  const size_t plane = (size_t)l[0] * l[1];
+  for (int iw = 0; iw < g->nw; ++iw)
+    for (int z = 0; z < nz; ++z)
+      for (int y = 0; y < geo.ng[1]; ++y)
+        memcpy(w + (size_t)iw * plane * (nd == 3 ? l[2] : 1) + (size_t)z * plane + (size_t)y * l[0],
+               S.host + (size_t)iw * geo.ps + geo.off + (size_t)z * geo.s[2] + (size_t)y * geo.s[1],
+               sizeof(double) * geo.ng[0]);
+  if (it) *it = S.hsc->it;
+  if (t) *t = S.hsc->t;
+  return 0;
+}
+
 const char* vacgpu_last_error(const vacgpu_grid* g) {
   if (!g) return "null handle";
   return g->err.c_str();
@@ -1027,6 +1091,16 @@ void vacgpu_destroy(vacgpu_grid* g) {
   if (g->stage) (void)hipHostFree(g->stage);
   if (g->part) (void)hipFree(g->part);
   if (g->ctr) (void)hipFree(g->ctr);
+  for (auto& S : g->snap) {
+    if (S.pending) (void)hipEventSynchronize(S.done);
+    if (S.dev) (void)hipFree(S.dev);
+    if (S.host) (void)hipHostFree(S.host);
+    if (S.dsc) (void)hipFree(S.dsc);
+    if (S.hsc) (void)hipHostFree(S.hsc);
+    if (S.copied) (void)hipEventDestroy(S.copied);
+    if (S.done) (void)hipEventDestroy(S.done);
+  }
+  if (g->copy_stream) (void)hipStreamDestroy(g->copy_stream);
   if (g->own_stream && g->stream) (void)hipStreamDestroy(g->stream);
   delete g;
 }

@@ -62,6 +62,8 @@ SIGNATURES = {
     "vacgpu_kernel_time": ([C.c_void_p, C.c_int, _DP, C.POINTER(C.c_long)], C.c_int),
     "vacgpu_set_kernels": ([C.c_void_p, C.c_int, C.c_int], C.c_int),
     "vacgpu_check_quotients": ([_DP, _DP, C.c_longlong, _DP, _DP], C.c_int),
+    "vacgpu_snapshot_start": ([C.c_void_p, C.c_int], C.c_int),
+    "vacgpu_snapshot_fetch": ([C.c_void_p, C.c_int, _DP, _IP, _IP, _DP], C.c_int),
     "vacgpu_supported": ([C.POINTER(Config), C.POINTER(Method)], C.c_int),
     "vacgpu_last_error": ([C.c_void_p], C.c_char_p),
     "vacgpu_destroy": ([C.c_void_p], None),
@@ -225,6 +227,23 @@ class Grid:
     def get_mesh(self) -> np.ndarray:
         w = self.get_state()
         return np.ascontiguousarray(w[(slice(None),) + (slice(2, -2),) * self.ndim])
+
+    def snapshot_start(self, slot: int = 0):
+        """Start an asynchronous snapshot of the current state into `slot`
+        (0 or 1); returns at once, later steps may follow."""
+        self._check(self.lib.vacgpu_snapshot_start(self.h, slot), "snapshot_start")
+
+    def snapshot_fetch(self, slot: int = 0, mesh: bool = True):
+        """Wait for the snapshot in `slot`; returns (w, it, t) with w the mesh
+        (or, mesh=False, the whole ixG array) as get_mesh / get_state give it."""
+        w = np.empty(self.shape_full)
+        it = C.c_int()
+        t = C.c_double()
+        self._check(self.lib.vacgpu_snapshot_fetch(self.h, slot, _ptr(w), None, C.byref(it),
+                                                   C.byref(t)), "snapshot_fetch")
+        if mesh:
+            w = np.ascontiguousarray(w[(slice(None),) + (slice(2, -2),) * self.ndim])
+        return w, it.value, t.value
 
     def set_time(self, it: int, t: float):
         self._check(self.lib.vacgpu_set_time(self.h, it, t), "set_time")
