@@ -10,7 +10,7 @@ sweep per direction, each with the Roe dissipation and the Powell / Ryu
 source terms).  Metric: Mcell-updates/s = nx*ny*steps / seconds / 1e6.
 
     python bench.py [--gpus 1] [--steps 200] [--warmup 5] [--no-cpu] [--precision fp64]
-                    [--no-batched] [--no-serving]
+                    [--no-tvdlf] [--no-3d] [--no-snapshots] [--no-batched] [--no-serving]
 
 Prints one JSON line.  Beside the headline it reports the batched workload
 (512 independent 2D MHD grids, sides log-uniform in [64, 512], TVDLF +
@@ -110,6 +110,7 @@ def main():
     ap.add_argument("--no-3d", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-serving", action="store_true")
+    ap.add_argument("--no-snapshots", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
@@ -188,6 +189,8 @@ def main():
         out["tvdlf_2048"] = tvdlf_2048(w0, dxs, min(args.steps, 100))
     if not args.no_3d:
         out["mhd3d_256"] = mhd3d_256()
+    if not args.no_snapshots:
+        out["snapshots"] = snapshot_overhead(w0, coords, dxs, method)
     if not args.no_batched:
         out["batched"] = batched()
     if not args.no_serving:
@@ -318,6 +321,55 @@ def mhd3d_256(steps=100):
     res["workload"] = (f"3D MHD 256^3 periodic random-Fourier (B = curl A), TVDLF+minmod, "
                        f"courantpar 0.6, {steps} steps; B_alg 960 B/cell-update")
     return res
+
+
+def snapshot_overhead(w0, coords, dxs, method, steps=400, every=100):
+    """The headline workload with a VAC snapshot every `every` steps: written
+    while the device steps (vac_amd.snapshots, asynchronous device snapshots)
+    against the same snapshots taken with a synchronous get_state; the file
+    goes to /dev/null, so the figures are the step loop's cost of saving."""
+    import os as _os
+
+    from vac_amd import snapshots, vacgpu
+    from vac_amd.vacfile import write_snapshot
+
+    def fresh():
+        g = vacgpu.Grid([NX, NY], "mhd", GAMMA, dxs)
+        g.put_mesh(w0)
+        return g
+
+    g = fresh()
+    g.run(method, 5)  # warm-up: first launches, snapshot buffers
+    g.snapshot_start(0)
+    g.snapshot_fetch(0)
+    g.close()
+    g = fresh()
+    t0 = time.perf_counter()
+    g.run(method, steps)
+    plain = time.perf_counter() - t0
+    g.close()
+    g = fresh()
+    t0 = time.perf_counter()
+    snapshots.run_with_snapshots(g, method, steps, every, _os.devnull, coords, "mhd", GAMMA,
+                                 save_initial=False)
+    asyn = time.perf_counter() - t0
+    g.close()
+    g = fresh()
+    t0 = time.perf_counter()
+    for _ in range(steps // every):
+        g.run(method, every)
+        w = g.get_mesh()
+        it, t = g.get_time()
+        write_snapshot(_os.devnull, snapshots.snapshot_record(w, it, t, coords, "mhd", GAMMA))
+    sync = time.perf_counter() - t0
+    g.close()
+    return {"steps": steps, "every": every, "snapshots": steps // every,
+            "ms_per_step_no_snapshots": round(plain / steps * 1e3, 4),
+            "ms_per_step_async_snapshots": round(asyn / steps * 1e3, 4),
+            "ms_per_step_sync_get_state": round(sync / steps * 1e3, 4),
+            "async_overhead": round(asyn / plain - 1.0, 4),
+            "sync_overhead": round(sync / plain - 1.0, 4),
+            "workload": f"2048^2 TVD headline, a VAC snapshot record every {every} steps (to /dev/null)"}
 
 
 def batched():

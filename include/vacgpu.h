@@ -149,6 +149,10 @@ int vacgpu_check_quotients(const double* a, const double* b, long long n, double
 #define VACGPU_SNAP_SLOTS 2
 int vacgpu_snapshot_start(vacgpu_grid* g, int slot);
 int vacgpu_snapshot_fetch(vacgpu_grid* g, int slot, double* w, const int* ld, int* it, double* t);
+/* The same, mesh cells only (ixM, no ghost cells), packed: w holds nw planes
+ * of nx[0] * nx[1] (* nx[2]) doubles, x fastest -- the layout of the w
+ * records of VAC's files. */
+int vacgpu_snapshot_fetch_mesh(vacgpu_grid* g, int slot, double* w, int* it, double* t);
 
 /* Start-up check of a selector set: 0 if supported, else VACGPU_E_ARG. */
 int vacgpu_supported(const vacgpu_config* cfg, const vacgpu_method* m);

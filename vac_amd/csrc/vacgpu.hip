@@ -1046,10 +1046,36 @@ int vacgpu_snapshot_start(vacgpu_grid* g, int slot) {
   return 0;
 }
 
-int vacgpu_snapshot_fetch(vacgpu_grid* g, int slot, double* w, const int* ld, int* it, double* t) {
+// wait for snapshot `slot` and copy its ixG rows [lo, lo + n) of every
+// variable into w with row pitch l[0] and plane pitch l[0] * l[1]
+static int snapshot_rows(vacgpu_grid* g, int slot, double* w, const int* l, int lo, int* it,
+                         double* t) {
   if (!g || !w || slot < 0 || slot >= VACGPU_SNAP_SLOTS) return VACGPU_E_ARG;
   auto& S = g->snap[slot];
   if (!S.valid) return fail(g, VACGPU_E_STATE, "no snapshot in this slot (call vacgpu_snapshot_start)");
+  const Geo& geo = g->geo;
+  const int nd = g->cfg.ndim;
+  HIPCHK(g, hipEventSynchronize(S.done));
+  S.pending = false;
+  const int n0 = geo.ng[0] - 2 * lo, n1 = geo.ng[1] - 2 * lo;
+  const int nz = nd == 3 ? geo.ng[2] - 2 * lo : 1;
+  const int z0 = nd == 3 ? lo : 0;
+  const size_t plane = (size_t)l[0] * l[1];
+  const size_t var = plane * (nd == 3 ? l[2] : 1);
+  for (int iw = 0; iw < g->nw; ++iw)
+    for (int z = 0; z < nz; ++z)
+      for (int y = 0; y < n1; ++y)
+        memcpy(w + (size_t)iw * var + (size_t)z * plane + (size_t)y * l[0],
+               S.host + (size_t)iw * geo.ps + geo.off + (size_t)(z + z0) * geo.s[2] +
+                   (size_t)(y + lo) * geo.s[1] + lo,
+               sizeof(double) * n0);
+  if (it) *it = S.hsc->it;
+  if (t) *t = S.hsc->t;
+  return 0;
+}
+
+int vacgpu_snapshot_fetch(vacgpu_grid* g, int slot, double* w, const int* ld, int* it, double* t) {
+  if (!g || !w) return VACGPU_E_ARG;
   const Geo& geo = g->geo;
   const int nd = g->cfg.ndim;
   int l[3] = {geo.ng[0], geo.ng[1], nd == 3 ? geo.ng[2] : 1};
@@ -1058,19 +1084,13 @@ int vacgpu_snapshot_fetch(vacgpu_grid* g, int slot, double* w, const int* ld, in
       if (ld[d] < geo.ng[d]) return fail(g, VACGPU_E_ARG, "leading dimension smaller than ixG");
       l[d] = ld[d];
     }
-  HIPCHK(g, hipEventSynchronize(S.done));
-  S.pending = false;
-  const int nz = nd == 3 ? geo.ng[2] : 1;
-  const size_t plane = (size_t)l[0] * l[1];
-  for (int iw = 0; iw < g->nw; ++iw)
-    for (int z = 0; z < nz; ++z)
-      for (int y = 0; y < geo.ng[1]; ++y)
-        memcpy(w + (size_t)iw * plane * (nd == 3 ? l[2] : 1) + (size_t)z * plane + (size_t)y * l[0],
-               S.host + (size_t)iw * geo.ps + geo.off + (size_t)z * geo.s[2] + (size_t)y * geo.s[1],
-               sizeof(double) * geo.ng[0]);
-  if (it) *it = S.hsc->it;
-  if (t) *t = S.hsc->t;
-  return 0;
+  return snapshot_rows(g, slot, w, l, 0, it, t);
+}
+
+int vacgpu_snapshot_fetch_mesh(vacgpu_grid* g, int slot, double* w, int* it, double* t) {
+  if (!g || !w) return VACGPU_E_ARG;
+  const int l[3] = {g->geo.n[0], g->geo.n[1], g->cfg.ndim == 3 ? g->geo.n[2] : 1};
+  return snapshot_rows(g, slot, w, l, 2, it, t);
 }
 
 const char* vacgpu_last_error(const vacgpu_grid* g) {

@@ -34,13 +34,14 @@ def varnames(physics: str, ndim: int) -> str:
 
 
 def snapshot_record(w_mesh: np.ndarray, it: int, t: float, coords: list, physics: str,
-                    gamma: float, header: str = "vacgpu") -> Snapshot:
-    """One snapshot record of a mesh state (w[iw] with x fastest)."""
+                    gamma: float, header: str = "vacgpu", x: np.ndarray | None = None) -> Snapshot:
+    """One snapshot record of a mesh state (w[iw] with x fastest); x, the
+    full coordinate array, may be passed in to be built only once."""
     ndim = len(coords)
     eqpar = np.array([gamma, 0.0]) if physics == "mhd" else np.array([gamma])
     return Snapshot(f"{header}_{physics}{ndim}{ndim}", int(it), float(t), ndim, w_mesh.shape[0],
                     tuple(len(c) for c in coords), eqpar, varnames(physics, ndim),
-                    coords_array(coords), np.ascontiguousarray(w_mesh))
+                    coords_array(coords) if x is None else x, np.ascontiguousarray(w_mesh))
 
 
 def run_with_snapshots(grid, method, nsteps: int, every: int, path: str, coords: list,
@@ -59,10 +60,11 @@ def run_with_snapshots(grid, method, nsteps: int, every: int, path: str, coords:
 
     written = []
     jobs = [None, None]
+    x = coords_array(coords)
 
     def write(slot, first):
         w, it, t = grid.snapshot_fetch(slot)
-        write_snapshot(path, snapshot_record(w, it, t, coords, physics, gamma, header),
+        write_snapshot(path, snapshot_record(w, it, t, coords, physics, gamma, header, x),
                        append=not first)
         written.append((it, t))
 

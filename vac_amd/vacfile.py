@@ -59,8 +59,8 @@ def _fstr(s: str) -> bytes:
     return b.ljust(LENNAME, b" ")
 
 
-def snapshot_bytes(snap: Snapshot) -> bytes:
-    """Serialise one snapshot in VAC's binary layout."""
+def _records(snap: Snapshot):
+    """The records of one snapshot as buffers, the large ones without copies."""
     ndim = snap.ndim
     nx = tuple(int(n) for n in snap.nx)
     if len(nx) != ndim:
@@ -73,24 +73,37 @@ def snapshot_bytes(snap: Snapshot) -> bytes:
     if w.shape != (snap.nw,) + shape:
         raise ValueError(f"w shape {w.shape} != {(snap.nw,) + shape}")
     ndimout = -ndim if snap.gencoord else ndim
-    out = [
-        _rec(_fstr(snap.header)),
-        _rec(struct.pack("<idiii", int(snap.it), float(snap.t), ndimout,
-                         len(snap.eqpar), int(snap.nw))),
-        _rec(struct.pack("<" + "i" * ndim, *nx)),
-        _rec(np.asarray(snap.eqpar, dtype="<f8").tobytes()),
-        _rec(_fstr(snap.varnames)),
-        # x(ix^S, idim) for idim=1..ndim, column-major == planes of x fastest
-        _rec(np.ascontiguousarray(np.moveaxis(x, -1, 0)).tobytes()),
+    small = [
+        _fstr(snap.header),
+        struct.pack("<idiii", int(snap.it), float(snap.t), ndimout, len(snap.eqpar),
+                    int(snap.nw)),
+        struct.pack("<" + "i" * ndim, *nx),
+        np.asarray(snap.eqpar, dtype="<f8").tobytes(),
+        _fstr(snap.varnames),
     ]
-    for iw in range(snap.nw):
-        out.append(_rec(w[iw].tobytes()))
-    return b"".join(out)
+    for b in small:
+        yield _rec(b)
+    # x(ix^S, idim) for idim=1..ndim, column-major == planes of x fastest,
+    # then w(ix^S, iw) one record per variable
+    big = [np.ascontiguousarray(np.moveaxis(x, -1, 0))] + [w[iw] for iw in range(snap.nw)]
+    for a in big:
+        n = struct.pack("<i", a.nbytes)
+        yield n
+        yield memoryview(a).cast("B")
+        yield n
+
+
+def snapshot_bytes(snap: Snapshot) -> bytes:
+    """Serialise one snapshot in VAC's binary layout."""
+    return b"".join(bytes(b) for b in _records(snap))
 
 
 def write_snapshot(path: str, snap: Snapshot, append: bool = False) -> None:
+    """Write (or append) one snapshot; the arrays go to the file unbuffered
+    and uncopied."""
     with open(path, "ab" if append else "wb") as f:
-        f.write(snapshot_bytes(snap))
+        for b in _records(snap):
+            f.write(b)
 
 
 class _Reader:

@@ -64,6 +64,7 @@ SIGNATURES = {
     "vacgpu_check_quotients": ([_DP, _DP, C.c_longlong, _DP, _DP], C.c_int),
     "vacgpu_snapshot_start": ([C.c_void_p, C.c_int], C.c_int),
     "vacgpu_snapshot_fetch": ([C.c_void_p, C.c_int, _DP, _IP, _IP, _DP], C.c_int),
+    "vacgpu_snapshot_fetch_mesh": ([C.c_void_p, C.c_int, _DP, _IP, _DP], C.c_int),
     "vacgpu_supported": ([C.POINTER(Config), C.POINTER(Method)], C.c_int),
     "vacgpu_last_error": ([C.c_void_p], C.c_char_p),
     "vacgpu_destroy": ([C.c_void_p], None),
@@ -236,13 +237,16 @@ class Grid:
     def snapshot_fetch(self, slot: int = 0, mesh: bool = True):
         """Wait for the snapshot in `slot`; returns (w, it, t) with w the mesh
         (or, mesh=False, the whole ixG array) as get_mesh / get_state give it."""
-        w = np.empty(self.shape_full)
         it = C.c_int()
         t = C.c_double()
-        self._check(self.lib.vacgpu_snapshot_fetch(self.h, slot, _ptr(w), None, C.byref(it),
-                                                   C.byref(t)), "snapshot_fetch")
         if mesh:
-            w = np.ascontiguousarray(w[(slice(None),) + (slice(2, -2),) * self.ndim])
+            w = np.empty((self.nw,) + tuple(reversed(self.nx)))
+            self._check(self.lib.vacgpu_snapshot_fetch_mesh(self.h, slot, _ptr(w), C.byref(it),
+                                                            C.byref(t)), "snapshot_fetch")
+        else:
+            w = np.empty(self.shape_full)
+            self._check(self.lib.vacgpu_snapshot_fetch(self.h, slot, _ptr(w), None, C.byref(it),
+                                                       C.byref(t)), "snapshot_fetch")
         return w, it.value, t.value
 
     def set_time(self, it: int, t: float):
