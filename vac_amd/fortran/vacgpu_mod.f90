@@ -137,6 +137,33 @@ module vacgpu_mod
        integer(c_int), value :: fused, seglen
      end function vacgpu_set_kernels
 
+     integer(c_int) function vacgpu_snapshot_start(grid, slot) &
+          bind(C, name='vacgpu_snapshot_start')
+       import :: c_int, c_ptr
+       type(c_ptr), value :: grid
+       integer(c_int), value :: slot
+     end function vacgpu_snapshot_start
+
+     integer(c_int) function vacgpu_snapshot_fetch(grid, slot, w, ld, it, t) &
+          bind(C, name='vacgpu_snapshot_fetch')
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: grid, ld
+       integer(c_int), value :: slot
+       real(c_double), intent(out) :: w(*)
+       integer(c_int), intent(out) :: it
+       real(c_double), intent(out) :: t
+     end function vacgpu_snapshot_fetch
+
+     integer(c_int) function vacgpu_snapshot_fetch_mesh(grid, slot, w, it, t) &
+          bind(C, name='vacgpu_snapshot_fetch_mesh')
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: grid
+       integer(c_int), value :: slot
+       real(c_double), intent(out) :: w(*)
+       integer(c_int), intent(out) :: it
+       real(c_double), intent(out) :: t
+     end function vacgpu_snapshot_fetch_mesh
+
      integer(c_int) function vacgpu_check_quotients(a, b, n, q_kernel, q_ieee) &
           bind(C, name='vacgpu_check_quotients')
        import :: c_int, c_double, c_long_long
