@@ -361,9 +361,16 @@ static void hancock(ctx_t* x, double qdt, int d, const double* wCT, double* wnew
   if (x->p->sourceunsplit) addsource(x, qdt * 1.0 / x->ndim, 1, wCT, wnew);
 }
 
-/* tvdmusclf for method='tvdlf' (src/vactvdlf.t:92-299); wnew is the base
- * of the limiter (typelimited='original') and receives the update. */
-static void tvdlf_corrector(ctx_t* x, double qdt, int d, const double* wCT, double* wnew) {
+static void tvdlimit2(ctx_t* x, double qdt, int d, const double* wL, const double* wR,
+                      double* wnew, int mu);
+
+/* tvdmusclf for method='tvdlf' (mu = 0) or 'tvdmu' (mu = 1)
+ * (src/vactvdlf.t:92-299); wnew is the base of the limiter
+ * (typelimited='original') and receives the update.  'tvdmu' has no Lax-
+ * Friedrichs dissipation (and no cmax Courant check): the central flux is
+ * added, then tvdlimit2 adds the Roe dissipation of the reconstructed face
+ * states (src/vactvdlf.t:279-283). */
+static void tvdlf_corrector(ctx_t* x, double qdt, int d, const double* wCT, double* wnew, int mu) {
   int lo[3], hi[3];
   for (int dd = 0; dd < 3; ++dd) { lo[dd] = x->mlo[dd]; hi[dd] = x->mhi[dd]; }
   lo[d] -= 1;  /* ixC = [ixOmin-1, ixOmax] */
@@ -382,13 +389,14 @@ static void tvdlf_corrector(ctx_t* x, double qdt, int d, const double* wCT, doub
       pl[idx] = pc[idx] + 0.5 * lj;
     }
   }
+  double* cmaxC = x->sc1;
+  if (!mu) {
   /* mean state in wLC, cmax of it */
   for (int iw = 0; iw < x->nw; ++iw) {
     double* pl = PL(x, wL, iw);
     const double* pr = PL(x, wR, iw);
     LOOP(lo, hi) { long idx = IDX(x); pl[idx] = 0.5 * (pl[idx] + pr[idx]); }
   }
-  double* cmaxC = x->sc1;
   getcmax(x, wL, lo, hi, d, cmaxC);
   if (x->p->courantpar > 0.0 && x->istep == x->nstep) {
     double cm = -BIGD;
@@ -406,6 +414,7 @@ static void tvdlf_corrector(ctx_t* x, double qdt, int d, const double* wCT, doub
     const double* pr = PL(x, wR, iw);
     LOOP(lo, hi) { long idx = IDX(x); pl[idx] = 2.0 * pl[idx] - pr[idx]; }
   }
+  }
   double* fL = x->f;
   double* fR = x->fC;
   int olo[3], ohi[3];
@@ -418,6 +427,7 @@ static void tvdlf_corrector(ctx_t* x, double qdt, int d, const double* wCT, doub
     LOOP(lo, hi) {
       long idx = IDX(x);
       double fm = 0.5 * (fL[idx] + fR[idx]);
+      if (mu) { fL[idx] = fm; continue; }
       double diss = -(cmaxC[idx] * 0.5 * (pr[idx] - pl[idx]));
       fL[idx] = fm + diss;
     }
@@ -427,6 +437,7 @@ static void tvdlf_corrector(ctx_t* x, double qdt, int d, const double* wCT, doub
       wn[idx] = wn[idx] - qdt / x->dx[d][CRD(d)] * (fL[idx] - fL[idx - s]);
     }
   }
+  if (mu) tvdlimit2(x, qdt, d, wL, wR, wnew, 1);
   /* ixI = ixO expanded by 2 in idim only, so ensurebound refills wCT ghosts */
   if (x->p->sourceunsplit) addsource(x, qdt * 1.0 / x->ndim, x->ndim == 1, wCT, wnew);
 }
@@ -715,9 +726,13 @@ static void roe_rtimes(ctx_t* x, const double* q, const double* wroe, const int*
 }
 
 /* tvdlimit2 (src/vactvd.t:86-206) with getphi for typetvd='roe' (:209-302),
- * typeentropy='nul', acmcoef<0, acmexpo=0. */
+ * typeentropy='nul', acmcoef<0, acmexpo=0.  mu = 0: method 'tvd', called by
+ * tvdlimit over ixIC = ixC widened by one face below (the limiter of phi
+ * needs the neighbour jumps); mu = 1: method 'tvdmu', called by tvdmusclf
+ * (src/vactvdlf.t:279-283) with ixIC = ixC, where getphi returns
+ * phi = |a dt/dx| jump (src/vactvd.t:227-240). */
 static void tvdlimit2(ctx_t* x, double qdt, int d, const double* wL, const double* wR,
-                      double* wnew) {
+                      double* wnew, int mu) {
   int icl[3], ich[3], cl[3], ch[3];
   for (int dd = 0; dd < 3; ++dd) {
     icl[dd] = cl[dd] = x->mlo[dd];
@@ -725,6 +740,7 @@ static void tvdlimit2(ctx_t* x, double qdt, int d, const double* wL, const doubl
   }
   icl[d] = x->mlo[d] - 2; ich[d] = x->mhi[d] + 1;  /* ixIC */
   cl[d] = x->mlo[d] - 1;                            /* ixC */
+  if (mu) { icl[d] = cl[d]; ich[d] = ch[d]; }
   long s = x->st[d];
   roe_average(x, wL, wR, icl, ich, d, x->wroe);
   for (int il = 0; il < x->nw; ++il) {
@@ -746,6 +762,12 @@ static void tvdlimit2(ctx_t* x, double qdt, int d, const double* wL, const doubl
       if (cm > 1.0) x->s->nerror[2]++;
       if (cm > SMALLD) x->s->dtcourant[d] = dmin(x->s->dtcourant[d], qdt * x->p->courantpar / cm);
     }
+    if (mu) {  /* getphi, method 'tvdmu' */
+      LOOP(cl, ch) {
+        long idx = IDX(x);
+        x->phi[idx] = fabs(x->adtdx[idx]) * x->jump[idx];
+      }
+    } else {
     /* getphi, method 'tvd' */
     LOOP(icl, ich) {
       long idx = IDX(x);
@@ -765,6 +787,7 @@ static void tvdlimit2(ctx_t* x, double qdt, int d, const double* wL, const doubl
       if (av <= 0.0) x->phi[idx] = x->phi[idx] * (x->jump[idx] - x->ljump[idx + s]);
       else x->phi[idx] = x->phi[idx] * (x->jump[idx] - x->ljump[idx]);
       x->phi[idx] = x->phi[idx] + av * av * x->jump[idx];
+    }
     }
     for (int iw = 0; iw < x->nw; ++iw) {
       roe_rtimes(x, x->phi, x->wroe, cl, ch, iw, il, d, x->rphi);
@@ -818,7 +841,7 @@ static void tvdlimit(ctx_t* x, double qdt, int d, double* wCT, double* wnew) {
     double* pr = PL(x, x->wRC, iw);
     for (long idx = 0; idx + s < x->nc; ++idx) pr[idx] = pc[idx + s];
   }
-  tvdlimit2(x, qdt, d, wCT, x->wRC, wnew);
+  tvdlimit2(x, qdt, d, wCT, x->wRC, wnew, 0);
   if (x->lastsweep && x->p->sourceunsplit) {
     /* Ryu et al. (ApJ 452, 364): whalf=(wold+whydro+S)/2, wnew=whydro+S(whalf) */
     double* wt = wCT;  /* VAC overwrites its w argument (the wCT array) */
@@ -843,7 +866,8 @@ static void advect1(ctx_t* x, int method, double qdt, int d, double* wCT, double
   x->istep += 1;
   switch (method) {
     case 0: hancock(x, qdt, d, wCT, w); break;           /* 'hancock' */
-    case 1: tvdlf_corrector(x, qdt, d, wCT, w); break;   /* 'tvdlf' */
+    case 1: tvdlf_corrector(x, qdt, d, wCT, w, 0); break;   /* 'tvdlf' */
+    case 3: tvdlf_corrector(x, qdt, d, wCT, w, 1); break;   /* 'tvdmu' */
     case 2: centdiff(x, qdt, d, wCT, w); tvdlimit(x, qdt, d, wCT, w); break;  /* 'tvd' */
   }
   getboundary(x, w);
@@ -854,9 +878,9 @@ static void advect(ctx_t* x, int d, double* w) {
   memcpy(x->w1, w, sizeof(double) * x->nc * x->nw);
   x->istep = 0;
   double dt = x->s->dt;
-  if (x->p->typefull == VACREF_TVDLF) {
+  if (x->p->typefull == VACREF_TVDLF || x->p->typefull == VACREF_TVDMU) {
     advect1(x, 0, dt / 2, d, w, x->w1);
-    advect1(x, 1, dt, d, x->w1, w);
+    advect1(x, x->p->typefull == VACREF_TVDMU ? 3 : 1, dt, d, x->w1, w);
   } else {
     advect1(x, 2, dt, d, x->w1, w);
   }
@@ -907,7 +931,7 @@ static int ctx_init(ctx_t* x, const vacref_params* p, vacref_state* s,
   x->ndir = p->ndim;
   x->mhd = (p->phys == VACREF_MHD);
   x->nw = x->mhd ? 2 + 2 * x->ndir : 2 + x->ndim;
-  if (!x->mhd && p->typefull == VACREF_TVD) return -2;  /* HD Roe solver not restated */
+  if (!x->mhd && p->typefull != VACREF_TVDLF) return -2;  /* HD Roe solver not restated */
   const double* dxs[3] = {dx0, dx1, dx2};
   long st = 1;
   for (int d = 0; d < 3; ++d) {
@@ -925,7 +949,7 @@ static int ctx_init(ctx_t* x, const vacref_params* p, vacref_state* s,
     }
   }
   x->nc = st;
-  x->nstep = (p->typefull == VACREF_TVDLF) ? 2 : 1;
+  x->nstep = (p->typefull == VACREF_TVD) ? 1 : 2;
   size_t pl = sizeof(double) * (size_t)x->nc;
   x->w1 = calloc(x->nw, pl);
   x->wold = calloc(x->nw, pl);

@@ -1,5 +1,5 @@
 /*
- * oracle/vacref.h -- CPU restatement of VAC's explicit TVDLF / TVD (Roe)
+ * oracle/vacref.h -- CPU restatement of VAC's explicit TVDLF / TVD / TVD-MUSCL (Roe)
  * advect path for ideal MHD and HD.  TEST INFRASTRUCTURE ONLY: tests, the
  * smoke check and the CPU-baseline leg of bench.py use it as the checker;
  * the product (libvacgpu) never links or calls it.
@@ -16,7 +16,7 @@ extern "C" {
 #endif
 
 enum { VACREF_HD = 0, VACREF_MHD = 1 };
-enum { VACREF_TVDLF = 0, VACREF_TVD = 1 };
+enum { VACREF_TVDLF = 0, VACREF_TVD = 1, VACREF_TVDMU = 2 };
 enum { VACREF_MINMOD = 0, VACREF_WOODWARD = 1, VACREF_SUPERBEE = 2 };
 enum { VACREF_CONT = 0, VACREF_PERIODIC = 1 };
 
@@ -25,7 +25,7 @@ typedef struct {
   int phys;              /* VACREF_HD / VACREF_MHD */
   int nx[3];             /* mesh cells per dimension */
   double gamma;          /* eqpar(gamma_) */
-  int typefull;          /* VACREF_TVDLF (twostep, hancock) / VACREF_TVD (onestep) */
+  int typefull;          /* VACREF_TVDLF / VACREF_TVDMU (twostep, hancock), VACREF_TVD (onestep) */
   int limiter[8];        /* typelimiter(1:nw) */
   int bc[6][8];          /* typeB(iw, iB), iB = 2*d + upper */
   int divbfix, divbwave, sourceunsplit;

@@ -16,7 +16,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB = os.path.join(HERE, "libvacref.so")
 
 HD, MHD = 0, 1
-TVDLF, TVD = 0, 1
+TVDLF, TVD, TVDMU = 0, 1, 2
 LIMITERS = {"minmod": 0, "woodward": 1, "superbee": 2}
 CONT, PERIODIC = 0, 1
 
@@ -83,7 +83,7 @@ def make_params(ndim, physics, nx, gamma, typefull="tvdlf", limiter="minmod",
     for d in range(ndim):
         p.nx[d] = int(nx[d])
     p.gamma = gamma
-    p.typefull = TVDLF if typefull == "tvdlf" else TVD
+    p.typefull = {"tvdlf": TVDLF, "tvd": TVD, "tvdmu": TVDMU}[typefull]
     lims = [limiter] * 8 if isinstance(limiter, str) else list(limiter) + ["minmod"] * 8
     for i in range(8):
         p.limiter[i] = LIMITERS[lims[i]]

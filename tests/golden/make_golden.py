@@ -5,7 +5,7 @@ reference's Fortran sources by oracle/build_ref.sh) on an initial state made
 by vac_amd.initial:  the initial mesh state, the parameters, the time t after
 every step (from the per-step snapshots VAC writes) and the final mesh state.
 
-    python tests/golden/make_golden.py            # needs /root/reference
+    python tests/golden/make_golden.py [name ...]  # needs /root/reference
 
 The config-1 run (Orszag-Tang 128^2, TVDLF + minmod, 100 steps) is recorded
 as SHA-256 digests of its initial and final states plus its t sequence,
@@ -44,6 +44,9 @@ CASES = {
     "hd22_rf24x24_tvdlf_woodward_cont": ("vac_hd22_132", "hd", lambda g: random_fourier((24, 24), 106, g, physics="hd"), 5 / 3, 20, "tvdlf", "woodward", "cont", True),
     "mhd33_rf12x10x8_tvdlf_minmod": ("vac_mhd33_36", "mhd", lambda g: random_fourier((12, 10, 8), 107, g), 5 / 3, 10, "tvdlf", "minmod", "periodic", True),
     "mhd33_rf8x10x12_tvd_woodward": ("vac_mhd33_36", "mhd", lambda g: random_fourier((8, 10, 12), 108, g), 5 / 3, 10, "tvd", "woodward", "periodic", True),
+    "mhd22_rf32x24_tvdmu_minmod": ("vac_mhd22_132", "mhd", lambda g: random_fourier((32, 24), 109, g), 5 / 3, 20, "tvdmu", "minmod", "periodic", True),
+    "mhd22_rf24x28_tvdmu_woodward_cont": ("vac_mhd22_132", "mhd", lambda g: random_fourier((24, 28), 110, g), 1.4, 15, "tvdmu", "woodward", "cont", True),
+    "mhd33_rf10x8x12_tvdmu_superbee": ("vac_mhd33_36", "mhd", lambda g: random_fourier((10, 8, 12), 111, g), 5 / 3, 8, "tvdmu", "superbee", "periodic", True),
 }
 
 
@@ -69,9 +72,12 @@ def sha(a: np.ndarray) -> str:
 
 
 def main() -> None:
+    only = sys.argv[1:]  # optional: write only the fixtures whose names contain one of these
     for b in ("vac_mhd22_132", "vac_hd22_132", "vac_mhd33_36"):
         ensure_binary(b)
     for name, (binary, physics, build, gamma, steps, typefull, limiter, bc, divbfix) in CASES.items():
+        if only and not any(o in name for o in only):
+            continue
         w0, coords = build(gamma)
         wf, t = run_case(binary, physics, w0, coords, gamma, steps, typefull, limiter, bc, divbfix)
         np.savez_compressed(
@@ -80,6 +86,8 @@ def main() -> None:
             physics=physics, typefull=typefull, limiter=limiter, bc=bc,
             divbfix=divbfix)
         print("wrote", name, w0.shape, "steps", len(t) - 1)
+    if only:
+        return
     # config 1 at full size: digests only
     g = 5.0 / 3.0
     w0, coords = orszag_tang(128, g)
