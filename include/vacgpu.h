@@ -31,7 +31,7 @@ enum {
 };
 
 enum { VACGPU_HD = 0, VACGPU_MHD = 1 };                 /* setvac -p= */
-enum { VACGPU_TVDLF = 0, VACGPU_TVD = 1 };              /* typefull */
+enum { VACGPU_TVDLF = 0, VACGPU_TVD = 1, VACGPU_TVDMU = 2 };  /* typefull */
 enum { VACGPU_MINMOD = 0, VACGPU_WOODWARD = 1, VACGPU_SUPERBEE = 2 };  /* typelimiter */
 enum { VACGPU_CONT = 0, VACGPU_PERIODIC = 1 };          /* typeB */
 enum { VACGPU_FP64 = 0, VACGPU_FP32BULK = 1 };          /* arithmetic tier */
@@ -51,7 +51,7 @@ typedef struct {
 
 /* Method selectors read from the par file (&methodlist, &paramlist). */
 typedef struct {
-  int typefull;         /* VACGPU_TVDLF (twostep, hancock) or VACGPU_TVD (onestep) */
+  int typefull;         /* VACGPU_TVDLF or VACGPU_TVDMU (twostep, hancock), VACGPU_TVD (onestep) */
   int typelimiter[8];   /* per variable (TVDLF) or per characteristic wave (TVD) */
   int divbfix;          /* Powell source terms (MHD) */
   int divbwave;         /* divergence wave in the Roe solver (MHD) */

@@ -30,6 +30,10 @@ CASES = [
      "tvdlf", "woodward", "periodic", True),
     ("vac_mhd33_36", "vacgpu_mhd33_36", "mhd", lambda g: random_fourier((16, 12, 14), 64, g), 5 / 3, 8,
      "tvd", "woodward", "periodic", True),
+    ("vac_mhd22_132", "vacgpu_mhd22_132", "mhd", lambda g: random_fourier((44, 36), 65, g), 5 / 3, 20,
+     "tvdmu", "woodward", "periodic", True),
+    ("vac_mhd33_36", "vacgpu_mhd33_36", "mhd", lambda g: random_fourier((12, 14, 10), 66, g), 1.4, 6,
+     "tvdmu", "minmod", "cont", True),
 ]
 
 

@@ -34,6 +34,7 @@ CASES = [
     ("hd-rf192-tvdlf-woodward", lambda g: random_fourier((192, 160), 4, g, physics="hd"), "hd", 1.4, "tvdlf", "woodward", 60, "periodic"),
     ("rf256-tvdlf-minmod-200", lambda g: random_fourier((256, 256), 6, g), "mhd", 5 / 3, "tvdlf", "minmod", 200, "periodic"),
     ("rf160x96-tvdlf-superbee-cont", lambda g: random_fourier((160, 96), 8, g), "mhd", 1.4, "tvdlf", "superbee", 80, "cont"),
+    ("rf128-tvdmu-woodward", lambda g: random_fourier((128, 128), 5, g), "mhd", 5 / 3, "tvdmu", "woodward", 40, "periodic"),
     ("hd-rf128-tvdlf-minmod-cont", lambda g: random_fourier((128, 128), 9, g, physics="hd"), "hd", 5 / 3, "tvdlf", "minmod", 80, "cont"),
 ]
 

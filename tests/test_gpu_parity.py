@@ -131,6 +131,10 @@ def test_host_loop_through_advect_matches_run(typefull, limiter):
     ("hd", (16, 20, 12), "tvdlf", "minmod", "periodic", 8),
     ("mhd", (64, 40), "tvd", "superbee", "cont", 15),
     ("mhd", (14, 12, 10), "tvd", "minmod", "cont", 5),
+    ("mhd", (72, 40), "tvdmu", "woodward", "periodic", 12),
+    ("mhd", (40, 56), "tvdmu", "superbee", "cont", 10),
+    ("mhd", (12, 14, 10), "tvdmu", "minmod", "periodic", 5),
+    ("mhd", (10, 12, 14), "tvdmu", "woodward", "cont", 5),
 ])
 def test_gpu_matches_oracle(case):
     physics, nx, typefull, limiter, bc, steps = case
@@ -147,7 +151,7 @@ def test_gpu_matches_oracle(case):
 
 
 @pytest.mark.parametrize("typefull,limiter", [("tvd", "woodward"), ("tvd", "minmod"),
-                                              ("tvdlf", "minmod")])
+                                              ("tvdlf", "minmod"), ("tvdmu", "minmod")])
 def test_courant_violation_counted_like_vac(typefull, limiter):
     # courantpar 2: every sweep's Courant fold sees CFL > 1 for the extreme
     # waves and must count it (nerror(couranterr_)) exactly as VAC does, with

@@ -1193,6 +1193,105 @@ __device__ __forceinline__ S shfl_up_struct(const S& v) {
   return o;
 }
 
+// TVD-MUSCL corrector (typefull='tvdmu', src/vactvdlf.t:92-299 with
+// tvdlimit2 at :279-283, src/vactvd.t:86-206): per cell, the two faces'
+// states from the predictor w1 and the limited slopes of w
+// (typelimited='original'), the central flux (F(wL) + F(wR))/2, then the Roe
+// dissipation of those face states with phi = |a dt 2/(dx_J + dx_J+1)| jump
+// (getphi for 'tvdmu', src/vactvd.t:227-240), then the Powell source of w1 as
+// in the TVDLF corrector.  The two extreme waves' |a dt 2/(dx_J+dx_J+1)| feed
+// the Courant fold (src/vactvd.t:145-157): each cell gives its right face,
+// the first cell also its left one.  One thread per cell, both faces.
+template <int ND, class R, int D>
+__global__ __launch_bounds__(256) void k_tvdmu_corr(Geo g, Sel s, const double* __restrict__ w,
+                                                   const double* __restrict__ w1,
+                                                   double* __restrict__ wout,
+                                                   const Scal* __restrict__ sc,
+                                                   double* __restrict__ part, long long pstride) {
+  constexpr int d = D;
+  using L = Layout<ND, true>;
+  constexpr int NW = L::nw;
+  __shared__ double sm[8];
+  const double dt = sc->dt, smallp = sc->smallp;
+  const CellIx cl = cell_of(g);
+  double c0 = -INFINITY, c1 = -INFINITY;
+  if (cl.ok) {
+    const long long sd = g.s[d];
+    const int ci = cl.c[d];
+    double lm[NW], l0[NW], lp[NW];
+#pragma unroll
+    for (int iw = 0; iw < NW; ++iw) {
+      const double* p = w + iw * g.ps + cl.idx;
+      const double a = p[-2 * sd], b = p[-sd], c = p[0], e = p[sd], f = p[2 * sd];
+      lm[iw] = slope<R>(s.lim[iw], a, b, c);
+      l0[iw] = slope<R>(s.lim[iw], b, c, e);
+      lp[iw] = slope<R>(s.lim[iw], c, e, f);
+    }
+    double um[NW], u0[NW], up[NW];
+    load<NW>(w1, g.ps, cl.idx - sd, um);
+    load<NW>(w1, g.ps, cl.idx, u0);
+    load<NW>(w1, g.ps, cl.idx + sd, up);
+    // face states: a = face ci-1/2, b = face ci+1/2
+    double aL[NW], aR[NW], bL[NW], bR[NW];
+#pragma unroll
+    for (int iw = 0; iw < NW; ++iw) {
+      aR[iw] = u0[iw] - 0.5 * l0[iw];
+      aL[iw] = um[iw] + 0.5 * lm[iw];
+      bR[iw] = up[iw] - 0.5 * lp[iw];
+      bL[iw] = u0[iw] + 0.5 * l0[iw];
+    }
+    double out[NW];
+    {
+      double fL[NW], fR[NW], Fa[NW];
+      fluxes<ND, true, R>(aL, d, s.gamma, smallp, fL);
+      fluxes<ND, true, R>(aR, d, s.gamma, smallp, fR);
+#pragma unroll
+      for (int iw = 0; iw < NW; ++iw) Fa[iw] = 0.5 * (fL[iw] + fR[iw]);
+      fluxes<ND, true, R>(bL, d, s.gamma, smallp, fL);
+      fluxes<ND, true, R>(bR, d, s.gamma, smallp, fR);
+      const double qdx = pdiv(dt, g.dx[d][ci]);
+#pragma unroll
+      for (int iw = 0; iw < NW; ++iw)
+        out[iw] = w[iw * g.ps + cl.idx] - qdx * (0.5 * (fL[iw] + fR[iw]) - Fa[iw]);
+    }
+    TvdFace<ND, R> fa, fb;
+    double ca0, ca1;
+    tvd_face<ND, R>(aL, aR, d, s, smallp, dt, g.dx[d][ci - 1] + g.dx[d][ci], fa, ca0, ca1);
+    tvd_face<ND, R>(bL, bR, d, s, smallp, dt, g.dx[d][ci] + g.dx[d][ci + 1], fb, c0, c1);
+    if (ci == 2) {
+      c0 = c0 < ca0 ? ca0 : c0;
+      c1 = c1 < ca1 ? ca1 : c1;
+    }
+    R phia[NW], phib[NW];
+#pragma unroll
+    for (int il = 0; il < NW; ++il) {
+      phia[il] = R(fabs(fa.adt[il])) * fa.jump[il];
+      phib[il] = R(fabs(fb.adt[il])) * fb.jump[il];
+    }
+    // dvolume of cells ci-1, ci, ci+1 ((dx1*dx2)*dx3 in dimension order)
+    auto dvol = [&](int k) {
+      int c3[3] = {cl.c[0], cl.c[1], cl.c[2]};
+      c3[d] = k;
+      double v = g.dx[0][c3[0]];
+      v = v * g.dx[1][c3[1]];
+      if (ND == 3) v = v * g.dx[2][c3[2]];
+      return v;
+    };
+    const double dm = dvol(ci - 1), d0 = dvol(ci), dp = dvol(ci + 1);
+    tvd_dissipate<ND, R>(fa.rt, phia, fb.rt, phib, d, s, R(d0 + dm), R(dp + d0), d0, out);
+    if (s.sourceunsplit && s.divbfix) {
+      const double dv = divb_at<ND>(g, w1, cl.idx, cl.c);
+      powell<ND>(u0, dv, dt * 1.0 / ND, out);
+    }
+#pragma unroll
+    for (int iw = 0; iw < NW; ++iw) wout[iw * g.ps + cl.idx] = out[iw];
+  }
+  const double b0 = block_reduce<true>(c0, sm);
+  if (threadIdx.x == 0 && threadIdx.y == 0) part[SLOT_C0 * pstride + block_linear()] = b0;
+  const double b1 = block_reduce<true>(c1, sm);
+  if (threadIdx.x == 0 && threadIdx.y == 0) part[SLOT_C1 * pstride + block_linear()] = b1;
+}
+
 constexpr int TX_STRIDE = 61;  // cells updated per 64-lane wavefront along x
 
 // x sweep: lane L of a wavefront owns face J = x0 - 2 + L (between cells J and

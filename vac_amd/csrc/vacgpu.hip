@@ -160,8 +160,8 @@ int check_config(const vacgpu_config* c, std::string& why) {
 
 int check_method(const vacgpu_config* c, const vacgpu_method* m, std::string& why) {
   if (!m) { why = "null method"; return VACGPU_E_ARG; }
-  if (m->typefull != VACGPU_TVDLF && m->typefull != VACGPU_TVD) { why = "typefull must be 'tvdlf' or 'tvd'"; return VACGPU_E_ARG; }
-  if (m->typefull == VACGPU_TVD && c->physics == VACGPU_HD) { why = "typefull='tvd' needs the HD Roe solver, which is not on the GPU path"; return VACGPU_E_ARG; }
+  if (m->typefull != VACGPU_TVDLF && m->typefull != VACGPU_TVD && m->typefull != VACGPU_TVDMU) { why = "typefull must be 'tvdlf', 'tvd' or 'tvdmu'"; return VACGPU_E_ARG; }
+  if (m->typefull != VACGPU_TVDLF && c->physics == VACGPU_HD) { why = "typefull='tvd' and 'tvdmu' need the HD Roe solver, which is not on the GPU path"; return VACGPU_E_ARG; }
   int nw = c->physics == VACGPU_MHD ? 2 + 2 * c->ndim : 2 + c->ndim;
   for (int iw = 0; iw < nw; ++iw)
     if (m->typelimiter[iw] < VACGPU_MINMOD || m->typelimiter[iw] > VACGPU_SUPERBEE) { why = "typelimiter must be minmod, woodward or superbee"; return VACGPU_E_ARG; }
@@ -392,6 +392,41 @@ int sweep_device(vacgpu_grid* g, const vacgpu_method* m, int d, int firstsweep, 
     // the fused kernel has folded the Courant maximum and filled wout's ghost cells
     g->cur = outi;
     return 0;
+  } else if (m->typefull == VACGPU_TVDMU) {
+    // TVD-MUSCL: the Hancock predictor of TVDLF, then one thread per cell for
+    // the central flux, the Roe dissipation and the Powell source
+    const int ghosts = (g->bc_uniform && g->use_fused) ? 1 : 0;
+    {
+      Timed t(g, KF_PRED);
+      DISPATCH(g, {
+        if (d == 0) hipLaunchKernelGGL((k_tvdlf_pred<ND, MHD, R, 0>), g->grid, g->block, 0, g->stream, g->geo, s, g->bc, win, g->w1, g->sc, ghosts);
+        else if (d == 1) hipLaunchKernelGGL((k_tvdlf_pred<ND, MHD, R, 1>), g->grid, g->block, 0, g->stream, g->geo, s, g->bc, win, g->w1, g->sc, ghosts);
+        else hipLaunchKernelGGL((k_tvdlf_pred<ND, MHD, R, (ND == 3 ? 2 : 1)>), g->grid, g->block, 0, g->stream, g->geo, s, g->bc, win, g->w1, g->sc, ghosts);
+      });
+      if ((rc = launch_check(g, "k_tvdlf_pred"))) return rc;
+    }
+    if (!ghosts && (rc = bound(g, g->w1))) return rc;
+    {
+      Timed t(g, KF_CORR);
+#define MU_L(NDV, RT, DV) hipLaunchKernelGGL((k_tvdmu_corr<NDV, RT, DV>), g->grid, g->block, 0, g->stream, g->geo, s, win, g->w1, wout, g->sc, g->part, g->pstride)
+      const bool f32 = g->cfg.precision == VACGPU_FP32BULK;
+      if (g->cfg.ndim == 2) {
+        if (d == 0) { if (f32) MU_L(2, float, 0); else MU_L(2, double, 0); }
+        else { if (f32) MU_L(2, float, 1); else MU_L(2, double, 1); }
+      } else {
+        if (d == 0) { if (f32) MU_L(3, float, 0); else MU_L(3, double, 0); }
+        else if (d == 1) { if (f32) MU_L(3, float, 1); else MU_L(3, double, 1); }
+        else { if (f32) MU_L(3, float, 2); else MU_L(3, double, 2); }
+      }
+#undef MU_L
+      if ((rc = launch_check(g, "k_tvdmu_corr"))) return rc;
+    }
+    if (courant) {
+      Timed t(g, KF_RED);
+      hipLaunchKernelGGL(k_fin_courant_tvd, dim3(1), dim3(1024), 0, g->stream, g->sc, g->part,
+                         g->pstride, nb, d, m->courantpar);
+      if ((rc = launch_check(g, "k_fin_courant_tvd"))) return rc;
+    }
   } else if (m->typefull == VACGPU_TVDLF) {
     long long corr_blocks = nb;
     // with one boundary type per face the kernels write their outputs' ghost cells

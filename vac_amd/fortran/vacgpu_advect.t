@@ -58,6 +58,11 @@ case('tvdlf')
    if(typeadvance/='twostep' .or. typepred1/='hancock') &
       call die('vacgpu advect: tvdlf needs typeadvance=twostep, typepred=hancock')
    m%typefull=VACGPU_TVDLF
+case('tvdmu')
+   if(typeadvance/='twostep' .or. typepred1/='hancock') &
+      call die('vacgpu advect: tvdmu needs typeadvance=twostep, typepred=hancock')
+   if(typetvd/='roe') call die('vacgpu advect: typetvd must be roe')
+   m%typefull=VACGPU_TVDMU
 case('tvd')
    if(typeadvance/='onestep') call die('vacgpu advect: tvd needs typeadvance=onestep')
    if(typetvd/='roe') call die('vacgpu advect: typetvd must be roe')

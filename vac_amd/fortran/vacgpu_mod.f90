@@ -14,7 +14,7 @@ module vacgpu_mod
   integer(c_int), parameter :: VACGPU_OK = 0, VACGPU_E_ARG = -1, VACGPU_E_HIP = -2, &
                                VACGPU_E_STATE = -3, VACGPU_E_NOMEM = -4
   integer(c_int), parameter :: VACGPU_HD = 0, VACGPU_MHD = 1
-  integer(c_int), parameter :: VACGPU_TVDLF = 0, VACGPU_TVD = 1
+  integer(c_int), parameter :: VACGPU_TVDLF = 0, VACGPU_TVD = 1, VACGPU_TVDMU = 2
   integer(c_int), parameter :: VACGPU_MINMOD = 0, VACGPU_WOODWARD = 1, VACGPU_SUPERBEE = 2
   integer(c_int), parameter :: VACGPU_CONT = 0, VACGPU_PERIODIC = 1
   integer(c_int), parameter :: VACGPU_FP64 = 0, VACGPU_FP32BULK = 1

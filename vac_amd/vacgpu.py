@@ -20,7 +20,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libvacgpu.so")
 
 HD, MHD = 0, 1
-TVDLF, TVD = 0, 1
+TVDLF, TVD, TVDMU = 0, 1, 2
 LIMITERS = {"minmod": 0, "woodward": 1, "superbee": 2}
 CONT, PERIODIC = 0, 1
 FP64, FP32BULK = 0, 1
@@ -126,7 +126,7 @@ class MethodSpec:
 
     def to_c(self) -> Method:
         m = Method()
-        m.typefull = TVDLF if self.typefull == "tvdlf" else TVD
+        m.typefull = {"tvdlf": TVDLF, "tvd": TVD, "tvdmu": TVDMU}[self.typefull]
         lims = [self.limiter] * 8 if isinstance(self.limiter, str) else list(self.limiter) + ["minmod"] * 8
         for i in range(8):
             m.typelimiter[i] = LIMITERS[lims[i]]
