@@ -107,6 +107,7 @@ def main():
                          "device copy (calibration), then 3 steps each of the 2048^2 TVD "
                          "and TVDLF workloads; prints no JSON")
     ap.add_argument("--no-tvdlf", action="store_true")
+    ap.add_argument("--no-tvdmu", action="store_true")
     ap.add_argument("--no-3d", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-serving", action="store_true")
@@ -187,6 +188,8 @@ def main():
     }
     if not args.no_tvdlf:
         out["tvdlf_2048"] = tvdlf_2048(w0, dxs, min(args.steps, 100))
+    if not args.no_tvdmu:
+        out["tvdmu_2048"] = tvdmu_2048(w0, dxs, min(args.steps, 50))
     if not args.no_3d:
         out["mhd3d_256"] = mhd3d_256()
     if not args.no_snapshots:
@@ -288,6 +291,26 @@ def tvdlf_2048(w0, dxs, steps):
     res["workload"] = (f"2D MHD 2048x2048 periodic random-Fourier, TVDLF+minmod, courantpar 0.8, "
                        f"{steps} steps; B_alg 480 B/cell-update")
     return res
+
+
+def tvdmu_2048(w0, dxs, steps):
+    """2D MHD 2048^2, TVD-MUSCL (typefull='tvdmu') + woodward, FP64.  Same
+    dataflow as TVDLF: per sweep the predictor reads w and writes w1, the
+    corrector reads w and w1 and writes the result, so B_alg is 480 B per
+    cell-update (240 B per sweep; the corrector's share is 144 B)."""
+    from vac_amd import vacgpu
+
+    method = vacgpu.MethodSpec(typefull="tvdmu", limiter="woodward", courantpar=0.8)
+    el, w, kt = timed_run([NX, NY], "mhd", dxs, w0, method, "fp64", steps,
+                          ("tvdlf_pred", "tvdlf_corr", "bound", "reduce"))
+    rate = NX * NY * steps / el
+    kc = kt["tvdlf_corr"]
+    return {"mcell_updates_per_s": round(rate / 1e6, 2), "ms_per_step": round(el / steps * 1e3, 4),
+            "effective_hbm_frac": round(480 * rate / 1e9 / HBM_PEAK_GBS, 4),
+            "kernel_avg_ms": kt,
+            "corrector_effective_gbs": round(144 * NX * NY / (kc * 1e-3) / 1e9, 1) if kc else None,
+            "workload": (f"2D MHD 2048x2048 periodic random-Fourier, TVD-MUSCL+woodward, "
+                         f"courantpar 0.8, {steps} steps; B_alg 480 B/cell-update")}
 
 
 def mhd3d_256(steps=100):

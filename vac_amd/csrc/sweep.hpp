@@ -1194,97 +1194,204 @@ __device__ __forceinline__ S shfl_up_struct(const S& v) {
 }
 
 // TVD-MUSCL corrector (typefull='tvdmu', src/vactvdlf.t:92-299 with
-// tvdlimit2 at :279-283, src/vactvd.t:86-206): per cell, the two faces'
-// states from the predictor w1 and the limited slopes of w
-// (typelimited='original'), the central flux (F(wL) + F(wR))/2, then the Roe
-// dissipation of those face states with phi = |a dt 2/(dx_J + dx_J+1)| jump
-// (getphi for 'tvdmu', src/vactvd.t:227-240), then the Powell source of w1 as
-// in the TVDLF corrector.  The two extreme waves' |a dt 2/(dx_J+dx_J+1)| feed
-// the Courant fold (src/vactvd.t:145-157): each cell gives its right face,
-// the first cell also its left one.  One thread per cell, both faces.
+// tvdlimit2 at :279-283, src/vactvd.t:86-206), face-parallel like the TVDLF
+// corrector (k_tvdlf_corrx / k_tvdlf_corrm: the same face states from the
+// predictor w1 and the limited slopes of w, typelimited='original').  A face
+// carries the central flux (F(wL) + F(wR))/2 and the Roe decomposition of its
+// states with phi = |a dt 2/(dx_J + dx_J+1)| jump (getphi for 'tvdmu',
+// src/vactvd.t:227-240); a cell takes the flux difference, the dissipation of
+// its two faces and the Powell source of w1 as in the TVDLF corrector.  The
+// two extreme waves' |a dt 2/(dx_J + dx_J+1)| over the faces ixC feed the
+// Courant fold (src/vactvd.t:145-157).
 template <int ND, class R, int D>
-__global__ __launch_bounds__(256) void k_tvdmu_corr(Geo g, Sel s, const double* __restrict__ w,
-                                                   const double* __restrict__ w1,
-                                                   double* __restrict__ wout,
-                                                   const Scal* __restrict__ sc,
-                                                   double* __restrict__ part, long long pstride) {
-  constexpr int d = D;
+__device__ __forceinline__ void mu_face(const Sel& s, const double* u0, const double* u1,
+                                        const double* ll, const double* lr, double dt,
+                                        double smallp, double dxsum, double* F,
+                                        TvdFace<ND, R>& f, R* phi, double& c0, double& c1) {
+  constexpr int NW = Layout<ND, true>::nw;
+  double uL[NW], uR[NW];
+#pragma unroll
+  for (int iw = 0; iw < NW; ++iw) {
+    uR[iw] = u1[iw] - 0.5 * lr[iw];
+    uL[iw] = u0[iw] + 0.5 * ll[iw];
+  }
+  {
+    double fL[NW], fR[NW];
+    fluxes<ND, true, R>(uL, D, s.gamma, smallp, fL);
+    fluxes<ND, true, R>(uR, D, s.gamma, smallp, fR);
+#pragma unroll
+    for (int iw = 0; iw < NW; ++iw) F[iw] = 0.5 * (fL[iw] + fR[iw]);
+  }
+  tvd_face<ND, R>(uL, uR, D, s, smallp, dt, dxsum, f, c0, c1);
+#pragma unroll
+  for (int il = 0; il < NW; ++il) phi[il] = R(fabs(f.adt[il])) * f.jump[il];
+}
+
+// x sweep: lane L owns face c+1/2 of cell c = x0 - 1 + L and updates c for
+// L in [1, 62]; the left face's flux and dissipation term come from lane L-1.
+template <int ND, class R, int LIM>
+__global__ __launch_bounds__(256) void k_tvdmu_x(Geo g, Sel s, BC bc, const double* __restrict__ w,
+                                                const double* __restrict__ w1,
+                                                double* __restrict__ wout,
+                                                const Scal* __restrict__ sc,
+                                                double* __restrict__ part, long long pstride,
+                                                int ghosts) {
   using L = Layout<ND, true>;
   constexpr int NW = L::nw;
   __shared__ double sm[8];
   const double dt = sc->dt, smallp = sc->smallp;
-  const CellIx cl = cell_of(g);
+  const int lane = (int)threadIdx.x;
+  const int c = 1 + (int)blockIdx.x * CX_STRIDE + lane;
+  const int y = 2 + (int)(blockIdx.y * BY + threadIdx.y);
+  const int z = (ND == 3) ? 2 + (int)blockIdx.z : 0;
+  const bool rowok = y <= g.n[1] + 1;
+  const bool useful = rowok && lane >= 1 && lane <= CX_STRIDE && c <= g.n[0] + 1;
+  const bool face_ok = rowok && lane <= CX_STRIDE && c <= g.n[0] + 1;
+  const int cc = c < g.n[0] + 2 ? c : g.n[0] + 2;  // idle lanes: in-bounds, unused
+  const int yc = rowok ? y : g.n[1] + 1;
+  const long long idx = g.off + cc + (long long)yc * g.s[1] + (long long)z * g.s[2];
+  double l[NW], lr[NW], u0[NW], u1[NW], wc[NW];
+#pragma unroll
+  for (int iw = 0; iw < NW; ++iw) {
+    const double* p = w + iw * g.ps + idx;
+    wc[iw] = p[0];
+    l[iw] = slope<R, LIM>(s.lim[iw], p[-1], wc[iw], p[1]);
+    u0[iw] = w1[iw * g.ps + idx];
+    u1[iw] = w1[iw * g.ps + idx + 1];
+  }
+#pragma unroll
+  for (int iw = 0; iw < NW; ++iw) lr[iw] = lane_next(l[iw]);
+  double F[NW], c0, c1;
+  TvdFace<ND, R> f;
+  R phi[NW];
+  mu_face<ND, R, 0>(s, u0, u1, l, lr, dt, smallp, g.dx[0][cc] + g.dx[0][cc + 1], F, f, phi, c0, c1);
+  if (!face_ok) { c0 = -INFINITY; c1 = -INFINITY; }
+  double out[NW];
+  const double qdx = pdiv(dt, g.dx[0][cc]);
+#pragma unroll
+  for (int iw = 0; iw < NW; ++iw) {
+    const double Fm = lane_prev(F[iw]);
+    out[iw] = wc[iw] - qdx * (F[iw] - Fm);
+  }
+  // dvolume = (dx1*dx2)*dx3 evaluated left to right
+  auto dvol = [&](int xi) {
+    double v = g.dx[0][xi];
+    v = v * g.dx[1][yc];
+    if (ND == 3) v = v * g.dx[2][z];
+    return v;
+  };
+  const double d0 = dvol(cc), dp = dvol(cc + 1);
+  tvd_dissipate_x<ND, R>(f.rt, phi, s, R(dp + d0), d0, out);
+  const int c3[3] = {cc, yc, z};
+  if (s.sourceunsplit && s.divbfix) {
+    const double dv = divb_at<ND>(g, w1, idx, c3);
+    powell<ND>(u0, dv, dt * 1.0 / ND, out);
+  }
+  if (useful) store_cell<ND, NW>(g, bc, wout, idx, c3, out, ghosts);
+  const double b0 = block_reduce<true>(c0, sm);
+  if (threadIdx.x == 0 && threadIdx.y == 0) part[SLOT_C0 * pstride + block_linear()] = b0;
+  const double b1 = block_reduce<true>(c1, sm);
+  if (threadIdx.x == 0 && threadIdx.y == 0) part[SLOT_C1 * pstride + block_linear()] = b1;
+}
+
+// y / z sweep: thread (x, segment) marches through cells s0..s1 along D,
+// carrying the face below (flux, Roe decomposition, phi) and a window of w and w1.
+template <int ND, class R, int D, int LIM>
+__global__ __launch_bounds__(256) void k_tvdmu_m(Geo g, Sel s, BC bc, const double* __restrict__ w,
+                                                const double* __restrict__ w1,
+                                                double* __restrict__ wout,
+                                                const Scal* __restrict__ sc,
+                                                double* __restrict__ part, long long pstride,
+                                                int seglen, int ghosts) {
+  using L = Layout<ND, true>;
+  constexpr int NW = L::nw;
+  __shared__ double sm[8];
+  const double dt = sc->dt, smallp = sc->smallp;
+  const int x = 2 + (int)(blockIdx.x * BX + threadIdx.x);
+  const int seg = (int)(blockIdx.y * BY + threadIdx.y);
+  const int s0 = 2 + seg * seglen;
+  const int t = (ND == 3) ? 2 + (int)blockIdx.z : 0;  // the coordinate that is neither x nor D
+  const int other = (D == 1) ? 2 : 1;
+  const int nd_ = g.n[D];
   double c0 = -INFINITY, c1 = -INFINITY;
-  if (cl.ok) {
-    const long long sd = g.s[d];
-    const int ci = cl.c[d];
-    double lm[NW], l0[NW], lp[NW];
-#pragma unroll
-    for (int iw = 0; iw < NW; ++iw) {
-      const double* p = w + iw * g.ps + cl.idx;
-      const double a = p[-2 * sd], b = p[-sd], c = p[0], e = p[sd], f = p[2 * sd];
-      lm[iw] = slope<R>(s.lim[iw], a, b, c);
-      l0[iw] = slope<R>(s.lim[iw], b, c, e);
-      lp[iw] = slope<R>(s.lim[iw], c, e, f);
-    }
-    double um[NW], u0[NW], up[NW];
-    load<NW>(w1, g.ps, cl.idx - sd, um);
-    load<NW>(w1, g.ps, cl.idx, u0);
-    load<NW>(w1, g.ps, cl.idx + sd, up);
-    // face states: a = face ci-1/2, b = face ci+1/2
-    double aL[NW], aR[NW], bL[NW], bR[NW];
-#pragma unroll
-    for (int iw = 0; iw < NW; ++iw) {
-      aR[iw] = u0[iw] - 0.5 * l0[iw];
-      aL[iw] = um[iw] + 0.5 * lm[iw];
-      bR[iw] = up[iw] - 0.5 * lp[iw];
-      bL[iw] = u0[iw] + 0.5 * l0[iw];
-    }
-    double out[NW];
-    {
-      double fL[NW], fR[NW], Fa[NW];
-      fluxes<ND, true, R>(aL, d, s.gamma, smallp, fL);
-      fluxes<ND, true, R>(aR, d, s.gamma, smallp, fR);
-#pragma unroll
-      for (int iw = 0; iw < NW; ++iw) Fa[iw] = 0.5 * (fL[iw] + fR[iw]);
-      fluxes<ND, true, R>(bL, d, s.gamma, smallp, fL);
-      fluxes<ND, true, R>(bR, d, s.gamma, smallp, fR);
-      const double qdx = pdiv(dt, g.dx[d][ci]);
-#pragma unroll
-      for (int iw = 0; iw < NW; ++iw)
-        out[iw] = w[iw * g.ps + cl.idx] - qdx * (0.5 * (fL[iw] + fR[iw]) - Fa[iw]);
-    }
-    TvdFace<ND, R> fa, fb;
-    double ca0, ca1;
-    tvd_face<ND, R>(aL, aR, d, s, smallp, dt, g.dx[d][ci - 1] + g.dx[d][ci], fa, ca0, ca1);
-    tvd_face<ND, R>(bL, bR, d, s, smallp, dt, g.dx[d][ci] + g.dx[d][ci + 1], fb, c0, c1);
-    if (ci == 2) {
-      c0 = c0 < ca0 ? ca0 : c0;
-      c1 = c1 < ca1 ? ca1 : c1;
-    }
-    R phia[NW], phib[NW];
-#pragma unroll
-    for (int il = 0; il < NW; ++il) {
-      phia[il] = R(fabs(fa.adt[il])) * fa.jump[il];
-      phib[il] = R(fabs(fb.adt[il])) * fb.jump[il];
-    }
-    // dvolume of cells ci-1, ci, ci+1 ((dx1*dx2)*dx3 in dimension order)
-    auto dvol = [&](int k) {
-      int c3[3] = {cl.c[0], cl.c[1], cl.c[2]};
-      c3[d] = k;
-      double v = g.dx[0][c3[0]];
-      v = v * g.dx[1][c3[1]];
-      if (ND == 3) v = v * g.dx[2][c3[2]];
-      return v;
+  if (x <= g.n[0] + 1 && s0 <= nd_ + 1) {
+    const int s1 = (s0 + seglen - 1 < nd_ + 1) ? s0 + seglen - 1 : nd_ + 1;
+    const long long sd = g.s[D];
+    const long long base = g.off + x + (ND == 3 ? (long long)t * g.s[other] : 0);
+    const double* dxd = g.dx[D];
+    const double dxt = (ND == 3) ? g.dx[other][t] : 1.0;
+    // volume of cell c along D: (dx1*dx2)*dx3 in dimension order
+    auto dvol = [&](int cd) {
+      double v = g.dx[0][x];
+      if (ND == 2) return v * dxd[cd];
+      if (D == 1) return (v * dxd[cd]) * dxt;
+      return (v * dxt) * dxd[cd];
     };
-    const double dm = dvol(ci - 1), d0 = dvol(ci), dp = dvol(ci + 1);
-    tvd_dissipate<ND, R>(fa.rt, phia, fb.rt, phib, d, s, R(d0 + dm), R(dp + d0), d0, out);
-    if (s.sourceunsplit && s.divbfix) {
-      const double dv = divb_at<ND>(g, w1, cl.idx, cl.c);
-      powell<ND>(u0, dv, dt * 1.0 / ND, out);
-    }
+    int c3[3] = {x, 0, 0};
+    if (ND == 3) c3[other] = t;
+    double a, b;
+    // window: w at J, J+1, w1 at J, slope at J; the face J-1/2 below
+    double wb[NW], wc[NW], u0[NW], lc[NW], Fp[NW];
+    TvdFace<ND, R> fp;
+    R phip[NW];
+    {
+      double wa[NW], u1[NW], ln[NW], wd[NW];
+      load<NW>(w, g.ps, base + (long long)(s0 - 2) * sd, wa);
+      load<NW>(w, g.ps, base + (long long)(s0 - 1) * sd, wb);
+      load<NW>(w, g.ps, base + (long long)s0 * sd, wc);
+      load<NW>(w1, g.ps, base + (long long)(s0 - 1) * sd, u0);
 #pragma unroll
-    for (int iw = 0; iw < NW; ++iw) wout[iw * g.ps + cl.idx] = out[iw];
+      for (int iw = 0; iw < NW; ++iw) lc[iw] = slope<R, LIM>(s.lim[iw], wa[iw], wb[iw], wc[iw]);
+      // face s0-1/2
+      load<NW>(w1, g.ps, base + (long long)s0 * sd, u1);
+      load<NW>(w, g.ps, base + (long long)(s0 + 1) * sd, wd);
+#pragma unroll
+      for (int iw = 0; iw < NW; ++iw) ln[iw] = slope<R, LIM>(s.lim[iw], wb[iw], wc[iw], wd[iw]);
+      mu_face<ND, R, D>(s, u0, u1, lc, ln, dt, smallp, dxd[s0 - 1] + dxd[s0], Fp, fp, phip, a, b);
+      c0 = c0 < a ? a : c0; c1 = c1 < b ? b : c1;
+#pragma unroll
+      for (int iw = 0; iw < NW; ++iw) {
+        u0[iw] = u1[iw];
+        lc[iw] = ln[iw];
+        wb[iw] = wc[iw];
+        wc[iw] = wd[iw];
+      }
+    }
+    // now: u0 = w1(i), lc = slope(i), wb = w(i), wc = w(i+1)
+    for (int i = s0; i <= s1; ++i) {
+      double u1[NW], ln[NW], wd[NW], F[NW];
+      TvdFace<ND, R> f;
+      R phi[NW];
+      load<NW>(w1, g.ps, base + (long long)(i + 1) * sd, u1);
+      load<NW>(w, g.ps, base + (long long)(i + 2) * sd, wd);
+#pragma unroll
+      for (int iw = 0; iw < NW; ++iw) ln[iw] = slope<R, LIM>(s.lim[iw], wb[iw], wc[iw], wd[iw]);
+      mu_face<ND, R, D>(s, u0, u1, lc, ln, dt, smallp, dxd[i] + dxd[i + 1], F, f, phi, a, b);
+      c0 = c0 < a ? a : c0; c1 = c1 < b ? b : c1;
+      const long long idx = base + (long long)i * sd;
+      const double qdx = pdiv(dt, dxd[i]);
+      double out[NW];
+#pragma unroll
+      for (int iw = 0; iw < NW; ++iw) out[iw] = wb[iw] - qdx * (F[iw] - Fp[iw]);
+      const double dm = dvol(i - 1), d0 = dvol(i), dp = dvol(i + 1);
+      tvd_dissipate<ND, R>(fp.rt, phip, f.rt, phi, D, s, R(d0 + dm), R(dp + d0), d0, out);
+      c3[D] = i;
+      if (s.sourceunsplit && s.divbfix) {
+        const double dv = divb_at<ND>(g, w1, idx, c3);
+        powell<ND>(u0, dv, dt * 1.0 / ND, out);
+      }
+      store_cell<ND, NW>(g, bc, wout, idx, c3, out, ghosts);
+#pragma unroll
+      for (int iw = 0; iw < NW; ++iw) {
+        Fp[iw] = F[iw];
+        u0[iw] = u1[iw];
+        lc[iw] = ln[iw];
+        wb[iw] = wc[iw];
+        wc[iw] = wd[iw];
+        phip[iw] = phi[iw];
+      }
+      fp = f;
+    }
   }
   const double b0 = block_reduce<true>(c0, sm);
   if (threadIdx.x == 0 && threadIdx.y == 0) part[SLOT_C0 * pstride + block_linear()] = b0;

@@ -406,26 +406,61 @@ int sweep_device(vacgpu_grid* g, const vacgpu_method* m, int d, int firstsweep, 
       if ((rc = launch_check(g, "k_tvdlf_pred"))) return rc;
     }
     if (!ghosts && (rc = bound(g, g->w1))) return rc;
+    long long nbc = 0;
     {
+      // face-parallel corrector: x sweep one face per lane, y / z sweeps marching
       Timed t(g, KF_CORR);
-#define MU_L(NDV, RT, DV) hipLaunchKernelGGL((k_tvdmu_corr<NDV, RT, DV>), g->grid, g->block, 0, g->stream, g->geo, s, win, g->w1, wout, g->sc, g->part, g->pstride)
       const bool f32 = g->cfg.precision == VACGPU_FP32BULK;
-      if (g->cfg.ndim == 2) {
-        if (d == 0) { if (f32) MU_L(2, float, 0); else MU_L(2, double, 0); }
-        else { if (f32) MU_L(2, float, 1); else MU_L(2, double, 1); }
+      const int lim = common_limiter(g, m);
+#define LIMSW(X) do { if (lim == LIM_MINMOD) X(LIM_MINMOD); else if (lim == LIM_WOODWARD) X(LIM_WOODWARD); else if (lim == LIM_SUPERBEE) X(LIM_SUPERBEE); else X(-1); } while (0)
+      if (d == 0) {
+        const dim3 gr((g->geo.n[0] + CX_STRIDE - 1) / CX_STRIDE, g->grid.y, g->grid.z);
+        nbc = (long long)gr.x * gr.y * gr.z;
+#define MUX(NDV, RT, LV) hipLaunchKernelGGL((k_tvdmu_x<NDV, RT, LV>), gr, g->block, 0, g->stream, g->geo, s, g->bc, win, g->w1, wout, g->sc, g->part, g->pstride, ghosts)
+#define X2D(LV) MUX(2, double, LV)
+#define X2F(LV) MUX(2, float, LV)
+#define X3D(LV) MUX(3, double, LV)
+#define X3F(LV) MUX(3, float, LV)
+        if (g->cfg.ndim == 2) { if (f32) LIMSW(X2F); else LIMSW(X2D); }
+        else { if (f32) LIMSW(X3F); else LIMSW(X3D); }
+#undef X2D
+#undef X2F
+#undef X3D
+#undef X3F
+#undef MUX
       } else {
-        if (d == 0) { if (f32) MU_L(3, float, 0); else MU_L(3, double, 0); }
-        else if (d == 1) { if (f32) MU_L(3, float, 1); else MU_L(3, double, 1); }
-        else { if (f32) MU_L(3, float, 2); else MU_L(3, double, 2); }
+        const dim3 gr = g->grid_tm[d];
+        nbc = (long long)gr.x * gr.y * gr.z;
+#define MUM(NDV, RT, DV, LV) hipLaunchKernelGGL((k_tvdmu_m<NDV, RT, DV, LV>), gr, g->block, 0, g->stream, g->geo, s, g->bc, win, g->w1, wout, g->sc, g->part, g->pstride, g->seglen, ghosts)
+#define M21D(LV) MUM(2, double, 1, LV)
+#define M21F(LV) MUM(2, float, 1, LV)
+#define M31D(LV) MUM(3, double, 1, LV)
+#define M31F(LV) MUM(3, float, 1, LV)
+#define M32D(LV) MUM(3, double, 2, LV)
+#define M32F(LV) MUM(3, float, 2, LV)
+        if (g->cfg.ndim == 2) { if (f32) LIMSW(M21F); else LIMSW(M21D); }
+        else if (d == 1) { if (f32) LIMSW(M31F); else LIMSW(M31D); }
+        else { if (f32) LIMSW(M32F); else LIMSW(M32D); }
+#undef M21D
+#undef M21F
+#undef M31D
+#undef M31F
+#undef M32D
+#undef M32F
+#undef MUM
       }
-#undef MU_L
+#undef LIMSW
       if ((rc = launch_check(g, "k_tvdmu_corr"))) return rc;
     }
     if (courant) {
       Timed t(g, KF_RED);
       hipLaunchKernelGGL(k_fin_courant_tvd, dim3(1), dim3(1024), 0, g->stream, g->sc, g->part,
-                         g->pstride, nb, d, m->courantpar);
+                         g->pstride, nbc, d, m->courantpar);
       if ((rc = launch_check(g, "k_fin_courant_tvd"))) return rc;
+    }
+    if (ghosts) {
+      g->cur = outi;
+      return 0;
     }
   } else if (m->typefull == VACGPU_TVDLF) {
     long long corr_blocks = nb;
