@@ -1,6 +1,7 @@
 """The reference executables oracle/build_ref.sh makes (test infrastructure).
 
 BINARIES: plain VAC 4.52 builds (the CPU oracle and baseline).
+POISSON:  plain builds with the Poisson module switched on (projection of B).
 HOOKED:   the same VAC host with the GPU drop-in linked in place of advect.
 Each entry: name -> (setvac -d value, physics, compile-time grid ixGhi).
 """
@@ -9,6 +10,11 @@ BINARIES = {
     "vac_hd22_132": ("22", "hd", "132,132"),
     "vac_mhd33_36": ("33", "mhd", "36,36,36"),
     "vac_mhd22_2052": ("22", "mhd", "2052,2052"),
+}
+# VAC with the Poisson module on (setvac -on=poisson): process() projects B
+POISSON = {
+    "vacp_mhd22_132": ("22", "mhd", "132,132"),
+    "vacp_mhd33_36": ("33", "mhd", "36,36,36"),
 }
 HOOKED = {
     "vacgpu_mhd22_132": ("22", "mhd", "132,132"),

@@ -44,6 +44,9 @@ typedef struct {
   /* Roe common block (src/vacphys.mhdroe.t:17-18) and saved arrays */
   double *cfast, *cslow, *afast, *aslow, *csound2, *dp, *rhodv;
   double *bdv, *bdb, *bv, *v2a2;
+  /* projectb / Poisson work planes: divb (the residual), phi, CG's Z and A.Z,
+   * gradients */
+  double *pr_rhs, *pr_phi, *pr_z, *pr_az, *pr_qdx, *pr_qddx;
 } ctx_t;
 
 /* variable indices (0-based), src/vacpar.t.mhd:9-11, src/vacpar.t.hd:9-10 */
@@ -101,7 +104,7 @@ double vacref_limit(int type, double a, double b) {
 /* ------------------------------------------------------------ boundaries */
 /* getboundary for 'periodic' and 'cont' (src/vacgrid.t:195-413): regions
  * x-low, x-high, y-low, ... each spanning the full transverse ixG range. */
-static void getboundary(ctx_t* x, double* w) {
+static void getboundary_vars(ctx_t* x, double* w, int iw0, int iw1) {
   for (int d = 0; d < x->ndim; ++d) {
     for (int up = 0; up < 2; ++up) {
       int iB = 2 * d + up;
@@ -110,7 +113,7 @@ static void getboundary(ctx_t* x, double* w) {
       if (up) { lo[d] = x->mhi[d] + 1; hi[d] = x->ghi[d]; }
       else { lo[d] = x->glo[d]; hi[d] = x->mlo[d] - 1; }
       int n = x->p->nx[d];
-      for (int iw = 0; iw < x->nw; ++iw) {
+      for (int iw = iw0; iw < iw1; ++iw) {
         double* pw = PL(x, w, iw);
         int bc = x->p->bc[iB][iw];
         LOOP(lo, hi) {
@@ -125,6 +128,8 @@ static void getboundary(ctx_t* x, double* w) {
     }
   }
 }
+
+static void getboundary(ctx_t* x, double* w) { getboundary_vars(x, w, 0, x->nw); }
 
 /* ------------------------------------------------------------- physics */
 /* getpthermal (MHD src/vacphys0.t.mhd:165-222, HD src/vacphys0.t.hd:103-134) */
@@ -860,6 +865,222 @@ static void tvdlimit(ctx_t* x, double qdt, int d, double* wCT, double* wnew) {
   }
 }
 
+/* ------------------------------------------------------ divB projection */
+enum { STOP_MAX = 0, STOP_REL = 1 };
+
+/* the region ixM^L^LADD1: the mesh widened by one cell in every dimension */
+static void widen1(const ctx_t* x, int* lo, int* hi) {
+  for (int d = 0; d < 3; ++d) {
+    lo[d] = x->mlo[d]; hi[d] = x->mhi[d];
+    if (d < x->ndim) { lo[d] -= 1; hi[d] += 1; }
+  }
+}
+
+/* gradient (src/vacgrid.t:775-835), slab branch :815-821:
+ * gradq = half*(q(i+1) - q(i-1))/dx(i) along d over [lo, hi] */
+static void gradient(ctx_t* x, const double* q, const int* lo, const int* hi, int d, double* g) {
+  long s = x->st[d];
+  LOOP(lo, hi) {
+    long idx = IDX(x);
+    g[idx] = 0.5 * (q[idx + s] - q[idx - s]) / x->dx[d][CRD(d)];
+  }
+}
+
+/* getdivb (src/vacphys.mhd0.t:106-146), second order, over ixM */
+static void getdivb(ctx_t* x, const double* w, double* divb) {
+  const int* lo = x->mlo;
+  const int* hi = x->mhi;
+  LOOP(lo, hi) divb[IDX(x)] = 0.0;
+  for (int d = 0; d < x->ndim; ++d) {
+    gradient(x, PL(x, w, B(x, d)), lo, hi, d, x->pr_qdx);
+    LOOP(lo, hi) { long idx = IDX(x); divb[idx] = divb[idx] + x->pr_qdx[idx]; }
+  }
+}
+
+/* typeBscalar of face iB from the type of the normal B component there
+ * (src/vacproc.projectb.t:82-101): 'periodic' stays 'periodic'; 'cont' (the
+ * other type on this path) falls to the default case, 'nul'. */
+static int bscalar_periodic(const ctx_t* x, int iB) {
+  return x->p->bc[iB][B(x, iB / 2)] == VACREF_PERIODIC;
+}
+
+/* boundscalar for 'periodic' and 'nul' (src/vacpoisson.t:158-300), the
+ * faces in VAC's order, each region spanning the full transverse ixG range */
+static void boundscalar(ctx_t* x, double* q) {
+  for (int d = 0; d < x->ndim; ++d) {
+    for (int up = 0; up < 2; ++up) {
+      int iB = 2 * d + up;
+      int lo[3], hi[3];
+      for (int dd = 0; dd < 3; ++dd) { lo[dd] = x->glo[dd]; hi[dd] = x->ghi[dd]; }
+      if (up) { lo[d] = x->mhi[d] + 1; hi[d] = x->ghi[d]; }
+      else { lo[d] = x->glo[d]; hi[d] = x->mlo[d] - 1; }
+      int n = x->p->nx[d];
+      int per = bscalar_periodic(x, iB);
+      LOOP(lo, hi) {
+        long idx = IDX(x);
+        if (per) {
+          int c = CRD(d);
+          int src = up ? c - n : c + n;  /* the pair face shifted by dixB, :183-194 */
+          q[idx] = q[idx + (long)(src - c) * x->st[d]];
+        } else {
+          q[idx] = 0.0;                  /* 'nul', :286-287 */
+        }
+      }
+    }
+  }
+}
+
+/* matvec_poisson (src/vacpoisson.t:100-155), second order:
+ * qy = sum_d d/dx_d (d qx/dx_d), the gradient over ixM+1, its divergence over ixM */
+static void matvec_poisson(ctx_t* x, double* qx, double* qy) {
+  boundscalar(x, qx);
+  const int* lo = x->mlo;
+  const int* hi = x->mhi;
+  int lo1[3], hi1[3];
+  widen1(x, lo1, hi1);
+  LOOP(lo, hi) qy[IDX(x)] = 0.0;
+  for (int d = 0; d < x->ndim; ++d) {
+    gradient(x, qx, lo1, hi1, d, x->pr_qdx);
+    gradient(x, x->pr_qdx, lo, hi, d, x->pr_qddx);
+    LOOP(lo, hi) { long idx = IDX(x); qy[idx] = qy[idx] + x->pr_qddx[idx]; }
+  }
+}
+
+/* sum(q(ixM)**2), maxval(abs(q(ixM))), sum(a(ixM)*b(ixM)): Fortran array
+ * reductions in array element order */
+static double sum_sq(ctx_t* x, const double* q) {
+  const int* lo = x->mlo;
+  const int* hi = x->mhi;
+  double r = 0.0;
+  LOOP(lo, hi) { long idx = IDX(x); r = r + q[idx] * q[idx]; }
+  return r;
+}
+static double max_abs(ctx_t* x, const double* q) {
+  const int* lo = x->mlo;
+  const int* hi = x->mhi;
+  double r = -INFINITY;
+  LOOP(lo, hi) r = dmax(r, fabs(q[IDX(x)]));
+  return r;
+}
+static double sum_prod(ctx_t* x, const double* a, const double* b) {
+  const int* lo = x->mlo;
+  const int* hi = x->mhi;
+  double r = 0.0;
+  LOOP(lo, hi) { long idx = IDX(x); r = r + a[idx] * b[idx]; }
+  return r;
+}
+
+/* cgscalar (src/vacpoisson.t:391-595) with a zero initial guess
+ * (nonzero=.false.), typestop 'max' or 'rel'.  rhs: right-hand side in,
+ * residual out; qx: solution; iter: matvec limit in, iterations out; tol:
+ * tolerance in, achieved residual out. */
+static void cgscalar(ctx_t* x, double* rhs, double* qx, int* iter, double* tol, int typestop,
+                     int* info) {
+  const int* lo = x->mlo;
+  const int* hi = x->mhi;
+  double* z = x->pr_z;
+  double* az = x->pr_az;
+  double assumedzero = 1.0e-16;
+  int itr = 0;
+  double rho = sqrt(sum_sq(x, rhs));
+  double res0 = rho;
+  if (typestop == STOP_MAX) res0 = max_abs(x, rhs);
+  double res = res0;
+  assumedzero = assumedzero * res0;
+  if (res0 < SMALLD || (typestop != STOP_REL && res0 <= *tol)) {
+    *info = 3;
+  } else {
+    rho = rho * rho;
+    LOOP(lo, hi) { long idx = IDX(x); z[idx] = rhs[idx]; }
+    for (;;) {
+      matvec_poisson(x, z, az);
+      double alf = sum_prod(x, z, az);
+      if (fabs(alf) <= assumedzero * assumedzero) { *info = 1; break; }
+      alf = rho / alf;
+      LOOP(lo, hi) { long idx = IDX(x); qx[idx] = qx[idx] + alf * z[idx]; }
+      LOOP(lo, hi) { long idx = IDX(x); rhs[idx] = rhs[idx] - alf * az[idx]; }
+      double rhonew = sqrt(sum_sq(x, rhs));
+      if (typestop == STOP_MAX) res = max_abs(x, rhs);
+      else res = rhonew / res0;
+      rhonew = rhonew * rhonew;
+      itr = itr + 1;
+      if (res <= *tol) { *info = 0; break; }
+      if (itr >= *iter) { *info = 2; break; }
+      if (rho <= assumedzero * assumedzero) { *info = 1; break; }
+      double bet = rhonew / rho;
+      LOOP(lo, hi) { long idx = IDX(x); z[idx] = rhs[idx] + bet * z[idx]; }
+      rho = rhonew;
+    }
+  }
+  *iter = itr;
+  *tol = res;
+  if ((typestop == STOP_REL && res > 1.0) || (typestop != STOP_REL && res > res0)) *info = -*info;
+}
+
+/* projectb (src/vacproc.projectb.t:4-157) with the Poisson driver's CG
+ * branch (src/vacpoisson.t:6-97; slab grids select 'cg').  Returns -4 where
+ * VAC dies (a zero allowed maximum of div B). */
+static int projectb(ctx_t* x, double* w) {
+  const vacref_params* p = x->p;
+  vacref_state* s = x->s;
+  double* divb = x->pr_rhs;
+  double* phi = x->pr_phi;
+  const int* lo = x->mlo;
+  const int* hi = x->mhi;
+  getdivb(x, w, divb);
+  if (s->divbmax < SMALLD) {
+    /* the first call fixes divbmax, typestop and matvecmax, :29-58 */
+    s->divbmax = max_abs(x, divb);
+    double coeff = p->procpar[0] == -1.0 ? 1.0 / 10 : p->procpar[0];
+    double cnst = p->procpar[1] == -1.0 ? 0.0 : p->procpar[1];
+    if (coeff < 0.0 && coeff > -1.0) {
+      s->divbmax = -coeff;
+      s->typestop = STOP_REL;
+    } else {
+      s->divbmax = s->divbmax * dmax(0.0, coeff) + dmax(0.0, cnst);
+      s->typestop = STOP_MAX;
+      if (s->divbmax < SMALLD) return -4;
+    }
+    s->matvecmax = 1000;
+    if (cnst < -1.0) s->matvecmax = (int)lround(-cnst);
+    if (coeff < -1.0) s->matvecmax = (int)lround(-coeff);
+  }
+  LOOP(lo, hi) phi[IDX(x)] = 0.0;
+  int iter = s->matvecmax, info = 0;
+  double resid = s->divbmax;
+  cgscalar(x, divb, phi, &iter, &resid, s->typestop, &info);
+  if (info != 0 && info != 3) s->nerror[3] += 1;  /* nerror(poissonerr_), :79-81 */
+  s->proj_info = info;
+  s->proj_iter = iter;
+  s->proj_resid = resid;
+  s->nproj += 1;
+  if (info == 3 || info < 0) return 0;
+  boundscalar(x, phi);
+  int lo1[3], hi1[3];
+  widen1(x, lo1, hi1);
+  double* g = x->pr_qdx;
+  const int bnd = (int)lround(fabs(p->procpar[2]));
+  const double gam = p->gamma;
+  for (int d = 0; d < x->ndim; ++d) {
+    gradient(x, phi, lo1, hi1, d, g);
+    double* pe = PL(x, w, E_(x));
+    double* pb = PL(x, w, B(x, d));
+    if (bnd == 2) {
+      /* keep the thermal pressure, :129-131 */
+      LOOP(lo1, hi1) { long idx = IDX(x); pe[idx] = pe[idx] - pb[idx] * g[idx] + g[idx] * g[idx] / 2; }
+    } else if (bnd == 3 && gam != 2.0 && gam != 1.0) {
+      /* keep the total pressure, :132-135 */
+      LOOP(lo1, hi1) {
+        long idx = IDX(x);
+        pe[idx] = pe[idx] + (gam - 2.0) / (gam - 1.0) * (-(pb[idx] * g[idx]) + g[idx] * g[idx] / 2);
+      }
+    }
+    LOOP(lo1, hi1) { long idx = IDX(x); pb[idx] = pb[idx] - g[idx]; }
+  }
+  if (p->procpar[2] > 0.0) getboundary_vars(x, w, B(x, 0), B(x, 0) + x->ndim);
+  return 0;
+}
+
 /* -------------------------------------------------------- time stepping */
 /* advect1 (src/vac.t:489-572) */
 static void advect1(ctx_t* x, int method, double qdt, int d, double* wCT, double* w) {
@@ -874,7 +1095,16 @@ static void advect1(ctx_t* x, int method, double qdt, int d, double* wCT, double
 }
 
 /* advect (src/vac.t:363-486) */
-static void advect(ctx_t* x, int d, double* w) {
+static int advect(ctx_t* x, int d, double* w) {
+  /* process(1) when due, :392-399 */
+  const int np = x->p->nproc1;
+  if (np != 0) {
+    const int di = x->s->it - x->s->itmin;
+    if (np < 0 || (x->firstsweep && di == (di / np) * np)) {
+      int rc = projectb(x, w);
+      if (rc) return rc;
+    }
+  }
   memcpy(x->w1, w, sizeof(double) * x->nc * x->nw);
   x->istep = 0;
   double dt = x->s->dt;
@@ -885,17 +1115,19 @@ static void advect(ctx_t* x, int d, double* w) {
     advect1(x, 2, dt, d, x->w1, w);
   }
   x->firstsweep = 0;
+  return 0;
 }
 
 /* advance_expl with dimsplit (src/vac.t:283-360) */
-static void advance(ctx_t* x, double* w) {
+static int advance(ctx_t* x, double* w) {
   x->firstsweep = 1;
-  int it = x->s->it;
+  int it = x->s->it, rc = 0;
   if ((it / 2) * 2 == it || x->p->dimsplit_xy) {
-    for (int d = 0; d < x->ndim; ++d) { x->lastsweep = (d == x->ndim - 1); advect(x, d, w); }
+    for (int d = 0; d < x->ndim && !rc; ++d) { x->lastsweep = (d == x->ndim - 1); rc = advect(x, d, w); }
   } else {
-    for (int d = x->ndim - 1; d >= 0; --d) { x->lastsweep = (d == 0); advect(x, d, w); }
+    for (int d = x->ndim - 1; d >= 0 && !rc; --d) { x->lastsweep = (d == 0); rc = advect(x, d, w); }
   }
+  return rc;
 }
 
 /* getdt_courant (src/vac.t:664-722) */
@@ -960,7 +1192,8 @@ static int ctx_init(ctx_t* x, const vacref_params* p, vacref_state* s,
   double** one[] = {&x->f, &x->fC, &x->sc1, &x->sc2, &x->sc3, &x->sc4, &x->sc5,
                     &x->jump, &x->adtdx, &x->phi, &x->rphi, &x->ljump,
                     &x->cfast, &x->cslow, &x->afast, &x->aslow, &x->csound2, &x->dp,
-                    &x->rhodv, &x->bdv, &x->bdb, &x->bv, &x->v2a2};
+                    &x->rhodv, &x->bdv, &x->bdb, &x->bv, &x->v2a2,
+                    &x->pr_rhs, &x->pr_phi, &x->pr_z, &x->pr_az, &x->pr_qdx, &x->pr_qddx};
   for (size_t q = 0; q < sizeof(one) / sizeof(one[0]); ++q) *one[q] = calloc(1, pl);
   return 0;
 }
@@ -969,8 +1202,32 @@ static void ctx_free(ctx_t* x) {
   double* all[] = {x->w1, x->wold, x->wLC, x->wRC, x->wroe, x->tmpw, x->f, x->fC,
                    x->sc1, x->sc2, x->sc3, x->sc4, x->sc5, x->jump, x->adtdx, x->phi,
                    x->rphi, x->ljump, x->cfast, x->cslow, x->afast, x->aslow,
-                   x->csound2, x->dp, x->rhodv, x->bdv, x->bdb, x->bv, x->v2a2};
+                   x->csound2, x->dp, x->rhodv, x->bdv, x->bdb, x->bv, x->v2a2,
+                   x->pr_rhs, x->pr_phi, x->pr_z, x->pr_az, x->pr_qdx, x->pr_qddx};
   for (size_t q = 0; q < sizeof(all) / sizeof(all[0]); ++q) free(all[q]);
+}
+
+int vacref_getboundary(const vacref_params* p, double* w) {
+  ctx_t x;
+  vacref_state s;
+  memset(&s, 0, sizeof(s));
+  double one = 1.0;
+  int rc = ctx_init(&x, p, &s, &one, &one, &one);
+  if (rc) return rc;
+  getboundary(&x, w);
+  ctx_free(&x);
+  return 0;
+}
+
+int vacref_projectb(const vacref_params* p, vacref_state* s, double* w,
+                    const double* dx0, const double* dx1, const double* dx2) {
+  ctx_t x;
+  int rc = ctx_init(&x, p, s, dx0, dx1, dx2);
+  if (rc) return rc;
+  if (!x.mhd || x.ndim < 2) { ctx_free(&x); return -5; }
+  rc = projectb(&x, w);
+  ctx_free(&x);
+  return rc;
 }
 
 int vacref_getdt(const vacref_params* p, vacref_state* s, const double* w,
@@ -991,6 +1248,7 @@ int vacref_run(const vacref_params* p, vacref_state* s, double* w,
   int rc = ctx_init(&x, p, s, dx0, dx1, dx2);
   if (rc) return rc;
   if (p->dtpar > 0.0 && s->smallp < 0.0) { ctx_free(&x); return -3; }
+  if (p->nproc1 != 0 && (!x.mhd || x.ndim < 2)) { ctx_free(&x); return -5; }  /* ^NOONED, MHD */
   getboundary(&x, w);
   int done = 0;
   for (;;) {
@@ -1003,7 +1261,7 @@ int vacref_run(const vacref_params* p, vacref_state* s, double* w,
     if (s->it >= p->itmax || s->t >= p->tmax || s->dt < 1.0e-99 || done >= nsteps) break;
     if (p->tmaxexact) s->dt = dmin(s->dt, p->tmax - s->t + SMALLD);
     memcpy(x.wold, w, sizeof(double) * x.nc * x.nw);
-    advance(&x, w);
+    if ((rc = advance(&x, w))) { ctx_free(&x); return rc; }
     s->it += 1;
     s->t = s->t + s->dt;
     done += 1;

@@ -36,6 +36,12 @@ typedef struct {
   double tmax;
   int tmaxexact;
   int itmax;
+  /* processing before the step (process(1), src/vac.t:392-399): nproc(1);
+   * 0 = none, n > 0 = every n-th step on the first sweep, n < 0 = every
+   * sweep.  The only processing of an MHD grid is the projection of B
+   * (projectb, src/vacproc.projectb.t) with CG (src/vacpoisson.t:391-595). */
+  int nproc1;
+  double procpar[3];     /* procpar(divbcoeff_, divbconst_, divbbound_); -1 = unset */
 } vacref_params;
 
 typedef struct {
@@ -44,6 +50,14 @@ typedef struct {
   double smallp;         /* < 0: set on the first pressure evaluation */
   double dtcourant[3];
   int nerror[4];
+  /* projectb's saved variables (src/vacproc.projectb.t:16-17): divbmax = 0
+   * until the first call sets it, with matvecmax and typestop (0 'max',
+   * 1 'rel'); and what the last Poisson solve returned */
+  double divbmax;
+  int matvecmax, typestop;
+  int proj_info, proj_iter;
+  double proj_resid;
+  int nproj;             /* projections done */
 } vacref_state;
 
 /* Main loop of src/vac.t:55-106 for up to nsteps steps (and the itmax/tmax
@@ -54,6 +68,15 @@ typedef struct {
 int vacref_run(const vacref_params* p, vacref_state* s, double* w,
                const double* dx0, const double* dx1, const double* dx2,
                int nsteps, double* t_hist);
+
+/* getboundary (src/vacgrid.t:195-413) of every variable of w. */
+int vacref_getboundary(const vacref_params* p, double* w);
+
+/* One projectb call (src/vacproc.projectb.t:4-157) on the full ixG state w,
+ * with p->procpar and the saved variables in s.  Returns 0, or < 0 on bad
+ * input (HD, ndim < 2). */
+int vacref_projectb(const vacref_params* p, vacref_state* s, double* w,
+                    const double* dx0, const double* dx1, const double* dx2);
 
 /* One pass of VAC's getdt_courant (src/vac.t:664-722): sets s->dt. */
 int vacref_getdt(const vacref_params* p, vacref_state* s, const double* w,

@@ -29,7 +29,7 @@ class Params(C.Structure):
         ("sourceunsplit", C.c_int), ("dimsplit_xy", C.c_int),
         ("courantpar", C.c_double), ("smallpcoeff", C.c_double),
         ("dtpar", C.c_double), ("tmax", C.c_double), ("tmaxexact", C.c_int),
-        ("itmax", C.c_int),
+        ("itmax", C.c_int), ("nproc1", C.c_int), ("procpar", C.c_double * 3),
     ]
 
 
@@ -37,7 +37,9 @@ class State(C.Structure):
     _fields_ = [
         ("it", C.c_int), ("itmin", C.c_int), ("t", C.c_double), ("dt", C.c_double),
         ("smallp", C.c_double), ("dtcourant", C.c_double * 3),
-        ("nerror", C.c_int * 4),
+        ("nerror", C.c_int * 4), ("divbmax", C.c_double), ("matvecmax", C.c_int),
+        ("typestop", C.c_int), ("proj_info", C.c_int), ("proj_iter", C.c_int),
+        ("proj_resid", C.c_double), ("nproj", C.c_int),
     ]
 
 
@@ -58,6 +60,10 @@ def lib():
         L.vacref_run.argtypes = [C.POINTER(Params), C.POINTER(State), dp, dp, dp, dp,
                                  C.c_int, dp]
         L.vacref_run.restype = C.c_int
+        L.vacref_getboundary.argtypes = [C.POINTER(Params), dp]
+        L.vacref_getboundary.restype = C.c_int
+        L.vacref_projectb.argtypes = [C.POINTER(Params), C.POINTER(State), dp, dp, dp, dp]
+        L.vacref_projectb.restype = C.c_int
         L.vacref_getdt.argtypes = [C.POINTER(Params), C.POINTER(State), dp, dp, dp, dp]
         L.vacref_getdt.restype = C.c_int
         L.vacref_limit.argtypes = [C.c_int, C.c_double, C.c_double]
@@ -76,7 +82,8 @@ def _ptr(a):
 def make_params(ndim, physics, nx, gamma, typefull="tvdlf", limiter="minmod",
                 bc="periodic", divbfix=True, divbwave=True, sourceunsplit=True,
                 dimsplit_xy=False, courantpar=0.8, smallpcoeff=1e-6, dtpar=-1.0,
-                tmax=1e99, tmaxexact=False, itmax=1000000) -> Params:
+                tmax=1e99, tmaxexact=False, itmax=1000000, nproc1=0,
+                procpar=(-1.0, -1.0, -1.0)) -> Params:
     p = Params()
     p.ndim = ndim
     p.phys = MHD if physics == "mhd" else HD
@@ -105,6 +112,9 @@ def make_params(ndim, physics, nx, gamma, typefull="tvdlf", limiter="minmod",
     p.tmax = tmax
     p.tmaxexact = int(tmaxexact)
     p.itmax = itmax
+    p.nproc1 = int(nproc1)
+    for i in range(3):
+        p.procpar[i] = float(procpar[i])
     return p
 
 
@@ -143,6 +153,24 @@ def run(params: Params, w_mesh: np.ndarray, dxs: list, nsteps: int, it0=0, t0=0.
     if n < 0:
         raise RuntimeError(f"vacref_run failed ({n})")
     return mesh_of(w), hist[: n + 1], st
+
+
+def projectb(params: Params, w_mesh: np.ndarray, dxs: list, state: State | None = None):
+    """One projectb call on a mesh state (ghosts filled as getboundary does);
+    returns (w_full_out, state) with the whole ixG array."""
+    L = lib()
+    w = pad_state(np.asarray(w_mesh, dtype=np.float64))
+    st = state if state is not None else State()
+    dx = [np.ascontiguousarray(d_, dtype=np.float64) for d_ in dxs]
+    while len(dx) < 3:
+        dx.append(np.ones(1))
+    # fill the ghost cells first, as the state of a running VAC has them
+    L.vacref_getboundary(C.byref(params), _ptr(w))
+    rc = L.vacref_projectb(C.byref(params), C.byref(st), _ptr(w), _ptr(dx[0]), _ptr(dx[1]),
+                           _ptr(dx[2]))
+    if rc:
+        raise RuntimeError(f"vacref_projectb failed ({rc})")
+    return w, st
 
 
 def limit(kind: str, a: float, b: float) -> float:

@@ -29,7 +29,7 @@ sys.path.insert(0, os.path.join(ROOT, "oracle"))
 from vac_amd.initial import orszag_tang, random_fourier  # noqa: E402
 from run_ref import have_ref, make_ini, problem_par, run_ref  # noqa: E402
 
-from refbuilds import BINARIES  # noqa: E402
+from refbuilds import BINARIES, POISSON  # noqa: E402
 
 NAMES = {6: "rho m1 m2 e b1 b2", 4: "rho m1 m2 e", 8: "rho m1 m2 m3 e b1 b2 b3"}
 
@@ -50,18 +50,46 @@ CASES = {
 }
 
 
+# projection of B before every step (nproc=1, procpar): the reference's own
+# Orszag-Tang recipe (TVD + woodward, divbfix=F, divbwave=F, procpar=0,0.01,1),
+# a relative stopping rule with the thermal-pressure energy fix, cont faces
+# (the Poisson boundary is then 'nul') with the total-pressure fix, a matvec
+# limit that makes every solve stop early (nerror(poissonerr_)), and 3D TVD
+# with the Powell/Ryu source on (its wold is the state before projection).
+# name: (binary, physics, builder, gamma, steps, typefull, limiter, bc, divbfix,
+#        divbwave, procpar)
+PROJ_CASES = {
+    "mhd22_ot32_tvd_woodward_proj": ("vacp_mhd22_132", "mhd", lambda g: orszag_tang(32, g), 5 / 3, 20, "tvd", "woodward", "periodic", False, False, (0.0, 0.01, 1.0)),
+    "mhd22_rf28x32_tvdlf_minmod_projrel": ("vacp_mhd22_132", "mhd", lambda g: random_fourier((28, 32), 112, g), 5 / 3, 12, "tvdlf", "minmod", "periodic", True, True, (-0.001, -1.0, 2.0)),
+    "mhd22_rf24x30_tvdlf_woodward_cont_proj3": ("vacp_mhd22_132", "mhd", lambda g: random_fourier((24, 30), 113, g), 1.4, 12, "tvdlf", "woodward", "cont", True, True, (0.01, 0.0, 3.0)),
+    "mhd22_rf30x26_tvdmu_superbee_projmax3": ("vacp_mhd22_132", "mhd", lambda g: random_fourier((30, 26), 114, g), 5 / 3, 10, "tvdmu", "superbee", "periodic", True, True, (-3.0, 1e-9, 1.0)),
+    "mhd33_rf10x12x8_tvd_woodward_proj": ("vacp_mhd33_36", "mhd", lambda g: random_fourier((10, 12, 8), 115, g), 5 / 3, 6, "tvd", "woodward", "periodic", True, True, (0.0, 1e-5, 1.0)),
+}
+
+
 def ensure_binary(name: str) -> None:
     if have_ref(name):
         return
-    d, phys, g = BINARIES[name]
-    subprocess.run([os.path.join(ROOT, "oracle", "build_ref.sh"), name, d, phys, g], check=True)
+    if name in POISSON:
+        d, phys, g = POISSON[name]
+        extra = ["cpu", "poisson"]
+    else:
+        d, phys, g = BINARIES[name]
+        extra = []
+    subprocess.run([os.path.join(ROOT, "oracle", "build_ref.sh"), name, d, phys, g, *extra],
+                   check=True)
 
 
-def run_case(binary, physics, w, coords, gamma, steps, typefull, limiter, bc, divbfix):
+def run_case(binary, physics, w, coords, gamma, steps, typefull, limiter, bc, divbfix,
+             divbwave=True, procpar=None):
     nw = w.shape[0]
     ini = make_ini(w, coords, physics, gamma, name="golden")
+    extra = ""
+    if procpar is not None:
+        extra = " nproc=1\n procpar=" + ",".join(repr(float(v)) for v in procpar) + "\n"
     par = problem_par(nw, NAMES[nw], steps, typefull=typefull, limiter=limiter,
-                      typeB=[bc] * (2 * len(coords)), divbfix=divbfix, every_step=True)
+                      typeB=[bc] * (2 * len(coords)), divbfix=divbfix, divbwave=divbwave,
+                      every_step=True, extra_method=extra)
     r = run_ref(binary, ini, par)
     t = np.array([s.t for s in r.snapshots])
     return r.snapshots[-1].w, t
@@ -85,6 +113,20 @@ def main() -> None:
             nx=np.array([len(c) for c in coords]), gamma=gamma, steps=steps,
             physics=physics, typefull=typefull, limiter=limiter, bc=bc,
             divbfix=divbfix)
+        print("wrote", name, w0.shape, "steps", len(t) - 1)
+    for name, (binary, physics, build, gamma, steps, typefull, limiter, bc, divbfix, divbwave,
+               procpar) in PROJ_CASES.items():
+        if only and not any(o in name for o in only):
+            continue
+        ensure_binary(binary)
+        w0, coords = build(gamma)
+        wf, t = run_case(binary, physics, w0, coords, gamma, steps, typefull, limiter, bc,
+                         divbfix, divbwave, procpar)
+        np.savez_compressed(
+            os.path.join(HERE, name + ".npz"), w0=w0, wf=wf, t=t,
+            nx=np.array([len(c) for c in coords]), gamma=gamma, steps=steps,
+            physics=physics, typefull=typefull, limiter=limiter, bc=bc,
+            divbfix=divbfix, divbwave=divbwave, nproc1=1, procpar=np.array(procpar))
         print("wrote", name, w0.shape, "steps", len(t) - 1)
     if only:
         return

@@ -38,9 +38,13 @@ def run_fixture(fx):
     gamma = float(fx["gamma"])
     from vac_amd.grid import cell_centres
     dxs = [widths_1d(cell_centres(n)) for n in nx]
+    kw = {}
+    if "nproc1" in fx:  # projection of B before every step
+        kw = dict(divbwave=bool(fx["divbwave"]), nproc1=int(fx["nproc1"]),
+                  procpar=[float(v) for v in fx["procpar"]])
     p = vacref.make_params(ndim, str(fx["physics"]), nx, gamma,
                            typefull=str(fx["typefull"]), limiter=str(fx["limiter"]),
-                           bc=str(fx["bc"]), divbfix=bool(fx["divbfix"]))
+                           bc=str(fx["bc"]), divbfix=bool(fx["divbfix"]), **kw)
     return vacref.run(p, fx["w0"], dxs, int(fx["steps"]))
 
 
