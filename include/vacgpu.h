@@ -114,7 +114,8 @@ int vacgpu_get_scalars(vacgpu_grid* g, double* dtcourant, int* nerror, double* s
 int vacgpu_profile(vacgpu_grid* g, int enable);
 /* Average duration (ms) and launch count of kernel family k since enabling:
  * k = 0 TVD sweep, 1 TVDLF predictor, 2 TVDLF corrector, 3 Ryu source,
- * 4 ghost cells, 5 reductions, 6 fused TVDLF sweep. */
+ * 4 ghost cells, 5 reductions, 6 fused TVDLF sweep, 7 projection of B (a
+ * whole projectb call: its kernels and the host's polls of the CG state). */
 int vacgpu_kernel_time(vacgpu_grid* g, int k, double* avg_ms, long* count);
 
 /* Kernel choice for 2D TVDLF sweeps (no counterpart in VAC; results are
@@ -153,6 +154,36 @@ int vacgpu_snapshot_fetch(vacgpu_grid* g, int slot, double* w, const int* ld, in
  * of nx[0] * nx[1] (* nx[2]) doubles, x fastest -- the layout of the w
  * records of VAC's files. */
 int vacgpu_snapshot_fetch_mesh(vacgpu_grid* g, int slot, double* w, int* it, double* t);
+
+/* Processing of the state: VAC's process() for MHD (src/vacphys.mhd0.t:35-80),
+ * which projects B (projectb, src/vacproc.projectb.t:4-157: div B by centred
+ * differences, Laplace(phi) = div B solved by the CG scheme of
+ * src/vacpoisson.t:391-595 from phi = 0, then B = B - grad(phi) over the mesh
+ * and its first ghost layer, with the energy fixes of |procpar(3)| = 2 / 3 and
+ * B's ghost cells refilled when procpar(3) > 0).  The Poisson boundary of a
+ * face follows the normal B component there: periodic stays periodic, cont
+ * gives phi = 0 ghosts ('nul').  The allowed maximum of div B, the stopping
+ * rule ('max' or 'rel') and the matvec limit are fixed on the first call from
+ * procpar and the div B found then, and kept for later calls, as projectb's
+ * saved variables are.  A solve that stops without meeting its tolerance
+ * counts nerror(4) (poissonerr_) as VAC's poisson() does.
+ *
+ * vacgpu_set_process replaces the nproc(1)/procpar/itmin part of readparameters
+ * (src/vacio.t:88-97): vacgpu_run then projects where VAC's advect calls
+ * process(1) (src/vac.t:392-399) -- before the first sweep of every nproc1-th
+ * step counted from itmin, or before every sweep when nproc1 < 0.  procpar may
+ * be NULL (all -1: VAC's defaults 0.1, 0, -1).  It resets the saved values.
+ *
+ * vacgpu_process replaces one process(count, ...) call (src/vac.t:392-399 for
+ * count 1; src/vacio.t:910, count ifile+2, before a save).  With count 1 at a
+ * step boundary the state before the projection is kept as the next step's
+ * wold (VAC stores wold before advect projects).  procpar NULL: the values of
+ * vacgpu_set_process.  info / iter / resid (optional) receive what the solve
+ * returned (VAC's info codes; info 3 or < 0 leaves B unchanged).  Count 2
+ * (processing between a two-step method's halves) is not on the GPU path. */
+int vacgpu_set_process(vacgpu_grid* g, int nproc1, const double* procpar, int itmin);
+int vacgpu_process(vacgpu_grid* g, int count, const double* procpar, int* info, int* iter,
+                   double* resid);
 
 /* Start-up check of a selector set: 0 if supported, else VACGPU_E_ARG. */
 int vacgpu_supported(const vacgpu_config* cfg, const vacgpu_method* m);

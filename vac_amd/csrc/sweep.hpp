@@ -18,7 +18,8 @@ namespace {
 constexpr int BX = 64, BY = 4;      // cell kernels: 64 lanes along x, 4 rows
 constexpr int NSLOT = 8;            // partial-reduction slots
 enum { SLOT_C0 = 0, SLOT_C1 = 1, SLOT_PMIN = 2, SLOT_PMAX = 3, SLOT_DX = 4 };
-enum { KF_TVD = 0, KF_PRED = 1, KF_CORR = 2, KF_RYU = 3, KF_BOUND = 4, KF_RED = 5, KF_FUSED = 6, KF_N = 7 };
+enum { KF_TVD = 0, KF_PRED = 1, KF_CORR = 2, KF_RYU = 3, KF_BOUND = 4, KF_RED = 5, KF_FUSED = 6,
+       KF_PROJ = 7, KF_N = 8 };
 
 struct Geo {
   int nd;
@@ -136,7 +137,8 @@ __device__ __forceinline__ double divb_at(const Geo& g, const double* __restrict
 // which is what this kernel writes (one thread per ghost cell).
 template <int NW>
 __device__ __forceinline__ void bound_body(const Geo& g, const BC& bc, double* __restrict__ w,
-                                           long long t, long long slab0, long long slab1) {
+                                           long long t, long long slab0, long long slab1,
+                                           int iw0 = 0, int iw1 = NW) {
   int d;
   if (t < slab0) d = 0;
   else if (t < slab0 + slab1) { d = 1; t -= slab0; }
@@ -153,6 +155,7 @@ __device__ __forceinline__ void bound_body(const Geo& g, const BC& bc, double* _
   const long long dst = g.off + c[0] + c[1] * g.s[1] + c[2] * g.s[2];
 #pragma unroll
   for (int iw = 0; iw < NW; ++iw) {
+    if (iw < iw0 || iw >= iw1) continue;
     long long src = g.off;
     for (int k = 0; k < g.nd; ++k) {
       int s = c[k];
@@ -183,10 +186,11 @@ inline long long ghost_slabs(const Geo& g, long long* slab) {
 
 template <int NW>
 __global__ __launch_bounds__(256) void k_bound(Geo g, BC bc, double* __restrict__ w,
-                                              long long total, long long slab0, long long slab1) {
+                                              long long total, long long slab0, long long slab1,
+                                              int iw0, int iw1) {
   long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= total) return;
-  bound_body<NW>(g, bc, w, t, slab0, slab1);
+  bound_body<NW>(g, bc, w, t, slab0, slab1, iw0, iw1);
 }
 
 // ------------------------------------------------------------ TVDLF

@@ -17,9 +17,11 @@
 #include "../../include/vacgpu.h"
 #include "physics.hpp"
 #include "sweep.hpp"
+#include "project.hpp"
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -64,6 +66,28 @@ struct vacgpu_grid {
   size_t stage_bytes = 0;
   long long nbound = 0, slab[3] = {0, 0, 0};
   int cur = 0, old = 0;
+  // a state kept as the next step's wold (the state before a projection at
+  // the start of a step, as VAC stores wold before advance), or -1
+  int old_pending = -1;
+  bool mid_step = false;  // between a step's first and last sweep
+  // the state's ghost cells are not all copies of mesh cells (after a
+  // projection: B's second ghost layer keeps its old values, and 'nul'
+  // Poisson faces give the first layer values of its own); the next sweep
+  // then reads ghost cells from memory instead of mapping them to the mesh
+  bool ghosts_dirty = false;
+  // projection of B (process / projectb): work planes, CG scalars, the
+  // values projectb saves across calls, and the processing schedule
+  struct Proj {
+    double* rhs = nullptr;
+    double* phi = nullptr;
+    double* z = nullptr;
+    double* az = nullptr;
+    Cg* cg = nullptr;
+    Cg* cg_host = nullptr;  // pinned
+    double divbmax = 0.0;
+    int nproc1 = 0, itmin = 0;
+    double procpar[3] = {-1.0, -1.0, -1.0};
+  } pr;
   bool have_state = false;
   hipStream_t stream = nullptr;
   bool own_stream = false;
@@ -207,11 +231,12 @@ int launch_check(vacgpu_grid* g, const char* what) {
   return 0;
 }
 
-int bound(vacgpu_grid* g, double* w) {
+// ghost cells of variables iw0 <= iw < iw1 (all of them by default)
+int bound(vacgpu_grid* g, double* w, int iw0 = 0, int iw1 = 8) {
   if (g->nbound == 0) return 0;
   Timed t(g, KF_BOUND);
   const unsigned nb = (unsigned)((g->nbound + 255) / 256);
-#define BOUND_NW(NWV) hipLaunchKernelGGL(k_bound<NWV>, dim3(nb), dim3(256), 0, g->stream, g->geo, g->bc, w, g->nbound, g->slab[0], g->slab[1])
+#define BOUND_NW(NWV) hipLaunchKernelGGL(k_bound<NWV>, dim3(nb), dim3(256), 0, g->stream, g->geo, g->bc, w, g->nbound, g->slab[0], g->slab[1], iw0, iw1)
   switch (g->nw) {
     case 4: BOUND_NW(4); break;
     case 5: BOUND_NW(5); break;
@@ -350,7 +375,13 @@ int common_limiter(const vacgpu_grid* g, const vacgpu_method* m) {
 
 // one dimensional sweep (advect, src/vac.t:363-486), dt already on device
 int sweep_device(vacgpu_grid* g, const vacgpu_method* m, int d, int firstsweep, int lastsweep) {
-  if (firstsweep) g->old = g->cur;
+  if (firstsweep) {
+    g->old = g->old_pending >= 0 ? g->old_pending : g->cur;
+    g->old_pending = -1;
+  }
+  g->mid_step = !lastsweep;
+  const bool dirty = g->ghosts_dirty;
+  g->ghosts_dirty = false;
   const int outi = other_buffer(g);
   double* win = g->buf[g->cur];
   double* wout = g->buf[outi];
@@ -366,7 +397,7 @@ int sweep_device(vacgpu_grid* g, const vacgpu_method* m, int d, int firstsweep, 
   bool one_lim = true;
   for (int iw = 1; iw < g->nw; ++iw) one_lim &= m->typelimiter[iw] == m->typelimiter[0];
   const bool fused = m->typefull == VACGPU_TVDLF && g->cfg.ndim == 2 && g->fused_ok &&
-                     g->use_fused && one_lim && (!mhd || src);
+                     g->use_fused && one_lim && (!mhd || src) && !dirty;
   if (fused) {
     const long long nbf = (long long)g->grid_fs.x * g->grid_fs.y;
     {
@@ -518,7 +549,7 @@ int sweep_device(vacgpu_grid* g, const vacgpu_method* m, int d, int firstsweep, 
     // first sweep and the first half of the Ryu step into the last one
     // (one limiter for all characteristic waves is fixed at compile time)
     const int lim = common_limiter(g, m);
-    const bool fold = g->cfg.ndim == 2 && g->bc_uniform && src && g->use_fused && lim >= 0;
+    const bool fold = g->cfg.ndim == 2 && g->bc_uniform && src && g->use_fused && lim >= 0 && !dirty;
     // (the Ryu half folds into the x sweep only: in the marching y sweep it
     // would cost occupancy)
     const int srcmode = (fold && firstsweep && !lastsweep) ? 1
@@ -653,15 +684,159 @@ int sweep_device(vacgpu_grid* g, const vacgpu_method* m, int d, int firstsweep, 
   return 0;
 }
 
+// boundscalar's face types: those of the normal B component
+// (src/vacproc.projectb.t:82-101; 'cont' falls to the default 'nul')
+SBC make_sbc(const vacgpu_grid* g) {
+  SBC sb;
+  const int nd = g->cfg.ndim;
+  for (int f = 0; f < 6; ++f)
+    sb.per[f] = (f < 2 * nd && g->cfg.typeB[f][2 + nd + f / 2] == VACGPU_PERIODIC) ? 1 : 0;
+  return sb;
+}
+
+int ensure_proj(vacgpu_grid* g) {
+  auto& P = g->pr;
+  if (P.cg) return 0;
+  const size_t plane = sizeof(double) * (size_t)g->geo.ps;
+  double** planes[4] = {&P.rhs, &P.phi, &P.z, &P.az};
+  for (auto q : planes) {
+    HIPCHK(g, hipMalloc(q, plane));
+    HIPCHK(g, hipMemset(*q, 0, plane));
+  }
+  HIPCHK(g, hipMalloc(&P.cg, sizeof(Cg)));
+  HIPCHK(g, hipMemset(P.cg, 0, sizeof(Cg)));
+  HIPCHK(g, hipHostMalloc(&P.cg_host, sizeof(Cg), hipHostMallocDefault));
+  return 0;
+}
+
+// projectb (src/vacproc.projectb.t:4-157) on the current state.  keep_old:
+// the state before the projection becomes the next first sweep's wold.
+int projectb_device(vacgpu_grid* g, const double* procpar, bool keep_old, int* info_out,
+                    int* iter_out, double* resid_out) {
+  if (g->cfg.physics != VACGPU_MHD) return fail(g, VACGPU_E_ARG, "the projection of B needs MHD");
+  int rc;
+  if ((rc = ensure_proj(g))) return rc;
+  auto& P = g->pr;
+  const int nd = g->cfg.ndim;
+  const long long nb = (long long)g->grid.x * g->grid.y * g->grid.z;
+  if (keep_old) {
+    const int x = other_buffer(g);
+    const size_t bytes = sizeof(double) * (size_t)g->geo.ps * g->nw;
+    HIPCHK(g, hipMemcpyAsync(g->buf[x], g->buf[g->cur], bytes, hipMemcpyDeviceToDevice, g->stream));
+    g->old_pending = x;
+  }
+  double* w = g->buf[g->cur];
+  const SBC sb = make_sbc(g);
+  Timed t(g, KF_PROJ);
+  if (nd == 2)
+    hipLaunchKernelGGL(k_pr_divb<2>, g->grid, g->block, 0, g->stream, g->geo, w, P.rhs, P.z, P.phi, g->part, g->pstride);
+  else
+    hipLaunchKernelGGL(k_pr_divb<3>, g->grid, g->block, 0, g->stream, g->geo, w, P.rhs, P.z, P.phi, g->part, g->pstride);
+  hipLaunchKernelGGL(k_pr_fold, dim3(1), dim3(1024), 0, g->stream, P.cg, g->part, g->pstride, nb);
+  if ((rc = launch_check(g, "k_pr_divb"))) return rc;
+  if (P.divbmax < 1.0e-99) {
+    // the first call fixes the allowed maximum, the stopping rule and the
+    // matvec limit from the initial div B (:29-58)
+    HIPCHK(g, hipMemcpyAsync(P.cg_host, P.cg, sizeof(Cg), hipMemcpyDeviceToHost, g->stream));
+    HIPCHK(g, hipStreamSynchronize(g->stream));
+    const double coeff = procpar[0] == -1.0 ? 1.0 / 10 : procpar[0];
+    const double cnst = procpar[1] == -1.0 ? 0.0 : procpar[1];
+    double divbmax = P.cg_host->mx;
+    int typestop;
+    if (coeff < 0.0 && coeff > -1.0) {
+      divbmax = -coeff;
+      typestop = CG_STOP_REL;
+    } else {
+      divbmax = divbmax * std::max(0.0, coeff) + std::max(0.0, cnst);
+      typestop = CG_STOP_MAX;
+      if (divbmax < 1.0e-99)
+        return fail(g, VACGPU_E_ARG, "projectb: too small value for divbmax (procpar)");
+    }
+    int maxit = 1000;
+    if (cnst < -1.0) maxit = (int)std::lround(-cnst);
+    if (coeff < -1.0) maxit = (int)std::lround(-coeff);
+    P.divbmax = divbmax;
+    hipLaunchKernelGGL(k_cg_set, dim3(1), dim3(1), 0, g->stream, P.cg, divbmax, typestop, maxit);
+    if ((rc = launch_check(g, "k_cg_set"))) return rc;
+  }
+  hipLaunchKernelGGL(k_cg_init, dim3(1), dim3(1), 0, g->stream, P.cg);
+  if ((rc = launch_check(g, "k_cg_init"))) return rc;
+  // CG steps in chunks; a step after the stopping test returns at once
+  int chunk = 2;
+  long long queued = 0;
+  for (;;) {
+    for (int k = 0; k < chunk; ++k) {
+      if (nd == 2)
+        hipLaunchKernelGGL(k_cg_matvec<2>, g->grid, g->block, 0, g->stream, g->geo, sb, P.z, P.az, g->part, g->pstride, P.cg);
+      else
+        hipLaunchKernelGGL(k_cg_matvec<3>, g->grid, g->block, 0, g->stream, g->geo, sb, P.z, P.az, g->part, g->pstride, P.cg);
+      hipLaunchKernelGGL(k_cg_alf, dim3(1), dim3(1024), 0, g->stream, P.cg, g->part, g->pstride, nb);
+      if (nd == 2)
+        hipLaunchKernelGGL(k_cg_update<2>, g->grid, g->block, 0, g->stream, g->geo, P.phi, P.rhs, P.z, P.az, g->part, g->pstride, P.cg);
+      else
+        hipLaunchKernelGGL(k_cg_update<3>, g->grid, g->block, 0, g->stream, g->geo, P.phi, P.rhs, P.z, P.az, g->part, g->pstride, P.cg);
+      hipLaunchKernelGGL(k_cg_res, dim3(1), dim3(1024), 0, g->stream, P.cg, g->part, g->pstride, nb);
+      if (nd == 2)
+        hipLaunchKernelGGL(k_cg_dir<2>, g->grid, g->block, 0, g->stream, g->geo, P.rhs, P.z, P.cg);
+      else
+        hipLaunchKernelGGL(k_cg_dir<3>, g->grid, g->block, 0, g->stream, g->geo, P.rhs, P.z, P.cg);
+    }
+    if ((rc = launch_check(g, "k_cg_step"))) return rc;
+    queued += chunk;
+    HIPCHK(g, hipMemcpyAsync(P.cg_host, P.cg, sizeof(Cg), hipMemcpyDeviceToHost, g->stream));
+    HIPCHK(g, hipStreamSynchronize(g->stream));
+    if (P.cg_host->done) break;
+    if (queued > (long long)P.cg_host->maxit + 2)
+      return fail(g, VACGPU_E_STATE, "projectb: the CG iteration did not stop");
+    chunk = chunk < 16 ? 2 * chunk : 16;
+  }
+  hipLaunchKernelGGL(k_cg_final, dim3(1), dim3(1), 0, g->stream, P.cg, g->sc);
+  if ((rc = launch_check(g, "k_cg_final"))) return rc;
+  {
+    const dim3 gr((g->geo.n[0] + 2 + 63) / 64, (g->geo.n[1] + 2 + 3) / 4, nd == 3 ? g->geo.n[2] + 2 : 1);
+    const int eb = (int)std::lround(std::fabs(procpar[2]));
+    if (nd == 2)
+      hipLaunchKernelGGL(k_pr_apply<2>, gr, dim3(64, 4), 0, g->stream, g->geo, sb, P.phi, w, P.cg, eb, g->cfg.eqpar[0]);
+    else
+      hipLaunchKernelGGL(k_pr_apply<3>, gr, dim3(64, 4), 0, g->stream, g->geo, sb, P.phi, w, P.cg, eb, g->cfg.eqpar[0]);
+    if ((rc = launch_check(g, "k_pr_apply"))) return rc;
+  }
+  // B's ghost cells when procpar(divbbound_) > 0 (:143-144)
+  if (procpar[2] > 0.0 && (rc = bound(g, w, 2 + nd, 2 + 2 * nd))) return rc;
+  // what the solve returned (the sign rule of :570-573, as k_cg_final applies it)
+  const Cg& c = *P.cg_host;
+  int info = c.info;
+  if ((c.typestop == CG_STOP_REL && c.res > 1.0) || (c.typestop != CG_STOP_REL && c.res > c.res0))
+    info = -info;
+  if (info != 3 && info >= 0) g->ghosts_dirty = true;
+  if (info_out) *info_out = info;
+  if (iter_out) *iter_out = c.itr;
+  if (resid_out) *resid_out = c.res;
+  return 0;
+}
+
 int advance_device(vacgpu_grid* g, const vacgpu_method* m, int it) {
   const int nd = g->cfg.ndim;
+  const int np = g->pr.nproc1;
+  const int di = it - g->pr.itmin;
+  // TVD's Ryu step reads the state from before the step's projection
+  const bool needs_old = m->typefull == VACGPU_TVD && g->cfg.physics == VACGPU_MHD &&
+                         m->divbfix && m->sourceunsplit;
+  auto sweep = [&](int d, bool first, bool last) {
+    // process(1) before the sweep when due (advect, src/vac.t:392-399)
+    if (np != 0 && (np < 0 || (first && di == (di / np) * np))) {
+      int rc = projectb_device(g, g->pr.procpar, first && needs_old, nullptr, nullptr, nullptr);
+      if (rc) return rc;
+    }
+    return sweep_device(g, m, d, first, last);
+  };
   int rc;
   if ((it / 2) * 2 == it || m->dimsplit_xy) {
     for (int d = 0; d < nd; ++d)
-      if ((rc = sweep_device(g, m, d, d == 0, d == nd - 1))) return rc;
+      if ((rc = sweep(d, d == 0, d == nd - 1))) return rc;
   } else {
     for (int d = nd - 1; d >= 0; --d)
-      if ((rc = sweep_device(g, m, d, d == nd - 1, d == 0))) return rc;
+      if ((rc = sweep(d, d == nd - 1, d == 0))) return rc;
   }
   return 0;
 }
@@ -1163,6 +1338,27 @@ int vacgpu_snapshot_fetch_mesh(vacgpu_grid* g, int slot, double* w, int* it, dou
   return snapshot_rows(g, slot, w, l, 2, it, t);
 }
 
+int vacgpu_set_process(vacgpu_grid* g, int nproc1, const double* procpar, int itmin) {
+  if (!g) return VACGPU_E_ARG;
+  if (nproc1 != 0 && g->cfg.physics != VACGPU_MHD)
+    return fail(g, VACGPU_E_ARG, "processing (the projection of B) needs MHD");
+  g->pr.nproc1 = nproc1;
+  g->pr.itmin = itmin;
+  for (int i = 0; i < 3; ++i) g->pr.procpar[i] = procpar ? procpar[i] : -1.0;
+  g->pr.divbmax = 0.0;  // projectb's saved values start over
+  return 0;
+}
+
+int vacgpu_process(vacgpu_grid* g, int count, const double* procpar, int* info, int* iter,
+                   double* resid) {
+  if (!g) return VACGPU_E_ARG;
+  if (!g->have_state) return fail(g, VACGPU_E_STATE, "no state on the device (call vacgpu_put_state)");
+  if (count != 1 && count < 3)
+    return fail(g, VACGPU_E_ARG, "process: count must be 1 (before a step or sweep) or ifile+2 (before a save)");
+  const double* pp = procpar ? procpar : g->pr.procpar;
+  return projectb_device(g, pp, count == 1 && !g->mid_step, info, iter, resid);
+}
+
 const char* vacgpu_last_error(const vacgpu_grid* g) {
   if (!g) return "null handle";
   return g->err.c_str();
@@ -1181,6 +1377,14 @@ void vacgpu_destroy(vacgpu_grid* g) {
   if (g->stage) (void)hipHostFree(g->stage);
   if (g->part) (void)hipFree(g->part);
   if (g->ctr) (void)hipFree(g->ctr);
+  {
+    auto& P = g->pr;
+    double* planes[4] = {P.rhs, P.phi, P.z, P.az};
+    for (double* q : planes)
+      if (q) (void)hipFree(q);
+    if (P.cg) (void)hipFree(P.cg);
+    if (P.cg_host) (void)hipHostFree(P.cg_host);
+  }
   for (auto& S : g->snap) {
     if (S.pending) (void)hipEventSynchronize(S.done);
     if (S.dev) (void)hipFree(S.dev);

@@ -34,7 +34,8 @@ data created /.false./
 type(vacgpu_config), save :: cfg
 type(vacgpu_method) :: m
 integer(c_int) :: rc
-integer(c_int), target :: ld(3), ne(4)
+integer(c_int), target :: ld(3), ne(4), pinfo, piter
+real(c_double), target :: pp(3), pres
 integer:: iw, iB, code, ngx(3), mlo(3), mhi(3)
 double precision, save, target :: dxl(ixGhimax,3)
 real(c_double), target :: dtc(3), sp
@@ -42,7 +43,7 @@ real(c_double), target :: dtc(3), sp
 
 if(ix^L/=ixG^L|.or.|.or.) call die('vacgpu advect: subgrids are not supported')
 if(idimmin/=idimmax) call die('vacgpu advect: needs dimsplit=T')
-if(nproc(1)/=0 .or. nproc(2)/=0) call die('vacgpu advect: nproc is not on the GPU path')
+if(nproc(2)/=0) call die('vacgpu advect: nproc(2) processing is not on the GPU path')
 if(typeaxial/='slab' .or. gencoord) call die('vacgpu advect: slab Cartesian grids only')
 if(useprimitive) call die('vacgpu advect: useprimitive=T is not on the GPU path')
 if(typelimited/='original') call die('vacgpu advect: typelimited must be original')
@@ -128,6 +129,21 @@ m%smallpcoeff=smallpcoeff
 if(firstsweep)then
    rc=vacgpu_put_state(grid,w,ld)
    if(rc/=0) call die('vacgpu advect: '//vacgpu_errmsg(grid))
+endif
+
+! process(1) when due, as VAC's advect does (src/vac.t:392-399): the
+! projection of B on the device
+if(nproc(1)/=0.and.implpar<=zero)then
+   if(nproc(1)<0.or.(firstsweep.and.it-itmin==((it-itmin)/nproc(1))*nproc(1)))then
+      if(typephys/='mhd' .or. ndim<2) &
+         call die('vacgpu advect: processing (projection of B) needs 2D or 3D MHD')
+      pp=-one
+      pp(1:min(3,nprocpar))=procpar(1:min(3,nprocpar))
+      rc=vacgpu_process(grid,1,c_loc(pp),c_loc(pinfo),c_loc(piter),c_loc(pres))
+      if(rc/=0) call die('vacgpu advect: '//vacgpu_errmsg(grid))
+      ! nerror(poissonerr_) as poisson() counts it (src/vacpoisson.t:79-81)
+      if(pinfo/=0.and.pinfo/=3) nerror(poissonerr_)=nerror(poissonerr_)+1
+   endif
 endif
 
 dtc=bigdouble

@@ -164,6 +164,22 @@ module vacgpu_mod
        real(c_double), intent(out) :: t
      end function vacgpu_snapshot_fetch_mesh
 
+     integer(c_int) function vacgpu_set_process(grid, nproc1, procpar, itmin) &
+          bind(C, name='vacgpu_set_process')
+       import :: c_int, c_ptr
+       type(c_ptr), value :: grid
+       integer(c_int), value :: nproc1, itmin
+       type(c_ptr), value :: procpar
+     end function vacgpu_set_process
+
+     integer(c_int) function vacgpu_process(grid, count, procpar, info, iter, resid) &
+          bind(C, name='vacgpu_process')
+       import :: c_int, c_ptr
+       type(c_ptr), value :: grid
+       integer(c_int), value :: count
+       type(c_ptr), value :: procpar, info, iter, resid
+     end function vacgpu_process
+
      integer(c_int) function vacgpu_check_quotients(a, b, n, q_kernel, q_ieee) &
           bind(C, name='vacgpu_check_quotients')
        import :: c_int, c_double, c_long_long

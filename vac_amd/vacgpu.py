@@ -25,7 +25,7 @@ LIMITERS = {"minmod": 0, "woodward": 1, "superbee": 2}
 CONT, PERIODIC = 0, 1
 FP64, FP32BULK = 0, 1
 KERNELS = {"tvd_sweep": 0, "tvdlf_pred": 1, "tvdlf_corr": 2, "ryu": 3, "bound": 4, "reduce": 5,
-           "tvdlf_fused": 6}
+           "tvdlf_fused": 6, "projectb": 7}
 
 
 class Config(C.Structure):
@@ -65,6 +65,8 @@ SIGNATURES = {
     "vacgpu_snapshot_start": ([C.c_void_p, C.c_int], C.c_int),
     "vacgpu_snapshot_fetch": ([C.c_void_p, C.c_int, _DP, _IP, _IP, _DP], C.c_int),
     "vacgpu_snapshot_fetch_mesh": ([C.c_void_p, C.c_int, _DP, _IP, _DP], C.c_int),
+    "vacgpu_set_process": ([C.c_void_p, C.c_int, _DP, C.c_int], C.c_int),
+    "vacgpu_process": ([C.c_void_p, C.c_int, _DP, _IP, _IP, _DP], C.c_int),
     "vacgpu_supported": ([C.POINTER(Config), C.POINTER(Method)], C.c_int),
     "vacgpu_last_error": ([C.c_void_p], C.c_char_p),
     "vacgpu_destroy": ([C.c_void_p], None),
@@ -248,6 +250,25 @@ class Grid:
             self._check(self.lib.vacgpu_snapshot_fetch(self.h, slot, _ptr(w), None, C.byref(it),
                                                        C.byref(t)), "snapshot_fetch")
         return w, it.value, t.value
+
+    def set_process(self, nproc1: int, procpar=(-1.0, -1.0, -1.0), itmin: int | None = None):
+        """VAC's nproc(1) and procpar: project B before the steps (or sweeps)
+        VAC's advect would; itmin defaults to the current it."""
+        if itmin is None:
+            itmin = self.get_time()[0]
+        pp = (C.c_double * 3)(*[float(v) for v in procpar])
+        self._check(self.lib.vacgpu_set_process(self.h, int(nproc1), pp, int(itmin)),
+                    "set_process")
+
+    def process(self, count: int = 1, procpar=None):
+        """One process(count) call (the projection of B); returns the
+        solve's (info, iterations, residual)."""
+        pp = None if procpar is None else (C.c_double * 3)(*[float(v) for v in procpar])
+        info, it = C.c_int(), C.c_int()
+        res = C.c_double()
+        self._check(self.lib.vacgpu_process(self.h, int(count), pp, C.byref(info), C.byref(it),
+                                            C.byref(res)), "process")
+        return info.value, it.value, res.value
 
     def set_time(self, it: int, t: float):
         self._check(self.lib.vacgpu_set_time(self.h, it, t), "set_time")
