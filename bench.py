@@ -108,6 +108,7 @@ def main():
                          "and TVDLF workloads; prints no JSON")
     ap.add_argument("--no-tvdlf", action="store_true")
     ap.add_argument("--no-tvdmu", action="store_true")
+    ap.add_argument("--no-projection", action="store_true")
     ap.add_argument("--no-3d", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-serving", action="store_true")
@@ -190,6 +191,8 @@ def main():
         out["tvdlf_2048"] = tvdlf_2048(w0, dxs, min(args.steps, 100))
     if not args.no_tvdmu:
         out["tvdmu_2048"] = tvdmu_2048(w0, dxs, min(args.steps, 50))
+    if not args.no_projection:
+        out["ot_projection_2048"] = ot_projection_2048()
     if not args.no_3d:
         out["mhd3d_256"] = mhd3d_256()
     if not args.no_snapshots:
@@ -311,6 +314,53 @@ def tvdmu_2048(w0, dxs, steps):
             "corrector_effective_gbs": round(144 * NX * NY / (kc * 1e-3) / 1e9, 1) if kc else None,
             "workload": (f"2D MHD 2048x2048 periodic random-Fourier, TVD-MUSCL+woodward, "
                          f"courantpar 0.8, {steps} steps; B_alg 480 B/cell-update")}
+
+
+def ot_projection_2048(steps=20):
+    """The reference's own Orszag-Tang recipe (Man/par.html:1205-1232: TVD +
+    woodward, divbfix=F, divbwave=F, nproc=1, procpar=0,0.01,1, periodic) at
+    2048^2, FP64: B is projected (CG Poisson solve) before every step.
+    Reported beside the same run without processing; the CG kernels stream
+    88 B per cell and iteration (Laplacian 16, update 48, direction 24)."""
+    from vac_amd import vacgpu
+    from vac_amd.grid import widths_1d
+    from vac_amd.initial import orszag_tang
+
+    w0, coords = orszag_tang(NX, GAMMA)
+    dxs = [widths_1d(c) for c in coords]
+    method = vacgpu.MethodSpec(typefull="tvd", limiter="woodward", divbfix=False,
+                               divbwave=False, courantpar=0.8)
+    procpar = (0.0, 0.01, 1.0)
+    res = {}
+    for nproc1 in (1, 0):
+        g = vacgpu.Grid([NX, NX], "mhd", GAMMA, dxs)
+        g.put_mesh(w0)
+        g.set_process(nproc1, procpar)
+        g.run(method, 3)
+        g.put_mesh(w0)
+        g.set_time(0, 0.0)
+        g.set_process(nproc1, procpar)
+        t0 = time.perf_counter()
+        n, _ = g.run(method, steps)
+        el = time.perf_counter() - t0
+        if n != steps:
+            raise RuntimeError(f"ran {n} of {steps} steps")
+        r = {"mcell_updates_per_s": round(NX * NX * steps / el / 1e6, 2),
+             "ms_per_step": round(el / steps * 1e3, 4)}
+        if nproc1:
+            g.profile(True)
+            g.run(method, 5)
+            r["projection_avg_ms"] = round(g.kernel_time("projectb")[0], 4)
+            g.profile(False)
+            info, it, resid = g.process(1)
+            r["cg_iterations_next_projection"] = it
+            r["cg_info"] = info
+            r["nerror"] = [int(v) for v in g.scalars()["nerror"]]
+        g.close()
+        res["with_projection" if nproc1 else "without_processing"] = r
+    res["workload"] = (f"Orszag-Tang 2048x2048 periodic, gamma 5/3, TVD+woodward, divbfix=F, "
+                       f"divbwave=F, nproc=1, procpar=0,0.01,1, courantpar 0.8, {steps} steps")
+    return res
 
 
 def mhd3d_256(steps=100):

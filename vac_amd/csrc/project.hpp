@@ -6,16 +6,18 @@
 // first ghost layer.
 //
 // Every array here is one plane of the state's layout (same strides and
-// offset).  Each CG step is a handful of stream-ordered launches: the
-// Laplacian with the first dot product, a one-block fold, the vector update
-// with the residual norms, a one-block fold with the stopping tests, and the
-// new search direction.  The CG scalars stay in device memory (Cg); once a
-// stopping test fires, every later launch of the solve returns at once
+// offset).  Each CG step is four launches: the new search direction with the
+// Laplacian and the first dot product, a one-block fold, the vector update
+// with the residual norms, a one-block fold with the stopping tests.  (A fold
+// by the last block to finish, through one atomic counter, costs more: the
+// blocks' increments of one address serialise.)  The CG scalars stay in device memory (Cg);
+// once a stopping test fires, every later launch of the solve returns at once
 // (Cg::done), so the host can queue several steps before it looks.
 //
-// Arithmetic is VAC's statement by statement (divisions and square roots
-// correctly rounded); the dot products are sums in a fixed tree order, so a
-// solve is deterministic but its sums are not VAC's left-to-right ones.
+// Arithmetic is VAC's statement by statement (divisions as pdiv forms them,
+// square roots correctly rounded); the dot products are sums in a fixed tree
+// order, so a solve is deterministic but its sums are not VAC's left-to-right
+// ones.
 #pragma once
 
 #include "sweep.hpp"
@@ -34,6 +36,27 @@ struct Cg {
 // boundscalar's type per face (src/vacpoisson.t:158-300): 1 'periodic', 0 'nul'
 struct SBC {
   unsigned char per[6];
+};
+
+// a / b with y the refined reciprocal of b that pdiv computes (rdx below):
+// the same operations as pdiv after its reciprocal, so the same double
+__device__ __forceinline__ double pdiv_y(double a, double b, double y) {
+  const double q0 = a * y;
+  return __builtin_fma(-__builtin_fma(b, q0, -a), y, q0);
+}
+
+// pdiv's reciprocal of every cell width along one dimension
+__global__ void k_recip(const double* __restrict__ dx, double* __restrict__ r, int n) {
+  const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  if (i >= n) return;
+  const double b = dx[i];
+  const double y0 = __builtin_amdgcn_rcp(b);
+  const double y1 = __builtin_fma(y0, __builtin_fma(-b, y0, 1.0), y0);
+  r[i] = __builtin_fma(y1, __builtin_fma(-b, y1, 1.0), y1);
+}
+
+struct RDX {
+  const double* r[3];
 };
 
 // the ixG coordinate whose value boundscalar leaves at coordinate c along k:
@@ -96,7 +119,7 @@ __device__ __forceinline__ void fold_partials(const double* __restrict__ part, l
 // getdivb (src/vacphys.mhd0.t:106-146, gradient at src/vacgrid.t:815-821):
 // rhs = z = div B over the mesh, phi = 0; partial sum of rhs^2 and max |rhs|
 template <int ND>
-__global__ __launch_bounds__(256) void k_pr_divb(Geo g, const double* __restrict__ w,
+__global__ __launch_bounds__(256) void k_pr_divb(Geo g, RDX rx, const double* __restrict__ w,
                                                 double* __restrict__ rhs, double* __restrict__ z,
                                                 double* __restrict__ phi, double* __restrict__ part,
                                                 long long pstride) {
@@ -109,7 +132,8 @@ __global__ __launch_bounds__(256) void k_pr_divb(Geo g, const double* __restrict
 #pragma unroll
     for (int d = 0; d < ND; ++d) {
       const double* b = w + L::b(d) * g.ps;
-      const double gq = 0.5 * (b[cl.idx + g.s[d]] - b[cl.idx - g.s[d]]) / g.dx[d][cl.c[d]];
+      const double gq = pdiv_y(0.5 * (b[cl.idx + g.s[d]] - b[cl.idx - g.s[d]]), g.dx[d][cl.c[d]],
+                               rx.r[d][cl.c[d]]);
       dv = dv + gq;
     }
     rhs[cl.idx] = dv;
@@ -164,34 +188,61 @@ __global__ void k_cg_init(Cg* cg) {
   }
 }
 
-// matvec_poisson (src/vacpoisson.t:100-155): az = Laplace(z) over the mesh as
-// the divergence of the gradient, each a centred difference
-// half*(q(i+1) - q(i-1))/dx(i), with z's ghost values as boundscalar sets
-// them; partial sums of z*az (alf = sum(tmp*tmp2), :513)
+// One CG step's first half (src/vacpoisson.t:509-513 and :562): Z = rhs +
+// bet*Z (the previous step's new search direction; the first step uses Z as
+// set up), AZ = Laplace(Z) as matvec_poisson (src/vacpoisson.t:100-155)
+// computes it -- the divergence of the gradient, each a centred difference
+// half*(q(i+1) - q(i-1))/dx(i), Z's ghost values as boundscalar sets them --
+// and the block's partial sum of Z*AZ.  Z is read from zo and written to zn
+// (the new direction is evaluated at the stencil's points).
 template <int ND>
-__global__ __launch_bounds__(256) void k_cg_matvec(Geo g, SBC sb, const double* __restrict__ z,
+__global__ __launch_bounds__(256) void k_cg_matvec(Geo g, SBC sb, RDX rx,
+                                                  const double* __restrict__ rhs,
+                                                  const double* __restrict__ zo,
+                                                  double* __restrict__ zn,
                                                   double* __restrict__ az,
                                                   double* __restrict__ part, long long pstride,
-                                                  const Cg* __restrict__ cg) {
+                                                  const Cg* __restrict__ cg, int first) {
   if (cg->done) return;  // uniform: every thread reads the same flag
   __shared__ double sm[16];
+  const double bet = cg->bet;
+  // the search direction at ixG point c (mapped as boundscalar does)
+  auto zdir = [&](const int* c) -> double {
+    long long idx = g.off;
+#pragma unroll
+    for (int k = 0; k < ND; ++k) {
+      const int m = smap(g, sb, k, c[k]);
+      if (m < 0) return 0.0;
+      idx += (long long)m * g.s[k];
+    }
+    return first ? zo[idx] : rhs[idx] + bet * zo[idx];
+  };
   const CellIx cl = cell_of(g);
   double pz = 0.0;
   if (cl.ok) {
-    const double zc = z[cl.idx];
+    const double zc = first ? zo[cl.idx] : rhs[cl.idx] + bet * zo[cl.idx];
+    zn[cl.idx] = zc;
     double qy = 0.0;
 #pragma unroll
     for (int d = 0; d < ND; ++d) {
       const double* dxd = g.dx[d];
+      const double* rd = rx.r[d];
       const int cd = cl.c[d];
-      int c2[3] = {cl.c[0], cl.c[1], cl.c[2]};
-      c2[d] = cd - 2;
-      const double zmm = sval<ND>(g, sb, z, c2);
-      c2[d] = cd + 2;
-      const double zpp = sval<ND>(g, sb, z, c2);
-      const double qdm = 0.5 * (zc - zmm) / dxd[cd - 1];
-      const double qdp = 0.5 * (zpp - zc) / dxd[cd + 1];
-      const double qdd = 0.5 * (qdp - qdm) / dxd[cd];
+      double zmm, zpp;
+      if (cd >= 4 && cd <= g.n[d] - 1) {
+        const long long s2 = 2 * g.s[d];
+        zmm = first ? zo[cl.idx - s2] : rhs[cl.idx - s2] + bet * zo[cl.idx - s2];
+        zpp = first ? zo[cl.idx + s2] : rhs[cl.idx + s2] + bet * zo[cl.idx + s2];
+      } else {
+        int c2[3] = {cl.c[0], cl.c[1], cl.c[2]};
+        c2[d] = cd - 2;
+        zmm = zdir(c2);
+        c2[d] = cd + 2;
+        zpp = zdir(c2);
+      }
+      const double qdm = pdiv_y(0.5 * (zc - zmm), dxd[cd - 1], rd[cd - 1]);
+      const double qdp = pdiv_y(0.5 * (zpp - zc), dxd[cd + 1], rd[cd + 1]);
+      const double qdd = pdiv_y(0.5 * (qdp - qdm), dxd[cd], rd[cd]);
       qy = qy + qdd;
     }
     az[cl.idx] = qy;
@@ -277,16 +328,6 @@ __global__ __launch_bounds__(1024) void k_cg_res(Cg* cg, const double* __restric
   }
 }
 
-// Z = rhs + bet*Z (:562)
-template <int ND>
-__global__ __launch_bounds__(256) void k_cg_dir(Geo g, const double* __restrict__ rhs,
-                                               double* __restrict__ z, const Cg* __restrict__ cg) {
-  if (cg->done) return;
-  const CellIx cl = cell_of(g);
-  if (!cl.ok) return;
-  z[cl.idx] = rhs[cl.idx] + cg->bet * z[cl.idx];
-}
-
 // the returned info's sign (:570-573) and nerror(poissonerr_) (:79-81)
 __global__ void k_cg_final(Cg* cg, Scal* sc) {
   int info = cg->info;
@@ -302,7 +343,7 @@ __global__ void k_cg_final(Cg* cg, Scal* sc) {
 // corrects e to keep the thermal / total pressure (:128-136).  Nothing is done
 // when the solve returned info 3 or info < 0 (:104-108).
 template <int ND>
-__global__ __launch_bounds__(256) void k_pr_apply(Geo g, SBC sb, const double* __restrict__ phi,
+__global__ __launch_bounds__(256) void k_pr_apply(Geo g, SBC sb, RDX rx, const double* __restrict__ phi,
                                                  double* __restrict__ w,
                                                  const Cg* __restrict__ cg, int ebound,
                                                  double gamma) {
@@ -324,7 +365,7 @@ __global__ __launch_bounds__(256) void k_pr_apply(Geo g, SBC sb, const double* _
     const double pp = sval<ND>(g, sb, phi, cc);
     cc[d] = c[d] - 1;
     const double pm = sval<ND>(g, sb, phi, cc);
-    const double gq = 0.5 * (pp - pm) / g.dx[d][c[d]];
+    const double gq = pdiv_y(0.5 * (pp - pm), g.dx[d][c[d]], rx.r[d][c[d]]);
     double* pb = w + L::b(d) * g.ps;
     const double b = pb[idx];
     if (ebound == 2) {

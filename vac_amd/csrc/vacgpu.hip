@@ -82,9 +82,12 @@ struct vacgpu_grid {
     double* phi = nullptr;
     double* z = nullptr;
     double* az = nullptr;
+    double* z2 = nullptr;   // the search direction alternates between z and z2
+    double* rdx[3] = {nullptr, nullptr, nullptr};  // pdiv's reciprocals of dx
     Cg* cg = nullptr;
     Cg* cg_host = nullptr;  // pinned
     double divbmax = 0.0;
+    int last_iters = 0;  // CG steps of the previous solve (the first chunk's length)
     int nproc1 = 0, itmin = 0;
     double procpar[3] = {-1.0, -1.0, -1.0};
   } pr;
@@ -698,10 +701,17 @@ int ensure_proj(vacgpu_grid* g) {
   auto& P = g->pr;
   if (P.cg) return 0;
   const size_t plane = sizeof(double) * (size_t)g->geo.ps;
-  double** planes[4] = {&P.rhs, &P.phi, &P.z, &P.az};
+  double** planes[5] = {&P.rhs, &P.phi, &P.z, &P.z2, &P.az};
   for (auto q : planes) {
     HIPCHK(g, hipMalloc(q, plane));
     HIPCHK(g, hipMemset(*q, 0, plane));
+  }
+  for (int d = 0; d < g->cfg.ndim; ++d) {
+    const int n = g->geo.ng[d];
+    HIPCHK(g, hipMalloc(&P.rdx[d], sizeof(double) * n));
+    hipLaunchKernelGGL(k_recip, dim3((n + 255) / 256), dim3(256), 0, g->stream, g->geo.dx[d], P.rdx[d], n);
+    int rc;
+    if ((rc = launch_check(g, "k_recip"))) return rc;
   }
   HIPCHK(g, hipMalloc(&P.cg, sizeof(Cg)));
   HIPCHK(g, hipMemset(P.cg, 0, sizeof(Cg)));
@@ -727,11 +737,12 @@ int projectb_device(vacgpu_grid* g, const double* procpar, bool keep_old, int* i
   }
   double* w = g->buf[g->cur];
   const SBC sb = make_sbc(g);
+  const RDX rx = {{P.rdx[0], P.rdx[1], P.rdx[2]}};
   Timed t(g, KF_PROJ);
   if (nd == 2)
-    hipLaunchKernelGGL(k_pr_divb<2>, g->grid, g->block, 0, g->stream, g->geo, w, P.rhs, P.z, P.phi, g->part, g->pstride);
+    hipLaunchKernelGGL(k_pr_divb<2>, g->grid, g->block, 0, g->stream, g->geo, rx, w, P.rhs, P.z, P.phi, g->part, g->pstride);
   else
-    hipLaunchKernelGGL(k_pr_divb<3>, g->grid, g->block, 0, g->stream, g->geo, w, P.rhs, P.z, P.phi, g->part, g->pstride);
+    hipLaunchKernelGGL(k_pr_divb<3>, g->grid, g->block, 0, g->stream, g->geo, rx, w, P.rhs, P.z, P.phi, g->part, g->pstride);
   hipLaunchKernelGGL(k_pr_fold, dim3(1), dim3(1024), 0, g->stream, P.cg, g->part, g->pstride, nb);
   if ((rc = launch_check(g, "k_pr_divb"))) return rc;
   if (P.divbmax < 1.0e-99) {
@@ -762,24 +773,28 @@ int projectb_device(vacgpu_grid* g, const double* procpar, bool keep_old, int* i
   hipLaunchKernelGGL(k_cg_init, dim3(1), dim3(1), 0, g->stream, P.cg);
   if ((rc = launch_check(g, "k_cg_init"))) return rc;
   // CG steps in chunks; a step after the stopping test returns at once
-  int chunk = 2;
+  double* zb[2] = {P.z, P.z2};
+  int par = 0;
+  bool first = true;
+  // the first chunk as long as the previous solve (successive solves of a
+  // run take similar numbers of steps), then 2, 4, 8, 16 steps between polls
+  int chunk = std::min(64, std::max(2, P.last_iters));
+  int next = 2;
   long long queued = 0;
   for (;;) {
     for (int k = 0; k < chunk; ++k) {
       if (nd == 2)
-        hipLaunchKernelGGL(k_cg_matvec<2>, g->grid, g->block, 0, g->stream, g->geo, sb, P.z, P.az, g->part, g->pstride, P.cg);
+        hipLaunchKernelGGL(k_cg_matvec<2>, g->grid, g->block, 0, g->stream, g->geo, sb, rx, P.rhs, zb[par], zb[1 - par], P.az, g->part, g->pstride, P.cg, first ? 1 : 0);
       else
-        hipLaunchKernelGGL(k_cg_matvec<3>, g->grid, g->block, 0, g->stream, g->geo, sb, P.z, P.az, g->part, g->pstride, P.cg);
+        hipLaunchKernelGGL(k_cg_matvec<3>, g->grid, g->block, 0, g->stream, g->geo, sb, rx, P.rhs, zb[par], zb[1 - par], P.az, g->part, g->pstride, P.cg, first ? 1 : 0);
       hipLaunchKernelGGL(k_cg_alf, dim3(1), dim3(1024), 0, g->stream, P.cg, g->part, g->pstride, nb);
       if (nd == 2)
-        hipLaunchKernelGGL(k_cg_update<2>, g->grid, g->block, 0, g->stream, g->geo, P.phi, P.rhs, P.z, P.az, g->part, g->pstride, P.cg);
+        hipLaunchKernelGGL(k_cg_update<2>, g->grid, g->block, 0, g->stream, g->geo, P.phi, P.rhs, zb[1 - par], P.az, g->part, g->pstride, P.cg);
       else
-        hipLaunchKernelGGL(k_cg_update<3>, g->grid, g->block, 0, g->stream, g->geo, P.phi, P.rhs, P.z, P.az, g->part, g->pstride, P.cg);
+        hipLaunchKernelGGL(k_cg_update<3>, g->grid, g->block, 0, g->stream, g->geo, P.phi, P.rhs, zb[1 - par], P.az, g->part, g->pstride, P.cg);
       hipLaunchKernelGGL(k_cg_res, dim3(1), dim3(1024), 0, g->stream, P.cg, g->part, g->pstride, nb);
-      if (nd == 2)
-        hipLaunchKernelGGL(k_cg_dir<2>, g->grid, g->block, 0, g->stream, g->geo, P.rhs, P.z, P.cg);
-      else
-        hipLaunchKernelGGL(k_cg_dir<3>, g->grid, g->block, 0, g->stream, g->geo, P.rhs, P.z, P.cg);
+      par = 1 - par;
+      first = false;
     }
     if ((rc = launch_check(g, "k_cg_step"))) return rc;
     queued += chunk;
@@ -788,17 +803,19 @@ int projectb_device(vacgpu_grid* g, const double* procpar, bool keep_old, int* i
     if (P.cg_host->done) break;
     if (queued > (long long)P.cg_host->maxit + 2)
       return fail(g, VACGPU_E_STATE, "projectb: the CG iteration did not stop");
-    chunk = chunk < 16 ? 2 * chunk : 16;
+    chunk = next;
+    next = next < 16 ? 2 * next : 16;
   }
+  P.last_iters = P.cg_host->itr;
   hipLaunchKernelGGL(k_cg_final, dim3(1), dim3(1), 0, g->stream, P.cg, g->sc);
   if ((rc = launch_check(g, "k_cg_final"))) return rc;
   {
     const dim3 gr((g->geo.n[0] + 2 + 63) / 64, (g->geo.n[1] + 2 + 3) / 4, nd == 3 ? g->geo.n[2] + 2 : 1);
     const int eb = (int)std::lround(std::fabs(procpar[2]));
     if (nd == 2)
-      hipLaunchKernelGGL(k_pr_apply<2>, gr, dim3(64, 4), 0, g->stream, g->geo, sb, P.phi, w, P.cg, eb, g->cfg.eqpar[0]);
+      hipLaunchKernelGGL(k_pr_apply<2>, gr, dim3(64, 4), 0, g->stream, g->geo, sb, rx, P.phi, w, P.cg, eb, g->cfg.eqpar[0]);
     else
-      hipLaunchKernelGGL(k_pr_apply<3>, gr, dim3(64, 4), 0, g->stream, g->geo, sb, P.phi, w, P.cg, eb, g->cfg.eqpar[0]);
+      hipLaunchKernelGGL(k_pr_apply<3>, gr, dim3(64, 4), 0, g->stream, g->geo, sb, rx, P.phi, w, P.cg, eb, g->cfg.eqpar[0]);
     if ((rc = launch_check(g, "k_pr_apply"))) return rc;
   }
   // B's ghost cells when procpar(divbbound_) > 0 (:143-144)
@@ -1379,7 +1396,7 @@ void vacgpu_destroy(vacgpu_grid* g) {
   if (g->ctr) (void)hipFree(g->ctr);
   {
     auto& P = g->pr;
-    double* planes[4] = {P.rhs, P.phi, P.z, P.az};
+    double* planes[8] = {P.rhs, P.phi, P.z, P.z2, P.az, P.rdx[0], P.rdx[1], P.rdx[2]};
     for (double* q : planes)
       if (q) (void)hipFree(q);
     if (P.cg) (void)hipFree(P.cg);
