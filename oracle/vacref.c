@@ -47,6 +47,9 @@ typedef struct {
   /* projectb / Poisson work planes: divb (the residual), phi, CG's Z and A.Z,
    * gradients */
   double *pr_rhs, *pr_phi, *pr_z, *pr_az, *pr_qdx, *pr_qddx;
+  /* 'fixed' boundaries: the state the first getboundary call left */
+  double* fixw;
+  int fix_ready;
 } ctx_t;
 
 /* variable indices (0-based), src/vacpar.t.mhd:9-11, src/vacpar.t.hd:9-10 */
@@ -102,8 +105,12 @@ double vacref_limit(int type, double a, double b) {
 }
 
 /* ------------------------------------------------------------ boundaries */
-/* getboundary for 'periodic' and 'cont' (src/vacgrid.t:195-413): regions
- * x-low, x-high, y-low, ... each spanning the full transverse ixG range. */
+/* getboundary for 'cont', 'periodic', 'symm', 'asymm' and 'fixed'
+ * (src/vacgrid.t:195-413, slab grids): regions x-low, x-high, y-low, ...
+ * each spanning the full transverse ixG range.  'symm' / 'asymm' reflect the
+ * mesh into the ghost layers (times -1 for 'asymm', :317-352); 'fixed' copies
+ * like 'cont' on the first call and keeps the values that call left ever
+ * after (fixB, :249-257 and :372-404). */
 static void getboundary_vars(ctx_t* x, double* w, int iw0, int iw1) {
   for (int d = 0; d < x->ndim; ++d) {
     for (int up = 0; up < 2; ++up) {
@@ -116,16 +123,28 @@ static void getboundary_vars(ctx_t* x, double* w, int iw0, int iw1) {
       for (int iw = iw0; iw < iw1; ++iw) {
         double* pw = PL(x, w, iw);
         int bc = x->p->bc[iB][iw];
+        const double* fx = PL(x, x->fixw, iw);
         LOOP(lo, hi) {
           int c = CRD(d), src;
-          if (bc == VACREF_PERIODIC) src = up ? c - n : c + n;   /* :311-313, pairs :107-120 */
-          else src = up ? x->mhi[d] : x->mlo[d];                  /* 'cont' :251-273 */
           long idx = IDX(x);
+          if (bc == VACREF_FIXED && x->fix_ready) { pw[idx] = fx[idx]; continue; }
+          double sg = 1.0;
+          if (bc == VACREF_PERIODIC) src = up ? c - n : c + n;   /* :311-313, pairs :107-120 */
+          else if (bc == VACREF_SYMM || bc == VACREF_ASYMM) {
+            /* ixe = 2*ixmin-1 (upper) or 2*ixmax+1 (lower), :329-333 */
+            src = up ? 2 * (x->mhi[d] + 1) - 1 - c : 2 * (x->mlo[d] - 1) + 1 - c;
+            if (bc == VACREF_ASYMM) sg = -1.0;
+          } else src = up ? x->mhi[d] : x->mlo[d];                /* 'cont', 'fixed' :251-273 */
           long sidx = idx + (long)(src - c) * x->st[d];
-          pw[idx] = pw[sidx];
+          pw[idx] = sg == 1.0 ? pw[sidx] : -pw[sidx];
         }
       }
     }
+  }
+  /* the first call stores what it left for the 'fixed' regions (:372-404) */
+  if (x->fixw && !x->fix_ready) {
+    memcpy(x->fixw, w, sizeof(double) * x->nc * x->nw);
+    x->fix_ready = 1;
   }
 }
 
@@ -898,14 +917,22 @@ static void getdivb(ctx_t* x, const double* w, double* divb) {
 }
 
 /* typeBscalar of face iB from the type of the normal B component there
- * (src/vacproc.projectb.t:82-101): 'periodic' stays 'periodic'; 'cont' (the
- * other type on this path) falls to the default case, 'nul'. */
-static int bscalar_periodic(const ctx_t* x, int iB) {
-  return x->p->bc[iB][B(x, iB / 2)] == VACREF_PERIODIC;
+ * (src/vacproc.projectb.t:82-101): periodic -> periodic, symm -> asymm,
+ * asymm -> symm, fixed -> grad0, anything else ('cont') -> nul. */
+enum { SB_NUL = 0, SB_PERIODIC = 1, SB_SYMM = 2, SB_ASYMM = 3, SB_GRAD0 = 4 };
+static int bscalar_type(const ctx_t* x, int iB) {
+  switch (x->p->bc[iB][B(x, iB / 2)]) {
+    case VACREF_PERIODIC: return SB_PERIODIC;
+    case VACREF_SYMM: return SB_ASYMM;
+    case VACREF_ASYMM: return SB_SYMM;
+    case VACREF_FIXED: return SB_GRAD0;
+  }
+  return SB_NUL;
 }
 
-/* boundscalar for 'periodic' and 'nul' (src/vacpoisson.t:158-300), the
- * faces in VAC's order, each region spanning the full transverse ixG range */
+/* boundscalar for 'periodic', 'symm', 'asymm', 'grad0' and 'nul'
+ * (src/vacpoisson.t:158-300, slab grids), the faces in VAC's order, each
+ * region spanning the full transverse ixG range */
 static void boundscalar(ctx_t* x, double* q) {
   for (int d = 0; d < x->ndim; ++d) {
     for (int up = 0; up < 2; ++up) {
@@ -915,16 +942,26 @@ static void boundscalar(ctx_t* x, double* q) {
       if (up) { lo[d] = x->mhi[d] + 1; hi[d] = x->ghi[d]; }
       else { lo[d] = x->glo[d]; hi[d] = x->mlo[d] - 1; }
       int n = x->p->nx[d];
-      int per = bscalar_periodic(x, iB);
+      int type = bscalar_type(x, iB);
       LOOP(lo, hi) {
         long idx = IDX(x);
-        if (per) {
-          int c = CRD(d);
-          int src = up ? c - n : c + n;  /* the pair face shifted by dixB, :183-194 */
-          q[idx] = q[idx + (long)(src - c) * x->st[d]];
-        } else {
-          q[idx] = 0.0;                  /* 'nul', :286-287 */
+        int c = CRD(d);
+        int src = c;
+        double sg = 1.0;
+        switch (type) {
+          case SB_PERIODIC: src = up ? c - n : c + n; break;  /* pair face shifted by dixB, :183-194 */
+          case SB_SYMM: case SB_ASYMM:                        /* :226-253 */
+            src = up ? 2 * (x->mhi[d] + 1) - 1 - c : 2 * (x->mlo[d] - 1) + 1 - c;
+            if (type == SB_ASYMM) sg = -1.0;
+            break;
+          case SB_GRAD0:  /* 0 in the first ghost layer, the edge value in the second, :254-285 */
+            if (c == (up ? x->mhi[d] + 1 : x->mlo[d] - 1)) { q[idx] = 0.0; continue; }
+            src = up ? x->mhi[d] : x->mlo[d];
+            break;
+          default: q[idx] = 0.0; continue;                    /* 'nul', :286-287 */
         }
+        double v = q[idx + (long)(src - c) * x->st[d]];
+        q[idx] = sg == 1.0 ? v : -v;
       }
     }
   }
@@ -1195,6 +1232,11 @@ static int ctx_init(ctx_t* x, const vacref_params* p, vacref_state* s,
                     &x->rhodv, &x->bdv, &x->bdb, &x->bv, &x->v2a2,
                     &x->pr_rhs, &x->pr_phi, &x->pr_z, &x->pr_az, &x->pr_qdx, &x->pr_qddx};
   for (size_t q = 0; q < sizeof(one) / sizeof(one[0]); ++q) *one[q] = calloc(1, pl);
+  int anyfixed = 0;
+  for (int f = 0; f < 2 * x->ndim; ++f)
+    for (int iw = 0; iw < x->nw; ++iw) anyfixed |= p->bc[f][iw] == VACREF_FIXED;
+  x->fixw = anyfixed ? calloc(x->nw, pl) : NULL;
+  x->fix_ready = 0;
   return 0;
 }
 
@@ -1203,7 +1245,7 @@ static void ctx_free(ctx_t* x) {
                    x->sc1, x->sc2, x->sc3, x->sc4, x->sc5, x->jump, x->adtdx, x->phi,
                    x->rphi, x->ljump, x->cfast, x->cslow, x->afast, x->aslow,
                    x->csound2, x->dp, x->rhodv, x->bdv, x->bdb, x->bv, x->v2a2,
-                   x->pr_rhs, x->pr_phi, x->pr_z, x->pr_az, x->pr_qdx, x->pr_qddx};
+                   x->pr_rhs, x->pr_phi, x->pr_z, x->pr_az, x->pr_qdx, x->pr_qddx, x->fixw};
   for (size_t q = 0; q < sizeof(all) / sizeof(all[0]); ++q) free(all[q]);
 }
 

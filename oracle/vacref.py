@@ -18,7 +18,8 @@ LIB = os.path.join(HERE, "libvacref.so")
 HD, MHD = 0, 1
 TVDLF, TVD, TVDMU = 0, 1, 2
 LIMITERS = {"minmod": 0, "woodward": 1, "superbee": 2}
-CONT, PERIODIC = 0, 1
+CONT, PERIODIC, SYMM, ASYMM, FIXED = 0, 1, 2, 3, 4
+BC_CODES = {"cont": CONT, "periodic": PERIODIC, "symm": SYMM, "asymm": ASYMM, "fixed": FIXED}
 
 
 class Params(C.Structure):
@@ -101,7 +102,7 @@ def make_params(ndim, physics, nx, gamma, typefull="tvdlf", limiter="minmod",
         spec = bcs[f] if f < len(bcs) else "cont"
         for iw in range(8):
             name = spec if isinstance(spec, str) else spec[min(iw, len(spec) - 1)]
-            p.bc[f][iw] = PERIODIC if name == "periodic" else CONT
+            p.bc[f][iw] = BC_CODES[name]
     p.divbfix = int(divbfix)
     p.divbwave = int(divbwave)
     p.sourceunsplit = int(sourceunsplit)
@@ -116,6 +117,14 @@ def make_params(ndim, physics, nx, gamma, typefull="tvdlf", limiter="minmod",
     for i in range(3):
         p.procpar[i] = float(procpar[i])
     return p
+
+
+def bc_spec(v):
+    """The boundary spec a fixture stores: one type name, or a JSON list with
+    one entry per face (a type, or a list of types per variable)."""
+    import json
+    s = str(v)
+    return json.loads(s) if s.startswith("[") else s
 
 
 def pad_state(w_mesh: np.ndarray) -> np.ndarray:

@@ -67,6 +67,26 @@ PROJ_CASES = {
 }
 
 
+# boundary types beyond cont / periodic, one entry per face (x-low, x-high,
+# y-low, ...), each a type for all variables or a list per variable: walls
+# (symm with the normal momentum and field asymm), fixed faces, mixes; the
+# last two with the projection of B (boundscalar's symm / asymm / grad0)
+WALL_X = ["symm", "asymm", "symm", "symm", "asymm", "symm"]
+WALL_Y = ["symm", "symm", "asymm", "symm", "symm", "asymm"]
+WALL_Y_HD = ["symm", "symm", "asymm", "symm"]
+WALL_Z = ["symm", "symm", "symm", "asymm", "symm", "symm", "symm", "asymm"]
+# name: (binary, physics, builder, gamma, steps, typefull, limiter, bc spec, divbfix, procpar)
+BC_CASES = {
+    "mhd22_rf32x24_tvdlf_minmod_bcwallx": ("vac_mhd22_132", "mhd", lambda g: random_fourier((32, 24), 401, g), 5 / 3, 12, "tvdlf", "minmod", [WALL_X, WALL_X, "periodic", "periodic"], True, None),
+    "mhd22_rf28x30_tvd_woodward_bcfixedx_wally": ("vac_mhd22_132", "mhd", lambda g: random_fourier((28, 30), 402, g), 5 / 3, 10, "tvd", "woodward", ["fixed", "fixed", WALL_Y, WALL_Y], True, None),
+    "hd22_rf30x26_tvdlf_woodward_bcwally": ("vac_hd22_132", "hd", lambda g: random_fourier((30, 26), 403, g, physics="hd"), 1.4, 15, "tvdlf", "woodward", ["cont", "cont", WALL_Y_HD, WALL_Y_HD], True, None),
+    "mhd22_rf26x28_tvdmu_superbee_bcmixed": ("vac_mhd22_132", "mhd", lambda g: random_fourier((26, 28), 404, g), 5 / 3, 10, "tvdmu", "superbee", [WALL_X, "fixed", "periodic", "periodic"], True, None),
+    "mhd33_rf10x8x12_tvdlf_minmod_bcwallz": ("vac_mhd33_36", "mhd", lambda g: random_fourier((10, 8, 12), 407, g), 5 / 3, 6, "tvdlf", "minmod", ["periodic", "periodic", "cont", "cont", WALL_Z, WALL_Z], True, None),
+    "mhd22_rf30x32_tvd_woodward_bcwall_proj": ("vacp_mhd22_132", "mhd", lambda g: random_fourier((30, 32), 405, g), 5 / 3, 10, "tvd", "woodward", [WALL_X, WALL_X, "fixed", "fixed"], False, (0.0, 1e-4, 1.0)),
+    "mhd22_rf32x28_tvdlf_minmod_bcsymm_proj": ("vacp_mhd22_132", "mhd", lambda g: random_fourier((32, 28), 406, g), 5 / 3, 10, "tvdlf", "minmod", [["symm"] * 6, ["symm"] * 6, "periodic", "periodic"], True, (0.0, 1e-4, 2.0)),
+}
+
+
 def ensure_binary(name: str) -> None:
     if have_ref(name):
         return
@@ -87,8 +107,9 @@ def run_case(binary, physics, w, coords, gamma, steps, typefull, limiter, bc, di
     extra = ""
     if procpar is not None:
         extra = " nproc=1\n procpar=" + ",".join(repr(float(v)) for v in procpar) + "\n"
+    typeB = [bc] * (2 * len(coords)) if isinstance(bc, str) else bc
     par = problem_par(nw, NAMES[nw], steps, typefull=typefull, limiter=limiter,
-                      typeB=[bc] * (2 * len(coords)), divbfix=divbfix, divbwave=divbwave,
+                      typeB=typeB, divbfix=divbfix, divbwave=divbwave,
                       every_step=True, extra_method=extra)
     r = run_ref(binary, ini, par)
     t = np.array([s.t for s in r.snapshots])
@@ -127,6 +148,22 @@ def main() -> None:
             nx=np.array([len(c) for c in coords]), gamma=gamma, steps=steps,
             physics=physics, typefull=typefull, limiter=limiter, bc=bc,
             divbfix=divbfix, divbwave=divbwave, nproc1=1, procpar=np.array(procpar))
+        print("wrote", name, w0.shape, "steps", len(t) - 1)
+    for name, (binary, physics, build, gamma, steps, typefull, limiter, bc, divbfix,
+               procpar) in BC_CASES.items():
+        if only and not any(o in name for o in only):
+            continue
+        ensure_binary(binary)
+        w0, coords = build(gamma)
+        wf, t = run_case(binary, physics, w0, coords, gamma, steps, typefull, limiter, bc,
+                         divbfix, True, procpar)
+        extra = {} if procpar is None else dict(nproc1=1, procpar=np.array(procpar),
+                                                divbwave=True)
+        np.savez_compressed(
+            os.path.join(HERE, name + ".npz"), w0=w0, wf=wf, t=t,
+            nx=np.array([len(c) for c in coords]), gamma=gamma, steps=steps,
+            physics=physics, typefull=typefull, limiter=limiter, bc=json.dumps(bc),
+            divbfix=divbfix, **extra)
         print("wrote", name, w0.shape, "steps", len(t) - 1)
     if only:
         return

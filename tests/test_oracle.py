@@ -44,7 +44,7 @@ def run_fixture(fx):
                   procpar=[float(v) for v in fx["procpar"]])
     p = vacref.make_params(ndim, str(fx["physics"]), nx, gamma,
                            typefull=str(fx["typefull"]), limiter=str(fx["limiter"]),
-                           bc=str(fx["bc"]), divbfix=bool(fx["divbfix"]), **kw)
+                           bc=vacref.bc_spec(fx["bc"]), divbfix=bool(fx["divbfix"]), **kw)
     return vacref.run(p, fx["w0"], dxs, int(fx["steps"]))
 
 

@@ -78,7 +78,7 @@ def test_gpu_projection_matches_reference_fixture(path):
     method = vacgpu.MethodSpec(typefull=str(z["typefull"]), limiter=str(z["limiter"]),
                                divbfix=bool(z["divbfix"]), divbwave=bool(z["divbwave"]))
     print(f"case {os.path.basename(path)} nx={nx} procpar={procpar}", flush=True)
-    g, dxs = gpu_grid(z["w0"], nx, float(z["gamma"]), str(z["bc"]))
+    g, dxs = gpu_grid(z["w0"], nx, float(z["gamma"]), vacref.bc_spec(z["bc"]))
     g.set_process(int(z["nproc1"]), procpar)
     n, t = g.run(method, int(z["steps"]), record_t=True)
     w = g.get_mesh()
@@ -89,7 +89,7 @@ def test_gpu_projection_matches_reference_fixture(path):
     assert_close(w, z["wf"], os.path.basename(path))
     # the same error counts as the oracle (VAC keeps them in memory only)
     p = vacref.make_params(len(nx), "mhd", nx, float(z["gamma"]), typefull=str(z["typefull"]),
-                           limiter=str(z["limiter"]), bc=str(z["bc"]),
+                           limiter=str(z["limiter"]), bc=vacref.bc_spec(z["bc"]),
                            divbfix=bool(z["divbfix"]), divbwave=bool(z["divbwave"]),
                            nproc1=int(z["nproc1"]), procpar=procpar)
     _, _, st = vacref.run(p, z["w0"], dxs, int(z["steps"]))
