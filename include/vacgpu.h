@@ -33,7 +33,8 @@ enum {
 enum { VACGPU_HD = 0, VACGPU_MHD = 1 };                 /* setvac -p= */
 enum { VACGPU_TVDLF = 0, VACGPU_TVD = 1, VACGPU_TVDMU = 2 };  /* typefull */
 enum { VACGPU_MINMOD = 0, VACGPU_WOODWARD = 1, VACGPU_SUPERBEE = 2 };  /* typelimiter */
-enum { VACGPU_CONT = 0, VACGPU_PERIODIC = 1 };          /* typeB */
+enum { VACGPU_CONT = 0, VACGPU_PERIODIC = 1, VACGPU_SYMM = 2, VACGPU_ASYMM = 3,
+       VACGPU_FIXED = 4 };                             /* typeB */
 enum { VACGPU_FP64 = 0, VACGPU_FP32BULK = 1 };          /* arithmetic tier */
 
 typedef struct vacgpu_grid vacgpu_grid;

@@ -17,6 +17,8 @@ from vac_amd.initial import orszag_tang, random_fourier
 pytestmark = pytest.mark.gpu
 
 NAMES = {6: "rho m1 m2 e b1 b2", 4: "rho m1 m2 e", 8: "rho m1 m2 m3 e b1 b2 b3"}
+WALL_X = ["symm", "asymm", "symm", "symm", "asymm", "symm"]
+WALL_Y = ["symm", "symm", "asymm", "symm", "symm", "asymm"]
 
 CASES = [
     # (cpu binary, gpu binary, physics, builder, gamma, steps, typefull, limiter, bc, divbfix)
@@ -34,10 +36,17 @@ CASES = [
      "tvdmu", "woodward", "periodic", True),
     ("vac_mhd33_36", "vacgpu_mhd33_36", "mhd", lambda g: random_fourier((12, 14, 10), 66, g), 1.4, 6,
      "tvdmu", "minmod", "cont", True),
+    # reflecting walls (symm; the normal momentum and field asymm) and fixed faces
+    ("vac_mhd22_132", "vacgpu_mhd22_132", "mhd", lambda g: random_fourier((40, 36), 68, g), 5 / 3, 15,
+     "tvdlf", "minmod", [WALL_X, WALL_X, "fixed", "fixed"], True),
+    ("vac_mhd22_132", "vacgpu_mhd22_132", "mhd", lambda g: random_fourier((36, 44), 69, g), 5 / 3, 12,
+     "tvd", "woodward", ["periodic", "periodic", WALL_Y, WALL_Y], True),
 ]
 
 
-@pytest.mark.parametrize("case", CASES, ids=[f"{c[6]}-{c[7]}-{c[2]}{len(c[3](1.4)[1])}d" for c in CASES])
+@pytest.mark.parametrize("case", CASES, ids=[f"{c[6]}-{c[7]}-{c[2]}{len(c[3](1.4)[1])}d"
+                                             + ("" if isinstance(c[8], str) else "-walls")
+                                             for c in CASES])
 def test_vac_with_gpu_advect_matches_vac(case):
     cpu, gpu, physics, build, gamma, steps, typefull, limiter, bc, divbfix = case
     if not (have_ref(cpu) and have_ref(gpu)):
@@ -45,8 +54,9 @@ def test_vac_with_gpu_advect_matches_vac(case):
     w0, coords = build(gamma)
     nw = w0.shape[0]
     ini = make_ini(w0, coords, physics, gamma, name="dropin")
+    typeB = [bc] * (2 * len(coords)) if isinstance(bc, str) else bc
     par = problem_par(nw, NAMES[nw], steps, typefull=typefull, limiter=limiter,
-                      typeB=[bc] * (2 * len(coords)), divbfix=divbfix, every_step=True)
+                      typeB=typeB, divbfix=divbfix, every_step=True)
     print(f"case {cpu} vs {gpu} {typefull} {limiter} {bc}", flush=True)
     rc = run_ref(cpu, ini, par, timeout=300)
     rg = run_ref(gpu, ini, par, timeout=300)

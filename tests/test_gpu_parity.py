@@ -124,6 +124,11 @@ def test_host_loop_through_advect_matches_run(typefull, limiter):
     assert_bitwise(wa, wr, "host loop vs oracle")
 
 
+WALL_X = ["symm", "asymm", "symm", "symm", "asymm", "symm"]
+WALL_Y = ["symm", "symm", "asymm", "symm", "symm", "asymm"]
+WALL_Z = ["symm", "symm", "symm", "asymm", "symm", "symm", "symm", "asymm"]
+
+
 @pytest.mark.parametrize("case", [
     ("mhd", (20, 16, 12), "tvdlf", "minmod", "periodic", 8),
     ("mhd", (16, 12, 20), "tvd", "woodward", "periodic", 6),
@@ -137,6 +142,12 @@ def test_host_loop_through_advect_matches_run(typefull, limiter):
     ("mhd", (40, 56), "tvdmu", "superbee", "cont", 10),
     ("mhd", (12, 14, 10), "tvdmu", "minmod", "periodic", 5),
     ("mhd", (10, 12, 14), "tvdmu", "woodward", "cont", 5),
+    # walls (symm, with the normal momentum and field asymm), fixed faces
+    ("mhd", (72, 56), "tvdlf", "woodward", [WALL_X, WALL_X, WALL_Y, WALL_Y], 12),
+    ("mhd", (64, 48), "tvd", "minmod", ["fixed", "fixed", WALL_Y, WALL_Y], 10),
+    ("mhd", (50, 60), "tvdmu", "woodward", [WALL_X, "fixed", "periodic", "periodic"], 10),
+    ("hd", (80, 64), "tvdlf", "minmod", [["symm", "asymm", "symm", "symm"]] * 2 + ["fixed"] * 2, 15),
+    ("mhd", (14, 12, 16), "tvd", "woodward", ["periodic", "periodic", "fixed", "fixed", WALL_Z, WALL_Z], 5),
 ])
 def test_gpu_matches_oracle(case):
     physics, nx, typefull, limiter, bc, steps = case

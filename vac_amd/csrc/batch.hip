@@ -697,6 +697,10 @@ int vacgpu_batch_add(vacgpu_batch* b, const vacgpu_config* cfg, const vacgpu_met
   if (m->dimsplit_xy) return bfail(b, VACGPU_E_ARG, "the batch engine runs typedimsplit='xyyx'");
   if (m->courantpar <= 0.0) return bfail(b, VACGPU_E_ARG, "the batch engine needs courantpar > 0");
   if (vacgpu_supported(cfg, m) != 0) return bfail(b, VACGPU_E_ARG, "unsupported selectors (see vacgpu_supported)");
+  for (int f = 0; f < 4; ++f)
+    for (int iw = 0; iw < 8; ++iw)
+      if (cfg->typeB[f][iw] != VACGPU_CONT && cfg->typeB[f][iw] != VACGPU_PERIODIC)
+        return bfail(b, VACGPU_E_ARG, "the batch engine runs 'cont' and 'periodic' boundaries");
   const long long need = footprint(*cfg);
   if (b->used_bytes + need > b->budget) return bfail(b, VACGPU_E_NOMEM, "HBM budget exhausted");
   int s = -1;
@@ -714,8 +718,10 @@ int vacgpu_batch_add(vacgpu_batch* b, const vacgpu_config* cfg, const vacgpu_met
   const int nw = cfg->physics == VACGPU_MHD ? 6 : 4;
   I.mhd = cfg->physics == VACGPU_MHD;
   for (int f = 0; f < 6; ++f)
-    for (int iw = 0; iw < 8; ++iw)
+    for (int iw = 0; iw < 8; ++iw) {
       I.bc.t[f][iw] = (f < 4 && iw < nw) ? (unsigned char)(cfg->typeB[f][iw] == VACGPU_PERIODIC) : 0;
+      I.bc.k[f][iw] = BK_COPY;
+    }
   for (int i = 0; i < 8; ++i) I.sel.lim[i] = m->typelimiter[i];
   I.sel.divbfix = m->divbfix;
   I.sel.divbwave = m->divbwave;

@@ -33,9 +33,10 @@ struct Cg {
   int itr, maxit, info, done, typestop;
 };
 
-// boundscalar's type per face (src/vacpoisson.t:158-300): 1 'periodic', 0 'nul'
+// boundscalar's type per face (src/vacpoisson.t:158-300)
+enum { SB_NUL = 0, SB_PERIODIC = 1, SB_SYMM = 2, SB_ASYMM = 3, SB_GRAD0 = 4 };
 struct SBC {
-  unsigned char per[6];
+  unsigned char type[6];
 };
 
 // a / b with y the refined reciprocal of b that pdiv computes (rdx below):
@@ -60,26 +61,42 @@ struct RDX {
 };
 
 // the ixG coordinate whose value boundscalar leaves at coordinate c along k:
-// c itself on the mesh, the periodic image, or -1 for a 'nul' ghost (zero).
-// Mapping each coordinate on its own gives what VAC's face-by-face fill
-// gives, since every face region spans the full transverse ixG range.
-__device__ __forceinline__ int smap(const Geo& g, const SBC& sb, int k, int c) {
-  if (c < 2) return sb.per[2 * k] ? c + g.n[k] : -1;
-  if (c > g.n[k] + 1) return sb.per[2 * k + 1] ? c - g.n[k] : -1;
-  return c;
+// c itself on the mesh, the periodic image, the mirror image ('symm', and
+// 'asymm' with neg toggled), the edge cell ('grad0', second ghost layer), or
+// -1 for a zero ('nul'; 'grad0', first ghost layer).  These are copies, sign
+// changes and zeros, so mapping each coordinate on its own gives what VAC's
+// face-by-face fill gives (every face region spans the full transverse ixG
+// range).
+__device__ __forceinline__ int smap(const Geo& g, const SBC& sb, int k, int c, bool& neg) {
+  const int n = g.n[k];
+  int f;
+  if (c < 2) f = 2 * k;
+  else if (c > n + 1) f = 2 * k + 1;
+  else return c;
+  const bool up = f & 1;
+  switch (sb.type[f]) {
+    case SB_PERIODIC: return up ? c - n : c + n;
+    case SB_ASYMM: neg = !neg; return up ? 2 * n + 3 - c : 3 - c;
+    case SB_SYMM: return up ? 2 * n + 3 - c : 3 - c;
+    case SB_GRAD0:
+      if (c == (up ? n + 2 : 1)) return -1;
+      return up ? n + 1 : 2;
+    default: return -1;
+  }
 }
 
 template <int ND>
 __device__ __forceinline__ double sval(const Geo& g, const SBC& sb, const double* __restrict__ q,
                                        const int* c) {
   long long idx = g.off;
+  bool neg = false;
 #pragma unroll
   for (int k = 0; k < ND; ++k) {
-    const int m = smap(g, sb, k, c[k]);
+    const int m = smap(g, sb, k, c[k], neg);
     if (m < 0) return 0.0;
     idx += (long long)m * g.s[k];
   }
-  return q[idx];
+  return neg ? -q[idx] : q[idx];
 }
 
 // sum over the block (all threads call; valid in thread 0), fixed order
@@ -209,13 +226,15 @@ __global__ __launch_bounds__(256) void k_cg_matvec(Geo g, SBC sb, RDX rx,
   // the search direction at ixG point c (mapped as boundscalar does)
   auto zdir = [&](const int* c) -> double {
     long long idx = g.off;
+    bool neg = false;
 #pragma unroll
     for (int k = 0; k < ND; ++k) {
-      const int m = smap(g, sb, k, c[k]);
+      const int m = smap(g, sb, k, c[k], neg);
       if (m < 0) return 0.0;
       idx += (long long)m * g.s[k];
     }
-    return first ? zo[idx] : rhs[idx] + bet * zo[idx];
+    const double v = first ? zo[idx] : rhs[idx] + bet * zo[idx];
+    return neg ? -v : v;
   };
   const CellIx cl = cell_of(g);
   double pz = 0.0;

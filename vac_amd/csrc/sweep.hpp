@@ -36,8 +36,13 @@ struct Sel {
   double courantpar, gamma;
 };
 
+// boundary types per (face, variable): t = 1 for 'periodic'; k the kind of a
+// non-periodic face: a copy of the edge ('cont'), a mirror image ('symm', or
+// 'asymm' with the sign flipped) or the values the first fill left ('fixed')
+enum { BK_COPY = 0, BK_SYMM = 1, BK_ASYMM = 2, BK_FIXED = 3 };
 struct BC {
   unsigned char t[6][8];
+  unsigned char k[6][8];
 };
 
 // device-resident scalars
@@ -130,15 +135,23 @@ __device__ __forceinline__ double divb_at(const Geo& g, const double* __restrict
 }
 
 // ------------------------------------------------------------ ghost cells
-// getboundary for 'periodic' and 'cont' (src/vacgrid.t:195-413).  VAC fills
+// getboundary (src/vacgrid.t:195-413) for 'cont', 'periodic', 'symm',
+// 'asymm' and 'fixed' on slab grids, one thread per ghost cell.  VAC fills
 // face regions in the order x-low, x-high, y-low, ... over the full
-// transverse ixG range; since both types copy, every ghost cell ends up
-// equal to the mesh cell obtained by mapping each coordinate on its own,
-// which is what this kernel writes (one thread per ghost cell).
+// transverse ixG range, so a ghost cell holds what its last face (the highest
+// dimension along which it is a ghost) puts there: a copy of another cell --
+// the edge ('cont'), the periodic image, the mirror image ('symm'; times -1
+// for 'asymm', :317-352) -- which, if still a ghost along a lower dimension,
+// got its value from that dimension's face earlier in the same fill; or, for
+// a 'fixed' face, the value the first fill left there (fix, a copy of the
+// state after that fill, :372-404; null during the first fill itself, when
+// 'fixed' copies like 'cont').  For copies alone this is the mesh cell
+// reached by mapping each coordinate on its own.
 template <int NW>
 __device__ __forceinline__ void bound_body(const Geo& g, const BC& bc, double* __restrict__ w,
                                            long long t, long long slab0, long long slab1,
-                                           int iw0 = 0, int iw1 = NW) {
+                                           int iw0 = 0, int iw1 = NW,
+                                           const double* __restrict__ fix = nullptr) {
   int d;
   if (t < slab0) d = 0;
   else if (t < slab0 + slab1) { d = 1; t -= slab0; }
@@ -156,14 +169,31 @@ __device__ __forceinline__ void bound_body(const Geo& g, const BC& bc, double* _
 #pragma unroll
   for (int iw = 0; iw < NW; ++iw) {
     if (iw < iw0 || iw >= iw1) continue;
-    long long src = g.off;
-    for (int k = 0; k < g.nd; ++k) {
-      int s = c[k];
-      if (c[k] < 2) s = bc.t[2 * k][iw] ? c[k] + g.n[k] : 2;
-      else if (c[k] > g.n[k] + 1) s = bc.t[2 * k + 1][iw] ? c[k] - g.n[k] : g.n[k] + 1;
-      src += s * g.s[k];
+    int cc[3] = {c[0], c[1], c[2]};
+    bool neg = false;
+    const double* from = w;
+    for (int k = g.nd - 1; k >= 0; --k) {
+      const int ck = cc[k], n = g.n[k];
+      int f;
+      if (ck < 2) f = 2 * k;
+      else if (ck > n + 1) f = 2 * k + 1;
+      else continue;
+      const int kind = bc.k[f][iw];
+      if (bc.t[f][iw]) {
+        cc[k] = (f & 1) ? ck - n : ck + n;
+      } else if (kind == BK_FIXED && fix) {
+        from = fix;
+        break;
+      } else if (kind == BK_SYMM || kind == BK_ASYMM) {
+        cc[k] = (f & 1) ? 2 * n + 3 - ck : 3 - ck;
+        if (kind == BK_ASYMM) neg = !neg;
+      } else {
+        cc[k] = (f & 1) ? n + 1 : 2;
+      }
     }
-    w[iw * g.ps + dst] = w[iw * g.ps + src];
+    const long long src = g.off + cc[0] + cc[1] * g.s[1] + cc[2] * g.s[2];
+    const double v = from[iw * g.ps + src];
+    w[iw * g.ps + dst] = neg ? -v : v;
   }
 }
 
@@ -187,10 +217,10 @@ inline long long ghost_slabs(const Geo& g, long long* slab) {
 template <int NW>
 __global__ __launch_bounds__(256) void k_bound(Geo g, BC bc, double* __restrict__ w,
                                               long long total, long long slab0, long long slab1,
-                                              int iw0, int iw1) {
+                                              int iw0, int iw1, const double* __restrict__ fix) {
   long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= total) return;
-  bound_body<NW>(g, bc, w, t, slab0, slab1, iw0, iw1);
+  bound_body<NW>(g, bc, w, t, slab0, slab1, iw0, iw1, fix);
 }
 
 // ------------------------------------------------------------ TVDLF

@@ -55,6 +55,10 @@ struct vacgpu_grid {
   // for all variables; use_fused selects them (default on)
   bool fused_ok = false, use_fused = true;
   bool bc_uniform = false;  // one boundary type per face for all variables, n >= 4
+  // 'fixed' faces: a copy of the state after the first ghost fill, whose
+  // ghost values those faces keep (null until the first vacgpu_put_state)
+  bool any_fixed = false;
+  double* fixw = nullptr;
   int fs_seglen = 16;
   dim3 grid_fs, block_fs;
   int ry_seglen = 16;  // k_ryu_ab_y (2D one-pass Ryu step)
@@ -172,8 +176,8 @@ int check_config(const vacgpu_config* c, std::string& why) {
   int nw = c->physics == VACGPU_MHD ? 2 + 2 * c->ndim : 2 + c->ndim;
   for (int f = 0; f < 2 * c->ndim; ++f)
     for (int iw = 0; iw < nw; ++iw)
-      if (c->typeB[f][iw] != VACGPU_CONT && c->typeB[f][iw] != VACGPU_PERIODIC) {
-        why = "typeB must be 'cont' or 'periodic'";
+      if (c->typeB[f][iw] < VACGPU_CONT || c->typeB[f][iw] > VACGPU_FIXED) {
+        why = "typeB must be 'cont', 'periodic', 'symm', 'asymm' or 'fixed'";
         return VACGPU_E_ARG;
       }
   for (int d = 0; d < c->ndim; ++d)
@@ -239,7 +243,7 @@ int bound(vacgpu_grid* g, double* w, int iw0 = 0, int iw1 = 8) {
   if (g->nbound == 0) return 0;
   Timed t(g, KF_BOUND);
   const unsigned nb = (unsigned)((g->nbound + 255) / 256);
-#define BOUND_NW(NWV) hipLaunchKernelGGL(k_bound<NWV>, dim3(nb), dim3(256), 0, g->stream, g->geo, g->bc, w, g->nbound, g->slab[0], g->slab[1], iw0, iw1)
+#define BOUND_NW(NWV) hipLaunchKernelGGL(k_bound<NWV>, dim3(nb), dim3(256), 0, g->stream, g->geo, g->bc, w, g->nbound, g->slab[0], g->slab[1], iw0, iw1, g->fixw)
   switch (g->nw) {
     case 4: BOUND_NW(4); break;
     case 5: BOUND_NW(5); break;
@@ -687,13 +691,19 @@ int sweep_device(vacgpu_grid* g, const vacgpu_method* m, int d, int firstsweep, 
   return 0;
 }
 
-// boundscalar's face types: those of the normal B component
-// (src/vacproc.projectb.t:82-101; 'cont' falls to the default 'nul')
+// boundscalar's face types from the normal B component's (typeBscalar,
+// src/vacproc.projectb.t:82-101): periodic -> periodic, symm -> asymm,
+// asymm -> symm, fixed -> grad0, cont -> nul (the default case)
 SBC make_sbc(const vacgpu_grid* g) {
   SBC sb;
   const int nd = g->cfg.ndim;
-  for (int f = 0; f < 6; ++f)
-    sb.per[f] = (f < 2 * nd && g->cfg.typeB[f][2 + nd + f / 2] == VACGPU_PERIODIC) ? 1 : 0;
+  for (int f = 0; f < 6; ++f) {
+    const int tb = f < 2 * nd ? g->cfg.typeB[f][2 + nd + f / 2] : VACGPU_CONT;
+    sb.type[f] = (unsigned char)(tb == VACGPU_PERIODIC ? SB_PERIODIC
+                                 : tb == VACGPU_SYMM ? SB_ASYMM
+                                 : tb == VACGPU_ASYMM ? SB_SYMM
+                                 : tb == VACGPU_FIXED ? SB_GRAD0 : SB_NUL);
+  }
   return sb;
 }
 
@@ -922,7 +932,14 @@ int vacgpu_create(const vacgpu_config* cfg, const double* dx0, const double* dx1
   geo.ps = ((geo.off + px * rows + 31) / 32) * 32;
   for (int f = 0; f < 6; ++f)
     for (int iw = 0; iw < 8; ++iw)
-      g->bc.t[f][iw] = (f < 2 * nd && iw < g->nw) ? (unsigned char)(cfg->typeB[f][iw] == VACGPU_PERIODIC) : 0;
+    {
+      const int tb = (f < 2 * nd && iw < g->nw) ? cfg->typeB[f][iw] : VACGPU_CONT;
+      g->bc.t[f][iw] = (unsigned char)(tb == VACGPU_PERIODIC);
+      g->bc.k[f][iw] = (unsigned char)(tb == VACGPU_SYMM ? BK_SYMM
+                                       : tb == VACGPU_ASYMM ? BK_ASYMM
+                                       : tb == VACGPU_FIXED ? BK_FIXED : BK_COPY);
+      if (tb == VACGPU_FIXED) g->any_fixed = true;
+    }
   // ghost-cell enumeration: slab d has ghost coordinate d, mesh coordinates
   // below d and full ixG coordinates above d
   long long tot = 0;
@@ -953,8 +970,12 @@ int vacgpu_create(const vacgpu_config* cfg, const double* dx0, const double* dx1
   g->bc_uniform = true;
   for (int d = 0; d < nd; ++d) g->bc_uniform &= geo.n[d] >= 4;
   for (int f = 0; f < 2 * nd; ++f)
-    for (int iw = 1; iw < g->nw; ++iw)
+    for (int iw = 0; iw < g->nw; ++iw) {
       if (g->bc.t[f][iw] != g->bc.t[f][0]) g->bc_uniform = false;
+      // kernels that write ghost cells as copies of mesh cells, or read them
+      // through that mapping, need copy-type faces
+      if (g->bc.k[f][iw] != BK_COPY) g->bc_uniform = false;
+    }
   g->fused_ok = nd == 2 && g->bc_uniform && (unsigned long long)geo.ps * 8ULL < (1ULL << 32);
   set_fused_shape(g, 0);
   if (nd == 2) set_ryu_shape(g);
@@ -1084,6 +1105,13 @@ int vacgpu_put_state(vacgpu_grid* g, const double* w, const int* ld) {
   g->have_state = true;
   // VAC fills the ghost cells once after reading the initial state (src/vac.t:54)
   if ((rc = bound(g, g->buf[g->cur]))) return rc;
+  if (g->any_fixed && !g->fixw) {
+    // that first fill copies 'fixed' faces like 'cont'; their ghost values
+    // stay what it left (src/vacgrid.t:372-404)
+    const size_t bytes = sizeof(double) * (size_t)g->geo.ps * g->nw;
+    HIPCHK(g, hipMalloc(&g->fixw, bytes));
+    HIPCHK(g, hipMemcpyAsync(g->fixw, g->buf[g->cur], bytes, hipMemcpyDeviceToDevice, g->stream));
+  }
   HIPCHK(g, hipStreamSynchronize(g->stream));
   return 0;
 }
@@ -1394,6 +1422,7 @@ void vacgpu_destroy(vacgpu_grid* g) {
   if (g->stage) (void)hipHostFree(g->stage);
   if (g->part) (void)hipFree(g->part);
   if (g->ctr) (void)hipFree(g->ctr);
+  if (g->fixw) (void)hipFree(g->fixw);
   {
     auto& P = g->pr;
     double* planes[8] = {P.rhs, P.phi, P.z, P.z2, P.az, P.rdx[0], P.rdx[1], P.rdx[2]};

@@ -96,6 +96,10 @@ if(.not.created)then
       do iw=1,nw
          code=vacgpu_bc_code(typeB(iw,iB))
          if(code<0) call die('vacgpu advect: typeB='//typeB(iw,iB)//' is not on the GPU path')
+         ! 'fixed' keeps the first fill's copies; fullgridini=T would keep
+         ! the file's ghost values instead (src/vacgrid.t:249-257)
+         if(code==VACGPU_FIXED .and. fullgridini) &
+            call die('vacgpu advect: typeB=fixed with fullgridini=T is not on the GPU path')
          cfg%typeB(iw,iB)=code
       end do
    end do

@@ -16,7 +16,8 @@ module vacgpu_mod
   integer(c_int), parameter :: VACGPU_HD = 0, VACGPU_MHD = 1
   integer(c_int), parameter :: VACGPU_TVDLF = 0, VACGPU_TVD = 1, VACGPU_TVDMU = 2
   integer(c_int), parameter :: VACGPU_MINMOD = 0, VACGPU_WOODWARD = 1, VACGPU_SUPERBEE = 2
-  integer(c_int), parameter :: VACGPU_CONT = 0, VACGPU_PERIODIC = 1
+  integer(c_int), parameter :: VACGPU_CONT = 0, VACGPU_PERIODIC = 1, VACGPU_SYMM = 2, &
+       VACGPU_ASYMM = 3, VACGPU_FIXED = 4
   integer(c_int), parameter :: VACGPU_FP64 = 0, VACGPU_FP32BULK = 1
 
   type, bind(C) :: vacgpu_config
@@ -255,6 +256,12 @@ contains
        vacgpu_bc_code = VACGPU_CONT
     case ('periodic')
        vacgpu_bc_code = VACGPU_PERIODIC
+    case ('symm')
+       vacgpu_bc_code = VACGPU_SYMM
+    case ('asymm')
+       vacgpu_bc_code = VACGPU_ASYMM
+    case ('fixed')
+       vacgpu_bc_code = VACGPU_FIXED
     case default
        vacgpu_bc_code = -1
     end select

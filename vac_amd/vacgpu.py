@@ -22,7 +22,8 @@ LIB_PATH = os.path.join(HERE, "libvacgpu.so")
 HD, MHD = 0, 1
 TVDLF, TVD, TVDMU = 0, 1, 2
 LIMITERS = {"minmod": 0, "woodward": 1, "superbee": 2}
-CONT, PERIODIC = 0, 1
+CONT, PERIODIC, SYMM, ASYMM, FIXED = 0, 1, 2, 3, 4
+BC_CODES = {"cont": CONT, "periodic": PERIODIC, "symm": SYMM, "asymm": ASYMM, "fixed": FIXED}
 FP64, FP32BULK = 0, 1
 KERNELS = {"tvd_sweep": 0, "tvdlf_pred": 1, "tvdlf_corr": 2, "ryu": 3, "bound": 4, "reduce": 5,
            "tvdlf_fused": 6, "projectb": 7}
@@ -161,7 +162,9 @@ def make_config(nx, physics: str, gamma: float, bc="periodic", precision: str = 
         spec = bcs[f]
         for iw in range(8):
             name = spec if isinstance(spec, str) else spec[min(iw, len(spec) - 1)]
-            cfg.typeB[f][iw] = PERIODIC if name == "periodic" else CONT
+            if name not in BC_CODES:
+                raise ValueError(f"typeB {name!r} is not on the GPU path")
+            cfg.typeB[f][iw] = BC_CODES[name]
     cfg.precision = FP32BULK if precision == "fp32bulk" else FP64
     return cfg
 
