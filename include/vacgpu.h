@@ -74,6 +74,11 @@ int vacgpu_set_stream(vacgpu_grid* g, void* hip_stream);
 /* Copy the full ixG state in / out (src/vacio.t readfileini / savefile hand-off).
  * ld[d] is the leading dimension of the caller's array along d (VAC: ixGhi^D). */
 int vacgpu_put_state(vacgpu_grid* g, const double* w, const int* ld);
+/* As vacgpu_put_state, but the caller's ghost cells are kept as they are
+ * (VAC's w always carries them, src/vac.t:54; after a projection they need
+ * not be copies of mesh cells, src/vacproc.projectb.t:111-144), and the next
+ * sweep reads them from memory.  The drop-in's transfer of VAC's w. */
+int vacgpu_put_state_ghosts(vacgpu_grid* g, const double* w, const int* ld);
 int vacgpu_get_state(vacgpu_grid* g, double* w, const int* ld);
 
 /* getdt_courant (src/vac.t:664-722).  dtcourant (ndim, in/out) and smallp

@@ -78,6 +78,12 @@ if [ "$MODE" = gpu ]; then
   perl -pi -e 's/^subroutine advect\(method/subroutine advect_cpu(method/' vac.t
   $FC $FFLAGS -c vacgpu_mod.f90 -o vacgpu_mod.o
   MODS="$MODS vacgpu_advect"
+  if [ "$POISSON" = poisson ]; then
+    # VAC's projectb is renamed and unused: every projection runs on the device
+    cp "$REPO/vac_amd/fortran/vacgpu_projectb.t" .
+    perl -pi -e 's/^subroutine projectb\(w\)/subroutine projectb_cpu(w)/' vacproc.projectb.t
+    MODS="$MODS vacgpu_projectb"
+  fi
   LINK="vacgpu_mod.o -L$REPO/vac_amd -lvacgpu -Wl,-rpath,\$ORIGIN/../../vac_amd -L/opt/rocm/lib -Wl,-rpath,/opt/rocm/lib"
 fi
 for m in vacdef vacpar vacusrpar $MODS; do

@@ -69,31 +69,36 @@ def test_vac_with_gpu_advect_matches_vac(case):
 
 
 PROJ_CASES = [
-    # (physics builder, gamma, steps, typefull, limiter, bc, divbfix, divbwave, procpar)
+    # (physics builder, gamma, steps, typefull, limiter, bc, divbfix, divbwave, procpar, nproc)
     # the reference's Orszag-Tang recipe (Man/par.html): TVD + woodward,
     # divbfix=F, divbwave=F, nproc=1, procpar=0,0.01,1
     (lambda g: orszag_tang(64, g), 5 / 3, 30, "tvd", "woodward", "periodic", False, False,
-     (0.0, 0.01, 1.0)),
+     (0.0, 0.01, 1.0), "1"),
     (lambda g: random_fourier((40, 48), 67, g), 1.4, 15, "tvdlf", "minmod", "cont", True, True,
-     (0.01, 0.0, 3.0)),
+     (0.01, 0.0, 3.0), "1"),
+    # as the reference's par/peak22proj: also processed before every save
+    # (nproc(4) = 1; this test saves every step), cont faces
+    (lambda g: random_fourier((44, 40), 70, g), 5 / 3, 12, "tvd", "woodward", "cont", True, True,
+     (-1.0, -1.0, -1.0), "1,0,0,1"),
 ]
 
 
-@pytest.mark.parametrize("case", PROJ_CASES, ids=[f"{c[3]}-{c[5]}-proj" for c in PROJ_CASES])
+@pytest.mark.parametrize("case", PROJ_CASES, ids=[f"{c[3]}-{c[5]}-proj-nproc{c[9]}" for c in PROJ_CASES])
 def test_vac_with_gpu_advect_and_projection_matches_vac(case):
-    # VAC built with its Poisson module, plain and with the drop-in: the
-    # drop-in projects on the device (vacgpu_process) where VAC's advect calls
-    # process(1).  The CG dot products are summed in another order on the
-    # device, so the results agree to rounding: relative L1 <= 1e-12 per
-    # variable and the t of every step to 1e-12 relative.
-    build, gamma, steps, typefull, limiter, bc, divbfix, divbwave, procpar = case
+    # VAC built with its Poisson module, plain and with the drop-in, whose
+    # projectb projects on the device (vacgpu_process) wherever VAC's process
+    # calls it: before a step (process(1) in advect) and before a save.  The
+    # CG dot products are summed in another order on the device, so the
+    # results agree to rounding: relative L1 <= 1e-12 per variable and the t
+    # of every step to 1e-12 relative.
+    build, gamma, steps, typefull, limiter, bc, divbfix, divbwave, procpar, nproc = case
     cpu, gpu = "vacp_mhd22_132", "vacgpup_mhd22_132"
     if not (have_ref(cpu) and have_ref(gpu)):
         pytest.skip("reference executables not built (oracle/build_ref.sh ... poisson)")
     w0, coords = build(gamma)
     nw = w0.shape[0]
     ini = make_ini(w0, coords, "mhd", gamma, name="dropin")
-    extra = " nproc=1\n procpar=" + ",".join(repr(float(v)) for v in procpar) + "\n"
+    extra = f" nproc={nproc}\n procpar=" + ",".join(repr(float(v)) for v in procpar) + "\n"
     par = problem_par(nw, NAMES[nw], steps, typefull=typefull, limiter=limiter,
                       typeB=[bc] * (2 * len(coords)), divbfix=divbfix, divbwave=divbwave,
                       every_step=True, extra_method=extra)

@@ -1116,6 +1116,25 @@ int vacgpu_put_state(vacgpu_grid* g, const double* w, const int* ld) {
   return 0;
 }
 
+int vacgpu_put_state_ghosts(vacgpu_grid* g, const double* w, const int* ld) {
+  if (!g || !w) return VACGPU_E_ARG;
+  int rc = copy_state(g, const_cast<double*>(w), ld, true);
+  if (rc) return rc;
+  g->old = g->cur;
+  g->have_state = true;
+  // the caller's ghost cells need not be copies of mesh cells (a projection
+  // leaves its own there): the next sweep reads them from memory
+  g->ghosts_dirty = true;
+  if (g->any_fixed && !g->fixw) {
+    // the values of 'fixed' faces are those of the first state put here
+    const size_t bytes = sizeof(double) * (size_t)g->geo.ps * g->nw;
+    HIPCHK(g, hipMalloc(&g->fixw, bytes));
+    HIPCHK(g, hipMemcpyAsync(g->fixw, g->buf[g->cur], bytes, hipMemcpyDeviceToDevice, g->stream));
+  }
+  HIPCHK(g, hipStreamSynchronize(g->stream));
+  return 0;
+}
+
 int vacgpu_get_state(vacgpu_grid* g, double* w, const int* ld) {
   if (!g || !w) return VACGPU_E_ARG;
   if (!g->have_state) return fail(g, VACGPU_E_STATE, "no state on the device (call vacgpu_put_state)");

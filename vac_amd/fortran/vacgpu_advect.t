@@ -28,16 +28,11 @@ double precision:: w1(ixG^T,nw),w(ixG^T,nw)
 logical:: firstsweep,lastsweep
 common/first/firstsweep,lastsweep
 
-type(c_ptr), save :: grid
-logical, save :: created
-data created /.false./
-type(vacgpu_config), save :: cfg
+type(c_ptr) :: grid
 type(vacgpu_method) :: m
 integer(c_int) :: rc
-integer(c_int), target :: ld(3), ne(4), pinfo, piter
-real(c_double), target :: pp(3), pres
-integer:: iw, iB, code, ngx(3), mlo(3), mhi(3)
-double precision, save, target :: dxl(ixGhimax,3)
+integer(c_int), target :: ld(3), ne(4)
+integer:: iw, code
 real(c_double), target :: dtc(3), sp
 !-----------------------------------------------------------------------------
 
@@ -76,45 +71,8 @@ end select
 if(ndim<3) ld(3)=1
 if(ndim<2) ld(2)=1
 
-if(.not.created)then
-   cfg%ndim=ndim
-   select case(typephys)
-   case('mhd')
-      cfg%physics=VACGPU_MHD
-   case('hd')
-      cfg%physics=VACGPU_HD
-   case default
-      call die('vacgpu advect: physics '//typephys//' is not on the GPU path')
-   end select
-   cfg%nx=1
-   ^D&cfg%nx(^D)=ixMmax^D-ixMmin^D+1;
-   cfg%eqpar=zero
-   cfg%eqpar(1:min(2,neqpar))=eqpar(1:min(2,neqpar))
-   if(nB/=2*ndim) call die('vacgpu advect: only the default boundary regions are supported')
-   cfg%typeB=VACGPU_CONT
-   do iB=1,nB
-      do iw=1,nw
-         code=vacgpu_bc_code(typeB(iw,iB))
-         if(code<0) call die('vacgpu advect: typeB='//typeB(iw,iB)//' is not on the GPU path')
-         ! 'fixed' keeps the first fill's copies; fullgridini=T would keep
-         ! the file's ghost values instead (src/vacgrid.t:249-257)
-         if(code==VACGPU_FIXED .and. fullgridini) &
-            call die('vacgpu advect: typeB=fixed with fullgridini=T is not on the GPU path')
-         cfg%typeB(iw,iB)=code
-      end do
-   end do
-   cfg%precision=VACGPU_FP64
-   ngx=1
-   mlo=1
-   mhi=1
-   ^D&ngx(^D)=ixGmax^D;
-   ^D&mlo(^D)=ixMmin^D;
-   ^D&mhi(^D)=ixMmax^D;
-   call vacgpu_dxlines(dx,ld,ndim,ngx,mlo,mhi,dxl,ixGhimax)
-   rc=vacgpu_create(cfg,c_loc(dxl(1,1)),c_loc(dxl(1,2)),c_loc(dxl(1,3)),grid)
-   if(rc/=0) call die('vacgpu advect: vacgpu_create failed (GPU missing or unsupported grid)')
-   created=.true.
-endif
+call vacgpu_dropin_init
+grid=vacgpu_dropin_grid
 
 m%typelimiter=VACGPU_MINMOD
 do iw=1,nw
@@ -131,22 +89,19 @@ if(implpar>=zero) m%courantpar=-one
 m%smallpcoeff=smallpcoeff
 
 if(firstsweep)then
-   rc=vacgpu_put_state(grid,w,ld)
+   rc=vacgpu_put_state_ghosts(grid,w,ld)
    if(rc/=0) call die('vacgpu advect: '//vacgpu_errmsg(grid))
 endif
 
-! process(1) when due, as VAC's advect does (src/vac.t:392-399): the
-! projection of B on the device
+! process(1) when due, as VAC's advect does (src/vac.t:392-399).  VAC's
+! process calls projectb, which a build with the Poisson module replaces by
+! the device projection (vacgpu_projectb.t); inside advect it works on the
+! device state put there above.
 if(nproc(1)/=0.and.implpar<=zero)then
    if(nproc(1)<0.or.(firstsweep.and.it-itmin==((it-itmin)/nproc(1))*nproc(1)))then
-      if(typephys/='mhd' .or. ndim<2) &
-         call die('vacgpu advect: processing (projection of B) needs 2D or 3D MHD')
-      pp=-one
-      pp(1:min(3,nprocpar))=procpar(1:min(3,nprocpar))
-      rc=vacgpu_process(grid,1,c_loc(pp),c_loc(pinfo),c_loc(piter),c_loc(pres))
-      if(rc/=0) call die('vacgpu advect: '//vacgpu_errmsg(grid))
-      ! nerror(poissonerr_) as poisson() counts it (src/vacpoisson.t:79-81)
-      if(pinfo/=0.and.pinfo/=3) nerror(poissonerr_)=nerror(poissonerr_)+1
+      vacgpu_in_advect=.true.
+      call process(1,idim^LIM,w)
+      vacgpu_in_advect=.false.
    endif
 endif
 
@@ -167,6 +122,71 @@ if(lastsweep)then
 endif
 
 firstsweep=.false.
+
+return
+end
+
+!=============================================================================
+subroutine vacgpu_dropin_init
+
+! Creates the device grid from VAC's common block (ixG, ixM, dx, typeB,
+! eqpar) on the first call: from the first advect, or from the first
+! projection when a save comes first.
+
+use iso_c_binding
+use vacgpu_mod
+
+include 'vacdef.f'
+
+type(vacgpu_config), save :: cfg
+type(c_ptr) :: grid
+integer(c_int) :: rc
+integer(c_int) :: ld(3)
+integer:: iw, iB, code, ngx(3), mlo(3), mhi(3)
+double precision, save, target :: dxl(ixGhimax,3)
+!-----------------------------------------------------------------------------
+
+if(c_associated(vacgpu_dropin_grid)) return
+^D&ld(^D)=ixGhi^D;
+if(ndim<3) ld(3)=1
+if(ndim<2) ld(2)=1
+cfg%ndim=ndim
+select case(typephys)
+case('mhd')
+   cfg%physics=VACGPU_MHD
+case('hd')
+   cfg%physics=VACGPU_HD
+case default
+   call die('vacgpu advect: physics '//typephys//' is not on the GPU path')
+end select
+cfg%nx=1
+^D&cfg%nx(^D)=ixMmax^D-ixMmin^D+1;
+cfg%eqpar=zero
+cfg%eqpar(1:min(2,neqpar))=eqpar(1:min(2,neqpar))
+if(nB/=2*ndim) call die('vacgpu advect: only the default boundary regions are supported')
+cfg%typeB=VACGPU_CONT
+do iB=1,nB
+   do iw=1,nw
+      code=vacgpu_bc_code(typeB(iw,iB))
+      if(code<0) call die('vacgpu advect: typeB='//typeB(iw,iB)//' is not on the GPU path')
+      ! 'fixed' keeps the first fill's copies; fullgridini=T would keep
+      ! the file's ghost values instead (src/vacgrid.t:249-257)
+      if(code==VACGPU_FIXED .and. fullgridini) &
+         call die('vacgpu advect: typeB=fixed with fullgridini=T is not on the GPU path')
+      cfg%typeB(iw,iB)=code
+   end do
+end do
+cfg%precision=VACGPU_FP64
+ngx=1
+mlo=1
+mhi=1
+^D&ngx(^D)=ixGmax^D;
+^D&mlo(^D)=ixMmin^D;
+^D&mhi(^D)=ixMmax^D;
+call vacgpu_dxlines(dx,ld,ndim,ngx,mlo,mhi,dxl,ixGhimax)
+rc=vacgpu_create(cfg,c_loc(dxl(1,1)),c_loc(dxl(1,2)),c_loc(dxl(1,3)),grid)
+if(rc/=0) call die('vacgpu advect: vacgpu_create failed (GPU missing or unsupported grid)')
+vacgpu_dropin_grid=grid
 
 return
 end

@@ -20,6 +20,11 @@ module vacgpu_mod
        VACGPU_ASYMM = 3, VACGPU_FIXED = 4
   integer(c_int), parameter :: VACGPU_FP64 = 0, VACGPU_FP32BULK = 1
 
+  ! the drop-in's device grid (made by the first advect or projection), and
+  ! whether a process() call comes from inside advect
+  type(c_ptr), save :: vacgpu_dropin_grid = c_null_ptr
+  logical, save :: vacgpu_in_advect = .false.
+
   type, bind(C) :: vacgpu_config
      integer(c_int) :: ndim
      integer(c_int) :: physics
@@ -59,6 +64,13 @@ module vacgpu_mod
        real(c_double), intent(in) :: w(*)
        integer(c_int), intent(in) :: ld(*)
      end function vacgpu_put_state
+
+     integer(c_int) function vacgpu_put_state_ghosts(grid, w, ld) bind(C, name='vacgpu_put_state_ghosts')
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: grid
+       real(c_double), intent(in) :: w(*)
+       integer(c_int), intent(in) :: ld(*)
+     end function vacgpu_put_state_ghosts
 
      integer(c_int) function vacgpu_get_state(grid, w, ld) bind(C, name='vacgpu_get_state')
        import :: c_int, c_ptr, c_double

@@ -50,6 +50,7 @@ SIGNATURES = {
     "vacgpu_create": ([C.POINTER(Config), _DP, _DP, _DP, C.POINTER(C.c_void_p)], C.c_int),
     "vacgpu_set_stream": ([C.c_void_p, C.c_void_p], C.c_int),
     "vacgpu_put_state": ([C.c_void_p, _DP, _IP], C.c_int),
+    "vacgpu_put_state_ghosts": ([C.c_void_p, _DP, _IP], C.c_int),
     "vacgpu_get_state": ([C.c_void_p, _DP, _IP], C.c_int),
     "vacgpu_getdt_courant": ([C.c_void_p, C.POINTER(Method), _DP, _DP, _DP], C.c_int),
     "vacgpu_advect": ([C.c_void_p, C.POINTER(Method), C.c_int, C.c_int, C.c_double, C.c_int,
@@ -219,6 +220,13 @@ class Grid:
         if w.shape != self.shape_full:
             raise ValueError(f"state shape {w.shape} != {self.shape_full}")
         self._check(self.lib.vacgpu_put_state(self.h, _ptr(w), None), "put_state")
+
+    def put_state_ghosts(self, w_full: np.ndarray):
+        """Put a full ixG state keeping its ghost cells as they are."""
+        w = np.ascontiguousarray(w_full, dtype=np.float64)
+        if w.shape != self.shape_full:
+            raise ValueError(f"state shape {w.shape} != {self.shape_full}")
+        self._check(self.lib.vacgpu_put_state_ghosts(self.h, _ptr(w), None), "put_state_ghosts")
 
     def put_mesh(self, w_mesh: np.ndarray):
         """Put a mesh-only state; ghost cells are filled by the first call."""
