@@ -10,6 +10,7 @@ Each entry: name -> (setvac -d value, physics, compile-time grid ixGhi).
 BINARIES = {
     "vac_mhd22_132": ("22", "mhd", "132,132"),
     "vac_hd22_132": ("22", "hd", "132,132"),
+    "vac_hd33_36": ("33", "hd", "36,36,36"),
     "vac_mhd33_36": ("33", "mhd", "36,36,36"),
     "vac_mhd22_2052": ("22", "mhd", "2052,2052"),
 }

@@ -749,6 +749,139 @@ static void roe_rtimes(ctx_t* x, const double* q, const double* wroe, const int*
   }
 }
 
+/* ------------------------------------------------------------ HD Roe */
+/* average (src/vacphys.hdroe.t:4-46) over an interface region; wroe receives
+ * sqrt(rhoL*rhoR), the Roe-averaged velocity and the Roe-averaged enthalpy
+ * h (in the energy slot). */
+static void roe_average_hd(ctx_t* x, const double* wL, const double* wR,
+                           const int* lo, const int* hi, double* wroe) {
+  const int nd = x->ndir, eI = E_(x);
+  double* t1 = x->sc2;
+  double* t2 = x->sc3;
+  LOOP(lo, hi) {
+    long idx = IDX(x);
+    t1[idx] = sqrt(PL(x, wL, RHO)[idx]);
+    t2[idx] = sqrt(PL(x, wR, RHO)[idx]);
+    PL(x, wroe, RHO)[idx] = t1[idx] * t2[idx];
+    t1[idx] = t1[idx] / t2[idx];  /* sqrt(rhoL/rhoR) */
+    for (int c = 0; c < nd; ++c)
+      PL(x, wroe, M(c))[idx] = (PL(x, wL, M(c))[idx] / PL(x, wL, RHO)[idx] * t1[idx] +
+                                PL(x, wR, M(c))[idx] / PL(x, wR, RHO)[idx]) / (1 + t1[idx]);
+  }
+  getpthermal(x, 1, wL, lo, hi, t2);
+  LOOP(lo, hi) {
+    long idx = IDX(x);
+    PL(x, wroe, eI)[idx] = (t2[idx] + PL(x, wL, eI)[idx]) / PL(x, wL, RHO)[idx];
+  }
+  getpthermal(x, 1, wR, lo, hi, t2);
+  LOOP(lo, hi) {
+    long idx = IDX(x);
+    t2[idx] = (t2[idx] + PL(x, wR, eI)[idx]) / PL(x, wR, RHO)[idx];
+    PL(x, wroe, eI)[idx] = (PL(x, wroe, eI)[idx] * t1[idx] + t2[idx]) / (1 + t1[idx]);
+  }
+}
+
+/* geteigenjump (src/vacphys.hdroe.t:48-150), entropy fix 'nul'.  The first
+ * wave sets the sound speed (common /roe/ csound), (pR-pL)/c^2 and
+ * (vR-vL)/c along idim, which the others reuse.  Waves: 0 sound right,
+ * 1 sound left, 2 entropy, 3.. shear (direction il-3, idim skipped). */
+static void roe_eigenjump_hd(ctx_t* x, const double* wL, const double* wR,
+                             const double* wroe, const int* lo, const int* hi,
+                             int il, int idim, double* a, double* jump) {
+  const int nd = x->ndir, eI = E_(x);
+  const double g = x->p->gamma;
+  double* cs = x->cfast;
+  double* dpc = x->dp;
+  double* dvc = x->rhodv;
+  const double* rr = PL(x, wroe, RHO);
+  const double* vd = PL(x, wroe, M(idim));
+  if (il == 0) {
+    LOOP(lo, hi) {
+      long idx = IDX(x);
+      double vs = 0.0;
+      for (int c = 0; c < nd; ++c) {
+        double v = PL(x, wroe, M(c))[idx];
+        vs = (c == 0) ? v * v : vs + v * v;
+      }
+      cs[idx] = (g - 1) * (PL(x, wroe, eI)[idx] - 0.5 * vs);
+      cs[idx] = dmax(g * x->s->smallp / rr[idx], cs[idx]);
+    }
+    getpthermal(x, 1, wL, lo, hi, x->sc2);
+    getpthermal(x, 1, wR, lo, hi, x->sc3);
+    LOOP(lo, hi) {
+      long idx = IDX(x);
+      dpc[idx] = (x->sc3[idx] - x->sc2[idx]) / cs[idx];
+      cs[idx] = sqrt(cs[idx]);
+      dvc[idx] = (PL(x, wR, M(idim))[idx] / PL(x, wR, RHO)[idx] -
+                  PL(x, wL, M(idim))[idx] / PL(x, wL, RHO)[idx]) / cs[idx];
+    }
+  }
+  LOOP(lo, hi) {
+    long idx = IDX(x);
+    switch (il) {
+      case 0: a[idx] = vd[idx] + cs[idx]; jump[idx] = 0.5 * (dpc[idx] + rr[idx] * dvc[idx]); break;
+      case 1: a[idx] = vd[idx] - cs[idx]; jump[idx] = 0.5 * (dpc[idx] - rr[idx] * dvc[idx]); break;
+      case 2:
+        a[idx] = vd[idx];
+        jump[idx] = -dpc[idx] + PL(x, wR, RHO)[idx] - PL(x, wL, RHO)[idx];
+        break;
+      default: {
+        int idir = il - 3;
+        if (idir >= idim) idir += 1;
+        a[idx] = vd[idx];
+        jump[idx] = rr[idx] * (PL(x, wR, M(idir))[idx] / PL(x, wR, RHO)[idx] -
+                               PL(x, wL, M(idir))[idx] / PL(x, wL, RHO)[idx]);
+      }
+    }
+  }
+}
+
+/* rtimes (src/vacphys.hdroe.t:152-215): rq = R(iw, il) * q */
+static void roe_rtimes_hd(ctx_t* x, const double* q, const double* wroe, const int* lo,
+                          const int* hi, int iw, int il, int idim, double* rq) {
+  const int nd = x->ndir, eI = E_(x);
+  const double* cs = x->cfast;
+  const double* vd = PL(x, wroe, M(idim));
+  const double* h = PL(x, wroe, eI);
+  const int shear = il >= 3;
+  int idir = il - 3;
+  if (shear && idir >= idim) idir += 1;
+  LOOP(lo, hi) {
+    long idx = IDX(x);
+    double r;
+    if (iw == RHO) {
+      r = shear ? 0.0 : q[idx];
+    } else if (iw == eI) {
+      switch (il) {
+        case 0: r = q[idx] * (h[idx] + vd[idx] * cs[idx]); break;
+        case 1: r = q[idx] * (h[idx] - vd[idx] * cs[idx]); break;
+        case 2: {
+          double vs = 0.0;
+          for (int c = 0; c < nd; ++c) {
+            double v = PL(x, wroe, M(c))[idx];
+            vs = (c == 0) ? v * v : vs + v * v;
+          }
+          r = q[idx] * 0.5 * vs;
+          break;
+        }
+        default: r = q[idx] * PL(x, wroe, M(idir))[idx];
+      }
+    } else if (iw == M(idim)) {
+      switch (il) {
+        case 0: r = q[idx] * (vd[idx] + cs[idx]); break;
+        case 1: r = q[idx] * (vd[idx] - cs[idx]); break;
+        case 2: r = q[idx] * vd[idx]; break;
+        default: r = 0.0;
+      }
+    } else if (shear) {
+      r = (iw == M(idir)) ? q[idx] : 0.0;
+    } else {
+      r = q[idx] * PL(x, wroe, iw)[idx];
+    }
+    rq[idx] = r;
+  }
+}
+
 /* tvdlimit2 (src/vactvd.t:86-206) with getphi for typetvd='roe' (:209-302),
  * typeentropy='nul', acmcoef<0, acmexpo=0.  mu = 0: method 'tvd', called by
  * tvdlimit over ixIC = ixC widened by one face below (the limiter of phi
@@ -766,9 +899,11 @@ static void tvdlimit2(ctx_t* x, double qdt, int d, const double* wL, const doubl
   cl[d] = x->mlo[d] - 1;                            /* ixC */
   if (mu) { icl[d] = cl[d]; ich[d] = ch[d]; }
   long s = x->st[d];
-  roe_average(x, wL, wR, icl, ich, d, x->wroe);
+  if (x->mhd) roe_average(x, wL, wR, icl, ich, d, x->wroe);
+  else roe_average_hd(x, wL, wR, icl, ich, x->wroe);
   for (int il = 0; il < x->nw; ++il) {
-    roe_eigenjump(x, wL, wR, x->wroe, icl, ich, il, d, x->adtdx, x->jump);
+    if (x->mhd) roe_eigenjump(x, wL, wR, x->wroe, icl, ich, il, d, x->adtdx, x->jump);
+    else roe_eigenjump_hd(x, wL, wR, x->wroe, icl, ich, il, d, x->adtdx, x->jump);
     LOOP(icl, ich) {
       long idx = IDX(x);
       int c = CRD(d);
@@ -814,7 +949,8 @@ static void tvdlimit2(ctx_t* x, double qdt, int d, const double* wL, const doubl
     }
     }
     for (int iw = 0; iw < x->nw; ++iw) {
-      roe_rtimes(x, x->phi, x->wroe, cl, ch, iw, il, d, x->rphi);
+      if (x->mhd) roe_rtimes(x, x->phi, x->wroe, cl, ch, iw, il, d, x->rphi);
+      else roe_rtimes_hd(x, x->phi, x->wroe, cl, ch, iw, il, d, x->rphi);
       LOOP(cl, ch) {
         long idx = IDX(x);
         x->rphi[idx] = x->rphi[idx] * 0.25 * (dvol(x, idx + s) + dvol(x, idx));
@@ -1200,7 +1336,7 @@ static int ctx_init(ctx_t* x, const vacref_params* p, vacref_state* s,
   x->ndir = p->ndim;
   x->mhd = (p->phys == VACREF_MHD);
   x->nw = x->mhd ? 2 + 2 * x->ndir : 2 + x->ndim;
-  if (!x->mhd && p->typefull != VACREF_TVDLF) return -2;  /* HD Roe solver not restated */
+
   const double* dxs[3] = {dx0, dx1, dx2};
   long st = 1;
   for (int d = 0; d < 3; ++d) {

@@ -124,7 +124,7 @@ def bc_spec(v):
     one entry per face (a type, or a list of types per variable)."""
     import json
     s = str(v)
-    return json.loads(s) if s.startswith("[") else s
+    return json.loads(s) if s[:1] in ("[", '"') else s
 
 
 def pad_state(w_mesh: np.ndarray) -> np.ndarray:

@@ -31,7 +31,8 @@ from run_ref import have_ref, make_ini, problem_par, run_ref  # noqa: E402
 
 from refbuilds import BINARIES, POISSON  # noqa: E402
 
-NAMES = {6: "rho m1 m2 e b1 b2", 4: "rho m1 m2 e", 8: "rho m1 m2 m3 e b1 b2 b3"}
+NAMES = {6: "rho m1 m2 e b1 b2", 4: "rho m1 m2 e", 8: "rho m1 m2 m3 e b1 b2 b3",
+         5: "rho m1 m2 m3 e"}
 
 # name: (binary, physics, builder, gamma, steps, typefull, limiter, bc, divbfix)
 CASES = {
@@ -85,6 +86,16 @@ BC_CASES = {
     "mhd22_rf30x32_tvd_woodward_bcwall_proj": ("vacp_mhd22_132", "mhd", lambda g: random_fourier((30, 32), 405, g), 5 / 3, 10, "tvd", "woodward", [WALL_X, WALL_X, "fixed", "fixed"], False, (0.0, 1e-4, 1.0)),
     "mhd22_rf32x28_tvdlf_minmod_bcsymm_proj": ("vacp_mhd22_132", "mhd", lambda g: random_fourier((32, 28), 406, g), 5 / 3, 10, "tvdlf", "minmod", [["symm"] * 6, ["symm"] * 6, "periodic", "periodic"], True, (0.0, 1e-4, 2.0)),
 }
+
+
+# TVD and TVD-MUSCL with the HD Roe solver (src/vacphys.hdroe.t)
+HD_TVD_CASES = {
+    "hd22_rf32x28_tvd_woodward": ("vac_hd22_132", "hd", lambda g: random_fourier((32, 28), 501, g, physics="hd"), 1.4, 15, "tvd", "woodward", "periodic", True, None),
+    "hd22_rf28x30_tvd_minmod_cont": ("vac_hd22_132", "hd", lambda g: random_fourier((28, 30), 502, g, physics="hd"), 5 / 3, 15, "tvd", "minmod", "cont", True, None),
+    "hd22_rf30x26_tvdmu_superbee": ("vac_hd22_132", "hd", lambda g: random_fourier((30, 26), 503, g, physics="hd"), 1.4, 12, "tvdmu", "superbee", "periodic", True, None),
+    "hd33_rf10x12x8_tvd_woodward": ("vac_hd33_36", "hd", lambda g: random_fourier((10, 12, 8), 505, g, physics="hd"), 1.4, 8, "tvd", "woodward", "periodic", True, None),
+}
+BC_CASES.update(HD_TVD_CASES)
 
 
 def ensure_binary(name: str) -> None:
