@@ -38,8 +38,11 @@ def test_supported_rejects_unsupported_selectors():
     cfg.nx[0] = cfg.nx[1] = 16
     cfg.eqpar[0] = 1.4
     m = vacgpu.MethodSpec(typefull="tvd").to_c()
-    # HD with TVD needs the HD Roe solver: rejected with an error, not a fallback
+    assert lib.vacgpu_supported(C.byref(cfg), C.byref(m)) == 0  # HD Roe solver
+    # a typeB the path does not implement: rejected with an error, not a fallback
+    cfg.typeB[0][0] = 7
     assert lib.vacgpu_supported(C.byref(cfg), C.byref(m)) == -1
+    cfg.typeB[0][0] = vacgpu.CONT
     m2 = vacgpu.MethodSpec(typefull="tvdlf").to_c()
     assert lib.vacgpu_supported(C.byref(cfg), C.byref(m2)) == 0
     cfg.physics = vacgpu.MHD

@@ -36,6 +36,8 @@ CASES = [
      "tvdmu", "woodward", "periodic", True),
     ("vac_mhd33_36", "vacgpu_mhd33_36", "mhd", lambda g: random_fourier((12, 14, 10), 66, g), 1.4, 6,
      "tvdmu", "minmod", "cont", True),
+    ("vac_hd22_132", "vacgpu_hd22_132", "hd", lambda g: random_fourier((44, 40), 71, g, physics="hd"), 1.4, 20,
+     "tvd", "woodward", "periodic", True),
     # reflecting walls (symm; the normal momentum and field asymm) and fixed faces
     ("vac_mhd22_132", "vacgpu_mhd22_132", "mhd", lambda g: random_fourier((40, 36), 68, g), 5 / 3, 15,
      "tvdlf", "minmod", [WALL_X, WALL_X, "fixed", "fixed"], True),

@@ -35,6 +35,7 @@ CASES = [
     ("rf256-tvdlf-minmod-200", lambda g: random_fourier((256, 256), 6, g), "mhd", 5 / 3, "tvdlf", "minmod", 200, "periodic"),
     ("rf160x96-tvdlf-superbee-cont", lambda g: random_fourier((160, 96), 8, g), "mhd", 1.4, "tvdlf", "superbee", 80, "cont"),
     ("rf128-tvdmu-woodward", lambda g: random_fourier((128, 128), 5, g), "mhd", 5 / 3, "tvdmu", "woodward", 40, "periodic"),
+    ("hd-rf160x128-tvd-woodward", lambda g: random_fourier((160, 128), 10, g, physics="hd"), "hd", 1.4, "tvd", "woodward", 40, "periodic"),
     ("hd-rf128-tvdlf-minmod-cont", lambda g: random_fourier((128, 128), 9, g, physics="hd"), "hd", 5 / 3, "tvdlf", "minmod", 80, "cont"),
 ]
 

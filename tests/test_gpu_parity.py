@@ -148,6 +148,12 @@ WALL_Z = ["symm", "symm", "symm", "asymm", "symm", "symm", "symm", "asymm"]
     ("mhd", (50, 60), "tvdmu", "woodward", [WALL_X, "fixed", "periodic", "periodic"], 10),
     ("hd", (80, 64), "tvdlf", "minmod", [["symm", "asymm", "symm", "symm"]] * 2 + ["fixed"] * 2, 15),
     ("mhd", (14, 12, 16), "tvd", "woodward", ["periodic", "periodic", "fixed", "fixed", WALL_Z, WALL_Z], 5),
+    # hydrodynamics with the HD Roe solver
+    ("hd", (72, 48), "tvd", "woodward", "periodic", 15),
+    ("hd", (40, 56), "tvdmu", "minmod", "cont", 12),
+    ("hd", (16, 12, 14), "tvd", "superbee", "cont", 6),
+    ("hd", (12, 14, 10), "tvdmu", "woodward", "periodic", 6),
+    ("hd", (60, 44), "tvd", "minmod", [["symm", "asymm", "symm", "symm"]] * 2 + ["fixed"] * 2, 12),
 ])
 def test_gpu_matches_oracle(case):
     physics, nx, typefull, limiter, bc, steps = case

@@ -472,6 +472,141 @@ __device__ __forceinline__ R roe_rtimes(const RoeRt<ND, R>& r, R q, int iw, int 
   }
 }
 
+// ---- HD Roe solver (src/vacphys.hdroe.t) for TVD and TVD-MUSCL.
+// Waves: 0 sound right, 1 sound left, 2 entropy, 3.. shear (direction il-3,
+// the sweep direction skipped).
+template <int ND, class R>
+struct RoeH {
+  R rho, v[ND], h, c, dpc, dvc;  // sqrt(rhoL*rhoR), v, h, c, (pR-pL)/c^2, (vR-vL)/c
+};
+template <int ND, class R>
+struct RoeRtH {
+  R v[ND];
+  R h, c, vsq;
+};
+
+// average (:4-46) and the il == 1 part of geteigenjump (:66-92).  uL, uR are
+// the conserved states left and right of the face.
+template <int ND, class R>
+__device__ __forceinline__ void roe_setup(const double* uL, const double* uR, int d, double g,
+                                          double smallp, RoeH<ND, R>& r) {
+  using L = Layout<ND, false>;
+  const R rL = R(uL[L::rho]), rR = R(uR[L::rho]);
+  R t = R(sqrt(rL));
+  const R t2 = R(sqrt(rR));
+  r.rho = t * t2;
+  t = pdiv(t, t2);  // sqrt(rhoL/rhoR)
+  const R w1 = R(1) + t;
+#pragma unroll
+  for (int c = 0; c < ND; ++c)
+    r.v[c] = pdiv(pdiv(R(uL[L::m(c)]), rL) * t + pdiv(R(uR[L::m(c)]), rR), w1);
+  const R pL = R(pthermal<ND, false>(uL, g, smallp));
+  const R pR = R(pthermal<ND, false>(uR, g, smallp));
+  const R hL = pdiv(pL + R(uL[L::e]), rL);
+  const R hR = pdiv(pR + R(uR[L::e]), rR);
+  r.h = pdiv(hL * t + hR, w1);
+  R vs = r.v[0] * r.v[0];
+#pragma unroll
+  for (int c = 1; c < ND; ++c) vs = vs + r.v[c] * r.v[c];
+  R c2 = (R(g) - R(1)) * (r.h - R(0.5) * vs);
+  c2 = rmax(c2, pdiv(R(g) * R(smallp), r.rho));
+  r.dpc = pdiv(pR - pL, c2);
+  r.c = R(sqrt(c2));
+  r.dvc = pdiv(pdiv(R(uR[L::m(d)]), rR) - pdiv(R(uL[L::m(d)]), rL), r.c);
+}
+
+template <int ND, class R>
+__device__ __forceinline__ RoeRtH<ND, R> roe_rt(const RoeH<ND, R>& r) {
+  RoeRtH<ND, R> t;
+#pragma unroll
+  for (int c = 0; c < ND; ++c) t.v[c] = r.v[c];
+  t.h = r.h;
+  t.c = r.c;
+  R vs = r.v[0] * r.v[0];
+#pragma unroll
+  for (int c = 1; c < ND; ++c) vs = vs + r.v[c] * r.v[c];
+  t.vsq = vs;
+  return t;
+}
+
+// the direction of shear wave il (il >= 3) for a sweep along d
+__device__ __forceinline__ int shear_dir(int il, int d) {
+  const int k = il - 3;
+  return k >= d ? k + 1 : k;
+}
+
+// wave speed a and characteristic jump of wave il (geteigenjump :94-118,
+// entropy fix 'nul')
+template <int ND, class R>
+__device__ __forceinline__ void roe_wave(const RoeH<ND, R>& r, const double* uL, const double* uR,
+                                         int il, int d, int divbwave, R& a, R& jump) {
+  using L = Layout<ND, false>;
+  (void)divbwave;
+  const R vd = r.v[d];
+  switch (il) {
+    case 0:
+      a = vd + r.c;
+      jump = R(0.5) * (r.dpc + r.rho * r.dvc);
+      break;
+    case 1:
+      a = vd - r.c;
+      jump = R(0.5) * (r.dpc - r.rho * r.dvc);
+      break;
+    case 2:
+      a = vd;
+      jump = -r.dpc + R(uR[L::rho]) - R(uL[L::rho]);
+      break;
+    default: {
+      const int k = shear_dir(il, d);
+      a = vd;
+      jump = r.rho * (pdiv(R(uR[L::m(k)]), R(uR[L::rho])) - pdiv(R(uL[L::m(k)]), R(uL[L::rho])));
+    }
+  }
+}
+
+// R(iw, il) * q (rtimes :152-215)
+template <int ND, class R>
+__device__ __forceinline__ R roe_rtimes(const RoeRtH<ND, R>& r, R q, int iw, int il, int d,
+                                        int divbwave) {
+  using L = Layout<ND, false>;
+  (void)divbwave;
+  const bool shear = il >= 3;
+  const int k = shear ? shear_dir(il, d) : 0;
+  if (iw == L::rho) return shear ? R(0) : q;
+  const R vd = r.v[d];
+  if (iw == L::e) {
+    switch (il) {
+      case 0: return q * (r.h + vd * r.c);
+      case 1: return q * (r.h - vd * r.c);
+      case 2: return q * R(0.5) * r.vsq;
+      default: return q * r.v[k];
+    }
+  }
+  const int c = iw - 1;  // momentum component
+  if (c == d) {
+    switch (il) {
+      case 0: return q * (vd + r.c);
+      case 1: return q * (vd - r.c);
+      case 2: return q * vd;
+      default: return R(0);
+    }
+  }
+  if (shear) return (c == k) ? q : R(0);
+  return q * r.v[c];
+}
+
+// the Roe state types of the MHD and HD solvers
+template <int ND, class R, bool MHD>
+struct RoeOf {
+  using St = Roe<ND, R>;
+  using Rt = RoeRt<ND, R>;
+};
+template <int ND, class R>
+struct RoeOf<ND, R, false> {
+  using St = RoeH<ND, R>;
+  using Rt = RoeRtH<ND, R>;
+};
+
 // getphi for method 'tvd', typetvd 'roe', entropy fix 'nul'
 // (src/vactvd.t:246-247, :294-302): jm, j0, jp are the jumps of one wave at
 // faces J-1, J, J+1 and av = a*dt*2/(dx_J+dx_J+1) at face J.

@@ -1147,46 +1147,46 @@ __device__ __forceinline__ void ryu_first_half(const Geo& g, const BC& bc,
 // (covering ixIC, src/vactvd.t:42,145-157) to part[SLOT_C0/C1].
 
 // Everything one face contributes to the update of its two cells.
-template <int ND, class R>
+template <int ND, class R, bool MHD = true>
 struct TvdFace {
-  R jump[Layout<ND, true>::nw];
-  R adt[Layout<ND, true>::nw];
-  RoeRt<ND, R> rt;
+  R jump[Layout<ND, MHD>::nw];
+  R adt[Layout<ND, MHD>::nw];
+  typename RoeOf<ND, R, MHD>::Rt rt;
 };
 
-template <int ND, class R>
+template <int ND, class R, bool MHD = true>
 __device__ __forceinline__ void tvd_face(const double* uL, const double* uR, int d, const Sel& s,
-                                         double smallp, double dt, double dxsum, TvdFace<ND, R>& f,
+                                         double smallp, double dt, double dxsum, TvdFace<ND, R, MHD>& f,
                                          double& c0, double& c1) {
-  constexpr int NW = Layout<ND, true>::nw;
-  Roe<ND, R> r;
-  roe_setup<ND, R>(uL, uR, d, s.gamma, smallp, r);
+  constexpr int NW = Layout<ND, MHD>::nw;
+  typename RoeOf<ND, R, MHD>::St r;
+  roe_setup(uL, uR, d, s.gamma, smallp, r);
   const R sdx = R(dxsum);
 #pragma unroll
   for (int il = 0; il < NW; ++il) {
     R a, jp;
-    roe_wave<ND, R>(r, uL, uR, il, d, s.divbwave, a, jp);
+    roe_wave(r, uL, uR, il, d, s.divbwave, a, jp);
     const R ad = pdiv(a * R(dt) * R(2), sdx);
     f.jump[il] = jp;
     f.adt[il] = ad;
   }
   c0 = fabs(double(f.adt[0]));
   c1 = fabs(double(f.adt[1]));
-  f.rt = roe_rt<ND, R>(r);
+  f.rt = roe_rt(r);
 }
 
 // out[iw] += sum over il (in order) of (R.phi(J) q(J) - R.phi(J-1) q(J-1)) / dV(i)
-template <int ND, class R>
-__device__ __forceinline__ void tvd_dissipate(const RoeRt<ND, R>& rm, const R* phim,
-                                              const RoeRt<ND, R>& r0, const R* phi0, int d,
+template <int ND, class R, bool MHD = true>
+__device__ __forceinline__ void tvd_dissipate(const typename RoeOf<ND, R, MHD>::Rt& rm, const R* phim,
+                                              const typename RoeOf<ND, R, MHD>::Rt& r0, const R* phi0, int d,
                                               const Sel& s, R dvm, R dv0, double dvi, double* out) {
-  constexpr int NW = Layout<ND, true>::nw;
+  constexpr int NW = Layout<ND, MHD>::nw;
 #pragma unroll
   for (int il = 0; il < NW; ++il) {
 #pragma unroll
     for (int iw = 0; iw < NW; ++iw) {
-      const R a0 = roe_rtimes<ND, R>(rm, phim[il], iw, il, d, s.divbwave) * R(0.25) * dvm;
-      const R a1 = roe_rtimes<ND, R>(r0, phi0[il], iw, il, d, s.divbwave) * R(0.25) * dv0;
+      const R a0 = roe_rtimes(rm, phim[il], iw, il, d, s.divbwave) * R(0.25) * dvm;
+      const R a1 = roe_rtimes(r0, phi0[il], iw, il, d, s.divbwave) * R(0.25) * dv0;
       out[iw] = out[iw] + pdiv(double(a1) - double(a0), dvi);
     }
   }
@@ -1196,15 +1196,15 @@ __device__ __forceinline__ void tvd_dissipate(const RoeRt<ND, R>& rm, const R* p
 // term of cell J is lane L-1's face term: a0(il, iw) of this lane is a1(il, iw)
 // of the previous lane (the same R.phi product, scaled by the same sum
 // dvol(J) + dvol(J-1)), so it is taken from there instead of recomputed.
-template <int ND, class R>
-__device__ __forceinline__ void tvd_dissipate_x(const RoeRt<ND, R>& r0, const R* phi0, const Sel& s,
+template <int ND, class R, bool MHD = true>
+__device__ __forceinline__ void tvd_dissipate_x(const typename RoeOf<ND, R, MHD>::Rt& r0, const R* phi0, const Sel& s,
                                                 R dv0, double dvi, double* out) {
-  constexpr int NW = Layout<ND, true>::nw;
+  constexpr int NW = Layout<ND, MHD>::nw;
 #pragma unroll
   for (int il = 0; il < NW; ++il) {
 #pragma unroll
     for (int iw = 0; iw < NW; ++iw) {
-      const R a1 = roe_rtimes<ND, R>(r0, phi0[il], iw, il, 0, s.divbwave) * R(0.25) * dv0;
+      const R a1 = roe_rtimes(r0, phi0[il], iw, il, 0, s.divbwave) * R(0.25) * dv0;
       const R a0 = lane_prev(a1);
       out[iw] = out[iw] + pdiv(double(a1) - double(a0), dvi);
     }
@@ -1237,12 +1237,12 @@ __device__ __forceinline__ S shfl_up_struct(const S& v) {
 // its two faces and the Powell source of w1 as in the TVDLF corrector.  The
 // two extreme waves' |a dt 2/(dx_J + dx_J+1)| over the faces ixC feed the
 // Courant fold (src/vactvd.t:145-157).
-template <int ND, class R, int D>
+template <int ND, class R, int D, bool MHD = true>
 __device__ __forceinline__ void mu_face(const Sel& s, const double* u0, const double* u1,
                                         const double* ll, const double* lr, double dt,
                                         double smallp, double dxsum, double* F,
-                                        TvdFace<ND, R>& f, R* phi, double& c0, double& c1) {
-  constexpr int NW = Layout<ND, true>::nw;
+                                        TvdFace<ND, R, MHD>& f, R* phi, double& c0, double& c1) {
+  constexpr int NW = Layout<ND, MHD>::nw;
   double uL[NW], uR[NW];
 #pragma unroll
   for (int iw = 0; iw < NW; ++iw) {
@@ -1251,26 +1251,26 @@ __device__ __forceinline__ void mu_face(const Sel& s, const double* u0, const do
   }
   {
     double fL[NW], fR[NW];
-    fluxes<ND, true, R>(uL, D, s.gamma, smallp, fL);
-    fluxes<ND, true, R>(uR, D, s.gamma, smallp, fR);
+    fluxes<ND, MHD, R>(uL, D, s.gamma, smallp, fL);
+    fluxes<ND, MHD, R>(uR, D, s.gamma, smallp, fR);
 #pragma unroll
     for (int iw = 0; iw < NW; ++iw) F[iw] = 0.5 * (fL[iw] + fR[iw]);
   }
-  tvd_face<ND, R>(uL, uR, D, s, smallp, dt, dxsum, f, c0, c1);
+  tvd_face<ND, R, MHD>(uL, uR, D, s, smallp, dt, dxsum, f, c0, c1);
 #pragma unroll
   for (int il = 0; il < NW; ++il) phi[il] = R(fabs(f.adt[il])) * f.jump[il];
 }
 
 // x sweep: lane L owns face c+1/2 of cell c = x0 - 1 + L and updates c for
 // L in [1, 62]; the left face's flux and dissipation term come from lane L-1.
-template <int ND, class R, int LIM>
+template <int ND, class R, int LIM, bool MHD = true>
 __global__ __launch_bounds__(256) void k_tvdmu_x(Geo g, Sel s, BC bc, const double* __restrict__ w,
                                                 const double* __restrict__ w1,
                                                 double* __restrict__ wout,
                                                 const Scal* __restrict__ sc,
                                                 double* __restrict__ part, long long pstride,
                                                 int ghosts) {
-  using L = Layout<ND, true>;
+  using L = Layout<ND, MHD>;
   constexpr int NW = L::nw;
   __shared__ double sm[8];
   const double dt = sc->dt, smallp = sc->smallp;
@@ -1296,9 +1296,9 @@ __global__ __launch_bounds__(256) void k_tvdmu_x(Geo g, Sel s, BC bc, const doub
 #pragma unroll
   for (int iw = 0; iw < NW; ++iw) lr[iw] = lane_next(l[iw]);
   double F[NW], c0, c1;
-  TvdFace<ND, R> f;
+  TvdFace<ND, R, MHD> f;
   R phi[NW];
-  mu_face<ND, R, 0>(s, u0, u1, l, lr, dt, smallp, g.dx[0][cc] + g.dx[0][cc + 1], F, f, phi, c0, c1);
+  mu_face<ND, R, 0, MHD>(s, u0, u1, l, lr, dt, smallp, g.dx[0][cc] + g.dx[0][cc + 1], F, f, phi, c0, c1);
   if (!face_ok) { c0 = -INFINITY; c1 = -INFINITY; }
   double out[NW];
   const double qdx = pdiv(dt, g.dx[0][cc]);
@@ -1315,11 +1315,13 @@ __global__ __launch_bounds__(256) void k_tvdmu_x(Geo g, Sel s, BC bc, const doub
     return v;
   };
   const double d0 = dvol(cc), dp = dvol(cc + 1);
-  tvd_dissipate_x<ND, R>(f.rt, phi, s, R(dp + d0), d0, out);
+  tvd_dissipate_x<ND, R, MHD>(f.rt, phi, s, R(dp + d0), d0, out);
   const int c3[3] = {cc, yc, z};
-  if (s.sourceunsplit && s.divbfix) {
-    const double dv = divb_at<ND>(g, w1, idx, c3);
-    powell<ND>(u0, dv, dt * 1.0 / ND, out);
+  if constexpr (MHD) {
+    if (s.sourceunsplit && s.divbfix) {
+      const double dv = divb_at<ND>(g, w1, idx, c3);
+      powell<ND>(u0, dv, dt * 1.0 / ND, out);
+    }
   }
   if (useful) store_cell<ND, NW>(g, bc, wout, idx, c3, out, ghosts);
   const double b0 = block_reduce<true>(c0, sm);
@@ -1330,14 +1332,14 @@ __global__ __launch_bounds__(256) void k_tvdmu_x(Geo g, Sel s, BC bc, const doub
 
 // y / z sweep: thread (x, segment) marches through cells s0..s1 along D,
 // carrying the face below (flux, Roe decomposition, phi) and a window of w and w1.
-template <int ND, class R, int D, int LIM>
+template <int ND, class R, int D, int LIM, bool MHD = true>
 __global__ __launch_bounds__(256) void k_tvdmu_m(Geo g, Sel s, BC bc, const double* __restrict__ w,
                                                 const double* __restrict__ w1,
                                                 double* __restrict__ wout,
                                                 const Scal* __restrict__ sc,
                                                 double* __restrict__ part, long long pstride,
                                                 int seglen, int ghosts) {
-  using L = Layout<ND, true>;
+  using L = Layout<ND, MHD>;
   constexpr int NW = L::nw;
   __shared__ double sm[8];
   const double dt = sc->dt, smallp = sc->smallp;
@@ -1366,7 +1368,7 @@ __global__ __launch_bounds__(256) void k_tvdmu_m(Geo g, Sel s, BC bc, const doub
     double a, b;
     // window: w at J, J+1, w1 at J, slope at J; the face J-1/2 below
     double wb[NW], wc[NW], u0[NW], lc[NW], Fp[NW];
-    TvdFace<ND, R> fp;
+    TvdFace<ND, R, MHD> fp;
     R phip[NW];
     {
       double wa[NW], u1[NW], ln[NW], wd[NW];
@@ -1381,7 +1383,7 @@ __global__ __launch_bounds__(256) void k_tvdmu_m(Geo g, Sel s, BC bc, const doub
       load<NW>(w, g.ps, base + (long long)(s0 + 1) * sd, wd);
 #pragma unroll
       for (int iw = 0; iw < NW; ++iw) ln[iw] = slope<R, LIM>(s.lim[iw], wb[iw], wc[iw], wd[iw]);
-      mu_face<ND, R, D>(s, u0, u1, lc, ln, dt, smallp, dxd[s0 - 1] + dxd[s0], Fp, fp, phip, a, b);
+      mu_face<ND, R, D, MHD>(s, u0, u1, lc, ln, dt, smallp, dxd[s0 - 1] + dxd[s0], Fp, fp, phip, a, b);
       c0 = c0 < a ? a : c0; c1 = c1 < b ? b : c1;
 #pragma unroll
       for (int iw = 0; iw < NW; ++iw) {
@@ -1394,13 +1396,13 @@ __global__ __launch_bounds__(256) void k_tvdmu_m(Geo g, Sel s, BC bc, const doub
     // now: u0 = w1(i), lc = slope(i), wb = w(i), wc = w(i+1)
     for (int i = s0; i <= s1; ++i) {
       double u1[NW], ln[NW], wd[NW], F[NW];
-      TvdFace<ND, R> f;
+      TvdFace<ND, R, MHD> f;
       R phi[NW];
       load<NW>(w1, g.ps, base + (long long)(i + 1) * sd, u1);
       load<NW>(w, g.ps, base + (long long)(i + 2) * sd, wd);
 #pragma unroll
       for (int iw = 0; iw < NW; ++iw) ln[iw] = slope<R, LIM>(s.lim[iw], wb[iw], wc[iw], wd[iw]);
-      mu_face<ND, R, D>(s, u0, u1, lc, ln, dt, smallp, dxd[i] + dxd[i + 1], F, f, phi, a, b);
+      mu_face<ND, R, D, MHD>(s, u0, u1, lc, ln, dt, smallp, dxd[i] + dxd[i + 1], F, f, phi, a, b);
       c0 = c0 < a ? a : c0; c1 = c1 < b ? b : c1;
       const long long idx = base + (long long)i * sd;
       const double qdx = pdiv(dt, dxd[i]);
@@ -1408,11 +1410,13 @@ __global__ __launch_bounds__(256) void k_tvdmu_m(Geo g, Sel s, BC bc, const doub
 #pragma unroll
       for (int iw = 0; iw < NW; ++iw) out[iw] = wb[iw] - qdx * (F[iw] - Fp[iw]);
       const double dm = dvol(i - 1), d0 = dvol(i), dp = dvol(i + 1);
-      tvd_dissipate<ND, R>(fp.rt, phip, f.rt, phi, D, s, R(d0 + dm), R(dp + d0), d0, out);
+      tvd_dissipate<ND, R, MHD>(fp.rt, phip, f.rt, phi, D, s, R(d0 + dm), R(dp + d0), d0, out);
       c3[D] = i;
-      if (s.sourceunsplit && s.divbfix) {
-        const double dv = divb_at<ND>(g, w1, idx, c3);
-        powell<ND>(u0, dv, dt * 1.0 / ND, out);
+      if constexpr (MHD) {
+        if (s.sourceunsplit && s.divbfix) {
+          const double dv = divb_at<ND>(g, w1, idx, c3);
+          powell<ND>(u0, dv, dt * 1.0 / ND, out);
+        }
       }
       store_cell<ND, NW>(g, bc, wout, idx, c3, out, ghosts);
 #pragma unroll
@@ -1437,7 +1441,7 @@ constexpr int TX_STRIDE = 61;  // cells updated per 64-lane wavefront along x
 
 // x sweep: lane L of a wavefront owns face J = x0 - 2 + L (between cells J and
 // J+1) and updates cell J for L in [2, 62].
-template <int ND, class R, int LIM = -1, int SRC = 0>
+template <int ND, class R, int LIM = -1, int SRC = 0, bool MHD = true>
 __global__ __launch_bounds__(256) void k_tvd_x(Geo g, Sel s, BC bc, const double* __restrict__ wct,
                                               const double* __restrict__ win,
                                               double* __restrict__ wout,
@@ -1445,7 +1449,7 @@ __global__ __launch_bounds__(256) void k_tvd_x(Geo g, Sel s, BC bc, const double
                                               double* __restrict__ part, long long pstride,
                                               const double* __restrict__ wold,
                                               double* __restrict__ wt, int ghosts) {
-  using L = Layout<ND, true>;
+  using L = Layout<ND, MHD>;
   constexpr int NW = L::nw;
   __shared__ double sm[8];
   const double dt = sc->dt, smallp = sc->smallp;
@@ -1467,14 +1471,14 @@ __global__ __launch_bounds__(256) void k_tvd_x(Geo g, Sel s, BC bc, const double
     load<NW>(wct, g.ps, base + Jc, uL);
     load<NW>(wct, g.ps, base + Jc + 1, uR);
   }
-  TvdFace<ND, R> f;
+  TvdFace<ND, R, MHD> f;
   double c0, c1;
-  tvd_face<ND, R>(uL, uR, 0, s, smallp, dt, g.dx[0][Jc] + g.dx[0][Jc + 1], f, c0, c1);
+  tvd_face<ND, R, MHD>(uL, uR, 0, s, smallp, dt, g.dx[0][Jc] + g.dx[0][Jc + 1], f, c0, c1);
   const bool faceok = rowok && J <= g.n[0] + 2;
   if (!faceok) { c0 = -INFINITY; c1 = -INFINITY; }
   // centdiff: flux at cell J, face average fC(J) and fC(J-1)
   double F[NW], fC[NW], fCm[NW];
-  fluxes<ND, true, R>(uL, 0, s.gamma, smallp, F);
+  fluxes<ND, MHD, R>(uL, 0, s.gamma, smallp, F);
 #pragma unroll
   for (int iw = 0; iw < NW; ++iw) fC[iw] = 0.5 * (F[iw] + shfl_down1(F[iw]));
 #pragma unroll
@@ -1506,7 +1510,7 @@ __global__ __launch_bounds__(256) void k_tvd_x(Geo g, Sel s, BC bc, const double
       return v;
     };
     const double d0 = dvol(ic), dp = dvol(ic + 1);
-    tvd_dissipate_x<ND, R>(f.rt, phi, s, R(dp + d0), d0, out);
+    tvd_dissipate_x<ND, R, MHD>(f.rt, phi, s, R(dp + d0), d0, out);
   }
   if (rowok && lane >= 2 && lane <= 62 && i <= g.n[0] + 1) {
     const long long idx = base + i;
@@ -1522,7 +1526,7 @@ __global__ __launch_bounds__(256) void k_tvd_x(Geo g, Sel s, BC bc, const double
 }
 
 // y / z sweep: thread (x, segment) marches through cells s0..s1 along d.
-template <int ND, class R, int D, int LIM = -1, int SRC = 0>
+template <int ND, class R, int D, int LIM = -1, int SRC = 0, bool MHD = true>
 __global__ __launch_bounds__(256) void k_tvd_m(Geo g, Sel s, BC bc, const double* __restrict__ wct,
                                               const double* __restrict__ win,
                                               double* __restrict__ wout,
@@ -1535,7 +1539,7 @@ __global__ __launch_bounds__(256) void k_tvd_m(Geo g, Sel s, BC bc, const double
   static_assert(SRC != 2, "the Ryu half is folded into x sweeps only");
   (void)ghosts;
   constexpr int d = D;
-  using L = Layout<ND, true>;
+  using L = Layout<ND, MHD>;
   constexpr int NW = L::nw;
   __shared__ double sm[8];
   const double dt = sc->dt, smallp = sc->smallp;
@@ -1574,28 +1578,28 @@ __global__ __launch_bounds__(256) void k_tvd_m(Geo g, Sel s, BC bc, const double
       }
     };
     double uL[NW], uR[NW];
-    TvdFace<ND, R> fm, f0, fp;  // faces i-1, i, i+1
+    TvdFace<ND, R, MHD> fm, f0, fp;  // faces i-1, i, i+1
     R phim[NW], phi0[NW];
     double Fm[NW], F0[NW], fCm[NW];
     double a, b;
     // prologue: faces s0-2, s0-1, s0; centred fluxes at cells s0-1, s0
     get_ct(s0 - 2, uL);
     get_ct(s0 - 1, uR);
-    tvd_face<ND, R>(uL, uR, d, s, smallp, dt, dxd[s0 - 2] + dxd[s0 - 1], fm, a, b);
+    tvd_face<ND, R, MHD>(uL, uR, d, s, smallp, dt, dxd[s0 - 2] + dxd[s0 - 1], fm, a, b);
     c0 = c0 < a ? a : c0; c1 = c1 < b ? b : c1;
-    fluxes<ND, true, R>(uR, d, s.gamma, smallp, Fm);
+    fluxes<ND, MHD, R>(uR, d, s.gamma, smallp, Fm);
 #pragma unroll
     for (int iw = 0; iw < NW; ++iw) uL[iw] = uR[iw];
     get_ct(s0, uR);
-    tvd_face<ND, R>(uL, uR, d, s, smallp, dt, dxd[s0 - 1] + dxd[s0], f0, a, b);
+    tvd_face<ND, R, MHD>(uL, uR, d, s, smallp, dt, dxd[s0 - 1] + dxd[s0], f0, a, b);
     c0 = c0 < a ? a : c0; c1 = c1 < b ? b : c1;
-    fluxes<ND, true, R>(uR, d, s.gamma, smallp, F0);
+    fluxes<ND, MHD, R>(uR, d, s.gamma, smallp, F0);
 #pragma unroll
     for (int iw = 0; iw < NW; ++iw) fCm[iw] = 0.5 * (Fm[iw] + F0[iw]);  // face s0-1
 #pragma unroll
     for (int iw = 0; iw < NW; ++iw) uL[iw] = uR[iw];
     get_ct(s0 + 1, uR);
-    tvd_face<ND, R>(uL, uR, d, s, smallp, dt, dxd[s0] + dxd[s0 + 1], fp, a, b);
+    tvd_face<ND, R, MHD>(uL, uR, d, s, smallp, dt, dxd[s0] + dxd[s0 + 1], fp, a, b);
     c0 = c0 < a ? a : c0; c1 = c1 < b ? b : c1;
     // phi at face s0-1 from faces s0-2, s0-1, s0
 #pragma unroll
@@ -1605,7 +1609,7 @@ __global__ __launch_bounds__(256) void k_tvd_m(Geo g, Sel s, BC bc, const double
     f0 = fp;
     for (int i = s0;; ++i) {
       // f0 = face i, fm = face i-1 (with phim), uL = w(i), uR = w(i+1)
-      fluxes<ND, true, R>(uR, d, s.gamma, smallp, Fm);  // F(i+1), reuse Fm as scratch
+      fluxes<ND, MHD, R>(uR, d, s.gamma, smallp, Fm);  // F(i+1), reuse Fm as scratch
       double fC[NW];
 #pragma unroll
       for (int iw = 0; iw < NW; ++iw) fC[iw] = 0.5 * (F0[iw] + Fm[iw]);  // face i
@@ -1615,7 +1619,7 @@ __global__ __launch_bounds__(256) void k_tvd_m(Geo g, Sel s, BC bc, const double
 #pragma unroll
       for (int iw = 0; iw < NW; ++iw) uL[iw] = uR[iw];
       get_ct(i + 2, uR);
-      tvd_face<ND, R>(uL, uR, d, s, smallp, dt, dxd[i + 1] + dxd[i + 2], fp, a, b);
+      tvd_face<ND, R, MHD>(uL, uR, d, s, smallp, dt, dxd[i + 1] + dxd[i + 2], fp, a, b);
       c0 = c0 < a ? a : c0; c1 = c1 < b ? b : c1;
 #pragma unroll
       for (int il = 0; il < NW; ++il) phi0[il] = tvd_phi_t<LIM, R>(s.lim[il], fm.jump[il], f0.jump[il], fp.jump[il], f0.adt[il]);
@@ -1626,7 +1630,7 @@ __global__ __launch_bounds__(256) void k_tvd_m(Geo g, Sel s, BC bc, const double
 #pragma unroll
       for (int iw = 0; iw < NW; ++iw) out[iw] = win[iw * g.ps + idx] - qdx * (fC[iw] - fCm[iw]);
       const double dm = dvol(i - 1), d0 = dvol(i), dp = dvol(i + 1);
-      tvd_dissipate<ND, R>(fm.rt, phim, f0.rt, phi0, d, s, R(d0 + dm), R(dp + d0), d0, out);
+      tvd_dissipate<ND, R, MHD>(fm.rt, phim, f0.rt, phi0, d, s, R(d0 + dm), R(dp + d0), d0, out);
 #pragma unroll
       for (int iw = 0; iw < NW; ++iw) wout[iw * g.ps + idx] = out[iw];
       if (i == s1) break;

@@ -192,7 +192,6 @@ int check_config(const vacgpu_config* c, std::string& why) {
 int check_method(const vacgpu_config* c, const vacgpu_method* m, std::string& why) {
   if (!m) { why = "null method"; return VACGPU_E_ARG; }
   if (m->typefull != VACGPU_TVDLF && m->typefull != VACGPU_TVD && m->typefull != VACGPU_TVDMU) { why = "typefull must be 'tvdlf', 'tvd' or 'tvdmu'"; return VACGPU_E_ARG; }
-  if (m->typefull != VACGPU_TVDLF && c->physics == VACGPU_HD) { why = "typefull='tvd' and 'tvdmu' need the HD Roe solver, which is not on the GPU path"; return VACGPU_E_ARG; }
   int nw = c->physics == VACGPU_MHD ? 2 + 2 * c->ndim : 2 + c->ndim;
   for (int iw = 0; iw < nw; ++iw)
     if (m->typelimiter[iw] < VACGPU_MINMOD || m->typelimiter[iw] > VACGPU_SUPERBEE) { why = "typelimiter must be minmod, woodward or superbee"; return VACGPU_E_ARG; }
@@ -431,8 +430,8 @@ int sweep_device(vacgpu_grid* g, const vacgpu_method* m, int d, int firstsweep, 
     g->cur = outi;
     return 0;
   } else if (m->typefull == VACGPU_TVDMU) {
-    // TVD-MUSCL: the Hancock predictor of TVDLF, then one thread per cell for
-    // the central flux, the Roe dissipation and the Powell source
+    // TVD-MUSCL: the Hancock predictor of TVDLF, then the face-parallel
+    // corrector (central flux, Roe dissipation, Powell source for MHD)
     const int ghosts = (g->bc_uniform && g->use_fused) ? 1 : 0;
     {
       Timed t(g, KF_PRED);
@@ -454,7 +453,8 @@ int sweep_device(vacgpu_grid* g, const vacgpu_method* m, int d, int firstsweep, 
       if (d == 0) {
         const dim3 gr((g->geo.n[0] + CX_STRIDE - 1) / CX_STRIDE, g->grid.y, g->grid.z);
         nbc = (long long)gr.x * gr.y * gr.z;
-#define MUX(NDV, RT, LV) hipLaunchKernelGGL((k_tvdmu_x<NDV, RT, LV>), gr, g->block, 0, g->stream, g->geo, s, g->bc, win, g->w1, wout, g->sc, g->part, g->pstride, ghosts)
+#define MUX(NDV, RT, LV) do { if (mhd) hipLaunchKernelGGL((k_tvdmu_x<NDV, RT, LV, true>), gr, g->block, 0, g->stream, g->geo, s, g->bc, win, g->w1, wout, g->sc, g->part, g->pstride, ghosts); \
+                               else hipLaunchKernelGGL((k_tvdmu_x<NDV, RT, LV, false>), gr, g->block, 0, g->stream, g->geo, s, g->bc, win, g->w1, wout, g->sc, g->part, g->pstride, ghosts); } while (0)
 #define X2D(LV) MUX(2, double, LV)
 #define X2F(LV) MUX(2, float, LV)
 #define X3D(LV) MUX(3, double, LV)
@@ -469,7 +469,8 @@ int sweep_device(vacgpu_grid* g, const vacgpu_method* m, int d, int firstsweep, 
       } else {
         const dim3 gr = g->grid_tm[d];
         nbc = (long long)gr.x * gr.y * gr.z;
-#define MUM(NDV, RT, DV, LV) hipLaunchKernelGGL((k_tvdmu_m<NDV, RT, DV, LV>), gr, g->block, 0, g->stream, g->geo, s, g->bc, win, g->w1, wout, g->sc, g->part, g->pstride, g->seglen, ghosts)
+#define MUM(NDV, RT, DV, LV) do { if (mhd) hipLaunchKernelGGL((k_tvdmu_m<NDV, RT, DV, LV, true>), gr, g->block, 0, g->stream, g->geo, s, g->bc, win, g->w1, wout, g->sc, g->part, g->pstride, g->seglen, ghosts); \
+                                   else hipLaunchKernelGGL((k_tvdmu_m<NDV, RT, DV, LV, false>), gr, g->block, 0, g->stream, g->geo, s, g->bc, win, g->w1, wout, g->sc, g->part, g->pstride, g->seglen, ghosts); } while (0)
 #define M21D(LV) MUM(2, double, 1, LV)
 #define M21F(LV) MUM(2, float, 1, LV)
 #define M31D(LV) MUM(3, double, 1, LV)
@@ -583,7 +584,41 @@ int sweep_device(vacgpu_grid* g, const vacgpu_method* m, int d, int firstsweep, 
       const bool f32 = g->cfg.precision == VACGPU_FP32BULK;
 #define LIMSW(X) do { if (lim == LIM_MINMOD) X(LIM_MINMOD); else if (lim == LIM_WOODWARD) X(LIM_WOODWARD); else if (lim == LIM_SUPERBEE) X(LIM_SUPERBEE); else X(-1); } while (0)
 #define LIMSW3(X) do { if (lim == LIM_MINMOD) X(LIM_MINMOD); else if (lim == LIM_WOODWARD) X(LIM_WOODWARD); else X(LIM_SUPERBEE); } while (0)
-      if (d == 0) {
+      if (!mhd) {
+        // HD: the HD Roe solver, no source steps
+        const dim3 gr = d == 0 ? g->grid_tx : g->grid_tm[d];
+        nbt = (long long)gr.x * gr.y * gr.z;
+#define HX(NDV, RT, LV) hipLaunchKernelGGL((k_tvd_x<NDV, RT, LV, 0, false>), gr, g->block, 0, g->stream, g->geo, s, g->bc, wct, win, wout, g->sc, g->part, g->pstride, wold, g->w1, ghosts)
+#define HM(NDV, RT, DV, LV) hipLaunchKernelGGL((k_tvd_m<NDV, RT, DV, LV, 0, false>), gr, g->block, 0, g->stream, g->geo, s, g->bc, wct, win, wout, g->sc, g->part, g->pstride, g->seglen, wold, g->w1, ghosts)
+#define HX2D(LV) HX(2, double, LV)
+#define HX2F(LV) HX(2, float, LV)
+#define HX3D(LV) HX(3, double, LV)
+#define HX3F(LV) HX(3, float, LV)
+#define HM21D(LV) HM(2, double, 1, LV)
+#define HM21F(LV) HM(2, float, 1, LV)
+#define HM31D(LV) HM(3, double, 1, LV)
+#define HM31F(LV) HM(3, float, 1, LV)
+#define HM32D(LV) HM(3, double, 2, LV)
+#define HM32F(LV) HM(3, float, 2, LV)
+        if (d == 0) {
+          if (g->cfg.ndim == 2) { if (f32) LIMSW(HX2F); else LIMSW(HX2D); }
+          else { if (f32) LIMSW(HX3F); else LIMSW(HX3D); }
+        } else if (g->cfg.ndim == 2) { if (f32) LIMSW(HM21F); else LIMSW(HM21D); }
+        else if (d == 1) { if (f32) LIMSW(HM31F); else LIMSW(HM31D); }
+        else { if (f32) LIMSW(HM32F); else LIMSW(HM32D); }
+#undef HX2D
+#undef HX2F
+#undef HX3D
+#undef HX3F
+#undef HM21D
+#undef HM21F
+#undef HM31D
+#undef HM31F
+#undef HM32D
+#undef HM32F
+#undef HX
+#undef HM
+      } else if (d == 0) {
         const dim3 gr = g->grid_tx;
         nbt = (long long)gr.x * gr.y * gr.z;
 #define TVDX_L(NDV, RT, LV, SV) hipLaunchKernelGGL((k_tvd_x<NDV, RT, LV, SV>), gr, g->block, 0, g->stream, g->geo, s, g->bc, wct, win, wout, g->sc, g->part, g->pstride, wold, g->w1, ghosts)
