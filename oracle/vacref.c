@@ -401,14 +401,16 @@ static void tvdlf_corrector(ctx_t* x, double qdt, int d, const double* wCT, doub
   long s = x->st[d];
   double* wL = x->wLC;
   double* wR = x->wRC;
+  /* the limiter's base, typelimited (src/vactvdlf.t:150-161) */
+  const double* wlim = x->p->typelimited == 1 ? x->wold : (x->p->typelimited == 2 ? wCT : wnew);
   for (int iw = 0; iw < x->nw; ++iw) {
     const double* pc = PL(x, wCT, iw);
     double* pl = PL(x, wL, iw);
     double* pr = PL(x, wR, iw);
     LOOP(lo, hi) {
       long idx = IDX(x);
-      double lj = ldw_at(x, wnew, iw, d, idx);
-      double lj1 = ldw_at(x, wnew, iw, d, idx + s);
+      double lj = ldw_at(x, wlim, iw, d, idx);
+      double lj1 = ldw_at(x, wlim, iw, d, idx + s);
       pr[idx] = pc[idx + s] - 0.5 * lj1;
       pl[idx] = pc[idx] + 0.5 * lj;
     }
@@ -1427,6 +1429,8 @@ int vacref_run(const vacref_params* p, vacref_state* s, double* w,
   if (rc) return rc;
   if (p->dtpar > 0.0 && s->smallp < 0.0) { ctx_free(&x); return -3; }
   if (p->nproc1 != 0 && (!x.mhd || x.ndim < 2)) { ctx_free(&x); return -5; }  /* ^NOONED, MHD */
+  /* TVD with typelimited='predictor' limits the centdiff result (src/vactvd.t:30-36) */
+  if (p->typefull == VACREF_TVD && p->typelimited == 2) { ctx_free(&x); return -6; }
   getboundary(&x, w);
   int done = 0;
   for (;;) {

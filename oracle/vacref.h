@@ -42,6 +42,11 @@ typedef struct {
    * (projectb, src/vacproc.projectb.t) with CG (src/vacpoisson.t:391-595). */
   int nproc1;
   double procpar[3];     /* procpar(divbcoeff_, divbconst_, divbbound_); -1 = unset */
+  /* typelimited: the state the corrector of TVDLF / TVD-MUSCL limits
+   * (src/vactvdlf.t:150-161): 0 'original' (the sweep's input), 1 'previous'
+   * (wold, the step's start), 2 'predictor' (wCT); for TVD 'previous' limits
+   * wCT as 'original' does (src/vactvd.t:37-38) */
+  int typelimited;
 } vacref_params;
 
 typedef struct {

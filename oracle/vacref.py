@@ -31,6 +31,7 @@ class Params(C.Structure):
         ("courantpar", C.c_double), ("smallpcoeff", C.c_double),
         ("dtpar", C.c_double), ("tmax", C.c_double), ("tmaxexact", C.c_int),
         ("itmax", C.c_int), ("nproc1", C.c_int), ("procpar", C.c_double * 3),
+        ("typelimited", C.c_int),
     ]
 
 
@@ -84,7 +85,7 @@ def make_params(ndim, physics, nx, gamma, typefull="tvdlf", limiter="minmod",
                 bc="periodic", divbfix=True, divbwave=True, sourceunsplit=True,
                 dimsplit_xy=False, courantpar=0.8, smallpcoeff=1e-6, dtpar=-1.0,
                 tmax=1e99, tmaxexact=False, itmax=1000000, nproc1=0,
-                procpar=(-1.0, -1.0, -1.0)) -> Params:
+                procpar=(-1.0, -1.0, -1.0), typelimited="original") -> Params:
     p = Params()
     p.ndim = ndim
     p.phys = MHD if physics == "mhd" else HD
@@ -113,6 +114,7 @@ def make_params(ndim, physics, nx, gamma, typefull="tvdlf", limiter="minmod",
     p.tmax = tmax
     p.tmaxexact = int(tmaxexact)
     p.itmax = itmax
+    p.typelimited = {"original": 0, "previous": 1, "predictor": 2}[typelimited]
     p.nproc1 = int(nproc1)
     for i in range(3):
         p.procpar[i] = float(procpar[i])

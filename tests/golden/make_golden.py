@@ -98,6 +98,17 @@ HD_TVD_CASES = {
 BC_CASES.update(HD_TVD_CASES)
 
 
+# typelimited: the corrector of TVDLF / TVD-MUSCL limits the step's start
+# ('previous') or the predictor ('predictor') instead of the sweep's input
+# name: (binary, physics, builder, gamma, steps, typefull, limiter, bc, typelimited)
+TL_CASES = {
+    "mhd22_rf32x28_tvdlf_minmod_previous": ("vac_mhd22_132", "mhd", lambda g: random_fourier((32, 28), 601, g), 5 / 3, 12, "tvdlf", "minmod", "periodic", "previous"),
+    "mhd22_rf28x30_tvdlf_woodward_cont_predictor": ("vac_mhd22_132", "mhd", lambda g: random_fourier((28, 30), 602, g), 1.4, 12, "tvdlf", "woodward", "cont", "predictor"),
+    "mhd22_rf30x26_tvdmu_superbee_previous": ("vac_mhd22_132", "mhd", lambda g: random_fourier((30, 26), 603, g), 5 / 3, 10, "tvdmu", "superbee", "periodic", "previous"),
+    "hd22_rf26x32_tvdlf_minmod_predictor": ("vac_hd22_132", "hd", lambda g: random_fourier((26, 32), 604, g, physics="hd"), 1.4, 15, "tvdlf", "minmod", "periodic", "predictor"),
+}
+
+
 def ensure_binary(name: str) -> None:
     if have_ref(name):
         return
@@ -175,6 +186,24 @@ def main() -> None:
             nx=np.array([len(c) for c in coords]), gamma=gamma, steps=steps,
             physics=physics, typefull=typefull, limiter=limiter, bc=json.dumps(bc),
             divbfix=divbfix, **extra)
+        print("wrote", name, w0.shape, "steps", len(t) - 1)
+    for name, (binary, physics, build, gamma, steps, typefull, limiter, bc, tl) in TL_CASES.items():
+        if only and not any(o in name for o in only):
+            continue
+        ensure_binary(binary)
+        w0, coords = build(gamma)
+        nw = w0.shape[0]
+        ini = make_ini(w0, coords, physics, gamma, name="golden")
+        par = problem_par(nw, NAMES[nw], steps, typefull=typefull, limiter=limiter,
+                          typeB=[bc] * (2 * len(coords)), every_step=True,
+                          extra_method=f" typelimited='{tl}'\n")
+        r = run_ref(binary, ini, par)
+        t = np.array([s_.t for s_ in r.snapshots])
+        np.savez_compressed(
+            os.path.join(HERE, name + ".npz"), w0=w0, wf=r.snapshots[-1].w, t=t,
+            nx=np.array([len(c) for c in coords]), gamma=gamma, steps=steps,
+            physics=physics, typefull=typefull, limiter=limiter, bc=bc, divbfix=True,
+            typelimited=tl)
         print("wrote", name, w0.shape, "steps", len(t) - 1)
     if only:
         return

@@ -39,8 +39,10 @@ def run_fixture(fx):
     from vac_amd.grid import cell_centres
     dxs = [widths_1d(cell_centres(n)) for n in nx]
     kw = {}
+    if "typelimited" in fx:
+        kw["typelimited"] = str(fx["typelimited"])
     if "nproc1" in fx:  # projection of B before every step
-        kw = dict(divbwave=bool(fx["divbwave"]), nproc1=int(fx["nproc1"]),
+        kw.update(divbwave=bool(fx["divbwave"]), nproc1=int(fx["nproc1"]),
                   procpar=[float(v) for v in fx["procpar"]])
     p = vacref.make_params(ndim, str(fx["physics"]), nx, gamma,
                            typefull=str(fx["typefull"]), limiter=str(fx["limiter"]),
