@@ -36,6 +36,7 @@ enum { VACGPU_MINMOD = 0, VACGPU_WOODWARD = 1, VACGPU_SUPERBEE = 2 };  /* typeli
 enum { VACGPU_CONT = 0, VACGPU_PERIODIC = 1, VACGPU_SYMM = 2, VACGPU_ASYMM = 3,
        VACGPU_FIXED = 4 };                             /* typeB */
 enum { VACGPU_FP64 = 0, VACGPU_FP32BULK = 1 };          /* arithmetic tier */
+enum { VACGPU_LIM_ORIGINAL = 0, VACGPU_LIM_PREVIOUS = 1, VACGPU_LIM_PREDICTOR = 2 };  /* typelimited */
 
 typedef struct vacgpu_grid vacgpu_grid;
 
@@ -60,6 +61,11 @@ typedef struct {
   int dimsplit_xy;      /* 1: typedimsplit='xy'; 0: 'xyyx' (sweep order flips with it) */
   double courantpar;
   double smallpcoeff;
+  int typelimited;      /* the state the TVDLF / TVD-MUSCL corrector limits
+                           (src/vactvdlf.t:150-161): VACGPU_LIM_ORIGINAL (the
+                           sweep's input), VACGPU_LIM_PREVIOUS (the step's
+                           start, wold), VACGPU_LIM_PREDICTOR (wCT); TVD takes
+                           'original' or 'previous', which limit alike */
 } vacgpu_method;
 
 /* Replaces the geometry hand-off after gridsetup1 (src/vacgrid.t:457-593):

@@ -43,14 +43,19 @@ CASES = [
      "tvdlf", "minmod", [WALL_X, WALL_X, "fixed", "fixed"], True),
     ("vac_mhd22_132", "vacgpu_mhd22_132", "mhd", lambda g: random_fourier((36, 44), 69, g), 5 / 3, 12,
      "tvd", "woodward", ["periodic", "periodic", WALL_Y, WALL_Y], True),
+    # typelimited='previous': the corrector limits the step's start
+    ("vac_mhd22_132", "vacgpu_mhd22_132", "mhd", lambda g: random_fourier((40, 40), 72, g), 5 / 3, 12,
+     "tvdlf", "woodward", "periodic", True, "previous"),
 ]
 
 
 @pytest.mark.parametrize("case", CASES, ids=[f"{c[6]}-{c[7]}-{c[2]}{len(c[3](1.4)[1])}d"
                                              + ("" if isinstance(c[8], str) else "-walls")
+                                             + (f"-{c[10]}" if len(c) > 10 else "")
                                              for c in CASES])
 def test_vac_with_gpu_advect_matches_vac(case):
-    cpu, gpu, physics, build, gamma, steps, typefull, limiter, bc, divbfix = case
+    cpu, gpu, physics, build, gamma, steps, typefull, limiter, bc, divbfix = case[:10]
+    extra = f" typelimited='{case[10]}'\n" if len(case) > 10 else ""
     if not (have_ref(cpu) and have_ref(gpu)):
         pytest.skip("reference executables not built (oracle/build_ref.sh)")
     w0, coords = build(gamma)
@@ -58,7 +63,7 @@ def test_vac_with_gpu_advect_matches_vac(case):
     ini = make_ini(w0, coords, physics, gamma, name="dropin")
     typeB = [bc] * (2 * len(coords)) if isinstance(bc, str) else bc
     par = problem_par(nw, NAMES[nw], steps, typefull=typefull, limiter=limiter,
-                      typeB=typeB, divbfix=divbfix, every_step=True)
+                      typeB=typeB, divbfix=divbfix, every_step=True, extra_method=extra)
     print(f"case {cpu} vs {gpu} {typefull} {limiter} {bc}", flush=True)
     rc = run_ref(cpu, ini, par, timeout=300)
     rg = run_ref(gpu, ini, par, timeout=300)

@@ -61,8 +61,9 @@ def gpu_run(w0, nx, physics, gamma, steps, method, bc="periodic", precision="fp6
 def test_gpu_matches_reference_fixture(path):
     z = np.load(path, allow_pickle=False)
     nx = [int(v) for v in z["nx"]]
+    tl = str(z["typelimited"]) if "typelimited" in z.files else "original"
     method = vacgpu.MethodSpec(typefull=str(z["typefull"]), limiter=str(z["limiter"]),
-                               divbfix=bool(z["divbfix"]))
+                               divbfix=bool(z["divbfix"]), typelimited=tl)
     print(f"case {os.path.basename(path)} nx={nx}", flush=True)
     w, t, sc = gpu_run(z["w0"], nx, str(z["physics"]), float(z["gamma"]), int(z["steps"]),
                        method, bc=vacref.bc_spec(z["bc"]))
@@ -188,3 +189,29 @@ def test_courant_violation_counted_like_vac(typefull, limiter):
     assert list(sc["nerror"]) == list(st.nerror)
     assert np.array_equal(t, tr)
     assert_bitwise(w, wr, "courant violation")
+
+
+@pytest.mark.parametrize("case", [
+    ("mhd", (64, 56), "tvdlf", "minmod", "periodic", "previous", 12),
+    ("mhd", (48, 60), "tvdlf", "woodward", "cont", "predictor", 12),
+    ("mhd", (14, 12, 16), "tvdlf", "superbee", "periodic", "previous", 5),
+    ("mhd", (56, 40), "tvdmu", "woodward", "periodic", "predictor", 10),
+    ("mhd", (12, 16, 10), "tvdmu", "minmod", "cont", "previous", 5),
+    ("hd", (64, 48), "tvdlf", "woodward", "periodic", "predictor", 15),
+    ("mhd", (48, 48), "tvd", "woodward", "periodic", "previous", 10),
+])
+def test_gpu_typelimited_matches_oracle(case):
+    # the corrector limits the step's start ('previous') or the predictor
+    # ('predictor'); the fused 2D TVDLF sweep, which shares the predictor's
+    # slopes, stands aside for the two-launch form
+    physics, nx, typefull, limiter, bc, tl, steps = case
+    gamma = 1.4
+    w0, coords = random_fourier(nx, 21 + len(nx), gamma, physics=physics)
+    method = vacgpu.MethodSpec(typefull=typefull, limiter=limiter, typelimited=tl)
+    print(f"case {case}", flush=True)
+    w, t, sc = gpu_run(w0, list(nx), physics, gamma, steps, method, bc=bc)
+    p = vacref.make_params(len(nx), physics, list(nx), gamma, typefull=typefull,
+                           limiter=limiter, bc=bc, typelimited=tl)
+    wr, tr, st = vacref.run(p, w0, [widths_1d(c) for c in coords], steps)
+    assert np.array_equal(t, tr)
+    assert_bitwise(w, wr, str(case))

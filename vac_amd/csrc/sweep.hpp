@@ -542,7 +542,7 @@ __global__ __launch_bounds__(256) void k_tvdlf_corrx(Geo g, Sel s, BC bc, const 
                                                     double* __restrict__ wout,
                                                     const Scal* __restrict__ sc,
                                                     double* __restrict__ part, long long pstride,
-                                                    int ghosts) {
+                                                    int ghosts, const double* __restrict__ wl) {
   constexpr int D = 0;
   using L = Layout<ND, MHD>;
   constexpr int NW = L::nw;
@@ -561,7 +561,7 @@ __global__ __launch_bounds__(256) void k_tvdlf_corrx(Geo g, Sel s, BC bc, const 
   double l[NW], lr[NW], u0[NW], u1[NW];
 #pragma unroll
   for (int iw = 0; iw < NW; ++iw) {
-    const double* p = w + iw * g.ps + idx;
+    const double* p = wl + iw * g.ps + idx;  // the limiter's base (typelimited)
     l[iw] = slope<R, LIM>(s.lim[iw], p[-1], p[0], p[1]);
     u0[iw] = w1[iw * g.ps + idx];
     u1[iw] = w1[iw * g.ps + idx + 1];
@@ -599,7 +599,8 @@ __global__ __launch_bounds__(256) void k_tvdlf_corrm(Geo g, Sel s, BC bc, const 
                                                     double* __restrict__ wout,
                                                     const Scal* __restrict__ sc,
                                                     double* __restrict__ part, long long pstride,
-                                                    int seglen, int ghosts) {
+                                                    int seglen, int ghosts,
+                                                    const double* __restrict__ wl) {
   using L = Layout<ND, MHD>;
   constexpr int NW = L::nw;
   __shared__ double sm[8];
@@ -618,11 +619,14 @@ __global__ __launch_bounds__(256) void k_tvdlf_corrm(Geo g, Sel s, BC bc, const 
     const double* dxd = g.dx[D];
     int c3[3] = {x, 0, 0};
     if (ND == 3) c3[other] = t;
-    // window: w at J-1, J, J+1 (for the slope at J) and w1 at J; slope at J
+    // window: the limiter's base wl (typelimited) at J-1, J, J+1 (for the
+    // slope at J) and w1 at J; slope at J.  The update's base is w, the
+    // window itself when wl is w.
+    const bool own = wl == w;
     double wa[NW], wb[NW], wc[NW], u0[NW], lc[NW], Fp[NW];
-    load<NW>(w, g.ps, base + (long long)(s0 - 2) * sd, wa);
-    load<NW>(w, g.ps, base + (long long)(s0 - 1) * sd, wb);
-    load<NW>(w, g.ps, base + (long long)s0 * sd, wc);
+    load<NW>(wl, g.ps, base + (long long)(s0 - 2) * sd, wa);
+    load<NW>(wl, g.ps, base + (long long)(s0 - 1) * sd, wb);
+    load<NW>(wl, g.ps, base + (long long)s0 * sd, wc);
     load<NW>(w1, g.ps, base + (long long)(s0 - 1) * sd, u0);
 #pragma unroll
     for (int iw = 0; iw < NW; ++iw) lc[iw] = slope<R, LIM>(s.lim[iw], wa[iw], wb[iw], wc[iw]);
@@ -630,7 +634,7 @@ __global__ __launch_bounds__(256) void k_tvdlf_corrm(Geo g, Sel s, BC bc, const 
     {
       double u1[NW], ln[NW], wd[NW];
       load<NW>(w1, g.ps, base + (long long)s0 * sd, u1);
-      load<NW>(w, g.ps, base + (long long)(s0 + 1) * sd, wd);
+      load<NW>(wl, g.ps, base + (long long)(s0 + 1) * sd, wd);
 #pragma unroll
       for (int iw = 0; iw < NW; ++iw) ln[iw] = slope<R, LIM>(s.lim[iw], wb[iw], wc[iw], wd[iw]);
       double cv;
@@ -649,7 +653,7 @@ __global__ __launch_bounds__(256) void k_tvdlf_corrm(Geo g, Sel s, BC bc, const 
     for (int i = s0; i <= s1; ++i) {
       double u1[NW], ln[NW], wd[NW], F[NW], cv;
       load<NW>(w1, g.ps, base + (long long)(i + 1) * sd, u1);
-      load<NW>(w, g.ps, base + (long long)(i + 2) * sd, wd);
+      load<NW>(wl, g.ps, base + (long long)(i + 2) * sd, wd);
 #pragma unroll
       for (int iw = 0; iw < NW; ++iw) ln[iw] = slope<R, LIM>(s.lim[iw], wb[iw], wc[iw], wd[iw]);
       corr_face<ND, MHD, R, D>(s, u0, u1, lc, ln, dxd[i] + dxd[i + 1], smallp, F, cv);
@@ -658,7 +662,7 @@ __global__ __launch_bounds__(256) void k_tvdlf_corrm(Geo g, Sel s, BC bc, const 
       const double qdx = pdiv(dt, dxd[i]);
       double out[NW];
 #pragma unroll
-      for (int iw = 0; iw < NW; ++iw) out[iw] = wb[iw] - qdx * (F[iw] - Fp[iw]);
+      for (int iw = 0; iw < NW; ++iw) out[iw] = (own ? wb[iw] : w[iw * g.ps + idx]) - qdx * (F[iw] - Fp[iw]);
       if constexpr (MHD) {
         if (s.sourceunsplit && s.divbfix) {
           c3[D] = i;
@@ -1269,7 +1273,7 @@ __global__ __launch_bounds__(256) void k_tvdmu_x(Geo g, Sel s, BC bc, const doub
                                                 double* __restrict__ wout,
                                                 const Scal* __restrict__ sc,
                                                 double* __restrict__ part, long long pstride,
-                                                int ghosts) {
+                                                int ghosts, const double* __restrict__ wl) {
   using L = Layout<ND, MHD>;
   constexpr int NW = L::nw;
   __shared__ double sm[8];
@@ -1285,11 +1289,13 @@ __global__ __launch_bounds__(256) void k_tvdmu_x(Geo g, Sel s, BC bc, const doub
   const int yc = rowok ? y : g.n[1] + 1;
   const long long idx = g.off + cc + (long long)yc * g.s[1] + (long long)z * g.s[2];
   double l[NW], lr[NW], u0[NW], u1[NW], wc[NW];
+  const bool own = wl == w;  // the limiter's base (typelimited) is the sweep's input
 #pragma unroll
   for (int iw = 0; iw < NW; ++iw) {
-    const double* p = w + iw * g.ps + idx;
-    wc[iw] = p[0];
-    l[iw] = slope<R, LIM>(s.lim[iw], p[-1], wc[iw], p[1]);
+    const double* p = wl + iw * g.ps + idx;
+    const double pc = p[0];
+    wc[iw] = own ? pc : w[iw * g.ps + idx];
+    l[iw] = slope<R, LIM>(s.lim[iw], p[-1], pc, p[1]);
     u0[iw] = w1[iw * g.ps + idx];
     u1[iw] = w1[iw * g.ps + idx + 1];
   }
@@ -1338,7 +1344,8 @@ __global__ __launch_bounds__(256) void k_tvdmu_m(Geo g, Sel s, BC bc, const doub
                                                 double* __restrict__ wout,
                                                 const Scal* __restrict__ sc,
                                                 double* __restrict__ part, long long pstride,
-                                                int seglen, int ghosts) {
+                                                int seglen, int ghosts,
+                                                const double* __restrict__ wl) {
   using L = Layout<ND, MHD>;
   constexpr int NW = L::nw;
   __shared__ double sm[8];
@@ -1366,21 +1373,24 @@ __global__ __launch_bounds__(256) void k_tvdmu_m(Geo g, Sel s, BC bc, const doub
     int c3[3] = {x, 0, 0};
     if (ND == 3) c3[other] = t;
     double a, b;
-    // window: w at J, J+1, w1 at J, slope at J; the face J-1/2 below
+    // window: the limiter's base wl (typelimited) at J, J+1, w1 at J, slope
+    // at J; the face J-1/2 below.  The update's base is w, the window itself
+    // when wl is w.
+    const bool own = wl == w;
     double wb[NW], wc[NW], u0[NW], lc[NW], Fp[NW];
     TvdFace<ND, R, MHD> fp;
     R phip[NW];
     {
       double wa[NW], u1[NW], ln[NW], wd[NW];
-      load<NW>(w, g.ps, base + (long long)(s0 - 2) * sd, wa);
-      load<NW>(w, g.ps, base + (long long)(s0 - 1) * sd, wb);
-      load<NW>(w, g.ps, base + (long long)s0 * sd, wc);
+      load<NW>(wl, g.ps, base + (long long)(s0 - 2) * sd, wa);
+      load<NW>(wl, g.ps, base + (long long)(s0 - 1) * sd, wb);
+      load<NW>(wl, g.ps, base + (long long)s0 * sd, wc);
       load<NW>(w1, g.ps, base + (long long)(s0 - 1) * sd, u0);
 #pragma unroll
       for (int iw = 0; iw < NW; ++iw) lc[iw] = slope<R, LIM>(s.lim[iw], wa[iw], wb[iw], wc[iw]);
       // face s0-1/2
       load<NW>(w1, g.ps, base + (long long)s0 * sd, u1);
-      load<NW>(w, g.ps, base + (long long)(s0 + 1) * sd, wd);
+      load<NW>(wl, g.ps, base + (long long)(s0 + 1) * sd, wd);
 #pragma unroll
       for (int iw = 0; iw < NW; ++iw) ln[iw] = slope<R, LIM>(s.lim[iw], wb[iw], wc[iw], wd[iw]);
       mu_face<ND, R, D, MHD>(s, u0, u1, lc, ln, dt, smallp, dxd[s0 - 1] + dxd[s0], Fp, fp, phip, a, b);
@@ -1399,7 +1409,7 @@ __global__ __launch_bounds__(256) void k_tvdmu_m(Geo g, Sel s, BC bc, const doub
       TvdFace<ND, R, MHD> f;
       R phi[NW];
       load<NW>(w1, g.ps, base + (long long)(i + 1) * sd, u1);
-      load<NW>(w, g.ps, base + (long long)(i + 2) * sd, wd);
+      load<NW>(wl, g.ps, base + (long long)(i + 2) * sd, wd);
 #pragma unroll
       for (int iw = 0; iw < NW; ++iw) ln[iw] = slope<R, LIM>(s.lim[iw], wb[iw], wc[iw], wd[iw]);
       mu_face<ND, R, D, MHD>(s, u0, u1, lc, ln, dt, smallp, dxd[i] + dxd[i + 1], F, f, phi, a, b);
@@ -1408,7 +1418,7 @@ __global__ __launch_bounds__(256) void k_tvdmu_m(Geo g, Sel s, BC bc, const doub
       const double qdx = pdiv(dt, dxd[i]);
       double out[NW];
 #pragma unroll
-      for (int iw = 0; iw < NW; ++iw) out[iw] = wb[iw] - qdx * (F[iw] - Fp[iw]);
+      for (int iw = 0; iw < NW; ++iw) out[iw] = (own ? wb[iw] : w[iw * g.ps + idx]) - qdx * (F[iw] - Fp[iw]);
       const double dm = dvol(i - 1), d0 = dvol(i), dp = dvol(i + 1);
       tvd_dissipate<ND, R, MHD>(fp.rt, phip, f.rt, phi, D, s, R(d0 + dm), R(dp + d0), d0, out);
       c3[D] = i;

@@ -192,6 +192,8 @@ int check_config(const vacgpu_config* c, std::string& why) {
 int check_method(const vacgpu_config* c, const vacgpu_method* m, std::string& why) {
   if (!m) { why = "null method"; return VACGPU_E_ARG; }
   if (m->typefull != VACGPU_TVDLF && m->typefull != VACGPU_TVD && m->typefull != VACGPU_TVDMU) { why = "typefull must be 'tvdlf', 'tvd' or 'tvdmu'"; return VACGPU_E_ARG; }
+  if (m->typelimited < VACGPU_LIM_ORIGINAL || m->typelimited > VACGPU_LIM_PREDICTOR) { why = "typelimited must be 'original', 'previous' or 'predictor'"; return VACGPU_E_ARG; }
+  if (m->typefull == VACGPU_TVD && m->typelimited == VACGPU_LIM_PREDICTOR) { why = "typelimited='predictor' with typefull='tvd' (limiting the centred-difference result) is not on the GPU path"; return VACGPU_E_ARG; }
   int nw = c->physics == VACGPU_MHD ? 2 + 2 * c->ndim : 2 + c->ndim;
   for (int iw = 0; iw < nw; ++iw)
     if (m->typelimiter[iw] < VACGPU_MINMOD || m->typelimiter[iw] > VACGPU_SUPERBEE) { why = "typelimiter must be minmod, woodward or superbee"; return VACGPU_E_ARG; }
@@ -402,8 +404,12 @@ int sweep_device(vacgpu_grid* g, const vacgpu_method* m, int d, int firstsweep, 
   // variables, and for MHD the Powell source on (VAC's defaults)
   bool one_lim = true;
   for (int iw = 1; iw < g->nw; ++iw) one_lim &= m->typelimiter[iw] == m->typelimiter[0];
+  // the limiter's base (typelimited, src/vactvdlf.t:150-161)
+  const double* wl = m->typelimited == VACGPU_LIM_PREVIOUS ? wold
+                     : (m->typelimited == VACGPU_LIM_PREDICTOR ? g->w1 : win);
   const bool fused = m->typefull == VACGPU_TVDLF && g->cfg.ndim == 2 && g->fused_ok &&
-                     g->use_fused && one_lim && (!mhd || src) && !dirty;
+                     g->use_fused && one_lim && (!mhd || src) && !dirty &&
+                     m->typelimited == VACGPU_LIM_ORIGINAL;
   if (fused) {
     const long long nbf = (long long)g->grid_fs.x * g->grid_fs.y;
     {
@@ -453,8 +459,8 @@ int sweep_device(vacgpu_grid* g, const vacgpu_method* m, int d, int firstsweep, 
       if (d == 0) {
         const dim3 gr((g->geo.n[0] + CX_STRIDE - 1) / CX_STRIDE, g->grid.y, g->grid.z);
         nbc = (long long)gr.x * gr.y * gr.z;
-#define MUX(NDV, RT, LV) do { if (mhd) hipLaunchKernelGGL((k_tvdmu_x<NDV, RT, LV, true>), gr, g->block, 0, g->stream, g->geo, s, g->bc, win, g->w1, wout, g->sc, g->part, g->pstride, ghosts); \
-                               else hipLaunchKernelGGL((k_tvdmu_x<NDV, RT, LV, false>), gr, g->block, 0, g->stream, g->geo, s, g->bc, win, g->w1, wout, g->sc, g->part, g->pstride, ghosts); } while (0)
+#define MUX(NDV, RT, LV) do { if (mhd) hipLaunchKernelGGL((k_tvdmu_x<NDV, RT, LV, true>), gr, g->block, 0, g->stream, g->geo, s, g->bc, win, g->w1, wout, g->sc, g->part, g->pstride, ghosts, wl); \
+                               else hipLaunchKernelGGL((k_tvdmu_x<NDV, RT, LV, false>), gr, g->block, 0, g->stream, g->geo, s, g->bc, win, g->w1, wout, g->sc, g->part, g->pstride, ghosts, wl); } while (0)
 #define X2D(LV) MUX(2, double, LV)
 #define X2F(LV) MUX(2, float, LV)
 #define X3D(LV) MUX(3, double, LV)
@@ -469,8 +475,8 @@ int sweep_device(vacgpu_grid* g, const vacgpu_method* m, int d, int firstsweep, 
       } else {
         const dim3 gr = g->grid_tm[d];
         nbc = (long long)gr.x * gr.y * gr.z;
-#define MUM(NDV, RT, DV, LV) do { if (mhd) hipLaunchKernelGGL((k_tvdmu_m<NDV, RT, DV, LV, true>), gr, g->block, 0, g->stream, g->geo, s, g->bc, win, g->w1, wout, g->sc, g->part, g->pstride, g->seglen, ghosts); \
-                                   else hipLaunchKernelGGL((k_tvdmu_m<NDV, RT, DV, LV, false>), gr, g->block, 0, g->stream, g->geo, s, g->bc, win, g->w1, wout, g->sc, g->part, g->pstride, g->seglen, ghosts); } while (0)
+#define MUM(NDV, RT, DV, LV) do { if (mhd) hipLaunchKernelGGL((k_tvdmu_m<NDV, RT, DV, LV, true>), gr, g->block, 0, g->stream, g->geo, s, g->bc, win, g->w1, wout, g->sc, g->part, g->pstride, g->seglen, ghosts, wl); \
+                                   else hipLaunchKernelGGL((k_tvdmu_m<NDV, RT, DV, LV, false>), gr, g->block, 0, g->stream, g->geo, s, g->bc, win, g->w1, wout, g->sc, g->part, g->pstride, g->seglen, ghosts, wl); } while (0)
 #define M21D(LV) MUM(2, double, 1, LV)
 #define M21F(LV) MUM(2, float, 1, LV)
 #define M31D(LV) MUM(3, double, 1, LV)
@@ -524,13 +530,13 @@ int sweep_device(vacgpu_grid* g, const vacgpu_method* m, int d, int firstsweep, 
       if (d == 0) {
         const dim3 gr((g->geo.n[0] + CX_STRIDE - 1) / CX_STRIDE, g->grid.y, g->grid.z);
         nbc = (long long)gr.x * gr.y * gr.z;
-#define CX(LV) hipLaunchKernelGGL((k_tvdlf_corrx<ND, MHD, R, LV>), gr, g->block, 0, g->stream, g->geo, s, g->bc, win, g->w1, wout, g->sc, g->part, g->pstride, ghosts)
+#define CX(LV) hipLaunchKernelGGL((k_tvdlf_corrx<ND, MHD, R, LV>), gr, g->block, 0, g->stream, g->geo, s, g->bc, win, g->w1, wout, g->sc, g->part, g->pstride, ghosts, wl)
         DISPATCH(g, { LIMSW(CX); });
 #undef CX
       } else {
         const dim3 gr = g->grid_tm[d];
         nbc = (long long)gr.x * gr.y * gr.z;
-#define CM(DV, LV) hipLaunchKernelGGL((k_tvdlf_corrm<ND, MHD, R, DV, LV>), gr, g->block, 0, g->stream, g->geo, s, g->bc, win, g->w1, wout, g->sc, g->part, g->pstride, g->seglen, ghosts)
+#define CM(DV, LV) hipLaunchKernelGGL((k_tvdlf_corrm<ND, MHD, R, DV, LV>), gr, g->block, 0, g->stream, g->geo, s, g->bc, win, g->w1, wout, g->sc, g->part, g->pstride, g->seglen, ghosts, wl)
 #define CM1(LV) CM(1, LV)
 #define CM2(LV) CM((ND == 3 ? 2 : 1), LV)
         DISPATCH(g, { if (d == 1) LIMSW(CM1); else LIMSW(CM2); });

@@ -41,7 +41,6 @@ if(idimmin/=idimmax) call die('vacgpu advect: needs dimsplit=T')
 if(nproc(2)/=0) call die('vacgpu advect: nproc(2) processing is not on the GPU path')
 if(typeaxial/='slab' .or. gencoord) call die('vacgpu advect: slab Cartesian grids only')
 if(useprimitive) call die('vacgpu advect: useprimitive=T is not on the GPU path')
-if(typelimited/='original') call die('vacgpu advect: typelimited must be original')
 if(smallfix .or. extraB) call die('vacgpu advect: smallfix/extraB are not on the GPU path')
 if(acmexpo>zero) call die('vacgpu advect: acmexpo>0 is not on the GPU path')
 if(ndir/=ndim) call die('vacgpu advect: ndir must equal ndim')
@@ -87,6 +86,16 @@ m%dimsplit_xy=merge(1,0,typedimsplit=='xy')
 m%courantpar=courantpar
 if(implpar>=zero) m%courantpar=-one
 m%smallpcoeff=smallpcoeff
+select case(typelimited)
+case('original')
+   m%typelimited=VACGPU_LIM_ORIGINAL
+case('previous')
+   m%typelimited=VACGPU_LIM_PREVIOUS
+case('predictor')
+   m%typelimited=VACGPU_LIM_PREDICTOR
+case default
+   call die('vacgpu advect: typelimited='//typelimited//' is not on the GPU path')
+end select
 
 if(firstsweep)then
    rc=vacgpu_put_state_ghosts(grid,w,ld)

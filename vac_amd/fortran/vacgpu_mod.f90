@@ -19,6 +19,8 @@ module vacgpu_mod
   integer(c_int), parameter :: VACGPU_CONT = 0, VACGPU_PERIODIC = 1, VACGPU_SYMM = 2, &
        VACGPU_ASYMM = 3, VACGPU_FIXED = 4
   integer(c_int), parameter :: VACGPU_FP64 = 0, VACGPU_FP32BULK = 1
+  integer(c_int), parameter :: VACGPU_LIM_ORIGINAL = 0, VACGPU_LIM_PREVIOUS = 1, &
+       VACGPU_LIM_PREDICTOR = 2
 
   ! the drop-in's device grid (made by the first advect or projection), and
   ! whether a process() call comes from inside advect
@@ -43,6 +45,7 @@ module vacgpu_mod
      integer(c_int) :: dimsplit_xy
      real(c_double) :: courantpar
      real(c_double) :: smallpcoeff
+     integer(c_int) :: typelimited
   end type vacgpu_method
 
   interface

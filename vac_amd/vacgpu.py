@@ -38,7 +38,8 @@ class Config(C.Structure):
 class Method(C.Structure):
     _fields_ = [("typefull", C.c_int), ("typelimiter", C.c_int * 8), ("divbfix", C.c_int),
                 ("divbwave", C.c_int), ("sourceunsplit", C.c_int), ("dimsplit_xy", C.c_int),
-                ("courantpar", C.c_double), ("smallpcoeff", C.c_double)]
+                ("courantpar", C.c_double), ("smallpcoeff", C.c_double),
+                ("typelimited", C.c_int)]
 
 
 _lib = None
@@ -127,6 +128,7 @@ class MethodSpec:
     dimsplit_xy: bool = False
     courantpar: float = 0.8
     smallpcoeff: float = 1e-6
+    typelimited: str = "original"       # or "previous", "predictor"
 
     def to_c(self) -> Method:
         m = Method()
@@ -140,6 +142,7 @@ class MethodSpec:
         m.dimsplit_xy = int(self.dimsplit_xy)
         m.courantpar = self.courantpar
         m.smallpcoeff = self.smallpcoeff
+        m.typelimited = {"original": 0, "previous": 1, "predictor": 2}[self.typelimited]
         return m
 
 
