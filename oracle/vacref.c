@@ -47,6 +47,9 @@ typedef struct {
   /* projectb / Poisson work planes: divb (the residual), phi, CG's Z and A.Z,
    * gradients */
   double *pr_rhs, *pr_phi, *pr_z, *pr_az, *pr_qdx, *pr_qddx;
+  /* entropy fixes: smalla, the left / right wave speeds, and the saved
+   * sound speeds of geteigenjump (src/vacphys.mhdroe.t:147-148) */
+  double *smalla, *eaL, *eaR, *cs2L, *cs2R, *cs2ca2L, *cs2ca2R;
   /* 'fixed' boundaries: the state the first getboundary call left */
   double* fixw;
   int fix_ready;
@@ -884,6 +887,106 @@ static void roe_rtimes_hd(ctx_t* x, const double* q, const double* wroe, const i
   }
 }
 
+/* ---------------------------------------------------------- entropy fix */
+/* The end of geteigenjump (src/vacphys.mhdroe.t:228-281,
+ * src/vacphys.hdroe.t:120-147) and entropyfix (src/vactvd.t:422-449) for
+ * wave il over [lo, hi]: 'yee' sets smalla = entropycoef(il); 'harten',
+ * 'powell' and (MHD) 'ratio' take the wave's speed at the left and right
+ * states -- the velocity, plus or minus the fast, slow or Alfven speed (the
+ * sound speed for HD) -- and set smalla, or modify a ('ratio').  The fast
+ * wave (il 0) sets the saved sound speeds the slow waves use; the right
+ * Alfven wave saves the Alfven speeds for the left one. */
+static void entropy_fix(ctx_t* x, const double* wL, const double* wR, const int* lo,
+                        const int* hi, int il, int idim, double* a, double* smalla) {
+  const int type = x->p->typeentropy[il];
+  if (type == VACREF_ENT_NUL) return;
+  if (type == VACREF_ENT_YEE) {
+    LOOP(lo, hi) smalla[IDX(x)] = x->p->entropycoef[il];
+    return;
+  }
+  const double g = x->p->gamma;
+  double* aL = x->eaL;
+  double* aR = x->eaR;
+  LOOP(lo, hi) {
+    long idx = IDX(x);
+    aL[idx] = PL(x, wL, M(idim))[idx] / PL(x, wL, RHO)[idx];
+    aR[idx] = PL(x, wR, M(idim))[idx] / PL(x, wR, RHO)[idx];
+  }
+  if (x->mhd) {
+    double *c2L = x->cs2L, *c2R = x->cs2R, *caL = x->cs2ca2L, *caR = x->cs2ca2R;
+    if (il == 0) {
+      getpthermal(x, 1, wL, lo, hi, c2L);
+      getpthermal(x, 1, wR, lo, hi, c2R);
+      LOOP(lo, hi) {
+        long idx = IDX(x);
+        const double rL = PL(x, wL, RHO)[idx], rR = PL(x, wR, RHO)[idx];
+        c2L[idx] = g * c2L[idx] / rL;  /* getcsound2, src/vacphys0.t.mhd:140-162 */
+        c2R[idx] = g * c2R[idx] / rR;
+        double bl = 0.0, br = 0.0;
+        for (int c = 0; c < x->ndir; ++c) {
+          double u = PL(x, wL, B(x, c))[idx], v = PL(x, wR, B(x, c))[idx];
+          bl = (c == 0) ? u * u : bl + u * u;
+          br = (c == 0) ? v * v : br + v * v;
+        }
+        caL[idx] = c2L[idx] + bl / rL;
+        caR[idx] = c2R[idx] + br / rR;
+        const double bdL = PL(x, wL, B(x, idim))[idx], bdR = PL(x, wR, B(x, idim))[idx];
+        c2L[idx] = sqrt(caL[idx] * caL[idx] - 4 * c2L[idx] * (bdL * bdL) / rL);
+        c2R[idx] = sqrt(caR[idx] * caR[idx] - 4 * c2R[idx] * (bdR * bdR) / rR);
+      }
+    }
+    if (il == 6) {
+      LOOP(lo, hi) {
+        long idx = IDX(x);
+        caL[idx] = fabs(PL(x, wL, B(x, idim))[idx]) / sqrt(PL(x, wL, RHO)[idx]);
+        caR[idx] = fabs(PL(x, wR, B(x, idim))[idx]) / sqrt(PL(x, wR, RHO)[idx]);
+      }
+    }
+    LOOP(lo, hi) {
+      long idx = IDX(x);
+      switch (il) {
+        case 0: aL[idx] = aL[idx] + sqrt(0.5 * (caL[idx] + c2L[idx]));
+                aR[idx] = aR[idx] + sqrt(0.5 * (caR[idx] + c2R[idx])); break;
+        case 1: aL[idx] = aL[idx] - sqrt(0.5 * (caL[idx] + c2L[idx]));
+                aR[idx] = aR[idx] - sqrt(0.5 * (caR[idx] + c2R[idx])); break;
+        case 2: aL[idx] = aL[idx] + sqrt(0.5 * (caL[idx] - c2L[idx]));
+                aR[idx] = aR[idx] + sqrt(0.5 * (caR[idx] - c2R[idx])); break;
+        case 3: aL[idx] = aL[idx] - sqrt(0.5 * (caL[idx] - c2L[idx]));
+                aR[idx] = aR[idx] - sqrt(0.5 * (caR[idx] - c2R[idx])); break;
+        case 6: aL[idx] = aL[idx] + caL[idx]; aR[idx] = aR[idx] + caR[idx]; break;
+        case 7: aL[idx] = aL[idx] - caL[idx]; aR[idx] = aR[idx] - caR[idx]; break;
+        default: break;
+      }
+    }
+  } else if (il <= 1) {
+    getpthermal(x, 1, wL, lo, hi, x->cs2L);
+    getpthermal(x, 1, wR, lo, hi, x->cs2R);
+    const double sg = il == 0 ? 1.0 : -1.0;
+    LOOP(lo, hi) {
+      long idx = IDX(x);
+      const double cl = sqrt(g * x->cs2L[idx] / PL(x, wL, RHO)[idx]);
+      const double cr = sqrt(g * x->cs2R[idx] / PL(x, wR, RHO)[idx]);
+      aL[idx] = sg > 0 ? aL[idx] + cl : aL[idx] - cl;
+      aR[idx] = sg > 0 ? aR[idx] + cr : aR[idx] - cr;
+    }
+  }
+  LOOP(lo, hi) {
+    long idx = IDX(x);
+    switch (type) {
+      case VACREF_ENT_HARTEN:
+        smalla[idx] = dmax(dmax(0.0, a[idx] - aL[idx]), aR[idx] - a[idx]);
+        break;
+      case VACREF_ENT_POWELL:
+        smalla[idx] = dmax(0.0, 2.0 * (aR[idx] - aL[idx]));
+        break;
+      case VACREF_ENT_RATIO:
+        if (aL[idx] < 0.0 && aR[idx] > 0.0)
+          a[idx] = a[idx] - 2 * aR[idx] * aL[idx] / (aR[idx] - aL[idx]);
+        break;
+    }
+  }
+}
+
 /* tvdlimit2 (src/vactvd.t:86-206) with getphi for typetvd='roe' (:209-302),
  * typeentropy='nul', acmcoef<0, acmexpo=0.  mu = 0: method 'tvd', called by
  * tvdlimit over ixIC = ixC widened by one face below (the limiter of phi
@@ -906,10 +1009,15 @@ static void tvdlimit2(ctx_t* x, double qdt, int d, const double* wL, const doubl
   for (int il = 0; il < x->nw; ++il) {
     if (x->mhd) roe_eigenjump(x, wL, wR, x->wroe, icl, ich, il, d, x->adtdx, x->jump);
     else roe_eigenjump_hd(x, wL, wR, x->wroe, icl, ich, il, d, x->adtdx, x->jump);
+    entropy_fix(x, wL, wR, icl, ich, il, d, x->adtdx, x->smalla);
+    const int et = x->p->typeentropy[il];
+    const int fixed = et != VACREF_ENT_NUL && et != VACREF_ENT_RATIO;
     LOOP(icl, ich) {
       long idx = IDX(x);
       int c = CRD(d);
       x->adtdx[idx] = x->adtdx[idx] * qdt * 2.0 / (x->dx[d][c] + x->dx[d][c + 1]);
+      if (et == VACREF_ENT_HARTEN || et == VACREF_ENT_POWELL)  /* :130-141 */
+        x->smalla[idx] = x->smalla[idx] * qdt * 2.0 / (x->dx[d][c] + x->dx[d][c + 1]);
     }
     if ((il == 0 || il == 1) && x->p->courantpar > 0.0 && x->istep == x->nstep) {
       double cm = 0.0;
@@ -923,17 +1031,20 @@ static void tvdlimit2(ctx_t* x, double qdt, int d, const double* wL, const doubl
       if (cm > 1.0) x->s->nerror[2]++;
       if (cm > SMALLD) x->s->dtcourant[d] = dmin(x->s->dtcourant[d], qdt * x->p->courantpar / cm);
     }
-    if (mu) {  /* getphi, method 'tvdmu' */
+    if (mu) {  /* getphi, method 'tvdmu', with the entropy fix (:227-240) */
       LOOP(cl, ch) {
         long idx = IDX(x);
-        x->phi[idx] = fabs(x->adtdx[idx]) * x->jump[idx];
+        double av = x->adtdx[idx], sa = x->smalla[idx];
+        if (!fixed || fabs(av) >= sa) x->phi[idx] = fabs(av) * x->jump[idx];
+        else x->phi[idx] = 0.5 * (sa + av * av / sa) * x->jump[idx];
       }
     } else {
-    /* getphi, method 'tvd' */
+    /* getphi, method 'tvd', with the entropy fix (:244-256) */
     LOOP(icl, ich) {
       long idx = IDX(x);
-      double av = x->adtdx[idx];
-      x->phi[idx] = fabs(av) - av * av;
+      double av = x->adtdx[idx], sa = x->smalla[idx];
+      if (!fixed || fabs(av) >= sa) x->phi[idx] = fabs(av) - av * av;
+      else x->phi[idx] = 0.5 * sa + (0.5 / sa - 1.0) * (av * av);
     }
     int ll[3];
     for (int dd = 0; dd < 3; ++dd) ll[dd] = icl[dd];
@@ -1368,7 +1479,8 @@ static int ctx_init(ctx_t* x, const vacref_params* p, vacref_state* s,
                     &x->jump, &x->adtdx, &x->phi, &x->rphi, &x->ljump,
                     &x->cfast, &x->cslow, &x->afast, &x->aslow, &x->csound2, &x->dp,
                     &x->rhodv, &x->bdv, &x->bdb, &x->bv, &x->v2a2,
-                    &x->pr_rhs, &x->pr_phi, &x->pr_z, &x->pr_az, &x->pr_qdx, &x->pr_qddx};
+                    &x->pr_rhs, &x->pr_phi, &x->pr_z, &x->pr_az, &x->pr_qdx, &x->pr_qddx,
+                    &x->smalla, &x->eaL, &x->eaR, &x->cs2L, &x->cs2R, &x->cs2ca2L, &x->cs2ca2R};
   for (size_t q = 0; q < sizeof(one) / sizeof(one[0]); ++q) *one[q] = calloc(1, pl);
   int anyfixed = 0;
   for (int f = 0; f < 2 * x->ndim; ++f)
@@ -1383,7 +1495,8 @@ static void ctx_free(ctx_t* x) {
                    x->sc1, x->sc2, x->sc3, x->sc4, x->sc5, x->jump, x->adtdx, x->phi,
                    x->rphi, x->ljump, x->cfast, x->cslow, x->afast, x->aslow,
                    x->csound2, x->dp, x->rhodv, x->bdv, x->bdb, x->bv, x->v2a2,
-                   x->pr_rhs, x->pr_phi, x->pr_z, x->pr_az, x->pr_qdx, x->pr_qddx, x->fixw};
+                   x->pr_rhs, x->pr_phi, x->pr_z, x->pr_az, x->pr_qdx, x->pr_qddx, x->fixw,
+                   x->smalla, x->eaL, x->eaR, x->cs2L, x->cs2R, x->cs2ca2L, x->cs2ca2R};
   for (size_t q = 0; q < sizeof(all) / sizeof(all[0]); ++q) free(all[q]);
 }
 
@@ -1429,6 +1542,10 @@ int vacref_run(const vacref_params* p, vacref_state* s, double* w,
   if (rc) return rc;
   if (p->dtpar > 0.0 && s->smallp < 0.0) { ctx_free(&x); return -3; }
   if (p->nproc1 != 0 && (!x.mhd || x.ndim < 2)) { ctx_free(&x); return -5; }  /* ^NOONED, MHD */
+  /* 'ratio' needs the MHD wave speeds: the HD solver leaves them unset */
+  if (!x.mhd)
+    for (int il = 0; il < x.nw; ++il)
+      if (p->typeentropy[il] == VACREF_ENT_RATIO) { ctx_free(&x); return -7; }
   /* TVD with typelimited='predictor' limits the centdiff result (src/vactvd.t:30-36) */
   if (p->typefull == VACREF_TVD && p->typelimited == 2) { ctx_free(&x); return -6; }
   getboundary(&x, w);

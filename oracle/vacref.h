@@ -47,7 +47,16 @@ typedef struct {
    * (wold, the step's start), 2 'predictor' (wCT); for TVD 'previous' limits
    * wCT as 'original' does (src/vactvd.t:37-38) */
   int typelimited;
+  /* typeentropy(il) per characteristic wave (src/vactvd.t:422-449, the wave
+   * speeds of src/vacphys.mhdroe.t:228-281 and src/vacphys.hdroe.t:120-147):
+   * VACREF_ENT_NUL, _HARTEN, _POWELL, _RATIO (MHD), _YEE; entropycoef(il) is
+   * 'yee''s lower limit of |a dt/dx| */
+  int typeentropy[8];
+  double entropycoef[8];
 } vacref_params;
+
+enum { VACREF_ENT_NUL = 0, VACREF_ENT_HARTEN = 1, VACREF_ENT_POWELL = 2, VACREF_ENT_RATIO = 3,
+       VACREF_ENT_YEE = 4 };
 
 typedef struct {
   int it, itmin;

@@ -31,7 +31,7 @@ class Params(C.Structure):
         ("courantpar", C.c_double), ("smallpcoeff", C.c_double),
         ("dtpar", C.c_double), ("tmax", C.c_double), ("tmaxexact", C.c_int),
         ("itmax", C.c_int), ("nproc1", C.c_int), ("procpar", C.c_double * 3),
-        ("typelimited", C.c_int),
+        ("typelimited", C.c_int), ("typeentropy", C.c_int * 8), ("entropycoef", C.c_double * 8),
     ]
 
 
@@ -85,7 +85,8 @@ def make_params(ndim, physics, nx, gamma, typefull="tvdlf", limiter="minmod",
                 bc="periodic", divbfix=True, divbwave=True, sourceunsplit=True,
                 dimsplit_xy=False, courantpar=0.8, smallpcoeff=1e-6, dtpar=-1.0,
                 tmax=1e99, tmaxexact=False, itmax=1000000, nproc1=0,
-                procpar=(-1.0, -1.0, -1.0), typelimited="original") -> Params:
+                procpar=(-1.0, -1.0, -1.0), typelimited="original", typeentropy="nul",
+                entropycoef=None) -> Params:
     p = Params()
     p.ndim = ndim
     p.phys = MHD if physics == "mhd" else HD
@@ -115,10 +116,32 @@ def make_params(ndim, physics, nx, gamma, typefull="tvdlf", limiter="minmod",
     p.tmaxexact = int(tmaxexact)
     p.itmax = itmax
     p.typelimited = {"original": 0, "previous": 1, "predictor": 2}[typelimited]
+    ents = [typeentropy] * 8 if isinstance(typeentropy, str) else list(typeentropy) + ["nul"] * 8
+    for i in range(8):
+        p.typeentropy[i] = ENTROPY[ents[i]]
+    coef = entropy_defaults(physics, ndim) if entropycoef is None else list(entropycoef) + [-1.0] * 8
+    for i in range(8):
+        p.entropycoef[i] = float(coef[i])
     p.nproc1 = int(nproc1)
     for i in range(3):
         p.procpar[i] = float(procpar[i])
     return p
+
+
+ENTROPY = {"nul": 0, "harten": 1, "powell": 2, "ratio": 3, "yee": 4}
+
+
+def entropy_defaults(physics, ndim):
+    """physini's entropycoef (src/vacphys.mhd0.t:18-29, src/vacphys.t.hd:20-28):
+    0.2 for the fast and slow (sound) waves, 0.4 for the Alfven waves, -1 else.
+    The literals there are default (single precision) reals, so the stored
+    doubles are 0.2 and 0.4 rounded to single precision."""
+    f2, f4 = float(np.float32(0.2)), float(np.float32(0.4))
+    if physics == "mhd":
+        c = [f2, f2, f2, f2, -1.0, -1.0, f4, f4]
+    else:
+        c = [f2, f2] + [-1.0] * 6
+    return c
 
 
 def bc_spec(v):

@@ -109,6 +109,21 @@ TL_CASES = {
 }
 
 
+# entropy fixes of the Roe solvers (typeentropy for every wave; entropycoef
+# keeps physini's defaults)
+# name: (binary, physics, builder, gamma, steps, typefull, limiter, typeentropy)
+ENT_CASES = {
+    "mhd22_rf32x28_tvd_woodward_harten": ("vac_mhd22_132", "mhd", lambda g: random_fourier((32, 28), 701, g), 5 / 3, 12, "tvd", "woodward", "harten"),
+    "mhd22_rf28x30_tvd_minmod_powell": ("vac_mhd22_132", "mhd", lambda g: random_fourier((28, 30), 702, g), 1.4, 12, "tvd", "minmod", "powell"),
+    "mhd22_rf30x26_tvd_superbee_yee": ("vac_mhd22_132", "mhd", lambda g: random_fourier((30, 26), 703, g), 5 / 3, 12, "tvd", "superbee", "yee"),
+    "mhd22_rf26x32_tvd_woodward_ratio": ("vac_mhd22_132", "mhd", lambda g: random_fourier((26, 32), 704, g), 5 / 3, 12, "tvd", "woodward", "ratio"),
+    "mhd22_rf30x30_tvdmu_woodward_harten": ("vac_mhd22_132", "mhd", lambda g: random_fourier((30, 30), 705, g), 5 / 3, 10, "tvdmu", "woodward", "harten"),
+    "hd22_rf32x26_tvd_woodward_harten": ("vac_hd22_132", "hd", lambda g: random_fourier((32, 26), 706, g, physics="hd"), 1.4, 15, "tvd", "woodward", "harten"),
+    "hd22_rf26x28_tvdmu_superbee_powell": ("vac_hd22_132", "hd", lambda g: random_fourier((26, 28), 707, g, physics="hd"), 1.4, 12, "tvdmu", "superbee", "powell"),
+    "mhd33_rf10x12x8_tvd_woodward_powell": ("vac_mhd33_36", "mhd", lambda g: random_fourier((10, 12, 8), 709, g), 5 / 3, 6, "tvd", "woodward", "powell"),
+}
+
+
 def ensure_binary(name: str) -> None:
     if have_ref(name):
         return
@@ -204,6 +219,24 @@ def main() -> None:
             nx=np.array([len(c) for c in coords]), gamma=gamma, steps=steps,
             physics=physics, typefull=typefull, limiter=limiter, bc=bc, divbfix=True,
             typelimited=tl)
+        print("wrote", name, w0.shape, "steps", len(t) - 1)
+    for name, (binary, physics, build, gamma, steps, typefull, limiter, ent) in ENT_CASES.items():
+        if only and not any(o in name for o in only):
+            continue
+        ensure_binary(binary)
+        w0, coords = build(gamma)
+        nw = w0.shape[0]
+        ini = make_ini(w0, coords, physics, gamma, name="golden")
+        par = problem_par(nw, NAMES[nw], steps, typefull=typefull, limiter=limiter,
+                          typeB=["periodic"] * (2 * len(coords)), every_step=True,
+                          extra_method=f" typeentropy={nw}*'{ent}'\n")
+        r = run_ref(binary, ini, par)
+        t = np.array([s_.t for s_ in r.snapshots])
+        np.savez_compressed(
+            os.path.join(HERE, name + ".npz"), w0=w0, wf=r.snapshots[-1].w, t=t,
+            nx=np.array([len(c) for c in coords]), gamma=gamma, steps=steps,
+            physics=physics, typefull=typefull, limiter=limiter, bc="periodic", divbfix=True,
+            typeentropy=ent)
         print("wrote", name, w0.shape, "steps", len(t) - 1)
     if only:
         return

@@ -41,6 +41,8 @@ def run_fixture(fx):
     kw = {}
     if "typelimited" in fx:
         kw["typelimited"] = str(fx["typelimited"])
+    if "typeentropy" in fx:
+        kw["typeentropy"] = str(fx["typeentropy"])
     if "nproc1" in fx:  # projection of B before every step
         kw.update(divbwave=bool(fx["divbwave"]), nproc1=int(fx["nproc1"]),
                   procpar=[float(v) for v in fx["procpar"]])
