@@ -37,6 +37,8 @@ enum { VACGPU_CONT = 0, VACGPU_PERIODIC = 1, VACGPU_SYMM = 2, VACGPU_ASYMM = 3,
        VACGPU_FIXED = 4 };                             /* typeB */
 enum { VACGPU_FP64 = 0, VACGPU_FP32BULK = 1 };          /* arithmetic tier */
 enum { VACGPU_LIM_ORIGINAL = 0, VACGPU_LIM_PREVIOUS = 1, VACGPU_LIM_PREDICTOR = 2 };  /* typelimited */
+enum { VACGPU_ENT_NUL = 0, VACGPU_ENT_HARTEN = 1, VACGPU_ENT_POWELL = 2, VACGPU_ENT_RATIO = 3,
+       VACGPU_ENT_YEE = 4 };                           /* typeentropy */
 
 typedef struct vacgpu_grid vacgpu_grid;
 
@@ -66,6 +68,12 @@ typedef struct {
                            sweep's input), VACGPU_LIM_PREVIOUS (the step's
                            start, wold), VACGPU_LIM_PREDICTOR (wCT); TVD takes
                            'original' or 'previous', which limit alike */
+  int typeentropy[8];   /* entropy fix per characteristic wave (TVD, TVD-MUSCL;
+                           src/vactvd.t:422-449): VACGPU_ENT_NUL, _HARTEN,
+                           _POWELL, _RATIO (MHD only), _YEE */
+  double entropycoef[8];/* 'yee''s lower limit per wave (VAC's physini
+                           defaults: 0.2 fast/slow/sound, 0.4 Alfven, -1 else,
+                           stored from single-precision literals) */
 } vacgpu_method;
 
 /* Replaces the geometry hand-off after gridsetup1 (src/vacgrid.t:457-593):

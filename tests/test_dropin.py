@@ -46,16 +46,22 @@ CASES = [
     # typelimited='previous': the corrector limits the step's start
     ("vac_mhd22_132", "vacgpu_mhd22_132", "mhd", lambda g: random_fourier((40, 40), 72, g), 5 / 3, 12,
      "tvdlf", "woodward", "periodic", True, "previous"),
+    # an entropy fix on every wave of the Roe dissipation
+    ("vac_mhd22_132", "vacgpu_mhd22_132", "mhd", lambda g: random_fourier((44, 36), 73, g), 5 / 3, 12,
+     "tvd", "woodward", "periodic", True, "", "harten"),
 ]
 
 
 @pytest.mark.parametrize("case", CASES, ids=[f"{c[6]}-{c[7]}-{c[2]}{len(c[3](1.4)[1])}d"
                                              + ("" if isinstance(c[8], str) else "-walls")
-                                             + (f"-{c[10]}" if len(c) > 10 else "")
+                                             + (f"-{c[10]}" if len(c) > 10 and c[10] else "")
+                                             + (f"-{c[11]}" if len(c) > 11 else "")
                                              for c in CASES])
 def test_vac_with_gpu_advect_matches_vac(case):
     cpu, gpu, physics, build, gamma, steps, typefull, limiter, bc, divbfix = case[:10]
-    extra = f" typelimited='{case[10]}'\n" if len(case) > 10 else ""
+    extra = f" typelimited='{case[10]}'\n" if len(case) > 10 and case[10] else ""
+    if len(case) > 11:
+        extra += f" typeentropy={len(NAMES[6 if physics == 'mhd' else 4].split())}*'{case[11]}'\n"
     if not (have_ref(cpu) and have_ref(gpu)):
         pytest.skip("reference executables not built (oracle/build_ref.sh)")
     w0, coords = build(gamma)

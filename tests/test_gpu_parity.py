@@ -62,8 +62,9 @@ def test_gpu_matches_reference_fixture(path):
     z = np.load(path, allow_pickle=False)
     nx = [int(v) for v in z["nx"]]
     tl = str(z["typelimited"]) if "typelimited" in z.files else "original"
+    te = str(z["typeentropy"]) if "typeentropy" in z.files else "nul"
     method = vacgpu.MethodSpec(typefull=str(z["typefull"]), limiter=str(z["limiter"]),
-                               divbfix=bool(z["divbfix"]), typelimited=tl)
+                               divbfix=bool(z["divbfix"]), typelimited=tl, typeentropy=te)
     print(f"case {os.path.basename(path)} nx={nx}", flush=True)
     w, t, sc = gpu_run(z["w0"], nx, str(z["physics"]), float(z["gamma"]), int(z["steps"]),
                        method, bc=vacref.bc_spec(z["bc"]))
@@ -212,6 +213,32 @@ def test_gpu_typelimited_matches_oracle(case):
     w, t, sc = gpu_run(w0, list(nx), physics, gamma, steps, method, bc=bc)
     p = vacref.make_params(len(nx), physics, list(nx), gamma, typefull=typefull,
                            limiter=limiter, bc=bc, typelimited=tl)
+    wr, tr, st = vacref.run(p, w0, [widths_1d(c) for c in coords], steps)
+    assert np.array_equal(t, tr)
+    assert_bitwise(w, wr, str(case))
+
+
+@pytest.mark.parametrize("case", [
+    ("mhd", (64, 56), "tvd", "woodward", "harten", "periodic", 10),
+    ("mhd", (48, 60), "tvd", "minmod", "powell", "cont", 10),
+    ("mhd", (56, 48), "tvd", "superbee", "yee", "periodic", 10),
+    ("mhd", (60, 52), "tvd", "woodward", "ratio", "cont", 10),
+    ("mhd", (52, 60), "tvdmu", "woodward", "harten", "periodic", 10),
+    ("mhd", (14, 12, 16), "tvd", "minmod", "powell", "periodic", 5),
+    ("mhd", (12, 16, 14), "tvdmu", "woodward", "yee", "cont", 5),
+    ("hd", (64, 48), "tvd", "woodward", "harten", "periodic", 15),
+    ("hd", (48, 56), "tvdmu", "minmod", "yee", "cont", 12),
+])
+def test_gpu_entropy_fix_matches_oracle(case):
+    # typeentropy on every wave, VAC's default entropycoef
+    physics, nx, typefull, limiter, ent, bc, steps = case
+    gamma = 1.4
+    w0, coords = random_fourier(nx, 31 + len(nx), gamma, physics=physics)
+    method = vacgpu.MethodSpec(typefull=typefull, limiter=limiter, typeentropy=ent)
+    print(f"case {case}", flush=True)
+    w, t, sc = gpu_run(w0, list(nx), physics, gamma, steps, method, bc=bc)
+    p = vacref.make_params(len(nx), physics, list(nx), gamma, typefull=typefull,
+                           limiter=limiter, bc=bc, typeentropy=ent)
     wr, tr, st = vacref.run(p, w0, [widths_1d(c) for c in coords], steps)
     assert np.array_equal(t, tr)
     assert_bitwise(w, wr, str(case))

@@ -611,13 +611,90 @@ struct RoeOf<ND, R, false> {
 // (src/vactvd.t:246-247, :294-302): jm, j0, jp are the jumps of one wave at
 // faces J-1, J, J+1 and av = a*dt*2/(dx_J+dx_J+1) at face J.
 template <int LIM, class R>
-__device__ __forceinline__ R tvd_phi_t(int lim, R jm, R j0, R jp, R av) {
-  R ph = R(fabs(av)) - av * av;
+__device__ __forceinline__ R tvd_phi_ph(int lim, R jm, R j0, R jp, R av, R ph) {
   const R ljl = limiter_t<LIM, R>(lim, j0, jm);  // ljump(J)
   const R ljr = limiter_t<LIM, R>(lim, jp, j0);  // ljump(J+1)
   if (av <= R(0)) ph = ph * (j0 - ljr);
   else ph = ph * (j0 - ljl);
   return ph + av * av * j0;
+}
+template <int LIM, class R>
+__device__ __forceinline__ R tvd_phi_t(int lim, R jm, R j0, R jp, R av) {
+  return tvd_phi_ph<LIM, R>(lim, jm, j0, jp, av, R(fabs(av)) - av * av);
+}
+
+// ---- entropy fixes (typeentropy, src/vactvd.t:422-449)
+enum { ENT_NUL = 0, ENT_HARTEN = 1, ENT_POWELL = 2, ENT_RATIO = 3, ENT_YEE = 4 };
+
+// getphi's first factors with the fix (src/vactvd.t:227-256): where |a| is
+// below smalla, |a| becomes (smalla + a^2/smalla)/2 ('tvdmu') and |a| - a^2
+// becomes smalla/2 + (1/(2 smalla) - 1) a^2 ('tvd'); 'nul' and 'ratio' keep them
+template <class R>
+__device__ __forceinline__ R ent_abs(R av, R sa, int type) {
+  if (type != ENT_NUL && type != ENT_RATIO && !(R(fabs(av)) >= sa))
+    return R(0.5) * (sa + pdiv(av * av, sa));
+  return R(fabs(av));
+}
+template <class R>
+__device__ __forceinline__ R ent_tvd(R av, R sa, int type) {
+  if (type != ENT_NUL && type != ENT_RATIO && !(R(fabs(av)) >= sa))
+    return R(0.5) * sa + (pdiv(R(0.5), sa) - R(1)) * (av * av);
+  return R(fabs(av)) - av * av;
+}
+
+// The left and right speeds of wave il for the entropy fix: the velocity
+// along d plus or minus the fast, slow or Alfven speed of each state (MHD,
+// src/vacphys.mhdroe.t:236-279) or the sound speed (HD,
+// src/vacphys.hdroe.t:124-145).  The fast right wave (il 0) sets the sound
+// speed quantities the other fast and slow waves reuse (c2 = the
+// discriminant, ca = cs^2 + ca^2); the right Alfven wave (il 6) stores the
+// Alfven speeds in ca for the left one, as VAC's saved arrays do.
+template <int ND, class R, bool MHD>
+__device__ __forceinline__ void ent_speeds(const double* uL, const double* uR, int il, int d,
+                                           double g, double smallp, R& aL, R& aR, R& c2L,
+                                           R& c2R, R& caL, R& caR) {
+  using L = Layout<ND, MHD>;
+  const R rL = R(uL[L::rho]), rR = R(uR[L::rho]);
+  aL = pdiv(R(uL[L::m(d)]), rL);
+  aR = pdiv(R(uR[L::m(d)]), rR);
+  if constexpr (MHD) {
+    if (il == 0) {
+      c2L = pdiv(R(g) * R(pthermal<ND, true>(uL, g, smallp)), rL);
+      c2R = pdiv(R(g) * R(pthermal<ND, true>(uR, g, smallp)), rR);
+      R bl = R(uL[L::b(0)]) * R(uL[L::b(0)]), br = R(uR[L::b(0)]) * R(uR[L::b(0)]);
+#pragma unroll
+      for (int c = 1; c < ND; ++c) {
+        bl = bl + R(uL[L::b(c)]) * R(uL[L::b(c)]);
+        br = br + R(uR[L::b(c)]) * R(uR[L::b(c)]);
+      }
+      caL = c2L + pdiv(bl, rL);
+      caR = c2R + pdiv(br, rR);
+      const R bdL = R(uL[L::b(d)]), bdR = R(uR[L::b(d)]);
+      c2L = R(sqrt(caL * caL - pdiv(R(4) * c2L * (bdL * bdL), rL)));
+      c2R = R(sqrt(caR * caR - pdiv(R(4) * c2R * (bdR * bdR), rR)));
+    }
+    if (il == 6) {
+      caL = pdiv(R(fabs(R(uL[L::b(d)]))), R(sqrt(rL)));
+      caR = pdiv(R(fabs(R(uR[L::b(d)]))), R(sqrt(rR)));
+    }
+    switch (il) {
+      case 0: aL = aL + R(sqrt(R(0.5) * (caL + c2L))); aR = aR + R(sqrt(R(0.5) * (caR + c2R))); break;
+      case 1: aL = aL - R(sqrt(R(0.5) * (caL + c2L))); aR = aR - R(sqrt(R(0.5) * (caR + c2R))); break;
+      case 2: aL = aL + R(sqrt(R(0.5) * (caL - c2L))); aR = aR + R(sqrt(R(0.5) * (caR - c2R))); break;
+      case 3: aL = aL - R(sqrt(R(0.5) * (caL - c2L))); aR = aR - R(sqrt(R(0.5) * (caR - c2R))); break;
+      case 6: aL = aL + caL; aR = aR + caR; break;
+      case 7: aL = aL - caL; aR = aR - caR; break;
+      default: break;
+    }
+  } else {
+    (void)c2L; (void)c2R; (void)caL; (void)caR;
+    if (il <= 1) {
+      const R cl = R(sqrt(pdiv(R(g) * R(pthermal<ND, false>(uL, g, smallp)), rL)));
+      const R cr = R(sqrt(pdiv(R(g) * R(pthermal<ND, false>(uR, g, smallp)), rR)));
+      if (il == 0) { aL = aL + cl; aR = aR + cr; }
+      else { aL = aL - cl; aR = aR - cr; }
+    }
+  }
 }
 
 template <class R>

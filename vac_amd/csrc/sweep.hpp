@@ -34,6 +34,8 @@ struct Sel {
   int lim[8];
   int divbfix, divbwave, sourceunsplit;
   double courantpar, gamma;
+  int ent[8];        // typeentropy per wave (ENT_*)
+  double ecoef[8];   // entropycoef per wave ('yee')
 };
 
 // boundary types per (face, variable): t = 1 for 'periodic'; k the kind of a
@@ -1150,26 +1152,51 @@ __device__ __forceinline__ void ryu_first_half(const Geo& g, const BC& bc,
 // |a dt 2/(dx_J+dx_J+1)| of the two fast waves over the faces they compute
 // (covering ixIC, src/vactvd.t:42,145-157) to part[SLOT_C0/C1].
 
+// getphi for 'tvd' at a face, with the entropy fix when ENT (sa: the face's smalla)
+template <int LIM, class R, bool ENT>
+__device__ __forceinline__ R tvd_phi_f(const Sel& s, int il, R jm, R j0, R jp, R av, const R* sa) {
+  if constexpr (ENT) return tvd_phi_ph<LIM, R>(s.lim[il], jm, j0, jp, av, ent_tvd(av, sa[il], s.ent[il]));
+  else return tvd_phi_t<LIM, R>(s.lim[il], jm, j0, jp, av);
+}
+
 // Everything one face contributes to the update of its two cells.
-template <int ND, class R, bool MHD = true>
+template <int ND, class R, bool MHD = true, bool ENT = false>
 struct TvdFace {
   R jump[Layout<ND, MHD>::nw];
   R adt[Layout<ND, MHD>::nw];
+  R sa[ENT ? Layout<ND, MHD>::nw : 1];  // smalla of the entropy fix (scaled like adt)
   typename RoeOf<ND, R, MHD>::Rt rt;
 };
 
-template <int ND, class R, bool MHD = true>
+template <int ND, class R, bool MHD = true, bool ENT = false>
 __device__ __forceinline__ void tvd_face(const double* uL, const double* uR, int d, const Sel& s,
-                                         double smallp, double dt, double dxsum, TvdFace<ND, R, MHD>& f,
+                                         double smallp, double dt, double dxsum, TvdFace<ND, R, MHD, ENT>& f,
                                          double& c0, double& c1) {
   constexpr int NW = Layout<ND, MHD>::nw;
   typename RoeOf<ND, R, MHD>::St r;
   roe_setup(uL, uR, d, s.gamma, smallp, r);
   const R sdx = R(dxsum);
+  [[maybe_unused]] R c2L = R(0), c2R = R(0), caL = R(0), caR = R(0);
 #pragma unroll
   for (int il = 0; il < NW; ++il) {
     R a, jp;
     roe_wave(r, uL, uR, il, d, s.divbwave, a, jp);
+    if constexpr (ENT) {
+      // entropyfix (src/vactvd.t:422-449), smalla scaled like a (:130-141)
+      const int et = s.ent[il];
+      R sa = R(0);
+      if (et == ENT_YEE) {
+        sa = R(s.ecoef[il]);
+      } else if (et != ENT_NUL) {
+        R aL, aR;
+        ent_speeds<ND, R, MHD>(uL, uR, il, d, s.gamma, smallp, aL, aR, c2L, c2R, caL, caR);
+        if (et == ENT_HARTEN) sa = rmax(rmax(R(0), a - aL), aR - a);
+        else if (et == ENT_POWELL) sa = rmax(R(0), R(2) * (aR - aL));
+        else if (aL < R(0) && aR > R(0)) a = a - pdiv(R(2) * aR * aL, aR - aL);
+      }
+      if (et == ENT_HARTEN || et == ENT_POWELL) sa = pdiv(sa * R(dt) * R(2), sdx);
+      f.sa[il] = sa;
+    }
     const R ad = pdiv(a * R(dt) * R(2), sdx);
     f.jump[il] = jp;
     f.adt[il] = ad;
@@ -1241,11 +1268,11 @@ __device__ __forceinline__ S shfl_up_struct(const S& v) {
 // its two faces and the Powell source of w1 as in the TVDLF corrector.  The
 // two extreme waves' |a dt 2/(dx_J + dx_J+1)| over the faces ixC feed the
 // Courant fold (src/vactvd.t:145-157).
-template <int ND, class R, int D, bool MHD = true>
+template <int ND, class R, int D, bool MHD = true, bool ENT = false>
 __device__ __forceinline__ void mu_face(const Sel& s, const double* u0, const double* u1,
                                         const double* ll, const double* lr, double dt,
                                         double smallp, double dxsum, double* F,
-                                        TvdFace<ND, R, MHD>& f, R* phi, double& c0, double& c1) {
+                                        TvdFace<ND, R, MHD, ENT>& f, R* phi, double& c0, double& c1) {
   constexpr int NW = Layout<ND, MHD>::nw;
   double uL[NW], uR[NW];
 #pragma unroll
@@ -1260,14 +1287,17 @@ __device__ __forceinline__ void mu_face(const Sel& s, const double* u0, const do
 #pragma unroll
     for (int iw = 0; iw < NW; ++iw) F[iw] = 0.5 * (fL[iw] + fR[iw]);
   }
-  tvd_face<ND, R, MHD>(uL, uR, D, s, smallp, dt, dxsum, f, c0, c1);
+  tvd_face<ND, R, MHD, ENT>(uL, uR, D, s, smallp, dt, dxsum, f, c0, c1);
 #pragma unroll
-  for (int il = 0; il < NW; ++il) phi[il] = R(fabs(f.adt[il])) * f.jump[il];
+  for (int il = 0; il < NW; ++il) {
+    if constexpr (ENT) phi[il] = ent_abs(f.adt[il], f.sa[il], s.ent[il]) * f.jump[il];
+    else phi[il] = R(fabs(f.adt[il])) * f.jump[il];
+  }
 }
 
 // x sweep: lane L owns face c+1/2 of cell c = x0 - 1 + L and updates c for
 // L in [1, 62]; the left face's flux and dissipation term come from lane L-1.
-template <int ND, class R, int LIM, bool MHD = true>
+template <int ND, class R, int LIM, bool MHD = true, bool ENT = false>
 __global__ __launch_bounds__(256) void k_tvdmu_x(Geo g, Sel s, BC bc, const double* __restrict__ w,
                                                 const double* __restrict__ w1,
                                                 double* __restrict__ wout,
@@ -1302,9 +1332,9 @@ __global__ __launch_bounds__(256) void k_tvdmu_x(Geo g, Sel s, BC bc, const doub
 #pragma unroll
   for (int iw = 0; iw < NW; ++iw) lr[iw] = lane_next(l[iw]);
   double F[NW], c0, c1;
-  TvdFace<ND, R, MHD> f;
+  TvdFace<ND, R, MHD, ENT> f;
   R phi[NW];
-  mu_face<ND, R, 0, MHD>(s, u0, u1, l, lr, dt, smallp, g.dx[0][cc] + g.dx[0][cc + 1], F, f, phi, c0, c1);
+  mu_face<ND, R, 0, MHD, ENT>(s, u0, u1, l, lr, dt, smallp, g.dx[0][cc] + g.dx[0][cc + 1], F, f, phi, c0, c1);
   if (!face_ok) { c0 = -INFINITY; c1 = -INFINITY; }
   double out[NW];
   const double qdx = pdiv(dt, g.dx[0][cc]);
@@ -1338,7 +1368,7 @@ __global__ __launch_bounds__(256) void k_tvdmu_x(Geo g, Sel s, BC bc, const doub
 
 // y / z sweep: thread (x, segment) marches through cells s0..s1 along D,
 // carrying the face below (flux, Roe decomposition, phi) and a window of w and w1.
-template <int ND, class R, int D, int LIM, bool MHD = true>
+template <int ND, class R, int D, int LIM, bool MHD = true, bool ENT = false>
 __global__ __launch_bounds__(256) void k_tvdmu_m(Geo g, Sel s, BC bc, const double* __restrict__ w,
                                                 const double* __restrict__ w1,
                                                 double* __restrict__ wout,
@@ -1378,7 +1408,7 @@ __global__ __launch_bounds__(256) void k_tvdmu_m(Geo g, Sel s, BC bc, const doub
     // when wl is w.
     const bool own = wl == w;
     double wb[NW], wc[NW], u0[NW], lc[NW], Fp[NW];
-    TvdFace<ND, R, MHD> fp;
+    TvdFace<ND, R, MHD, ENT> fp;
     R phip[NW];
     {
       double wa[NW], u1[NW], ln[NW], wd[NW];
@@ -1393,7 +1423,7 @@ __global__ __launch_bounds__(256) void k_tvdmu_m(Geo g, Sel s, BC bc, const doub
       load<NW>(wl, g.ps, base + (long long)(s0 + 1) * sd, wd);
 #pragma unroll
       for (int iw = 0; iw < NW; ++iw) ln[iw] = slope<R, LIM>(s.lim[iw], wb[iw], wc[iw], wd[iw]);
-      mu_face<ND, R, D, MHD>(s, u0, u1, lc, ln, dt, smallp, dxd[s0 - 1] + dxd[s0], Fp, fp, phip, a, b);
+      mu_face<ND, R, D, MHD, ENT>(s, u0, u1, lc, ln, dt, smallp, dxd[s0 - 1] + dxd[s0], Fp, fp, phip, a, b);
       c0 = c0 < a ? a : c0; c1 = c1 < b ? b : c1;
 #pragma unroll
       for (int iw = 0; iw < NW; ++iw) {
@@ -1406,13 +1436,13 @@ __global__ __launch_bounds__(256) void k_tvdmu_m(Geo g, Sel s, BC bc, const doub
     // now: u0 = w1(i), lc = slope(i), wb = w(i), wc = w(i+1)
     for (int i = s0; i <= s1; ++i) {
       double u1[NW], ln[NW], wd[NW], F[NW];
-      TvdFace<ND, R, MHD> f;
+      TvdFace<ND, R, MHD, ENT> f;
       R phi[NW];
       load<NW>(w1, g.ps, base + (long long)(i + 1) * sd, u1);
       load<NW>(wl, g.ps, base + (long long)(i + 2) * sd, wd);
 #pragma unroll
       for (int iw = 0; iw < NW; ++iw) ln[iw] = slope<R, LIM>(s.lim[iw], wb[iw], wc[iw], wd[iw]);
-      mu_face<ND, R, D, MHD>(s, u0, u1, lc, ln, dt, smallp, dxd[i] + dxd[i + 1], F, f, phi, a, b);
+      mu_face<ND, R, D, MHD, ENT>(s, u0, u1, lc, ln, dt, smallp, dxd[i] + dxd[i + 1], F, f, phi, a, b);
       c0 = c0 < a ? a : c0; c1 = c1 < b ? b : c1;
       const long long idx = base + (long long)i * sd;
       const double qdx = pdiv(dt, dxd[i]);
@@ -1451,7 +1481,7 @@ constexpr int TX_STRIDE = 61;  // cells updated per 64-lane wavefront along x
 
 // x sweep: lane L of a wavefront owns face J = x0 - 2 + L (between cells J and
 // J+1) and updates cell J for L in [2, 62].
-template <int ND, class R, int LIM = -1, int SRC = 0, bool MHD = true>
+template <int ND, class R, int LIM = -1, int SRC = 0, bool MHD = true, bool ENT = false>
 __global__ __launch_bounds__(256) void k_tvd_x(Geo g, Sel s, BC bc, const double* __restrict__ wct,
                                               const double* __restrict__ win,
                                               double* __restrict__ wout,
@@ -1481,9 +1511,9 @@ __global__ __launch_bounds__(256) void k_tvd_x(Geo g, Sel s, BC bc, const double
     load<NW>(wct, g.ps, base + Jc, uL);
     load<NW>(wct, g.ps, base + Jc + 1, uR);
   }
-  TvdFace<ND, R, MHD> f;
+  TvdFace<ND, R, MHD, ENT> f;
   double c0, c1;
-  tvd_face<ND, R, MHD>(uL, uR, 0, s, smallp, dt, g.dx[0][Jc] + g.dx[0][Jc + 1], f, c0, c1);
+  tvd_face<ND, R, MHD, ENT>(uL, uR, 0, s, smallp, dt, g.dx[0][Jc] + g.dx[0][Jc + 1], f, c0, c1);
   const bool faceok = rowok && J <= g.n[0] + 2;
   if (!faceok) { c0 = -INFINITY; c1 = -INFINITY; }
   // centdiff: flux at cell J, face average fC(J) and fC(J-1)
@@ -1498,7 +1528,11 @@ __global__ __launch_bounds__(256) void k_tvd_x(Geo g, Sel s, BC bc, const double
 #pragma unroll
   for (int il = 0; il < NW; ++il) {
     const R jm = shfl_up1(f.jump[il]), jp = shfl_down1(f.jump[il]);
-    phi[il] = tvd_phi_t<LIM, R>(s.lim[il], jm, f.jump[il], jp, f.adt[il]);
+    if constexpr (ENT)
+      phi[il] = tvd_phi_ph<LIM, R>(s.lim[il], jm, f.jump[il], jp, f.adt[il],
+                                   ent_tvd(f.adt[il], f.sa[il], s.ent[il]));
+    else
+      phi[il] = tvd_phi_t<LIM, R>(s.lim[il], jm, f.jump[il], jp, f.adt[il]);
   }
   const int i = J;
   // the dissipation needs every lane's face term (lane_prev inside), so it is
@@ -1536,7 +1570,7 @@ __global__ __launch_bounds__(256) void k_tvd_x(Geo g, Sel s, BC bc, const double
 }
 
 // y / z sweep: thread (x, segment) marches through cells s0..s1 along d.
-template <int ND, class R, int D, int LIM = -1, int SRC = 0, bool MHD = true>
+template <int ND, class R, int D, int LIM = -1, int SRC = 0, bool MHD = true, bool ENT = false>
 __global__ __launch_bounds__(256) void k_tvd_m(Geo g, Sel s, BC bc, const double* __restrict__ wct,
                                               const double* __restrict__ win,
                                               double* __restrict__ wout,
@@ -1588,20 +1622,20 @@ __global__ __launch_bounds__(256) void k_tvd_m(Geo g, Sel s, BC bc, const double
       }
     };
     double uL[NW], uR[NW];
-    TvdFace<ND, R, MHD> fm, f0, fp;  // faces i-1, i, i+1
+    TvdFace<ND, R, MHD, ENT> fm, f0, fp;  // faces i-1, i, i+1
     R phim[NW], phi0[NW];
     double Fm[NW], F0[NW], fCm[NW];
     double a, b;
     // prologue: faces s0-2, s0-1, s0; centred fluxes at cells s0-1, s0
     get_ct(s0 - 2, uL);
     get_ct(s0 - 1, uR);
-    tvd_face<ND, R, MHD>(uL, uR, d, s, smallp, dt, dxd[s0 - 2] + dxd[s0 - 1], fm, a, b);
+    tvd_face<ND, R, MHD, ENT>(uL, uR, d, s, smallp, dt, dxd[s0 - 2] + dxd[s0 - 1], fm, a, b);
     c0 = c0 < a ? a : c0; c1 = c1 < b ? b : c1;
     fluxes<ND, MHD, R>(uR, d, s.gamma, smallp, Fm);
 #pragma unroll
     for (int iw = 0; iw < NW; ++iw) uL[iw] = uR[iw];
     get_ct(s0, uR);
-    tvd_face<ND, R, MHD>(uL, uR, d, s, smallp, dt, dxd[s0 - 1] + dxd[s0], f0, a, b);
+    tvd_face<ND, R, MHD, ENT>(uL, uR, d, s, smallp, dt, dxd[s0 - 1] + dxd[s0], f0, a, b);
     c0 = c0 < a ? a : c0; c1 = c1 < b ? b : c1;
     fluxes<ND, MHD, R>(uR, d, s.gamma, smallp, F0);
 #pragma unroll
@@ -1609,11 +1643,11 @@ __global__ __launch_bounds__(256) void k_tvd_m(Geo g, Sel s, BC bc, const double
 #pragma unroll
     for (int iw = 0; iw < NW; ++iw) uL[iw] = uR[iw];
     get_ct(s0 + 1, uR);
-    tvd_face<ND, R, MHD>(uL, uR, d, s, smallp, dt, dxd[s0] + dxd[s0 + 1], fp, a, b);
+    tvd_face<ND, R, MHD, ENT>(uL, uR, d, s, smallp, dt, dxd[s0] + dxd[s0 + 1], fp, a, b);
     c0 = c0 < a ? a : c0; c1 = c1 < b ? b : c1;
     // phi at face s0-1 from faces s0-2, s0-1, s0
 #pragma unroll
-    for (int il = 0; il < NW; ++il) phim[il] = tvd_phi_t<LIM, R>(s.lim[il], fm.jump[il], f0.jump[il], fp.jump[il], f0.adt[il]);
+    for (int il = 0; il < NW; ++il) phim[il] = tvd_phi_f<LIM, R, ENT>(s, il, fm.jump[il], f0.jump[il], fp.jump[il], f0.adt[il], f0.sa);
     // window now: fm <- face s0-1, f0 <- face s0 (fp holds face s0)
     fm = f0;
     f0 = fp;
@@ -1629,10 +1663,10 @@ __global__ __launch_bounds__(256) void k_tvd_m(Geo g, Sel s, BC bc, const double
 #pragma unroll
       for (int iw = 0; iw < NW; ++iw) uL[iw] = uR[iw];
       get_ct(i + 2, uR);
-      tvd_face<ND, R, MHD>(uL, uR, d, s, smallp, dt, dxd[i + 1] + dxd[i + 2], fp, a, b);
+      tvd_face<ND, R, MHD, ENT>(uL, uR, d, s, smallp, dt, dxd[i + 1] + dxd[i + 2], fp, a, b);
       c0 = c0 < a ? a : c0; c1 = c1 < b ? b : c1;
 #pragma unroll
-      for (int il = 0; il < NW; ++il) phi0[il] = tvd_phi_t<LIM, R>(s.lim[il], fm.jump[il], f0.jump[il], fp.jump[il], f0.adt[il]);
+      for (int il = 0; il < NW; ++il) phi0[il] = tvd_phi_f<LIM, R, ENT>(s, il, fm.jump[il], f0.jump[il], fp.jump[il], f0.adt[il], f0.sa);
       // update cell i
       const long long idx = base + (long long)i * sd;
       double out[NW];

@@ -194,6 +194,24 @@ int check_method(const vacgpu_config* c, const vacgpu_method* m, std::string& wh
   if (m->typefull != VACGPU_TVDLF && m->typefull != VACGPU_TVD && m->typefull != VACGPU_TVDMU) { why = "typefull must be 'tvdlf', 'tvd' or 'tvdmu'"; return VACGPU_E_ARG; }
   if (m->typelimited < VACGPU_LIM_ORIGINAL || m->typelimited > VACGPU_LIM_PREDICTOR) { why = "typelimited must be 'original', 'previous' or 'predictor'"; return VACGPU_E_ARG; }
   if (m->typefull == VACGPU_TVD && m->typelimited == VACGPU_LIM_PREDICTOR) { why = "typelimited='predictor' with typefull='tvd' (limiting the centred-difference result) is not on the GPU path"; return VACGPU_E_ARG; }
+  {
+    const int nw = c->physics == VACGPU_MHD ? 2 + 2 * c->ndim : 2 + c->ndim;
+    auto speeds = [&](int il) {
+      const int t = m->typeentropy[il];
+      return t == VACGPU_ENT_HARTEN || t == VACGPU_ENT_POWELL || t == VACGPU_ENT_RATIO;
+    };
+    for (int il = 0; il < nw; ++il) {
+      const int t = m->typeentropy[il];
+      if (t < VACGPU_ENT_NUL || t > VACGPU_ENT_YEE) { why = "typeentropy must be 'nul', 'harten', 'powell', 'ratio' or 'yee'"; return VACGPU_E_ARG; }
+      if (t == VACGPU_ENT_RATIO && c->physics == VACGPU_HD) { why = "typeentropy='ratio' needs the MHD Roe solver"; return VACGPU_E_ARG; }
+    }
+    // VAC's MHD solver computes the sound speeds for the fast right wave and
+    // the Alfven speeds for the right Alfven wave, and the others reuse them
+    if (c->physics == VACGPU_MHD) {
+      if ((speeds(1) || speeds(2) || speeds(3)) && !speeds(0)) { why = "typeentropy: the fast and slow waves take their speeds from the fast right wave's fix (give it 'harten', 'powell' or 'ratio' too)"; return VACGPU_E_ARG; }
+      if (c->ndim == 3 && speeds(7) && !speeds(6)) { why = "typeentropy: the left Alfven wave takes its speeds from the right one's fix"; return VACGPU_E_ARG; }
+    }
+  }
   int nw = c->physics == VACGPU_MHD ? 2 + 2 * c->ndim : 2 + c->ndim;
   for (int iw = 0; iw < nw; ++iw)
     if (m->typelimiter[iw] < VACGPU_MINMOD || m->typelimiter[iw] > VACGPU_SUPERBEE) { why = "typelimiter must be minmod, woodward or superbee"; return VACGPU_E_ARG; }
@@ -210,7 +228,19 @@ Sel make_sel(const vacgpu_grid* g, const vacgpu_method* m) {
   s.sourceunsplit = m->sourceunsplit;
   s.courantpar = m->courantpar;
   s.gamma = g->cfg.eqpar[0];
+  for (int i = 0; i < 8; ++i) {
+    s.ent[i] = m->typefull == VACGPU_TVDLF ? ENT_NUL : m->typeentropy[i];
+    s.ecoef[i] = m->entropycoef[i];
+  }
   return s;
+}
+
+// an entropy fix on any wave of the TVD / TVD-MUSCL Roe dissipation
+bool entropy_on(const vacgpu_grid* g, const vacgpu_method* m) {
+  if (m->typefull == VACGPU_TVDLF) return false;
+  for (int i = 0; i < g->nw; ++i)
+    if (m->typeentropy[i] != VACGPU_ENT_NUL) return true;
+  return false;
 }
 
 // template dispatch over (ndim, physics, precision)
@@ -454,13 +484,16 @@ int sweep_device(vacgpu_grid* g, const vacgpu_method* m, int d, int firstsweep, 
       // face-parallel corrector: x sweep one face per lane, y / z sweeps marching
       Timed t(g, KF_CORR);
       const bool f32 = g->cfg.precision == VACGPU_FP32BULK;
-      const int lim = common_limiter(g, m);
+      const bool ent = entropy_on(g, m);
+      // with entropy fixes the limiter types are read at run time
+      const int lim = ent ? -1 : common_limiter(g, m);
 #define LIMSW(X) do { if (lim == LIM_MINMOD) X(LIM_MINMOD); else if (lim == LIM_WOODWARD) X(LIM_WOODWARD); else if (lim == LIM_SUPERBEE) X(LIM_SUPERBEE); else X(-1); } while (0)
       if (d == 0) {
         const dim3 gr((g->geo.n[0] + CX_STRIDE - 1) / CX_STRIDE, g->grid.y, g->grid.z);
         nbc = (long long)gr.x * gr.y * gr.z;
-#define MUX(NDV, RT, LV) do { if (mhd) hipLaunchKernelGGL((k_tvdmu_x<NDV, RT, LV, true>), gr, g->block, 0, g->stream, g->geo, s, g->bc, win, g->w1, wout, g->sc, g->part, g->pstride, ghosts, wl); \
+#define MUX(NDV, RT, LV) do { if (ent) { if (mhd) MUXE(NDV, RT, true); else MUXE(NDV, RT, false); } else if (mhd) hipLaunchKernelGGL((k_tvdmu_x<NDV, RT, LV, true>), gr, g->block, 0, g->stream, g->geo, s, g->bc, win, g->w1, wout, g->sc, g->part, g->pstride, ghosts, wl); \
                                else hipLaunchKernelGGL((k_tvdmu_x<NDV, RT, LV, false>), gr, g->block, 0, g->stream, g->geo, s, g->bc, win, g->w1, wout, g->sc, g->part, g->pstride, ghosts, wl); } while (0)
+#define MUXE(NDV, RT, MH) hipLaunchKernelGGL((k_tvdmu_x<NDV, RT, -1, MH, true>), gr, g->block, 0, g->stream, g->geo, s, g->bc, win, g->w1, wout, g->sc, g->part, g->pstride, ghosts, wl)
 #define X2D(LV) MUX(2, double, LV)
 #define X2F(LV) MUX(2, float, LV)
 #define X3D(LV) MUX(3, double, LV)
@@ -472,10 +505,12 @@ int sweep_device(vacgpu_grid* g, const vacgpu_method* m, int d, int firstsweep, 
 #undef X3D
 #undef X3F
 #undef MUX
+#undef MUXE
       } else {
         const dim3 gr = g->grid_tm[d];
         nbc = (long long)gr.x * gr.y * gr.z;
-#define MUM(NDV, RT, DV, LV) do { if (mhd) hipLaunchKernelGGL((k_tvdmu_m<NDV, RT, DV, LV, true>), gr, g->block, 0, g->stream, g->geo, s, g->bc, win, g->w1, wout, g->sc, g->part, g->pstride, g->seglen, ghosts, wl); \
+#define MUME(NDV, RT, DV, MH) hipLaunchKernelGGL((k_tvdmu_m<NDV, RT, DV, -1, MH, true>), gr, g->block, 0, g->stream, g->geo, s, g->bc, win, g->w1, wout, g->sc, g->part, g->pstride, g->seglen, ghosts, wl)
+#define MUM(NDV, RT, DV, LV) do { if (ent) { if (mhd) MUME(NDV, RT, DV, true); else MUME(NDV, RT, DV, false); } else if (mhd) hipLaunchKernelGGL((k_tvdmu_m<NDV, RT, DV, LV, true>), gr, g->block, 0, g->stream, g->geo, s, g->bc, win, g->w1, wout, g->sc, g->part, g->pstride, g->seglen, ghosts, wl); \
                                    else hipLaunchKernelGGL((k_tvdmu_m<NDV, RT, DV, LV, false>), gr, g->block, 0, g->stream, g->geo, s, g->bc, win, g->w1, wout, g->sc, g->part, g->pstride, g->seglen, ghosts, wl); } while (0)
 #define M21D(LV) MUM(2, double, 1, LV)
 #define M21F(LV) MUM(2, float, 1, LV)
@@ -493,6 +528,7 @@ int sweep_device(vacgpu_grid* g, const vacgpu_method* m, int d, int firstsweep, 
 #undef M32D
 #undef M32F
 #undef MUM
+#undef MUME
       }
 #undef LIMSW
       if ((rc = launch_check(g, "k_tvdmu_corr"))) return rc;
@@ -563,7 +599,9 @@ int sweep_device(vacgpu_grid* g, const vacgpu_method* m, int d, int firstsweep, 
     // first sweep and the first half of the Ryu step into the last one
     // (one limiter for all characteristic waves is fixed at compile time)
     const int lim = common_limiter(g, m);
-    const bool fold = g->cfg.ndim == 2 && g->bc_uniform && src && g->use_fused && lim >= 0 && !dirty;
+    const bool ent = entropy_on(g, m);
+    const bool fold = g->cfg.ndim == 2 && g->bc_uniform && src && g->use_fused && lim >= 0 && !dirty &&
+                      !ent;
     // (the Ryu half folds into the x sweep only: in the marching y sweep it
     // would cost occupancy)
     const int srcmode = (fold && firstsweep && !lastsweep) ? 1
@@ -590,7 +628,35 @@ int sweep_device(vacgpu_grid* g, const vacgpu_method* m, int d, int firstsweep, 
       const bool f32 = g->cfg.precision == VACGPU_FP32BULK;
 #define LIMSW(X) do { if (lim == LIM_MINMOD) X(LIM_MINMOD); else if (lim == LIM_WOODWARD) X(LIM_WOODWARD); else if (lim == LIM_SUPERBEE) X(LIM_SUPERBEE); else X(-1); } while (0)
 #define LIMSW3(X) do { if (lim == LIM_MINMOD) X(LIM_MINMOD); else if (lim == LIM_WOODWARD) X(LIM_WOODWARD); else X(LIM_SUPERBEE); } while (0)
-      if (!mhd) {
+      if (ent) {
+        // entropy fixes: limiter types read at run time, source steps unfolded
+        const dim3 gr = d == 0 ? g->grid_tx : g->grid_tm[d];
+        nbt = (long long)gr.x * gr.y * gr.z;
+#define EX(NDV, RT, MH) hipLaunchKernelGGL((k_tvd_x<NDV, RT, -1, 0, MH, true>), gr, g->block, 0, g->stream, g->geo, s, g->bc, wct, win, wout, g->sc, g->part, g->pstride, wold, g->w1, ghosts)
+#define EM(NDV, RT, DV, MH) hipLaunchKernelGGL((k_tvd_m<NDV, RT, DV, -1, 0, MH, true>), gr, g->block, 0, g->stream, g->geo, s, g->bc, wct, win, wout, g->sc, g->part, g->pstride, g->seglen, wold, g->w1, ghosts)
+#define EPH(X2, X3) do { if (g->cfg.ndim == 2) { if (f32) X2(float); else X2(double); } else { if (f32) X3(float); else X3(double); } } while (0)
+#define EX2M(RT) EX(2, RT, true)
+#define EX3M(RT) EX(3, RT, true)
+#define EX2H(RT) EX(2, RT, false)
+#define EX3H(RT) EX(3, RT, false)
+#define EM2M(RT) EM(2, RT, 1, true)
+#define EM3M(RT) do { if (d == 1) EM(3, RT, 1, true); else EM(3, RT, 2, true); } while (0)
+#define EM2H(RT) EM(2, RT, 1, false)
+#define EM3H(RT) do { if (d == 1) EM(3, RT, 1, false); else EM(3, RT, 2, false); } while (0)
+        if (d == 0) { if (mhd) EPH(EX2M, EX3M); else EPH(EX2H, EX3H); }
+        else { if (mhd) EPH(EM2M, EM3M); else EPH(EM2H, EM3H); }
+#undef EX2M
+#undef EX3M
+#undef EX2H
+#undef EX3H
+#undef EM2M
+#undef EM3M
+#undef EM2H
+#undef EM3H
+#undef EPH
+#undef EX
+#undef EM
+      } else if (!mhd) {
         // HD: the HD Roe solver, no source steps
         const dim3 gr = d == 0 ? g->grid_tx : g->grid_tm[d];
         nbt = (long long)gr.x * gr.y * gr.z;

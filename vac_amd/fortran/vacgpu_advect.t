@@ -46,7 +46,6 @@ if(acmexpo>zero) call die('vacgpu advect: acmexpo>0 is not on the GPU path')
 if(ndir/=ndim) call die('vacgpu advect: ndir must equal ndim')
 do iw=1,nw
    if(acmcoef(iw)>=zero) call die('vacgpu advect: acmcoef is not on the GPU path')
-   if(typeentropy(iw)/='nul') call die('vacgpu advect: typeentropy must be nul')
 end do
 select case(method)
 case('tvdlf')
@@ -86,6 +85,25 @@ m%dimsplit_xy=merge(1,0,typedimsplit=='xy')
 m%courantpar=courantpar
 if(implpar>=zero) m%courantpar=-one
 m%smallpcoeff=smallpcoeff
+m%typeentropy=VACGPU_ENT_NUL
+m%entropycoef=-one
+do iw=1,nw
+   select case(typeentropy(iw))
+   case('nul')
+      m%typeentropy(iw)=VACGPU_ENT_NUL
+   case('harten')
+      m%typeentropy(iw)=VACGPU_ENT_HARTEN
+   case('powell')
+      m%typeentropy(iw)=VACGPU_ENT_POWELL
+   case('ratio')
+      m%typeentropy(iw)=VACGPU_ENT_RATIO
+   case('yee')
+      m%typeentropy(iw)=VACGPU_ENT_YEE
+   case default
+      call die('vacgpu advect: typeentropy='//typeentropy(iw)//' is not on the GPU path')
+   end select
+   m%entropycoef(iw)=entropycoef(iw)
+end do
 select case(typelimited)
 case('original')
    m%typelimited=VACGPU_LIM_ORIGINAL

@@ -21,6 +21,8 @@ module vacgpu_mod
   integer(c_int), parameter :: VACGPU_FP64 = 0, VACGPU_FP32BULK = 1
   integer(c_int), parameter :: VACGPU_LIM_ORIGINAL = 0, VACGPU_LIM_PREVIOUS = 1, &
        VACGPU_LIM_PREDICTOR = 2
+  integer(c_int), parameter :: VACGPU_ENT_NUL = 0, VACGPU_ENT_HARTEN = 1, VACGPU_ENT_POWELL = 2, &
+       VACGPU_ENT_RATIO = 3, VACGPU_ENT_YEE = 4
 
   ! the drop-in's device grid (made by the first advect or projection), and
   ! whether a process() call comes from inside advect
@@ -46,6 +48,8 @@ module vacgpu_mod
      real(c_double) :: courantpar
      real(c_double) :: smallpcoeff
      integer(c_int) :: typelimited
+     integer(c_int) :: typeentropy(8)
+     real(c_double) :: entropycoef(8)
   end type vacgpu_method
 
   interface

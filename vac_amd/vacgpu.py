@@ -22,6 +22,7 @@ LIB_PATH = os.path.join(HERE, "libvacgpu.so")
 HD, MHD = 0, 1
 TVDLF, TVD, TVDMU = 0, 1, 2
 LIMITERS = {"minmod": 0, "woodward": 1, "superbee": 2}
+ENTROPY = {"nul": 0, "harten": 1, "powell": 2, "ratio": 3, "yee": 4}
 CONT, PERIODIC, SYMM, ASYMM, FIXED = 0, 1, 2, 3, 4
 BC_CODES = {"cont": CONT, "periodic": PERIODIC, "symm": SYMM, "asymm": ASYMM, "fixed": FIXED}
 FP64, FP32BULK = 0, 1
@@ -39,7 +40,8 @@ class Method(C.Structure):
     _fields_ = [("typefull", C.c_int), ("typelimiter", C.c_int * 8), ("divbfix", C.c_int),
                 ("divbwave", C.c_int), ("sourceunsplit", C.c_int), ("dimsplit_xy", C.c_int),
                 ("courantpar", C.c_double), ("smallpcoeff", C.c_double),
-                ("typelimited", C.c_int)]
+                ("typelimited", C.c_int), ("typeentropy", C.c_int * 8),
+                ("entropycoef", C.c_double * 8)]
 
 
 _lib = None
@@ -129,8 +131,10 @@ class MethodSpec:
     courantpar: float = 0.8
     smallpcoeff: float = 1e-6
     typelimited: str = "original"       # or "previous", "predictor"
+    typeentropy: object = "nul"         # one name, or a list per wave
+    entropycoef: object = None          # per wave; None: VAC's defaults
 
-    def to_c(self) -> Method:
+    def to_c(self, physics: str = "mhd") -> Method:
         m = Method()
         m.typefull = {"tvdlf": TVDLF, "tvd": TVD, "tvdmu": TVDMU}[self.typefull]
         lims = [self.limiter] * 8 if isinstance(self.limiter, str) else list(self.limiter) + ["minmod"] * 8
@@ -143,6 +147,21 @@ class MethodSpec:
         m.courantpar = self.courantpar
         m.smallpcoeff = self.smallpcoeff
         m.typelimited = {"original": 0, "previous": 1, "predictor": 2}[self.typelimited]
+        ents = [self.typeentropy] * 8 if isinstance(self.typeentropy, str) else \
+            list(self.typeentropy) + ["nul"] * 8
+        for i in range(8):
+            m.typeentropy[i] = ENTROPY[ents[i]]
+        coef = self.entropycoef
+        if coef is None:
+            # physini's defaults (0.2 fast/slow/sound, 0.4 Alfven, -1 else;
+            # single-precision literals stored as doubles)
+            f2, f4 = float(np.float32(0.2)), float(np.float32(0.4))
+            coef = [f2, f2, f2, f2, -1.0, -1.0, f4, f4]
+            if physics == "hd":
+                coef = [f2, f2] + [-1.0] * 6
+        coef = list(coef) + [-1.0] * 8
+        for i in range(8):
+            m.entropycoef[i] = float(coef[i])
         return m
 
 
@@ -303,7 +322,7 @@ class Grid:
         return {"dtcourant": dtc[: self.ndim], "nerror": ne, "smallp": sp.value, "dt": dt.value}
 
     def getdt_courant(self, method: MethodSpec, dtcourant=None, smallp=None):
-        m = method.to_c()
+        m = method.to_c(self.physics)
         dt = C.c_double()
         dtc = None if dtcourant is None else np.ascontiguousarray(dtcourant, dtype=np.float64)
         sp = None if smallp is None else C.c_double(smallp)
@@ -314,7 +333,7 @@ class Grid:
 
     def advect(self, method: MethodSpec, idim: int, dt: float, firstsweep: bool,
                lastsweep: bool, dtcourant=None, nerror=None, smallp=None):
-        m = method.to_c()
+        m = method.to_c(self.physics)
         ne = None if nerror is None else nerror.ctypes.data_as(_IP)
         sp = None if smallp is None else C.c_double(smallp)
         self._check(self.lib.vacgpu_advect(self.h, C.byref(m), idim, idim, dt, int(firstsweep),
@@ -324,7 +343,7 @@ class Grid:
 
     def run(self, method: MethodSpec, nsteps: int, itmax: int = 1 << 30, tmax: float = 1e99,
             tmaxexact: bool = False, dtpar: float = -1.0, record_t: bool = False):
-        m = method.to_c()
+        m = method.to_c(self.physics)
         hist = np.zeros(nsteps + 1) if record_t else None
         n = self._check(self.lib.vacgpu_run(self.h, C.byref(m), nsteps, itmax, tmax,
                                             int(tmaxexact), dtpar, _ptr(hist)), "run")
@@ -402,7 +421,7 @@ class Batch:
     def add(self, w_mesh, nx, physics, gamma, dxs, method: MethodSpec, bc="periodic",
             precision="fp64", it=0, t=0.0, itmax=1 << 30, tmax=1e99, tmaxexact=False) -> int:
         cfg = make_config(nx, physics, gamma, bc=bc, precision=precision)
-        m = method.to_c()
+        m = method.to_c(physics)
         w = np.ascontiguousarray(np.pad(np.asarray(w_mesh, dtype=np.float64),
                                         [(0, 0), (2, 2), (2, 2)]))
         dx0 = np.ascontiguousarray(dxs[0], dtype=np.float64)
