@@ -74,6 +74,11 @@ typedef struct {
   double entropycoef[8];/* 'yee''s lower limit per wave (VAC's physini
                            defaults: 0.2 fast/slow/sound, 0.4 Alfven, -1 else,
                            stored from single-precision literals) */
+  int useprimitive;     /* 1: TVDLF / TVD-MUSCL limit the primitive variables
+                           (upwindLR, src/vactvdlf.t:328-332, :360-364; FP64,
+                           typelimited 'original'), and the MHD Roe solver of
+                           TVD / TVD-MUSCL takes dp = pR - pL
+                           (src/vacphys.mhdroe.t:42-44) */
 } vacgpu_method;
 
 /* Replaces the geometry hand-off after gridsetup1 (src/vacgrid.t:457-593):

@@ -192,6 +192,48 @@ static void getpthermal(ctx_t* x, int clip, const double* w, const int* lo,
   }
 }
 
+/* primitive (MHD src/vacphys0.t.mhd:31-52, HD src/vacphys0.t.hd:30-51):
+ * e -> p (clipped getpthermal), m -> v = m/rho, in place over lo..hi */
+static void primitive(ctx_t* x, double* w, const int* lo, const int* hi) {
+  double* p = x->sc5;
+  getpthermal(x, 1, w, lo, hi, p);
+  double* e = PL(x, w, E_(x));
+  const double* rho = PL(x, w, RHO);
+  LOOP(lo, hi) { long idx = IDX(x); e[idx] = p[idx]; }
+  for (int c = 0; c < x->ndir; ++c) {
+    double* m = PL(x, w, M(c));
+    LOOP(lo, hi) { long idx = IDX(x); m[idx] = m[idx] / rho[idx]; }
+  }
+}
+
+/* conserve (MHD src/vacphys0.t.mhd:5-28, HD src/vacphys0.t.hd:5-27):
+ * p -> e = p/(g-1) + (rho v^2 + B^2)/2 (HD rho v^2/2), v -> m = rho v */
+static void conserve(ctx_t* x, double* w, const int* lo, const int* hi) {
+  const double g = x->p->gamma;
+  double* e = PL(x, w, E_(x));
+  const double* rho = PL(x, w, RHO);
+  LOOP(lo, hi) {
+    long idx = IDX(x);
+    double vsq = 0.0, bsq = 0.0;
+    for (int c = 0; c < x->ndir; ++c) {
+      double v = PL(x, w, M(c))[idx];
+      vsq = (c == 0) ? v * v : vsq + v * v;
+      if (x->mhd) {
+        double b = PL(x, w, B(x, c))[idx];
+        bsq = (c == 0) ? b * b : bsq + b * b;
+      }
+    }
+    if (x->mhd)
+      e[idx] = e[idx] / (g - 1.0) + 0.5 * (rho[idx] * vsq + bsq);
+    else
+      e[idx] = e[idx] / (g - 1.0) + 0.5 * rho[idx] * vsq;
+  }
+  for (int c = 0; c < x->ndir; ++c) {
+    double* m = PL(x, w, M(c));
+    LOOP(lo, hi) { long idx = IDX(x); m[idx] = rho[idx] * m[idx]; }
+  }
+}
+
 /* getcmax (MHD src/vacphys0.t.mhd:79-116 with getcsound2 :140-162;
  * HD src/vacphys0.t.hd:72-100) */
 static void getcmax(ctx_t* x, const double* w, const int* lo, const int* hi,
@@ -363,6 +405,32 @@ static void hancock(ctx_t* x, double qdt, int d, const double* wCT, double* wnew
   const int* hi = x->mhi;
   double* wL = x->wLC;  /* wLC(ixO) */
   double* wR = x->wRC;  /* wRC(hxO) stored at the cell index (ixO) */
+  if (x->p->useprimitive) {
+    /* upwindLR with useprimitive: wLC, wRC are copies of wCT made before
+     * the conversion; wCT itself is converted over ixG and back, in place */
+    for (int iw = 0; iw < x->nw; ++iw) {
+      const double* pc = PL(x, wCT, iw);
+      double* pl = PL(x, wL, iw);
+      double* pr = PL(x, wR, iw);
+      LOOP(lo, hi) { long idx = IDX(x); pl[idx] = pc[idx]; pr[idx] = pc[idx]; }
+    }
+    primitive(x, (double*)wCT, x->glo, x->ghi);
+    primitive(x, wL, lo, hi);
+    primitive(x, wR, lo, hi);
+    for (int iw = 0; iw < x->nw; ++iw) {
+      double* pl = PL(x, wL, iw);
+      double* pr = PL(x, wR, iw);
+      LOOP(lo, hi) {
+        long idx = IDX(x);
+        double l = ldw_at(x, wCT, iw, d, idx);
+        pr[idx] = pr[idx] - 0.5 * l;
+        pl[idx] = pl[idx] + 0.5 * l;
+      }
+    }
+    conserve(x, (double*)wCT, x->glo, x->ghi);
+    conserve(x, wL, lo, hi);
+    conserve(x, wR, lo, hi);
+  } else
   for (int iw = 0; iw < x->nw; ++iw) {
     const double* pc = PL(x, wCT, iw);
     double* pl = PL(x, wL, iw);
@@ -406,6 +474,33 @@ static void tvdlf_corrector(ctx_t* x, double qdt, int d, const double* wCT, doub
   double* wR = x->wRC;
   /* the limiter's base, typelimited (src/vactvdlf.t:150-161) */
   const double* wlim = x->p->typelimited == 1 ? x->wold : (x->p->typelimited == 2 ? wCT : wnew);
+  if (x->p->useprimitive) {
+    /* upwindLR with useprimitive: the copies of wCT and the limiter's base
+     * (over ixG, in place and back) in primitive variables */
+    for (int iw = 0; iw < x->nw; ++iw) {
+      const double* pc = PL(x, wCT, iw);
+      double* pl = PL(x, wL, iw);
+      double* pr = PL(x, wR, iw);
+      LOOP(lo, hi) { long idx = IDX(x); pr[idx] = pc[idx + s]; pl[idx] = pc[idx]; }
+    }
+    primitive(x, (double*)wlim, x->glo, x->ghi);
+    primitive(x, wL, lo, hi);
+    primitive(x, wR, lo, hi);
+    for (int iw = 0; iw < x->nw; ++iw) {
+      double* pl = PL(x, wL, iw);
+      double* pr = PL(x, wR, iw);
+      LOOP(lo, hi) {
+        long idx = IDX(x);
+        double lj = ldw_at(x, wlim, iw, d, idx);
+        double lj1 = ldw_at(x, wlim, iw, d, idx + s);
+        pr[idx] = pr[idx] - 0.5 * lj1;
+        pl[idx] = pl[idx] + 0.5 * lj;
+      }
+    }
+    conserve(x, (double*)wlim, x->glo, x->ghi);
+    conserve(x, wL, lo, hi);
+    conserve(x, wR, lo, hi);
+  } else
   for (int iw = 0; iw < x->nw; ++iw) {
     const double* pc = PL(x, wCT, iw);
     double* pl = PL(x, wL, iw);
@@ -505,7 +600,8 @@ static void roe_average(ctx_t* x, const double* wL, const double* wR,
     }
     double drho = PL(x, wR, RHO)[idx] - PL(x, wL, RHO)[idx];
     double t = PL(x, wR, eI)[idx] - PL(x, wL, eI)[idx] - s1 + 0.5 * s2 * drho - 0.5 * s3;
-    x->dp[idx] = (g - 1.0) * t;
+    /* useprimitive: dp = pR - pL (src/vacphys.mhdroe.t:42-44) */
+    x->dp[idx] = x->p->useprimitive ? x->aslow[idx] - x->afast[idx] : (g - 1.0) * t;
     x->rhodv[idx] = PL(x, wR, M(idim))[idx] - PL(x, wL, M(idim))[idx] -
                     PL(x, wroe, M(idim))[idx] * drho;
     /* getcsound2prim (src/vacphys0.t.mhd:119-137) */

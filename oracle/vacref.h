@@ -53,6 +53,10 @@ typedef struct {
    * 'yee''s lower limit of |a dt/dx| */
   int typeentropy[8];
   double entropycoef[8];
+  /* useprimitive: upwindLR limits the primitive variables (src/vactvdlf.t:
+   * 328-332, :360-364), converting its base state in place and back; the MHD
+   * Roe average takes dp = pR - pL (src/vacphys.mhdroe.t:42-44) */
+  int useprimitive;
 } vacref_params;
 
 enum { VACREF_ENT_NUL = 0, VACREF_ENT_HARTEN = 1, VACREF_ENT_POWELL = 2, VACREF_ENT_RATIO = 3,

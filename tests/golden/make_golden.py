@@ -124,6 +124,20 @@ ENT_CASES = {
 }
 
 
+# useprimitive=T: upwindLR limits the primitive variables (TVDLF and
+# TVD-MUSCL), the MHD Roe average takes dp = pR - pL (TVD and TVD-MUSCL)
+# name: (binary, physics, builder, gamma, steps, typefull, limiter, bc)
+PRIM_CASES = {
+    "mhd22_rf32x28_tvdlf_minmod_prim": ("vac_mhd22_132", "mhd", lambda g: random_fourier((32, 28), 801, g), 5 / 3, 12, "tvdlf", "minmod", "periodic"),
+    "mhd22_rf28x30_tvdlf_woodward_cont_prim": ("vac_mhd22_132", "mhd", lambda g: random_fourier((28, 30), 802, g), 1.4, 12, "tvdlf", "woodward", "cont"),
+    "mhd22_rf30x26_tvdmu_superbee_prim": ("vac_mhd22_132", "mhd", lambda g: random_fourier((30, 26), 803, g), 5 / 3, 10, "tvdmu", "superbee", "periodic"),
+    "mhd22_rf26x32_tvd_woodward_prim": ("vac_mhd22_132", "mhd", lambda g: random_fourier((26, 32), 804, g), 5 / 3, 12, "tvd", "woodward", "periodic"),
+    "hd22_rf30x28_tvdlf_minmod_prim": ("vac_hd22_132", "hd", lambda g: random_fourier((30, 28), 805, g, physics="hd"), 1.4, 15, "tvdlf", "minmod", "periodic"),
+    "hd22_rf28x26_tvdmu_woodward_cont_prim": ("vac_hd22_132", "hd", lambda g: random_fourier((28, 26), 806, g, physics="hd"), 1.4, 12, "tvdmu", "woodward", "cont"),
+    "mhd33_rf10x12x8_tvdlf_minmod_prim": ("vac_mhd33_36", "mhd", lambda g: random_fourier((10, 12, 8), 807, g), 5 / 3, 6, "tvdlf", "minmod", "periodic"),
+}
+
+
 def ensure_binary(name: str) -> None:
     if have_ref(name):
         return
@@ -237,6 +251,24 @@ def main() -> None:
             nx=np.array([len(c) for c in coords]), gamma=gamma, steps=steps,
             physics=physics, typefull=typefull, limiter=limiter, bc="periodic", divbfix=True,
             typeentropy=ent)
+        print("wrote", name, w0.shape, "steps", len(t) - 1)
+    for name, (binary, physics, build, gamma, steps, typefull, limiter, bc) in PRIM_CASES.items():
+        if only and not any(o in name for o in only):
+            continue
+        ensure_binary(binary)
+        w0, coords = build(gamma)
+        nw = w0.shape[0]
+        ini = make_ini(w0, coords, physics, gamma, name="golden")
+        par = problem_par(nw, NAMES[nw], steps, typefull=typefull, limiter=limiter,
+                          typeB=[bc] * (2 * len(coords)), every_step=True,
+                          extra_method=" useprimitive=T\n")
+        r = run_ref(binary, ini, par)
+        t = np.array([s_.t for s_ in r.snapshots])
+        np.savez_compressed(
+            os.path.join(HERE, name + ".npz"), w0=w0, wf=r.snapshots[-1].w, t=t,
+            nx=np.array([len(c) for c in coords]), gamma=gamma, steps=steps,
+            physics=physics, typefull=typefull, limiter=limiter, bc=bc, divbfix=True,
+            useprimitive=True)
         print("wrote", name, w0.shape, "steps", len(t) - 1)
     if only:
         return

@@ -49,19 +49,29 @@ CASES = [
     # an entropy fix on every wave of the Roe dissipation
     ("vac_mhd22_132", "vacgpu_mhd22_132", "mhd", lambda g: random_fourier((44, 36), 73, g), 5 / 3, 12,
      "tvd", "woodward", "periodic", True, "", "harten"),
+    # useprimitive=T: limiting in primitive variables, dp = pR - pL in the Roe solver
+    ("vac_mhd22_132", "vacgpu_mhd22_132", "mhd", lambda g: random_fourier((40, 36), 74, g), 5 / 3, 15,
+     "tvdlf", "woodward", "periodic", True, "", "", True),
+    ("vac_hd22_132", "vacgpu_hd22_132", "hd", lambda g: random_fourier((36, 40), 75, g, physics="hd"), 1.4, 15,
+     "tvdmu", "minmod", "cont", True, "", "", True),
+    ("vac_mhd33_36", "vacgpu_mhd33_36", "mhd", lambda g: random_fourier((12, 10, 14), 76, g), 5 / 3, 6,
+     "tvdlf", "minmod", "periodic", True, "", "", True),
 ]
 
 
 @pytest.mark.parametrize("case", CASES, ids=[f"{c[6]}-{c[7]}-{c[2]}{len(c[3](1.4)[1])}d"
                                              + ("" if isinstance(c[8], str) else "-walls")
                                              + (f"-{c[10]}" if len(c) > 10 and c[10] else "")
-                                             + (f"-{c[11]}" if len(c) > 11 else "")
+                                             + (f"-{c[11]}" if len(c) > 11 and c[11] else "")
+                                             + ("-prim" if len(c) > 12 and c[12] else "")
                                              for c in CASES])
 def test_vac_with_gpu_advect_matches_vac(case):
     cpu, gpu, physics, build, gamma, steps, typefull, limiter, bc, divbfix = case[:10]
     extra = f" typelimited='{case[10]}'\n" if len(case) > 10 and case[10] else ""
-    if len(case) > 11:
+    if len(case) > 11 and case[11]:
         extra += f" typeentropy={len(NAMES[6 if physics == 'mhd' else 4].split())}*'{case[11]}'\n"
+    if len(case) > 12 and case[12]:
+        extra += " useprimitive=T\n"
     if not (have_ref(cpu) and have_ref(gpu)):
         pytest.skip("reference executables not built (oracle/build_ref.sh)")
     w0, coords = build(gamma)

@@ -63,8 +63,10 @@ def test_gpu_matches_reference_fixture(path):
     nx = [int(v) for v in z["nx"]]
     tl = str(z["typelimited"]) if "typelimited" in z.files else "original"
     te = str(z["typeentropy"]) if "typeentropy" in z.files else "nul"
+    up = bool(z["useprimitive"]) if "useprimitive" in z.files else False
     method = vacgpu.MethodSpec(typefull=str(z["typefull"]), limiter=str(z["limiter"]),
-                               divbfix=bool(z["divbfix"]), typelimited=tl, typeentropy=te)
+                               divbfix=bool(z["divbfix"]), typelimited=tl, typeentropy=te,
+                               useprimitive=up)
     print(f"case {os.path.basename(path)} nx={nx}", flush=True)
     w, t, sc = gpu_run(z["w0"], nx, str(z["physics"]), float(z["gamma"]), int(z["steps"]),
                        method, bc=vacref.bc_spec(z["bc"]))
@@ -242,3 +244,46 @@ def test_gpu_entropy_fix_matches_oracle(case):
     wr, tr, st = vacref.run(p, w0, [widths_1d(c) for c in coords], steps)
     assert np.array_equal(t, tr)
     assert_bitwise(w, wr, str(case))
+
+
+@pytest.mark.parametrize("case", [
+    ("mhd", (64, 56), "tvdlf", "woodward", "periodic", 10),
+    ("mhd", (48, 60), "tvdlf", "superbee", "cont", 10),
+    ("mhd", (56, 48), "tvdmu", "minmod", "periodic", 10),
+    ("mhd", (60, 52), "tvd", "woodward", "cont", 10),
+    ("mhd", (14, 12, 16), "tvdlf", "minmod", "periodic", 5),
+    ("mhd", (12, 16, 14), "tvdmu", "woodward", "cont", 5),
+    ("hd", (64, 48), "tvdlf", "minmod", "cont", 15),
+    ("hd", (48, 56), "tvdmu", "superbee", "periodic", 12),
+    ("hd", (12, 10, 14), "tvdlf", "woodward", "periodic", 6),
+])
+def test_gpu_useprimitive_matches_oracle(case):
+    # useprimitive=T: TVDLF / TVD-MUSCL limit the primitive variables (the
+    # two-launch form; the fused 2D TVDLF sweep stands aside), the MHD Roe
+    # solver takes dp = pR - pL
+    physics, nx, typefull, limiter, bc, steps = case
+    gamma = 5 / 3
+    w0, coords = random_fourier(nx, 41 + len(nx), gamma, physics=physics)
+    method = vacgpu.MethodSpec(typefull=typefull, limiter=limiter, useprimitive=True)
+    print(f"case {case}", flush=True)
+    w, t, sc = gpu_run(w0, list(nx), physics, gamma, steps, method, bc=bc)
+    p = vacref.make_params(len(nx), physics, list(nx), gamma, typefull=typefull,
+                           limiter=limiter, bc=bc, useprimitive=True)
+    wr, tr, st = vacref.run(p, w0, [widths_1d(c) for c in coords], steps)
+    assert np.array_equal(t, tr)
+    assert_bitwise(w, wr, str(case))
+    assert list(sc["nerror"]) == list(st.nerror)
+
+
+def test_gpu_useprimitive_rejections():
+    nx = [16, 16]
+    w0, coords = random_fourier((16, 16), 1, 1.4)
+    dxs = [widths_1d(c) for c in coords]
+    for kw, prec in (({"typefull": "tvdlf", "typelimited": "previous"}, "fp64"),
+                     ({"typefull": "tvdmu", "typelimited": "predictor"}, "fp64"),
+                     ({"typefull": "tvdlf"}, "fp32bulk")):
+        g = vacgpu.Grid(nx, "mhd", 1.4, dxs, precision=prec)
+        g.put_mesh(w0)
+        with pytest.raises(vacgpu.VacGpuError):
+            g.run(vacgpu.MethodSpec(useprimitive=True, **kw), 1)
+        g.close()

@@ -43,6 +43,8 @@ def run_fixture(fx):
         kw["typelimited"] = str(fx["typelimited"])
     if "typeentropy" in fx:
         kw["typeentropy"] = str(fx["typeentropy"])
+    if "useprimitive" in fx:
+        kw["useprimitive"] = bool(fx["useprimitive"])
     if "nproc1" in fx:  # projection of B before every step
         kw.update(divbwave=bool(fx["divbwave"]), nproc1=int(fx["nproc1"]),
                   procpar=[float(v) for v in fx["procpar"]])
@@ -140,13 +142,18 @@ def _run_state(w, nx, physics, gamma, steps, **kw):
     return vacref.run(p, w, dxs, steps)
 
 
-@pytest.mark.parametrize("typefull,limiter", [("tvdlf", "minmod"), ("tvd", "woodward")])
-def test_uniform_state_unchanged(typefull, limiter):
+@pytest.mark.parametrize("typefull,limiter,useprimitive", [
+    ("tvdlf", "minmod", False), ("tvd", "woodward", False),
+    ("tvdlf", "woodward", True), ("tvdmu", "superbee", True), ("tvd", "minmod", True)])
+def test_uniform_state_unchanged(typefull, limiter, useprimitive):
+    # (useprimitive: the conversion round trips change a uniform state by
+    # rounding only)
     g = 5.0 / 3.0
     nx = [16, 12]
     u = np.array([1.3, 0.2, -0.1, 3.0, 0.4, 0.3])
     w = np.broadcast_to(u[:, None, None], (6, 12, 16)).copy()
-    out, t, st = _run_state(w, nx, "mhd", g, 5, typefull=typefull, limiter=limiter)
+    out, t, st = _run_state(w, nx, "mhd", g, 5, typefull=typefull, limiter=limiter,
+                            useprimitive=useprimitive)
     np.testing.assert_allclose(out, w, rtol=1e-13, atol=1e-14)
 
 
@@ -196,3 +203,30 @@ def test_oracle_per_variable_boundaries_against_live_reference():
     out, t, st = vacref.run(p, w, [widths_1d(cc) for cc in c], 10)
     assert np.array_equal(np.array([s.t for s in r.snapshots]), t)
     assert np.array_equal(r.snapshots[-1].w, out)
+
+
+def test_useprimitive_limits_primitive_variables():
+    """useprimitive=T against =F.  A smooth density wave at uniform v, p and
+    B: m and e are then linear in rho, so limiting the primitive or the
+    conserved variables reconstructs the same face states up to rounding.
+    A density jump at uniform momentum makes v jump as well: the limited
+    quantities differ and so do the results, at leading order."""
+    g = 5.0 / 3.0
+    nx = [32, 8]
+    x = (np.arange(32) + 0.5) / 32
+    base = np.array([1.0, 0.3, 0.1, 2.0, 0.2, 0.1])
+    w = np.broadcast_to(base[:, None, None], (6, 8, 32)).copy()
+    w[0] += 0.01 * np.sin(2 * np.pi * x)[None, :]
+    a, _, _ = _run_state(w, nx, "mhd", g, 2, typefull="tvdlf", limiter="woodward",
+                         useprimitive=True)
+    b, _, _ = _run_state(w, nx, "mhd", g, 2, typefull="tvdlf", limiter="woodward",
+                         useprimitive=False)
+    assert not np.array_equal(a, b)
+    assert np.abs(a - b).max() < 1e-5
+    w2 = w.copy()
+    w2[0, :, 16:] *= 4.0  # a density jump at fixed momentum: v jumps too
+    a2, _, _ = _run_state(w2, nx, "mhd", g, 2, typefull="tvdlf", limiter="minmod",
+                          useprimitive=True)
+    b2, _, _ = _run_state(w2, nx, "mhd", g, 2, typefull="tvdlf", limiter="minmod",
+                          useprimitive=False)
+    assert np.abs(a2 - b2).max() > 1e-4

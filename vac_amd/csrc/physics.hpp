@@ -106,6 +106,52 @@ __device__ __forceinline__ double pthermal(const double* u, double g, double sma
   return p < smallp ? smallp : p;
 }
 
+// primitive (MHD src/vacphys0.t.mhd:31-52, HD src/vacphys0.t.hd:30-51):
+// e -> the clipped thermal pressure, m -> v = m/rho; the rest is copied.
+template <int ND, bool MHD>
+__device__ __forceinline__ void to_prim(const double* u, double g, double smallp, double* q) {
+  using L = Layout<ND, MHD>;
+#pragma unroll
+  for (int iw = 0; iw < L::nw; ++iw) q[iw] = u[iw];
+  q[L::e] = pthermal<ND, MHD>(u, g, smallp);
+#pragma unroll
+  for (int c = 0; c < ND; ++c) q[L::m(c)] = pdiv(u[L::m(c)], u[L::rho]);
+}
+
+// conserve (MHD src/vacphys0.t.mhd:5-28, HD src/vacphys0.t.hd:5-27):
+// e = p/(g-1) + (rho*v^2 + B^2)/2 (HD: rho*v^2/2), then m = rho*v; in place.
+template <int ND, bool MHD>
+__device__ __forceinline__ void to_cons(double* q, double g) {
+  using L = Layout<ND, MHD>;
+  double vsq = q[L::m(0)] * q[L::m(0)];
+#pragma unroll
+  for (int c = 1; c < ND; ++c) vsq = vsq + q[L::m(c)] * q[L::m(c)];
+  if constexpr (MHD) {
+    double bsq = q[L::b(0)] * q[L::b(0)];
+#pragma unroll
+    for (int c = 1; c < ND; ++c) bsq = bsq + q[L::b(c)] * q[L::b(c)];
+    q[L::e] = pdiv(q[L::e], g - 1.0) + 0.5 * (q[L::rho] * vsq + bsq);
+  } else {
+    q[L::e] = pdiv(q[L::e], g - 1.0) + 0.5 * q[L::rho] * vsq;
+  }
+#pragma unroll
+  for (int c = 0; c < ND; ++c) q[L::m(c)] = q[L::rho] * q[L::m(c)];
+}
+
+// With useprimitive every upwindLR call converts its base state to primitive
+// variables and back in place (src/vactvdlf.t:328-332, :360-364).  The
+// TVDLF / TVD-MUSCL corrector limits a state the predictor of the same sweep
+// has already converted once, so its limiter sees prim(cons(prim(u))) of the
+// sweep's input u (computed here in place), and the update's base is
+// cons of that.
+template <int ND, bool MHD>
+__device__ __forceinline__ void prim_of_roundtrip(double* u, double g, double smallp) {
+  double q[Layout<ND, MHD>::nw];
+  to_prim<ND, MHD>(u, g, smallp, q);
+  to_cons<ND, MHD>(q, g);
+  to_prim<ND, MHD>(q, g, smallp, u);
+}
+
 // Total pressure (src/vacphys0.t.mhd:244-252), unclipped, in double.
 template <int ND>
 __device__ __forceinline__ double ptotal(const double* u, double g) {
@@ -240,9 +286,10 @@ struct RoeRt {
 // average + the parts of geteigenjump that do not depend on il
 // (src/vacphys.mhdroe.t:4-122 and :156-186), plus the rtimes helpers
 // (:320-345).  uL, uR are the conserved states left and right of the face.
+// prim: useprimitive, where dp = pR - pL (src/vacphys.mhdroe.t:42-44).
 template <int ND, class R>
 __device__ __forceinline__ void roe_setup(const double* uL, const double* uR, int d, double g,
-                                          double smallp, Roe<ND, R>& r) {
+                                          double smallp, Roe<ND, R>& r, int prim = 0) {
   using L = Layout<ND, true>;
   const R rL = R(uL[L::rho]), rR = R(uR[L::rho]);
   const R rho = R(0.5) * (rL + rR);
@@ -265,7 +312,8 @@ __device__ __forceinline__ void roe_setup(const double* uL, const double* uR, in
     s3 = s3 + R(uR[L::b(c)]) * R(uR[L::b(c)]) - R(uL[L::b(c)]) * R(uL[L::b(c)]);
   }
   const R drho = rR - rL;
-  r.dp = (R(g) - R(1)) * (R(uR[L::e]) - R(uL[L::e]) - s1 + R(0.5) * s2 * drho - R(0.5) * s3);
+  r.dp = prim ? R(pR - pL)
+              : (R(g) - R(1)) * (R(uR[L::e]) - R(uL[L::e]) - s1 + R(0.5) * s2 * drho - R(0.5) * s3);
   r.rhodv = R(uR[L::m(d)]) - R(uL[L::m(d)]) - r.v[d] * drho;
   const R c2 = pdiv(R(g) * pbar, rho);
   r.cs2 = c2;
@@ -489,7 +537,7 @@ struct RoeRtH {
 // the conserved states left and right of the face.
 template <int ND, class R>
 __device__ __forceinline__ void roe_setup(const double* uL, const double* uR, int d, double g,
-                                          double smallp, RoeH<ND, R>& r) {
+                                          double smallp, RoeH<ND, R>& r, int /*prim: no effect*/ = 0) {
   using L = Layout<ND, false>;
   const R rL = R(uL[L::rho]), rR = R(uR[L::rho]);
   R t = R(sqrt(rL));

@@ -36,6 +36,7 @@ struct Sel {
   double courantpar, gamma;
   int ent[8];        // typeentropy per wave (ENT_*)
   double ecoef[8];   // entropycoef per wave ('yee')
+  int prim;          // useprimitive
 };
 
 // boundary types per (face, variable): t = 1 for 'periodic'; k the kind of a
@@ -234,7 +235,10 @@ __global__ __launch_bounds__(256) void k_bound(Geo g, BC bc, double* __restrict_
 // The limited slopes l along D are returned in lout when it is not null.
 // LIM: limiter type fixed at compile time (>= 0) or per variable from s
 // (< 0); PWM: Powell source on (1), off (0) or as s selects (2).
-template <int ND, bool MHD, class R, int D, int LIM, int PWM>
+// PRIM (useprimitive): the limiter and the w -+ l/2 states in primitive
+// variables (upwindLR, src/vactvdlf.t:328-364); the source then sees the
+// state after its conversion to primitive variables and back.
+template <int ND, bool MHD, class R, int D, int LIM, int PWM, bool PRIM = false>
 __device__ __forceinline__ void pred_compute(const Sel& s, const double* uc, const double* um,
                                              const double* up, const double (*bn)[2],
                                              const double* dxk, double dt, double smallp,
@@ -244,12 +248,31 @@ __device__ __forceinline__ void pred_compute(const Sel& s, const double* uc, con
   constexpr int NW = L::nw;
   const double qdt = dt / 2;
   double wl[NW], wr[NW];
+  [[maybe_unused]] double ucr[NW];
+  if constexpr (PRIM) {
+    double qc[NW], qm[NW], qp[NW];
+    to_prim<ND, MHD>(uc, s.gamma, smallp, qc);
+    to_prim<ND, MHD>(um, s.gamma, smallp, qm);
+    to_prim<ND, MHD>(up, s.gamma, smallp, qp);
 #pragma unroll
-  for (int iw = 0; iw < NW; ++iw) {
-    const R l = limiter_t<LIM, R>(s.lim[iw], R(up[iw] - uc[iw]), R(uc[iw] - um[iw]));
-    wr[iw] = uc[iw] - 0.5 * double(l);
-    wl[iw] = uc[iw] + 0.5 * double(l);
-    if (lout) lout[iw] = double(l);
+    for (int iw = 0; iw < NW; ++iw) {
+      const R l = limiter_t<LIM, R>(s.lim[iw], R(qp[iw] - qc[iw]), R(qc[iw] - qm[iw]));
+      wr[iw] = qc[iw] - 0.5 * double(l);
+      wl[iw] = qc[iw] + 0.5 * double(l);
+      if (lout) lout[iw] = double(l);
+      ucr[iw] = qc[iw];
+    }
+    to_cons<ND, MHD>(wr, s.gamma);
+    to_cons<ND, MHD>(wl, s.gamma);
+    to_cons<ND, MHD>(ucr, s.gamma);
+  } else {
+#pragma unroll
+    for (int iw = 0; iw < NW; ++iw) {
+      const R l = limiter_t<LIM, R>(s.lim[iw], R(up[iw] - uc[iw]), R(uc[iw] - um[iw]));
+      wr[iw] = uc[iw] - 0.5 * double(l);
+      wl[iw] = uc[iw] + 0.5 * double(l);
+      if (lout) lout[iw] = double(l);
+    }
   }
   double fL[NW], fR[NW];
   fluxes<ND, MHD, R>(wl, d, s.gamma, smallp, fL);
@@ -268,13 +291,13 @@ __device__ __forceinline__ void pred_compute(const Sel& s, const double* uc, con
         const double gq = pdiv(0.5 * (bp - bm), dxk[k]);
         dv = dv + gq;
       }
-      powell<ND>(uc, dv, qdt * 1.0 / ND, out);
+      powell<ND>(PRIM ? ucr : uc, dv, qdt * 1.0 / ND, out);
     }
   }
 }
 
 // the predictor at cell idx / c of a grid in HBM
-template <int ND, bool MHD, class R, int D, int LIM = -1, int PWM = 2>
+template <int ND, bool MHD, class R, int D, int LIM = -1, int PWM = 2, bool PRIM = false>
 __device__ __forceinline__ void pred_point(const Geo& g, const Sel& s, const double* __restrict__ w,
                                            long long idx, const int* c, double dt, double smallp,
                                            double* out) {
@@ -299,7 +322,7 @@ __device__ __forceinline__ void pred_point(const Geo& g, const Sel& s, const dou
       }
     }
   }
-  pred_compute<ND, MHD, R, D, LIM, PWM>(s, uc, um, up, bn, dxk, dt, smallp, out, nullptr);
+  pred_compute<ND, MHD, R, D, LIM, PWM, PRIM>(s, uc, um, up, bn, dxk, dt, smallp, out, nullptr);
 }
 
 template <int ND, bool MHD, class R, int D>
@@ -317,24 +340,48 @@ __device__ __forceinline__ void tvdlf_pred_body(const Geo& g, const Sel& s,
 }
 
 
+// the MUSCL states of the face J+1/2 from w1(J) = pl, w1(J+1) = pr and the
+// limited slopes ll, lr: wL = pl + ll/2, wR = pr - lr/2, formed in primitive
+// variables and converted back with PRIM (upwindLR, src/vactvdlf.t:328-364)
+template <int ND, bool MHD, bool PRIM>
+__device__ __forceinline__ void muscl_states(const Sel& s, const double* pl, const double* pr,
+                                             const double* ll, const double* lr, double smallp,
+                                             double* wlf, double* wrf) {
+  constexpr int NW = Layout<ND, MHD>::nw;
+  if constexpr (PRIM) {
+    to_prim<ND, MHD>(pr, s.gamma, smallp, wrf);
+    to_prim<ND, MHD>(pl, s.gamma, smallp, wlf);
+#pragma unroll
+    for (int iw = 0; iw < NW; ++iw) {
+      wrf[iw] = wrf[iw] - 0.5 * lr[iw];
+      wlf[iw] = wlf[iw] + 0.5 * ll[iw];
+    }
+    to_cons<ND, MHD>(wrf, s.gamma);
+    to_cons<ND, MHD>(wlf, s.gamma);
+  } else {
+#pragma unroll
+    for (int iw = 0; iw < NW; ++iw) {
+      wrf[iw] = pr[iw] - 0.5 * lr[iw];
+      wlf[iw] = pl[iw] + 0.5 * ll[iw];
+    }
+  }
+}
+
 // TVDLF corrector flux at the face J+1/2 (src/vactvdlf.t:147-254): MUSCL
 // states wR = w1(J+1) - l(J+1)/2, wL = w1(J) + l(J)/2 from the predictor
 // states pl = w1(J), pr = w1(J+1) and the limited slopes ll, lr of the
 // start-of-sweep w (typelimited='original'); cmax of the mean state; the
 // wLC = 2*mean - wRC restore (:205); F = (F(wL) + F(wR))/2 - cmax*(wR - wL)/2.
 // cv = 2*cmax/(dx(J) + dx(J+1)) is the face's Courant value (:190).
-template <int ND, bool MHD, class R, int D>
+// PRIM (useprimitive): the MUSCL states are formed in primitive variables.
+template <int ND, bool MHD, class R, int D, bool PRIM = false>
 __device__ __forceinline__ void corr_face(const Sel& s, const double* pl, const double* pr,
                                           const double* ll, const double* lr, double dxsum,
                                           double smallp, double* F, double& cv) {
   constexpr int d = D;
   constexpr int NW = Layout<ND, MHD>::nw;
   double wlf[NW], wrf[NW], mean[NW];
-#pragma unroll
-  for (int iw = 0; iw < NW; ++iw) {
-    wrf[iw] = pr[iw] - 0.5 * lr[iw];
-    wlf[iw] = pl[iw] + 0.5 * ll[iw];
-  }
+  muscl_states<ND, MHD, PRIM>(s, pl, pr, ll, lr, smallp, wlf, wrf);
 #pragma unroll
   for (int iw = 0; iw < NW; ++iw) mean[iw] = 0.5 * (wlf[iw] + wrf[iw]);
   const R cm = cmax<ND, MHD, R>(mean, d, s.gamma, smallp);
@@ -517,7 +564,7 @@ __device__ __forceinline__ void store_cell(const Geo& g, const BC& bc, double* _
 
 // predictor launch: one cell per thread; with ghosts set the ghost cells of
 // w1 that take a cell's value are written too (one boundary type per face)
-template <int ND, bool MHD, class R, int D>
+template <int ND, bool MHD, class R, int D, bool PRIM = false>
 __global__ __launch_bounds__(256) void k_tvdlf_pred(Geo g, Sel s, BC bc, const double* __restrict__ w,
                                                    double* __restrict__ w1,
                                                    const Scal* __restrict__ sc, int ghosts) {
@@ -525,7 +572,7 @@ __global__ __launch_bounds__(256) void k_tvdlf_pred(Geo g, Sel s, BC bc, const d
   const CellIx cl = cell_of(g);
   if (!cl.ok) return;
   double out[NW];
-  pred_point<ND, MHD, R, D>(g, s, w, cl.idx, cl.c, sc->dt, sc->smallp, out);
+  pred_point<ND, MHD, R, D, -1, 2, PRIM>(g, s, w, cl.idx, cl.c, sc->dt, sc->smallp, out);
   store_cell<ND, NW>(g, bc, w1, cl.idx, cl.c, out, ghosts);
 }
 
@@ -538,7 +585,7 @@ __global__ __launch_bounds__(256) void k_tvdlf_pred(Geo g, Sel s, BC bc, const d
 // face and of a cell update is tvdlf_corr_body's.
 constexpr int CX_STRIDE = 62;
 
-template <int ND, bool MHD, class R, int LIM>
+template <int ND, bool MHD, class R, int LIM, bool PRIM = false>
 __global__ __launch_bounds__(256) void k_tvdlf_corrx(Geo g, Sel s, BC bc, const double* __restrict__ w,
                                                     const double* __restrict__ w1,
                                                     double* __restrict__ wout,
@@ -561,24 +608,42 @@ __global__ __launch_bounds__(256) void k_tvdlf_corrx(Geo g, Sel s, BC bc, const 
   const int yc = rowok ? y : g.n[1] + 1;
   const long long idx = g.off + cc + (long long)yc * g.s[1] + (long long)z * g.s[2];
   double l[NW], lr[NW], u0[NW], u1[NW];
+  [[maybe_unused]] double ub[NW];
+  if constexpr (PRIM) {
+    // useprimitive ('original' only): limit prim(cons(prim(w))) and update
+    // cons of it, the state the predictor's conversion left
+    double qa[NW];
+    load<NW>(wl, g.ps, idx - 1, qa);
+    load<NW>(wl, g.ps, idx, ub);
+    load<NW>(wl, g.ps, idx + 1, u1);
+    prim_of_roundtrip<ND, MHD>(qa, s.gamma, smallp);
+    prim_of_roundtrip<ND, MHD>(ub, s.gamma, smallp);
+    prim_of_roundtrip<ND, MHD>(u1, s.gamma, smallp);
 #pragma unroll
-  for (int iw = 0; iw < NW; ++iw) {
-    const double* p = wl + iw * g.ps + idx;  // the limiter's base (typelimited)
-    l[iw] = slope<R, LIM>(s.lim[iw], p[-1], p[0], p[1]);
-    u0[iw] = w1[iw * g.ps + idx];
-    u1[iw] = w1[iw * g.ps + idx + 1];
+    for (int iw = 0; iw < NW; ++iw) l[iw] = slope<R, LIM>(s.lim[iw], qa[iw], ub[iw], u1[iw]);
+    to_cons<ND, MHD>(ub, s.gamma);
+    load<NW>(w1, g.ps, idx, u0);
+    load<NW>(w1, g.ps, idx + 1, u1);
+  } else {
+#pragma unroll
+    for (int iw = 0; iw < NW; ++iw) {
+      const double* p = wl + iw * g.ps + idx;  // the limiter's base (typelimited)
+      l[iw] = slope<R, LIM>(s.lim[iw], p[-1], p[0], p[1]);
+      u0[iw] = w1[iw * g.ps + idx];
+      u1[iw] = w1[iw * g.ps + idx + 1];
+    }
   }
 #pragma unroll
   for (int iw = 0; iw < NW; ++iw) lr[iw] = lane_next(l[iw]);
   double F[NW], cv;
-  corr_face<ND, MHD, R, D>(s, u0, u1, l, lr, g.dx[0][cc] + g.dx[0][cc + 1], smallp, F, cv);
+  corr_face<ND, MHD, R, D, PRIM>(s, u0, u1, l, lr, g.dx[0][cc] + g.dx[0][cc + 1], smallp, F, cv);
   double cmx = face_ok ? cv : -INFINITY;
   double out[NW];
   const double qdx = pdiv(dt, g.dx[0][cc]);
 #pragma unroll
   for (int iw = 0; iw < NW; ++iw) {
     const double Fm = lane_prev(F[iw]);
-    out[iw] = w[iw * g.ps + idx] - qdx * (F[iw] - Fm);
+    out[iw] = (PRIM ? ub[iw] : w[iw * g.ps + idx]) - qdx * (F[iw] - Fm);
   }
   if constexpr (MHD) {
     if (s.sourceunsplit && s.divbfix) {
@@ -595,7 +660,7 @@ __global__ __launch_bounds__(256) void k_tvdlf_corrx(Geo g, Sel s, BC bc, const 
   if (threadIdx.x == 0 && threadIdx.y == 0) part[SLOT_C0 * pstride + block_linear()] = bm;
 }
 
-template <int ND, bool MHD, class R, int D, int LIM>
+template <int ND, bool MHD, class R, int D, int LIM, bool PRIM = false>
 __global__ __launch_bounds__(256) void k_tvdlf_corrm(Geo g, Sel s, BC bc, const double* __restrict__ w,
                                                     const double* __restrict__ w1,
                                                     double* __restrict__ wout,
@@ -624,11 +689,17 @@ __global__ __launch_bounds__(256) void k_tvdlf_corrm(Geo g, Sel s, BC bc, const 
     // window: the limiter's base wl (typelimited) at J-1, J, J+1 (for the
     // slope at J) and w1 at J; slope at J.  The update's base is w, the
     // window itself when wl is w.
+    // With PRIM (useprimitive, 'original') the window holds
+    // prim(cons(prim(w))), the update's base is cons of it.
     const bool own = wl == w;
+    auto lwin = [&](long long at, double* v) {
+      load<NW>(wl, g.ps, at, v);
+      if constexpr (PRIM) prim_of_roundtrip<ND, MHD>(v, s.gamma, smallp);
+    };
     double wa[NW], wb[NW], wc[NW], u0[NW], lc[NW], Fp[NW];
-    load<NW>(wl, g.ps, base + (long long)(s0 - 2) * sd, wa);
-    load<NW>(wl, g.ps, base + (long long)(s0 - 1) * sd, wb);
-    load<NW>(wl, g.ps, base + (long long)s0 * sd, wc);
+    lwin(base + (long long)(s0 - 2) * sd, wa);
+    lwin(base + (long long)(s0 - 1) * sd, wb);
+    lwin(base + (long long)s0 * sd, wc);
     load<NW>(w1, g.ps, base + (long long)(s0 - 1) * sd, u0);
 #pragma unroll
     for (int iw = 0; iw < NW; ++iw) lc[iw] = slope<R, LIM>(s.lim[iw], wa[iw], wb[iw], wc[iw]);
@@ -636,11 +707,11 @@ __global__ __launch_bounds__(256) void k_tvdlf_corrm(Geo g, Sel s, BC bc, const 
     {
       double u1[NW], ln[NW], wd[NW];
       load<NW>(w1, g.ps, base + (long long)s0 * sd, u1);
-      load<NW>(wl, g.ps, base + (long long)(s0 + 1) * sd, wd);
+      lwin(base + (long long)(s0 + 1) * sd, wd);
 #pragma unroll
       for (int iw = 0; iw < NW; ++iw) ln[iw] = slope<R, LIM>(s.lim[iw], wb[iw], wc[iw], wd[iw]);
       double cv;
-      corr_face<ND, MHD, R, D>(s, u0, u1, lc, ln, dxd[s0 - 1] + dxd[s0], smallp, Fp, cv);
+      corr_face<ND, MHD, R, D, PRIM>(s, u0, u1, lc, ln, dxd[s0 - 1] + dxd[s0], smallp, Fp, cv);
       cmx = cmx < cv ? cv : cmx;
 #pragma unroll
       for (int iw = 0; iw < NW; ++iw) {
@@ -655,16 +726,23 @@ __global__ __launch_bounds__(256) void k_tvdlf_corrm(Geo g, Sel s, BC bc, const 
     for (int i = s0; i <= s1; ++i) {
       double u1[NW], ln[NW], wd[NW], F[NW], cv;
       load<NW>(w1, g.ps, base + (long long)(i + 1) * sd, u1);
-      load<NW>(wl, g.ps, base + (long long)(i + 2) * sd, wd);
+      lwin(base + (long long)(i + 2) * sd, wd);
 #pragma unroll
       for (int iw = 0; iw < NW; ++iw) ln[iw] = slope<R, LIM>(s.lim[iw], wb[iw], wc[iw], wd[iw]);
-      corr_face<ND, MHD, R, D>(s, u0, u1, lc, ln, dxd[i] + dxd[i + 1], smallp, F, cv);
+      corr_face<ND, MHD, R, D, PRIM>(s, u0, u1, lc, ln, dxd[i] + dxd[i + 1], smallp, F, cv);
       cmx = cmx < cv ? cv : cmx;
       const long long idx = base + (long long)i * sd;
       const double qdx = pdiv(dt, dxd[i]);
       double out[NW];
+      [[maybe_unused]] double ub[NW];
+      if constexpr (PRIM) {
 #pragma unroll
-      for (int iw = 0; iw < NW; ++iw) out[iw] = (own ? wb[iw] : w[iw * g.ps + idx]) - qdx * (F[iw] - Fp[iw]);
+        for (int iw = 0; iw < NW; ++iw) ub[iw] = wb[iw];
+        to_cons<ND, MHD>(ub, s.gamma);
+      }
+#pragma unroll
+      for (int iw = 0; iw < NW; ++iw)
+        out[iw] = (PRIM ? ub[iw] : (own ? wb[iw] : w[iw * g.ps + idx])) - qdx * (F[iw] - Fp[iw]);
       if constexpr (MHD) {
         if (s.sourceunsplit && s.divbfix) {
           c3[D] = i;
@@ -1168,13 +1246,15 @@ struct TvdFace {
   typename RoeOf<ND, R, MHD>::Rt rt;
 };
 
-template <int ND, class R, bool MHD = true, bool ENT = false>
+// PDP: the Roe solver's dp = pR - pL of useprimitive (src/vacphys.mhdroe.t:
+// 42-44) fixed at compile time; the general kernels (ENT) read it from s.
+template <int ND, class R, bool MHD = true, bool ENT = false, bool PDP = false>
 __device__ __forceinline__ void tvd_face(const double* uL, const double* uR, int d, const Sel& s,
                                          double smallp, double dt, double dxsum, TvdFace<ND, R, MHD, ENT>& f,
                                          double& c0, double& c1) {
   constexpr int NW = Layout<ND, MHD>::nw;
   typename RoeOf<ND, R, MHD>::St r;
-  roe_setup(uL, uR, d, s.gamma, smallp, r);
+  roe_setup(uL, uR, d, s.gamma, smallp, r, (PDP || (ENT && s.prim)) ? 1 : 0);
   const R sdx = R(dxsum);
   [[maybe_unused]] R c2L = R(0), c2R = R(0), caL = R(0), caR = R(0);
 #pragma unroll
@@ -1268,18 +1348,14 @@ __device__ __forceinline__ S shfl_up_struct(const S& v) {
 // its two faces and the Powell source of w1 as in the TVDLF corrector.  The
 // two extreme waves' |a dt 2/(dx_J + dx_J+1)| over the faces ixC feed the
 // Courant fold (src/vactvd.t:145-157).
-template <int ND, class R, int D, bool MHD = true, bool ENT = false>
+template <int ND, class R, int D, bool MHD = true, bool ENT = false, bool PRIM = false>
 __device__ __forceinline__ void mu_face(const Sel& s, const double* u0, const double* u1,
                                         const double* ll, const double* lr, double dt,
                                         double smallp, double dxsum, double* F,
                                         TvdFace<ND, R, MHD, ENT>& f, R* phi, double& c0, double& c1) {
   constexpr int NW = Layout<ND, MHD>::nw;
   double uL[NW], uR[NW];
-#pragma unroll
-  for (int iw = 0; iw < NW; ++iw) {
-    uR[iw] = u1[iw] - 0.5 * lr[iw];
-    uL[iw] = u0[iw] + 0.5 * ll[iw];
-  }
+  muscl_states<ND, MHD, PRIM>(s, u0, u1, ll, lr, smallp, uL, uR);
   {
     double fL[NW], fR[NW];
     fluxes<ND, MHD, R>(uL, D, s.gamma, smallp, fL);
@@ -1287,7 +1363,7 @@ __device__ __forceinline__ void mu_face(const Sel& s, const double* u0, const do
 #pragma unroll
     for (int iw = 0; iw < NW; ++iw) F[iw] = 0.5 * (fL[iw] + fR[iw]);
   }
-  tvd_face<ND, R, MHD, ENT>(uL, uR, D, s, smallp, dt, dxsum, f, c0, c1);
+  tvd_face<ND, R, MHD, ENT, PRIM>(uL, uR, D, s, smallp, dt, dxsum, f, c0, c1);
 #pragma unroll
   for (int il = 0; il < NW; ++il) {
     if constexpr (ENT) phi[il] = ent_abs(f.adt[il], f.sa[il], s.ent[il]) * f.jump[il];
@@ -1297,7 +1373,7 @@ __device__ __forceinline__ void mu_face(const Sel& s, const double* u0, const do
 
 // x sweep: lane L owns face c+1/2 of cell c = x0 - 1 + L and updates c for
 // L in [1, 62]; the left face's flux and dissipation term come from lane L-1.
-template <int ND, class R, int LIM, bool MHD = true, bool ENT = false>
+template <int ND, class R, int LIM, bool MHD = true, bool ENT = false, bool PRIM = false>
 __global__ __launch_bounds__(256) void k_tvdmu_x(Geo g, Sel s, BC bc, const double* __restrict__ w,
                                                 const double* __restrict__ w1,
                                                 double* __restrict__ wout,
@@ -1320,21 +1396,38 @@ __global__ __launch_bounds__(256) void k_tvdmu_x(Geo g, Sel s, BC bc, const doub
   const long long idx = g.off + cc + (long long)yc * g.s[1] + (long long)z * g.s[2];
   double l[NW], lr[NW], u0[NW], u1[NW], wc[NW];
   const bool own = wl == w;  // the limiter's base (typelimited) is the sweep's input
+  if constexpr (PRIM) {
+    // useprimitive ('original' only): limit prim(cons(prim(w))) and update
+    // cons of it, the state the predictor's conversion left
+    double qa[NW];
+    load<NW>(wl, g.ps, idx - 1, qa);
+    load<NW>(wl, g.ps, idx, wc);
+    load<NW>(wl, g.ps, idx + 1, u1);
+    prim_of_roundtrip<ND, MHD>(qa, s.gamma, smallp);
+    prim_of_roundtrip<ND, MHD>(wc, s.gamma, smallp);
+    prim_of_roundtrip<ND, MHD>(u1, s.gamma, smallp);
 #pragma unroll
-  for (int iw = 0; iw < NW; ++iw) {
-    const double* p = wl + iw * g.ps + idx;
-    const double pc = p[0];
-    wc[iw] = own ? pc : w[iw * g.ps + idx];
-    l[iw] = slope<R, LIM>(s.lim[iw], p[-1], pc, p[1]);
-    u0[iw] = w1[iw * g.ps + idx];
-    u1[iw] = w1[iw * g.ps + idx + 1];
+    for (int iw = 0; iw < NW; ++iw) l[iw] = slope<R, LIM>(s.lim[iw], qa[iw], wc[iw], u1[iw]);
+    to_cons<ND, MHD>(wc, s.gamma);
+    load<NW>(w1, g.ps, idx, u0);
+    load<NW>(w1, g.ps, idx + 1, u1);
+  } else {
+#pragma unroll
+    for (int iw = 0; iw < NW; ++iw) {
+      const double* p = wl + iw * g.ps + idx;
+      const double pc = p[0];
+      wc[iw] = own ? pc : w[iw * g.ps + idx];
+      l[iw] = slope<R, LIM>(s.lim[iw], p[-1], pc, p[1]);
+      u0[iw] = w1[iw * g.ps + idx];
+      u1[iw] = w1[iw * g.ps + idx + 1];
+    }
   }
 #pragma unroll
   for (int iw = 0; iw < NW; ++iw) lr[iw] = lane_next(l[iw]);
   double F[NW], c0, c1;
   TvdFace<ND, R, MHD, ENT> f;
   R phi[NW];
-  mu_face<ND, R, 0, MHD, ENT>(s, u0, u1, l, lr, dt, smallp, g.dx[0][cc] + g.dx[0][cc + 1], F, f, phi, c0, c1);
+  mu_face<ND, R, 0, MHD, ENT, PRIM>(s, u0, u1, l, lr, dt, smallp, g.dx[0][cc] + g.dx[0][cc + 1], F, f, phi, c0, c1);
   if (!face_ok) { c0 = -INFINITY; c1 = -INFINITY; }
   double out[NW];
   const double qdx = pdiv(dt, g.dx[0][cc]);
@@ -1368,7 +1461,7 @@ __global__ __launch_bounds__(256) void k_tvdmu_x(Geo g, Sel s, BC bc, const doub
 
 // y / z sweep: thread (x, segment) marches through cells s0..s1 along D,
 // carrying the face below (flux, Roe decomposition, phi) and a window of w and w1.
-template <int ND, class R, int D, int LIM, bool MHD = true, bool ENT = false>
+template <int ND, class R, int D, int LIM, bool MHD = true, bool ENT = false, bool PRIM = false>
 __global__ __launch_bounds__(256) void k_tvdmu_m(Geo g, Sel s, BC bc, const double* __restrict__ w,
                                                 const double* __restrict__ w1,
                                                 double* __restrict__ wout,
@@ -1406,24 +1499,30 @@ __global__ __launch_bounds__(256) void k_tvdmu_m(Geo g, Sel s, BC bc, const doub
     // window: the limiter's base wl (typelimited) at J, J+1, w1 at J, slope
     // at J; the face J-1/2 below.  The update's base is w, the window itself
     // when wl is w.
+    // With PRIM (useprimitive, 'original') the window holds
+    // prim(cons(prim(w))), the update's base is cons of it.
     const bool own = wl == w;
+    auto lwin = [&](long long at, double* v) {
+      load<NW>(wl, g.ps, at, v);
+      if constexpr (PRIM) prim_of_roundtrip<ND, MHD>(v, s.gamma, smallp);
+    };
     double wb[NW], wc[NW], u0[NW], lc[NW], Fp[NW];
     TvdFace<ND, R, MHD, ENT> fp;
     R phip[NW];
     {
       double wa[NW], u1[NW], ln[NW], wd[NW];
-      load<NW>(wl, g.ps, base + (long long)(s0 - 2) * sd, wa);
-      load<NW>(wl, g.ps, base + (long long)(s0 - 1) * sd, wb);
-      load<NW>(wl, g.ps, base + (long long)s0 * sd, wc);
+      lwin(base + (long long)(s0 - 2) * sd, wa);
+      lwin(base + (long long)(s0 - 1) * sd, wb);
+      lwin(base + (long long)s0 * sd, wc);
       load<NW>(w1, g.ps, base + (long long)(s0 - 1) * sd, u0);
 #pragma unroll
       for (int iw = 0; iw < NW; ++iw) lc[iw] = slope<R, LIM>(s.lim[iw], wa[iw], wb[iw], wc[iw]);
       // face s0-1/2
       load<NW>(w1, g.ps, base + (long long)s0 * sd, u1);
-      load<NW>(wl, g.ps, base + (long long)(s0 + 1) * sd, wd);
+      lwin(base + (long long)(s0 + 1) * sd, wd);
 #pragma unroll
       for (int iw = 0; iw < NW; ++iw) ln[iw] = slope<R, LIM>(s.lim[iw], wb[iw], wc[iw], wd[iw]);
-      mu_face<ND, R, D, MHD, ENT>(s, u0, u1, lc, ln, dt, smallp, dxd[s0 - 1] + dxd[s0], Fp, fp, phip, a, b);
+      mu_face<ND, R, D, MHD, ENT, PRIM>(s, u0, u1, lc, ln, dt, smallp, dxd[s0 - 1] + dxd[s0], Fp, fp, phip, a, b);
       c0 = c0 < a ? a : c0; c1 = c1 < b ? b : c1;
 #pragma unroll
       for (int iw = 0; iw < NW; ++iw) {
@@ -1439,16 +1538,23 @@ __global__ __launch_bounds__(256) void k_tvdmu_m(Geo g, Sel s, BC bc, const doub
       TvdFace<ND, R, MHD, ENT> f;
       R phi[NW];
       load<NW>(w1, g.ps, base + (long long)(i + 1) * sd, u1);
-      load<NW>(wl, g.ps, base + (long long)(i + 2) * sd, wd);
+      lwin(base + (long long)(i + 2) * sd, wd);
 #pragma unroll
       for (int iw = 0; iw < NW; ++iw) ln[iw] = slope<R, LIM>(s.lim[iw], wb[iw], wc[iw], wd[iw]);
-      mu_face<ND, R, D, MHD, ENT>(s, u0, u1, lc, ln, dt, smallp, dxd[i] + dxd[i + 1], F, f, phi, a, b);
+      mu_face<ND, R, D, MHD, ENT, PRIM>(s, u0, u1, lc, ln, dt, smallp, dxd[i] + dxd[i + 1], F, f, phi, a, b);
       c0 = c0 < a ? a : c0; c1 = c1 < b ? b : c1;
       const long long idx = base + (long long)i * sd;
       const double qdx = pdiv(dt, dxd[i]);
       double out[NW];
+      [[maybe_unused]] double ub[NW];
+      if constexpr (PRIM) {
 #pragma unroll
-      for (int iw = 0; iw < NW; ++iw) out[iw] = (own ? wb[iw] : w[iw * g.ps + idx]) - qdx * (F[iw] - Fp[iw]);
+        for (int iw = 0; iw < NW; ++iw) ub[iw] = wb[iw];
+        to_cons<ND, MHD>(ub, s.gamma);
+      }
+#pragma unroll
+      for (int iw = 0; iw < NW; ++iw)
+        out[iw] = (PRIM ? ub[iw] : (own ? wb[iw] : w[iw * g.ps + idx])) - qdx * (F[iw] - Fp[iw]);
       const double dm = dvol(i - 1), d0 = dvol(i), dp = dvol(i + 1);
       tvd_dissipate<ND, R, MHD>(fp.rt, phip, f.rt, phi, D, s, R(d0 + dm), R(dp + d0), d0, out);
       c3[D] = i;

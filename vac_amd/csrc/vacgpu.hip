@@ -26,6 +26,7 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 using namespace vacgpu;
@@ -217,6 +218,13 @@ int check_method(const vacgpu_config* c, const vacgpu_method* m, std::string& wh
     if (m->typelimiter[iw] < VACGPU_MINMOD || m->typelimiter[iw] > VACGPU_SUPERBEE) { why = "typelimiter must be minmod, woodward or superbee"; return VACGPU_E_ARG; }
   if (c->physics == VACGPU_MHD && m->divbfix && !m->sourceunsplit) { why = "divbfix=T requires sourceunsplit=T (split sources are not on the GPU path)"; return VACGPU_E_ARG; }
   if (!(m->smallpcoeff >= 0.0)) { why = "smallpcoeff must be >= 0"; return VACGPU_E_ARG; }
+  if (m->useprimitive != 0 && m->useprimitive != 1) { why = "useprimitive must be 0 or 1"; return VACGPU_E_ARG; }
+  if (m->useprimitive && m->typefull != VACGPU_TVD) {
+    // upwindLR converts its base in place (src/vactvdlf.t:328-364): with
+    // 'previous' that is wold, converted again on every sweep of a step
+    if (m->typelimited != VACGPU_LIM_ORIGINAL) { why = "useprimitive=T with typelimited other than 'original' is not on the GPU path"; return VACGPU_E_ARG; }
+    if (c->precision != VACGPU_FP64) { why = "useprimitive=T with TVDLF / TVD-MUSCL runs in FP64 only"; return VACGPU_E_ARG; }
+  }
   return 0;
 }
 
@@ -232,6 +240,7 @@ Sel make_sel(const vacgpu_grid* g, const vacgpu_method* m) {
     s.ent[i] = m->typefull == VACGPU_TVDLF ? ENT_NUL : m->typeentropy[i];
     s.ecoef[i] = m->entropycoef[i];
   }
+  s.prim = m->useprimitive;
   return s;
 }
 
@@ -439,7 +448,7 @@ int sweep_device(vacgpu_grid* g, const vacgpu_method* m, int d, int firstsweep, 
                      : (m->typelimited == VACGPU_LIM_PREDICTOR ? g->w1 : win);
   const bool fused = m->typefull == VACGPU_TVDLF && g->cfg.ndim == 2 && g->fused_ok &&
                      g->use_fused && one_lim && (!mhd || src) && !dirty &&
-                     m->typelimited == VACGPU_LIM_ORIGINAL;
+                     m->typelimited == VACGPU_LIM_ORIGINAL && !m->useprimitive;
   if (fused) {
     const long long nbf = (long long)g->grid_fs.x * g->grid_fs.y;
     {
@@ -471,6 +480,16 @@ int sweep_device(vacgpu_grid* g, const vacgpu_method* m, int d, int firstsweep, 
     const int ghosts = (g->bc_uniform && g->use_fused) ? 1 : 0;
     {
       Timed t(g, KF_PRED);
+      if (m->useprimitive) {
+        // primitive-variable limiting (FP64 only, checked with the method)
+        DISPATCH(g, {
+          if constexpr (std::is_same<R, double>::value) {
+            if (d == 0) hipLaunchKernelGGL((k_tvdlf_pred<ND, MHD, R, 0, true>), g->grid, g->block, 0, g->stream, g->geo, s, g->bc, win, g->w1, g->sc, ghosts);
+            else if (d == 1) hipLaunchKernelGGL((k_tvdlf_pred<ND, MHD, R, 1, true>), g->grid, g->block, 0, g->stream, g->geo, s, g->bc, win, g->w1, g->sc, ghosts);
+            else hipLaunchKernelGGL((k_tvdlf_pred<ND, MHD, R, (ND == 3 ? 2 : 1), true>), g->grid, g->block, 0, g->stream, g->geo, s, g->bc, win, g->w1, g->sc, ghosts);
+          }
+        });
+      } else
       DISPATCH(g, {
         if (d == 0) hipLaunchKernelGGL((k_tvdlf_pred<ND, MHD, R, 0>), g->grid, g->block, 0, g->stream, g->geo, s, g->bc, win, g->w1, g->sc, ghosts);
         else if (d == 1) hipLaunchKernelGGL((k_tvdlf_pred<ND, MHD, R, 1>), g->grid, g->block, 0, g->stream, g->geo, s, g->bc, win, g->w1, g->sc, ghosts);
@@ -498,7 +517,14 @@ int sweep_device(vacgpu_grid* g, const vacgpu_method* m, int d, int firstsweep, 
 #define X2F(LV) MUX(2, float, LV)
 #define X3D(LV) MUX(3, double, LV)
 #define X3F(LV) MUX(3, float, LV)
-        if (g->cfg.ndim == 2) { if (f32) LIMSW(X2F); else LIMSW(X2D); }
+        if (m->useprimitive) {
+          DISPATCH(g, {
+            if constexpr (std::is_same<R, double>::value) {
+              if (ent) hipLaunchKernelGGL((k_tvdmu_x<ND, R, -1, MHD, true, true>), gr, g->block, 0, g->stream, g->geo, s, g->bc, win, g->w1, wout, g->sc, g->part, g->pstride, ghosts, wl);
+              else hipLaunchKernelGGL((k_tvdmu_x<ND, R, -1, MHD, false, true>), gr, g->block, 0, g->stream, g->geo, s, g->bc, win, g->w1, wout, g->sc, g->part, g->pstride, ghosts, wl);
+            }
+          });
+        } else if (g->cfg.ndim == 2) { if (f32) LIMSW(X2F); else LIMSW(X2D); }
         else { if (f32) LIMSW(X3F); else LIMSW(X3D); }
 #undef X2D
 #undef X2F
@@ -518,7 +544,16 @@ int sweep_device(vacgpu_grid* g, const vacgpu_method* m, int d, int firstsweep, 
 #define M31F(LV) MUM(3, float, 1, LV)
 #define M32D(LV) MUM(3, double, 2, LV)
 #define M32F(LV) MUM(3, float, 2, LV)
-        if (g->cfg.ndim == 2) { if (f32) LIMSW(M21F); else LIMSW(M21D); }
+        if (m->useprimitive) {
+#define MUP(DV) do { if (ent) hipLaunchKernelGGL((k_tvdmu_m<ND, R, DV, -1, MHD, true, true>), gr, g->block, 0, g->stream, g->geo, s, g->bc, win, g->w1, wout, g->sc, g->part, g->pstride, g->seglen, ghosts, wl); \
+                     else hipLaunchKernelGGL((k_tvdmu_m<ND, R, DV, -1, MHD, false, true>), gr, g->block, 0, g->stream, g->geo, s, g->bc, win, g->w1, wout, g->sc, g->part, g->pstride, g->seglen, ghosts, wl); } while (0)
+          DISPATCH(g, {
+            if constexpr (std::is_same<R, double>::value) {
+              if (d == 1) MUP(1); else MUP((ND == 3 ? 2 : 1));
+            }
+          });
+#undef MUP
+        } else if (g->cfg.ndim == 2) { if (f32) LIMSW(M21F); else LIMSW(M21D); }
         else if (d == 1) { if (f32) LIMSW(M31F); else LIMSW(M31D); }
         else { if (f32) LIMSW(M32F); else LIMSW(M32D); }
 #undef M21D
@@ -549,6 +584,16 @@ int sweep_device(vacgpu_grid* g, const vacgpu_method* m, int d, int firstsweep, 
     const int ghosts = (g->bc_uniform && g->use_fused) ? 1 : 0;
     {
       Timed t(g, KF_PRED);
+      if (m->useprimitive) {
+        // primitive-variable limiting (FP64 only, checked with the method)
+        DISPATCH(g, {
+          if constexpr (std::is_same<R, double>::value) {
+            if (d == 0) hipLaunchKernelGGL((k_tvdlf_pred<ND, MHD, R, 0, true>), g->grid, g->block, 0, g->stream, g->geo, s, g->bc, win, g->w1, g->sc, ghosts);
+            else if (d == 1) hipLaunchKernelGGL((k_tvdlf_pred<ND, MHD, R, 1, true>), g->grid, g->block, 0, g->stream, g->geo, s, g->bc, win, g->w1, g->sc, ghosts);
+            else hipLaunchKernelGGL((k_tvdlf_pred<ND, MHD, R, (ND == 3 ? 2 : 1), true>), g->grid, g->block, 0, g->stream, g->geo, s, g->bc, win, g->w1, g->sc, ghosts);
+          }
+        });
+      } else
       DISPATCH(g, {
         if (d == 0) hipLaunchKernelGGL((k_tvdlf_pred<ND, MHD, R, 0>), g->grid, g->block, 0, g->stream, g->geo, s, g->bc, win, g->w1, g->sc, ghosts);
         else if (d == 1) hipLaunchKernelGGL((k_tvdlf_pred<ND, MHD, R, 1>), g->grid, g->block, 0, g->stream, g->geo, s, g->bc, win, g->w1, g->sc, ghosts);
@@ -567,7 +612,13 @@ int sweep_device(vacgpu_grid* g, const vacgpu_method* m, int d, int firstsweep, 
         const dim3 gr((g->geo.n[0] + CX_STRIDE - 1) / CX_STRIDE, g->grid.y, g->grid.z);
         nbc = (long long)gr.x * gr.y * gr.z;
 #define CX(LV) hipLaunchKernelGGL((k_tvdlf_corrx<ND, MHD, R, LV>), gr, g->block, 0, g->stream, g->geo, s, g->bc, win, g->w1, wout, g->sc, g->part, g->pstride, ghosts, wl)
-        DISPATCH(g, { LIMSW(CX); });
+        if (m->useprimitive)
+          DISPATCH(g, {
+            if constexpr (std::is_same<R, double>::value)
+              hipLaunchKernelGGL((k_tvdlf_corrx<ND, MHD, R, -1, true>), gr, g->block, 0, g->stream, g->geo, s, g->bc, win, g->w1, wout, g->sc, g->part, g->pstride, ghosts, wl);
+          });
+        else
+          DISPATCH(g, { LIMSW(CX); });
 #undef CX
       } else {
         const dim3 gr = g->grid_tm[d];
@@ -575,7 +626,16 @@ int sweep_device(vacgpu_grid* g, const vacgpu_method* m, int d, int firstsweep, 
 #define CM(DV, LV) hipLaunchKernelGGL((k_tvdlf_corrm<ND, MHD, R, DV, LV>), gr, g->block, 0, g->stream, g->geo, s, g->bc, win, g->w1, wout, g->sc, g->part, g->pstride, g->seglen, ghosts, wl)
 #define CM1(LV) CM(1, LV)
 #define CM2(LV) CM((ND == 3 ? 2 : 1), LV)
-        DISPATCH(g, { if (d == 1) LIMSW(CM1); else LIMSW(CM2); });
+#define CMP(DV) hipLaunchKernelGGL((k_tvdlf_corrm<ND, MHD, R, DV, -1, true>), gr, g->block, 0, g->stream, g->geo, s, g->bc, win, g->w1, wout, g->sc, g->part, g->pstride, g->seglen, ghosts, wl)
+        if (m->useprimitive)
+          DISPATCH(g, {
+            if constexpr (std::is_same<R, double>::value) {
+              if (d == 1) CMP(1); else CMP((ND == 3 ? 2 : 1));
+            }
+          });
+        else
+          DISPATCH(g, { if (d == 1) LIMSW(CM1); else LIMSW(CM2); });
+#undef CMP
 #undef CM2
 #undef CM1
 #undef CM
@@ -599,7 +659,9 @@ int sweep_device(vacgpu_grid* g, const vacgpu_method* m, int d, int firstsweep, 
     // first sweep and the first half of the Ryu step into the last one
     // (one limiter for all characteristic waves is fixed at compile time)
     const int lim = common_limiter(g, m);
-    const bool ent = entropy_on(g, m);
+    // entropy fixes, and useprimitive's dp for the MHD Roe solver, run in
+    // the general kernels
+    const bool ent = entropy_on(g, m) || (m->useprimitive && mhd);
     const bool fold = g->cfg.ndim == 2 && g->bc_uniform && src && g->use_fused && lim >= 0 && !dirty &&
                       !ent;
     // (the Ryu half folds into the x sweep only: in the marching y sweep it

@@ -40,7 +40,6 @@ if(ix^L/=ixG^L|.or.|.or.) call die('vacgpu advect: subgrids are not supported')
 if(idimmin/=idimmax) call die('vacgpu advect: needs dimsplit=T')
 if(nproc(2)/=0) call die('vacgpu advect: nproc(2) processing is not on the GPU path')
 if(typeaxial/='slab' .or. gencoord) call die('vacgpu advect: slab Cartesian grids only')
-if(useprimitive) call die('vacgpu advect: useprimitive=T is not on the GPU path')
 if(smallfix .or. extraB) call die('vacgpu advect: smallfix/extraB are not on the GPU path')
 if(acmexpo>zero) call die('vacgpu advect: acmexpo>0 is not on the GPU path')
 if(ndir/=ndim) call die('vacgpu advect: ndir must equal ndim')
@@ -78,6 +77,7 @@ do iw=1,nw
    if(code<0) call die('vacgpu advect: typelimiter='//typelimiter(iw)//' is not on the GPU path')
    m%typelimiter(iw)=code
 end do
+m%useprimitive=merge(1,0,useprimitive)
 m%divbfix=merge(1,0,divbfix)
 m%divbwave=merge(1,0,divbwave)
 m%sourceunsplit=merge(1,0,sourceunsplit)

@@ -50,6 +50,7 @@ module vacgpu_mod
      integer(c_int) :: typelimited
      integer(c_int) :: typeentropy(8)
      real(c_double) :: entropycoef(8)
+     integer(c_int) :: useprimitive
   end type vacgpu_method
 
   interface

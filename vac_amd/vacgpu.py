@@ -41,7 +41,7 @@ class Method(C.Structure):
                 ("divbwave", C.c_int), ("sourceunsplit", C.c_int), ("dimsplit_xy", C.c_int),
                 ("courantpar", C.c_double), ("smallpcoeff", C.c_double),
                 ("typelimited", C.c_int), ("typeentropy", C.c_int * 8),
-                ("entropycoef", C.c_double * 8)]
+                ("entropycoef", C.c_double * 8), ("useprimitive", C.c_int)]
 
 
 _lib = None
@@ -133,6 +133,7 @@ class MethodSpec:
     typelimited: str = "original"       # or "previous", "predictor"
     typeentropy: object = "nul"         # one name, or a list per wave
     entropycoef: object = None          # per wave; None: VAC's defaults
+    useprimitive: bool = False          # limit primitive variables (FP64)
 
     def to_c(self, physics: str = "mhd") -> Method:
         m = Method()
@@ -162,6 +163,7 @@ class MethodSpec:
         coef = list(coef) + [-1.0] * 8
         for i in range(8):
             m.entropycoef[i] = float(coef[i])
+        m.useprimitive = int(self.useprimitive)
         return m
 
 
