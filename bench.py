@@ -10,7 +10,8 @@ sweep per direction, each with the Roe dissipation and the Powell / Ryu
 source terms).  Metric: Mcell-updates/s = nx*ny*steps / seconds / 1e6.
 
     python bench.py [--gpus 1] [--steps 200] [--warmup 5] [--no-cpu] [--precision fp64]
-                    [--no-tvdlf] [--no-3d] [--no-snapshots] [--no-batched] [--no-serving]
+                    [--no-tvdlf] [--no-tvdmu] [--no-unsplit] [--no-projection] [--no-3d]
+                    [--no-snapshots] [--no-batched] [--no-serving]
 
 Prints one JSON line.  Beside the headline it reports the batched workload
 (512 independent 2D MHD grids, sides log-uniform in [64, 512], TVDLF +
@@ -48,7 +49,7 @@ NW = 6
 # other step) the first half of the Ryu step (read wold, read w, write wt:
 # 3*nw*8), so per launch on average 6*nw*8 + 2*nw*8/2 + 3*nw*8/4
 B_ALG_TVD_SWEEP = 6 * NW * 8 + (2 * NW * 8) // 2 + (3 * NW * 8) // 4
-# a whole TVD step (SURVEY-style count: wold copy 2nw, half source 2nw,
+# a whole TVD step (every array pass counted: wold copy 2nw, half source 2nw,
 # 6nw per sweep, Ryu 6nw) = (6*ndim + 10)*nw*8
 B_ALG_TVD_STEP = (6 * 2 + 10) * NW * 8
 HBM_PEAK_GBS = 8000.0
@@ -108,6 +109,7 @@ def main():
                          "and TVDLF workloads; prints no JSON")
     ap.add_argument("--no-tvdlf", action="store_true")
     ap.add_argument("--no-tvdmu", action="store_true")
+    ap.add_argument("--no-unsplit", action="store_true")
     ap.add_argument("--no-projection", action="store_true")
     ap.add_argument("--no-3d", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
@@ -116,7 +118,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
-              "(independent grids shard by instance; see DESIGN.md)", file=sys.stderr)
+              "(independent grids shard by instance)", file=sys.stderr)
         sys.exit(2)
 
     from vac_amd import vacgpu
@@ -193,6 +195,8 @@ def main():
         out["tvdlf_2048"] = tvdlf_2048(w0, dxs, min(args.steps, 100))
     if not args.no_tvdmu:
         out["tvdmu_2048"] = tvdmu_2048(w0, dxs, min(args.steps, 50))
+    if not args.no_unsplit:
+        out["tvdlf_unsplit_2048"] = tvdlf_unsplit_2048(w0, dxs, min(args.steps, 50))
     if not args.no_projection:
         out["ot_projection_2048"] = ot_projection_2048()
     if not args.no_3d:
@@ -316,6 +320,24 @@ def tvdmu_2048(w0, dxs, steps):
             "corrector_effective_gbs": round(144 * NX * NY / (kc * 1e-3) / 1e9, 1) if kc else None,
             "workload": (f"2D MHD 2048x2048 periodic random-Fourier, TVD-MUSCL+woodward, "
                          f"courantpar 0.8, {steps} steps; B_alg 480 B/cell-update")}
+
+
+def tvdlf_unsplit_2048(w0, dxs, steps):
+    """2D MHD 2048^2, TVDLF + minmod with dimsplit=F (one advect over both
+    directions per step; the limiter reads the step's start), FP64.  VAC's
+    dataflow per step: the predictor reads w and writes w1, the corrector
+    reads w1, w and wold and writes w: 6*nw*8 = 288 B per cell-update."""
+    from vac_amd import vacgpu
+
+    method = vacgpu.MethodSpec(typefull="tvdlf", limiter="minmod", courantpar=0.8, unsplit=True)
+    el, w, kt = timed_run([NX, NY], "mhd", dxs, w0, method, "fp64", steps,
+                          ("tvdlf_pred", "tvdlf_corr", "bound", "reduce"))
+    rate = NX * NY * steps / el
+    return {"mcell_updates_per_s": round(rate / 1e6, 2), "ms_per_step": round(el / steps * 1e3, 4),
+            "effective_hbm_frac": round(288 * rate / 1e9 / HBM_PEAK_GBS, 4),
+            "kernel_avg_ms": kt,
+            "workload": (f"2D MHD 2048x2048 periodic random-Fourier, TVDLF+minmod, dimsplit=F, "
+                         f"courantpar 0.8, {steps} steps; B_alg 288 B/cell-update")}
 
 
 def ot_projection_2048(steps=20):

@@ -79,6 +79,9 @@ typedef struct {
                            typelimited 'original'), and the MHD Roe solver of
                            TVD / TVD-MUSCL takes dp = pR - pL
                            (src/vacphys.mhdroe.t:42-44) */
+  int unsplit;          /* 1: dimsplit=F, one advect over idim = 1..ndim per
+                           step (src/vac.t:325-329), TVDLF only; 'original'
+                           limiting reads as 'previous' (src/vacio.t:363-364) */
 } vacgpu_method;
 
 /* Replaces the geometry hand-off after gridsetup1 (src/vacgrid.t:457-593):

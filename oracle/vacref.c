@@ -400,7 +400,7 @@ static double ldw_at(ctx_t* x, const double* w, int iw, int d, long idx) {
 }
 
 /* hancock (src/vactvdlf.t:5-89): predictor wnew += -qdt/dx*(F(wL)-F(wR)) */
-static void hancock(ctx_t* x, double qdt, int d, const double* wCT, double* wnew) {
+static void hancock(ctx_t* x, double qdt, int d, const double* wCT, double* wnew, int srcmul) {
   const int* lo = x->mlo;
   const int* hi = x->mhi;
   double* wL = x->wLC;  /* wLC(ixO) */
@@ -453,7 +453,8 @@ static void hancock(ctx_t* x, double qdt, int d, const double* wCT, double* wnew
       wn[idx] = wn[idx] - qdt / x->dx[d][CRD(d)] * (fL[idx] - fR[idx]);
     }
   }
-  if (x->p->sourceunsplit) addsource(x, qdt * 1.0 / x->ndim, 1, wCT, wnew);
+  /* after the last direction: qdt*(idimmax-idimmin+1)/ndim (src/vactvdlf.t:85-86) */
+  if (x->p->sourceunsplit && srcmul) addsource(x, qdt * (double)srcmul / x->ndim, 1, wCT, wnew);
 }
 
 static void tvdlimit2(ctx_t* x, double qdt, int d, const double* wL, const double* wR,
@@ -465,7 +466,8 @@ static void tvdlimit2(ctx_t* x, double qdt, int d, const double* wL, const doubl
  * Friedrichs dissipation (and no cmax Courant check): the central flux is
  * added, then tvdlimit2 adds the Roe dissipation of the reconstructed face
  * states (src/vactvdlf.t:279-283). */
-static void tvdlf_corrector(ctx_t* x, double qdt, int d, const double* wCT, double* wnew, int mu) {
+static void tvdlf_corrector(ctx_t* x, double qdt, int d, const double* wCT, double* wnew, int mu,
+                            int srcmul) {
   int lo[3], hi[3];
   for (int dd = 0; dd < 3; ++dd) { lo[dd] = x->mlo[dd]; hi[dd] = x->mhi[dd]; }
   lo[d] -= 1;  /* ixC = [ixOmin-1, ixOmax] */
@@ -473,7 +475,9 @@ static void tvdlf_corrector(ctx_t* x, double qdt, int d, const double* wCT, doub
   double* wL = x->wLC;
   double* wR = x->wRC;
   /* the limiter's base, typelimited (src/vactvdlf.t:150-161) */
-  const double* wlim = x->p->typelimited == 1 ? x->wold : (x->p->typelimited == 2 ? wCT : wnew);
+  /* unsplit sweeps limit 'previous' where 'original' is asked (src/vacio.t:363-364) */
+  const int tl = (x->p->unsplit && x->p->typelimited == 0) ? 1 : x->p->typelimited;
+  const double* wlim = tl == 1 ? x->wold : (tl == 2 ? wCT : wnew);
   if (x->p->useprimitive) {
     /* upwindLR with useprimitive: the copies of wCT and the limiter's base
      * (over ixG, in place and back) in primitive variables */
@@ -562,8 +566,10 @@ static void tvdlf_corrector(ctx_t* x, double qdt, int d, const double* wCT, doub
     }
   }
   if (mu) tvdlimit2(x, qdt, d, wL, wR, wnew, 1);
-  /* ixI = ixO expanded by 2 in idim only, so ensurebound refills wCT ghosts */
-  if (x->p->sourceunsplit) addsource(x, qdt * 1.0 / x->ndim, x->ndim == 1, wCT, wnew);
+  /* ixI = ixO expanded by 2 in the idim range: for dimensionally split
+   * sweeps in idim only, so ensurebound refills wCT ghosts */
+  if (x->p->sourceunsplit && srcmul)
+    addsource(x, qdt * (double)srcmul / x->ndim, x->ndim == 1 || x->p->unsplit, wCT, wnew);
 }
 
 /* ------------------------------------------------------------ MHD Roe */
@@ -1465,19 +1471,23 @@ static int projectb(ctx_t* x, double* w) {
 
 /* -------------------------------------------------------- time stepping */
 /* advect1 (src/vac.t:489-572) */
-static void advect1(ctx_t* x, int method, double qdt, int d, double* wCT, double* w) {
+static void advect1(ctx_t* x, int method, double qdt, int d0, int d1, double* wCT, double* w) {
   x->istep += 1;
-  switch (method) {
-    case 0: hancock(x, qdt, d, wCT, w); break;           /* 'hancock' */
-    case 1: tvdlf_corrector(x, qdt, d, wCT, w, 0); break;   /* 'tvdlf' */
-    case 3: tvdlf_corrector(x, qdt, d, wCT, w, 1); break;   /* 'tvdmu' */
-    case 2: centdiff(x, qdt, d, wCT, w); tvdlimit(x, qdt, d, wCT, w); break;  /* 'tvd' */
+  /* the directions d0..d1 in turn, the unsplit source after the last */
+  for (int d = d0; d <= d1; ++d) {
+    const int sm = d == d1 ? d1 - d0 + 1 : 0;
+    switch (method) {
+      case 0: hancock(x, qdt, d, wCT, w, sm); break;              /* 'hancock' */
+      case 1: tvdlf_corrector(x, qdt, d, wCT, w, 0, sm); break;   /* 'tvdlf' */
+      case 3: tvdlf_corrector(x, qdt, d, wCT, w, 1, sm); break;   /* 'tvdmu' */
+      case 2: centdiff(x, qdt, d, wCT, w); tvdlimit(x, qdt, d, wCT, w); break;  /* 'tvd' */
+    }
   }
   getboundary(x, w);
 }
 
 /* advect (src/vac.t:363-486) */
-static int advect(ctx_t* x, int d, double* w) {
+static int advect(ctx_t* x, int d0, int d1, double* w) {
   /* process(1) when due, :392-399 */
   const int np = x->p->nproc1;
   if (np != 0) {
@@ -1491,10 +1501,10 @@ static int advect(ctx_t* x, int d, double* w) {
   x->istep = 0;
   double dt = x->s->dt;
   if (x->p->typefull == VACREF_TVDLF || x->p->typefull == VACREF_TVDMU) {
-    advect1(x, 0, dt / 2, d, w, x->w1);
-    advect1(x, x->p->typefull == VACREF_TVDMU ? 3 : 1, dt, d, x->w1, w);
+    advect1(x, 0, dt / 2, d0, d1, w, x->w1);
+    advect1(x, x->p->typefull == VACREF_TVDMU ? 3 : 1, dt, d0, d1, x->w1, w);
   } else {
-    advect1(x, 2, dt, d, x->w1, w);
+    advect1(x, 2, dt, d0, d1, x->w1, w);
   }
   x->firstsweep = 0;
   return 0;
@@ -1504,10 +1514,15 @@ static int advect(ctx_t* x, int d, double* w) {
 static int advance(ctx_t* x, double* w) {
   x->firstsweep = 1;
   int it = x->s->it, rc = 0;
+  if (x->p->unsplit) {
+    /* fluxes from all directions at once (src/vac.t:325-329) */
+    x->lastsweep = 1;
+    return advect(x, 0, x->ndim - 1, w);
+  }
   if ((it / 2) * 2 == it || x->p->dimsplit_xy) {
-    for (int d = 0; d < x->ndim && !rc; ++d) { x->lastsweep = (d == x->ndim - 1); rc = advect(x, d, w); }
+    for (int d = 0; d < x->ndim && !rc; ++d) { x->lastsweep = (d == x->ndim - 1); rc = advect(x, d, d, w); }
   } else {
-    for (int d = x->ndim - 1; d >= 0 && !rc; --d) { x->lastsweep = (d == 0); rc = advect(x, d, w); }
+    for (int d = x->ndim - 1; d >= 0 && !rc; --d) { x->lastsweep = (d == 0); rc = advect(x, d, d, w); }
   }
   return rc;
 }
@@ -1644,6 +1659,9 @@ int vacref_run(const vacref_params* p, vacref_state* s, double* w,
       if (p->typeentropy[il] == VACREF_ENT_RATIO) { ctx_free(&x); return -7; }
   /* TVD with typelimited='predictor' limits the centdiff result (src/vactvd.t:30-36) */
   if (p->typefull == VACREF_TVD && p->typelimited == 2) { ctx_free(&x); return -6; }
+  /* unsplit sweeps are restated for TVDLF only (one-step TVD unsplit is a
+   * warning in VAC, src/vacio.t:508-509) */
+  if (p->typefull != VACREF_TVDLF && p->unsplit) { ctx_free(&x); return -8; }
   getboundary(&x, w);
   int done = 0;
   for (;;) {

@@ -57,6 +57,10 @@ typedef struct {
    * 328-332, :360-364), converting its base state in place and back; the MHD
    * Roe average takes dp = pR - pL (src/vacphys.mhdroe.t:42-44) */
   int useprimitive;
+  /* unsplit: dimsplit=F, one advect over all directions per step
+   * (src/vac.t:325-329) with 'original' limiting read as 'previous'
+   * (src/vacio.t:363-364); TVDLF only */
+  int unsplit;
 } vacref_params;
 
 enum { VACREF_ENT_NUL = 0, VACREF_ENT_HARTEN = 1, VACREF_ENT_POWELL = 2, VACREF_ENT_RATIO = 3,

@@ -32,7 +32,7 @@ class Params(C.Structure):
         ("dtpar", C.c_double), ("tmax", C.c_double), ("tmaxexact", C.c_int),
         ("itmax", C.c_int), ("nproc1", C.c_int), ("procpar", C.c_double * 3),
         ("typelimited", C.c_int), ("typeentropy", C.c_int * 8), ("entropycoef", C.c_double * 8),
-        ("useprimitive", C.c_int),
+        ("useprimitive", C.c_int), ("unsplit", C.c_int),
     ]
 
 
@@ -87,7 +87,7 @@ def make_params(ndim, physics, nx, gamma, typefull="tvdlf", limiter="minmod",
                 dimsplit_xy=False, courantpar=0.8, smallpcoeff=1e-6, dtpar=-1.0,
                 tmax=1e99, tmaxexact=False, itmax=1000000, nproc1=0,
                 procpar=(-1.0, -1.0, -1.0), typelimited="original", typeentropy="nul",
-                entropycoef=None, useprimitive=False) -> Params:
+                entropycoef=None, useprimitive=False, unsplit=False) -> Params:
     p = Params()
     p.ndim = ndim
     p.phys = MHD if physics == "mhd" else HD
@@ -124,6 +124,7 @@ def make_params(ndim, physics, nx, gamma, typefull="tvdlf", limiter="minmod",
     for i in range(8):
         p.entropycoef[i] = float(coef[i])
     p.useprimitive = int(useprimitive)
+    p.unsplit = int(unsplit)
     p.nproc1 = int(nproc1)
     for i in range(3):
         p.procpar[i] = float(procpar[i])

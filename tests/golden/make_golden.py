@@ -138,6 +138,17 @@ PRIM_CASES = {
 }
 
 
+# dimsplit=F: one advect over all directions per step; VAC reads
+# typelimited='original' as 'previous' then
+# name: (binary, physics, builder, gamma, steps, limiter, bc, typelimited)
+UNSPLIT_CASES = {
+    "mhd22_rf32x28_tvdlf_minmod_unsplit": ("vac_mhd22_132", "mhd", lambda g: random_fourier((32, 28), 901, g), 5 / 3, 12, "minmod", "periodic", "original"),
+    "mhd22_rf28x30_tvdlf_woodward_cont_unsplit_predictor": ("vac_mhd22_132", "mhd", lambda g: random_fourier((28, 30), 902, g), 1.4, 12, "woodward", "cont", "predictor"),
+    "hd22_rf30x26_tvdlf_superbee_unsplit": ("vac_hd22_132", "hd", lambda g: random_fourier((30, 26), 903, g, physics="hd"), 1.4, 15, "superbee", "periodic", "original"),
+    "mhd33_rf10x12x8_tvdlf_minmod_unsplit": ("vac_mhd33_36", "mhd", lambda g: random_fourier((10, 12, 8), 904, g), 5 / 3, 6, "minmod", "periodic", "original"),
+}
+
+
 def ensure_binary(name: str) -> None:
     if have_ref(name):
         return
@@ -269,6 +280,24 @@ def main() -> None:
             nx=np.array([len(c) for c in coords]), gamma=gamma, steps=steps,
             physics=physics, typefull=typefull, limiter=limiter, bc=bc, divbfix=True,
             useprimitive=True)
+        print("wrote", name, w0.shape, "steps", len(t) - 1)
+    for name, (binary, physics, build, gamma, steps, limiter, bc, tl) in UNSPLIT_CASES.items():
+        if only and not any(o in name for o in only):
+            continue
+        ensure_binary(binary)
+        w0, coords = build(gamma)
+        nw = w0.shape[0]
+        ini = make_ini(w0, coords, physics, gamma, name="golden")
+        par = problem_par(nw, NAMES[nw], steps, typefull="tvdlf", limiter=limiter,
+                          typeB=[bc] * (2 * len(coords)), every_step=True,
+                          extra_method=f" dimsplit=F\n typelimited='{tl}'\n")
+        r = run_ref(binary, ini, par)
+        t = np.array([s_.t for s_ in r.snapshots])
+        np.savez_compressed(
+            os.path.join(HERE, name + ".npz"), w0=w0, wf=r.snapshots[-1].w, t=t,
+            nx=np.array([len(c) for c in coords]), gamma=gamma, steps=steps,
+            physics=physics, typefull="tvdlf", limiter=limiter, bc=bc, divbfix=True,
+            typelimited=tl, unsplit=True)
         print("wrote", name, w0.shape, "steps", len(t) - 1)
     if only:
         return

@@ -56,6 +56,11 @@ CASES = [
      "tvdmu", "minmod", "cont", True, "", "", True),
     ("vac_mhd33_36", "vacgpu_mhd33_36", "mhd", lambda g: random_fourier((12, 10, 14), 76, g), 5 / 3, 6,
      "tvdlf", "minmod", "periodic", True, "", "", True),
+    # dimsplit=F: one advect over all directions per step
+    ("vac_mhd22_132", "vacgpu_mhd22_132", "mhd", lambda g: random_fourier((40, 36), 77, g), 5 / 3, 15,
+     "tvdlf", "woodward", "periodic", True, "", "", False, True),
+    ("vac_mhd33_36", "vacgpu_mhd33_36", "mhd", lambda g: random_fourier((12, 10, 14), 78, g), 5 / 3, 6,
+     "tvdlf", "minmod", "cont", True, "predictor", "", False, True),
 ]
 
 
@@ -64,6 +69,7 @@ CASES = [
                                              + (f"-{c[10]}" if len(c) > 10 and c[10] else "")
                                              + (f"-{c[11]}" if len(c) > 11 and c[11] else "")
                                              + ("-prim" if len(c) > 12 and c[12] else "")
+                                             + ("-unsplit" if len(c) > 13 and c[13] else "")
                                              for c in CASES])
 def test_vac_with_gpu_advect_matches_vac(case):
     cpu, gpu, physics, build, gamma, steps, typefull, limiter, bc, divbfix = case[:10]
@@ -72,6 +78,8 @@ def test_vac_with_gpu_advect_matches_vac(case):
         extra += f" typeentropy={len(NAMES[6 if physics == 'mhd' else 4].split())}*'{case[11]}'\n"
     if len(case) > 12 and case[12]:
         extra += " useprimitive=T\n"
+    if len(case) > 13 and case[13]:
+        extra += " dimsplit=F\n"
     if not (have_ref(cpu) and have_ref(gpu)):
         pytest.skip("reference executables not built (oracle/build_ref.sh)")
     w0, coords = build(gamma)

@@ -64,9 +64,10 @@ def test_gpu_matches_reference_fixture(path):
     tl = str(z["typelimited"]) if "typelimited" in z.files else "original"
     te = str(z["typeentropy"]) if "typeentropy" in z.files else "nul"
     up = bool(z["useprimitive"]) if "useprimitive" in z.files else False
+    us = bool(z["unsplit"]) if "unsplit" in z.files else False
     method = vacgpu.MethodSpec(typefull=str(z["typefull"]), limiter=str(z["limiter"]),
                                divbfix=bool(z["divbfix"]), typelimited=tl, typeentropy=te,
-                               useprimitive=up)
+                               useprimitive=up, unsplit=us)
     print(f"case {os.path.basename(path)} nx={nx}", flush=True)
     w, t, sc = gpu_run(z["w0"], nx, str(z["physics"]), float(z["gamma"]), int(z["steps"]),
                        method, bc=vacref.bc_spec(z["bc"]))
@@ -287,3 +288,61 @@ def test_gpu_useprimitive_rejections():
         with pytest.raises(vacgpu.VacGpuError):
             g.run(vacgpu.MethodSpec(useprimitive=True, **kw), 1)
         g.close()
+
+
+@pytest.mark.parametrize("case", [
+    ("mhd", (64, 56), "woodward", "periodic", "original", 10),
+    ("mhd", (48, 60), "superbee", "cont", "predictor", 10),
+    ("mhd", (56, 48), "minmod", [WALL_X, WALL_X, "periodic", "periodic"], "original", 10),
+    ("mhd", (14, 12, 16), "minmod", "periodic", "original", 5),
+    ("mhd", (12, 16, 14), "woodward", "cont", "predictor", 5),
+    ("hd", (64, 48), "minmod", "cont", "original", 15),
+    ("hd", (12, 10, 14), "superbee", "periodic", "original", 6),
+])
+def test_gpu_unsplit_matches_oracle(case):
+    # dimsplit=F: the predictor and the corrector add all directions' fluxes
+    # in one pass; 'original' limiting reads as 'previous'
+    physics, nx, limiter, bc, tl, steps = case
+    gamma = 5 / 3
+    w0, coords = random_fourier(nx, 51 + len(nx), gamma, physics=physics)
+    method = vacgpu.MethodSpec(typefull="tvdlf", limiter=limiter, typelimited=tl, unsplit=True)
+    print(f"case {case}", flush=True)
+    w, t, sc = gpu_run(w0, list(nx), physics, gamma, steps, method, bc=bc)
+    p = vacref.make_params(len(nx), physics, list(nx), gamma, typefull="tvdlf",
+                           limiter=limiter, bc=bc, typelimited=tl, unsplit=True)
+    wr, tr, st = vacref.run(p, w0, [widths_1d(c) for c in coords], steps)
+    assert np.array_equal(t, tr)
+    assert_bitwise(w, wr, str(case))
+    assert list(sc["nerror"]) == list(st.nerror)
+
+
+def test_gpu_unsplit_through_advect_matches_run():
+    # the C ABI's advect with idim = 1..ndim, as VAC's advance calls it
+    nx = [40, 36]
+    g0 = 5 / 3
+    w0, coords = random_fourier((40, 36), 9, g0)
+    dxs = [widths_1d(c) for c in coords]
+    method = vacgpu.MethodSpec(typefull="tvdlf", limiter="woodward", unsplit=True)
+    print("case unsplit host loop", flush=True)
+    grid = vacgpu.Grid(nx, "mhd", g0, dxs)
+    grid.put_mesh(w0)
+    hist, nerror = vacgpu.host_loop(grid, method, 8)
+    wa = grid.get_mesh()
+    grid.close()
+    p = vacref.make_params(2, "mhd", nx, g0, typefull="tvdlf", limiter="woodward", unsplit=True)
+    wr, tr, st = vacref.run(p, w0, dxs, 8)
+    assert np.array_equal(hist, tr)
+    assert_bitwise(wa, wr, "unsplit host loop vs oracle")
+
+
+def test_gpu_unsplit_rejections():
+    nx = [16, 16]
+    w0, coords = random_fourier((16, 16), 1, 1.4)
+    dxs = [widths_1d(c) for c in coords]
+    g = vacgpu.Grid(nx, "mhd", 1.4, dxs)
+    g.put_mesh(w0)
+    for kw in ({"typefull": "tvd"}, {"typefull": "tvdmu"},
+               {"typefull": "tvdlf", "useprimitive": True}):
+        with pytest.raises(vacgpu.VacGpuError):
+            g.run(vacgpu.MethodSpec(unsplit=True, **kw), 1)
+    g.close()

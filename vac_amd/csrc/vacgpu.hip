@@ -219,6 +219,9 @@ int check_method(const vacgpu_config* c, const vacgpu_method* m, std::string& wh
   if (c->physics == VACGPU_MHD && m->divbfix && !m->sourceunsplit) { why = "divbfix=T requires sourceunsplit=T (split sources are not on the GPU path)"; return VACGPU_E_ARG; }
   if (!(m->smallpcoeff >= 0.0)) { why = "smallpcoeff must be >= 0"; return VACGPU_E_ARG; }
   if (m->useprimitive != 0 && m->useprimitive != 1) { why = "useprimitive must be 0 or 1"; return VACGPU_E_ARG; }
+  if (m->unsplit != 0 && m->unsplit != 1) { why = "unsplit must be 0 or 1"; return VACGPU_E_ARG; }
+  if (m->unsplit && m->typefull != VACGPU_TVDLF) { why = "unsplit sweeps (dimsplit=F) are on the GPU path for typefull='tvdlf' only"; return VACGPU_E_ARG; }
+  if (m->unsplit && m->useprimitive) { why = "unsplit sweeps with useprimitive=T are not on the GPU path"; return VACGPU_E_ARG; }
   if (m->useprimitive && m->typefull != VACGPU_TVD) {
     // upwindLR converts its base in place (src/vactvdlf.t:328-364): with
     // 'previous' that is wold, converted again on every sweep of a step
@@ -860,6 +863,54 @@ int sweep_device(vacgpu_grid* g, const vacgpu_method* m, int d, int firstsweep, 
   return 0;
 }
 
+// unsplit sweep (advect over idim = 1..ndim, src/vac.t:325-329) of TVDLF:
+// the predictor and the corrector each add every direction's fluxes in one
+// pass per cell; 'original' limiting reads as 'previous' (src/vacio.t:
+// 363-364), the step's start, which the corrector's input still is
+int sweep_unsplit_device(vacgpu_grid* g, const vacgpu_method* m) {
+  g->old = g->old_pending >= 0 ? g->old_pending : g->cur;
+  g->old_pending = -1;
+  g->mid_step = false;
+  g->ghosts_dirty = false;  // these kernels read every ghost cell from memory
+  const int outi = other_buffer(g);
+  const double* win = g->buf[g->cur];
+  double* wout = g->buf[outi];
+  const double* wold = g->buf[g->old];
+  const Sel s = make_sel(g, m);
+  const long long nb = (long long)g->grid.x * g->grid.y * g->grid.z;
+  const int ghosts = (g->bc_uniform && g->use_fused) ? 1 : 0;
+  const double* wl = m->typelimited == VACGPU_LIM_PREDICTOR ? g->w1 : wold;
+  int rc = 0;
+  {
+    Timed t(g, KF_PRED);
+    DISPATCH(g, {
+      hipLaunchKernelGGL((k_tvdlf_pred_u<ND, MHD, R>), g->grid, g->block, 0, g->stream, g->geo, s, g->bc, win, g->w1, g->sc, ghosts);
+    });
+    if ((rc = launch_check(g, "k_tvdlf_pred_u"))) return rc;
+  }
+  if (!ghosts && (rc = bound(g, g->w1))) return rc;
+  {
+    Timed t(g, KF_CORR);
+    DISPATCH(g, {
+      hipLaunchKernelGGL((k_tvdlf_corr_u<ND, MHD, R>), g->grid, g->block, 0, g->stream, g->geo, s, g->bc, win, g->w1, wout, g->sc, g->part, g->pstride, ghosts, wl);
+    });
+    if ((rc = launch_check(g, "k_tvdlf_corr_u"))) return rc;
+  }
+  if (m->courantpar > 0.0) {
+    Timed t(g, KF_RED);
+    const int slot[3] = {SLOT_C0, SLOT_C1, SLOT_PMIN};
+    for (int d = 0; d < g->cfg.ndim; ++d) {
+      // (the fold reads slot C0 of the pointer it gets)
+      hipLaunchKernelGGL(k_fin_courant_tvdlf, dim3(1), dim3(1024), 0, g->stream, g->sc,
+                         g->part + (long long)slot[d] * g->pstride, g->pstride, nb, d, m->courantpar);
+      if ((rc = launch_check(g, "k_fin_courant_tvdlf"))) return rc;
+    }
+  }
+  if (!ghosts && (rc = bound(g, wout))) return rc;
+  g->cur = outi;
+  return 0;
+}
+
 // boundscalar's face types from the normal B component's (typeBscalar,
 // src/vacproc.projectb.t:82-101): periodic -> periodic, symm -> asymm,
 // asymm -> symm, fixed -> grad0, cont -> nul (the default case)
@@ -1024,9 +1075,10 @@ int advance_device(vacgpu_grid* g, const vacgpu_method* m, int it) {
       int rc = projectb_device(g, g->pr.procpar, first && needs_old, nullptr, nullptr, nullptr);
       if (rc) return rc;
     }
-    return sweep_device(g, m, d, first, last);
+    return d < 0 ? sweep_unsplit_device(g, m) : sweep_device(g, m, d, first, last);
   };
   int rc;
+  if (m->unsplit) return sweep(-1, true, true);  // all directions at once (src/vac.t:325-329)
   if ((it / 2) * 2 == it || m->dimsplit_xy) {
     for (int d = 0; d < nd; ++d)
       if ((rc = sweep(d, d == 0, d == nd - 1))) return rc;
@@ -1340,8 +1392,11 @@ int vacgpu_advect(vacgpu_grid* g, const vacgpu_method* m, int idimmin, int idimm
   std::string why;
   if (check_method(&g->cfg, m, why)) return fail(g, VACGPU_E_ARG, why);
   if (!g->have_state) return fail(g, VACGPU_E_STATE, "no state on the device (call vacgpu_put_state)");
-  if (idimmin != idimmax) return fail(g, VACGPU_E_ARG, "only dimensionally split sweeps (idimmin == idimmax) are on the GPU path");
-  if (idimmin < 1 || idimmin > g->cfg.ndim) return fail(g, VACGPU_E_ARG, "idim out of range");
+  if (idimmin < 1 || idimmin > g->cfg.ndim || idimmax < idimmin || idimmax > g->cfg.ndim)
+    return fail(g, VACGPU_E_ARG, "idim out of range");
+  const bool unsplit = idimmin != idimmax;
+  if (unsplit != (m->unsplit != 0) || (unsplit && (idimmin != 1 || idimmax != g->cfg.ndim)))
+    return fail(g, VACGPU_E_ARG, "an unsplit sweep covers idim = 1..ndim and needs method.unsplit = 1; a split one needs idimmin == idimmax and method.unsplit = 0");
   int rc;
   if (dtcourant || nerror || smallp) {
     if ((rc = read_scalars(g))) return rc;
@@ -1359,7 +1414,8 @@ int vacgpu_advect(vacgpu_grid* g, const vacgpu_method* m, int idimmin, int idimm
     return fail(g, VACGPU_E_STATE, "smallp is not set: call vacgpu_getdt_courant first or pass smallp >= 0");
   hipLaunchKernelGGL(k_set_dt, dim3(1), dim3(1), 0, g->stream, g->sc, dt);
   if ((rc = launch_check(g, "k_set_dt"))) return rc;
-  if ((rc = sweep_device(g, m, idimmin - 1, firstsweep, lastsweep))) return rc;
+  if ((rc = unsplit ? sweep_unsplit_device(g, m) : sweep_device(g, m, idimmin - 1, firstsweep, lastsweep)))
+    return rc;
   if (dtcourant || nerror || smallp) {
     if ((rc = read_scalars(g))) return rc;
     if (dtcourant)

@@ -37,7 +37,8 @@ real(c_double), target :: dtc(3), sp
 !-----------------------------------------------------------------------------
 
 if(ix^L/=ixG^L|.or.|.or.) call die('vacgpu advect: subgrids are not supported')
-if(idimmin/=idimmax) call die('vacgpu advect: needs dimsplit=T')
+if(idimmin/=idimmax .and. (idimmin/=1 .or. idimmax/=ndim)) &
+   call die('vacgpu advect: a sweep covers one direction or all of them')
 if(nproc(2)/=0) call die('vacgpu advect: nproc(2) processing is not on the GPU path')
 if(typeaxial/='slab' .or. gencoord) call die('vacgpu advect: slab Cartesian grids only')
 if(smallfix .or. extraB) call die('vacgpu advect: smallfix/extraB are not on the GPU path')
@@ -78,6 +79,7 @@ do iw=1,nw
    m%typelimiter(iw)=code
 end do
 m%useprimitive=merge(1,0,useprimitive)
+m%unsplit=merge(1,0,idimmin/=idimmax)
 m%divbfix=merge(1,0,divbfix)
 m%divbwave=merge(1,0,divbwave)
 m%sourceunsplit=merge(1,0,sourceunsplit)

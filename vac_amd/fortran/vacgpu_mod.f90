@@ -51,6 +51,7 @@ module vacgpu_mod
      integer(c_int) :: typeentropy(8)
      real(c_double) :: entropycoef(8)
      integer(c_int) :: useprimitive
+     integer(c_int) :: unsplit
   end type vacgpu_method
 
   interface

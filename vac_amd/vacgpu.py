@@ -41,7 +41,8 @@ class Method(C.Structure):
                 ("divbwave", C.c_int), ("sourceunsplit", C.c_int), ("dimsplit_xy", C.c_int),
                 ("courantpar", C.c_double), ("smallpcoeff", C.c_double),
                 ("typelimited", C.c_int), ("typeentropy", C.c_int * 8),
-                ("entropycoef", C.c_double * 8), ("useprimitive", C.c_int)]
+                ("entropycoef", C.c_double * 8), ("useprimitive", C.c_int),
+                ("unsplit", C.c_int)]
 
 
 _lib = None
@@ -134,6 +135,7 @@ class MethodSpec:
     typeentropy: object = "nul"         # one name, or a list per wave
     entropycoef: object = None          # per wave; None: VAC's defaults
     useprimitive: bool = False          # limit primitive variables (FP64)
+    unsplit: bool = False               # dimsplit=F (TVDLF)
 
     def to_c(self, physics: str = "mhd") -> Method:
         m = Method()
@@ -164,6 +166,7 @@ class MethodSpec:
         for i in range(8):
             m.entropycoef[i] = float(coef[i])
         m.useprimitive = int(self.useprimitive)
+        m.unsplit = int(self.unsplit)
         return m
 
 
@@ -334,11 +337,14 @@ class Grid:
         return dt.value, dtc, (sp.value if sp is not None else None)
 
     def advect(self, method: MethodSpec, idim: int, dt: float, firstsweep: bool,
-               lastsweep: bool, dtcourant=None, nerror=None, smallp=None):
+               lastsweep: bool, dtcourant=None, nerror=None, smallp=None, idimmax=None):
+        """One sweep along idim (1-based), or, with idimmax, the unsplit
+        sweep over idim..idimmax (= 1..ndim)."""
         m = method.to_c(self.physics)
         ne = None if nerror is None else nerror.ctypes.data_as(_IP)
         sp = None if smallp is None else C.c_double(smallp)
-        self._check(self.lib.vacgpu_advect(self.h, C.byref(m), idim, idim, dt, int(firstsweep),
+        i1 = idim if idimmax is None else idimmax
+        self._check(self.lib.vacgpu_advect(self.h, C.byref(m), idim, i1, dt, int(firstsweep),
                                            int(lastsweep), _ptr(dtcourant), ne,
                                            C.byref(sp) if sp is not None else None), "advect")
         return sp.value if sp is not None else None
@@ -379,7 +385,10 @@ def host_loop(grid: Grid, method: MethodSpec, nsteps: int, it0: int = 0, t0: flo
     hist = [t]
     for _ in range(nsteps):
         dt, dtcourant, smallp = grid.getdt_courant(method, dtcourant, smallp)
-        order = list(range(ndim)) if (it % 2 == 0 or method.dimsplit_xy) else list(range(ndim - 1, -1, -1))
+        if method.unsplit:  # all directions in one advect (src/vac.t:325-329)
+            grid.advect(method, 1, dt, True, True, dtcourant, nerror, smallp, idimmax=ndim)
+        order = [] if method.unsplit else \
+            list(range(ndim)) if (it % 2 == 0 or method.dimsplit_xy) else list(range(ndim - 1, -1, -1))
         for k, d in enumerate(order):
             grid.advect(method, d + 1, dt, k == 0, k == ndim - 1, dtcourant, nerror, smallp)
         it += 1
