@@ -34,7 +34,8 @@ enum { VACGPU_HD = 0, VACGPU_MHD = 1 };                 /* setvac -p= */
 enum { VACGPU_TVDLF = 0, VACGPU_TVD = 1, VACGPU_TVDMU = 2 };  /* typefull */
 enum { VACGPU_MINMOD = 0, VACGPU_WOODWARD = 1, VACGPU_SUPERBEE = 2 };  /* typelimiter */
 enum { VACGPU_CONT = 0, VACGPU_PERIODIC = 1, VACGPU_SYMM = 2, VACGPU_ASYMM = 3,
-       VACGPU_FIXED = 4 };                             /* typeB */
+       VACGPU_FIXED = 4, VACGPU_CONT1 = 5, VACGPU_FIXED1 = 6,
+       VACGPU_GRAD1 = 7 };                             /* typeB (src/vacgrid.t:234-404) */
 enum { VACGPU_FP64 = 0, VACGPU_FP32BULK = 1 };          /* arithmetic tier */
 enum { VACGPU_LIM_ORIGINAL = 0, VACGPU_LIM_PREVIOUS = 1, VACGPU_LIM_PREDICTOR = 2 };  /* typelimited */
 enum { VACGPU_ENT_NUL = 0, VACGPU_ENT_HARTEN = 1, VACGPU_ENT_POWELL = 2, VACGPU_ENT_RATIO = 3,

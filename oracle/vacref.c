@@ -113,7 +113,12 @@ double vacref_limit(int type, double a, double b) {
  * each spanning the full transverse ixG range.  'symm' / 'asymm' reflect the
  * mesh into the ghost layers (times -1 for 'asymm', :317-352); 'fixed' copies
  * like 'cont' on the first call and keeps the values that call left ever
- * after (fixB, :249-257 and :372-404). */
+ * after (fixB, :249-257 and :372-404).  'cont1' extrapolates linearly from
+ * the edge cell e and the next one f, (|c-e|+1)*w(e) - |c-e|*w(f)
+ * (:274-309); 'fixed1' and 'grad1' extrapolate so on the first call, then
+ * 'fixed1' keeps that call's values and 'grad1' its differences to the edge,
+ * w(e) + (fix(c) - fix(e)) (fixB holds w(c) - w(e) of the first call,
+ * :388-398). */
 static void getboundary_vars(ctx_t* x, double* w, int iw0, int iw1) {
   for (int d = 0; d < x->ndim; ++d) {
     for (int up = 0; up < 2; ++up) {
@@ -130,7 +135,19 @@ static void getboundary_vars(ctx_t* x, double* w, int iw0, int iw1) {
         LOOP(lo, hi) {
           int c = CRD(d), src;
           long idx = IDX(x);
-          if (bc == VACREF_FIXED && x->fix_ready) { pw[idx] = fx[idx]; continue; }
+          if ((bc == VACREF_FIXED || bc == VACREF_FIXED1) && x->fix_ready) { pw[idx] = fx[idx]; continue; }
+          if (bc == VACREF_CONT1 || bc == VACREF_FIXED1 || bc == VACREF_GRAD1) {
+            /* ixe, ixf: the edge cell and the next one inward (:281-286) */
+            const int e = up ? x->mhi[d] : x->mlo[d], f = up ? x->mhi[d] - 1 : x->mlo[d] + 1;
+            const long eidx = idx + (long)(e - c) * x->st[d], fidx = idx + (long)(f - c) * x->st[d];
+            if (bc == VACREF_GRAD1 && x->fix_ready) {
+              pw[idx] = (fx[idx] - fx[eidx]) + pw[eidx];
+            } else {
+              const int a = c > e ? c - e : e - c;
+              pw[idx] = (double)(a + 1) * pw[eidx] - (double)a * pw[fidx];
+            }
+            continue;
+          }
           double sg = 1.0;
           if (bc == VACREF_PERIODIC) src = up ? c - n : c + n;   /* :311-313, pairs :107-120 */
           else if (bc == VACREF_SYMM || bc == VACREF_ASYMM) {
@@ -1277,6 +1294,7 @@ static int bscalar_type(const ctx_t* x, int iB) {
     case VACREF_SYMM: return SB_ASYMM;
     case VACREF_ASYMM: return SB_SYMM;
     case VACREF_FIXED: return SB_GRAD0;
+    case VACREF_FIXED1: return SB_GRAD0;
   }
   return SB_NUL;
 }
@@ -1595,7 +1613,9 @@ static int ctx_init(ctx_t* x, const vacref_params* p, vacref_state* s,
   for (size_t q = 0; q < sizeof(one) / sizeof(one[0]); ++q) *one[q] = calloc(1, pl);
   int anyfixed = 0;
   for (int f = 0; f < 2 * x->ndim; ++f)
-    for (int iw = 0; iw < x->nw; ++iw) anyfixed |= p->bc[f][iw] == VACREF_FIXED;
+    for (int iw = 0; iw < x->nw; ++iw)
+      anyfixed |= p->bc[f][iw] == VACREF_FIXED || p->bc[f][iw] == VACREF_FIXED1 ||
+                  p->bc[f][iw] == VACREF_GRAD1;
   x->fixw = anyfixed ? calloc(x->nw, pl) : NULL;
   x->fix_ready = 0;
   return 0;

@@ -18,7 +18,8 @@ extern "C" {
 enum { VACREF_HD = 0, VACREF_MHD = 1 };
 enum { VACREF_TVDLF = 0, VACREF_TVD = 1, VACREF_TVDMU = 2 };
 enum { VACREF_MINMOD = 0, VACREF_WOODWARD = 1, VACREF_SUPERBEE = 2 };
-enum { VACREF_CONT = 0, VACREF_PERIODIC = 1, VACREF_SYMM = 2, VACREF_ASYMM = 3, VACREF_FIXED = 4 };
+enum { VACREF_CONT = 0, VACREF_PERIODIC = 1, VACREF_SYMM = 2, VACREF_ASYMM = 3, VACREF_FIXED = 4,
+       VACREF_CONT1 = 5, VACREF_FIXED1 = 6, VACREF_GRAD1 = 7 };
 
 typedef struct {
   int ndim;              /* 1..3; ndir == ndim (setvac -d=11/22/33) */

@@ -18,8 +18,9 @@ LIB = os.path.join(HERE, "libvacref.so")
 HD, MHD = 0, 1
 TVDLF, TVD, TVDMU = 0, 1, 2
 LIMITERS = {"minmod": 0, "woodward": 1, "superbee": 2}
-CONT, PERIODIC, SYMM, ASYMM, FIXED = 0, 1, 2, 3, 4
-BC_CODES = {"cont": CONT, "periodic": PERIODIC, "symm": SYMM, "asymm": ASYMM, "fixed": FIXED}
+CONT, PERIODIC, SYMM, ASYMM, FIXED, CONT1, FIXED1, GRAD1 = 0, 1, 2, 3, 4, 5, 6, 7
+BC_CODES = {"cont": CONT, "periodic": PERIODIC, "symm": SYMM, "asymm": ASYMM, "fixed": FIXED,
+            "cont1": CONT1, "fixed1": FIXED1, "grad1": GRAD1}
 
 
 class Params(C.Structure):

@@ -149,6 +149,18 @@ UNSPLIT_CASES = {
 }
 
 
+# first-order extrapolating boundaries: 'cont1' every fill, 'fixed1' /
+# 'grad1' on the first fill, then its values / differences to the edge
+# name: (binary, physics, builder, gamma, steps, typefull, limiter, typeB per face)
+EXTRAP_X = ["cont1", "fixed1", "grad1"]
+EXTRAP_CASES = {
+    "mhd22_rf32x28_tvdlf_minmod_cont1": ("vac_mhd22_132", "mhd", lambda g: random_fourier((32, 28), 1001, g), 5 / 3, 10, "tvdlf", "minmod", ["cont1", "cont1", "periodic", "periodic"]),
+    "mhd22_rf28x30_tvd_woodward_fixed1_grad1": ("vac_mhd22_132", "mhd", lambda g: random_fourier((28, 30), 1002, g), 5 / 3, 10, "tvd", "woodward", ["fixed1", "grad1", "cont1", "fixed1"]),
+    "hd22_rf30x26_tvdmu_superbee_grad1": ("vac_hd22_132", "hd", lambda g: random_fourier((30, 26), 1003, g, physics="hd"), 1.4, 12, "tvdmu", "superbee", ["periodic", "periodic", "grad1", "grad1"]),
+    "mhd33_rf10x12x8_tvdlf_woodward_extrap": ("vac_mhd33_36", "mhd", lambda g: random_fourier((10, 12, 8), 1004, g), 5 / 3, 5, "tvdlf", "woodward", ["cont1", "fixed1", "grad1", "cont1", "fixed1", "grad1"]),
+}
+
+
 def ensure_binary(name: str) -> None:
     if have_ref(name):
         return
@@ -298,6 +310,22 @@ def main() -> None:
             nx=np.array([len(c) for c in coords]), gamma=gamma, steps=steps,
             physics=physics, typefull="tvdlf", limiter=limiter, bc=bc, divbfix=True,
             typelimited=tl, unsplit=True)
+        print("wrote", name, w0.shape, "steps", len(t) - 1)
+    for name, (binary, physics, build, gamma, steps, typefull, limiter, bc) in EXTRAP_CASES.items():
+        if only and not any(o in name for o in only):
+            continue
+        ensure_binary(binary)
+        w0, coords = build(gamma)
+        nw = w0.shape[0]
+        ini = make_ini(w0, coords, physics, gamma, name="golden")
+        par = problem_par(nw, NAMES[nw], steps, typefull=typefull, limiter=limiter,
+                          typeB=bc, every_step=True)
+        r = run_ref(binary, ini, par)
+        t = np.array([s_.t for s_ in r.snapshots])
+        np.savez_compressed(
+            os.path.join(HERE, name + ".npz"), w0=w0, wf=r.snapshots[-1].w, t=t,
+            nx=np.array([len(c) for c in coords]), gamma=gamma, steps=steps,
+            physics=physics, typefull=typefull, limiter=limiter, bc=json.dumps(bc), divbfix=True)
         print("wrote", name, w0.shape, "steps", len(t) - 1)
     if only:
         return

@@ -61,6 +61,9 @@ CASES = [
      "tvdlf", "woodward", "periodic", True, "", "", False, True),
     ("vac_mhd33_36", "vacgpu_mhd33_36", "mhd", lambda g: random_fourier((12, 10, 14), 78, g), 5 / 3, 6,
      "tvdlf", "minmod", "cont", True, "predictor", "", False, True),
+    # first-order extrapolating faces
+    ("vac_mhd22_132", "vacgpu_mhd22_132", "mhd", lambda g: random_fourier((36, 40), 79, g), 5 / 3, 12,
+     "tvd", "woodward", ["cont1", "fixed1", "grad1", "cont1"], True),
 ]
 
 

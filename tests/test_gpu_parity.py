@@ -346,3 +346,27 @@ def test_gpu_unsplit_rejections():
         with pytest.raises(vacgpu.VacGpuError):
             g.run(vacgpu.MethodSpec(unsplit=True, **kw), 1)
     g.close()
+
+
+@pytest.mark.parametrize("case", [
+    ("mhd", (48, 40), "tvdlf", "woodward", ["cont1", "cont1", "periodic", "periodic"], 10),
+    ("mhd", (40, 44), "tvd", "minmod", ["fixed1", "grad1", "grad1", "fixed1"], 10),
+    ("mhd", (44, 36), "tvdmu", "superbee", ["grad1", "cont1", WALL_Y, WALL_Y], 10),
+    ("hd", (40, 48), "tvdlf", "minmod", ["periodic", "periodic", "fixed1", "cont1"], 12),
+    ("mhd", (12, 10, 14), "tvdlf", "woodward", ["grad1", "fixed1", "cont1", "grad1", "fixed1", "cont1"], 5),
+    ("mhd", (10, 14, 12), "tvd", "woodward", ["cont1", "cont1", "grad1", "grad1", "periodic", "periodic"], 5),
+])
+def test_gpu_extrapolating_boundaries_match_oracle(case):
+    # first-order extrapolation ('cont1' every fill; 'fixed1' / 'grad1' on the
+    # first fill, then its values / differences to the edge), corners included
+    physics, nx, typefull, limiter, bc, steps = case
+    gamma = 5 / 3
+    w0, coords = random_fourier(nx, 61 + len(nx), gamma, physics=physics)
+    method = vacgpu.MethodSpec(typefull=typefull, limiter=limiter)
+    print(f"case {case}", flush=True)
+    w, t, sc = gpu_run(w0, list(nx), physics, gamma, steps, method, bc=bc)
+    p = vacref.make_params(len(nx), physics, list(nx), gamma, typefull=typefull,
+                           limiter=limiter, bc=bc)
+    wr, tr, st = vacref.run(p, w0, [widths_1d(c) for c in coords], steps)
+    assert np.array_equal(t, tr)
+    assert_bitwise(w, wr, str(case))

@@ -42,7 +42,7 @@ struct Sel {
 // boundary types per (face, variable): t = 1 for 'periodic'; k the kind of a
 // non-periodic face: a copy of the edge ('cont'), a mirror image ('symm', or
 // 'asymm' with the sign flipped) or the values the first fill left ('fixed')
-enum { BK_COPY = 0, BK_SYMM = 1, BK_ASYMM = 2, BK_FIXED = 3 };
+enum { BK_COPY = 0, BK_SYMM = 1, BK_ASYMM = 2, BK_FIXED = 3, BK_CONT1 = 4, BK_FIXED1 = 5, BK_GRAD1 = 6 };
 struct BC {
   unsigned char t[6][8];
   unsigned char k[6][8];
@@ -150,6 +150,58 @@ __device__ __forceinline__ double divb_at(const Geo& g, const double* __restrict
 // state after that fill, :372-404; null during the first fill itself, when
 // 'fixed' copies like 'cont').  For copies alone this is the mesh cell
 // reached by mapping each coordinate on its own.
+// The value getboundary leaves in the cell c of variable iw (q = its plane,
+// fq = the plane of the first fill's state, or null during the first fill):
+// the face of the highest dimension K along which c is a ghost decides, from
+// values at positions that are mesh cells along K and are worked out the same
+// way along the lower dimensions.  The extrapolating faces ('cont1', and
+// 'fixed1' / 'grad1' on the first fill) combine two such values; every other
+// kind takes one, so the recursion reads at most 2^ndim mesh cells.
+template <int K>
+__device__ __forceinline__ double ghost_value(const Geo& g, const BC& bc, const double* __restrict__ q,
+                                              const double* __restrict__ fq, int iw, const int* c) {
+  if constexpr (K < 0) {
+    return q[g.off + c[0] + c[1] * g.s[1] + c[2] * g.s[2]];
+  } else {
+    if (K >= g.nd) return ghost_value<K - 1>(g, bc, q, fq, iw, c);
+    const int ck = c[K], n = g.n[K];
+    int f;
+    if (ck < 2) f = 2 * K;
+    else if (ck > n + 1) f = 2 * K + 1;
+    else return ghost_value<K - 1>(g, bc, q, fq, iw, c);
+    const int kind = bc.k[f][iw];
+    int cc[3] = {c[0], c[1], c[2]};
+    if (bc.t[f][iw]) {
+      cc[K] = (f & 1) ? ck - n : ck + n;
+      return ghost_value<K - 1>(g, bc, q, fq, iw, cc);
+    }
+    if ((kind == BK_FIXED || kind == BK_FIXED1) && fq)
+      return fq[g.off + c[0] + c[1] * g.s[1] + c[2] * g.s[2]];
+    if (kind == BK_SYMM || kind == BK_ASYMM) {
+      cc[K] = (f & 1) ? 2 * n + 3 - ck : 3 - ck;
+      const double v = ghost_value<K - 1>(g, bc, q, fq, iw, cc);
+      return kind == BK_ASYMM ? -v : v;
+    }
+    const int e = (f & 1) ? n + 1 : 2;  // the edge cell along K
+    cc[K] = e;
+    if (kind == BK_GRAD1 && fq) {
+      // fixB = w(c) - w(e) of the first fill, added to the current edge
+      const long long oc = g.off + c[0] + c[1] * g.s[1] + c[2] * g.s[2];
+      const long long oe = g.off + cc[0] + cc[1] * g.s[1] + cc[2] * g.s[2];
+      return (fq[oc] - fq[oe]) + ghost_value<K - 1>(g, bc, q, fq, iw, cc);
+    }
+    if (kind == BK_CONT1 || kind == BK_FIXED1 || kind == BK_GRAD1) {
+      // (|c-e|+1)*w(e) - |c-e|*w(f), f the next cell inward
+      const int a = ck > e ? ck - e : e - ck;
+      const double we = ghost_value<K - 1>(g, bc, q, fq, iw, cc);
+      cc[K] = (f & 1) ? n : 3;
+      const double wf = ghost_value<K - 1>(g, bc, q, fq, iw, cc);
+      return double(a + 1) * we - double(a) * wf;
+    }
+    return ghost_value<K - 1>(g, bc, q, fq, iw, cc);  // 'cont', 'fixed' on the first fill
+  }
+}
+
 template <int NW>
 __device__ __forceinline__ void bound_body(const Geo& g, const BC& bc, double* __restrict__ w,
                                            long long t, long long slab0, long long slab1,
@@ -172,31 +224,7 @@ __device__ __forceinline__ void bound_body(const Geo& g, const BC& bc, double* _
 #pragma unroll
   for (int iw = 0; iw < NW; ++iw) {
     if (iw < iw0 || iw >= iw1) continue;
-    int cc[3] = {c[0], c[1], c[2]};
-    bool neg = false;
-    const double* from = w;
-    for (int k = g.nd - 1; k >= 0; --k) {
-      const int ck = cc[k], n = g.n[k];
-      int f;
-      if (ck < 2) f = 2 * k;
-      else if (ck > n + 1) f = 2 * k + 1;
-      else continue;
-      const int kind = bc.k[f][iw];
-      if (bc.t[f][iw]) {
-        cc[k] = (f & 1) ? ck - n : ck + n;
-      } else if (kind == BK_FIXED && fix) {
-        from = fix;
-        break;
-      } else if (kind == BK_SYMM || kind == BK_ASYMM) {
-        cc[k] = (f & 1) ? 2 * n + 3 - ck : 3 - ck;
-        if (kind == BK_ASYMM) neg = !neg;
-      } else {
-        cc[k] = (f & 1) ? n + 1 : 2;
-      }
-    }
-    const long long src = g.off + cc[0] + cc[1] * g.s[1] + cc[2] * g.s[2];
-    const double v = from[iw * g.ps + src];
-    w[iw * g.ps + dst] = neg ? -v : v;
+    w[iw * g.ps + dst] = ghost_value<2>(g, bc, w + iw * g.ps, fix ? fix + iw * g.ps : nullptr, iw, c);
   }
 }
 

@@ -56,8 +56,9 @@ struct vacgpu_grid {
   // for all variables; use_fused selects them (default on)
   bool fused_ok = false, use_fused = true;
   bool bc_uniform = false;  // one boundary type per face for all variables, n >= 4
-  // 'fixed' faces: a copy of the state after the first ghost fill, whose
-  // ghost values those faces keep (null until the first vacgpu_put_state)
+  // 'fixed', 'fixed1' and 'grad1' faces: a copy of the state after the first
+  // ghost fill, whose ghost values (differences to the edge for 'grad1')
+  // those faces keep (null until the first vacgpu_put_state)
   bool any_fixed = false;
   double* fixw = nullptr;
   int fs_seglen = 16;
@@ -177,8 +178,8 @@ int check_config(const vacgpu_config* c, std::string& why) {
   int nw = c->physics == VACGPU_MHD ? 2 + 2 * c->ndim : 2 + c->ndim;
   for (int f = 0; f < 2 * c->ndim; ++f)
     for (int iw = 0; iw < nw; ++iw)
-      if (c->typeB[f][iw] < VACGPU_CONT || c->typeB[f][iw] > VACGPU_FIXED) {
-        why = "typeB must be 'cont', 'periodic', 'symm', 'asymm' or 'fixed'";
+      if (c->typeB[f][iw] < VACGPU_CONT || c->typeB[f][iw] > VACGPU_GRAD1) {
+        why = "typeB must be 'cont', 'periodic', 'symm', 'asymm', 'fixed', 'cont1', 'fixed1' or 'grad1'";
         return VACGPU_E_ARG;
       }
   for (int d = 0; d < c->ndim; ++d)
@@ -922,7 +923,7 @@ SBC make_sbc(const vacgpu_grid* g) {
     sb.type[f] = (unsigned char)(tb == VACGPU_PERIODIC ? SB_PERIODIC
                                  : tb == VACGPU_SYMM ? SB_ASYMM
                                  : tb == VACGPU_ASYMM ? SB_SYMM
-                                 : tb == VACGPU_FIXED ? SB_GRAD0 : SB_NUL);
+                                 : (tb == VACGPU_FIXED || tb == VACGPU_FIXED1) ? SB_GRAD0 : SB_NUL);
   }
   return sb;
 }
@@ -1158,8 +1159,11 @@ int vacgpu_create(const vacgpu_config* cfg, const double* dx0, const double* dx1
       g->bc.t[f][iw] = (unsigned char)(tb == VACGPU_PERIODIC);
       g->bc.k[f][iw] = (unsigned char)(tb == VACGPU_SYMM ? BK_SYMM
                                        : tb == VACGPU_ASYMM ? BK_ASYMM
-                                       : tb == VACGPU_FIXED ? BK_FIXED : BK_COPY);
-      if (tb == VACGPU_FIXED) g->any_fixed = true;
+                                       : tb == VACGPU_FIXED ? BK_FIXED
+                                       : tb == VACGPU_CONT1 ? BK_CONT1
+                                       : tb == VACGPU_FIXED1 ? BK_FIXED1
+                                       : tb == VACGPU_GRAD1 ? BK_GRAD1 : BK_COPY);
+      if (tb == VACGPU_FIXED || tb == VACGPU_FIXED1 || tb == VACGPU_GRAD1) g->any_fixed = true;
     }
   // ghost-cell enumeration: slab d has ghost coordinate d, mesh coordinates
   // below d and full ixG coordinates above d

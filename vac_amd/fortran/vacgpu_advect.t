@@ -200,7 +200,7 @@ do iB=1,nB
       if(code<0) call die('vacgpu advect: typeB='//typeB(iw,iB)//' is not on the GPU path')
       ! 'fixed' keeps the first fill's copies; fullgridini=T would keep
       ! the file's ghost values instead (src/vacgrid.t:249-257)
-      if(code==VACGPU_FIXED .and. fullgridini) &
+      if((code==VACGPU_FIXED .or. code==VACGPU_FIXED1 .or. code==VACGPU_GRAD1) .and. fullgridini) &
          call die('vacgpu advect: typeB=fixed with fullgridini=T is not on the GPU path')
       cfg%typeB(iw,iB)=code
    end do

@@ -17,7 +17,7 @@ module vacgpu_mod
   integer(c_int), parameter :: VACGPU_TVDLF = 0, VACGPU_TVD = 1, VACGPU_TVDMU = 2
   integer(c_int), parameter :: VACGPU_MINMOD = 0, VACGPU_WOODWARD = 1, VACGPU_SUPERBEE = 2
   integer(c_int), parameter :: VACGPU_CONT = 0, VACGPU_PERIODIC = 1, VACGPU_SYMM = 2, &
-       VACGPU_ASYMM = 3, VACGPU_FIXED = 4
+       VACGPU_ASYMM = 3, VACGPU_FIXED = 4, VACGPU_CONT1 = 5, VACGPU_FIXED1 = 6, VACGPU_GRAD1 = 7
   integer(c_int), parameter :: VACGPU_FP64 = 0, VACGPU_FP32BULK = 1
   integer(c_int), parameter :: VACGPU_LIM_ORIGINAL = 0, VACGPU_LIM_PREVIOUS = 1, &
        VACGPU_LIM_PREDICTOR = 2
@@ -283,6 +283,12 @@ contains
        vacgpu_bc_code = VACGPU_ASYMM
     case ('fixed')
        vacgpu_bc_code = VACGPU_FIXED
+    case ('cont1')
+       vacgpu_bc_code = VACGPU_CONT1
+    case ('fixed1')
+       vacgpu_bc_code = VACGPU_FIXED1
+    case ('grad1')
+       vacgpu_bc_code = VACGPU_GRAD1
     case default
        vacgpu_bc_code = -1
     end select

@@ -23,8 +23,9 @@ HD, MHD = 0, 1
 TVDLF, TVD, TVDMU = 0, 1, 2
 LIMITERS = {"minmod": 0, "woodward": 1, "superbee": 2}
 ENTROPY = {"nul": 0, "harten": 1, "powell": 2, "ratio": 3, "yee": 4}
-CONT, PERIODIC, SYMM, ASYMM, FIXED = 0, 1, 2, 3, 4
-BC_CODES = {"cont": CONT, "periodic": PERIODIC, "symm": SYMM, "asymm": ASYMM, "fixed": FIXED}
+CONT, PERIODIC, SYMM, ASYMM, FIXED, CONT1, FIXED1, GRAD1 = 0, 1, 2, 3, 4, 5, 6, 7
+BC_CODES = {"cont": CONT, "periodic": PERIODIC, "symm": SYMM, "asymm": ASYMM, "fixed": FIXED,
+            "cont1": CONT1, "fixed1": FIXED1, "grad1": GRAD1}
 FP64, FP32BULK = 0, 1
 KERNELS = {"tvd_sweep": 0, "tvdlf_pred": 1, "tvdlf_corr": 2, "ryu": 3, "bound": 4, "reduce": 5,
            "tvdlf_fused": 6, "projectb": 7}
