@@ -37,6 +37,7 @@ struct Sel {
   int ent[8];        // typeentropy per wave (ENT_*)
   double ecoef[8];   // entropycoef per wave ('yee')
   int prim;          // useprimitive
+  int srcmul;        // idimmax-idimmin+1 of the sweep carrying the source (TVDLF correctors)
 };
 
 // boundary types per (face, variable): t = 1 for 'periodic'; k the kind of a
@@ -677,7 +678,7 @@ __global__ __launch_bounds__(256) void k_tvdlf_corrx(Geo g, Sel s, BC bc, const 
     if (s.sourceunsplit && s.divbfix) {
       const int c3[3] = {cc, yc, z};
       const double dv = divb_at<ND>(g, w1, idx, c3);
-      powell<ND>(u0, dv, dt * 1.0 / ND, out);
+      powell<ND>(u0, dv, dt * double(s.srcmul) / ND, out);
     }
   }
   if (useful) {
@@ -775,7 +776,7 @@ __global__ __launch_bounds__(256) void k_tvdlf_corrm(Geo g, Sel s, BC bc, const 
         if (s.sourceunsplit && s.divbfix) {
           c3[D] = i;
           const double dv = divb_at<ND>(g, w1, idx, c3);
-          powell<ND>(u0, dv, dt * 1.0 / ND, out);
+          powell<ND>(u0, dv, dt * double(s.srcmul) / ND, out);
         }
       }
       c3[D] = i;
@@ -800,7 +801,10 @@ __global__ __launch_bounds__(256) void k_tvdlf_corrm(Geo g, Sel s, BC bc, const 
 // (src/vactvdlf.t:33-81, :137-287), adding each direction's flux
 // differences to wnew in turn, and add the unsplit source once after the
 // last direction with qdt*(idimmax-idimmin+1)/ndim (:85-86, :294-295).
-// One thread per cell, every direction in order.
+// The predictor: one thread per cell, every direction in order.  The
+// corrector: the face-parallel correctors of the split sweeps, one launch
+// per direction, each adding its direction's differences to the previous
+// one's result.
 
 // the predictor's flux difference of direction D at a cell, subtracted from out
 template <int ND, bool MHD, class R, int D>
@@ -856,75 +860,6 @@ __global__ __launch_bounds__(256) void k_tvdlf_pred_u(Geo g, Sel s, BC bc, const
   }
   store_cell<ND, NW>(g, bc, w1, cl.idx, cl.c, out, ghosts);
 }
-
-// the corrector's flux difference of direction D at a cell (both faces, as
-// tvdlf_corr_body), subtracted from out; returns the faces' Courant maximum
-template <int ND, bool MHD, class R, int D>
-__device__ __forceinline__ double corr_dir(const Geo& g, const Sel& s, const double* __restrict__ wl,
-                                           const double* __restrict__ w1, long long idx, const int* c,
-                                           const double* u0, double dt, double smallp, double* out) {
-  constexpr int NW = Layout<ND, MHD>::nw;
-  const long long sd = g.s[D];
-  const int ci = c[D];
-  double lm[NW], l0[NW], lp[NW], um[NW], up[NW];
-#pragma unroll
-  for (int iw = 0; iw < NW; ++iw) {
-    const double* p = wl + iw * g.ps + idx;
-    const double a = p[-2 * sd], b = p[-sd], cc = p[0], e = p[sd], f = p[2 * sd];
-    lm[iw] = slope<R>(s.lim[iw], a, b, cc);
-    l0[iw] = slope<R>(s.lim[iw], b, cc, e);
-    lp[iw] = slope<R>(s.lim[iw], cc, e, f);
-  }
-  load<NW>(w1, g.ps, idx - sd, um);
-  load<NW>(w1, g.ps, idx + sd, up);
-  double F0[NW], F1[NW], cv0, cv1;
-  corr_face<ND, MHD, R, D>(s, um, u0, lm, l0, g.dx[D][ci - 1] + g.dx[D][ci], smallp, F0, cv0);
-  corr_face<ND, MHD, R, D>(s, u0, up, l0, lp, g.dx[D][ci] + g.dx[D][ci + 1], smallp, F1, cv1);
-  const double qdx = pdiv(dt, g.dx[D][ci]);
-#pragma unroll
-  for (int iw = 0; iw < NW; ++iw) out[iw] = out[iw] - qdx * (F1[iw] - F0[iw]);
-  return cv0 < cv1 ? cv1 : cv0;
-}
-
-// corrector: w = w - sum_d dt/dx_d (F_d(i+1/2) - F_d(i-1/2)) with the MUSCL
-// states of w1 limited on wl (wold for 'previous', w1 for 'predictor'), the
-// Powell source of w1 with dt*ndim/ndim; the Courant maximum of direction d
-// goes to partial slot d (C0, C1, PMIN)
-template <int ND, bool MHD, class R>
-__global__ __launch_bounds__(256) void k_tvdlf_corr_u(Geo g, Sel s, BC bc, const double* __restrict__ w,
-                                                     const double* __restrict__ w1,
-                                                     double* __restrict__ wout,
-                                                     const Scal* __restrict__ sc,
-                                                     double* __restrict__ part, long long pstride,
-                                                     int ghosts, const double* __restrict__ wl) {
-  constexpr int NW = Layout<ND, MHD>::nw;
-  __shared__ double sm[8];
-  const CellIx cl = cell_of(g);
-  double cm[3] = {-INFINITY, -INFINITY, -INFINITY};
-  if (cl.ok) {
-    const double dt = sc->dt, smallp = sc->smallp;
-    double out[NW], u0[NW];
-    load<NW>(w, g.ps, cl.idx, out);
-    load<NW>(w1, g.ps, cl.idx, u0);
-    cm[0] = corr_dir<ND, MHD, R, 0>(g, s, wl, w1, cl.idx, cl.c, u0, dt, smallp, out);
-    cm[1] = corr_dir<ND, MHD, R, 1>(g, s, wl, w1, cl.idx, cl.c, u0, dt, smallp, out);
-    if constexpr (ND == 3) cm[2] = corr_dir<ND, MHD, R, 2>(g, s, wl, w1, cl.idx, cl.c, u0, dt, smallp, out);
-    if constexpr (MHD) {
-      if (s.sourceunsplit && s.divbfix) {
-        const double dv = divb_at<ND>(g, w1, cl.idx, cl.c);
-        powell<ND>(u0, dv, dt * double(ND) / ND, out);
-      }
-    }
-    store_cell<ND, NW>(g, bc, wout, cl.idx, cl.c, out, ghosts);
-  }
-  const int slot[3] = {SLOT_C0, SLOT_C1, SLOT_PMIN};
-#pragma unroll
-  for (int k = 0; k < ND; ++k) {
-    const double bm = block_reduce<true>(cm[k], sm);
-    if (threadIdx.x == 0 && threadIdx.y == 0) part[slot[k] * pstride + block_linear()] = bm;
-  }
-}
-
 
 // Wait for every vector memory load in flight (vmcnt(0); expcnt and lgkmcnt
 // left at their maxima).  The fused y sweep calls it after a step's

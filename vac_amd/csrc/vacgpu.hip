@@ -245,6 +245,7 @@ Sel make_sel(const vacgpu_grid* g, const vacgpu_method* m) {
     s.ecoef[i] = m->entropycoef[i];
   }
   s.prim = m->useprimitive;
+  s.srcmul = 1;
   return s;
 }
 
@@ -878,7 +879,6 @@ int sweep_unsplit_device(vacgpu_grid* g, const vacgpu_method* m) {
   double* wout = g->buf[outi];
   const double* wold = g->buf[g->old];
   const Sel s = make_sel(g, m);
-  const long long nb = (long long)g->grid.x * g->grid.y * g->grid.z;
   const int ghosts = (g->bc_uniform && g->use_fused) ? 1 : 0;
   const double* wl = m->typelimited == VACGPU_LIM_PREDICTOR ? g->w1 : wold;
   int rc = 0;
@@ -890,20 +890,48 @@ int sweep_unsplit_device(vacgpu_grid* g, const vacgpu_method* m) {
     if ((rc = launch_check(g, "k_tvdlf_pred_u"))) return rc;
   }
   if (!ghosts && (rc = bound(g, g->w1))) return rc;
-  {
-    Timed t(g, KF_CORR);
-    DISPATCH(g, {
-      hipLaunchKernelGGL((k_tvdlf_corr_u<ND, MHD, R>), g->grid, g->block, 0, g->stream, g->geo, s, g->bc, win, g->w1, wout, g->sc, g->part, g->pstride, ghosts, wl);
-    });
-    if ((rc = launch_check(g, "k_tvdlf_corr_u"))) return rc;
-  }
-  if (m->courantpar > 0.0) {
-    Timed t(g, KF_RED);
-    const int slot[3] = {SLOT_C0, SLOT_C1, SLOT_PMIN};
-    for (int d = 0; d < g->cfg.ndim; ++d) {
-      // (the fold reads slot C0 of the pointer it gets)
-      hipLaunchKernelGGL(k_fin_courant_tvdlf, dim3(1), dim3(1024), 0, g->stream, g->sc,
-                         g->part + (long long)slot[d] * g->pstride, g->pstride, nb, d, m->courantpar);
+  // corrector: direction by direction, each launch reading the previous
+  // one's result as its base (its own cell only) and writing another buffer:
+  // 2D win -> tmp -> wout, 3D win -> wout -> tmp -> wout; the limiter's base
+  // wl and w1 stay put.  The source comes with the last direction.
+  const int nd = g->cfg.ndim;
+  double* tmp = g->buf[3 - g->cur - outi];
+  const int lim = common_limiter(g, m);
+  for (int d = 0; d < nd; ++d) {
+    const double* base = d == 0 ? win : ((nd - d) % 2 ? tmp : wout);
+    double* dst = (nd - 1 - d) % 2 ? tmp : wout;
+    Sel sd = s;
+    if (d < nd - 1) sd.sourceunsplit = 0;
+    else sd.srcmul = nd;
+    const int gh = d == nd - 1 ? ghosts : 0;
+    long long nbc = 0;
+    {
+      Timed t(g, KF_CORR);
+#define LIMSW(X) do { if (lim == LIM_MINMOD) X(LIM_MINMOD); else if (lim == LIM_WOODWARD) X(LIM_WOODWARD); else if (lim == LIM_SUPERBEE) X(LIM_SUPERBEE); else X(-1); } while (0)
+      if (d == 0) {
+        const dim3 gr((g->geo.n[0] + CX_STRIDE - 1) / CX_STRIDE, g->grid.y, g->grid.z);
+        nbc = (long long)gr.x * gr.y * gr.z;
+#define CX(LV) hipLaunchKernelGGL((k_tvdlf_corrx<ND, MHD, R, LV>), gr, g->block, 0, g->stream, g->geo, sd, g->bc, base, g->w1, dst, g->sc, g->part, g->pstride, gh, wl)
+        DISPATCH(g, { LIMSW(CX); });
+#undef CX
+      } else {
+        const dim3 gr = g->grid_tm[d];
+        nbc = (long long)gr.x * gr.y * gr.z;
+#define CM(DV, LV) hipLaunchKernelGGL((k_tvdlf_corrm<ND, MHD, R, DV, LV>), gr, g->block, 0, g->stream, g->geo, sd, g->bc, base, g->w1, dst, g->sc, g->part, g->pstride, g->seglen, gh, wl)
+#define CM1(LV) CM(1, LV)
+#define CM2(LV) CM((ND == 3 ? 2 : 1), LV)
+        DISPATCH(g, { if (d == 1) LIMSW(CM1); else LIMSW(CM2); });
+#undef CM2
+#undef CM1
+#undef CM
+      }
+#undef LIMSW
+      if ((rc = launch_check(g, "k_tvdlf_corr"))) return rc;
+    }
+    if (m->courantpar > 0.0) {
+      Timed t(g, KF_RED);
+      hipLaunchKernelGGL(k_fin_courant_tvdlf, dim3(1), dim3(1024), 0, g->stream, g->sc, g->part,
+                         g->pstride, nbc, d, m->courantpar);
       if ((rc = launch_check(g, "k_fin_courant_tvdlf"))) return rc;
     }
   }
