@@ -5,6 +5,10 @@
 #   oracle/build_ref.sh NAME NDIMNDIR PHYS G1[,G2[,G3]] [cpu|gpu] [poisson]
 #   e.g. oracle/build_ref.sh vac_mhd22_132 22 mhd 132,132
 #
+# The user module is VAC's nul one (setvac -u=nul) unless VAC_USR names
+# another of the reference's (e.g. VAC_USR=gravity: vacusr.t.gravity,
+# vacusrpar.t.gravity and the vacusr.gravity.t they include).
+#
 # With "poisson" the Poisson module is switched on (what setvac -on=poisson
 # sets: the $if_poisson switch of vacpp.pl), so process() can call projectb
 # and vacpoisson.t is compiled in.
@@ -12,7 +16,8 @@
 # With the fifth argument "gpu" the same VAC host is linked with the GPU
 # drop-in instead of its own advect (vac_amd/fortran/vacgpu_advect.t, which
 # calls libvacgpu through vac_amd/fortran/vacgpu_mod.f90): the CPU advect of
-# vac.t is renamed advect_cpu and left unused.  This integration executable
+# vac.t is renamed advect_cpu and left unused, and VAC's savefile is renamed
+# savefile_cpu, which the drop-in's savefile calls after fetching the state.  This integration executable
 # is what the drop-in test runs against the plain reference build.
 #
 # The recipe is this script, not the reference's setvac/Makefile:
@@ -47,9 +52,13 @@ for f in vac.t vacio.t vacgrid.t vacgrid.setnozzle.t vacdef.t vaccd.t vactvdlf.t
   cp "$REF/$f" .
 done
 for f in "$REF"/vacphys.*.t "$REF"/vacproc.*.t; do cp "$f" .; done
+USR=${VAC_USR:-nul}
+if [ "$USR" != nul ]; then
+  cp "$REF/vacusr.t.$USR" "$REF/vacusrpar.t.$USR" "$REF/vacusr.$USR.t" .
+fi
 chmod u+w ./*
 cp vacpar.t.$PHYS vacpar.t; cp vacphys0.t.$PHYS vacphys0.t; cp vacphys.t.$PHYS vacphys.t
-cp vacusr.t.nul vacusr.t; cp vacusrpar.t.nul vacusrpar.t
+cp vacusr.t.$USR vacusr.t; cp vacusrpar.t.$USR vacusrpar.t
 # portability edit 1: Perl >= 5.22
 sed -i 's/defined(%shift0)/%shift0/' vacpp.pl
 # portability edit 2: F2003 flush statement instead of the NAG module
@@ -76,6 +85,9 @@ if [ "$MODE" = gpu ]; then
   REPO=$(cd "$HERE/.." && pwd)
   cp "$REPO/vac_amd/fortran/vacgpu_advect.t" "$REPO/vac_amd/fortran/vacgpu_mod.f90" .
   perl -pi -e 's/^subroutine advect\(method/subroutine advect_cpu(method/' vac.t
+  # the drop-in's savefile fetches the device state, then calls VAC's
+  perl -pi -e 's/^subroutine savefile\(ifile,w\)/subroutine savefile_cpu(ifile,w)/' vacio.t
+  grep -q '^subroutine savefile_cpu(ifile,w)' vacio.t || { echo "savefile not found in vacio.t" >&2; exit 1; }
   $FC $FFLAGS -c vacgpu_mod.f90 -o vacgpu_mod.o
   MODS="$MODS vacgpu_advect"
   if [ "$POISSON" = poisson ]; then

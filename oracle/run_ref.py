@@ -159,8 +159,9 @@ def problem_par(nw: int, wnames: str, steps: int, typefull: str = "tvdlf",
 
 
 def run_ref(binary: str, ini_bytes: bytes, par: str, timeout: float = 600.0,
-            workdir: str | None = None) -> RefResult:
-    """Run a reference executable on one initial state and parameter file."""
+            workdir: str | None = None, env_extra: dict | None = None) -> RefResult:
+    """Run a reference executable on one initial state and parameter file
+    (env_extra: further environment variables for the process)."""
     from vac_amd.vacfile import read_snapshots
 
     exe = binary if os.path.isabs(binary) else ref_binary(binary)
@@ -172,6 +173,7 @@ def run_ref(binary: str, ini_bytes: bytes, par: str, timeout: float = 600.0,
         with open(os.path.join(tmp, "in.ini"), "wb") as f:
             f.write(ini_bytes)
         env = dict(os.environ)
+        env.update(env_extra or {})
         proc = subprocess.run(
             ["bash", "-c", f"ulimit -s unlimited; exec {exe}"],
             input=par, capture_output=True, text=True, cwd=tmp,
@@ -203,7 +205,11 @@ def hd_varnames(ndim: int) -> tuple:
 
 
 def make_ini(w: np.ndarray, coords: list, physics: str, gamma: float,
-             it: int = 0, t: float = 0.0, name: str = "state") -> bytes:
+             it: int = 0, t: float = 0.0, name: str = "state",
+             special: tuple = ()) -> bytes:
+    """special: (name, value) pairs of a user module's equation parameters
+    (nspecialpar, e.g. ('grav1', 0.0), ('grav2', -1.0) for vacusr gravity),
+    appended to eqpar and to the header's names as VAC's savefile does."""
     from vac_amd.initial import coords_array
     from vac_amd.vacfile import Snapshot, snapshot_bytes
 
@@ -214,6 +220,9 @@ def make_ini(w: np.ndarray, coords: list, physics: str, gamma: float,
     else:
         _, vn = hd_varnames(ndim)
         eqpar = np.array([gamma])
+    if special:
+        eqpar = np.concatenate([eqpar, [float(v) for _, v in special]])
+        vn = vn + " " + " ".join(n for n, _ in special)
     header = f"{name}_{physics}{ndim}{ndim}"
     snap = Snapshot(header, it, t, ndim, w.shape[0],
                     tuple(len(c) for c in coords), eqpar, vn,

@@ -161,6 +161,19 @@ EXTRAP_CASES = {
 }
 
 
+# the projection of B before every step together with a corrector that
+# limits the step's start (typelimited='previous', and dimsplit=F, where
+# 'original' reads as 'previous'): VAC stores wold before advance, i.e.
+# before the projection, so the limiter's base is the unprojected state
+# name: (binary, builder, gamma, steps, typefull, limiter, bc, typelimited, unsplit, procpar)
+PROJ_TL_CASES = {
+    "mhd22_rf30x28_tvdlf_minmod_previous_proj": ("vacp_mhd22_132", lambda g: random_fourier((30, 28), 1101, g), 5 / 3, 12, "tvdlf", "minmod", "periodic", "previous", False, (0.0, 1e-4, 1.0)),
+    "mhd22_rf28x26_tvdlf_woodward_unsplit_proj": ("vacp_mhd22_132", lambda g: random_fourier((28, 26), 1102, g), 1.4, 12, "tvdlf", "woodward", "periodic", "original", True, (0.0, 1e-4, 2.0)),
+    "mhd33_rf10x8x12_tvdlf_minmod_unsplit_proj": ("vacp_mhd33_36", lambda g: random_fourier((10, 8, 12), 1103, g), 5 / 3, 6, "tvdlf", "minmod", "periodic", "original", True, (0.0, 1e-5, 1.0)),
+    "mhd22_rf26x30_tvdmu_superbee_cont_previous_proj": ("vacp_mhd22_132", lambda g: random_fourier((26, 30), 1104, g), 5 / 3, 10, "tvdmu", "superbee", "cont", "previous", False, (0.01, 0.0, 3.0)),
+}
+
+
 def ensure_binary(name: str) -> None:
     if have_ref(name):
         return
@@ -326,6 +339,27 @@ def main() -> None:
             os.path.join(HERE, name + ".npz"), w0=w0, wf=r.snapshots[-1].w, t=t,
             nx=np.array([len(c) for c in coords]), gamma=gamma, steps=steps,
             physics=physics, typefull=typefull, limiter=limiter, bc=json.dumps(bc), divbfix=True)
+        print("wrote", name, w0.shape, "steps", len(t) - 1)
+    for name, (binary, build, gamma, steps, typefull, limiter, bc, tl, unsplit,
+               procpar) in PROJ_TL_CASES.items():
+        if only and not any(o in name for o in only):
+            continue
+        ensure_binary(binary)
+        w0, coords = build(gamma)
+        nw = w0.shape[0]
+        ini = make_ini(w0, coords, "mhd", gamma, name="golden")
+        extra = (f" typelimited='{tl}'\n" + (" dimsplit=F\n" if unsplit else "")
+                 + " nproc=1\n procpar=" + ",".join(repr(float(v)) for v in procpar) + "\n")
+        par = problem_par(nw, NAMES[nw], steps, typefull=typefull, limiter=limiter,
+                          typeB=[bc] * (2 * len(coords)), every_step=True, extra_method=extra)
+        r = run_ref(binary, ini, par)
+        t = np.array([s_.t for s_ in r.snapshots])
+        np.savez_compressed(
+            os.path.join(HERE, name + ".npz"), w0=w0, wf=r.snapshots[-1].w, t=t,
+            nx=np.array([len(c) for c in coords]), gamma=gamma, steps=steps,
+            physics="mhd", typefull=typefull, limiter=limiter, bc=bc, divbfix=True,
+            divbwave=True, typelimited=tl, unsplit=unsplit, nproc1=1,
+            procpar=np.array(procpar))
         print("wrote", name, w0.shape, "steps", len(t) - 1)
     if only:
         return

@@ -114,10 +114,22 @@ PROJ_CASES = [
     # (nproc(4) = 1; this test saves every step), cont faces
     (lambda g: random_fourier((44, 40), 70, g), 5 / 3, 12, "tvd", "woodward", "cont", True, True,
      (-1.0, -1.0, -1.0), "1,0,0,1"),
+    # correctors that limit the step's start, which VAC stores before the
+    # projection: typelimited='previous', and dimsplit=F ('original' reads as
+    # 'previous'; the unsplit corrector's intermediate must not be wold)
+    (lambda g: random_fourier((40, 36), 80, g), 5 / 3, 12, "tvdlf", "minmod", "periodic", True,
+     True, (0.0, 1e-4, 1.0), "1", " typelimited='previous'\n"),
+    (lambda g: random_fourier((36, 40), 81, g), 1.4, 12, "tvdlf", "woodward", "periodic", True,
+     True, (0.0, 1e-4, 2.0), "1", " dimsplit=F\n"),
+    (lambda g: random_fourier((40, 44), 82, g), 5 / 3, 10, "tvdmu", "superbee", "cont", True,
+     True, (0.01, 0.0, 3.0), "1,0,0,1", " typelimited='previous'\n"),
 ]
 
 
-@pytest.mark.parametrize("case", PROJ_CASES, ids=[f"{c[3]}-{c[5]}-proj-nproc{c[9]}" for c in PROJ_CASES])
+@pytest.mark.parametrize("case", PROJ_CASES, ids=[f"{c[3]}-{c[5]}-proj-nproc{c[9]}"
+                                                  + ("-" + c[10].split("=")[1].strip().strip("'")
+                                                     if len(c) > 10 else "")
+                                                  for c in PROJ_CASES])
 def test_vac_with_gpu_advect_and_projection_matches_vac(case):
     # VAC built with its Poisson module, plain and with the drop-in, whose
     # projectb projects on the device (vacgpu_process) wherever VAC's process
@@ -125,14 +137,16 @@ def test_vac_with_gpu_advect_and_projection_matches_vac(case):
     # CG dot products are summed in another order on the device, so the
     # results agree to rounding: relative L1 <= 1e-12 per variable and the t
     # of every step to 1e-12 relative.
-    build, gamma, steps, typefull, limiter, bc, divbfix, divbwave, procpar, nproc = case
+    build, gamma, steps, typefull, limiter, bc, divbfix, divbwave, procpar, nproc = case[:10]
+    more = case[10] if len(case) > 10 else ""
     cpu, gpu = "vacp_mhd22_132", "vacgpup_mhd22_132"
     if not (have_ref(cpu) and have_ref(gpu)):
         pytest.skip("reference executables not built (oracle/build_ref.sh ... poisson)")
     w0, coords = build(gamma)
     nw = w0.shape[0]
     ini = make_ini(w0, coords, "mhd", gamma, name="dropin")
-    extra = f" nproc={nproc}\n procpar=" + ",".join(repr(float(v)) for v in procpar) + "\n"
+    extra = (f" nproc={nproc}\n procpar=" + ",".join(repr(float(v)) for v in procpar) + "\n"
+             + more)
     par = problem_par(nw, NAMES[nw], steps, typefull=typefull, limiter=limiter,
                       typeB=[bc] * (2 * len(coords)), divbfix=divbfix, divbwave=divbwave,
                       every_step=True, extra_method=extra)
@@ -148,3 +162,73 @@ def test_vac_with_gpu_advect_and_projection_matches_vac(case):
         for iw in range(nw):
             e = np.abs(sg.w[iw] - sc.w[iw]).sum() / np.abs(sc.w[iw]).sum()
             assert e <= 1e-12, f"it={sc.it} variable {iw}: rel L1 {e:.3e}"
+
+
+def _transfers(out):
+    return out.count("vacgpu: state to device"), out.count("vacgpu: state to host")
+
+
+def test_dropin_keeps_the_state_on_the_device():
+    # saves at it = 0 and at the end only: w goes to the device once, at the
+    # first sweep, and comes back once, for the final save (src/vac.t:108-111)
+    cpu, gpu = "vac_mhd22_132", "vacgpu_mhd22_132"
+    if not (have_ref(cpu) and have_ref(gpu)):
+        pytest.skip("reference executables not built (oracle/build_ref.sh)")
+    w0, coords = random_fourier((48, 44), 83, 5 / 3)
+    ini = make_ini(w0, coords, "mhd", 5 / 3, name="dropin")
+    steps = 30
+    par = problem_par(6, NAMES[6], steps, typefull="tvd", limiter="woodward")
+    rc = run_ref(cpu, ini, par, timeout=300)
+    rg = run_ref(gpu, ini, par, timeout=300, env_extra={"VACGPU_DROPIN_TRACE": "1"})
+    assert [s.it for s in rg.snapshots] == [0, steps]
+    for sc, sg in zip(rc.snapshots, rg.snapshots):
+        assert sc.t == sg.t
+        assert np.array_equal(sc.w, sg.w), f"state differs at it={sc.it}"
+    assert _transfers(rg.stdout) == (1, 1), rg.stdout[-2000:]
+
+
+def test_dropin_fetches_every_step_for_the_residual():
+    # residmin > 0: VAC evaluates ||w_new - w_old|| on the host after every
+    # step (src/vac.t:90-101), so the drop-in fetches w every step and puts
+    # it back at the next; the result stays VAC's bit for bit
+    cpu, gpu = "vac_mhd22_132", "vacgpu_mhd22_132"
+    if not (have_ref(cpu) and have_ref(gpu)):
+        pytest.skip("reference executables not built (oracle/build_ref.sh)")
+    w0, coords = random_fourier((40, 36), 84, 1.4)
+    ini = make_ini(w0, coords, "mhd", 1.4, name="dropin")
+    steps = 10
+    par = problem_par(6, NAMES[6], steps, typefull="tvdlf", limiter="minmod")
+    par = par.replace(" &stoplist\n", " &stoplist\n residmin=1.D-300\n")
+    rc = run_ref(cpu, ini, par, timeout=300)
+    rg = run_ref(gpu, ini, par, timeout=300, env_extra={"VACGPU_DROPIN_TRACE": "1"})
+    assert [s.it for s in rg.snapshots] == [0, steps]
+    assert np.array_equal(rc.snapshots[-1].w, rg.snapshots[-1].w)
+    assert rc.snapshots[-1].t == rg.snapshots[-1].t
+    assert _transfers(rg.stdout) == (steps, steps), rg.stdout[-2000:]
+
+
+def test_dropin_stops_on_a_user_source():
+    # VAC built with its gravity module (src/vacusr.t.gravity): specialsource
+    # adds rho*g to the momentum inside every substep of advect (src/vac.t:592),
+    # which the device path does not do, so the run must stop, not run
+    # without gravity
+    gpu = "vacgpu_mhd22_132_gravity"
+    if not have_ref(gpu):
+        pytest.skip("reference executables not built (oracle/build_ref.sh with VAC_USR=gravity)")
+    w0, coords = random_fourier((32, 28), 85, 5 / 3)
+    ini = make_ini(w0, coords, "mhd", 5 / 3, name="dropin",
+                   special=(("grav1", 0.0), ("grav2", -1.0)))
+    par = problem_par(6, NAMES[6], 5, typefull="tvdlf", limiter="minmod")
+    # VAC's die() prints the message and STOPs (src/vacio.t:1108-1116): only
+    # the it = 0 save is written
+    r = run_ref(gpu, ini, par, timeout=300)
+    assert "the linked specialsource changes w" in r.stdout, r.stdout[-2000:]
+    assert [s.it for s in r.snapshots] == [0]
+    # with g = 0 the module adds nothing: the run proceeds and equals the nul build
+    ini0 = make_ini(w0, coords, "mhd", 5 / 3, name="dropin",
+                    special=(("grav1", 0.0), ("grav2", 0.0)))
+    rg = run_ref(gpu, ini0, par, timeout=300)
+    if have_ref("vac_mhd22_132"):
+        rc = run_ref("vac_mhd22_132", make_ini(w0, coords, "mhd", 5 / 3, name="dropin"), par,
+                     timeout=300)
+        assert np.array_equal(rc.snapshots[-1].w, rg.snapshots[-1].w)

@@ -75,9 +75,14 @@ def test_gpu_projection_matches_reference_fixture(path):
     z = np.load(path, allow_pickle=False)
     nx = [int(v) for v in z["nx"]]
     procpar = [float(v) for v in z["procpar"]]
+    kw = {}
+    if "typelimited" in z.files:
+        kw["typelimited"] = str(z["typelimited"])
+    if "unsplit" in z.files:
+        kw["unsplit"] = bool(z["unsplit"])
     method = vacgpu.MethodSpec(typefull=str(z["typefull"]), limiter=str(z["limiter"]),
-                               divbfix=bool(z["divbfix"]), divbwave=bool(z["divbwave"]))
-    print(f"case {os.path.basename(path)} nx={nx} procpar={procpar}", flush=True)
+                               divbfix=bool(z["divbfix"]), divbwave=bool(z["divbwave"]), **kw)
+    print(f"case {os.path.basename(path)} nx={nx} procpar={procpar} {kw}", flush=True)
     g, dxs = gpu_grid(z["w0"], nx, float(z["gamma"]), vacref.bc_spec(z["bc"]))
     g.set_process(int(z["nproc1"]), procpar)
     n, t = g.run(method, int(z["steps"]), record_t=True)
@@ -91,7 +96,7 @@ def test_gpu_projection_matches_reference_fixture(path):
     p = vacref.make_params(len(nx), "mhd", nx, float(z["gamma"]), typefull=str(z["typefull"]),
                            limiter=str(z["limiter"]), bc=vacref.bc_spec(z["bc"]),
                            divbfix=bool(z["divbfix"]), divbwave=bool(z["divbwave"]),
-                           nproc1=int(z["nproc1"]), procpar=procpar)
+                           nproc1=int(z["nproc1"]), procpar=procpar, **kw)
     _, _, st = vacref.run(p, z["w0"], dxs, int(z["steps"]))
     assert list(sc["nerror"]) == list(st.nerror)
 

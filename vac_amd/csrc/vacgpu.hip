@@ -42,6 +42,9 @@ struct vacgpu_grid {
   BC bc{};
   int nw = 0;
   double* buf[3] = {nullptr, nullptr, nullptr};
+  // a fourth state buffer, allocated on first need: the unsplit corrector's
+  // intermediate when wold (kept from before a projection) holds the third
+  double* xbuf = nullptr;
   double* w1 = nullptr;
   double* dxdev = nullptr;
   Scal* sc = nullptr;
@@ -895,7 +898,16 @@ int sweep_unsplit_device(vacgpu_grid* g, const vacgpu_method* m) {
   // 2D win -> tmp -> wout, 3D win -> wout -> tmp -> wout; the limiter's base
   // wl and w1 stay put.  The source comes with the last direction.
   const int nd = g->cfg.ndim;
-  double* tmp = g->buf[3 - g->cur - outi];
+  // the chain's intermediate must be neither the input, the output nor wold
+  // (the limiter's base, read by every launch of the chain)
+  double* tmp = nullptr;
+  for (int i = 0; i < 3; ++i)
+    if (i != g->cur && i != outi && i != g->old) tmp = g->buf[i];
+  if (!tmp) {
+    if (!g->xbuf)
+      HIPCHK(g, hipMalloc(&g->xbuf, sizeof(double) * (size_t)g->geo.ps * g->nw));
+    tmp = g->xbuf;
+  }
   const int lim = common_limiter(g, m);
   for (int d = 0; d < nd; ++d) {
     const double* base = d == 0 ? win : ((nd - d) % 2 ? tmp : wout);
@@ -1095,9 +1107,14 @@ int advance_device(vacgpu_grid* g, const vacgpu_method* m, int it) {
   const int nd = g->cfg.ndim;
   const int np = g->pr.nproc1;
   const int di = it - g->pr.itmin;
-  // TVD's Ryu step reads the state from before the step's projection
-  const bool needs_old = m->typefull == VACGPU_TVD && g->cfg.physics == VACGPU_MHD &&
-                         m->divbfix && m->sourceunsplit;
+  // wold is the state from before the step's projection (VAC copies it
+  // before advance, src/vac.t:85): TVD's Ryu step reads it, and so does a
+  // corrector that limits the step's start ('previous', and 'original' of
+  // unsplit sweeps, src/vacio.t:363-364)
+  const bool needs_old = (m->typefull == VACGPU_TVD && g->cfg.physics == VACGPU_MHD &&
+                          m->divbfix && m->sourceunsplit) ||
+                         m->typelimited == VACGPU_LIM_PREVIOUS ||
+                         (m->unsplit && m->typelimited != VACGPU_LIM_PREDICTOR);
   auto sweep = [&](int d, bool first, bool last) {
     // process(1) before the sweep when due (advect, src/vac.t:392-399)
     if (np != 0 && (np < 0 || (first && di == (di / np) * np))) {
@@ -1691,6 +1708,7 @@ void vacgpu_destroy(vacgpu_grid* g) {
   harvest(g);
   for (int i = 0; i < 3; ++i)
     if (g->buf[i]) (void)hipFree(g->buf[i]);
+  if (g->xbuf) (void)hipFree(g->xbuf);
   if (g->w1) (void)hipFree(g->w1);
   if (g->dxdev) (void)hipFree(g->dxdev);
   if (g->sc) (void)hipFree(g->sc);

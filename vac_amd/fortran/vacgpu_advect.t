@@ -8,11 +8,27 @@
 ! typelimiter, courantpar, divbfix, typeB, ...  A selector the GPU path does
 ! not implement stops the run through die(); nothing falls back to the CPU.
 !
-! State transfer: w is copied to the device at the first sweep of a time step
-! and back at the last one, so the rest of VAC (saves, the first step's
-! getdt_courant, residuals, filters, user hooks) always sees a current w.
+! State transfer: the state stays on the device between steps.  VAC's w is
+! copied there at the first sweep of the run (and after the host changed it)
+! and fetched back only where VAC reads it on the host:
+!  * before every savefile (src/vacio.t:1008-1105), through the savefile
+!    below, which the drop-in build links in place of VAC's (renamed
+!    savefile_cpu);
+!  * every step, when a host routine reads or changes w between steps: the
+!    residual (residmin>0 or residmax<1d99, src/vac.t:90-101), split sources
+!    (sourcesplit, src/vac.t:236-272) or a filter (typefilter, src/vac.t:
+!    332-355); the state then goes back to the device at the next step;
+!  * after a step that left some direction without a lagged dtcourant, since
+!    getdt_courant then takes cmax from w (src/vac.t:691-700).
 ! dtcourant, nerror and smallp are exchanged with the common block on every
 ! call, as the CPU advect would update them.
+!
+! User hooks: specialsource is called by VAC inside advect, in every substep
+! (addsource2, src/vac.t:592).  The device path does not call host code
+! inside a sweep, so the first advect of a run calls the linked
+! specialsource once on copies of the state and stops the run if it changes
+! anything (vacusr=nul passes).  specialbound (typeB='special') is rejected
+! with the other boundary types the GPU does not implement.
 !=============================================================================
 subroutine advect(method,ix^L,iws,idim^LIM,w1,w)
 
@@ -32,8 +48,10 @@ type(c_ptr) :: grid
 type(vacgpu_method) :: m
 integer(c_int) :: rc
 integer(c_int), target :: ld(3), ne(4)
-integer:: iw, code
+integer:: iw, code, idim
 real(c_double), target :: dtc(3), sp
+logical:: hostsync, fetch
+logical, save :: probed = .false.
 !-----------------------------------------------------------------------------
 
 if(ix^L/=ixG^L|.or.|.or.) call die('vacgpu advect: subgrids are not supported')
@@ -117,9 +135,15 @@ case default
    call die('vacgpu advect: typelimited='//typelimited//' is not on the GPU path')
 end select
 
+! host routines that read or change w between steps
+hostsync = residmin>zero .or. residmax<bigdouble .or. sourcesplit .or. typefilter1/='nul'
+
 if(firstsweep)then
-   rc=vacgpu_put_state_ghosts(grid,w,ld)
-   if(rc/=0) call die('vacgpu advect: '//vacgpu_errmsg(grid))
+   if(.not.probed)then
+      probed=.true.
+      call vacgpu_probe_source(iws,w,w1)
+   endif
+   if(.not.vacgpu_dev_current) call vacgpu_dropin_put(w)
 endif
 
 ! process(1) when due, as VAC's advect does (src/vac.t:392-399).  VAC's
@@ -144,13 +168,122 @@ if(rc/=0) call die('vacgpu advect: '//vacgpu_errmsg(grid))
 dtcourant(1:ndim)=dtc(1:ndim)
 nerror(1:4)=ne
 smallp=sp
+vacgpu_host_current=.false.
 
 if(lastsweep)then
-   rc=vacgpu_get_state(grid,w,ld)
-   if(rc/=0) call die('vacgpu advect: '//vacgpu_errmsg(grid))
+   fetch=hostsync
+   if(dtpar<=zero .and. courantpar>zero)then
+      do idim=1,ndim
+         if(dtcourant(idim)>=bigdouble) fetch=.true.
+      end do
+   endif
+   if(fetch) call vacgpu_dropin_get(w)
+   ! the host may change w before the next step (split sources, filters)
+   if(hostsync) vacgpu_dev_current=.false.
 endif
 
 firstsweep=.false.
+
+return
+end
+
+!=============================================================================
+subroutine savefile(ifile,w)
+
+! Replaces VAC's savefile (src/vacio.t:1008-1105, renamed savefile_cpu in
+! the drop-in build): fetches the device state into w when w lags behind it,
+! then saves as VAC does.  A projection before the save (nproc(ifile+2),
+! src/vacio.t:1017) runs on the device through vacgpu_projectb.t and leaves
+! both copies current.
+
+use vacgpu_mod
+
+include 'vacdef.f'
+
+integer:: ifile
+double precision:: w(ixG^T,nw)
+!-----------------------------------------------------------------------------
+
+if(.not.vacgpu_host_current) call vacgpu_dropin_get(w)
+call savefile_cpu(ifile,w)
+
+return
+end
+
+!=============================================================================
+subroutine vacgpu_dropin_put(w)
+
+! VAC's w (ghost cells as they are) -> the device state
+
+use iso_c_binding
+use vacgpu_mod
+
+include 'vacdef.f'
+
+double precision:: w(ixG^T,nw)
+integer(c_int) :: rc
+integer(c_int), target :: ld(3)
+!-----------------------------------------------------------------------------
+
+call vacgpu_dropin_init
+ld=1
+^D&ld(^D)=ixGhi^D;
+rc=vacgpu_put_state_ghosts(vacgpu_dropin_grid,w,ld)
+if(rc/=0) call die('vacgpu: '//vacgpu_errmsg(vacgpu_dropin_grid))
+vacgpu_dev_current=.true.
+vacgpu_n_put=vacgpu_n_put+1
+if(vacgpu_trace) write(*,'(a,i8)')'vacgpu: state to device at it=',it
+
+return
+end
+
+!=============================================================================
+subroutine vacgpu_dropin_get(w)
+
+! the device state -> VAC's w
+
+use iso_c_binding
+use vacgpu_mod
+
+include 'vacdef.f'
+
+double precision:: w(ixG^T,nw)
+integer(c_int) :: rc
+integer(c_int), target :: ld(3)
+!-----------------------------------------------------------------------------
+
+ld=1
+^D&ld(^D)=ixGhi^D;
+rc=vacgpu_get_state(vacgpu_dropin_grid,w,ld)
+if(rc/=0) call die('vacgpu: '//vacgpu_errmsg(vacgpu_dropin_grid))
+vacgpu_host_current=.true.
+vacgpu_n_get=vacgpu_n_get+1
+if(vacgpu_trace) write(*,'(a,i8)')'vacgpu: state to host at it=',it
+
+return
+end
+
+!=============================================================================
+subroutine vacgpu_probe_source(iws,w,w1)
+
+! Calls the linked specialsource once, with qdt = dt, on copies of the
+! state (wCT and w both equal to VAC's w; w1 is advect's scratch array) and
+! stops the run if it changed anything: user source terms would be added by
+! VAC inside every substep of advect (addsource2, src/vac.t:592), which the
+! device path does not do.
+
+include 'vacdef.f'
+
+integer:: iws(niw_)
+double precision:: w(ixG^T,nw),w1(ixG^T,nw),wct(ixG^T,nw)
+!-----------------------------------------------------------------------------
+
+wct(ixG^S,1:nw)=w(ixG^S,1:nw)
+w1(ixG^S,1:nw)=w(ixG^S,1:nw)
+call specialsource(dt,ixG^L,ixM^L,iws,t,wct,t,w1)
+if(any(w1(ixG^S,1:nw)/=w(ixG^S,1:nw))) call die( &
+   'vacgpu advect: the linked specialsource changes w; user source terms '// &
+   'are not on the GPU path (build VAC with -u=nul)')
 
 return
 end
@@ -173,9 +306,12 @@ integer(c_int) :: rc
 integer(c_int) :: ld(3)
 integer:: iw, iB, code, ngx(3), mlo(3), mhi(3)
 double precision, save, target :: dxl(ixGhimax,3)
+character(len=8) :: trace
 !-----------------------------------------------------------------------------
 
 if(c_associated(vacgpu_dropin_grid)) return
+call get_environment_variable('VACGPU_DROPIN_TRACE',trace)
+vacgpu_trace = trace/=' ' .and. trace/='0'
 ^D&ld(^D)=ixGhi^D;
 if(ndim<3) ld(3)=1
 if(ndim<2) ld(2)=1

@@ -28,6 +28,16 @@ module vacgpu_mod
   ! whether a process() call comes from inside advect
   type(c_ptr), save :: vacgpu_dropin_grid = c_null_ptr
   logical, save :: vacgpu_in_advect = .false.
+  ! where the current state is: the device state is current once a step has
+  ! uploaded VAC's w and stays current across steps; VAC's w is current
+  ! until the first sweep and again after a fetch (before a save, or every
+  ! step when a host routine reads w between steps)
+  logical, save :: vacgpu_dev_current = .false.
+  logical, save :: vacgpu_host_current = .true.
+  ! host-side transfers of the state so far (reported by vacgpu_dropin_report)
+  integer, save :: vacgpu_n_put = 0, vacgpu_n_get = 0
+  ! print a line per transfer (environment variable VACGPU_DROPIN_TRACE)
+  logical, save :: vacgpu_trace = .false.
 
   type, bind(C) :: vacgpu_config
      integer(c_int) :: ndim

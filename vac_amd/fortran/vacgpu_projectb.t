@@ -36,11 +36,13 @@ if(vacgpu_in_advect)then
    rc=vacgpu_process(vacgpu_dropin_grid,1,c_loc(pp),c_loc(pinfo),c_loc(piter),c_loc(pres))
    if(rc/=0) call die('vacgpu projectb: '//vacgpu_errmsg(vacgpu_dropin_grid))
 else
-   rc=vacgpu_put_state_ghosts(vacgpu_dropin_grid,w,ld)
-   if(rc==0) rc=vacgpu_process(vacgpu_dropin_grid,3,c_loc(pp),c_loc(pinfo),c_loc(piter), &
-      c_loc(pres))
-   if(rc==0) rc=vacgpu_get_state(vacgpu_dropin_grid,w,ld)
+   ! before a save: the drop-in's savefile has made w current; project it on
+   ! the device and bring the result back, so both copies are current
+   if(.not.vacgpu_host_current) call die('vacgpu projectb: the host state lags the device')
+   call vacgpu_dropin_put(w)
+   rc=vacgpu_process(vacgpu_dropin_grid,3,c_loc(pp),c_loc(pinfo),c_loc(piter),c_loc(pres))
    if(rc/=0) call die('vacgpu projectb: '//vacgpu_errmsg(vacgpu_dropin_grid))
+   call vacgpu_dropin_get(w)
 endif
 if(pinfo/=0.and.pinfo/=3) nerror(poissonerr_)=nerror(poissonerr_)+1
 
