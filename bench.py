@@ -11,7 +11,7 @@ source terms).  Metric: Mcell-updates/s = nx*ny*steps / seconds / 1e6.
 
     python bench.py [--gpus 1] [--steps 200] [--warmup 5] [--no-cpu] [--precision fp64]
                     [--no-tvdlf] [--no-tvdmu] [--no-unsplit] [--no-projection] [--no-3d]
-                    [--no-snapshots] [--no-batched] [--no-serving]
+                    [--no-snapshots] [--no-batched] [--no-serving] [--no-dropin]
 
 Prints one JSON line.  Beside the headline it reports the batched workload
 (512 independent 2D MHD grids, sides log-uniform in [64, 512], TVDLF +
@@ -66,34 +66,145 @@ def measured_traffic(kernel):
         return None
 
 
-def cpu_baseline(w0, coords, seconds_budget=30.0):
-    """The reference VAC executable on one host core, on a one-step sample of
-    the same 2048^2 workload; falls back to the C restatement if the
-    reference executable is not present."""
+# FP64 VALU issue peak of MI355X: 78.6 TFLOP/s of 64-lane FMA wave
+# instructions (2 flops per lane) = 614 G wave-instructions/s; a non-packed
+# VALU instruction of any kind takes the same 4 cycles of a 16-lane SIMD
+VALU_PEAK_GINST = 78.6e3 / 128
+VALU_FILE = os.path.join(ROOT, "profiles", "valu.json")
+
+
+def valu_roofline(kernel, kernel_ms):
+    """The dominant kernel against the VALU issue ceiling: its VALU
+    wave-instructions per launch (SQ_INSTS_VALU from the committed counter
+    run, profiles/valu.json) over its live average duration."""
+    try:
+        with open(VALU_FILE) as f:
+            insts = json.load(f)[kernel]["valu_insts_per_launch"]
+    except (OSError, KeyError, ValueError):
+        return None
+    if not kernel_ms:
+        return None
+    rate = insts / (kernel_ms * 1e-3) / 1e9
+    return {"bound": "fp64-valu", "achieved": round(rate, 1), "peak": round(VALU_PEAK_GINST, 1),
+            "unit": "G wave-instructions/s", "frac": round(rate / VALU_PEAK_GINST, 4),
+            "valu_insts_per_launch": insts, "kernel": kernel,
+            "source": "profiles/valu.json (rocprofv3 --pmc SQ_INSTS_VALU of bench.py --probe)"}
+
+
+def cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def host_cores() -> list:
+    """The cores this process may use, at most 16 (one GPU's share of the host)."""
+    cores = sorted(os.sched_getaffinity(0))
+    return cores[:16]
+
+
+def _ref_runs(binary, ini, par, cores):
+    """One reference run per core in `cores`, all at once, each pinned to its
+    core; returns their advance times (s)."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    from run_ref import run_ref
+
+    with ThreadPoolExecutor(max_workers=len(cores)) as ex:
+        futs = [ex.submit(run_ref, binary, ini, par, 900, None, None, c) for c in cores]
+        return [f.result().advance_seconds for f in futs]
+
+
+def cpu_baseline(w0, coords):
+    """The reference VAC (compiled -O3 for x86-64-v3 by oracle/build_ref.sh) on
+    the host's cores: the headline workload (2048^2 TVD + woodward) for one
+    step, on one core and as N independent processes pinned one per core
+    (aggregate throughput = N x cells / the slowest advance time); the same
+    for config 1 (Orszag-Tang 128^2, TVDLF + minmod, 100 steps).  VAC is
+    serial (no threads), so N processes is its multi-core form.  Advance time
+    only ("Finish Advance" - "Start Advance", src/vac.t:52,115).  Falls back to
+    the C restatement on one core if the executables are missing."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
-    from run_ref import have_ref, make_ini, problem_par, run_ref
+    from run_ref import have_ref, make_ini, problem_par
+    from vac_amd.initial import orszag_tang
 
     cells = NX * NY
-    if have_ref("vac_mhd22_2052"):
+    if have_ref("vacO3_mhd22_2052") and have_ref("vacO3_mhd22_132"):
+        cores = host_cores()
+        n = len(cores)
         ini = make_ini(w0, coords, "mhd", GAMMA, name="bench")
         par = problem_par(6, "rho m1 m2 e b1 b2", 1, typefull="tvd", limiter="woodward",
                           courantpar=0.8)
-        r = run_ref("vac_mhd22_2052", ini, par, timeout=600)
-        return {"value": cells * 1 / r.advance_seconds / 1e6, "unit": "Mcell-updates/s",
-                "cores": 1, "kind": "reference",
-                "sample": "VAC 4.52 (flang -O2, fp-contract off), 2048x2048 TVD+woodward, "
-                          "1 of 200 steps, advance time only"}
+        one = _ref_runs("vacO3_mhd22_2052", ini, par, cores[:1])[0]
+        many = _ref_runs("vacO3_mhd22_2052", ini, par, cores)
+        w1, c1 = orszag_tang(128, GAMMA)
+        ini1 = make_ini(w1, c1, "mhd", GAMMA, name="bench")
+        par1 = problem_par(6, "rho m1 m2 e b1 b2", 100, typefull="tvdlf", limiter="minmod",
+                           courantpar=0.8)
+        one1 = _ref_runs("vacO3_mhd22_132", ini1, par1, cores[:1])[0]
+        many1 = _ref_runs("vacO3_mhd22_132", ini1, par1, cores)
+        return {"value": round(n * cells / max(many) / 1e6, 4), "unit": "Mcell-updates/s",
+                "cores": n, "kind": "reference",
+                "sample": (f"VAC 4.52 (flang -O3 -march=x86-64-v3), {n} independent processes "
+                           "pinned one per core, each one step of the 2048x2048 TVD+woodward "
+                           "workload (of 200); advance time only"),
+                "cpu_model": cpu_model(),
+                "single_core": {"value": round(cells / one / 1e6, 4), "seconds_per_step": one},
+                "per_process_seconds": [round(v, 3) for v in many],
+                "config1_ot128_tvdlf_100": {
+                    "single_core": round(128 * 128 * 100 / one1 / 1e6, 3),
+                    "aggregate": round(n * 128 * 128 * 100 / max(many1) / 1e6, 3),
+                    "cores": n, "unit": "Mcell-updates/s"}}
     import vacref
     from vac_amd.grid import widths_1d
 
-    n = 512
-    p = vacref.make_params(2, "mhd", [n, n], GAMMA, typefull="tvd", limiter="woodward")
-    sub = np.ascontiguousarray(w0[:, :n, :n])
+    m = 512
+    p = vacref.make_params(2, "mhd", [m, m], GAMMA, typefull="tvd", limiter="woodward")
+    sub = np.ascontiguousarray(w0[:, :m, :m])
     t0 = time.perf_counter()
-    vacref.run(p, sub, [widths_1d(c[:n]) for c in coords], 2)
+    vacref.run(p, sub, [widths_1d(c[:m]) for c in coords], 2)
     dt = time.perf_counter() - t0
-    return {"value": n * n * 2 / dt / 1e6, "unit": "Mcell-updates/s", "cores": 1,
-            "kind": "port", "sample": "C restatement, 512x512 TVD+woodward, 2 steps"}
+    return {"value": m * m * 2 / dt / 1e6, "unit": "Mcell-updates/s", "cores": 1,
+            "kind": "port", "sample": "C restatement, 512x512 TVD+woodward, 2 steps",
+            "cpu_model": cpu_model()}
+
+
+def dropin_2048(w0, coords, steps):
+    """VAC's own host loop (the reference's Fortran main program, compiled
+    here) with the drop-in advect on the GPU, on the headline workload.  The
+    per-step figure is the difference of the advance times of a (1 + steps)-
+    step and a 1-step run over `steps`, so the first step's host-side
+    getdt_courant and the state's upload are left out; the transfers are
+    counted from the drop-in's trace (one upload, one fetch for the final
+    save).  VAC's loop itself copies w into wold on the host every step
+    (src/vac.t:85)."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    from run_ref import have_ref, make_ini, problem_par, run_ref
+
+    name = "vacgpu_mhd22_2052"
+    if not have_ref(name):
+        return {"skipped": f"oracle/_ref/{name} not built"}
+    ini = make_ini(w0, coords, "mhd", GAMMA, name="bench")
+    adv, trans = {}, {}
+    for n in (1, 1 + steps):
+        par = problem_par(6, "rho m1 m2 e b1 b2", n, typefull="tvd", limiter="woodward",
+                          courantpar=0.8)
+        r = run_ref(name, ini, par, timeout=900, env_extra={"VACGPU_DROPIN_TRACE": "1"})
+        adv[n] = r.advance_seconds
+        trans[n] = (r.stdout.count("vacgpu: state to device"),
+                    r.stdout.count("vacgpu: state to host"))
+    per = (adv[1 + steps] - adv[1]) / steps
+    return {"mcell_updates_per_s": round(NX * NY / per / 1e6, 2),
+            "ms_per_step": round(per * 1e3, 3),
+            "first_step_s": round(adv[1], 3),
+            "state_transfers_to_device_to_host": list(trans[1 + steps]),
+            "workload": (f"VAC 4.52 host loop + GPU advect (vacgpu_advect.t), 2048x2048 TVD+"
+                         f"woodward, {steps} steps after the first, saves at the start and end")}
 
 
 def main():
@@ -107,6 +218,9 @@ def main():
                     help="run only a short workload for hardware-counter passes: a 1 GiB "
                          "device copy (calibration), then 3 steps each of the 2048^2 TVD "
                          "and TVDLF workloads; prints no JSON")
+    ap.add_argument("--probe-3d", action="store_true",
+                    help="as --probe, with 3 steps of the 256^3 TVDLF workload instead of the "
+                         "2048^2 ones")
     ap.add_argument("--no-tvdlf", action="store_true")
     ap.add_argument("--no-tvdmu", action="store_true")
     ap.add_argument("--no-unsplit", action="store_true")
@@ -115,6 +229,7 @@ def main():
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--no-snapshots", action="store_true")
+    ap.add_argument("--no-dropin", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
@@ -125,6 +240,9 @@ def main():
     from vac_amd.grid import widths_1d
     from vac_amd.initial import random_fourier
 
+    if args.probe_3d:
+        probe_3d(args.precision)
+        return
     w0, coords = random_fourier((NX, NY), SEED, GAMMA)
     if args.probe:
         probe(w0, [widths_1d(c) for c in coords], args.precision)
@@ -178,15 +296,19 @@ def main():
         "dtype": "f64" if args.precision == "fp64" else "f32-bulk/f64-accumulate",
         "data": "synthetic (seeded random-Fourier MHD state)",
         "config": {"workload": "2D ideal MHD 2048x2048 periodic, TVD (Roe) + woodward, "
-                               "courantpar 0.8, 200 steps, gamma 5/3",
+                               f"courantpar 0.8, {args.steps} steps, gamma 5/3",
                    "nx": NX, "ny": NY, "seed": SEED, "precision": args.precision},
         "t_end": t_end,
         "nerror": [int(v) for v in sc["nerror"]],
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": measured_traffic("k_tvd_sweep"), "kernel": "k_tvd_sweep",
+                     "traffic": measured_traffic("k_tvd_sweep"),
+                     "traffic_source": "profiles/traffic.json (rocprofv3 --pmc FETCH_SIZE, "
+                                       "WRITE_SIZE of bench.py --probe), not this run",
+                     "kernel": "k_tvd_sweep",
                      "kernel_avg_ms": round(k_ms, 5), "kernel_launches": k_cnt,
                      "b_alg_per_cell": B_ALG_TVD_SWEEP},
+        "roofline_valu": valu_roofline("k_tvd_sweep", k_ms),
         "other_kernels_avg_ms": {k: round(v[0], 5) for k, v in other.items()},
         "step_effective_hbm_frac": round(B_ALG_TVD_STEP * NX * NY / (ms_per_step * 1e-3) / 1e9
                                          / HBM_PEAK_GBS, 4),
@@ -207,6 +329,8 @@ def main():
         out["batched"] = batched()
     if not args.no_serving:
         out["serving"] = serving()
+    if not args.no_dropin:
+        out["dropin_2048"] = dropin_2048(w0, coords, min(args.steps, 20))
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(w0, coords)
     print(json.dumps(out))
@@ -232,6 +356,30 @@ def probe(w0, dxs, precision):
         g.run(vacgpu.MethodSpec(typefull=typefull, limiter=lim, courantpar=0.8), 3)
         g.close()
     print("probe done", flush=True)
+
+
+def probe_3d(precision):
+    """Short run for counter passes on config 3: the 1 GiB copy (calibration),
+    then 3 steps of the 256^3 TVDLF + minmod workload."""
+    import torch
+
+    from vac_amd import vacgpu
+    from vac_amd.grid import widths_1d
+    from vac_amd.initial import random_fourier
+
+    a = torch.empty(1 << 27, dtype=torch.float64, device="cuda")
+    b = torch.empty_like(a)
+    a.fill_(1.0)
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    n = 256
+    w0, coords = random_fourier((n, n, n), SEED + 3, GAMMA)
+    g = vacgpu.Grid([n, n, n], "mhd", GAMMA, [widths_1d(c) for c in coords], precision=precision)
+    g.put_mesh(w0)
+    g.run(vacgpu.MethodSpec(typefull="tvdlf", limiter="minmod", courantpar=0.6), 3)
+    g.close()
+    print("probe-3d done", flush=True)
 
 
 def errors(a, ref):

@@ -159,9 +159,11 @@ def problem_par(nw: int, wnames: str, steps: int, typefull: str = "tvdlf",
 
 
 def run_ref(binary: str, ini_bytes: bytes, par: str, timeout: float = 600.0,
-            workdir: str | None = None, env_extra: dict | None = None) -> RefResult:
+            workdir: str | None = None, env_extra: dict | None = None,
+            cpu: int | None = None) -> RefResult:
     """Run a reference executable on one initial state and parameter file
-    (env_extra: further environment variables for the process)."""
+    (env_extra: further environment variables for the process; cpu: pin the
+    process to that core)."""
     from vac_amd.vacfile import read_snapshots
 
     exe = binary if os.path.isabs(binary) else ref_binary(binary)
@@ -177,7 +179,8 @@ def run_ref(binary: str, ini_bytes: bytes, par: str, timeout: float = 600.0,
         proc = subprocess.run(
             ["bash", "-c", f"ulimit -s unlimited; exec {exe}"],
             input=par, capture_output=True, text=True, cwd=tmp,
-            timeout=timeout, env=env)
+            timeout=timeout, env=env,
+            preexec_fn=(lambda: os.sched_setaffinity(0, {cpu})) if cpu is not None else None)
         out = proc.stdout + proc.stderr
         if proc.returncode != 0 or not os.path.exists(os.path.join(tmp, "run.out")):
             raise RuntimeError(f"reference run failed (rc={proc.returncode}):\n{out[-4000:]}")
