@@ -214,6 +214,35 @@ int vacgpu_set_process(vacgpu_grid* g, int nproc1, const double* procpar, int it
 int vacgpu_process(vacgpu_grid* g, int count, const double* procpar, int* info, int* iter,
                    double* resid);
 
+/* getdt_courant (src/vac.t:664-722) on a state the caller holds in device
+ * memory, without a grid handle and without allocating anything.  No
+ * counterpart in the reference: for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops).  It runs the
+ * kernels of vacgpu_getdt_courant, with their block partials and scalars in
+ * a fixed scratch array of the library's device image (VACGPU_NOALLOC_
+ * SCRATCH_DOUBLES doubles, 4 MiB per GPU, held until the process ends).
+ *   w      device array w(ixG^T, nw) in VAC's layout, x fastest, leading
+ *          dimensions ld[d] >= nx[d] + 4; only mesh cells are read;
+ *   dxd    device arrays of the cell widths along d over ixG (nx[d] + 4);
+ *   dtcourant (ndim, host, may be NULL = none): lagged limits, in; reset to
+ *          1e99 on return as VAC does;
+ *   smallp (host, in/out): < 0 sets the pressure floor from the state's
+ *          pressure minimum times m->smallpcoeff (src/vacphys0.t.mhd:202-214);
+ *   dt     (host, out): the Courant time step;
+ *   hip_stream: the stream to run on (NULL: the null stream); the call
+ *          returns after synchronising it;
+ *   err    (may be NULL): receives a message on failure.
+ * Grids whose mesh needs more than VACGPU_NOALLOC_SCRATCH_DOUBLES - 32
+ * partials ((4 + ndim) per block of 64 x 4 cells: about 22M mesh cells in
+ * 2D, 19M in 3D) are rejected with VACGPU_E_ARG.  Every call on a GPU shares
+ * that GPU's scratch: calls that overlap on one GPU race. */
+#define VACGPU_NOALLOC_SCRATCH_DOUBLES 524288
+int vacgpu_getdt_courant_noalloc(const vacgpu_config* cfg, const vacgpu_method* m,
+                                 const double* w, const int* ld, const double* dx0,
+                                 const double* dx1, const double* dx2, double* dtcourant,
+                                 double* smallp, double* dt, void* hip_stream, char* err,
+                                 int errlen);
+
 /* Start-up check of a selector set: 0 if supported, else VACGPU_E_ARG. */
 int vacgpu_supported(const vacgpu_config* cfg, const vacgpu_method* m);
 

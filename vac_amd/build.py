@@ -15,7 +15,8 @@ import sys
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
-SRCS = [os.path.join(HERE, "csrc", "vacgpu.hip"), os.path.join(HERE, "csrc", "batch.hip")]
+SRCS = [os.path.join(HERE, "csrc", "vacgpu.hip"), os.path.join(HERE, "csrc", "batch.hip"),
+        os.path.join(HERE, "csrc", "noalloc.hip")]
 OUT = os.path.join(HERE, "libvacgpu.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("VACGPU_ARCH", "gfx950")

@@ -74,6 +74,9 @@ SIGNATURES = {
     "vacgpu_snapshot_fetch_mesh": ([C.c_void_p, C.c_int, _DP, _IP, _DP], C.c_int),
     "vacgpu_set_process": ([C.c_void_p, C.c_int, _DP, C.c_int], C.c_int),
     "vacgpu_process": ([C.c_void_p, C.c_int, _DP, _IP, _IP, _DP], C.c_int),
+    "vacgpu_getdt_courant_noalloc": ([C.POINTER(Config), C.POINTER(Method), C.c_void_p, _IP,
+                                      C.c_void_p, C.c_void_p, C.c_void_p, _DP, _DP, _DP,
+                                      C.c_void_p, C.c_char_p, C.c_int], C.c_int),
     "vacgpu_supported": ([C.POINTER(Config), C.POINTER(Method)], C.c_int),
     "vacgpu_last_error": ([C.c_void_p], C.c_char_p),
     "vacgpu_destroy": ([C.c_void_p], None),
