@@ -3,11 +3,14 @@
 An instance advanced in a batch must be bitwise identical to the same grid
 advanced alone through the single-grid path -- whatever the mix of sizes,
 physics, gamma and arithmetic tier around it, whether the step runs eagerly
-or from a replayed hipGraph, and whenever it was admitted.
+or from a replayed hipGraph, and whenever it was admitted -- and match the
+oracle run on its own initial state: bitwise in FP64, within 1e-5 relative
+L1 per variable in FP32-bulk.
 """
 import numpy as np
 import pytest
 
+from oracle_compare import assert_matches_oracle as _cmp
 from vac_amd import vacgpu
 from vac_amd.grid import cell_centres, widths_1d
 from vac_amd.initial import random_fourier
@@ -45,6 +48,10 @@ def alone(w, dxs, nx, ny, physics, gamma, precision, steps=None, tmax=1e99, tmax
     return out, it, tt
 
 
+def assert_matches_oracle(wb, it_b, t_b, w0, dxs, nx, ny, physics, gamma, prec, what, **kw):
+    _cmp(wb, it_b, t_b, w0, dxs, [nx, ny], physics, gamma, prec, what, **kw)
+
+
 def run_batch(batch, entries, max_steps):
     for _ in range(max_steps):
         batch.step(1)
@@ -67,6 +74,8 @@ def test_batch_matches_each_grid_alone():
         wa, it, t = alone(w, dxs, nx, ny, ph, g, prec, steps=12)
         assert it == 12 and t == st["t"], (s, t, st["t"])
         assert np.array_equal(wa, wb), f"instance {s} ({nx}x{ny} {ph} {prec}) differs"
+        assert_matches_oracle(wb, st["it"], st["t"], w, dxs, nx, ny, ph, g, prec,
+                              f"instance {s} ({nx}x{ny} {ph} {prec})")
     b.close()
 
 
@@ -118,6 +127,8 @@ def test_continuous_admission_matches_alone():
                 wa, it, t = alone(w, dxs, nx, ny, ph, g, prec, steps=itmax)
                 assert st["it"] == itmax and st["t"] == t
                 assert np.array_equal(wa, wb), f"instance admitted late differs ({nx}x{ny})"
+                assert_matches_oracle(wb, st["it"], st["t"], w, dxs, nx, ny, ph, g, prec,
+                                      f"instance admitted late ({nx}x{ny} {ph} {prec})")
         assert steps < 200
     b.close()
 
@@ -140,6 +151,8 @@ def test_tmax_exact_per_instance():
         wa, it, t = alone(w, dxs, nx, ny, ph, g, prec, tmax=tmax, tmaxexact=True)
         assert it == st["it"] and t == st["t"]
         assert np.array_equal(wa, b.get_mesh(s))
+        assert_matches_oracle(b.get_mesh(s), st["it"], st["t"], w, dxs, nx, ny, ph, g, prec,
+                              f"tmaxexact instance ({nx}x{ny})", tmax=tmax, tmaxexact=True)
     b.close()
 
 
@@ -197,6 +210,14 @@ def test_fused_and_tile_sweeps_agree():
         b.sync()
         res.append([(b.get_mesh(s), b.status(s)["t"]) for s in slots])
         b.close()
+    # the FP64 instances of the plain method against the oracle
+    for j, (nx, ny, ph, g, prec, seed) in enumerate(MIX[:6]):
+        if j == 3 or prec != "fp64":
+            continue
+        w, dxs = state(nx, ny, ph, g, seed)
+        wa, ta = res[0][j]
+        assert_matches_oracle(wa, 7, ta, w, dxs, nx, ny, ph, g, prec, f"fused instance {j}",
+                              bc="cont" if j == 2 else "periodic")
     for (wa, ta), (wb, tb), prec in zip(res[0], res[1], precs):
         if prec == "fp64":
             assert ta == tb
@@ -217,4 +238,48 @@ def test_tune_reports_segment_lengths():
     print("case tune tiles", rows, sl, flush=True)
     assert all(r in (1, 2, 4, 8) for r in rows)
     assert all(x in (8, 16, 32, 64) for x in sl)
+    b.close()
+
+
+def test_config4_scale_batch():
+    # 512 instances, sides log-uniform in [64, 512] (non-square), gamma in
+    # {5/3, 1.4}, TVDLF + minmod, both tiers: every instance reaches itmax
+    # with no error counted and sum(rho) conserved to its tier's rounding
+    # (periodic faces; rho's update is in flux form); a sample -- the
+    # largest, the smallest and random ones of both tiers -- against the
+    # oracle
+    from vac_amd import serve as sv
+
+    steps = 20
+    insts = [i.prepare() for i in sv.batched_instances(512, seed=4)]
+    rng = np.random.default_rng(44)
+    b = vacgpu.Batch(64 << 30, 512)
+    slots = []
+    for k, inst in enumerate(insts):
+        prec = "fp64" if k % 2 == 0 else "fp32bulk"
+        slots.append((b.add(inst.w, [inst.nx, inst.ny], "mhd", inst.gamma, inst.dxs, METHOD,
+                            precision=prec, itmax=steps), prec))
+    print(f"case config-4 scale: {len(slots)} instances, {steps} steps", flush=True)
+    b.graph(True)
+    b.step(steps + 1)
+    b.sync()
+    for (s, prec), inst in zip(slots, insts):
+        st = b.status(s)
+        assert st["done"] and st["it"] == steps, (inst.nx, inst.ny, st)
+        assert list(st["nerror"]) == [0, 0, 0, 0], (inst.nx, inst.ny, st["nerror"])
+        wb = b.get_mesh(s)
+        # FP64: the flux differences telescope up to FP64 rounding; FP32-bulk
+        # rounds each cell's flux difference to FP32 before the FP64 sum
+        r0, r1 = inst.w[0].sum(), wb[0].sum()
+        tol = 1e-12 if prec == "fp64" else 1e-10
+        assert abs(r1 - r0) <= tol * r0, (inst.nx, inst.ny, prec, r0, r1)
+    cells = [i.nx * i.ny for i in insts]
+    pick = {int(np.argmax(cells)), int(np.argmin(cells)), int(np.argmax(cells[1::2])) * 2 + 1}
+    pick |= set(int(k) for k in rng.choice(len(insts), 5, replace=False))
+    for k in sorted(pick):
+        inst, (s, prec) = insts[k], slots[k]
+        st = b.status(s)
+        print(f"  oracle check instance {k}: {inst.nx}x{inst.ny} {prec}", flush=True)
+        assert_matches_oracle(b.get_mesh(s), st["it"], st["t"], inst.w, inst.dxs, inst.nx,
+                              inst.ny, "mhd", inst.gamma, prec, f"instance {k}")
     b.close()

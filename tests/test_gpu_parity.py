@@ -109,6 +109,49 @@ def test_config2_form_tvd_woodward_96():
     assert_bitwise(w, wr, "tvd96")
 
 
+def test_config2_full_size_two_steps():
+    # the headline workload at its full size, 2048^2 from the bench's seed:
+    # two steps, so both sweep orders run (x then y, then y then x), through
+    # the launch shapes only this size has (the one-round fused layouts, the
+    # Courant folds over thousands of blocks); bitwise against the oracle,
+    # and the TVDLF form of the same state too
+    g = 5.0 / 3.0
+    w0, coords = random_fourier((2048, 2048), 20251203, g)
+    dxs = [widths_1d(c) for c in coords]
+    for typefull, limiter in (("tvd", "woodward"), ("tvdlf", "minmod")):
+        method = vacgpu.MethodSpec(typefull=typefull, limiter=limiter, courantpar=0.8)
+        print(f"case config2 full size 2048^2 {typefull} {limiter} 2 steps", flush=True)
+        w, t, sc = gpu_run(w0, [2048, 2048], "mhd", g, 2, method)
+        p = vacref.make_params(2, "mhd", [2048, 2048], g, typefull=typefull, limiter=limiter)
+        wr, tr, st = vacref.run(p, w0, dxs, 2)
+        assert np.array_equal(t, tr)
+        assert_bitwise(w, wr, f"{typefull} 2048^2")
+        assert list(sc["nerror"]) == list(st.nerror)
+
+
+def test_config3_full_size_two_steps():
+    # config 3 at its full size, 256^3 TVDLF + minmod, courantpar 0.6, from
+    # the bench's seed: two steps (both sweep orders), bitwise against the
+    # oracle; the FP32-bulk tier within 1e-5 relative L1 of it
+    g = 5.0 / 3.0
+    w0, coords = random_fourier((256, 256, 256), 20251203 + 3, g)
+    dxs = [widths_1d(c) for c in coords]
+    method = vacgpu.MethodSpec(typefull="tvdlf", limiter="minmod", courantpar=0.6)
+    print("case config3 full size 256^3 tvdlf minmod 2 steps", flush=True)
+    w, t, sc = gpu_run(w0, [256, 256, 256], "mhd", g, 2, method)
+    p = vacref.make_params(3, "mhd", [256, 256, 256], g, typefull="tvdlf", limiter="minmod",
+                           courantpar=0.6)
+    wr, tr, st = vacref.run(p, w0, dxs, 2)
+    assert np.array_equal(t, tr)
+    assert_bitwise(w, wr, "256^3")
+    del w
+    print("case config3 full size 256^3 fp32bulk 2 steps", flush=True)
+    w32, t32, _ = gpu_run(w0, [256, 256, 256], "mhd", g, 2, method, precision="fp32bulk")
+    assert abs(t32[-1] - tr[-1]) <= 1e-5 * tr[-1]
+    for iw in range(wr.shape[0]):
+        assert rel_l1(w32[iw], wr[iw]) <= 1e-5, iw
+
+
 @pytest.mark.parametrize("typefull,limiter", [("tvdlf", "minmod"), ("tvd", "woodward")])
 def test_host_loop_through_advect_matches_run(typefull, limiter):
     g = 5.0 / 3.0
