@@ -133,7 +133,7 @@ def problem_par(nw: int, wnames: str, steps: int, typefull: str = "tvdlf",
                 limiter: str = "minmod", typeB=None, courantpar: float = 0.8,
                 divbfix: bool = True, divbwave: bool = True,
                 every_step: bool = False, tmax: float | None = None,
-                tmaxexact: bool = False, extra_method: str = "") -> str:
+                tmaxexact: bool = False, extra_method: str = "", extra_param: str = "") -> str:
     ndim_faces = len(typeB) if typeB is not None else 4
     if typeB is None:
         typeB = ["periodic"] * ndim_faces
@@ -154,7 +154,7 @@ def problem_par(nw: int, wnames: str, steps: int, typefull: str = "tvdlf",
         f" typelimiter={nw}*'{limiter}'\n divbfix={_fbool(divbfix)}\n"
         f" divbwave={_fbool(divbwave)}\n{extra_method} /\n"
         f" &boundlist\n typeB={tb}\n /\n"
-        f" &paramlist\n courantpar={courantpar!r}\n /\n"
+        f" &paramlist\n courantpar={courantpar!r}\n{extra_param} /\n"
     )
 
 

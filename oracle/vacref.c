@@ -150,7 +150,7 @@ static void getboundary_vars(ctx_t* x, double* w, int iw0, int iw1) {
           }
           double sg = 1.0;
           if (bc == VACREF_PERIODIC) src = up ? c - n : c + n;   /* :311-313, pairs :107-120 */
-          else if (bc == VACREF_SYMM || bc == VACREF_ASYMM) {
+          else if (bc == VACREF_SYMM || bc == VACREF_SYMM0 || bc == VACREF_ASYMM) {
             /* ixe = 2*ixmin-1 (upper) or 2*ixmax+1 (lower), :329-333 */
             src = up ? 2 * (x->mhi[d] + 1) - 1 - c : 2 * (x->mlo[d] - 1) + 1 - c;
             if (bc == VACREF_ASYMM) sg = -1.0;
@@ -416,6 +416,22 @@ static double ldw_at(ctx_t* x, const double* w, int iw, int d, long idx) {
   return vacref_limit(x->p->limiter[iw], a, b);
 }
 
+/* setnoflux (src/vacgrid.t:416-449), called by addflux (:639-640) for a
+ * variable whose typeB is 'symm0' on a face of dimension d: the flux f
+ * through that face is zero.  f holds the flux of face idx+1/2 at idx
+ * (lo_at = mlo-1 is the lower boundary face, hi_at = mhi the upper); a
+ * predictor's per-cell fluxes pass the cell indices they are kept at, and
+ * -1 for a face they do not touch. */
+static void setnoflux(ctx_t* x, int iw, int d, double* f, int lo_at, int hi_at) {
+  for (int up = 0; up < 2; ++up) {
+    if (x->p->bc[2 * d + up][iw] != VACREF_SYMM0 || (up ? hi_at : lo_at) < 0) continue;
+    int lo[3], hi[3];
+    for (int dd = 0; dd < 3; ++dd) { lo[dd] = x->glo[dd]; hi[dd] = x->ghi[dd]; }
+    lo[d] = hi[d] = up ? hi_at : lo_at;
+    LOOP(lo, hi) f[IDX(x)] = 0.0;
+  }
+}
+
 /* hancock (src/vactvdlf.t:5-89): predictor wnew += -qdt/dx*(F(wL)-F(wR)) */
 static void hancock(ctx_t* x, double qdt, int d, const double* wCT, double* wnew, int srcmul) {
   const int* lo = x->mlo;
@@ -464,6 +480,10 @@ static void hancock(ctx_t* x, double qdt, int d, const double* wCT, double* wnew
   for (int iw = 0; iw < x->nw; ++iw) {
     getflux_full(x, wR, lo, hi, iw, d, fR);
     getflux_full(x, wL, lo, hi, iw, d, fL);
+    /* addflux(qdt,ixO,iw,idim,fLC,ixO,fRC,hxO,wnew): fLC is the right face's
+     * flux of a cell, fRC (kept here at the cell) its left face's */
+    setnoflux(x, iw, d, fR, x->mlo[d], -1);
+    setnoflux(x, iw, d, fL, -1, x->mhi[d]);
     double* wn = PL(x, wnew, iw);
     LOOP(lo, hi) {  /* addflux (src/vacgrid.t:676-677) */
       long idx = IDX(x);
@@ -576,6 +596,7 @@ static void tvdlf_corrector(ctx_t* x, double qdt, int d, const double* wCT, doub
       double diss = -(cmaxC[idx] * 0.5 * (pr[idx] - pl[idx]));
       fL[idx] = fm + diss;
     }
+    setnoflux(x, iw, d, fL, x->mlo[d] - 1, x->mhi[d]);
     double* wn = PL(x, wnew, iw);
     LOOP(olo, ohi) {
       long idx = IDX(x);
@@ -1187,6 +1208,7 @@ static void tvdlimit2(ctx_t* x, double qdt, int d, const double* wL, const doubl
         long idx = IDX(x);
         x->rphi[idx] = x->rphi[idx] * 0.25 * (dvol(x, idx + s) + dvol(x, idx));
       }
+      setnoflux(x, iw, d, x->rphi, x->mlo[d] - 1, x->mhi[d]);
       double* wn = PL(x, wnew, iw);
       LOOP(x->mlo, x->mhi) {  /* addflux with qdt<0 (src/vacgrid.t:716) */
         long idx = IDX(x);
@@ -1216,6 +1238,7 @@ static void centdiff(ctx_t* x, double qdt, int d, double* wCT, double* w) {
       long idx = IDX(x);
       x->fC[idx] = 0.5 * (x->f[idx] + x->f[idx + s]);
     }
+    setnoflux(x, iw, d, x->fC, x->mlo[d] - 1, x->mhi[d]);
     double* wn = PL(x, w, iw);
     LOOP(x->mlo, x->mhi) {
       long idx = IDX(x);
@@ -1291,7 +1314,7 @@ enum { SB_NUL = 0, SB_PERIODIC = 1, SB_SYMM = 2, SB_ASYMM = 3, SB_GRAD0 = 4 };
 static int bscalar_type(const ctx_t* x, int iB) {
   switch (x->p->bc[iB][B(x, iB / 2)]) {
     case VACREF_PERIODIC: return SB_PERIODIC;
-    case VACREF_SYMM: return SB_ASYMM;
+    case VACREF_SYMM: case VACREF_SYMM0: return SB_ASYMM;  /* src/vacproc.projectb.t:74 */
     case VACREF_ASYMM: return SB_SYMM;
     case VACREF_FIXED: return SB_GRAD0;
     case VACREF_FIXED1: return SB_GRAD0;
@@ -1679,19 +1702,29 @@ int vacref_run(const vacref_params* p, vacref_state* s, double* w,
       if (p->typeentropy[il] == VACREF_ENT_RATIO) { ctx_free(&x); return -7; }
   /* TVD with typelimited='predictor' limits the centdiff result (src/vactvd.t:30-36) */
   if (p->typefull == VACREF_TVD && p->typelimited == 2) { ctx_free(&x); return -6; }
-  /* unsplit sweeps are restated for TVDLF only (one-step TVD unsplit is a
-   * warning in VAC, src/vacio.t:508-509) */
-  if (p->typefull != VACREF_TVDLF && p->unsplit) { ctx_free(&x); return -8; }
+  /* unsplit sweeps are restated for TVDLF and TVD-MUSCL (one-step TVD
+   * unsplit is a warning in VAC, src/vacio.t:508-509) */
+  if (p->typefull == VACREF_TVD && p->unsplit) { ctx_free(&x); return -8; }
   getboundary(&x, w);
   int done = 0;
+  double dtpar = p->dtpar, dtold = 0.0;
   for (;;) {
     if (t_hist) t_hist[done] = s->t;
-    if (p->dtpar > 0.0) {
-      s->dt = p->dtpar;
-    } else if (p->courantpar > 0.0) {
-      getdt_courant(&x, w);
+    /* the time step (src/vac.t:60-81) */
+    if (dtpar > 0.0) {
+      s->dt = dtpar;
+    } else {
+      if (p->courantpar > 0.0) getdt_courant(&x, w);
+      if (p->dtcantgrow && s->it > s->itmin) s->dt = dmin(s->dt, dtold);
+      dtold = s->dt;
     }
     if (s->it >= p->itmax || s->t >= p->tmax || s->dt < 1.0e-99 || done >= nsteps) break;
+    if (p->slowsteps == 1 && s->it == s->itmin) dtpar = -1.0;
+    if (p->slowsteps > s->it - s->itmin + 1) {
+      const double q = ((double)(s->it - s->itmin) + 1.0) / (double)p->slowsteps;
+      const double a = 1.0 - q;
+      s->dt = s->dt * (1.0 - a * a);
+    }
     if (p->tmaxexact) s->dt = dmin(s->dt, p->tmax - s->t + SMALLD);
     memcpy(x.wold, w, sizeof(double) * x.nc * x.nw);
     if ((rc = advance(&x, w))) { ctx_free(&x); return rc; }

@@ -18,8 +18,10 @@ extern "C" {
 enum { VACREF_HD = 0, VACREF_MHD = 1 };
 enum { VACREF_TVDLF = 0, VACREF_TVD = 1, VACREF_TVDMU = 2 };
 enum { VACREF_MINMOD = 0, VACREF_WOODWARD = 1, VACREF_SUPERBEE = 2 };
+/* typeB (src/vacgrid.t:195-413); 'symm0' fills ghost cells as 'symm' does
+ * and zeroes the flux through the face (setnoflux, src/vacgrid.t:416-449) */
 enum { VACREF_CONT = 0, VACREF_PERIODIC = 1, VACREF_SYMM = 2, VACREF_ASYMM = 3, VACREF_FIXED = 4,
-       VACREF_CONT1 = 5, VACREF_FIXED1 = 6, VACREF_GRAD1 = 7 };
+       VACREF_CONT1 = 5, VACREF_FIXED1 = 6, VACREF_GRAD1 = 7, VACREF_SYMM0 = 8 };
 
 typedef struct {
   int ndim;              /* 1..3; ndir == ndim (setvac -d=11/22/33) */
@@ -60,8 +62,14 @@ typedef struct {
   int useprimitive;
   /* unsplit: dimsplit=F, one advect over all directions per step
    * (src/vac.t:325-329) with 'original' limiting read as 'previous'
-   * (src/vacio.t:363-364); TVDLF only */
+   * (src/vacio.t:363-364); TVDLF and TVD-MUSCL */
   int unsplit;
+  /* &paramlist time-step controls of the main loop (src/vac.t:68-80):
+   * dtcantgrow = dt may not exceed the previous step's Courant dt;
+   * slowsteps > 1 scales dt by 1-(1-(it-itmin+1)/slowsteps)^2 in the first
+   * steps, slowsteps = 1 takes dtpar for the first step only */
+  int dtcantgrow;
+  int slowsteps;
 } vacref_params;
 
 enum { VACREF_ENT_NUL = 0, VACREF_ENT_HARTEN = 1, VACREF_ENT_POWELL = 2, VACREF_ENT_RATIO = 3,

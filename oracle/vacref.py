@@ -18,9 +18,9 @@ LIB = os.path.join(HERE, "libvacref.so")
 HD, MHD = 0, 1
 TVDLF, TVD, TVDMU = 0, 1, 2
 LIMITERS = {"minmod": 0, "woodward": 1, "superbee": 2}
-CONT, PERIODIC, SYMM, ASYMM, FIXED, CONT1, FIXED1, GRAD1 = 0, 1, 2, 3, 4, 5, 6, 7
+CONT, PERIODIC, SYMM, ASYMM, FIXED, CONT1, FIXED1, GRAD1, SYMM0 = 0, 1, 2, 3, 4, 5, 6, 7, 8
 BC_CODES = {"cont": CONT, "periodic": PERIODIC, "symm": SYMM, "asymm": ASYMM, "fixed": FIXED,
-            "cont1": CONT1, "fixed1": FIXED1, "grad1": GRAD1}
+            "cont1": CONT1, "fixed1": FIXED1, "grad1": GRAD1, "symm0": SYMM0}
 
 
 class Params(C.Structure):
@@ -34,6 +34,7 @@ class Params(C.Structure):
         ("itmax", C.c_int), ("nproc1", C.c_int), ("procpar", C.c_double * 3),
         ("typelimited", C.c_int), ("typeentropy", C.c_int * 8), ("entropycoef", C.c_double * 8),
         ("useprimitive", C.c_int), ("unsplit", C.c_int),
+        ("dtcantgrow", C.c_int), ("slowsteps", C.c_int),
     ]
 
 
@@ -88,7 +89,8 @@ def make_params(ndim, physics, nx, gamma, typefull="tvdlf", limiter="minmod",
                 dimsplit_xy=False, courantpar=0.8, smallpcoeff=1e-6, dtpar=-1.0,
                 tmax=1e99, tmaxexact=False, itmax=1000000, nproc1=0,
                 procpar=(-1.0, -1.0, -1.0), typelimited="original", typeentropy="nul",
-                entropycoef=None, useprimitive=False, unsplit=False) -> Params:
+                entropycoef=None, useprimitive=False, unsplit=False, dtcantgrow=False,
+                slowsteps=0) -> Params:
     p = Params()
     p.ndim = ndim
     p.phys = MHD if physics == "mhd" else HD
@@ -126,6 +128,8 @@ def make_params(ndim, physics, nx, gamma, typefull="tvdlf", limiter="minmod",
         p.entropycoef[i] = float(coef[i])
     p.useprimitive = int(useprimitive)
     p.unsplit = int(unsplit)
+    p.dtcantgrow = int(dtcantgrow)
+    p.slowsteps = int(slowsteps)
     p.nproc1 = int(nproc1)
     for i in range(3):
         p.procpar[i] = float(procpar[i])

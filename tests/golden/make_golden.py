@@ -174,6 +174,42 @@ PROJ_TL_CASES = {
 }
 
 
+# the rest of the non-default selectors: 'symm0' faces (ghost cells as
+# 'symm', zero flux through the face in every addflux), unsplit TVD-MUSCL,
+# useprimitive with typelimited 'previous' / 'predictor' (VAC converts the
+# limiter's base in place every sweep), and the time-step controls
+# dtcantgrow and slowsteps.  Walls: normal momentum and field 'asymm', the
+# rest 'symm0'.
+W0X = ["symm0", "asymm", "symm0", "symm0", "asymm", "symm0"]
+W0Y = ["symm0", "symm0", "asymm", "symm0", "symm0", "asymm"]
+W0Y_HD = ["symm0", "symm0", "asymm", "symm0"]
+W0Z = ["symm0", "symm0", "symm0", "asymm", "symm0", "symm0", "symm0", "asymm"]
+# name: dict(binary, physics, build, gamma, steps, typefull, limiter, bc,
+#            method: extra &methodlist lines, param: extra &paramlist lines,
+#            keys: what the fixture records of them, procpar)
+GEN_CASES = {
+    "mhd22_rf30x28_tvdlf_minmod_symm0": dict(binary="vac_mhd22_132", physics="mhd", build=lambda g: random_fourier((30, 28), 1201, g), gamma=5 / 3, steps=12, typefull="tvdlf", limiter="minmod", bc=[W0X, W0X, "periodic", "periodic"]),
+    "mhd22_rf28x30_tvd_woodward_symm0": dict(binary="vac_mhd22_132", physics="mhd", build=lambda g: random_fourier((28, 30), 1202, g), gamma=5 / 3, steps=10, typefull="tvd", limiter="woodward", bc=["periodic", "periodic", W0Y, W0Y]),
+    "mhd22_rf26x28_tvdmu_superbee_symm0": dict(binary="vac_mhd22_132", physics="mhd", build=lambda g: random_fourier((26, 28), 1203, g), gamma=1.4, steps=10, typefull="tvdmu", limiter="superbee", bc=[W0X, "cont", W0Y, W0Y]),
+    "hd22_rf30x26_tvdlf_woodward_symm0": dict(binary="vac_hd22_132", physics="hd", build=lambda g: random_fourier((30, 26), 1204, g, physics="hd"), gamma=1.4, steps=15, typefull="tvdlf", limiter="woodward", bc=["cont", "cont", W0Y_HD, W0Y_HD]),
+    "hd22_rf28x28_tvd_minmod_symm0": dict(binary="vac_hd22_132", physics="hd", build=lambda g: random_fourier((28, 28), 1205, g, physics="hd"), gamma=1.4, steps=12, typefull="tvd", limiter="minmod", bc=[["symm0", "asymm", "symm0", "symm0"]] * 2 + ["periodic", "periodic"]),
+    "mhd33_rf10x12x8_tvdlf_minmod_symm0": dict(binary="vac_mhd33_36", physics="mhd", build=lambda g: random_fourier((10, 12, 8), 1206, g), gamma=5 / 3, steps=6, typefull="tvdlf", limiter="minmod", bc=["periodic", "periodic", "periodic", "periodic", W0Z, W0Z]),
+    "mhd22_rf28x30_tvd_woodward_symm0_proj": dict(binary="vacp_mhd22_132", physics="mhd", build=lambda g: random_fourier((28, 30), 1207, g), gamma=5 / 3, steps=10, typefull="tvd", limiter="woodward", bc=[W0X, W0X, "periodic", "periodic"], procpar=(0.0, 1e-4, 1.0)),
+    "mhd22_rf30x28_tvdmu_woodward_unsplit": dict(binary="vac_mhd22_132", physics="mhd", build=lambda g: random_fourier((30, 28), 1211, g), gamma=5 / 3, steps=12, typefull="tvdmu", limiter="woodward", bc="periodic", method=" dimsplit=F\n typelimited='original'\n", keys=dict(unsplit=True, typelimited="original")),
+    "hd22_rf28x26_tvdmu_minmod_cont_unsplit_predictor": dict(binary="vac_hd22_132", physics="hd", build=lambda g: random_fourier((28, 26), 1212, g, physics="hd"), gamma=1.4, steps=12, typefull="tvdmu", limiter="minmod", bc="cont", method=" dimsplit=F\n typelimited='predictor'\n", keys=dict(unsplit=True, typelimited="predictor")),
+    "mhd33_rf10x12x8_tvdmu_superbee_unsplit": dict(binary="vac_mhd33_36", physics="mhd", build=lambda g: random_fourier((10, 12, 8), 1213, g), gamma=5 / 3, steps=5, typefull="tvdmu", limiter="superbee", bc="periodic", method=" dimsplit=F\n", keys=dict(unsplit=True)),
+    "mhd22_rf30x28_tvdlf_minmod_prim_previous": dict(binary="vac_mhd22_132", physics="mhd", build=lambda g: random_fourier((30, 28), 1221, g), gamma=5 / 3, steps=12, typefull="tvdlf", limiter="minmod", bc="periodic", method=" useprimitive=T\n typelimited='previous'\n", keys=dict(useprimitive=True, typelimited="previous")),
+    "mhd22_rf28x30_tvdlf_woodward_cont_prim_predictor": dict(binary="vac_mhd22_132", physics="mhd", build=lambda g: random_fourier((28, 30), 1222, g), gamma=1.4, steps=12, typefull="tvdlf", limiter="woodward", bc="cont", method=" useprimitive=T\n typelimited='predictor'\n", keys=dict(useprimitive=True, typelimited="predictor")),
+    "mhd22_rf26x30_tvdmu_superbee_prim_previous": dict(binary="vac_mhd22_132", physics="mhd", build=lambda g: random_fourier((26, 30), 1223, g), gamma=5 / 3, steps=10, typefull="tvdmu", limiter="superbee", bc="periodic", method=" useprimitive=T\n typelimited='previous'\n", keys=dict(useprimitive=True, typelimited="previous")),
+    "hd22_rf28x26_tvdlf_minmod_prim_predictor": dict(binary="vac_hd22_132", physics="hd", build=lambda g: random_fourier((28, 26), 1224, g, physics="hd"), gamma=1.4, steps=15, typefull="tvdlf", limiter="minmod", bc="periodic", method=" useprimitive=T\n typelimited='predictor'\n", keys=dict(useprimitive=True, typelimited="predictor")),
+    "mhd33_rf10x12x8_tvdlf_minmod_prim_previous": dict(binary="vac_mhd33_36", physics="mhd", build=lambda g: random_fourier((10, 12, 8), 1225, g), gamma=5 / 3, steps=6, typefull="tvdlf", limiter="minmod", bc="periodic", method=" useprimitive=T\n typelimited='previous'\n", keys=dict(useprimitive=True, typelimited="previous")),
+    "mhd22_rf28x26_tvdlf_woodward_prim_unsplit": dict(binary="vac_mhd22_132", physics="mhd", build=lambda g: random_fourier((28, 26), 1226, g), gamma=5 / 3, steps=12, typefull="tvdlf", limiter="woodward", bc="periodic", method=" useprimitive=T\n dimsplit=F\n", keys=dict(useprimitive=True, unsplit=True)),
+    "mhd22_rf28x26_tvdlf_minmod_slowsteps5": dict(binary="vac_mhd22_132", physics="mhd", build=lambda g: random_fourier((28, 26), 1231, g), gamma=5 / 3, steps=12, typefull="tvdlf", limiter="minmod", bc="periodic", param=" slowsteps=5\n", keys=dict(slowsteps=5)),
+    "mhd22_rf26x28_tvd_woodward_dtcantgrow": dict(binary="vac_mhd22_132", physics="mhd", build=lambda g: orszag_tang(26, g), gamma=5 / 3, steps=15, typefull="tvd", limiter="woodward", bc="periodic", param=" dtcantgrow=T\n", keys=dict(dtcantgrow=True)),
+    "hd22_rf30x26_tvdmu_minmod_dtcantgrow_slowsteps3": dict(binary="vac_hd22_132", physics="hd", build=lambda g: random_fourier((30, 26), 1233, g, physics="hd"), gamma=1.4, steps=12, typefull="tvdmu", limiter="minmod", bc="cont", param=" dtcantgrow=T\n slowsteps=3\n", keys=dict(dtcantgrow=True, slowsteps=3)),
+}
+
+
 def ensure_binary(name: str) -> None:
     if have_ref(name):
         return
@@ -360,6 +396,33 @@ def main() -> None:
             physics="mhd", typefull=typefull, limiter=limiter, bc=bc, divbfix=True,
             divbwave=True, typelimited=tl, unsplit=unsplit, nproc1=1,
             procpar=np.array(procpar))
+        print("wrote", name, w0.shape, "steps", len(t) - 1)
+    for name, c in GEN_CASES.items():
+        if only and not any(o in name for o in only):
+            continue
+        ensure_binary(c["binary"])
+        w0, coords = c["build"](c["gamma"])
+        nw = w0.shape[0]
+        ini = make_ini(w0, coords, c["physics"], c["gamma"], name="golden")
+        bc = c["bc"]
+        typeB = [bc] * (2 * len(coords)) if isinstance(bc, str) else bc
+        method = c.get("method", "")
+        procpar = c.get("procpar")
+        if procpar is not None:
+            method += " nproc=1\n procpar=" + ",".join(repr(float(v)) for v in procpar) + "\n"
+        par = problem_par(nw, NAMES[nw], c["steps"], typefull=c["typefull"], limiter=c["limiter"],
+                          typeB=typeB, every_step=True, extra_method=method,
+                          extra_param=c.get("param", ""))
+        r = run_ref(c["binary"], ini, par)
+        t = np.array([s_.t for s_ in r.snapshots])
+        extra = dict(c.get("keys", {}))
+        if procpar is not None:
+            extra.update(nproc1=1, procpar=np.array(procpar), divbwave=True)
+        np.savez_compressed(
+            os.path.join(HERE, name + ".npz"), w0=w0, wf=r.snapshots[-1].w, t=t,
+            nx=np.array([len(cc) for cc in coords]), gamma=c["gamma"], steps=c["steps"],
+            physics=c["physics"], typefull=c["typefull"], limiter=c["limiter"],
+            bc=bc if isinstance(bc, str) else json.dumps(bc), divbfix=True, **extra)
         print("wrote", name, w0.shape, "steps", len(t) - 1)
     if only:
         return
