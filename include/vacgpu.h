@@ -35,7 +35,9 @@ enum { VACGPU_TVDLF = 0, VACGPU_TVD = 1, VACGPU_TVDMU = 2 };  /* typefull */
 enum { VACGPU_MINMOD = 0, VACGPU_WOODWARD = 1, VACGPU_SUPERBEE = 2 };  /* typelimiter */
 enum { VACGPU_CONT = 0, VACGPU_PERIODIC = 1, VACGPU_SYMM = 2, VACGPU_ASYMM = 3,
        VACGPU_FIXED = 4, VACGPU_CONT1 = 5, VACGPU_FIXED1 = 6,
-       VACGPU_GRAD1 = 7 };                             /* typeB (src/vacgrid.t:234-404) */
+       VACGPU_GRAD1 = 7,                               /* typeB (src/vacgrid.t:234-404) */
+       VACGPU_SYMM0 = 8 };                             /* 'symm', with zero flux through the face
+                                                          (setnoflux, src/vacgrid.t:416-449) */
 enum { VACGPU_FP64 = 0, VACGPU_FP32BULK = 1 };          /* arithmetic tier */
 enum { VACGPU_LIM_ORIGINAL = 0, VACGPU_LIM_PREVIOUS = 1, VACGPU_LIM_PREDICTOR = 2 };  /* typelimited */
 enum { VACGPU_ENT_NUL = 0, VACGPU_ENT_HARTEN = 1, VACGPU_ENT_POWELL = 2, VACGPU_ENT_RATIO = 3,
@@ -81,8 +83,15 @@ typedef struct {
                            TVD / TVD-MUSCL takes dp = pR - pL
                            (src/vacphys.mhdroe.t:42-44) */
   int unsplit;          /* 1: dimsplit=F, one advect over idim = 1..ndim per
-                           step (src/vac.t:325-329), TVDLF only; 'original'
-                           limiting reads as 'previous' (src/vacio.t:363-364) */
+                           step (src/vac.t:325-329), TVDLF and TVD-MUSCL;
+                           'original' limiting reads as 'previous'
+                           (src/vacio.t:363-364) */
+  int dtcantgrow;       /* vacgpu_run's time step may not exceed the previous
+                           Courant step (src/vac.t:68-69) */
+  int slowsteps;        /* vacgpu_run: > 1 scales dt by 1-(1-(it-itmin+1)/
+                           slowsteps)^2 over the first steps, 1 takes dtpar
+                           for the first step only (src/vac.t:75-78); itmin
+                           is the it of the last vacgpu_set_time */
 } vacgpu_method;
 
 /* Replaces the geometry hand-off after gridsetup1 (src/vacgrid.t:457-593):

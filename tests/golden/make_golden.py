@@ -204,6 +204,9 @@ GEN_CASES = {
     "hd22_rf28x26_tvdlf_minmod_prim_predictor": dict(binary="vac_hd22_132", physics="hd", build=lambda g: random_fourier((28, 26), 1224, g, physics="hd"), gamma=1.4, steps=15, typefull="tvdlf", limiter="minmod", bc="periodic", method=" useprimitive=T\n typelimited='predictor'\n", keys=dict(useprimitive=True, typelimited="predictor")),
     "mhd33_rf10x12x8_tvdlf_minmod_prim_previous": dict(binary="vac_mhd33_36", physics="mhd", build=lambda g: random_fourier((10, 12, 8), 1225, g), gamma=5 / 3, steps=6, typefull="tvdlf", limiter="minmod", bc="periodic", method=" useprimitive=T\n typelimited='previous'\n", keys=dict(useprimitive=True, typelimited="previous")),
     "mhd22_rf28x26_tvdlf_woodward_prim_unsplit": dict(binary="vac_mhd22_132", physics="mhd", build=lambda g: random_fourier((28, 26), 1226, g), gamma=5 / 3, steps=12, typefull="tvdlf", limiter="woodward", bc="periodic", method=" useprimitive=T\n dimsplit=F\n", keys=dict(useprimitive=True, unsplit=True)),
+    "mhd22_rf26x28_tvdlf_minmod_prim_unsplit_predictor": dict(binary="vac_mhd22_132", physics="mhd", build=lambda g: random_fourier((26, 28), 1227, g), gamma=5 / 3, steps=12, typefull="tvdlf", limiter="minmod", bc="cont", method=" useprimitive=T\n dimsplit=F\n typelimited='predictor'\n", keys=dict(useprimitive=True, unsplit=True, typelimited="predictor")),
+    "mhd22_rf28x26_tvdmu_woodward_prim_unsplit": dict(binary="vac_mhd22_132", physics="mhd", build=lambda g: random_fourier((28, 26), 1228, g), gamma=1.4, steps=10, typefull="tvdmu", limiter="woodward", bc="periodic", method=" useprimitive=T\n dimsplit=F\n", keys=dict(useprimitive=True, unsplit=True)),
+    "mhd33_rf10x12x8_tvdmu_minmod_prim_predictor": dict(binary="vac_mhd33_36", physics="mhd", build=lambda g: random_fourier((10, 12, 8), 1229, g), gamma=5 / 3, steps=5, typefull="tvdmu", limiter="minmod", bc="periodic", method=" useprimitive=T\n typelimited='predictor'\n", keys=dict(useprimitive=True, typelimited="predictor")),
     "mhd22_rf28x26_tvdlf_minmod_slowsteps5": dict(binary="vac_mhd22_132", physics="mhd", build=lambda g: random_fourier((28, 26), 1231, g), gamma=5 / 3, steps=12, typefull="tvdlf", limiter="minmod", bc="periodic", param=" slowsteps=5\n", keys=dict(slowsteps=5)),
     "mhd22_rf26x28_tvd_woodward_dtcantgrow": dict(binary="vac_mhd22_132", physics="mhd", build=lambda g: orszag_tang(26, g), gamma=5 / 3, steps=15, typefull="tvd", limiter="woodward", bc="periodic", param=" dtcantgrow=T\n", keys=dict(dtcantgrow=True)),
     "hd22_rf30x26_tvdmu_minmod_dtcantgrow_slowsteps3": dict(binary="vac_hd22_132", physics="hd", build=lambda g: random_fourier((30, 26), 1233, g, physics="hd"), gamma=1.4, steps=12, typefull="tvdmu", limiter="minmod", bc="cont", param=" dtcantgrow=T\n slowsteps=3\n", keys=dict(dtcantgrow=True, slowsteps=3)),

@@ -40,7 +40,7 @@ def test_supported_rejects_unsupported_selectors():
     m = vacgpu.MethodSpec(typefull="tvd").to_c()
     assert lib.vacgpu_supported(C.byref(cfg), C.byref(m)) == 0  # HD Roe solver
     # a typeB the path does not implement: rejected with an error, not a fallback
-    cfg.typeB[0][0] = 8  # (symm0 and the others: no code)
+    cfg.typeB[0][0] = 9  # ('special' and the others: no code)
     assert lib.vacgpu_supported(C.byref(cfg), C.byref(m)) == -1
     cfg.typeB[0][0] = vacgpu.CONT
     m2 = vacgpu.MethodSpec(typefull="tvdlf").to_c()

@@ -232,3 +232,58 @@ def test_dropin_stops_on_a_user_source():
         rc = run_ref("vac_mhd22_132", make_ini(w0, coords, "mhd", 5 / 3, name="dropin"), par,
                      timeout=300)
         assert np.array_equal(rc.snapshots[-1].w, rg.snapshots[-1].w)
+
+
+W0X = ["symm0", "asymm", "symm0", "symm0", "asymm", "symm0"]
+W0Y = ["symm0", "symm0", "asymm", "symm0", "symm0", "asymm"]
+MORE_CASES = [
+    # (id, cpu, gpu, physics, builder, gamma, steps, typefull, limiter, bc, &methodlist, &paramlist)
+    ("tvdlf-symm0-walls", "vac_mhd22_132", "vacgpu_mhd22_132", "mhd",
+     lambda g: random_fourier((40, 36), 86, g), 5 / 3, 12, "tvdlf", "minmod",
+     [W0X, W0X, "periodic", "periodic"], "", ""),
+    ("tvd-symm0-walls", "vac_mhd22_132", "vacgpu_mhd22_132", "mhd",
+     lambda g: random_fourier((36, 40), 87, g), 5 / 3, 10, "tvd", "woodward",
+     ["periodic", "periodic", W0Y, W0Y], "", ""),
+    ("tvdmu-unsplit", "vac_mhd22_132", "vacgpu_mhd22_132", "mhd",
+     lambda g: random_fourier((40, 36), 88, g), 5 / 3, 12, "tvdmu", "woodward", "periodic",
+     " dimsplit=F\n", ""),
+    ("tvdmu-unsplit-3d", "vac_mhd33_36", "vacgpu_mhd33_36", "mhd",
+     lambda g: random_fourier((12, 10, 14), 89, g), 5 / 3, 5, "tvdmu", "minmod", "periodic",
+     " dimsplit=F\n", ""),
+    ("tvdlf-prim-previous", "vac_mhd22_132", "vacgpu_mhd22_132", "mhd",
+     lambda g: random_fourier((40, 36), 90, g), 5 / 3, 12, "tvdlf", "woodward", "periodic",
+     " useprimitive=T\n typelimited='previous'\n", ""),
+    ("tvdmu-prim-predictor-hd", "vac_hd22_132", "vacgpu_hd22_132", "hd",
+     lambda g: random_fourier((36, 40), 91, g, physics="hd"), 1.4, 12, "tvdmu", "minmod", "cont",
+     " useprimitive=T\n typelimited='predictor'\n", ""),
+    ("tvdlf-prim-unsplit-3d", "vac_mhd33_36", "vacgpu_mhd33_36", "mhd",
+     lambda g: random_fourier((12, 10, 14), 92, g), 5 / 3, 5, "tvdlf", "minmod", "periodic",
+     " useprimitive=T\n dimsplit=F\n", ""),
+    # the main loop's time-step controls stay VAC's own with the drop-in
+    ("tvd-dtcantgrow-slowsteps", "vac_mhd22_132", "vacgpu_mhd22_132", "mhd",
+     lambda g: random_fourier((40, 36), 93, g), 5 / 3, 12, "tvd", "woodward", "periodic",
+     "", " dtcantgrow=T\n slowsteps=4\n"),
+]
+
+
+@pytest.mark.parametrize("case", MORE_CASES, ids=[c[0] for c in MORE_CASES])
+def test_vac_with_gpu_advect_more_selectors(case):
+    (_, cpu, gpu, physics, build, gamma, steps, typefull, limiter, bc, method, param) = case
+    if not (have_ref(cpu) and have_ref(gpu)):
+        pytest.skip("reference executables not built (oracle/build_ref.sh)")
+    w0, coords = build(gamma)
+    nw = w0.shape[0]
+    ini = make_ini(w0, coords, physics, gamma, name="dropin")
+    typeB = [bc] * (2 * len(coords)) if isinstance(bc, str) else bc
+    par = problem_par(nw, NAMES[nw], steps, typefull=typefull, limiter=limiter, typeB=typeB,
+                      every_step=True, extra_method=method, extra_param=param)
+    print(f"case {case[0]}: {cpu} vs {gpu}", flush=True)
+    rc = run_ref(cpu, ini, par, timeout=300)
+    rg = run_ref(gpu, ini, par, timeout=300)
+    assert len(rg.snapshots) == steps + 1, rg.stdout[-2000:]
+    tc = np.array([s.t for s in rc.snapshots])
+    tg = np.array([s.t for s in rg.snapshots])
+    assert np.array_equal(tc, tg), "time sequence differs"
+    for sc, sg in zip(rc.snapshots, rg.snapshots):
+        assert sc.it == sg.it
+        assert np.array_equal(sc.w, sg.w), f"state differs at it={sc.it}"

@@ -61,18 +61,16 @@ def gpu_run(w0, nx, physics, gamma, steps, method, bc="periodic", precision="fp6
 def test_gpu_matches_reference_fixture(path):
     z = np.load(path, allow_pickle=False)
     name = os.path.basename(path)
-    if any(k in name for k in ("symm0", "slowsteps", "dtcantgrow", "prim_previous",
-                               "prim_predictor", "prim_unsplit", "tvdmu_woodward_unsplit",
-                               "tvdmu_minmod_cont_unsplit", "tvdmu_superbee_unsplit")):
-        pytest.skip("selector not on the GPU path yet")
     nx = [int(v) for v in z["nx"]]
     tl = str(z["typelimited"]) if "typelimited" in z.files else "original"
     te = str(z["typeentropy"]) if "typeentropy" in z.files else "nul"
     up = bool(z["useprimitive"]) if "useprimitive" in z.files else False
     us = bool(z["unsplit"]) if "unsplit" in z.files else False
+    dg = bool(z["dtcantgrow"]) if "dtcantgrow" in z.files else False
+    ss = int(z["slowsteps"]) if "slowsteps" in z.files else 0
     method = vacgpu.MethodSpec(typefull=str(z["typefull"]), limiter=str(z["limiter"]),
                                divbfix=bool(z["divbfix"]), typelimited=tl, typeentropy=te,
-                               useprimitive=up, unsplit=us)
+                               useprimitive=up, unsplit=us, dtcantgrow=dg, slowsteps=ss)
     print(f"case {os.path.basename(path)} nx={nx}", flush=True)
     w, t, sc = gpu_run(z["w0"], nx, str(z["physics"]), float(z["gamma"]), int(z["steps"]),
                        method, bc=vacref.bc_spec(z["bc"]))
@@ -325,17 +323,46 @@ def test_gpu_useprimitive_matches_oracle(case):
 
 
 def test_gpu_useprimitive_rejections():
+    # primitive-variable limiting runs in FP64 only
     nx = [16, 16]
     w0, coords = random_fourier((16, 16), 1, 1.4)
     dxs = [widths_1d(c) for c in coords]
-    for kw, prec in (({"typefull": "tvdlf", "typelimited": "previous"}, "fp64"),
-                     ({"typefull": "tvdmu", "typelimited": "predictor"}, "fp64"),
-                     ({"typefull": "tvdlf"}, "fp32bulk")):
-        g = vacgpu.Grid(nx, "mhd", 1.4, dxs, precision=prec)
+    for kw in ({"typefull": "tvdlf"}, {"typefull": "tvdmu", "typelimited": "previous"}):
+        g = vacgpu.Grid(nx, "mhd", 1.4, dxs, precision="fp32bulk")
         g.put_mesh(w0)
         with pytest.raises(vacgpu.VacGpuError):
             g.run(vacgpu.MethodSpec(useprimitive=True, **kw), 1)
         g.close()
+
+
+PRIM_TL = [
+    # (physics, shape, typefull, limiter, bc, typelimited, unsplit, steps): useprimitive with
+    # every limiter base, split and unsplit, against the oracle
+    ("mhd", (56, 48), "tvdlf", "superbee", "periodic", "previous", False, 10),
+    ("mhd", (48, 56), "tvdmu", "woodward", "cont", "predictor", False, 10),
+    ("mhd", (12, 14, 10), "tvdmu", "minmod", "periodic", "previous", False, 5),
+    ("hd", (56, 44), "tvdlf", "woodward", "cont", "predictor", False, 12),
+    ("mhd", (48, 52), "tvdmu", "superbee", "periodic", "original", True, 10),
+    ("mhd", (14, 10, 12), "tvdlf", "woodward", "cont", "predictor", True, 5),
+]
+
+
+@pytest.mark.parametrize("case", PRIM_TL, ids=[f"{c[2]}-{c[5]}-{len(c[1])}d{'-unsplit' if c[6] else ''}"
+                                               for c in PRIM_TL])
+def test_gpu_useprimitive_limiter_bases_match_oracle(case):
+    physics, shape, typefull, limiter, bc, tl, unsplit, steps = case
+    g = 5 / 3
+    w0, coords = random_fourier(shape, 77, g, physics=physics)
+    nx = list(shape)
+    method = vacgpu.MethodSpec(typefull=typefull, limiter=limiter, typelimited=tl,
+                               useprimitive=True, unsplit=unsplit)
+    print(f"case prim {case}", flush=True)
+    w, t, sc = gpu_run(w0, nx, physics, g, steps, method, bc=bc)
+    p = vacref.make_params(len(nx), physics, nx, g, typefull=typefull, limiter=limiter, bc=bc,
+                           typelimited=tl, useprimitive=True, unsplit=unsplit)
+    wr, tr, st = vacref.run(p, w0, [widths_1d(c) for c in coords], steps)
+    assert np.array_equal(t, tr)
+    assert_bitwise(w, wr, f"prim {case}")
 
 
 @pytest.mark.parametrize("case", [
@@ -384,16 +411,44 @@ def test_gpu_unsplit_through_advect_matches_run():
 
 
 def test_gpu_unsplit_rejections():
+    # one-step TVD unsplit is not on the path (VAC warns against it)
     nx = [16, 16]
     w0, coords = random_fourier((16, 16), 1, 1.4)
     dxs = [widths_1d(c) for c in coords]
     g = vacgpu.Grid(nx, "mhd", 1.4, dxs)
     g.put_mesh(w0)
-    for kw in ({"typefull": "tvd"}, {"typefull": "tvdmu"},
-               {"typefull": "tvdlf", "useprimitive": True}):
-        with pytest.raises(vacgpu.VacGpuError):
-            g.run(vacgpu.MethodSpec(unsplit=True, **kw), 1)
+    with pytest.raises(vacgpu.VacGpuError):
+        g.run(vacgpu.MethodSpec(unsplit=True, typefull="tvd"), 1)
     g.close()
+
+
+MORE = [
+    # (physics, shape, typefull, limiter, bc, method keywords, steps): symm0 walls, unsplit
+    # TVD-MUSCL and the device loop's time-step controls against the oracle
+    ("mhd", (52, 44), "tvdmu", "woodward", [["symm0", "asymm", "symm0", "symm0", "asymm", "symm0"]] * 2 + ["periodic", "periodic"], {}, 10),
+    ("mhd", (12, 10, 14), "tvd", "woodward", ["periodic", "periodic"] + [["symm0", "symm0", "asymm", "symm0", "symm0", "symm0", "asymm", "symm0"]] * 2 + ["cont", "cont"], {}, 5),
+    ("hd", (48, 56), "tvdlf", "superbee", ["cont", "symm0"] + [["symm0", "symm0", "asymm", "symm0"]] * 2, {}, 12),
+    ("mhd", (56, 48), "tvdmu", "superbee", "cont", {"unsplit": True, "typelimited": "predictor"}, 10),
+    ("hd", (12, 14, 10), "tvdmu", "woodward", "periodic", {"unsplit": True}, 6),
+    ("mhd", (48, 48), "tvdlf", "woodward", "periodic", {"slowsteps": 6, "dtcantgrow": True}, 12),
+    ("mhd", (12, 10, 14), "tvd", "minmod", "cont", {"slowsteps": 3}, 6),
+]
+
+
+@pytest.mark.parametrize("case", MORE, ids=[f"{c[2]}-{len(c[1])}d-{i}" for i, c in enumerate(MORE)])
+def test_gpu_more_selectors_match_oracle(case):
+    physics, shape, typefull, limiter, bc, kw, steps = case
+    g = 1.4
+    w0, coords = random_fourier(shape, 78, g, physics=physics)
+    nx = list(shape)
+    method = vacgpu.MethodSpec(typefull=typefull, limiter=limiter, **kw)
+    print(f"case more selectors {case}", flush=True)
+    w, t, sc = gpu_run(w0, nx, physics, g, steps, method, bc=bc)
+    p = vacref.make_params(len(nx), physics, nx, g, typefull=typefull, limiter=limiter, bc=bc,
+                           **kw)
+    wr, tr, st = vacref.run(p, w0, [widths_1d(c) for c in coords], steps)
+    assert np.array_equal(t, tr)
+    assert_bitwise(w, wr, f"{case}")
 
 
 @pytest.mark.parametrize("case", [

@@ -35,13 +35,37 @@ FLAGS = [
 
 
 def build(verbose: bool = False) -> str:
-    cmd = [HIPCC, *FLAGS, *SRCS, "-o", OUT + ".tmp"]
+    """Compiles the translation units in parallel (one hipcc each), then links."""
+    objdir = os.path.join(HERE, "_build")
+    os.makedirs(objdir, exist_ok=True)
+    cflags = [f for f in FLAGS if f != "-shared"]
+    procs = []
+    objs = []
+    for src in SRCS:
+        obj = os.path.join(objdir, os.path.basename(src) + ".o")
+        objs.append(obj)
+        cmd = [HIPCC, *cflags, "-c", src, "-o", obj + ".tmp"]
+        if verbose:
+            print(" ".join(cmd))
+        procs.append((src, obj, subprocess.Popen(cmd, stdout=subprocess.PIPE,
+                                                 stderr=subprocess.STDOUT, text=True)))
+    failed = False
+    for src, obj, p in procs:
+        out, _ = p.communicate()
+        if p.returncode != 0:
+            sys.stderr.write(out)
+            failed = True
+        else:
+            os.replace(obj + ".tmp", obj)
+    if failed:
+        raise RuntimeError("hipcc failed for libvacgpu.so")
+    cmd = [HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", *objs, "-o", OUT + ".tmp"]
     if verbose:
         print(" ".join(cmd))
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         sys.stderr.write(r.stdout + r.stderr)
-        raise RuntimeError("hipcc failed for libvacgpu.so")
+        raise RuntimeError("linking libvacgpu.so failed")
     os.replace(OUT + ".tmp", OUT)
     return OUT
 

@@ -698,6 +698,8 @@ int vacgpu_batch_add(vacgpu_batch* b, const vacgpu_config* cfg, const vacgpu_met
   if (m->typelimited != VACGPU_LIM_ORIGINAL) return bfail(b, VACGPU_E_ARG, "the batch engine runs typelimited='original'");
   if (m->useprimitive) return bfail(b, VACGPU_E_ARG, "the batch engine limits the conserved variables (useprimitive=F)");
   if (m->unsplit) return bfail(b, VACGPU_E_ARG, "the batch engine runs dimensionally split sweeps");
+  if (m->dtcantgrow || m->slowsteps)
+    return bfail(b, VACGPU_E_ARG, "the batch engine's time step has no dtcantgrow / slowsteps");
   if (m->courantpar <= 0.0) return bfail(b, VACGPU_E_ARG, "the batch engine needs courantpar > 0");
   if (vacgpu_supported(cfg, m) != 0) return bfail(b, VACGPU_E_ARG, "unsupported selectors (see vacgpu_supported)");
   for (int f = 0; f < 4; ++f)

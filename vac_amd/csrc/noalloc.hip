@@ -60,6 +60,11 @@ __global__ void k_noalloc_scal(Scal* sc, double smallp, double dtc0, double dtc1
   for (int i = 0; i < 4; ++i) sc->nerror[i] = 0;
   sc->it = 0;
   sc->tmaxexact = 0;
+  sc->dtold = 0.0;
+  sc->dtchk = 0.0;
+  sc->itmin = 0;
+  sc->dtcantgrow = 0;
+  sc->slowsteps = 0;
 }
 
 int say(char* err, int errlen, const char* msg) {

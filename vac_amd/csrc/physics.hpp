@@ -152,6 +152,31 @@ __device__ __forceinline__ void prim_of_roundtrip(double* u, double g, double sm
   to_prim<ND, MHD>(q, g, smallp, u);
 }
 
+// n of those round trips in place: a state after n upwindLR calls on it
+template <int ND, bool MHD>
+__device__ __forceinline__ void roundtrips(double* u, int n, double g, double smallp) {
+  constexpr int NW = Layout<ND, MHD>::nw;
+  for (int k = 0; k < n; ++k) {
+    double q[NW];
+    to_prim<ND, MHD>(u, g, smallp, q);
+    to_cons<ND, MHD>(q, g);
+#pragma unroll
+    for (int iw = 0; iw < NW; ++iw) u[iw] = q[iw];
+  }
+}
+
+// the primitive variables of a state after n round trips, in place (n = 1
+// is prim_of_roundtrip)
+template <int ND, bool MHD>
+__device__ __forceinline__ void prim_after(double* u, int n, double g, double smallp) {
+  constexpr int NW = Layout<ND, MHD>::nw;
+  roundtrips<ND, MHD>(u, n, g, smallp);
+  double q[NW];
+  to_prim<ND, MHD>(u, g, smallp, q);
+#pragma unroll
+  for (int iw = 0; iw < NW; ++iw) u[iw] = q[iw];
+}
+
 // Total pressure (src/vacphys0.t.mhd:244-252), unclipped, in double.
 template <int ND>
 __device__ __forceinline__ double ptotal(const double* u, double g) {

@@ -38,7 +38,31 @@ struct Sel {
   double ecoef[8];   // entropycoef per wave ('yee')
   int prim;          // useprimitive
   int srcmul;        // idimmax-idimmin+1 of the sweep carrying the source (TVDLF correctors)
+  // useprimitive: every upwindLR call converts its base state to primitive
+  // variables and back in place (src/vactvdlf.t:328-364), so a corrector
+  // reads states that earlier calls of the same advect have round-tripped:
+  // the limiter's base rtw times, the update's base (w at the cell) rtb
+  // times, the w1 of the face states rtu times and the w1 of the source
+  // terms rts times
+  int rtw, rtb, rtu, rts;
+  // 'symm0' faces: bit iw of nf[2*d + upper] set when variable iw's flux
+  // through that face is zero (setnoflux, src/vacgrid.t:416-449); read by
+  // the general kernel variants only (LIM < 0 or ENT), which the host
+  // selects when any face has it
+  unsigned char nf[6];
 };
+
+// the variables whose flux through face J (between cells J and J+1) along d
+// is zero: J = 1 is the lower mesh boundary face, n+1 the upper one
+__device__ __forceinline__ unsigned noflux(const Sel& s, const Geo& g, int d, int J) {
+  return (J == 1 ? unsigned(s.nf[2 * d]) : 0u) | (J == g.n[d] + 1 ? unsigned(s.nf[2 * d + 1]) : 0u);
+}
+template <int NW>
+__device__ __forceinline__ void zero_flux(double* F, unsigned mask) {
+#pragma unroll
+  for (int iw = 0; iw < NW; ++iw)
+    if ((mask >> iw) & 1u) F[iw] = 0.0;
+}
 
 // boundary types per (face, variable): t = 1 for 'periodic'; k the kind of a
 // non-periodic face: a copy of the edge ('cont'), a mirror image ('symm', or
@@ -56,6 +80,11 @@ struct Scal {
   double tmax;
   int nerror[4];
   int it, tmaxexact;
+  // main-loop time-step controls (src/vac.t:60-81): the previous Courant
+  // step (dtcantgrow), the run's first it (slowsteps), and dt before the
+  // slowsteps and tmaxexact factors, which VAC's dt < dtmin test reads
+  double dtold, dtchk;
+  int itmin, dtcantgrow, slowsteps;
 };
 
 struct CellIx {
@@ -271,7 +300,8 @@ template <int ND, bool MHD, class R, int D, int LIM, int PWM, bool PRIM = false>
 __device__ __forceinline__ void pred_compute(const Sel& s, const double* uc, const double* um,
                                              const double* up, const double (*bn)[2],
                                              const double* dxk, double dt, double smallp,
-                                             double* out, double* lout) {
+                                             double* out, double* lout, unsigned nfl = 0u,
+                                             unsigned nfr = 0u) {
   constexpr int d = D;
   using L = Layout<ND, MHD>;
   constexpr int NW = L::nw;
@@ -306,6 +336,11 @@ __device__ __forceinline__ void pred_compute(const Sel& s, const double* uc, con
   double fL[NW], fR[NW];
   fluxes<ND, MHD, R>(wl, d, s.gamma, smallp, fL);
   fluxes<ND, MHD, R>(wr, d, s.gamma, smallp, fR);
+  // 'symm0': fL is the flux through the cell's right face, fR its left one's
+  if (nfl | nfr) {
+    zero_flux<NW>(fR, nfl);
+    zero_flux<NW>(fL, nfr);
+  }
   const double qdx = pdiv(qdt, dxk[d]);
 #pragma unroll
   for (int iw = 0; iw < NW; ++iw) out[iw] = uc[iw] - qdx * (fL[iw] - fR[iw]);
@@ -351,7 +386,11 @@ __device__ __forceinline__ void pred_point(const Geo& g, const Sel& s, const dou
       }
     }
   }
-  pred_compute<ND, MHD, R, D, LIM, PWM, PRIM>(s, uc, um, up, bn, dxk, dt, smallp, out, nullptr);
+  // 'symm0' faces at the mesh boundary (general variant only)
+  const unsigned nfl = LIM < 0 ? noflux(s, g, D, c[D] - 1) : 0u;
+  const unsigned nfr = LIM < 0 ? noflux(s, g, D, c[D]) : 0u;
+  pred_compute<ND, MHD, R, D, LIM, PWM, PRIM>(s, uc, um, up, bn, dxk, dt, smallp, out, nullptr,
+                                              nfl, nfr);
 }
 
 template <int ND, bool MHD, class R, int D>
@@ -639,20 +678,23 @@ __global__ __launch_bounds__(256) void k_tvdlf_corrx(Geo g, Sel s, BC bc, const 
   double l[NW], lr[NW], u0[NW], u1[NW];
   [[maybe_unused]] double ub[NW];
   if constexpr (PRIM) {
-    // useprimitive ('original' only): limit prim(cons(prim(w))) and update
-    // cons of it, the state the predictor's conversion left
+    // useprimitive: limit the primitive variables of wl after its round
+    // trips, update w after its own, face states from w1 after its own
     double qa[NW];
     load<NW>(wl, g.ps, idx - 1, qa);
     load<NW>(wl, g.ps, idx, ub);
     load<NW>(wl, g.ps, idx + 1, u1);
-    prim_of_roundtrip<ND, MHD>(qa, s.gamma, smallp);
-    prim_of_roundtrip<ND, MHD>(ub, s.gamma, smallp);
-    prim_of_roundtrip<ND, MHD>(u1, s.gamma, smallp);
+    prim_after<ND, MHD>(qa, s.rtw, s.gamma, smallp);
+    prim_after<ND, MHD>(ub, s.rtw, s.gamma, smallp);
+    prim_after<ND, MHD>(u1, s.rtw, s.gamma, smallp);
 #pragma unroll
     for (int iw = 0; iw < NW; ++iw) l[iw] = slope<R, LIM>(s.lim[iw], qa[iw], ub[iw], u1[iw]);
-    to_cons<ND, MHD>(ub, s.gamma);
+    load<NW>(w, g.ps, idx, ub);
+    roundtrips<ND, MHD>(ub, s.rtb, s.gamma, smallp);
     load<NW>(w1, g.ps, idx, u0);
     load<NW>(w1, g.ps, idx + 1, u1);
+    roundtrips<ND, MHD>(u0, s.rtu, s.gamma, smallp);
+    roundtrips<ND, MHD>(u1, s.rtu, s.gamma, smallp);
   } else {
 #pragma unroll
     for (int iw = 0; iw < NW; ++iw) {
@@ -666,6 +708,7 @@ __global__ __launch_bounds__(256) void k_tvdlf_corrx(Geo g, Sel s, BC bc, const 
   for (int iw = 0; iw < NW; ++iw) lr[iw] = lane_next(l[iw]);
   double F[NW], cv;
   corr_face<ND, MHD, R, D, PRIM>(s, u0, u1, l, lr, g.dx[0][cc] + g.dx[0][cc + 1], smallp, F, cv);
+  if constexpr (LIM < 0) zero_flux<NW>(F, noflux(s, g, 0, cc));
   double cmx = face_ok ? cv : -INFINITY;
   double out[NW];
   const double qdx = pdiv(dt, g.dx[0][cc]);
@@ -678,7 +721,15 @@ __global__ __launch_bounds__(256) void k_tvdlf_corrx(Geo g, Sel s, BC bc, const 
     if (s.sourceunsplit && s.divbfix) {
       const int c3[3] = {cc, yc, z};
       const double dv = divb_at<ND>(g, w1, idx, c3);
-      powell<ND>(u0, dv, dt * double(s.srcmul) / ND, out);
+      if constexpr (PRIM) {
+        double us[NW];
+#pragma unroll
+        for (int iw = 0; iw < NW; ++iw) us[iw] = u0[iw];
+        roundtrips<ND, MHD>(us, s.rts - s.rtu, s.gamma, smallp);
+        powell<ND>(us, dv, dt * double(s.srcmul) / ND, out);
+      } else {
+        powell<ND>(u0, dv, dt * double(s.srcmul) / ND, out);
+      }
     }
   }
   if (useful) {
@@ -718,29 +769,35 @@ __global__ __launch_bounds__(256) void k_tvdlf_corrm(Geo g, Sel s, BC bc, const 
     // window: the limiter's base wl (typelimited) at J-1, J, J+1 (for the
     // slope at J) and w1 at J; slope at J.  The update's base is w, the
     // window itself when wl is w.
-    // With PRIM (useprimitive, 'original') the window holds
-    // prim(cons(prim(w))), the update's base is cons of it.
+    // With PRIM (useprimitive) the window holds the primitive variables of
+    // wl after its round trips (s.rtw), the face states come from w1 after
+    // its own (s.rtu) and the update's base is w after its own (s.rtb).
     const bool own = wl == w;
     auto lwin = [&](long long at, double* v) {
       load<NW>(wl, g.ps, at, v);
-      if constexpr (PRIM) prim_of_roundtrip<ND, MHD>(v, s.gamma, smallp);
+      if constexpr (PRIM) prim_after<ND, MHD>(v, s.rtw, s.gamma, smallp);
+    };
+    auto lw1 = [&](long long at, double* v) {
+      load<NW>(w1, g.ps, at, v);
+      if constexpr (PRIM) roundtrips<ND, MHD>(v, s.rtu, s.gamma, smallp);
     };
     double wa[NW], wb[NW], wc[NW], u0[NW], lc[NW], Fp[NW];
     lwin(base + (long long)(s0 - 2) * sd, wa);
     lwin(base + (long long)(s0 - 1) * sd, wb);
     lwin(base + (long long)s0 * sd, wc);
-    load<NW>(w1, g.ps, base + (long long)(s0 - 1) * sd, u0);
+    lw1(base + (long long)(s0 - 1) * sd, u0);
 #pragma unroll
     for (int iw = 0; iw < NW; ++iw) lc[iw] = slope<R, LIM>(s.lim[iw], wa[iw], wb[iw], wc[iw]);
     // face s0-1/2
     {
       double u1[NW], ln[NW], wd[NW];
-      load<NW>(w1, g.ps, base + (long long)s0 * sd, u1);
+      lw1(base + (long long)s0 * sd, u1);
       lwin(base + (long long)(s0 + 1) * sd, wd);
 #pragma unroll
       for (int iw = 0; iw < NW; ++iw) ln[iw] = slope<R, LIM>(s.lim[iw], wb[iw], wc[iw], wd[iw]);
       double cv;
       corr_face<ND, MHD, R, D, PRIM>(s, u0, u1, lc, ln, dxd[s0 - 1] + dxd[s0], smallp, Fp, cv);
+      if constexpr (LIM < 0) zero_flux<NW>(Fp, noflux(s, g, D, s0 - 1));
       cmx = cmx < cv ? cv : cmx;
 #pragma unroll
       for (int iw = 0; iw < NW; ++iw) {
@@ -754,20 +811,20 @@ __global__ __launch_bounds__(256) void k_tvdlf_corrm(Geo g, Sel s, BC bc, const 
     // now: u0 = w1(i), lc = slope(i), wb = w(i), wc = w(i+1)
     for (int i = s0; i <= s1; ++i) {
       double u1[NW], ln[NW], wd[NW], F[NW], cv;
-      load<NW>(w1, g.ps, base + (long long)(i + 1) * sd, u1);
+      lw1(base + (long long)(i + 1) * sd, u1);
       lwin(base + (long long)(i + 2) * sd, wd);
 #pragma unroll
       for (int iw = 0; iw < NW; ++iw) ln[iw] = slope<R, LIM>(s.lim[iw], wb[iw], wc[iw], wd[iw]);
       corr_face<ND, MHD, R, D, PRIM>(s, u0, u1, lc, ln, dxd[i] + dxd[i + 1], smallp, F, cv);
+      if constexpr (LIM < 0) zero_flux<NW>(F, noflux(s, g, D, i));
       cmx = cmx < cv ? cv : cmx;
       const long long idx = base + (long long)i * sd;
       const double qdx = pdiv(dt, dxd[i]);
       double out[NW];
       [[maybe_unused]] double ub[NW];
       if constexpr (PRIM) {
-#pragma unroll
-        for (int iw = 0; iw < NW; ++iw) ub[iw] = wb[iw];
-        to_cons<ND, MHD>(ub, s.gamma);
+        load<NW>(w, g.ps, idx, ub);
+        roundtrips<ND, MHD>(ub, s.rtb, s.gamma, smallp);
       }
 #pragma unroll
       for (int iw = 0; iw < NW; ++iw)
@@ -776,7 +833,15 @@ __global__ __launch_bounds__(256) void k_tvdlf_corrm(Geo g, Sel s, BC bc, const 
         if (s.sourceunsplit && s.divbfix) {
           c3[D] = i;
           const double dv = divb_at<ND>(g, w1, idx, c3);
-          powell<ND>(u0, dv, dt * double(s.srcmul) / ND, out);
+          if constexpr (PRIM) {
+            double us[NW];
+#pragma unroll
+            for (int iw = 0; iw < NW; ++iw) us[iw] = u0[iw];
+            roundtrips<ND, MHD>(us, s.rts - s.rtu, s.gamma, smallp);
+            powell<ND>(us, dv, dt * double(s.srcmul) / ND, out);
+          } else {
+            powell<ND>(u0, dv, dt * double(s.srcmul) / ND, out);
+          }
         }
       }
       c3[D] = i;
@@ -806,8 +871,11 @@ __global__ __launch_bounds__(256) void k_tvdlf_corrm(Geo g, Sel s, BC bc, const 
 // per direction, each adding its direction's differences to the previous
 // one's result.
 
-// the predictor's flux difference of direction D at a cell, subtracted from out
-template <int ND, bool MHD, class R, int D>
+// the predictor's flux difference of direction D at a cell, subtracted from out.
+// PRIM (useprimitive): hancock's upwindLR converts its base in place for
+// every direction (src/vactvdlf.t:33-45), so direction D (the D-th in turn)
+// limits and reconstructs the primitive variables of w after D round trips.
+template <int ND, bool MHD, class R, int D, bool PRIM = false>
 __device__ __forceinline__ void pred_dir(const Geo& g, const Sel& s, const double* __restrict__ w,
                                          long long idx, const int* c, const double* uc, double qdt,
                                          double smallp, double* out, double (*bn)[2]) {
@@ -820,21 +888,45 @@ __device__ __forceinline__ void pred_dir(const Geo& g, const Sel& s, const doubl
     bn[D][0] = um[L::b(D)];
     bn[D][1] = up[L::b(D)];
   }
+  if constexpr (PRIM) {
+    double qc[NW], qm[NW], qp[NW];
 #pragma unroll
-  for (int iw = 0; iw < NW; ++iw) {
-    const R l = limiter_t<-1, R>(s.lim[iw], R(up[iw] - uc[iw]), R(uc[iw] - um[iw]));
-    wr[iw] = uc[iw] - 0.5 * double(l);
-    wl[iw] = uc[iw] + 0.5 * double(l);
+    for (int iw = 0; iw < NW; ++iw) qc[iw] = uc[iw];
+    prim_after<ND, MHD>(qc, D, s.gamma, smallp);
+    prim_after<ND, MHD>(um, D, s.gamma, smallp);
+    prim_after<ND, MHD>(up, D, s.gamma, smallp);
+#pragma unroll
+    for (int iw = 0; iw < NW; ++iw) {
+      qm[iw] = um[iw];
+      qp[iw] = up[iw];
+    }
+#pragma unroll
+    for (int iw = 0; iw < NW; ++iw) {
+      const R l = limiter_t<-1, R>(s.lim[iw], R(qp[iw] - qc[iw]), R(qc[iw] - qm[iw]));
+      wr[iw] = qc[iw] - 0.5 * double(l);
+      wl[iw] = qc[iw] + 0.5 * double(l);
+    }
+    to_cons<ND, MHD>(wr, s.gamma);
+    to_cons<ND, MHD>(wl, s.gamma);
+  } else {
+#pragma unroll
+    for (int iw = 0; iw < NW; ++iw) {
+      const R l = limiter_t<-1, R>(s.lim[iw], R(up[iw] - uc[iw]), R(uc[iw] - um[iw]));
+      wr[iw] = uc[iw] - 0.5 * double(l);
+      wl[iw] = uc[iw] + 0.5 * double(l);
+    }
   }
   double fL[NW], fR[NW];
   fluxes<ND, MHD, R>(wl, D, s.gamma, smallp, fL);
   fluxes<ND, MHD, R>(wr, D, s.gamma, smallp, fR);
+  zero_flux<NW>(fR, noflux(s, g, D, c[D] - 1));  // 'symm0' faces
+  zero_flux<NW>(fL, noflux(s, g, D, c[D]));
   const double qdx = pdiv(qdt, g.dx[D][c[D]]);
 #pragma unroll
   for (int iw = 0; iw < NW; ++iw) out[iw] = out[iw] - qdx * (fL[iw] - fR[iw]);
 }
 
-template <int ND, bool MHD, class R>
+template <int ND, bool MHD, class R, bool PRIM = false>
 __global__ __launch_bounds__(256) void k_tvdlf_pred_u(Geo g, Sel s, BC bc, const double* __restrict__ w,
                                                      double* __restrict__ w1,
                                                      const Scal* __restrict__ sc, int ghosts) {
@@ -847,18 +939,48 @@ __global__ __launch_bounds__(256) void k_tvdlf_pred_u(Geo g, Sel s, BC bc, const
   load<NW>(w, g.ps, cl.idx, uc);
 #pragma unroll
   for (int iw = 0; iw < NW; ++iw) out[iw] = uc[iw];
-  pred_dir<ND, MHD, R, 0>(g, s, w, cl.idx, cl.c, uc, qdt, smallp, out, bn);
-  pred_dir<ND, MHD, R, 1>(g, s, w, cl.idx, cl.c, uc, qdt, smallp, out, bn);
-  if constexpr (ND == 3) pred_dir<ND, MHD, R, 2>(g, s, w, cl.idx, cl.c, uc, qdt, smallp, out, bn);
+  pred_dir<ND, MHD, R, 0, PRIM>(g, s, w, cl.idx, cl.c, uc, qdt, smallp, out, bn);
+  pred_dir<ND, MHD, R, 1, PRIM>(g, s, w, cl.idx, cl.c, uc, qdt, smallp, out, bn);
+  if constexpr (ND == 3) pred_dir<ND, MHD, R, 2, PRIM>(g, s, w, cl.idx, cl.c, uc, qdt, smallp, out, bn);
   if constexpr (MHD) {
     if (s.sourceunsplit && s.divbfix) {
       double dv = 0.0;
 #pragma unroll
       for (int k = 0; k < ND; ++k) dv = dv + pdiv(0.5 * (bn[k][1] - bn[k][0]), g.dx[k][cl.c[k]]);
-      powell<ND>(uc, dv, qdt * double(ND) / ND, out);
+      if constexpr (PRIM) {
+        // the source sees the state after every direction's round trip
+        double us[NW];
+#pragma unroll
+        for (int iw = 0; iw < NW; ++iw) us[iw] = uc[iw];
+        roundtrips<ND, MHD>(us, ND, s.gamma, smallp);
+        powell<ND>(us, dv, qdt * double(ND) / ND, out);
+      } else {
+        powell<ND>(uc, dv, qdt * double(ND) / ND, out);
+      }
     }
   }
   store_cell<ND, NW>(g, bc, w1, cl.idx, cl.c, out, ghosts);
+}
+
+// useprimitive with the limiter's base wold: upwindLR converts wold to
+// primitive variables and back in place over ixG every time it limits it
+// (src/vactvdlf.t:328-332, :360-364); one thread per ixG cell
+template <int ND, bool MHD>
+__global__ __launch_bounds__(256) void k_roundtrip(Geo g, double* __restrict__ w, double gamma,
+                                                  const Scal* __restrict__ sc) {
+  constexpr int NW = Layout<ND, MHD>::nw;
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long n = (long long)g.ng[0] * g.ng[1] * (ND == 3 ? g.ng[2] : 1);
+  if (t >= n) return;
+  const int x = (int)(t % g.ng[0]);
+  const int y = (int)((t / g.ng[0]) % g.ng[1]);
+  const int z = ND == 3 ? (int)(t / ((long long)g.ng[0] * g.ng[1])) : 0;
+  const long long idx = g.off + x + (long long)y * g.s[1] + (long long)z * g.s[2];
+  double u[NW];
+  load<NW>(w, g.ps, idx, u);
+  roundtrips<ND, MHD>(u, 1, gamma, sc->smallp);
+#pragma unroll
+  for (int iw = 0; iw < NW; ++iw) w[iw * g.ps + idx] = u[iw];
 }
 
 // Wait for every vector memory load in flight (vmcnt(0); expcnt and lgkmcnt
@@ -1381,17 +1503,22 @@ __device__ __forceinline__ void tvd_face(const double* uL, const double* uR, int
 }
 
 // out[iw] += sum over il (in order) of (R.phi(J) q(J) - R.phi(J-1) q(J-1)) / dV(i)
+// nfm / nf0: the variables whose dissipative flux through face J-1 / J is
+// zero ('symm0' faces)
 template <int ND, class R, bool MHD = true>
 __device__ __forceinline__ void tvd_dissipate(const typename RoeOf<ND, R, MHD>::Rt& rm, const R* phim,
                                               const typename RoeOf<ND, R, MHD>::Rt& r0, const R* phi0, int d,
-                                              const Sel& s, R dvm, R dv0, double dvi, double* out) {
+                                              const Sel& s, R dvm, R dv0, double dvi, double* out,
+                                              unsigned nfm = 0u, unsigned nf0 = 0u) {
   constexpr int NW = Layout<ND, MHD>::nw;
 #pragma unroll
   for (int il = 0; il < NW; ++il) {
 #pragma unroll
     for (int iw = 0; iw < NW; ++iw) {
-      const R a0 = roe_rtimes(rm, phim[il], iw, il, d, s.divbwave) * R(0.25) * dvm;
-      const R a1 = roe_rtimes(r0, phi0[il], iw, il, d, s.divbwave) * R(0.25) * dv0;
+      R a0 = roe_rtimes(rm, phim[il], iw, il, d, s.divbwave) * R(0.25) * dvm;
+      R a1 = roe_rtimes(r0, phi0[il], iw, il, d, s.divbwave) * R(0.25) * dv0;
+      if ((nfm >> iw) & 1u) a0 = R(0);
+      if ((nf0 >> iw) & 1u) a1 = R(0);
       out[iw] = out[iw] + pdiv(double(a1) - double(a0), dvi);
     }
   }
@@ -1401,15 +1528,18 @@ __device__ __forceinline__ void tvd_dissipate(const typename RoeOf<ND, R, MHD>::
 // term of cell J is lane L-1's face term: a0(il, iw) of this lane is a1(il, iw)
 // of the previous lane (the same R.phi product, scaled by the same sum
 // dvol(J) + dvol(J-1)), so it is taken from there instead of recomputed.
+// nf0: the variables whose dissipative flux through this lane's face is zero
+// ('symm0' faces; the next lane takes the zero as its face J-1 term)
 template <int ND, class R, bool MHD = true>
 __device__ __forceinline__ void tvd_dissipate_x(const typename RoeOf<ND, R, MHD>::Rt& r0, const R* phi0, const Sel& s,
-                                                R dv0, double dvi, double* out) {
+                                                R dv0, double dvi, double* out, unsigned nf0 = 0u) {
   constexpr int NW = Layout<ND, MHD>::nw;
 #pragma unroll
   for (int il = 0; il < NW; ++il) {
 #pragma unroll
     for (int iw = 0; iw < NW; ++iw) {
-      const R a1 = roe_rtimes(r0, phi0[il], iw, il, 0, s.divbwave) * R(0.25) * dv0;
+      R a1 = roe_rtimes(r0, phi0[il], iw, il, 0, s.divbwave) * R(0.25) * dv0;
+      if ((nf0 >> iw) & 1u) a1 = R(0);
       const R a0 = lane_prev(a1);
       out[iw] = out[iw] + pdiv(double(a1) - double(a0), dvi);
     }
@@ -1491,20 +1621,23 @@ __global__ __launch_bounds__(256) void k_tvdmu_x(Geo g, Sel s, BC bc, const doub
   double l[NW], lr[NW], u0[NW], u1[NW], wc[NW];
   const bool own = wl == w;  // the limiter's base (typelimited) is the sweep's input
   if constexpr (PRIM) {
-    // useprimitive ('original' only): limit prim(cons(prim(w))) and update
-    // cons of it, the state the predictor's conversion left
+    // useprimitive: limit the primitive variables of wl after its round
+    // trips, update w after its own, face states from w1 after its own
     double qa[NW];
     load<NW>(wl, g.ps, idx - 1, qa);
     load<NW>(wl, g.ps, idx, wc);
     load<NW>(wl, g.ps, idx + 1, u1);
-    prim_of_roundtrip<ND, MHD>(qa, s.gamma, smallp);
-    prim_of_roundtrip<ND, MHD>(wc, s.gamma, smallp);
-    prim_of_roundtrip<ND, MHD>(u1, s.gamma, smallp);
+    prim_after<ND, MHD>(qa, s.rtw, s.gamma, smallp);
+    prim_after<ND, MHD>(wc, s.rtw, s.gamma, smallp);
+    prim_after<ND, MHD>(u1, s.rtw, s.gamma, smallp);
 #pragma unroll
     for (int iw = 0; iw < NW; ++iw) l[iw] = slope<R, LIM>(s.lim[iw], qa[iw], wc[iw], u1[iw]);
-    to_cons<ND, MHD>(wc, s.gamma);
+    load<NW>(w, g.ps, idx, wc);
+    roundtrips<ND, MHD>(wc, s.rtb, s.gamma, smallp);
     load<NW>(w1, g.ps, idx, u0);
     load<NW>(w1, g.ps, idx + 1, u1);
+    roundtrips<ND, MHD>(u0, s.rtu, s.gamma, smallp);
+    roundtrips<ND, MHD>(u1, s.rtu, s.gamma, smallp);
   } else {
 #pragma unroll
     for (int iw = 0; iw < NW; ++iw) {
@@ -1522,6 +1655,8 @@ __global__ __launch_bounds__(256) void k_tvdmu_x(Geo g, Sel s, BC bc, const doub
   TvdFace<ND, R, MHD, ENT> f;
   R phi[NW];
   mu_face<ND, R, 0, MHD, ENT, PRIM>(s, u0, u1, l, lr, dt, smallp, g.dx[0][cc] + g.dx[0][cc + 1], F, f, phi, c0, c1);
+  const unsigned nff = ENT ? noflux(s, g, 0, cc) : 0u;  // 'symm0' faces (general variant)
+  zero_flux<NW>(F, nff);
   if (!face_ok) { c0 = -INFINITY; c1 = -INFINITY; }
   double out[NW];
   const double qdx = pdiv(dt, g.dx[0][cc]);
@@ -1538,12 +1673,20 @@ __global__ __launch_bounds__(256) void k_tvdmu_x(Geo g, Sel s, BC bc, const doub
     return v;
   };
   const double d0 = dvol(cc), dp = dvol(cc + 1);
-  tvd_dissipate_x<ND, R, MHD>(f.rt, phi, s, R(dp + d0), d0, out);
+  tvd_dissipate_x<ND, R, MHD>(f.rt, phi, s, R(dp + d0), d0, out, nff);
   const int c3[3] = {cc, yc, z};
   if constexpr (MHD) {
     if (s.sourceunsplit && s.divbfix) {
       const double dv = divb_at<ND>(g, w1, idx, c3);
-      powell<ND>(u0, dv, dt * 1.0 / ND, out);
+      if constexpr (PRIM) {
+        double us[NW];
+#pragma unroll
+        for (int iw = 0; iw < NW; ++iw) us[iw] = u0[iw];
+        roundtrips<ND, MHD>(us, s.rts - s.rtu, s.gamma, smallp);
+        powell<ND>(us, dv, dt * double(s.srcmul) / ND, out);
+      } else {
+        powell<ND>(u0, dv, dt * double(s.srcmul) / ND, out);
+      }
     }
   }
   if (useful) store_cell<ND, NW>(g, bc, wout, idx, c3, out, ghosts);
@@ -1593,30 +1736,38 @@ __global__ __launch_bounds__(256) void k_tvdmu_m(Geo g, Sel s, BC bc, const doub
     // window: the limiter's base wl (typelimited) at J, J+1, w1 at J, slope
     // at J; the face J-1/2 below.  The update's base is w, the window itself
     // when wl is w.
-    // With PRIM (useprimitive, 'original') the window holds
-    // prim(cons(prim(w))), the update's base is cons of it.
+    // With PRIM (useprimitive) the window holds the primitive variables of
+    // wl after its round trips (s.rtw), the face states come from w1 after
+    // its own (s.rtu) and the update's base is w after its own (s.rtb).
     const bool own = wl == w;
     auto lwin = [&](long long at, double* v) {
       load<NW>(wl, g.ps, at, v);
-      if constexpr (PRIM) prim_of_roundtrip<ND, MHD>(v, s.gamma, smallp);
+      if constexpr (PRIM) prim_after<ND, MHD>(v, s.rtw, s.gamma, smallp);
+    };
+    auto lw1 = [&](long long at, double* v) {
+      load<NW>(w1, g.ps, at, v);
+      if constexpr (PRIM) roundtrips<ND, MHD>(v, s.rtu, s.gamma, smallp);
     };
     double wb[NW], wc[NW], u0[NW], lc[NW], Fp[NW];
     TvdFace<ND, R, MHD, ENT> fp;
     R phip[NW];
+    unsigned nfp = 0u;  // 'symm0' variables of the face below
     {
       double wa[NW], u1[NW], ln[NW], wd[NW];
       lwin(base + (long long)(s0 - 2) * sd, wa);
       lwin(base + (long long)(s0 - 1) * sd, wb);
       lwin(base + (long long)s0 * sd, wc);
-      load<NW>(w1, g.ps, base + (long long)(s0 - 1) * sd, u0);
+      lw1(base + (long long)(s0 - 1) * sd, u0);
 #pragma unroll
       for (int iw = 0; iw < NW; ++iw) lc[iw] = slope<R, LIM>(s.lim[iw], wa[iw], wb[iw], wc[iw]);
       // face s0-1/2
-      load<NW>(w1, g.ps, base + (long long)s0 * sd, u1);
+      lw1(base + (long long)s0 * sd, u1);
       lwin(base + (long long)(s0 + 1) * sd, wd);
 #pragma unroll
       for (int iw = 0; iw < NW; ++iw) ln[iw] = slope<R, LIM>(s.lim[iw], wb[iw], wc[iw], wd[iw]);
       mu_face<ND, R, D, MHD, ENT, PRIM>(s, u0, u1, lc, ln, dt, smallp, dxd[s0 - 1] + dxd[s0], Fp, fp, phip, a, b);
+      nfp = ENT ? noflux(s, g, D, s0 - 1) : 0u;
+      zero_flux<NW>(Fp, nfp);
       c0 = c0 < a ? a : c0; c1 = c1 < b ? b : c1;
 #pragma unroll
       for (int iw = 0; iw < NW; ++iw) {
@@ -1631,31 +1782,41 @@ __global__ __launch_bounds__(256) void k_tvdmu_m(Geo g, Sel s, BC bc, const doub
       double u1[NW], ln[NW], wd[NW], F[NW];
       TvdFace<ND, R, MHD, ENT> f;
       R phi[NW];
-      load<NW>(w1, g.ps, base + (long long)(i + 1) * sd, u1);
+      lw1(base + (long long)(i + 1) * sd, u1);
       lwin(base + (long long)(i + 2) * sd, wd);
 #pragma unroll
       for (int iw = 0; iw < NW; ++iw) ln[iw] = slope<R, LIM>(s.lim[iw], wb[iw], wc[iw], wd[iw]);
       mu_face<ND, R, D, MHD, ENT, PRIM>(s, u0, u1, lc, ln, dt, smallp, dxd[i] + dxd[i + 1], F, f, phi, a, b);
+      const unsigned nff = ENT ? noflux(s, g, D, i) : 0u;
+      zero_flux<NW>(F, nff);
       c0 = c0 < a ? a : c0; c1 = c1 < b ? b : c1;
       const long long idx = base + (long long)i * sd;
       const double qdx = pdiv(dt, dxd[i]);
       double out[NW];
       [[maybe_unused]] double ub[NW];
       if constexpr (PRIM) {
-#pragma unroll
-        for (int iw = 0; iw < NW; ++iw) ub[iw] = wb[iw];
-        to_cons<ND, MHD>(ub, s.gamma);
+        load<NW>(w, g.ps, idx, ub);
+        roundtrips<ND, MHD>(ub, s.rtb, s.gamma, smallp);
       }
 #pragma unroll
       for (int iw = 0; iw < NW; ++iw)
         out[iw] = (PRIM ? ub[iw] : (own ? wb[iw] : w[iw * g.ps + idx])) - qdx * (F[iw] - Fp[iw]);
       const double dm = dvol(i - 1), d0 = dvol(i), dp = dvol(i + 1);
-      tvd_dissipate<ND, R, MHD>(fp.rt, phip, f.rt, phi, D, s, R(d0 + dm), R(dp + d0), d0, out);
+      tvd_dissipate<ND, R, MHD>(fp.rt, phip, f.rt, phi, D, s, R(d0 + dm), R(dp + d0), d0, out,
+                                nfp, nff);
       c3[D] = i;
       if constexpr (MHD) {
         if (s.sourceunsplit && s.divbfix) {
           const double dv = divb_at<ND>(g, w1, idx, c3);
-          powell<ND>(u0, dv, dt * 1.0 / ND, out);
+          if constexpr (PRIM) {
+            double us[NW];
+#pragma unroll
+            for (int iw = 0; iw < NW; ++iw) us[iw] = u0[iw];
+            roundtrips<ND, MHD>(us, s.rts - s.rtu, s.gamma, smallp);
+            powell<ND>(us, dv, dt * double(s.srcmul) / ND, out);
+          } else {
+            powell<ND>(u0, dv, dt * double(s.srcmul) / ND, out);
+          }
         }
       }
       store_cell<ND, NW>(g, bc, wout, idx, c3, out, ghosts);
@@ -1669,6 +1830,7 @@ __global__ __launch_bounds__(256) void k_tvdmu_m(Geo g, Sel s, BC bc, const doub
         phip[iw] = phi[iw];
       }
       fp = f;
+      nfp = nff;
     }
   }
   const double b0 = block_reduce<true>(c0, sm);
@@ -1721,6 +1883,9 @@ __global__ __launch_bounds__(256) void k_tvd_x(Geo g, Sel s, BC bc, const double
   fluxes<ND, MHD, R>(uL, 0, s.gamma, smallp, F);
 #pragma unroll
   for (int iw = 0; iw < NW; ++iw) fC[iw] = 0.5 * (F[iw] + shfl_down1(F[iw]));
+  // 'symm0' faces (general variant): this lane's face J
+  const unsigned nfJ = ENT ? noflux(s, g, 0, J) : 0u;
+  zero_flux<NW>(fC, nfJ);
 #pragma unroll
   for (int iw = 0; iw < NW; ++iw) fCm[iw] = shfl_up1(fC[iw]);
   // phi at face J from the jumps at J-1, J, J+1
@@ -1754,7 +1919,7 @@ __global__ __launch_bounds__(256) void k_tvd_x(Geo g, Sel s, BC bc, const double
       return v;
     };
     const double d0 = dvol(ic), dp = dvol(ic + 1);
-    tvd_dissipate_x<ND, R, MHD>(f.rt, phi, s, R(dp + d0), d0, out);
+    tvd_dissipate_x<ND, R, MHD>(f.rt, phi, s, R(dp + d0), d0, out, nfJ);
   }
   if (rowok && lane >= 2 && lane <= 62 && i <= g.n[0] + 1) {
     const long long idx = base + i;
@@ -1840,6 +2005,9 @@ __global__ __launch_bounds__(256) void k_tvd_m(Geo g, Sel s, BC bc, const double
     fluxes<ND, MHD, R>(uR, d, s.gamma, smallp, F0);
 #pragma unroll
     for (int iw = 0; iw < NW; ++iw) fCm[iw] = 0.5 * (Fm[iw] + F0[iw]);  // face s0-1
+    // 'symm0' faces (general variant): of the face below and of face i
+    unsigned nfm = ENT ? noflux(s, g, d, s0 - 1) : 0u;
+    zero_flux<NW>(fCm, nfm);
 #pragma unroll
     for (int iw = 0; iw < NW; ++iw) uL[iw] = uR[iw];
     get_ct(s0 + 1, uR);
@@ -1857,6 +2025,8 @@ __global__ __launch_bounds__(256) void k_tvd_m(Geo g, Sel s, BC bc, const double
       double fC[NW];
 #pragma unroll
       for (int iw = 0; iw < NW; ++iw) fC[iw] = 0.5 * (F0[iw] + Fm[iw]);  // face i
+      const unsigned nf0 = ENT ? noflux(s, g, d, i) : 0u;
+      zero_flux<NW>(fC, nf0);
 #pragma unroll
       for (int iw = 0; iw < NW; ++iw) F0[iw] = Fm[iw];
       // face i+1 from w(i+1), w(i+2)
@@ -1874,13 +2044,15 @@ __global__ __launch_bounds__(256) void k_tvd_m(Geo g, Sel s, BC bc, const double
 #pragma unroll
       for (int iw = 0; iw < NW; ++iw) out[iw] = win[iw * g.ps + idx] - qdx * (fC[iw] - fCm[iw]);
       const double dm = dvol(i - 1), d0 = dvol(i), dp = dvol(i + 1);
-      tvd_dissipate<ND, R, MHD>(fm.rt, phim, f0.rt, phi0, d, s, R(d0 + dm), R(dp + d0), d0, out);
+      tvd_dissipate<ND, R, MHD>(fm.rt, phim, f0.rt, phi0, d, s, R(d0 + dm), R(dp + d0), d0, out,
+                                nfm, nf0);
 #pragma unroll
       for (int iw = 0; iw < NW; ++iw) wout[iw * g.ps + idx] = out[iw];
       if (i == s1) break;
       // shift the window
 #pragma unroll
       for (int iw = 0; iw < NW; ++iw) fCm[iw] = fC[iw];
+      nfm = nf0;
 #pragma unroll
       for (int il = 0; il < NW; ++il) phim[il] = phi0[il];
       fm = f0;
@@ -2152,8 +2324,9 @@ __global__ void k_fin_smallp(Scal* sc, const double* part, long long pstride, lo
   }
 }
 
-// getdt_courant's fold (src/vac.t:690-719) and the main loop's tmaxexact
-// clip (src/vac.t:83); dtpar > 0 replaces the Courant step.
+// getdt_courant's fold (src/vac.t:690-719) and the main loop's dtcantgrow,
+// slowsteps and tmaxexact (src/vac.t:68-83); dtpar > 0 replaces the
+// Courant step.
 // thread 0: dt from the lagged dtcourant and, where it is missing, the
 // cell-centre maxima cmd[] (src/vac.t:690-719), then the tmaxexact clip
 __device__ __forceinline__ void getdt_finish(Scal* sc, const double* dtc, const double* cmd, int nd,
@@ -2171,6 +2344,16 @@ __device__ __forceinline__ void getdt_finish(Scal* sc, const double* dtc, const 
       }
       for (int d = 0; d < nd; ++d) sc->dtcourant[d] = kBig;
       if (cm > kSmall) { double c = courantpar / cm; dt = c < dt ? c : dt; }
+      // dtcantgrow (src/vac.t:68-69)
+      if (sc->dtcantgrow && sc->it > sc->itmin) dt = sc->dtold < dt ? sc->dtold : dt;
+      sc->dtold = dt;
+    }
+    sc->dtchk = dt;
+    // slowsteps > 1 (src/vac.t:78)
+    if (sc->slowsteps > sc->it - sc->itmin + 1) {
+      const double q = ((double)(sc->it - sc->itmin) + 1.0) / (double)sc->slowsteps;
+      const double a = 1.0 - q;
+      dt = dt * (1.0 - a * a);
     }
     if (tmaxexact) {
       double c = sc->tmax - sc->t + kSmall;

@@ -64,6 +64,9 @@ struct vacgpu_grid {
   // those faces keep (null until the first vacgpu_put_state)
   bool any_fixed = false;
   double* fixw = nullptr;
+  // 'symm0' faces: the variables with zero flux through each face
+  unsigned char nf[6] = {0, 0, 0, 0, 0, 0};
+  bool nfany = false;
   int fs_seglen = 16;
   dim3 grid_fs, block_fs;
   int ry_seglen = 16;  // k_ryu_ab_y (2D one-pass Ryu step)
@@ -181,8 +184,8 @@ int check_config(const vacgpu_config* c, std::string& why) {
   int nw = c->physics == VACGPU_MHD ? 2 + 2 * c->ndim : 2 + c->ndim;
   for (int f = 0; f < 2 * c->ndim; ++f)
     for (int iw = 0; iw < nw; ++iw)
-      if (c->typeB[f][iw] < VACGPU_CONT || c->typeB[f][iw] > VACGPU_GRAD1) {
-        why = "typeB must be 'cont', 'periodic', 'symm', 'asymm', 'fixed', 'cont1', 'fixed1' or 'grad1'";
+      if (c->typeB[f][iw] < VACGPU_CONT || c->typeB[f][iw] > VACGPU_SYMM0) {
+        why = "typeB must be 'cont', 'periodic', 'symm', 'symm0', 'asymm', 'fixed', 'cont1', 'fixed1' or 'grad1'";
         return VACGPU_E_ARG;
       }
   for (int d = 0; d < c->ndim; ++d)
@@ -224,12 +227,10 @@ int check_method(const vacgpu_config* c, const vacgpu_method* m, std::string& wh
   if (!(m->smallpcoeff >= 0.0)) { why = "smallpcoeff must be >= 0"; return VACGPU_E_ARG; }
   if (m->useprimitive != 0 && m->useprimitive != 1) { why = "useprimitive must be 0 or 1"; return VACGPU_E_ARG; }
   if (m->unsplit != 0 && m->unsplit != 1) { why = "unsplit must be 0 or 1"; return VACGPU_E_ARG; }
-  if (m->unsplit && m->typefull != VACGPU_TVDLF) { why = "unsplit sweeps (dimsplit=F) are on the GPU path for typefull='tvdlf' only"; return VACGPU_E_ARG; }
-  if (m->unsplit && m->useprimitive) { why = "unsplit sweeps with useprimitive=T are not on the GPU path"; return VACGPU_E_ARG; }
+  if (m->unsplit && m->typefull == VACGPU_TVD) { why = "unsplit sweeps (dimsplit=F) are on the GPU path for typefull='tvdlf' and 'tvdmu' (VAC warns against one-step TVD unsplit, src/vacio.t:508-509)"; return VACGPU_E_ARG; }
+  if (m->dtcantgrow != 0 && m->dtcantgrow != 1) { why = "dtcantgrow must be 0 or 1"; return VACGPU_E_ARG; }
+  if (m->slowsteps < 0) { why = "slowsteps must be >= 0"; return VACGPU_E_ARG; }
   if (m->useprimitive && m->typefull != VACGPU_TVD) {
-    // upwindLR converts its base in place (src/vactvdlf.t:328-364): with
-    // 'previous' that is wold, converted again on every sweep of a step
-    if (m->typelimited != VACGPU_LIM_ORIGINAL) { why = "useprimitive=T with typelimited other than 'original' is not on the GPU path"; return VACGPU_E_ARG; }
     if (c->precision != VACGPU_FP64) { why = "useprimitive=T with TVDLF / TVD-MUSCL runs in FP64 only"; return VACGPU_E_ARG; }
   }
   return 0;
@@ -249,8 +250,34 @@ Sel make_sel(const vacgpu_grid* g, const vacgpu_method* m) {
   }
   s.prim = m->useprimitive;
   s.srcmul = 1;
+  // a split sweep limiting its own input ('original'): the predictor's
+  // upwindLR has converted it once, the corrector's converts it again
+  s.rtw = 1;
+  s.rtb = 2;
+  s.rtu = 0;
+  s.rts = 0;
+  for (int f = 0; f < 6; ++f) s.nf[f] = g->nf[f];
   return s;
 }
+
+// the round trips of a corrector with useprimitive (Sel::rtw ...) for
+// direction k (0-based, in turn) of an advect over nd directions (nd = 1
+// for a split sweep): the predictor's upwindLR converted the sweep's input
+// once per direction; the corrector's converts its limiter's base, which
+// for 'previous' is wold (round-tripped in place after each direction by
+// k_roundtrip) and for 'predictor' w1
+void set_prim_roundtrips(Sel& s, int typelimited, int k, int nd) {
+  const int base = k == 0 ? nd : 0;  // later directions update the previous one's output
+  if (typelimited == VACGPU_LIM_ORIGINAL) {
+    s.rtw = 1; s.rtb = 2; s.rtu = 0; s.rts = 0;
+  } else if (typelimited == VACGPU_LIM_PREVIOUS) {
+    s.rtw = 0; s.rtb = base; s.rtu = 0; s.rts = 0;
+  } else {
+    s.rtw = k; s.rtb = base; s.rtu = k; s.rts = nd;
+  }
+}
+
+
 
 // an entropy fix on any wave of the TVD / TVD-MUSCL Roe dissipation
 bool entropy_on(const vacgpu_grid* g, const vacgpu_method* m) {
@@ -423,12 +450,106 @@ int getdt_lagged(vacgpu_grid* g, const vacgpu_method* m, double dtpar, int tmaxe
 
 // the limiter type shared by every variable / characteristic wave, or -1
 int common_limiter(const vacgpu_grid* g, const vacgpu_method* m) {
+  if (g->nfany) return -1;  // 'symm0' faces: the general variants zero their fluxes
   for (int iw = 1; iw < g->nw; ++iw)
     if (m->typelimiter[iw] != m->typelimiter[0]) return -1;
   return m->typelimiter[0];
 }
 
 // one dimensional sweep (advect, src/vac.t:363-486), dt already on device
+// wold converted to primitive variables and back in place, as the
+// corrector's upwindLR leaves it (useprimitive with 'previous')
+int roundtrip_wold(vacgpu_grid* g) {
+  const long long n = (long long)g->geo.ng[0] * g->geo.ng[1] * (g->cfg.ndim == 3 ? g->geo.ng[2] : 1);
+  const unsigned nb = (unsigned)((n + 255) / 256);
+  DISPATCH(g, {
+    (void)sizeof(R);
+    hipLaunchKernelGGL((k_roundtrip<ND, MHD>), dim3(nb), dim3(256), 0, g->stream, g->geo,
+                       g->buf[g->old], g->cfg.eqpar[0], g->sc);
+  });
+  return launch_check(g, "k_roundtrip");
+}
+
+// TVD-MUSCL corrector of sweep direction d (face-parallel: x one face per
+// lane, y / z marching): win -> wout with the predictor state g->w1 and the
+// limiter's base wl; nbc receives the block count of its Courant partials
+int launch_tvdmu_corr(vacgpu_grid* g, const vacgpu_method* m, int d, const Sel& s,
+                      const double* win, double* wout, const double* wl, int ghosts,
+                      long long& nbc) {
+  int rc = 0;
+  const bool mhd = g->cfg.physics == VACGPU_MHD;
+  {
+    Timed t(g, KF_CORR);
+    const bool f32 = g->cfg.precision == VACGPU_FP32BULK;
+    // the general variant for entropy fixes and 'symm0' faces
+    const bool ent = entropy_on(g, m) || g->nfany;
+    // with entropy fixes the limiter types are read at run time
+    const int lim = ent ? -1 : common_limiter(g, m);
+  #define LIMSW(X) do { if (lim == LIM_MINMOD) X(LIM_MINMOD); else if (lim == LIM_WOODWARD) X(LIM_WOODWARD); else if (lim == LIM_SUPERBEE) X(LIM_SUPERBEE); else X(-1); } while (0)
+    if (d == 0) {
+      const dim3 gr((g->geo.n[0] + CX_STRIDE - 1) / CX_STRIDE, g->grid.y, g->grid.z);
+      nbc = (long long)gr.x * gr.y * gr.z;
+  #define MUX(NDV, RT, LV) do { if (ent) { if (mhd) MUXE(NDV, RT, true); else MUXE(NDV, RT, false); } else if (mhd) hipLaunchKernelGGL((k_tvdmu_x<NDV, RT, LV, true>), gr, g->block, 0, g->stream, g->geo, s, g->bc, win, g->w1, wout, g->sc, g->part, g->pstride, ghosts, wl); \
+                             else hipLaunchKernelGGL((k_tvdmu_x<NDV, RT, LV, false>), gr, g->block, 0, g->stream, g->geo, s, g->bc, win, g->w1, wout, g->sc, g->part, g->pstride, ghosts, wl); } while (0)
+  #define MUXE(NDV, RT, MH) hipLaunchKernelGGL((k_tvdmu_x<NDV, RT, -1, MH, true>), gr, g->block, 0, g->stream, g->geo, s, g->bc, win, g->w1, wout, g->sc, g->part, g->pstride, ghosts, wl)
+  #define X2D(LV) MUX(2, double, LV)
+  #define X2F(LV) MUX(2, float, LV)
+  #define X3D(LV) MUX(3, double, LV)
+  #define X3F(LV) MUX(3, float, LV)
+      if (m->useprimitive) {
+        DISPATCH(g, {
+          if constexpr (std::is_same<R, double>::value) {
+            if (ent) hipLaunchKernelGGL((k_tvdmu_x<ND, R, -1, MHD, true, true>), gr, g->block, 0, g->stream, g->geo, s, g->bc, win, g->w1, wout, g->sc, g->part, g->pstride, ghosts, wl);
+            else hipLaunchKernelGGL((k_tvdmu_x<ND, R, -1, MHD, false, true>), gr, g->block, 0, g->stream, g->geo, s, g->bc, win, g->w1, wout, g->sc, g->part, g->pstride, ghosts, wl);
+          }
+        });
+      } else if (g->cfg.ndim == 2) { if (f32) LIMSW(X2F); else LIMSW(X2D); }
+      else { if (f32) LIMSW(X3F); else LIMSW(X3D); }
+  #undef X2D
+  #undef X2F
+  #undef X3D
+  #undef X3F
+  #undef MUX
+  #undef MUXE
+    } else {
+      const dim3 gr = g->grid_tm[d];
+      nbc = (long long)gr.x * gr.y * gr.z;
+  #define MUME(NDV, RT, DV, MH) hipLaunchKernelGGL((k_tvdmu_m<NDV, RT, DV, -1, MH, true>), gr, g->block, 0, g->stream, g->geo, s, g->bc, win, g->w1, wout, g->sc, g->part, g->pstride, g->seglen, ghosts, wl)
+  #define MUM(NDV, RT, DV, LV) do { if (ent) { if (mhd) MUME(NDV, RT, DV, true); else MUME(NDV, RT, DV, false); } else if (mhd) hipLaunchKernelGGL((k_tvdmu_m<NDV, RT, DV, LV, true>), gr, g->block, 0, g->stream, g->geo, s, g->bc, win, g->w1, wout, g->sc, g->part, g->pstride, g->seglen, ghosts, wl); \
+                                 else hipLaunchKernelGGL((k_tvdmu_m<NDV, RT, DV, LV, false>), gr, g->block, 0, g->stream, g->geo, s, g->bc, win, g->w1, wout, g->sc, g->part, g->pstride, g->seglen, ghosts, wl); } while (0)
+  #define M21D(LV) MUM(2, double, 1, LV)
+  #define M21F(LV) MUM(2, float, 1, LV)
+  #define M31D(LV) MUM(3, double, 1, LV)
+  #define M31F(LV) MUM(3, float, 1, LV)
+  #define M32D(LV) MUM(3, double, 2, LV)
+  #define M32F(LV) MUM(3, float, 2, LV)
+      if (m->useprimitive) {
+  #define MUP(DV) do { if (ent) hipLaunchKernelGGL((k_tvdmu_m<ND, R, DV, -1, MHD, true, true>), gr, g->block, 0, g->stream, g->geo, s, g->bc, win, g->w1, wout, g->sc, g->part, g->pstride, g->seglen, ghosts, wl); \
+                   else hipLaunchKernelGGL((k_tvdmu_m<ND, R, DV, -1, MHD, false, true>), gr, g->block, 0, g->stream, g->geo, s, g->bc, win, g->w1, wout, g->sc, g->part, g->pstride, g->seglen, ghosts, wl); } while (0)
+        DISPATCH(g, {
+          if constexpr (std::is_same<R, double>::value) {
+            if (d == 1) MUP(1); else MUP((ND == 3 ? 2 : 1));
+          }
+        });
+  #undef MUP
+      } else if (g->cfg.ndim == 2) { if (f32) LIMSW(M21F); else LIMSW(M21D); }
+      else if (d == 1) { if (f32) LIMSW(M31F); else LIMSW(M31D); }
+      else { if (f32) LIMSW(M32F); else LIMSW(M32D); }
+  #undef M21D
+  #undef M21F
+  #undef M31D
+  #undef M31F
+  #undef M32D
+  #undef M32F
+  #undef MUM
+  #undef MUME
+    }
+  #undef LIMSW
+    if ((rc = launch_check(g, "k_tvdmu_corr"))) return rc;
+  }
+  return 0;
+}
+
 int sweep_device(vacgpu_grid* g, const vacgpu_method* m, int d, int firstsweep, int lastsweep) {
   if (firstsweep) {
     g->old = g->old_pending >= 0 ? g->old_pending : g->cur;
@@ -441,7 +562,10 @@ int sweep_device(vacgpu_grid* g, const vacgpu_method* m, int d, int firstsweep, 
   double* win = g->buf[g->cur];
   double* wout = g->buf[outi];
   double* wold = g->buf[g->old];
-  const Sel s = make_sel(g, m);
+  Sel s = make_sel(g, m);
+  if (m->useprimitive) set_prim_roundtrips(s, m->typelimited, 0, 1);
+  const bool rt_wold = m->useprimitive && m->typelimited == VACGPU_LIM_PREVIOUS &&
+                       m->typefull != VACGPU_TVD;
   const long long nb = (long long)g->grid.x * g->grid.y * g->grid.z;
   const bool mhd = g->cfg.physics == VACGPU_MHD;
   const bool src = mhd && m->divbfix && m->sourceunsplit;
@@ -507,81 +631,14 @@ int sweep_device(vacgpu_grid* g, const vacgpu_method* m, int d, int firstsweep, 
     }
     if (!ghosts && (rc = bound(g, g->w1))) return rc;
     long long nbc = 0;
-    {
-      // face-parallel corrector: x sweep one face per lane, y / z sweeps marching
-      Timed t(g, KF_CORR);
-      const bool f32 = g->cfg.precision == VACGPU_FP32BULK;
-      const bool ent = entropy_on(g, m);
-      // with entropy fixes the limiter types are read at run time
-      const int lim = ent ? -1 : common_limiter(g, m);
-#define LIMSW(X) do { if (lim == LIM_MINMOD) X(LIM_MINMOD); else if (lim == LIM_WOODWARD) X(LIM_WOODWARD); else if (lim == LIM_SUPERBEE) X(LIM_SUPERBEE); else X(-1); } while (0)
-      if (d == 0) {
-        const dim3 gr((g->geo.n[0] + CX_STRIDE - 1) / CX_STRIDE, g->grid.y, g->grid.z);
-        nbc = (long long)gr.x * gr.y * gr.z;
-#define MUX(NDV, RT, LV) do { if (ent) { if (mhd) MUXE(NDV, RT, true); else MUXE(NDV, RT, false); } else if (mhd) hipLaunchKernelGGL((k_tvdmu_x<NDV, RT, LV, true>), gr, g->block, 0, g->stream, g->geo, s, g->bc, win, g->w1, wout, g->sc, g->part, g->pstride, ghosts, wl); \
-                               else hipLaunchKernelGGL((k_tvdmu_x<NDV, RT, LV, false>), gr, g->block, 0, g->stream, g->geo, s, g->bc, win, g->w1, wout, g->sc, g->part, g->pstride, ghosts, wl); } while (0)
-#define MUXE(NDV, RT, MH) hipLaunchKernelGGL((k_tvdmu_x<NDV, RT, -1, MH, true>), gr, g->block, 0, g->stream, g->geo, s, g->bc, win, g->w1, wout, g->sc, g->part, g->pstride, ghosts, wl)
-#define X2D(LV) MUX(2, double, LV)
-#define X2F(LV) MUX(2, float, LV)
-#define X3D(LV) MUX(3, double, LV)
-#define X3F(LV) MUX(3, float, LV)
-        if (m->useprimitive) {
-          DISPATCH(g, {
-            if constexpr (std::is_same<R, double>::value) {
-              if (ent) hipLaunchKernelGGL((k_tvdmu_x<ND, R, -1, MHD, true, true>), gr, g->block, 0, g->stream, g->geo, s, g->bc, win, g->w1, wout, g->sc, g->part, g->pstride, ghosts, wl);
-              else hipLaunchKernelGGL((k_tvdmu_x<ND, R, -1, MHD, false, true>), gr, g->block, 0, g->stream, g->geo, s, g->bc, win, g->w1, wout, g->sc, g->part, g->pstride, ghosts, wl);
-            }
-          });
-        } else if (g->cfg.ndim == 2) { if (f32) LIMSW(X2F); else LIMSW(X2D); }
-        else { if (f32) LIMSW(X3F); else LIMSW(X3D); }
-#undef X2D
-#undef X2F
-#undef X3D
-#undef X3F
-#undef MUX
-#undef MUXE
-      } else {
-        const dim3 gr = g->grid_tm[d];
-        nbc = (long long)gr.x * gr.y * gr.z;
-#define MUME(NDV, RT, DV, MH) hipLaunchKernelGGL((k_tvdmu_m<NDV, RT, DV, -1, MH, true>), gr, g->block, 0, g->stream, g->geo, s, g->bc, win, g->w1, wout, g->sc, g->part, g->pstride, g->seglen, ghosts, wl)
-#define MUM(NDV, RT, DV, LV) do { if (ent) { if (mhd) MUME(NDV, RT, DV, true); else MUME(NDV, RT, DV, false); } else if (mhd) hipLaunchKernelGGL((k_tvdmu_m<NDV, RT, DV, LV, true>), gr, g->block, 0, g->stream, g->geo, s, g->bc, win, g->w1, wout, g->sc, g->part, g->pstride, g->seglen, ghosts, wl); \
-                                   else hipLaunchKernelGGL((k_tvdmu_m<NDV, RT, DV, LV, false>), gr, g->block, 0, g->stream, g->geo, s, g->bc, win, g->w1, wout, g->sc, g->part, g->pstride, g->seglen, ghosts, wl); } while (0)
-#define M21D(LV) MUM(2, double, 1, LV)
-#define M21F(LV) MUM(2, float, 1, LV)
-#define M31D(LV) MUM(3, double, 1, LV)
-#define M31F(LV) MUM(3, float, 1, LV)
-#define M32D(LV) MUM(3, double, 2, LV)
-#define M32F(LV) MUM(3, float, 2, LV)
-        if (m->useprimitive) {
-#define MUP(DV) do { if (ent) hipLaunchKernelGGL((k_tvdmu_m<ND, R, DV, -1, MHD, true, true>), gr, g->block, 0, g->stream, g->geo, s, g->bc, win, g->w1, wout, g->sc, g->part, g->pstride, g->seglen, ghosts, wl); \
-                     else hipLaunchKernelGGL((k_tvdmu_m<ND, R, DV, -1, MHD, false, true>), gr, g->block, 0, g->stream, g->geo, s, g->bc, win, g->w1, wout, g->sc, g->part, g->pstride, g->seglen, ghosts, wl); } while (0)
-          DISPATCH(g, {
-            if constexpr (std::is_same<R, double>::value) {
-              if (d == 1) MUP(1); else MUP((ND == 3 ? 2 : 1));
-            }
-          });
-#undef MUP
-        } else if (g->cfg.ndim == 2) { if (f32) LIMSW(M21F); else LIMSW(M21D); }
-        else if (d == 1) { if (f32) LIMSW(M31F); else LIMSW(M31D); }
-        else { if (f32) LIMSW(M32F); else LIMSW(M32D); }
-#undef M21D
-#undef M21F
-#undef M31D
-#undef M31F
-#undef M32D
-#undef M32F
-#undef MUM
-#undef MUME
-      }
-#undef LIMSW
-      if ((rc = launch_check(g, "k_tvdmu_corr"))) return rc;
-    }
+    if ((rc = launch_tvdmu_corr(g, m, d, s, win, wout, wl, ghosts, nbc))) return rc;
     if (courant) {
       Timed t(g, KF_RED);
       hipLaunchKernelGGL(k_fin_courant_tvd, dim3(1), dim3(1024), 0, g->stream, g->sc, g->part,
                          g->pstride, nbc, d, m->courantpar);
       if ((rc = launch_check(g, "k_fin_courant_tvd"))) return rc;
     }
+    if (rt_wold && (rc = roundtrip_wold(g))) return rc;
     if (ghosts) {
       g->cur = outi;
       return 0;
@@ -658,6 +715,7 @@ int sweep_device(vacgpu_grid* g, const vacgpu_method* m, int d, int firstsweep, 
                          g->pstride, corr_blocks, d, m->courantpar);
       if ((rc = launch_check(g, "k_fin_courant_tvdlf"))) return rc;
     }
+    if (rt_wold && (rc = roundtrip_wold(g))) return rc;
     if (ghosts) {
       g->cur = outi;
       return 0;
@@ -669,7 +727,7 @@ int sweep_device(vacgpu_grid* g, const vacgpu_method* m, int d, int firstsweep, 
     const int lim = common_limiter(g, m);
     // entropy fixes, and useprimitive's dp for the MHD Roe solver, run in
     // the general kernels
-    const bool ent = entropy_on(g, m) || (m->useprimitive && mhd);
+    const bool ent = entropy_on(g, m) || (m->useprimitive && mhd) || g->nfany;
     const bool fold = g->cfg.ndim == 2 && g->bc_uniform && src && g->use_fused && lim >= 0 && !dirty &&
                       !ent;
     // (the Ryu half folds into the x sweep only: in the marching y sweep it
@@ -868,10 +926,10 @@ int sweep_device(vacgpu_grid* g, const vacgpu_method* m, int d, int firstsweep, 
   return 0;
 }
 
-// unsplit sweep (advect over idim = 1..ndim, src/vac.t:325-329) of TVDLF:
-// the predictor and the corrector each add every direction's fluxes in one
-// pass per cell; 'original' limiting reads as 'previous' (src/vacio.t:
-// 363-364), the step's start, which the corrector's input still is
+// unsplit sweep (advect over idim = 1..ndim, src/vac.t:325-329) of TVDLF
+// and TVD-MUSCL: the predictor adds every direction's fluxes in one pass per
+// cell, the corrector runs direction by direction; 'original' limiting reads
+// as 'previous' (src/vacio.t:363-364), the step's start
 int sweep_unsplit_device(vacgpu_grid* g, const vacgpu_method* m) {
   g->old = g->old_pending >= 0 ? g->old_pending : g->cur;
   g->old_pending = -1;
@@ -882,14 +940,22 @@ int sweep_unsplit_device(vacgpu_grid* g, const vacgpu_method* m) {
   double* wout = g->buf[outi];
   const double* wold = g->buf[g->old];
   const Sel s = make_sel(g, m);
+  // 'original' reads as 'previous' here
+  const int tl = m->typelimited == VACGPU_LIM_PREDICTOR ? VACGPU_LIM_PREDICTOR : VACGPU_LIM_PREVIOUS;
   const int ghosts = (g->bc_uniform && g->use_fused) ? 1 : 0;
   const double* wl = m->typelimited == VACGPU_LIM_PREDICTOR ? g->w1 : wold;
   int rc = 0;
   {
     Timed t(g, KF_PRED);
-    DISPATCH(g, {
-      hipLaunchKernelGGL((k_tvdlf_pred_u<ND, MHD, R>), g->grid, g->block, 0, g->stream, g->geo, s, g->bc, win, g->w1, g->sc, ghosts);
-    });
+    if (m->useprimitive)
+      DISPATCH(g, {
+        if constexpr (std::is_same<R, double>::value)
+          hipLaunchKernelGGL((k_tvdlf_pred_u<ND, MHD, R, true>), g->grid, g->block, 0, g->stream, g->geo, s, g->bc, win, g->w1, g->sc, ghosts);
+      });
+    else
+      DISPATCH(g, {
+        hipLaunchKernelGGL((k_tvdlf_pred_u<ND, MHD, R>), g->grid, g->block, 0, g->stream, g->geo, s, g->bc, win, g->w1, g->sc, ghosts);
+      });
     if ((rc = launch_check(g, "k_tvdlf_pred_u"))) return rc;
   }
   if (!ghosts && (rc = bound(g, g->w1))) return rc;
@@ -915,8 +981,23 @@ int sweep_unsplit_device(vacgpu_grid* g, const vacgpu_method* m) {
     Sel sd = s;
     if (d < nd - 1) sd.sourceunsplit = 0;
     else sd.srcmul = nd;
+    if (m->useprimitive) set_prim_roundtrips(sd, tl, d, nd);
+    const bool rt_wold = m->useprimitive && tl == VACGPU_LIM_PREVIOUS;
     const int gh = d == nd - 1 ? ghosts : 0;
     long long nbc = 0;
+    if (m->typefull == VACGPU_TVDMU) {
+      // TVD-MUSCL: each direction's central flux and Roe dissipation
+      // (src/vactvdlf.t:137-283), its Courant check that of tvdlimit2
+      if ((rc = launch_tvdmu_corr(g, m, d, sd, base, dst, wl, gh, nbc))) return rc;
+      if (m->courantpar > 0.0) {
+        Timed t(g, KF_RED);
+        hipLaunchKernelGGL(k_fin_courant_tvd, dim3(1), dim3(1024), 0, g->stream, g->sc, g->part,
+                           g->pstride, nbc, d, m->courantpar);
+        if ((rc = launch_check(g, "k_fin_courant_tvd"))) return rc;
+      }
+      if (rt_wold && (rc = roundtrip_wold(g))) return rc;
+      continue;
+    }
     {
       Timed t(g, KF_CORR);
 #define LIMSW(X) do { if (lim == LIM_MINMOD) X(LIM_MINMOD); else if (lim == LIM_WOODWARD) X(LIM_WOODWARD); else if (lim == LIM_SUPERBEE) X(LIM_SUPERBEE); else X(-1); } while (0)
@@ -924,7 +1005,13 @@ int sweep_unsplit_device(vacgpu_grid* g, const vacgpu_method* m) {
         const dim3 gr((g->geo.n[0] + CX_STRIDE - 1) / CX_STRIDE, g->grid.y, g->grid.z);
         nbc = (long long)gr.x * gr.y * gr.z;
 #define CX(LV) hipLaunchKernelGGL((k_tvdlf_corrx<ND, MHD, R, LV>), gr, g->block, 0, g->stream, g->geo, sd, g->bc, base, g->w1, dst, g->sc, g->part, g->pstride, gh, wl)
-        DISPATCH(g, { LIMSW(CX); });
+        if (m->useprimitive)
+          DISPATCH(g, {
+            if constexpr (std::is_same<R, double>::value)
+              hipLaunchKernelGGL((k_tvdlf_corrx<ND, MHD, R, -1, true>), gr, g->block, 0, g->stream, g->geo, sd, g->bc, base, g->w1, dst, g->sc, g->part, g->pstride, gh, wl);
+          });
+        else
+          DISPATCH(g, { LIMSW(CX); });
 #undef CX
       } else {
         const dim3 gr = g->grid_tm[d];
@@ -932,7 +1019,16 @@ int sweep_unsplit_device(vacgpu_grid* g, const vacgpu_method* m) {
 #define CM(DV, LV) hipLaunchKernelGGL((k_tvdlf_corrm<ND, MHD, R, DV, LV>), gr, g->block, 0, g->stream, g->geo, sd, g->bc, base, g->w1, dst, g->sc, g->part, g->pstride, g->seglen, gh, wl)
 #define CM1(LV) CM(1, LV)
 #define CM2(LV) CM((ND == 3 ? 2 : 1), LV)
-        DISPATCH(g, { if (d == 1) LIMSW(CM1); else LIMSW(CM2); });
+#define CMP(DV) hipLaunchKernelGGL((k_tvdlf_corrm<ND, MHD, R, DV, -1, true>), gr, g->block, 0, g->stream, g->geo, sd, g->bc, base, g->w1, dst, g->sc, g->part, g->pstride, g->seglen, gh, wl)
+        if (m->useprimitive)
+          DISPATCH(g, {
+            if constexpr (std::is_same<R, double>::value) {
+              if (d == 1) CMP(1); else CMP((ND == 3 ? 2 : 1));
+            }
+          });
+        else
+          DISPATCH(g, { if (d == 1) LIMSW(CM1); else LIMSW(CM2); });
+#undef CMP
 #undef CM2
 #undef CM1
 #undef CM
@@ -946,6 +1042,7 @@ int sweep_unsplit_device(vacgpu_grid* g, const vacgpu_method* m) {
                          g->pstride, nbc, d, m->courantpar);
       if ((rc = launch_check(g, "k_fin_courant_tvdlf"))) return rc;
     }
+    if (rt_wold && (rc = roundtrip_wold(g))) return rc;
   }
   if (!ghosts && (rc = bound(g, wout))) return rc;
   g->cur = outi;
@@ -961,7 +1058,7 @@ SBC make_sbc(const vacgpu_grid* g) {
   for (int f = 0; f < 6; ++f) {
     const int tb = f < 2 * nd ? g->cfg.typeB[f][2 + nd + f / 2] : VACGPU_CONT;
     sb.type[f] = (unsigned char)(tb == VACGPU_PERIODIC ? SB_PERIODIC
-                                 : tb == VACGPU_SYMM ? SB_ASYMM
+                                 : (tb == VACGPU_SYMM || tb == VACGPU_SYMM0) ? SB_ASYMM
                                  : tb == VACGPU_ASYMM ? SB_SYMM
                                  : (tb == VACGPU_FIXED || tb == VACGPU_FIXED1) ? SB_GRAD0 : SB_NUL);
   }
@@ -1202,13 +1299,17 @@ int vacgpu_create(const vacgpu_config* cfg, const double* dx0, const double* dx1
     {
       const int tb = (f < 2 * nd && iw < g->nw) ? cfg->typeB[f][iw] : VACGPU_CONT;
       g->bc.t[f][iw] = (unsigned char)(tb == VACGPU_PERIODIC);
-      g->bc.k[f][iw] = (unsigned char)(tb == VACGPU_SYMM ? BK_SYMM
+      g->bc.k[f][iw] = (unsigned char)((tb == VACGPU_SYMM || tb == VACGPU_SYMM0) ? BK_SYMM
                                        : tb == VACGPU_ASYMM ? BK_ASYMM
                                        : tb == VACGPU_FIXED ? BK_FIXED
                                        : tb == VACGPU_CONT1 ? BK_CONT1
                                        : tb == VACGPU_FIXED1 ? BK_FIXED1
                                        : tb == VACGPU_GRAD1 ? BK_GRAD1 : BK_COPY);
       if (tb == VACGPU_FIXED || tb == VACGPU_FIXED1 || tb == VACGPU_GRAD1) g->any_fixed = true;
+      if (tb == VACGPU_SYMM0) {
+        g->nf[f] |= (unsigned char)(1u << iw);
+        g->nfany = true;
+      }
     }
   // ghost-cell enumeration: slab d has ghost coordinate d, mesh coordinates
   // below d and full ixG coordinates above d
@@ -1419,13 +1520,14 @@ int vacgpu_getdt_courant(vacgpu_grid* g, const vacgpu_method* m, double* dtcoura
   if (!g->have_state) return fail(g, VACGPU_E_STATE, "no state on the device (call vacgpu_put_state)");
   if (m->courantpar <= 0.0) return fail(g, VACGPU_E_ARG, "getdt_courant needs courantpar > 0");
   int rc;
-  if (dtcourant || smallp) {
-    if ((rc = read_scalars(g))) return rc;
-    if (dtcourant)
-      for (int d = 0; d < g->cfg.ndim; ++d) g->sc_host->dtcourant[d] = dtcourant[d];
-    if (smallp) g->sc_host->smallp = *smallp;
-    if ((rc = write_scalars(g))) return rc;
-  }
+  if ((rc = read_scalars(g))) return rc;
+  if (dtcourant)
+    for (int d = 0; d < g->cfg.ndim; ++d) g->sc_host->dtcourant[d] = dtcourant[d];
+  if (smallp) g->sc_host->smallp = *smallp;
+  // getdt_courant alone: the main loop's dtcantgrow / slowsteps are the host's
+  g->sc_host->dtcantgrow = 0;
+  g->sc_host->slowsteps = 0;
+  if ((rc = write_scalars(g))) return rc;
   if ((rc = getdt_device(g, m, -1.0, 0))) return rc;
   if ((rc = read_scalars(g))) return rc;
   if (dtcourant)
@@ -1482,6 +1584,7 @@ int vacgpu_set_time(vacgpu_grid* g, int it, double t) {
   if ((rc = read_scalars(g))) return rc;
   g->sc_host->it = it;
   g->sc_host->t = t;
+  g->sc_host->itmin = it;  // the run's first step (VAC's itmin, src/vacio.t:160-170)
   return write_scalars(g);
 }
 
@@ -1507,20 +1610,25 @@ int vacgpu_run(vacgpu_grid* g, const vacgpu_method* m, int nsteps, int itmax, do
     return fail(g, VACGPU_E_ARG, "dtpar > 0 needs an explicit smallp >= 0 (no Courant pass sets it)");
   g->sc_host->tmax = tmax;
   g->sc_host->tmaxexact = tmaxexact;
+  g->sc_host->dtcantgrow = m->dtcantgrow;
+  g->sc_host->slowsteps = m->slowsteps;
   if ((rc = write_scalars(g))) return rc;
   int it = g->sc_host->it;
+  const int itmin = g->sc_host->itmin;
   double t = g->sc_host->t;
   const bool sync_each = (tmax < kBig) || t_hist;
   int done = 0;
   for (;;) {
     if (t_hist) t_hist[done] = t;
-    if (done == 0) rc = getdt_device(g, m, dtpar, tmaxexact);
-    else rc = getdt_lagged(g, m, dtpar, tmaxexact);
+    // slowsteps = 1 takes dtpar for the run's first step only (src/vac.t:75)
+    const double dtp = (m->slowsteps == 1 && it > itmin) ? -1.0 : dtpar;
+    if (done == 0) rc = getdt_device(g, m, dtp, tmaxexact);
+    else rc = getdt_lagged(g, m, dtp, tmaxexact);
     if (rc) return rc;
     double dt = 1.0;
     if (sync_each) {
       if ((rc = read_scalars(g))) return rc;
-      dt = g->sc_host->dt;
+      dt = g->sc_host->dtchk;  // VAC tests dt < dtmin before slowsteps and tmaxexact
     }
     if (it >= itmax || t >= tmax || dt < 1.0e-99 || done >= nsteps) break;
     g->fold_step_end = true;

@@ -97,6 +97,9 @@ do iw=1,nw
    m%typelimiter(iw)=code
 end do
 m%useprimitive=merge(1,0,useprimitive)
+! VAC's own main loop applies dtcantgrow and slowsteps (src/vac.t:68-78)
+m%dtcantgrow=0
+m%slowsteps=0
 m%unsplit=merge(1,0,idimmin/=idimmax)
 m%divbfix=merge(1,0,divbfix)
 m%divbwave=merge(1,0,divbwave)

@@ -17,7 +17,8 @@ module vacgpu_mod
   integer(c_int), parameter :: VACGPU_TVDLF = 0, VACGPU_TVD = 1, VACGPU_TVDMU = 2
   integer(c_int), parameter :: VACGPU_MINMOD = 0, VACGPU_WOODWARD = 1, VACGPU_SUPERBEE = 2
   integer(c_int), parameter :: VACGPU_CONT = 0, VACGPU_PERIODIC = 1, VACGPU_SYMM = 2, &
-       VACGPU_ASYMM = 3, VACGPU_FIXED = 4, VACGPU_CONT1 = 5, VACGPU_FIXED1 = 6, VACGPU_GRAD1 = 7
+       VACGPU_ASYMM = 3, VACGPU_FIXED = 4, VACGPU_CONT1 = 5, VACGPU_FIXED1 = 6, VACGPU_GRAD1 = 7, &
+       VACGPU_SYMM0 = 8
   integer(c_int), parameter :: VACGPU_FP64 = 0, VACGPU_FP32BULK = 1
   integer(c_int), parameter :: VACGPU_LIM_ORIGINAL = 0, VACGPU_LIM_PREVIOUS = 1, &
        VACGPU_LIM_PREDICTOR = 2
@@ -62,6 +63,8 @@ module vacgpu_mod
      real(c_double) :: entropycoef(8)
      integer(c_int) :: useprimitive
      integer(c_int) :: unsplit
+     integer(c_int) :: dtcantgrow
+     integer(c_int) :: slowsteps
   end type vacgpu_method
 
   interface
@@ -299,6 +302,8 @@ contains
        vacgpu_bc_code = VACGPU_FIXED1
     case ('grad1')
        vacgpu_bc_code = VACGPU_GRAD1
+    case ('symm0')
+       vacgpu_bc_code = VACGPU_SYMM0
     case default
        vacgpu_bc_code = -1
     end select

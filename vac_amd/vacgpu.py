@@ -23,9 +23,9 @@ HD, MHD = 0, 1
 TVDLF, TVD, TVDMU = 0, 1, 2
 LIMITERS = {"minmod": 0, "woodward": 1, "superbee": 2}
 ENTROPY = {"nul": 0, "harten": 1, "powell": 2, "ratio": 3, "yee": 4}
-CONT, PERIODIC, SYMM, ASYMM, FIXED, CONT1, FIXED1, GRAD1 = 0, 1, 2, 3, 4, 5, 6, 7
+CONT, PERIODIC, SYMM, ASYMM, FIXED, CONT1, FIXED1, GRAD1, SYMM0 = 0, 1, 2, 3, 4, 5, 6, 7, 8
 BC_CODES = {"cont": CONT, "periodic": PERIODIC, "symm": SYMM, "asymm": ASYMM, "fixed": FIXED,
-            "cont1": CONT1, "fixed1": FIXED1, "grad1": GRAD1}
+            "cont1": CONT1, "fixed1": FIXED1, "grad1": GRAD1, "symm0": SYMM0}
 FP64, FP32BULK = 0, 1
 KERNELS = {"tvd_sweep": 0, "tvdlf_pred": 1, "tvdlf_corr": 2, "ryu": 3, "bound": 4, "reduce": 5,
            "tvdlf_fused": 6, "projectb": 7}
@@ -43,7 +43,7 @@ class Method(C.Structure):
                 ("courantpar", C.c_double), ("smallpcoeff", C.c_double),
                 ("typelimited", C.c_int), ("typeentropy", C.c_int * 8),
                 ("entropycoef", C.c_double * 8), ("useprimitive", C.c_int),
-                ("unsplit", C.c_int)]
+                ("unsplit", C.c_int), ("dtcantgrow", C.c_int), ("slowsteps", C.c_int)]
 
 
 _lib = None
@@ -139,7 +139,9 @@ class MethodSpec:
     typeentropy: object = "nul"         # one name, or a list per wave
     entropycoef: object = None          # per wave; None: VAC's defaults
     useprimitive: bool = False          # limit primitive variables (FP64)
-    unsplit: bool = False               # dimsplit=F (TVDLF)
+    unsplit: bool = False               # dimsplit=F (TVDLF, TVD-MUSCL)
+    dtcantgrow: bool = False            # the run's dt may not grow (src/vac.t:68-69)
+    slowsteps: int = 0                  # slow start over the first steps (src/vac.t:75-78)
 
     def to_c(self, physics: str = "mhd") -> Method:
         m = Method()
@@ -171,6 +173,8 @@ class MethodSpec:
             m.entropycoef[i] = float(coef[i])
         m.useprimitive = int(self.useprimitive)
         m.unsplit = int(self.unsplit)
+        m.dtcantgrow = int(self.dtcantgrow)
+        m.slowsteps = int(self.slowsteps)
         return m
 
 
