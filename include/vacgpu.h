@@ -280,15 +280,49 @@ int vacgpu_batch_create(long long hbm_budget_bytes, int capacity, vacgpu_batch**
 int vacgpu_batch_add(vacgpu_batch* b, const vacgpu_config* cfg, const vacgpu_method* m,
                      const double* dx0, const double* dx1, const double* w, const int* ld,
                      int it, double t, int itmax, double tmax, int tmaxexact, int* slot);
+/* The same with the initial state given as the mesh only, (nw, ny, nx)
+ * tight: the ghost cells are filled on the device, as vacgpu_batch_add
+ * refills them (src/vac.t:54). */
+int vacgpu_batch_add_mesh(vacgpu_batch* b, const vacgpu_config* cfg, const vacgpu_method* m,
+                          const double* dx0, const double* dx1, const double* wmesh, int it,
+                          double t, int itmax, double tmax, int tmaxexact, int* slot);
 int vacgpu_batch_remove(vacgpu_batch* b, int slot);
 
+/* Admission and removal do not wait for the device: the state is staged in
+ * pinned memory (the caller may release w on return) and uploaded, with its
+ * first dt, on a copy stream that the next step waits for; a removed
+ * instance's memory and slot are reused only after every step and copy
+ * enqueued before the removal has completed. */
+
 /* Enqueue nsteps time steps of every instance that has not reached its stop
- * condition (asynchronous; vacgpu_batch_sync waits and refreshes status). */
+ * condition (asynchronous). */
 int vacgpu_batch_step(vacgpu_batch* b, int nsteps);
+/* Status snapshots: vacgpu_batch_snapshot enqueues a copy of every
+ * instance's scalars after the steps enqueued so far and returns at once
+ * (at most two in flight: a third waits for the oldest);
+ * vacgpu_batch_snapshot_wait makes the oldest one in flight current --
+ * returns 1 when it did, 0 when none is in flight or (wait = 0) the oldest
+ * has not arrived yet.  vacgpu_batch_sync takes a snapshot and waits for it
+ * and for every copy. */
+int vacgpu_batch_snapshot(vacgpu_batch* b);
+int vacgpu_batch_snapshot_wait(vacgpu_batch* b, int wait);
 int vacgpu_batch_sync(vacgpu_batch* b);
-/* Status as of the last vacgpu_batch_sync.  Any output may be NULL. */
+/* Status in the current snapshot; an instance admitted after it reports its
+ * admission it and t, done = 0.  Any output may be NULL. */
 int vacgpu_batch_status(vacgpu_batch* b, int slot, int* it, double* t, double* dt, int* done,
                         int* nerror);
+/* Instances done in the current snapshot: returns their number and writes up
+ * to cap of their slots. */
+int vacgpu_batch_finished(vacgpu_batch* b, int* slots, int cap);
+/* Result copies: vacgpu_batch_fetch_start enqueues a copy of the instance's
+ * state -- ixG, or with mesh = 1 the mesh only, (nw, ny, nx) tight -- after
+ * the steps enqueued so far (the instance may be removed right after) and
+ * returns a ticket; vacgpu_batch_fetch_ready copies it into w (leading dims
+ * ld of that extent, NULL = tight; w may be NULL to discard) and releases
+ * the ticket -- returns 1, or 0 when (wait = 0) it has not arrived yet.
+ * vacgpu_batch_get_state fetches the ixG state, waiting. */
+int vacgpu_batch_fetch_start(vacgpu_batch* b, int slot, int mesh, int* ticket);
+int vacgpu_batch_fetch_ready(vacgpu_batch* b, int ticket, double* w, const int* ld, int wait);
 int vacgpu_batch_get_state(vacgpu_batch* b, int slot, double* w, const int* ld);
 
 /* Replay each step from a captured hipGraph (re-captured after membership changes). */

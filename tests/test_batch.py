@@ -283,3 +283,78 @@ def test_config4_scale_batch():
         assert_matches_oracle(b.get_mesh(s), st["it"], st["t"], inst.w, inst.dxs, inst.nx,
                               inst.ny, "mhd", inst.gamma, prec, f"instance {k}")
     b.close()
+
+
+def test_ixg_and_mesh_admission_agree():
+    # vacgpu_batch_add takes the ixG state (ghost values from the caller,
+    # refilled on the device), vacgpu_batch_add_mesh the mesh only
+    import ctypes as C
+    nx, ny, ph, g, prec, seed = MIX[0]
+    w, dxs = state(nx, ny, ph, g, seed)
+    b = vacgpu.Batch(2 << 30, 8)
+    s_mesh = b.add(w, [nx, ny], ph, g, dxs, METHOD, precision=prec, itmax=7)
+    cfg = vacgpu.make_config([nx, ny], ph, g, precision=prec)
+    m = METHOD.to_c(ph)
+    wg = np.ascontiguousarray(np.pad(w, [(0, 0), (2, 2), (2, 2)], constant_values=np.nan))
+    dx0 = np.ascontiguousarray(dxs[0])
+    dx1 = np.ascontiguousarray(dxs[1])
+    slot = C.c_int()
+    dp = C.POINTER(C.c_double)
+    rc = b.lib.vacgpu_batch_add(b.h, C.byref(cfg), C.byref(m), dx0.ctypes.data_as(dp),
+                                dx1.ctypes.data_as(dp), wg.ctypes.data_as(dp), None, 0, 0.0, 7,
+                                1e99, 0, C.byref(slot))
+    assert rc == 0
+    b.meta[slot.value] = ((nx, ny), w.shape[0])
+    print("case ixG and mesh admission", flush=True)
+    res = run_batch(b, [s_mesh, slot.value], 8)
+    assert np.array_equal(res[s_mesh][0], res[slot.value][0])
+    wa, it, t = alone(w, dxs, nx, ny, ph, g, prec, steps=7)
+    assert np.array_equal(wa, res[s_mesh][0])
+    b.close()
+
+
+def test_pipelined_slot_reuse_matches_alone():
+    # two slots, six instances: each is admitted into a slot freed by a
+    # removal while later steps are still queued, fetched asynchronously and
+    # compared with the same grid alone
+    b = vacgpu.Batch(2 << 30, 2)
+    b.graph(True)
+    pending = [MIX[i] + (5 + i,) for i in (0, 1, 2, 4, 6, 3)]
+    active, fetching, done = {}, {}, []
+    inflight = 0
+    print("case pipelined slot reuse", flush=True)
+    for _ in range(400):
+        while pending and len(active) < 2:
+            nx, ny, ph, g, prec, seed, itmax = pending[0]
+            w, dxs = state(nx, ny, ph, g, seed)
+            try:
+                s = b.add(w, [nx, ny], ph, g, dxs, METHOD, precision=prec, itmax=itmax)
+            except vacgpu.Batch.BudgetFull:
+                break
+            pending.pop(0)
+            active[s] = (w, dxs, nx, ny, ph, g, prec, itmax)
+        if active:
+            b.step(2)
+            b.snapshot()
+            inflight += 1
+        if inflight >= 2 or (inflight and not active):
+            assert b.snapshot_wait(True)
+            inflight -= 1
+            for s in b.finished():
+                st = b.status(s)
+                fetching[b.fetch_start(s)] = (active.pop(s), st)
+                b.remove(s)
+        for tk in list(fetching):
+            wb = b.fetch_ready(tk, wait=False)
+            if wb is not None:
+                done.append((fetching.pop(tk), wb))
+        if not pending and not active and not fetching and not inflight:
+            break
+    assert len(done) == 6
+    for (inst, st), wb in done:
+        w, dxs, nx, ny, ph, g, prec, itmax = inst
+        assert st["done"] and st["it"] == itmax
+        wa, it, t = alone(w, dxs, nx, ny, ph, g, prec, steps=itmax)
+        assert t == st["t"]
+        assert np.array_equal(wa, wb), f"reused-slot instance {nx}x{ny} {ph} {prec} differs"
+    b.close()

@@ -3,16 +3,27 @@
 // Every instance keeps its own state in HBM (state w, sweep output b,
 // predictor w1), its own device scalars (dt, t, it, dtcourant, smallp,
 // nerror, stop condition) and its own selectors.  One batched step runs the
-// TVDLF step (Courant dt, then one x and one y sweep, each predictor +
-// corrector + ghost cells) of every active instance with ~20 launches in
-// total: each launch walks a work list of (instance, tile) entries, so grids
-// of any size share a launch.  Instances are grouped by kernel instance
-// (physics, arithmetic tier) and by tile shape, which the autotuner picks
-// per size class.  The sweep order of an instance follows its own step
-// parity (src/vac.t:310-324): its tiles sit in the x-first or the y-first
-// list.  Stop conditions (itmax, tmax, tmaxexact) are evaluated on the
-// device, so a finished instance drops out without the host.  With fixed
-// membership a step is captured once into a hipGraph and replayed.
+// TVDLF step (one x and one y sweep) of every active instance: each launch
+// walks a work list of (instance, tile) entries, so grids of any size share
+// a launch.  Instances are grouped by kernel instance (physics, arithmetic
+// tier, limiter) and by tile shape, which the autotuner picks per size
+// class.  The sweep order of an instance follows its own step parity
+// (src/vac.t:310-324): its tiles sit in the x-first or the y-first list.
+//
+// The per-instance reductions ride on the sweeps: the last block of an
+// instance to finish a fused sweep takes the instance's Courant maximum
+// (src/vactvdlf.t:184-201) and, after the second sweep, ends the step (it,
+// t) and computes the next dt and the stop condition (getdt_courant with
+// lagged dtcourant, src/vac.t:664-722, and src/vac.t:73-83).  The step is
+// then a handful of sweep launches, captured once per membership into a
+// hipGraph and replayed.  The first dt of an instance (pressure floor and
+// cell-centre Courant maxima) is computed at admission, on the copy stream.
+//
+// Admission, removal, status and result copies do not wait for the steps:
+// states are staged through pinned buffers and uploaded on a copy stream
+// that the step stream waits for (by event) before the new instance's first
+// step; removed instances keep their memory and slot until every step and
+// copy enqueued before the removal has completed.
 //
 // The per-cell arithmetic is the single-grid one (sweep.hpp), so an instance
 // advanced in a batch is bitwise identical to the same grid advanced alone.
@@ -22,10 +33,15 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <condition_variable>
 #include <cstdio>
 #include <cstring>
+#include <deque>
 #include <map>
+#include <set>
+#include <mutex>
 #include <string>
+#include <thread>
 #include <vector>
 
 namespace {
@@ -49,8 +65,7 @@ struct InstDev {
   double* b;   // output of the first sweep
   double* w1;  // predictor
   long long nbound, slab0, slab1;
-  int pbase, ntiles;    // partials of the reduction tiles (pressure floor, cell-centre dt)
-  int pbase2, ntiles2;  // partials of the sweep launches (Courant maximum)
+  int pbase2, ntiles2;  // sweep blocks (fused) or corrector tiles and their partials
   int mhd;
   int seglen;           // fused sweeps: rows per marching segment
 };
@@ -59,8 +74,57 @@ struct BScal {
   Scal s;
   int itmax;
   int done;
+  int needdt;  // the next dt needs cell-centre Courant maxima (a dtcourant stayed unset)
   double smallpcoeff;
 };
+
+// dt of the next step from the lagged dtcourant and, where one is missing,
+// the cell-centre maxima cmd[], then the stop condition (src/vac.t:73) and
+// the tmaxexact clip (src/vac.t:83); thread 0 only
+__device__ void batch_getdt_finish(BScal& S, const InstDev& I, const double* dtc,
+                                   const double* cmd) {
+  const int nd = I.geo.nd;
+  double dt = kBig, cm = 0.0;
+  for (int d = 0; d < nd; ++d) {
+    if (dtc[d] < kBig) dt = dtc[d] < dt ? dtc[d] : dt;
+    else cm = cm < cmd[d] ? cmd[d] : cm;
+  }
+  for (int d = 0; d < nd; ++d) S.s.dtcourant[d] = kBig;
+  if (cm > kSmall) { double c = I.sel.courantpar / cm; dt = c < dt ? c : dt; }
+  if (S.s.it >= S.itmax || S.s.t >= S.s.tmax || dt < 1.0e-99) {
+    S.done = 1;
+  } else {
+    if (S.s.tmaxexact) { double c = S.s.tmax - S.s.t + kSmall; dt = c < dt ? c : dt; }
+    S.s.dt = dt;
+  }
+}
+
+// after the corrector of sweep k: the dtcourant update (src/vactvdlf.t:184-201)
+// from the instance's Courant maximum cm; after the last sweep also the end
+// of the step (src/vac.t:97-98) and the next dt; thread 0 only
+__device__ void batch_sweep_end(BScal& S, const InstDev& I, int k, double cm) {
+  const int nd = I.geo.nd;
+  if (I.sel.courantpar > 0.0) {
+    const bool fwd = (S.s.it / 2) * 2 == S.s.it;
+    const int d = fwd ? k : nd - 1 - k;
+    if (S.s.dt * cm > 1.0) S.s.nerror[2] += 1;
+    if (cm > kSmall) {
+      double c = I.sel.courantpar / cm;
+      S.s.dtcourant[d] = c < S.s.dtcourant[d] ? c : S.s.dtcourant[d];
+    }
+  }
+  if (k != nd - 1) return;
+  S.s.it += 1;
+  S.s.t = S.s.t + S.s.dt;
+  double dtc[3] = {kBig, kBig, kBig}, cmd[3] = {0.0, 0.0, 0.0};
+  bool lagged = true;
+  for (int d = 0; d < nd; ++d) {
+    dtc[d] = S.s.dtcourant[d];
+    lagged &= dtc[d] < kBig;
+  }
+  if (lagged) batch_getdt_finish(S, I, dtc, cmd);
+  else S.needdt = 1;
+}
 
 // block-uniform early exit for finished instances
 #define BATCH_PROLOGUE                        \
@@ -68,35 +132,6 @@ struct BScal {
   const InstDev& I = inst[wk.inst];           \
   const BScal& S = bs[wk.inst];               \
   if (S.done) return;
-
-template <int ND, bool MHD>
-__global__ __launch_bounds__(512) void kb_pminmax(const InstDev* __restrict__ inst,
-                                                 const BScal* __restrict__ bs,
-                                                 const Work* __restrict__ work,
-                                                 double* __restrict__ part, long long pstride) {
-  __shared__ double sm[8];
-  BATCH_PROLOGUE
-  if (S.s.smallp >= 0.0) return;
-  double mn, mx;
-  pminmax_body<ND, MHD>(I.geo, I.w, I.sel.gamma, wk.bx, wk.by, wk.bz, mn, mx);
-  const double bmn = block_reduce<false>(mn, sm);
-  if (threadIdx.x == 0 && threadIdx.y == 0) part[SLOT_PMIN * pstride + wk.pidx] = bmn;
-  const double bmx = block_reduce<true>(mx, sm);
-  if (threadIdx.x == 0 && threadIdx.y == 0) part[SLOT_PMAX * pstride + wk.pidx] = bmx;
-}
-
-template <int ND, bool MHD>
-__global__ __launch_bounds__(512) void kb_cmaxdx(const InstDev* __restrict__ inst,
-                                                const BScal* __restrict__ bs,
-                                                const Work* __restrict__ work,
-                                                double* __restrict__ part, long long pstride, int d) {
-  __shared__ double sm[8];
-  BATCH_PROLOGUE
-  if (S.s.dtcourant[d] < kBig) return;
-  const double v = cmaxdx_body<ND, MHD>(I.geo, I.w, I.sel.gamma, S.s.smallp, d, wk.bx, wk.by, wk.bz);
-  const double b = block_reduce<true>(v, sm);
-  if (threadIdx.x == 0 && threadIdx.y == 0) part[(SLOT_DX + d) * pstride + wk.pidx] = b;
-}
 
 // sweep k (0: x-first instances do x, 1: ...) -- D is the direction this
 // launch works on; src/dst roles: sweep 0 reads w and writes b, sweep 1
@@ -128,12 +163,15 @@ __global__ __launch_bounds__(512) void kb_corr(const InstDev* __restrict__ inst,
 // fused 2D TVDLF sweep of instances whose faces have one boundary type for all
 // variables (the single-grid kernel body): work item (strip, block of four
 // segments); sweep k reads w and writes b (k = 0) or reads b and writes w
-// (k = 1); ghost cells of the output are written by the sweep itself
+// (k = 1); ghost cells of the output are written by the sweep itself.  With
+// fold set, the last of an instance's blocks to finish (fold_courant_last
+// on the instance's two words of folds, zero between launches) runs
+// batch_sweep_end; the autotuner's timing launches pass fold = 0.
 template <bool MHD, class R, int D, int LIM>
 __global__ __launch_bounds__(256) void kb_fused(const InstDev* __restrict__ inst,
-                                               const BScal* __restrict__ bs,
+                                               BScal* __restrict__ bs,
                                                const Work* __restrict__ work, int k,
-                                               double* __restrict__ part, long long pstride) {
+                                               unsigned long long* __restrict__ folds, int fold) {
   __shared__ double sm[8];
   BATCH_PROLOGUE
   const double* src = (k == 0) ? I.w : I.b;
@@ -151,8 +189,11 @@ __global__ __launch_bounds__(256) void kb_fused(const InstDev* __restrict__ inst
   else
     cmx = tvdlf_fy_body<MHD, R, LIM>(I.geo, I.sel, I.bc, src, dst, S.s.dt, S.s.smallp, wk.bx, seg,
                                      I.seglen);
-  const double b = block_reduce<true>(cmx, sm);
-  if (threadIdx.x == 0 && threadIdx.y == 0) part[SLOT_C0 * pstride + wk.pidx] = b;
+  const double bm = block_reduce<true>(cmx, sm);
+  if (!fold || threadIdx.x != 0 || threadIdx.y != 0) return;
+  double cm;
+  if (fold_courant_last(folds + 2 * wk.inst, bm, (unsigned long long)I.ntiles2, &cm))
+    batch_sweep_end(bs[wk.inst], I, k, cm);
 }
 
 // ghost cells of one array of every listed instance: which = 0 w, 1 b, 2 w1
@@ -178,62 +219,8 @@ __device__ double fold_range(const double* p, int n, bool is_max, double* sm) {
   return is_max ? block_reduce<true>(v, sm) : block_reduce<false>(v, sm);
 }
 
-// one block per listed instance: pressure floor, then getdt_courant
-// (src/vac.t:664-722) and the stop condition of the main loop (src/vac.t:73)
-__global__ void kb_fin_smallp(const InstDev* __restrict__ inst, BScal* __restrict__ bs,
-                              const int* __restrict__ ids, const double* __restrict__ part,
-                              long long pstride) {
-  __shared__ double sm[8];
-  const int i = ids[blockIdx.x];
-  BScal& S = bs[i];
-  if (S.done || S.s.smallp >= 0.0) return;
-  const InstDev& I = inst[i];
-  const double mn = fold_range(part + SLOT_PMIN * pstride + I.pbase, I.ntiles, false, sm);
-  __syncthreads();
-  const double mx = fold_range(part + SLOT_PMAX * pstride + I.pbase, I.ntiles, true, sm);
-  if (threadIdx.x == 0) {
-    double sp = mn * S.smallpcoeff;
-    if (I.mhd && sp < 0.0) sp = mx * S.smallpcoeff;
-    S.s.smallp = sp;
-  }
-}
-
-__global__ void kb_fin_getdt(const InstDev* __restrict__ inst, BScal* __restrict__ bs,
-                             const int* __restrict__ ids, const double* __restrict__ part,
-                             long long pstride) {
-  __shared__ double sm[8];
-  const int i = ids[blockIdx.x];
-  BScal& S = bs[i];
-  if (S.done) return;
-  const InstDev& I = inst[i];
-  const int nd = I.geo.nd;
-  double dtc[3] = {kBig, kBig, kBig};
-  for (int d = 0; d < nd; ++d) dtc[d] = S.s.dtcourant[d];
-  double cmd[3] = {0.0, 0.0, 0.0};
-  for (int d = 0; d < nd; ++d)
-    if (dtc[d] >= kBig) {
-      cmd[d] = fold_range(part + (SLOT_DX + d) * pstride + I.pbase, I.ntiles, true, sm);
-      __syncthreads();
-    }
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    double dt = kBig, cm = 0.0;
-    for (int d = 0; d < nd; ++d) {
-      if (dtc[d] < kBig) dt = dtc[d] < dt ? dtc[d] : dt;
-      else cm = cm < cmd[d] ? cmd[d] : cm;
-    }
-    for (int d = 0; d < nd; ++d) S.s.dtcourant[d] = kBig;
-    if (cm > kSmall) { double c = I.sel.courantpar / cm; dt = c < dt ? c : dt; }
-    if (S.s.it >= S.itmax || S.s.t >= S.s.tmax || dt < 1.0e-99) {
-      S.done = 1;
-    } else {
-      if (S.s.tmaxexact) { double c = S.s.tmax - S.s.t + kSmall; dt = c < dt ? c : dt; }
-      S.s.dt = dt;
-    }
-  }
-}
-
-// dtcourant update after the corrector of sweep k (src/vactvdlf.t:184-201)
+// predictor + corrector instances: one block per listed instance folds the
+// corrector's partials of sweep k and runs batch_sweep_end
 __global__ void kb_fin_courant(const InstDev* __restrict__ inst, BScal* __restrict__ bs,
                                const int* __restrict__ ids, const double* __restrict__ part,
                                long long pstride, int k) {
@@ -243,29 +230,67 @@ __global__ void kb_fin_courant(const InstDev* __restrict__ inst, BScal* __restri
   if (S.done) return;
   const InstDev& I = inst[i];
   const double cm = fold_range(part + SLOT_C0 * pstride + I.pbase2, I.ntiles2, true, sm);
-  if (threadIdx.x == 0 && I.sel.courantpar > 0.0) {
-    const int nd = I.geo.nd;
-    const bool fwd = (S.s.it / 2) * 2 == S.s.it;
-    const int d = fwd ? k : nd - 1 - k;
-    if (S.s.dt * cm > 1.0) S.s.nerror[2] += 1;
-    if (cm > kSmall) {
-      double c = I.sel.courantpar / cm;
-      S.s.dtcourant[d] = c < S.s.dtcourant[d] ? c : S.s.dtcourant[d];
+  if (threadIdx.x == 0) batch_sweep_end(S, I, k, cm);
+}
+
+// The next dt of instances whose sweeps left a dtcourant unset (a Courant
+// maximum at or below smalldouble): the cell-centre maxima over the mesh
+// (src/vac.t:691-700) in one block per instance, then batch_getdt_finish.
+// Returns at once for every other instance.
+template <bool MHD>
+__device__ void getdt_rare_body(BScal& S, const InstDev& I, double* sm) {
+  constexpr int ND = 2;
+  const Geo& g = I.geo;
+  const long long ncell = (long long)g.n[0] * g.n[1];
+  double dtc[3] = {kBig, kBig, kBig}, cmd[3] = {0.0, 0.0, 0.0};
+  for (int d = 0; d < ND; ++d) dtc[d] = S.s.dtcourant[d];
+  for (int d = 0; d < ND; ++d) {
+    if (dtc[d] < kBig) continue;  // block-uniform
+    double v = -INFINITY;
+    for (long long t = threadIdx.x; t < ncell; t += blockDim.x) {
+      const int c0 = 2 + (int)(t % g.n[0]), c1 = 2 + (int)(t / g.n[0]);
+      const int cc[2] = {c0, c1};
+      double u[Layout<ND, MHD>::nw];
+      load<Layout<ND, MHD>::nw>(I.w, g.ps, g.off + c0 + (long long)c1 * g.s[1], u);
+      const double q = pdiv(cmax<ND, MHD, double>(u, d, I.sel.gamma, S.s.smallp), g.dx[d][cc[d]]);
+      v = v < q ? q : v;
     }
+    cmd[d] = block_reduce<true>(v, sm);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    batch_getdt_finish(S, I, dtc, cmd);
+    S.needdt = 0;
   }
 }
 
-__global__ void kb_step_end(BScal* __restrict__ bs, const int* __restrict__ ids, int n) {
-  const int j = blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= n) return;
-  BScal& S = bs[ids[j]];
-  if (S.done) return;
-  S.s.it += 1;
-  S.s.t = S.s.t + S.s.dt;
+__global__ __launch_bounds__(256) void kb_getdt_rare(const InstDev* __restrict__ inst,
+                                                    BScal* __restrict__ bs,
+                                                    const int* __restrict__ ids) {
+  __shared__ double sm[8];
+  const int i = ids[blockIdx.x];
+  BScal& S = bs[i];
+  if (S.done || !S.needdt) return;
+  if (inst[i].mhd) getdt_rare_body<true>(S, inst[i], sm);
+  else getdt_rare_body<false>(S, inst[i], sm);
+}
+
+// first dt of a newly admitted instance, after k_pminmax / k_fin_smallp /
+// k_cmaxdx on its own partials: fold the cell-centre maxima and run
+// batch_getdt_finish (every dtcourant is unset at admission)
+__global__ void kb_first_getdt(InstDev I, BScal* __restrict__ S, const double* __restrict__ part,
+                               long long pstride, int nb) {
+  __shared__ double sm[8];
+  double dtc[3] = {kBig, kBig, kBig}, cmd[3] = {0.0, 0.0, 0.0};
+  for (int d = 0; d < I.geo.nd; ++d) {
+    cmd[d] = fold_range(part + (SLOT_DX + d) * pstride, nb, true, sm);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) batch_getdt_finish(*S, I, dtc, cmd);
 }
 
 // ---------------------------------------------------------------- host
-enum { NCLASS = 4 };
+enum { NCLASS = 4, NSNAP = 3 };
 const int kTileRows[] = {1, 2, 4, 8};
 const int kSegLens[] = {8, 16, 32, 64};
 
@@ -287,9 +312,12 @@ struct HostInst {
   InstDev dev{};
   long long bytes = 0;
   int klass = 0;
-  int phase_class = 0;  // (it at admission - batch step at admission) mod 2
-  int ntiles = 0;
-  dim3 tiles;           // tile grid for the class tile shape
+  bool joined = false;  // in the work lists (from the rebuild after admission)
+  int phase_class = 0;  // (it - batch step) mod 2 when it joined
+  long long join_seq = 0;  // first status snapshot that shows this instance
+  int it0 = 0;
+  double t0 = 0.0;
+  dim3 tiles;           // predictor / corrector tile grid of an unfused instance
   bool fused = false;   // sweeps by kb_fused (else predictor + corrector tiles)
   int lim = -1;         // the common limiter type of a fused instance
 };
@@ -300,17 +328,140 @@ struct Launch {          // one work list on the device
   int cap = 0;
 };
 
+// pinned host buffer for state uploads and result copies; free for reuse
+// when not held by a result and its last copy (event) has completed
+struct Stage {
+  double* p = nullptr;
+  size_t bytes = 0;
+  hipEvent_t ev = nullptr;
+  bool recorded = false;
+  bool held = false;
+};
+
+// memory and slots of removed instances, and superseded device buffers and
+// graphs, released once the event (recorded after every step and copy
+// enqueued before the removal) has completed
+struct Retired {
+  hipEvent_t ev = nullptr, evc = nullptr;
+  int slot = -1;
+  long long bytes = 0;
+  std::vector<std::pair<size_t, double*>> blocks;
+  std::vector<void*> frees;
+  std::vector<hipGraphExec_t> graphs;
+};
+
+struct Fetch {
+  bool used = false;
+  int stage = -1;
+  int nw = 0, ng0 = 0, ng1 = 0;
+};
+
+// Host copies into and out of the pinned stages (states of up to hundreds of
+// MB) split over a few persistent threads and the caller: one thread's
+// memcpy, not the PCIe link, would otherwise bound admission and result
+// delivery.  One job at a time (the batch is not thread-safe anyway).
+class CopyPool {
+ public:
+  explicit CopyPool(int workers) {
+    for (int i = 0; i < workers; ++i) th_.emplace_back([this] { loop(); });
+  }
+  ~CopyPool() {
+    {
+      std::lock_guard<std::mutex> lk(m_);
+      stop_ = true;
+    }
+    cv_.notify_all();
+    for (auto& t : th_) t.join();
+  }
+  // n rows of row bytes, source rows spitch apart, destination rows dpitch apart
+  void copy(char* dst, size_t dpitch, const char* src, size_t spitch, size_t row, size_t n) {
+    const size_t total = row * n;
+    if (total < (size_t(4) << 20) || th_.empty()) {
+      copy_rows(dst, dpitch, src, spitch, row, 0, n);
+      return;
+    }
+    {
+      std::lock_guard<std::mutex> lk(m_);
+      dst_ = dst; src_ = src; dp_ = dpitch; sp_ = spitch; row_ = row; n_ = n;
+      parts_ = (int)th_.size() + 1;
+      next_ = 0;
+      left_ = parts_;
+      ++gen_;
+    }
+    cv_.notify_all();
+    work();
+    std::unique_lock<std::mutex> lk(m_);
+    done_.wait(lk, [this] { return left_ == 0; });
+  }
+
+ private:
+  static void copy_rows(char* dst, size_t dp, const char* src, size_t sp, size_t row, size_t r0,
+                        size_t r1) {
+    if (dp == row && sp == row) {
+      std::memcpy(dst + r0 * row, src + r0 * row, (r1 - r0) * row);
+      return;
+    }
+    for (size_t r = r0; r < r1; ++r) std::memcpy(dst + r * dp, src + r * sp, row);
+  }
+  void work() {
+    for (;;) {
+      char* d;
+      const char* s;
+      size_t dp, sp, row, r0, r1;
+      {
+        std::lock_guard<std::mutex> lk(m_);
+        if (next_ >= parts_) return;
+        const int k = next_++;
+        d = dst_; s = src_; dp = dp_; sp = sp_; row = row_;
+        if (dp == row && sp == row) {  // contiguous: split bytes, in whole rows of 64 KiB blocks
+          const size_t tot = row * n_, per = (tot / parts_ + 65535) / 65536 * 65536;
+          r0 = std::min(tot, per * k);
+          r1 = std::min(tot, per * (k + 1));
+          row = 1;
+          dp = sp = 1;
+        } else {
+          const size_t per = (n_ + parts_ - 1) / parts_;
+          r0 = std::min(n_, per * k);
+          r1 = std::min(n_, per * (k + 1));
+        }
+      }
+      if (r1 > r0) copy_rows(d, dp, s, sp, row, r0, r1);
+      std::lock_guard<std::mutex> lk(m_);
+      if (--left_ == 0) done_.notify_all();
+    }
+  }
+  void loop() {
+    unsigned long long seen = 0;
+    for (;;) {
+      {
+        std::unique_lock<std::mutex> lk(m_);
+        cv_.wait(lk, [&] { return stop_ || gen_ != seen; });
+        if (stop_) return;
+        seen = gen_;
+      }
+      work();
+    }
+  }
+  std::vector<std::thread> th_;
+  std::mutex m_;
+  std::condition_variable cv_, done_;
+  char* dst_ = nullptr;
+  const char* src_ = nullptr;
+  size_t dp_ = 0, sp_ = 0, row_ = 0, n_ = 0;
+  int parts_ = 0, next_ = 0, left_ = 0;
+  unsigned long long gen_ = 0;
+  bool stop_ = false;
+};
+
 }  // namespace
 
 struct vacgpu_batch {
   long long budget = 0, used_bytes = 0;
   std::vector<HostInst> inst;
+  std::vector<char> slot_busy;  // removed, not yet released
   InstDev* d_inst = nullptr;
-  InstDev* h_inst = nullptr;  // pinned mirror, uploaded whole by rebuild
-  double* stage = nullptr;    // pinned staging for state uploads / downloads
-  size_t stage_bytes = 0;
   BScal* d_bs = nullptr;
-  BScal* h_bs = nullptr;  // pinned mirror
+  unsigned long long* d_fold = nullptr;  // per-instance Courant fold words of the fused sweeps
   int cap = 0;
   double* part = nullptr;
   long long pstride = 0;
@@ -319,22 +470,39 @@ struct vacgpu_batch {
   bool use_fused = true;
   long long step_count = 0;
   bool dirty = true;
-  // work lists per (physics/tier key, tile rows): every tile, and the tiles
-  // of each phase class c (0/1)
+  // work lists per (kernel key, tile rows or segment length) and phase class
   struct Lists {
-    Launch all;          // every tile of the group/class
-    Launch ph[2];        // tiles of instances with phase class c
+    Launch ph[2];
   };
   std::map<std::pair<Key, int>, Lists> lists;
-  std::vector<GWork> hg[3];
   struct { GWork* d = nullptr; int n = 0, cap = 0; } gw4, gw6;  // ghost work per nw
-  int* d_ids = nullptr;
-  int nids = 0, ids_cap = 0;
-  std::vector<int> fresh;                     // slots whose ghost cells need a first fill
-  std::multimap<size_t, double*> pool;        // reusable device blocks by size
-  hipStream_t stream = nullptr;
+  int* d_ids = nullptr;    // every joined instance
+  int* d_idsu = nullptr;   // joined instances swept by predictor + corrector tiles
+  int nids = 0, nidsu = 0, ids_cap = 0;
+  std::multimap<size_t, double*> pool;  // reusable device blocks by size
+  hipStream_t stream = nullptr;         // steps
+  hipStream_t cstream = nullptr;        // admissions and result copies
   bool use_graph = false;
   hipGraphExec_t graph[2] = {nullptr, nullptr};  // per step parity
+  // admissions since the last rebuild: the step stream waits for ev_admit
+  hipEvent_t ev_admit = nullptr;
+  bool admit_pending = false;
+  std::vector<Stage> stages;
+  // rebuild uploads: two pinned arenas used in turn
+  struct Arena { char* h = nullptr; size_t cap = 0; hipEvent_t ev = nullptr; bool recorded = false; } arena[2];
+  int arena_cur = 0;
+  // status snapshots
+  BScal* h_snap[NSNAP] = {nullptr, nullptr, nullptr};
+  hipEvent_t ev_snap[NSNAP] = {nullptr, nullptr, nullptr};
+  long long snap_seqno[NSNAP] = {-1, -1, -1};
+  std::deque<int> snap_queue;  // mirrors with a copy in flight, oldest first
+  int snap_cur = -1;
+  long long snap_next = 0;
+  std::deque<Retired> retired;
+  std::vector<Fetch> fetches;
+  std::vector<hipEvent_t> evpool;
+  std::vector<InstDev> hdev;  // host image of d_inst
+  CopyPool* copier = nullptr;
   std::string err;
 };
 
@@ -413,55 +581,111 @@ long long footprint(const vacgpu_config& c) {
   return (long long)(3 * plane_bytes_of(g, nw) + dx_bytes_of(g));
 }
 
-int ensure_stage(vacgpu_batch* b, size_t bytes) {
-  if (bytes <= b->stage_bytes) return 0;
-  if (b->stage) BHIP(b, hipHostFree(b->stage));
-  b->stage = nullptr;
-  b->stage_bytes = 0;
-  BHIP(b, hipHostMalloc(&b->stage, bytes, hipHostMallocDefault));
-  b->stage_bytes = bytes;
-  return 0;
-}
-
-int ensure_launch(vacgpu_batch* b, Launch& L, const std::vector<Work>& v) {
-  if ((int)v.size() > L.cap) {
-    if (L.d) BHIP(b, hipFree(L.d));
-    L.cap = (int)v.size() * 2 + 16;
-    BHIP(b, hipMalloc(&L.d, sizeof(Work) * L.cap));
+int ev_get(vacgpu_batch* b, hipEvent_t* e) {
+  if (!b->evpool.empty()) {
+    *e = b->evpool.back();
+    b->evpool.pop_back();
+    return 0;
   }
-  L.n = (int)v.size();
-  if (L.n) BHIP(b, hipMemcpyAsync(L.d, v.data(), sizeof(Work) * L.n, hipMemcpyHostToDevice, b->stream));
+  BHIP(b, hipEventCreateWithFlags(e, hipEventDisableTiming));
   return 0;
 }
 
-// Rebuild work lists, partial ranges and the active-id list after a change
-// of membership or tile shapes.
+bool ev_done(hipEvent_t e) { return hipEventQuery(e) != hipErrorNotReady; }
+
+// release what retired entries hold, oldest first, as far as their events
+// have completed (or all of them, waiting, with wait set)
+int reap(vacgpu_batch* b, bool wait = false) {
+  while (!b->retired.empty()) {
+    Retired& r = b->retired.front();
+    if (wait) BHIP(b, hipEventSynchronize(r.ev));
+    else if (!ev_done(r.ev)) break;
+    for (auto& kv : r.blocks) pool_put(b, kv.first, kv.second);
+    for (void* p : r.frees) (void)hipFree(p);
+    for (hipGraphExec_t g : r.graphs) (void)hipGraphExecDestroy(g);
+    if (r.slot >= 0) b->slot_busy[r.slot] = 0;
+    b->used_bytes -= r.bytes;
+    b->evpool.push_back(r.ev);
+    b->evpool.push_back(r.evc);
+    b->retired.pop_front();
+  }
+  return 0;
+}
+
+// queue r for release after every step and copy enqueued so far
+int retire(vacgpu_batch* b, Retired&& r) {
+  int rc;
+  if ((rc = ev_get(b, &r.evc)) || (rc = ev_get(b, &r.ev))) return rc;
+  BHIP(b, hipEventRecord(r.evc, b->stream));
+  BHIP(b, hipStreamWaitEvent(b->cstream, r.evc, 0));
+  BHIP(b, hipEventRecord(r.ev, b->cstream));
+  b->retired.push_back(std::move(r));
+  return 0;
+}
+
+// a free pinned stage of at least bytes (the smallest that fits), else a new one
+int stage_get(vacgpu_batch* b, size_t bytes, int* idx) {
+  int best = -1;
+  for (int i = 0; i < (int)b->stages.size(); ++i) {
+    Stage& S = b->stages[i];
+    if (S.held || S.bytes < bytes) continue;
+    if (S.recorded && !ev_done(S.ev)) continue;
+    if (best < 0 || S.bytes < b->stages[best].bytes) best = i;
+  }
+  if (best < 0) {
+    Stage S;
+    size_t sz = size_t(1) << 20;
+    while (sz < bytes) sz <<= 1;
+    BHIP(b, hipHostMalloc(&S.p, sz, hipHostMallocDefault));
+    S.bytes = sz;
+    int rc;
+    if ((rc = ev_get(b, &S.ev))) { (void)hipHostFree(S.p); return rc; }
+    b->stages.push_back(S);
+    best = (int)b->stages.size() - 1;
+  }
+  *idx = best;
+  return 0;
+}
+
+int ensure_dev(vacgpu_batch* b, void** d, int* cap, int n, size_t elem, std::vector<void*>& old) {
+  if (n <= *cap) return 0;
+  if (*d) old.push_back(*d);
+  *d = nullptr;
+  *cap = n * 2 + 16;
+  BHIP(b, hipMalloc(d, elem * (size_t)*cap));
+  return 0;
+}
+
+// Rebuild work lists, partial ranges and the id lists after a change of
+// membership, tile shapes or kernels.  Nothing here waits for the device:
+// uploads go through a pinned arena on the step stream (the arena used two
+// rebuilds ago is reused once its copies have completed), superseded device
+// buffers and graphs are retired.
 int rebuild(vacgpu_batch* b) {
-  std::map<std::pair<Key, int>, std::vector<Work>> all, ph0, ph1;
+  int rc;
+  if ((rc = reap(b))) return rc;
+  if (b->admit_pending) {
+    BHIP(b, hipStreamWaitEvent(b->stream, b->ev_admit, 0));
+    b->admit_pending = false;
+  }
+  std::map<std::pair<Key, int>, std::vector<Work>> ph0, ph1;
   std::vector<GWork> g4, g6;
-  std::vector<int> ids;
-  int pbase = 0, pbase2 = 0;
+  std::vector<int> ids, idsu;
+  int pbase2 = 0;
+  int maxid = -1;
   for (int i = 0; i < (int)b->inst.size(); ++i) {
     HostInst& h = b->inst[i];
     if (!h.used) continue;
-    const int rows = b->rows[h.klass];
+    if (!h.joined) {
+      h.joined = true;
+      h.phase_class = (int)(((h.it0 - b->step_count) % 2 + 2) % 2);
+      h.join_seq = b->snap_next;
+    }
     const Geo& g = h.dev.geo;
-    // reduction tiles (64 x rows), also the sweep tiles of unfused instances
-    h.tiles = dim3((g.n[0] + BX - 1) / BX, (g.n[1] + rows - 1) / rows, 1);
-    h.ntiles = (int)(h.tiles.x * h.tiles.y);
-    h.dev.pbase = pbase;
-    h.dev.ntiles = h.ntiles;
     const bool fused = h.fused && b->use_fused;
     const Key kr{h.cfg.physics == VACGPU_MHD, h.cfg.precision == VACGPU_FP32BULK, -1};
-    auto key = std::make_pair(kr, rows);
     auto& phl = h.phase_class ? ph1 : ph0;
-    int t = 0;
-    for (unsigned by = 0; by < h.tiles.y; ++by)
-      for (unsigned bx = 0; bx < h.tiles.x; ++bx, ++t) {
-        all[key].push_back(Work{i, (int)bx, (int)by, 0, pbase + t});
-        if (!fused) phl[key].push_back(Work{i, (int)bx, (int)by, 0, pbase2 + t});
-      }
-    pbase += h.ntiles;
+    h.dev.pbase2 = pbase2;
     if (fused) {
       // sweep blocks: strips of 62 x cells, blocks of four segments
       const int sl = b->seglen[h.klass];
@@ -469,88 +693,111 @@ int rebuild(vacgpu_batch* b) {
       const int segs = (g.n[1] + sl - 1) / sl;
       const int sblocks = (segs + 3) / 4;
       h.dev.seglen = sl;
-      h.dev.pbase2 = pbase2;
       h.dev.ntiles2 = strips * sblocks;
       const Key kf{kr.mhd, kr.f32, h.lim};
-      auto keyf = std::make_pair(kf, sl);
-      all[keyf];  // the list exists (its reduction part stays empty)
-      int u = 0;
+      auto& v = phl[std::make_pair(kf, sl)];
       for (int sb = 0; sb < sblocks; ++sb)
-        for (int st = 0; st < strips; ++st, ++u) phl[keyf].push_back(Work{i, st, sb, 0, pbase2 + u});
-      pbase2 += h.dev.ntiles2;
+        for (int st = 0; st < strips; ++st) v.push_back(Work{i, st, sb, 0, 0});
     } else {
+      const int rows = b->rows[h.klass];
+      h.tiles = dim3((g.n[0] + BX - 1) / BX, (g.n[1] + rows - 1) / rows, 1);
       h.dev.seglen = 0;
-      h.dev.pbase2 = pbase2;
-      h.dev.ntiles2 = h.ntiles;
-      pbase2 += h.ntiles;
+      h.dev.ntiles2 = (int)(h.tiles.x * h.tiles.y);
+      auto& v = phl[std::make_pair(kr, rows)];
+      int t = 0;
+      for (unsigned by = 0; by < h.tiles.y; ++by)
+        for (unsigned bx = 0; bx < h.tiles.x; ++bx, ++t) v.push_back(Work{i, (int)bx, (int)by, 0, pbase2 + t});
       // ghost cells are filled by separate launches only for unfused instances
       for (long long q = 0; q < h.dev.nbound; q += 256) (kr.mhd ? g6 : g4).push_back(GWork{i, (int)q});
+      idsu.push_back(i);
+      pbase2 += h.dev.ntiles2;
     }
     ids.push_back(i);
-    b->h_inst[i] = h.dev;
+    b->hdev[i] = h.dev;
+    maxid = i;
   }
-  if (!ids.empty())
-    BHIP(b, hipMemcpyAsync(b->d_inst, b->h_inst, sizeof(InstDev) * (ids.back() + 1), hipMemcpyHostToDevice, b->stream));
-  // partial arrays
-  const int pneed = pbase > pbase2 ? pbase : pbase2;
-  if (pneed > b->pstride) {
-    if (b->part) BHIP(b, hipFree(b->part));
-    b->pstride = (long long)pneed * 2 + 64;
-    BHIP(b, hipMalloc(&b->part, sizeof(double) * NSLOT * b->pstride));
+  // device buffers (grown geometrically; old ones retired)
+  Retired old;
+  if (pbase2 > b->pstride) {
+    if (b->part) old.frees.push_back(b->part);
+    b->part = nullptr;
+    b->pstride = (long long)pbase2 * 2 + 64;
+    BHIP(b, hipMalloc(&b->part, sizeof(double) * b->pstride));
   }
-  for (auto& kv : b->lists) { kv.second.all.n = 0; kv.second.ph[0].n = 0; kv.second.ph[1].n = 0; }
-  int rc;
-  for (auto& kv : all) {
-    auto& L = b->lists[kv.first];
-    if ((rc = ensure_launch(b, L.all, kv.second))) return rc;
-    if ((rc = ensure_launch(b, L.ph[0], ph0[kv.first]))) return rc;
-    if ((rc = ensure_launch(b, L.ph[1], ph1[kv.first]))) return rc;
-  }
-  auto upload_g = [&](auto& G, const std::vector<GWork>& v) -> int {
-    if ((int)v.size() > G.cap) {
-      if (G.d) BHIP(b, hipFree(G.d));
-      G.cap = (int)v.size() * 2 + 16;
-      BHIP(b, hipMalloc(&G.d, sizeof(GWork) * G.cap));
+  std::set<std::pair<Key, int>> keys;
+  for (auto& kv : ph0) keys.insert(kv.first);
+  for (auto& kv : ph1) keys.insert(kv.first);
+  for (auto& kv : b->lists) { kv.second.ph[0].n = 0; kv.second.ph[1].n = 0; }
+  size_t total = sizeof(InstDev) * (size_t)(maxid + 1) + 256;
+  for (const auto& k : keys) {
+    auto& L = b->lists[k];
+    const std::vector<Work>* src[2] = {&ph0[k], &ph1[k]};
+    for (int c = 0; c < 2; ++c) {
+      void* d = L.ph[c].d;
+      if ((rc = ensure_dev(b, &d, &L.ph[c].cap, (int)src[c]->size(), sizeof(Work), old.frees))) return rc;
+      L.ph[c].d = static_cast<Work*>(d);
+      L.ph[c].n = (int)src[c]->size();
+      total += sizeof(Work) * src[c]->size() + 256;
     }
-    G.n = (int)v.size();
-    if (G.n) BHIP(b, hipMemcpyAsync(G.d, v.data(), sizeof(GWork) * G.n, hipMemcpyHostToDevice, b->stream));
-    return 0;
-  };
-  if ((rc = upload_g(b->gw4, g4))) return rc;
-  if ((rc = upload_g(b->gw6, g6))) return rc;
+  }
+  {
+    void* d = b->gw4.d;
+    if ((rc = ensure_dev(b, &d, &b->gw4.cap, (int)g4.size(), sizeof(GWork), old.frees))) return rc;
+    b->gw4.d = static_cast<GWork*>(d);
+    d = b->gw6.d;
+    if ((rc = ensure_dev(b, &d, &b->gw6.cap, (int)g6.size(), sizeof(GWork), old.frees))) return rc;
+    b->gw6.d = static_cast<GWork*>(d);
+    b->gw4.n = (int)g4.size();
+    b->gw6.n = (int)g6.size();
+    total += sizeof(GWork) * (g4.size() + g6.size()) + 512;
+  }
   if ((int)ids.size() > b->ids_cap) {
-    if (b->d_ids) BHIP(b, hipFree(b->d_ids));
+    if (b->d_ids) old.frees.push_back(b->d_ids);
+    if (b->d_idsu) old.frees.push_back(b->d_idsu);
+    b->d_ids = b->d_idsu = nullptr;
     b->ids_cap = (int)ids.size() * 2 + 16;
     BHIP(b, hipMalloc(&b->d_ids, sizeof(int) * b->ids_cap));
+    BHIP(b, hipMalloc(&b->d_idsu, sizeof(int) * b->ids_cap));
   }
   b->nids = (int)ids.size();
-  if (b->nids) BHIP(b, hipMemcpyAsync(b->d_ids, ids.data(), sizeof(int) * b->nids, hipMemcpyHostToDevice, b->stream));
-  // first ghost-cell fill of newly admitted states (src/vac.t:54)
-  if (!b->fresh.empty()) {
-    std::vector<GWork> f4, f6;
-    for (int sl : b->fresh) {
-      const HostInst& h = b->inst[sl];
-      if (!h.used) continue;
-      for (long long q = 0; q < h.dev.nbound; q += 256) (h.dev.mhd ? f6 : f4).push_back(GWork{sl, (int)q});
-    }
-    GWork* tmp = nullptr;
-    const size_t n = f4.size() + f6.size();
-    if (n) {
-      BHIP(b, hipMalloc(&tmp, sizeof(GWork) * n));
-      if (!f4.empty()) BHIP(b, hipMemcpyAsync(tmp, f4.data(), sizeof(GWork) * f4.size(), hipMemcpyHostToDevice, b->stream));
-      if (!f6.empty()) BHIP(b, hipMemcpyAsync(tmp + f4.size(), f6.data(), sizeof(GWork) * f6.size(), hipMemcpyHostToDevice, b->stream));
-      if (!f4.empty()) hipLaunchKernelGGL(kb_bound<4>, dim3((unsigned)f4.size()), dim3(256), 0, b->stream, b->d_inst, b->d_bs, tmp, 0);
-      if (!f6.empty()) hipLaunchKernelGGL(kb_bound<6>, dim3((unsigned)f6.size()), dim3(256), 0, b->stream, b->d_inst, b->d_bs, tmp + f4.size(), 0);
-      hipError_t e = hipGetLastError();
-      if (e != hipSuccess) return bfail(b, VACGPU_E_HIP, std::string("kb_bound: ") + hipGetErrorString(e));
-    }
-    BHIP(b, hipStreamSynchronize(b->stream));
-    if (tmp) BHIP(b, hipFree(tmp));
-    b->fresh.clear();
+  b->nidsu = (int)idsu.size();
+  total += sizeof(int) * (ids.size() + idsu.size()) + 512;
+  // uploads through the arena
+  vacgpu_batch::Arena& A = b->arena[b->arena_cur];
+  b->arena_cur ^= 1;
+  if (A.recorded) BHIP(b, hipEventSynchronize(A.ev));
+  A.recorded = false;
+  if (A.cap < total) {
+    if (A.h) BHIP(b, hipHostFree(A.h));
+    A.h = nullptr;
+    A.cap = total * 2;
+    BHIP(b, hipHostMalloc(reinterpret_cast<void**>(&A.h), A.cap, hipHostMallocDefault));
   }
-  BHIP(b, hipStreamSynchronize(b->stream));
+  if (!A.ev && (rc = ev_get(b, &A.ev))) return rc;
+  size_t off = 0;
+  auto up = [&](void* dst, const void* src, size_t bytes) -> int {
+    if (!bytes) return 0;
+    std::memcpy(A.h + off, src, bytes);
+    BHIP(b, hipMemcpyAsync(dst, A.h + off, bytes, hipMemcpyHostToDevice, b->stream));
+    off += (bytes + 255) / 256 * 256;
+    return 0;
+  };
+  if (maxid >= 0 && (rc = up(b->d_inst, b->hdev.data(), sizeof(InstDev) * (size_t)(maxid + 1)))) return rc;
+  for (const auto& k : keys) {
+    auto& L = b->lists[k];
+    if ((rc = up(L.ph[0].d, ph0[k].data(), sizeof(Work) * ph0[k].size()))) return rc;
+    if ((rc = up(L.ph[1].d, ph1[k].data(), sizeof(Work) * ph1[k].size()))) return rc;
+  }
+  if ((rc = up(b->gw4.d, g4.data(), sizeof(GWork) * g4.size()))) return rc;
+  if ((rc = up(b->gw6.d, g6.data(), sizeof(GWork) * g6.size()))) return rc;
+  if ((rc = up(b->d_ids, ids.data(), sizeof(int) * ids.size()))) return rc;
+  if ((rc = up(b->d_idsu, idsu.data(), sizeof(int) * idsu.size()))) return rc;
+  BHIP(b, hipEventRecord(A.ev, b->stream));
+  A.recorded = true;
   for (int p = 0; p < 2; ++p)
-    if (b->graph[p]) { (void)hipGraphExecDestroy(b->graph[p]); b->graph[p] = nullptr; }
+    if (b->graph[p]) { old.graphs.push_back(b->graph[p]); b->graph[p] = nullptr; }
+  if (!old.frees.empty() || !old.graphs.empty())
+    if ((rc = retire(b, std::move(old)))) return rc;
   b->dirty = false;
   return 0;
 }
@@ -580,29 +827,10 @@ int bound_all(vacgpu_batch* b, int which) {
 int enqueue_step(vacgpu_batch* b, int parity) {
   constexpr int ND = 2;
   int rc;
-  // Courant dt
-  for (auto& kv : b->lists) {
-    const auto& L = kv.second;
-    if (!L.all.n) continue;
-    const dim3 blk(BX, kv.first.second);
-    BDISPATCH(kv.first.first.mhd, kv.first.first.f32, {
-      (void)sizeof(R);
-      hipLaunchKernelGGL((kb_pminmax<ND, MHD>), dim3(L.all.n), blk, 0, b->stream, b->d_inst, b->d_bs, L.all.d, b->part, b->pstride);
-    });
-  }
-  if (b->nids) hipLaunchKernelGGL(kb_fin_smallp, dim3(b->nids), dim3(256), 0, b->stream, b->d_inst, b->d_bs, b->d_ids, b->part, b->pstride);
-  for (int d = 0; d < ND; ++d)
-    for (auto& kv : b->lists) {
-      const auto& L = kv.second;
-      if (!L.all.n) continue;
-      const dim3 blk(BX, kv.first.second);
-      BDISPATCH(kv.first.first.mhd, kv.first.first.f32, {
-        (void)sizeof(R);
-        hipLaunchKernelGGL((kb_cmaxdx<ND, MHD>), dim3(L.all.n), blk, 0, b->stream, b->d_inst, b->d_bs, L.all.d, b->part, b->pstride, d);
-      });
-    }
-  if (b->nids) hipLaunchKernelGGL(kb_fin_getdt, dim3(b->nids), dim3(256), 0, b->stream, b->d_inst, b->d_bs, b->d_ids, b->part, b->pstride);
-  if ((rc = launch_err(b, "getdt"))) return rc;
+  // dt of instances whose last sweeps left a dtcourant unset (returns at
+  // once for the others; every other dt was set by the previous step's end)
+  if (b->nids) hipLaunchKernelGGL(kb_getdt_rare, dim3(b->nids), dim3(256), 0, b->stream, b->d_inst, b->d_bs, b->d_ids);
+  if ((rc = launch_err(b, "kb_getdt_rare"))) return rc;
   // sweeps: phase class c has even it when (c + parity) is even
   for (int k = 0; k < ND; ++k) {
     for (int pass = 0; pass < 2; ++pass) {  // 0 predictor, 1 corrector
@@ -618,7 +846,7 @@ int enqueue_step(vacgpu_batch* b, int parity) {
           if (lim >= 0) {  // fused sweep: one launch, in the corrector pass
             if (pass == 1) {
               const dim3 fb(64, 4);
-#define KBF(DV, LV) hipLaunchKernelGGL((kb_fused<MHD, R, DV, LV>), dim3(W.n), fb, 0, b->stream, b->d_inst, b->d_bs, W.d, k, b->part, b->pstride)
+#define KBF(DV, LV) hipLaunchKernelGGL((kb_fused<MHD, R, DV, LV>), dim3(W.n), fb, 0, b->stream, b->d_inst, b->d_bs, W.d, k, b->d_fold, 1)
 #define KBFL(DV) do { if (lim == LIM_MINMOD) KBF(DV, LIM_MINMOD); else if (lim == LIM_WOODWARD) KBF(DV, LIM_WOODWARD); else KBF(DV, LIM_SUPERBEE); } while (0)
               BDISPATCH(kv.first.first.mhd, kv.first.first.f32, {
                 if (D == 0) KBFL(0); else KBFL(1);
@@ -643,13 +871,28 @@ int enqueue_step(vacgpu_batch* b, int parity) {
       if (pass == 0) {
         if ((rc = bound_all(b, 2))) return rc;
       } else {
-        if (b->nids) hipLaunchKernelGGL(kb_fin_courant, dim3(b->nids), dim3(256), 0, b->stream, b->d_inst, b->d_bs, b->d_ids, b->part, b->pstride, k);
+        if (b->nidsu) hipLaunchKernelGGL(kb_fin_courant, dim3(b->nidsu), dim3(256), 0, b->stream, b->d_inst, b->d_bs, b->d_idsu, b->part, b->pstride, k);
+        if ((rc = launch_err(b, "kb_fin_courant"))) return rc;
         if ((rc = bound_all(b, k == 0 ? 1 : 0))) return rc;
       }
     }
   }
-  if (b->nids) hipLaunchKernelGGL(kb_step_end, dim3((b->nids + 255) / 256), dim3(256), 0, b->stream, b->d_bs, b->d_ids, b->nids);
-  return launch_err(b, "kb_step_end");
+  return 0;
+}
+
+// wait for the snapshot copy of mirror m and make it current
+int snap_take(vacgpu_batch* b) {
+  const int m = b->snap_queue.front();
+  BHIP(b, hipEventSynchronize(b->ev_snap[m]));
+  b->snap_queue.pop_front();
+  b->snap_cur = m;
+  return 0;
+}
+
+int sync_all(vacgpu_batch* b) {
+  BHIP(b, hipStreamSynchronize(b->stream));
+  BHIP(b, hipStreamSynchronize(b->cstream));
+  return reap(b, true);
 }
 
 }  // namespace
@@ -668,19 +911,31 @@ int vacgpu_batch_create(long long hbm_budget_bytes, int capacity, vacgpu_batch**
   b->budget = hbm_budget_bytes;
   b->cap = capacity;
   b->inst.resize(capacity);
+  b->slot_busy.assign(capacity, 0);
+  b->hdev.resize(capacity);
+  {
+    const unsigned hc = std::thread::hardware_concurrency();
+    b->copier = new CopyPool(hc >= 16 ? 7 : (hc > 2 ? (int)hc / 2 - 1 : 0));
+  }
   hipError_t e = hipMalloc(&b->d_inst, sizeof(InstDev) * capacity);
-  if (e == hipSuccess) e = hipHostMalloc(&b->h_inst, sizeof(InstDev) * capacity, hipHostMallocDefault);
   if (e == hipSuccess) e = hipMalloc(&b->d_bs, sizeof(BScal) * capacity);
-  if (e == hipSuccess) e = hipHostMalloc(&b->h_bs, sizeof(BScal) * capacity, hipHostMallocDefault);
+  if (e == hipSuccess) e = hipMalloc(&b->d_fold, 2 * sizeof(unsigned long long) * capacity);
+  for (int m = 0; m < NSNAP && e == hipSuccess; ++m) {
+    e = hipHostMalloc(&b->h_snap[m], sizeof(BScal) * capacity, hipHostMallocDefault);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&b->ev_snap[m], hipEventDisableTiming);
+  }
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&b->ev_admit, hipEventDisableTiming);
   if (e == hipSuccess) e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipStreamCreateWithFlags(&b->cstream, hipStreamNonBlocking);
+  if (e == hipSuccess) {
+    std::vector<BScal> init(capacity);
+    for (auto& x : init) { std::memset(&x, 0, sizeof(BScal)); x.done = 1; }
+    e = hipMemcpy(b->d_bs, init.data(), sizeof(BScal) * capacity, hipMemcpyHostToDevice);
+  }
+  if (e == hipSuccess) e = hipMemset(b->d_fold, 0, 2 * sizeof(unsigned long long) * capacity);
+  if (e == hipSuccess) e = hipDeviceSynchronize();
   if (e != hipSuccess) {
     fprintf(stderr, "vacgpu_batch_create: %s\n", hipGetErrorString(e));
-    vacgpu_batch_destroy(b);
-    return VACGPU_E_HIP;
-  }
-  memset(b->h_bs, 0, sizeof(BScal) * capacity);
-  for (int i = 0; i < capacity; ++i) b->h_bs[i].done = 1;
-  if (hipMemcpy(b->d_bs, b->h_bs, sizeof(BScal) * capacity, hipMemcpyHostToDevice) != hipSuccess) {
     vacgpu_batch_destroy(b);
     return VACGPU_E_HIP;
   }
@@ -688,9 +943,15 @@ int vacgpu_batch_create(long long hbm_budget_bytes, int capacity, vacgpu_batch**
   return 0;
 }
 
-int vacgpu_batch_add(vacgpu_batch* b, const vacgpu_config* cfg, const vacgpu_method* m,
-                     const double* dx0, const double* dx1, const double* w, const int* ld,
-                     int it, double t, int itmax, double tmax, int tmaxexact, int* slot) {
+}  // extern "C"
+
+namespace {
+
+// admission; mesh = 1: w is the mesh only, (nw, ny, nx) tight, and the
+// ghost cells come from the device fill alone
+int batch_add(vacgpu_batch* b, const vacgpu_config* cfg, const vacgpu_method* m,
+              const double* dx0, const double* dx1, const double* w, const int* ld, int mesh,
+              int it, double t, int itmax, double tmax, int tmaxexact, int* slot) {
   if (!b || !cfg || !m || !dx0 || !dx1 || !w || !slot) return VACGPU_E_ARG;
   if (cfg->ndim != 2) return bfail(b, VACGPU_E_ARG, "the batch engine runs 2D grids");
   if (m->typefull != VACGPU_TVDLF) return bfail(b, VACGPU_E_ARG, "the batch engine runs typefull='tvdlf'");
@@ -706,18 +967,21 @@ int vacgpu_batch_add(vacgpu_batch* b, const vacgpu_config* cfg, const vacgpu_met
     for (int iw = 0; iw < 8; ++iw)
       if (cfg->typeB[f][iw] != VACGPU_CONT && cfg->typeB[f][iw] != VACGPU_PERIODIC)
         return bfail(b, VACGPU_E_ARG, "the batch engine runs 'cont' and 'periodic' boundaries");
+  int rc;
+  if ((rc = reap(b))) return rc;
   const long long need = footprint(*cfg);
   if (b->used_bytes + need > b->budget) return bfail(b, VACGPU_E_NOMEM, "HBM budget exhausted");
   int s = -1;
   for (int i = 0; i < b->cap; ++i)
-    if (!b->inst[i].used) { s = i; break; }
+    if (!b->inst[i].used && !b->slot_busy[i]) { s = i; break; }
   if (s < 0) return bfail(b, VACGPU_E_NOMEM, "no free slot");
-  HostInst& h = b->inst[s];
-  h = HostInst();
+  HostInst h;
   h.cfg = *cfg;
   h.m = *m;
   h.bytes = need;
   h.klass = size_class(*cfg);
+  h.it0 = it;
+  h.t0 = t;
   InstDev& I = h.dev;
   I.geo = make_geo(*cfg);
   const int nw = cfg->physics == VACGPU_MHD ? 6 : 4;
@@ -748,8 +1012,21 @@ int vacgpu_batch_add(vacgpu_batch* b, const vacgpu_config* cfg, const vacgpu_met
   I.nbound = ghost_slabs(I.geo, slab);
   I.slab0 = slab[0];
   I.slab1 = slab[1];
+  // first-dt partials live in w1 (unused until the first sweep): six slots
+  // of one value per 64 x 4 tile
+  const dim3 ftile((I.geo.n[0] + BX - 1) / BX, (I.geo.n[1] + BY - 1) / BY, 1);
+  const long long nb = (long long)ftile.x * ftile.y;
+  if (NSLOT * nb > I.geo.ps * nw) return bfail(b, VACGPU_E_ARG, "grid too small for the first-dt partials");
   const size_t plane_bytes = plane_bytes_of(I.geo, nw);
   const size_t dx_bytes = dx_bytes_of(I.geo);
+  const int l0 = mesh ? I.geo.n[0] : (ld ? ld[0] : I.geo.ng[0]);
+  const int l1 = mesh ? I.geo.n[1] : (ld ? ld[1] : I.geo.ng[1]);
+  if (!mesh && (l0 < I.geo.ng[0] || l1 < I.geo.ng[1]))
+    return bfail(b, VACGPU_E_ARG, "leading dimensions below ixG");
+  const size_t wbytes = sizeof(double) * (size_t)nw * l0 * l1;
+  const size_t dbytes = sizeof(double) * (size_t)(I.geo.ng[0] + I.geo.ng[1]);
+  int si;
+  if ((rc = stage_get(b, wbytes + dbytes + sizeof(BScal) + 512, &si))) return rc;
   I.w = pool_get(b, plane_bytes);
   I.b = pool_get(b, plane_bytes);
   I.w1 = pool_get(b, plane_bytes);
@@ -764,72 +1041,105 @@ int vacgpu_batch_add(vacgpu_batch* b, const vacgpu_config* cfg, const vacgpu_met
   I.geo.dx[0] = dxd;
   I.geo.dx[1] = dxd + I.geo.ng[0];
   I.geo.dx[2] = dxd;
-  BHIP(b, hipMemcpyAsync(dxd, dx0, sizeof(double) * I.geo.ng[0], hipMemcpyHostToDevice, b->stream));
-  BHIP(b, hipMemcpyAsync(dxd + I.geo.ng[0], dx1, sizeof(double) * I.geo.ng[1], hipMemcpyHostToDevice, b->stream));
-  // state (full ixG with caller leading dimensions), through pinned staging:
-  // one host copy, then pitched DMA per variable
-  const int l0 = ld ? ld[0] : I.geo.ng[0], l1 = ld ? ld[1] : I.geo.ng[1];
-  const size_t hbytes = sizeof(double) * (size_t)nw * l0 * l1;
-  int rc;
-  if ((rc = ensure_stage(b, hbytes))) {
-    pool_put(b, plane_bytes, I.w);
-    pool_put(b, plane_bytes, I.b);
-    pool_put(b, plane_bytes, I.w1);
-    pool_put(b, dx_bytes, dxd);
-    return rc;
+  // host copies into the pinned stage; the caller may release its arrays on return
+  Stage& St = b->stages[si];
+  char* sp = reinterpret_cast<char*>(St.p);
+  b->copier->copy(sp, wbytes, reinterpret_cast<const char*>(w), wbytes, wbytes, 1);
+  double* sdx = reinterpret_cast<double*>(sp + (wbytes + 255) / 256 * 256);
+  std::memcpy(sdx, dx0, sizeof(double) * I.geo.ng[0]);
+  std::memcpy(sdx + I.geo.ng[0], dx1, sizeof(double) * I.geo.ng[1]);
+  BScal* sbs = reinterpret_cast<BScal*>(reinterpret_cast<char*>(sdx) + (dbytes + 255) / 256 * 256);
+  std::memset(sbs, 0, sizeof(BScal));
+  sbs->s.dt = 0.0;
+  sbs->s.t = t;
+  sbs->s.smallp = -1.0;
+  for (int d = 0; d < 3; ++d) sbs->s.dtcourant[d] = kBig;
+  sbs->s.tmax = tmax;
+  sbs->s.tmaxexact = tmaxexact;
+  sbs->s.it = it;
+  sbs->itmax = itmax;
+  sbs->done = 0;
+  sbs->needdt = 0;
+  sbs->smallpcoeff = m->smallpcoeff;
+  // uploads, the initial ghost cells (src/vac.t:54) and the first dt, all on
+  // the copy stream
+  hipStream_t cs = b->cstream;
+  BHIP(b, hipMemcpyAsync(dxd, sdx, dbytes, hipMemcpyHostToDevice, cs));
+  if (mesh) {
+    BHIP(b, hipMemsetAsync(I.w, 0, plane_bytes, cs));
+    for (int iw = 0; iw < nw; ++iw)
+      BHIP(b, hipMemcpy2DAsync(I.w + iw * I.geo.ps + I.geo.off + 2 + 2 * I.geo.s[1],
+                               sizeof(double) * I.geo.s[1], St.p + (size_t)iw * l0 * l1,
+                               sizeof(double) * l0, sizeof(double) * I.geo.n[0], I.geo.n[1],
+                               hipMemcpyHostToDevice, cs));
+  } else {
+    for (int iw = 0; iw < nw; ++iw)
+      BHIP(b, hipMemcpy2DAsync(I.w + iw * I.geo.ps + I.geo.off, sizeof(double) * I.geo.s[1],
+                               St.p + (size_t)iw * l0 * l1, sizeof(double) * l0,
+                               sizeof(double) * I.geo.ng[0], I.geo.ng[1], hipMemcpyHostToDevice, cs));
   }
-  BHIP(b, hipStreamSynchronize(b->stream));  // the stage may still feed an earlier copy
-  memcpy(b->stage, w, hbytes);
-  for (int iw = 0; iw < nw; ++iw)
-    BHIP(b, hipMemcpy2DAsync(I.w + iw * I.geo.ps + I.geo.off, sizeof(double) * I.geo.s[1],
-                             b->stage + (size_t)iw * l0 * l1, sizeof(double) * l0,
-                             sizeof(double) * I.geo.ng[0], I.geo.ng[1], hipMemcpyHostToDevice, b->stream));
-  BHIP(b, hipMemsetAsync(I.b, 0, plane_bytes, b->stream));
-  BHIP(b, hipMemsetAsync(I.w1, 0, plane_bytes, b->stream));
-  BScal bs{};
-  bs.s.dt = 0.0;
-  bs.s.t = t;
-  bs.s.smallp = -1.0;
-  for (int d = 0; d < 3; ++d) bs.s.dtcourant[d] = kBig;
-  bs.s.tmax = tmax;
-  bs.s.tmaxexact = tmaxexact;
-  bs.s.it = it;
-  bs.itmax = itmax;
-  bs.done = 0;
-  bs.smallpcoeff = m->smallpcoeff;
-  b->h_bs[s] = bs;
-  BHIP(b, hipMemcpyAsync(b->d_bs + s, &b->h_bs[s], sizeof(BScal), hipMemcpyHostToDevice, b->stream));
+  BHIP(b, hipMemsetAsync(I.b, 0, plane_bytes, cs));
+  BHIP(b, hipMemsetAsync(I.w1, 0, plane_bytes, cs));
+  BHIP(b, hipMemcpyAsync(b->d_bs + s, sbs, sizeof(BScal), hipMemcpyHostToDevice, cs));
+  BHIP(b, hipEventRecord(St.ev, cs));
+  St.recorded = true;
+  Scal* sc = &(b->d_bs + s)->s;
+  double* fp = I.w1;
+  const dim3 tb(BX, BY);
+  constexpr int ND = 2;
+  BDISPATCH(I.mhd, false, {
+    (void)sizeof(R);
+    constexpr int NWV = Layout<ND, MHD>::nw;
+    hipLaunchKernelGGL((k_bound<NWV>), dim3((unsigned)((I.nbound + 255) / 256)), dim3(256), 0, cs, I.geo, I.bc,
+                       I.w, I.nbound, I.slab0, I.slab1, 0, NWV, (const double*)nullptr);
+    hipLaunchKernelGGL((k_pminmax<ND, MHD>), ftile, tb, 0, cs, I.geo, I.w, I.sel.gamma, sc, fp, nb);
+    hipLaunchKernelGGL(k_fin_smallp, dim3(1), dim3(256), 0, cs, sc, fp, nb, nb, m->smallpcoeff, (int)MHD);
+    for (int d = 0; d < ND; ++d)
+      hipLaunchKernelGGL((k_cmaxdx<ND, MHD>), ftile, tb, 0, cs, I.geo, I.w, I.sel.gamma, sc, fp, nb, d);
+  });
+  hipLaunchKernelGGL(kb_first_getdt, dim3(1), dim3(256), 0, cs, I, b->d_bs + s, fp, nb, (int)nb);
+  if ((rc = launch_err(b, "first dt"))) return rc;
+  BHIP(b, hipEventRecord(b->ev_admit, cs));
+  b->admit_pending = true;
   h.used = true;
-  // which phase list: parity of it at the next step is (it + 0) while the
-  // batch is at step_count, so class = (it - step_count) mod 2
-  h.phase_class = (int)(((it - b->step_count) % 2 + 2) % 2);
+  b->inst[s] = h;
   b->used_bytes += need;
   b->dirty = true;
-  b->fresh.push_back(s);
   *slot = s;
-  // the caller may release its host arrays on return
-  BHIP(b, hipStreamSynchronize(b->stream));
   return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int vacgpu_batch_add(vacgpu_batch* b, const vacgpu_config* cfg, const vacgpu_method* m,
+                     const double* dx0, const double* dx1, const double* w, const int* ld,
+                     int it, double t, int itmax, double tmax, int tmaxexact, int* slot) {
+  return batch_add(b, cfg, m, dx0, dx1, w, ld, 0, it, t, itmax, tmax, tmaxexact, slot);
+}
+
+int vacgpu_batch_add_mesh(vacgpu_batch* b, const vacgpu_config* cfg, const vacgpu_method* m,
+                          const double* dx0, const double* dx1, const double* wmesh, int it,
+                          double t, int itmax, double tmax, int tmaxexact, int* slot) {
+  return batch_add(b, cfg, m, dx0, dx1, wmesh, nullptr, 1, it, t, itmax, tmax, tmaxexact, slot);
 }
 
 int vacgpu_batch_remove(vacgpu_batch* b, int slot) {
   if (!b || slot < 0 || slot >= b->cap || !b->inst[slot].used) return VACGPU_E_ARG;
-  // nothing in flight may still use the buffers when they return to the pool
-  BHIP(b, hipStreamSynchronize(b->stream));
   HostInst& h = b->inst[slot];
   const int nw = h.dev.mhd ? 6 : 4;
   const size_t plane_bytes = plane_bytes_of(h.dev.geo, nw);
   const size_t dx_bytes = dx_bytes_of(h.dev.geo);
-  pool_put(b, plane_bytes, h.dev.w);
-  pool_put(b, plane_bytes, h.dev.b);
-  pool_put(b, plane_bytes, h.dev.w1);
-  pool_put(b, dx_bytes, const_cast<double*>(h.dev.geo.dx[0]));
-  b->used_bytes -= h.bytes;
+  Retired r;
+  r.slot = slot;
+  r.bytes = h.bytes;
+  r.blocks = {{plane_bytes, h.dev.w}, {plane_bytes, h.dev.b}, {plane_bytes, h.dev.w1},
+              {dx_bytes, const_cast<double*>(h.dev.geo.dx[0])}};
+  b->slot_busy[slot] = 1;
   h = HostInst();
-  b->h_bs[slot].done = 1;
-  BHIP(b, hipMemcpyAsync(b->d_bs + slot, &b->h_bs[slot], sizeof(BScal), hipMemcpyHostToDevice, b->stream));
   b->dirty = true;
-  return 0;
+  return retire(b, std::move(r));
 }
 
 int vacgpu_batch_step(vacgpu_batch* b, int nsteps) {
@@ -858,17 +1168,57 @@ int vacgpu_batch_step(vacgpu_batch* b, int nsteps) {
   return 0;
 }
 
+int vacgpu_batch_snapshot(vacgpu_batch* b) {
+  if (!b) return VACGPU_E_ARG;
+  int rc;
+  if ((int)b->snap_queue.size() == NSNAP - 1 && (rc = snap_take(b))) return rc;
+  int m = 0;
+  for (; m < NSNAP; ++m) {
+    bool busy = m == b->snap_cur;
+    for (int q : b->snap_queue) busy |= q == m;
+    if (!busy) break;
+  }
+  BHIP(b, hipMemcpyAsync(b->h_snap[m], b->d_bs, sizeof(BScal) * b->cap, hipMemcpyDeviceToHost, b->stream));
+  BHIP(b, hipEventRecord(b->ev_snap[m], b->stream));
+  b->snap_seqno[m] = b->snap_next++;
+  b->snap_queue.push_back(m);
+  return 0;
+}
+
+int vacgpu_batch_snapshot_wait(vacgpu_batch* b, int wait) {
+  if (!b) return VACGPU_E_ARG;
+  if (b->snap_queue.empty()) return 0;
+  if (!wait && !ev_done(b->ev_snap[b->snap_queue.front()])) return 0;
+  int rc;
+  if ((rc = snap_take(b))) return rc;
+  return 1;
+}
+
 int vacgpu_batch_sync(vacgpu_batch* b) {
   if (!b) return VACGPU_E_ARG;
-  BHIP(b, hipMemcpyAsync(b->h_bs, b->d_bs, sizeof(BScal) * b->cap, hipMemcpyDeviceToHost, b->stream));
-  BHIP(b, hipStreamSynchronize(b->stream));
-  return 0;
+  int rc;
+  if (b->dirty && (rc = rebuild(b))) return rc;
+  if ((rc = vacgpu_batch_snapshot(b))) return rc;
+  while (!b->snap_queue.empty())
+    if ((rc = snap_take(b))) return rc;
+  return sync_all(b);
 }
 
 int vacgpu_batch_status(vacgpu_batch* b, int slot, int* it, double* t, double* dt, int* done,
                         int* nerror) {
-  if (!b || slot < 0 || slot >= b->cap) return VACGPU_E_ARG;
-  const BScal& S = b->h_bs[slot];
+  if (!b || slot < 0 || slot >= b->cap || !b->inst[slot].used) return VACGPU_E_ARG;
+  const HostInst& h = b->inst[slot];
+  if (b->snap_cur < 0 || !h.joined || b->snap_seqno[b->snap_cur] < h.join_seq) {
+    // no snapshot taken since the instance joined: its admission values
+    if (it) *it = h.it0;
+    if (t) *t = h.t0;
+    if (dt) *dt = 0.0;
+    if (done) *done = 0;
+    if (nerror)
+      for (int i = 0; i < 4; ++i) nerror[i] = 0;
+    return 0;
+  }
+  const BScal& S = b->h_snap[b->snap_cur][slot];
   if (it) *it = S.s.it;
   if (t) *t = S.s.t;
   if (dt) *dt = S.s.dt;
@@ -878,32 +1228,93 @@ int vacgpu_batch_status(vacgpu_batch* b, int slot, int* it, double* t, double* d
   return 0;
 }
 
-int vacgpu_batch_get_state(vacgpu_batch* b, int slot, double* w, const int* ld) {
-  if (!b || !w || slot < 0 || slot >= b->cap || !b->inst[slot].used) return VACGPU_E_ARG;
-  const HostInst& h = b->inst[slot];
-  const InstDev& I = h.dev;
-  const int nw = I.mhd ? 6 : 4;
-  const int l0 = ld ? ld[0] : I.geo.ng[0], l1 = ld ? ld[1] : I.geo.ng[1];
-  const size_t hbytes = sizeof(double) * (size_t)nw * l0 * l1;
+int vacgpu_batch_finished(vacgpu_batch* b, int* slots, int cap) {
+  if (!b || (!slots && cap > 0)) return VACGPU_E_ARG;
+  if (b->snap_cur < 0) return 0;
+  const BScal* S = b->h_snap[b->snap_cur];
+  const long long seq = b->snap_seqno[b->snap_cur];
+  int n = 0;
+  for (int i = 0; i < b->cap; ++i) {
+    const HostInst& h = b->inst[i];
+    if (!h.used || !h.joined || seq < h.join_seq || !S[i].done) continue;
+    if (n < cap) slots[n] = i;
+    ++n;
+  }
+  return n;
+}
+
+int vacgpu_batch_fetch_start(vacgpu_batch* b, int slot, int mesh, int* ticket) {
+  if (!b || !ticket || slot < 0 || slot >= b->cap || !b->inst[slot].used) return VACGPU_E_ARG;
   int rc;
-  BHIP(b, hipStreamSynchronize(b->stream));
-  if ((rc = ensure_stage(b, hbytes))) return rc;
+  // the copy runs on the copy stream after the instance's own upload and
+  // after the steps enqueued so far (an instance that has not joined yet
+  // has taken no step: its admitted state is copied)
+  const InstDev& I = b->inst[slot].dev;
+  const int nw = I.mhd ? 6 : 4;
+  const int e0 = mesh ? I.geo.n[0] : I.geo.ng[0], e1 = mesh ? I.geo.n[1] : I.geo.ng[1];
+  const long long o0 = mesh ? 2 + 2 * I.geo.s[1] : 0;
+  const size_t bytes = sizeof(double) * (size_t)nw * e0 * e1;
+  int si;
+  if ((rc = stage_get(b, bytes, &si))) return rc;
+  Stage& St = b->stages[si];
+  hipEvent_t ec;
+  if ((rc = ev_get(b, &ec))) return rc;
+  BHIP(b, hipEventRecord(ec, b->stream));
+  BHIP(b, hipStreamWaitEvent(b->cstream, ec, 0));
+  b->evpool.push_back(ec);
   for (int iw = 0; iw < nw; ++iw)
-    BHIP(b, hipMemcpy2DAsync(b->stage + (size_t)iw * l0 * l1, sizeof(double) * l0,
-                             I.w + iw * I.geo.ps + I.geo.off, sizeof(double) * I.geo.s[1],
-                             sizeof(double) * I.geo.ng[0], I.geo.ng[1], hipMemcpyDeviceToHost, b->stream));
-  BHIP(b, hipStreamSynchronize(b->stream));
-  // rows beyond ixG in the caller's layout are left as the caller had them
-  for (int iw = 0; iw < nw; ++iw)
-    for (int y = 0; y < I.geo.ng[1]; ++y)
-      memcpy(w + (size_t)iw * l0 * l1 + (size_t)y * l0, b->stage + (size_t)iw * l0 * l1 + (size_t)y * l0,
-             sizeof(double) * I.geo.ng[0]);
+    BHIP(b, hipMemcpy2DAsync(St.p + (size_t)iw * e0 * e1, sizeof(double) * e0,
+                             I.w + iw * I.geo.ps + I.geo.off + o0, sizeof(double) * I.geo.s[1],
+                             sizeof(double) * e0, e1, hipMemcpyDeviceToHost, b->cstream));
+  BHIP(b, hipEventRecord(St.ev, b->cstream));
+  St.recorded = true;
+  St.held = true;
+  int k = 0;
+  while (k < (int)b->fetches.size() && b->fetches[k].used) ++k;
+  if (k == (int)b->fetches.size()) b->fetches.emplace_back();
+  b->fetches[k] = Fetch{true, si, nw, e0, e1};
+  *ticket = k;
   return 0;
+}
+
+int vacgpu_batch_fetch_ready(vacgpu_batch* b, int ticket, double* w, const int* ld, int wait) {
+  if (!b || ticket < 0 || ticket >= (int)b->fetches.size() || !b->fetches[ticket].used) return VACGPU_E_ARG;
+  Fetch& F = b->fetches[ticket];
+  Stage& St = b->stages[F.stage];
+  if (!wait && !ev_done(St.ev)) return 0;
+  BHIP(b, hipEventSynchronize(St.ev));
+  const int l0 = ld ? ld[0] : F.ng0, l1 = ld ? ld[1] : F.ng1;
+  if (w) {
+    if (l0 < F.ng0 || l1 < F.ng1) return bfail(b, VACGPU_E_ARG, "leading dimensions below the fetched extent");
+    // rows beyond the fetched extent in the caller's layout are left as the caller had them
+    if (l0 == F.ng0 && l1 == F.ng1) {
+      const size_t bytes = sizeof(double) * (size_t)F.nw * F.ng0 * F.ng1;
+      b->copier->copy(reinterpret_cast<char*>(w), bytes, reinterpret_cast<const char*>(St.p), bytes,
+                      bytes, 1);
+    } else {
+      for (int iw = 0; iw < F.nw; ++iw)
+        b->copier->copy(reinterpret_cast<char*>(w + (size_t)iw * l0 * l1), sizeof(double) * l0,
+                        reinterpret_cast<const char*>(St.p + (size_t)iw * F.ng0 * F.ng1),
+                        sizeof(double) * F.ng0, sizeof(double) * F.ng0, F.ng1);
+    }
+  }
+  St.held = false;
+  F = Fetch{};
+  return 1;
+}
+
+int vacgpu_batch_get_state(vacgpu_batch* b, int slot, double* w, const int* ld) {
+  if (!b || !w) return VACGPU_E_ARG;
+  int ticket, rc;
+  if ((rc = vacgpu_batch_fetch_start(b, slot, 0, &ticket))) return rc;
+  rc = vacgpu_batch_fetch_ready(b, ticket, w, ld, 1);
+  return rc == 1 ? 0 : rc;
 }
 
 int vacgpu_batch_graph(vacgpu_batch* b, int enable) {
   if (!b) return VACGPU_E_ARG;
-  BHIP(b, hipStreamSynchronize(b->stream));
+  int rc;
+  if ((rc = sync_all(b))) return rc;
   b->use_graph = enable != 0;
   for (int p = 0; p < 2; ++p)
     if (b->graph[p]) { (void)hipGraphExecDestroy(b->graph[p]); b->graph[p] = nullptr; }
@@ -913,16 +1324,20 @@ int vacgpu_batch_graph(vacgpu_batch* b, int enable) {
 // Autotuner: for every size class present, time the predictor + corrector
 // launches of both directions for each candidate tile shape (64 x rows) and
 // keep the fastest.  The timed launches write only the scratch arrays b and
-// w1, so the state is unchanged.
+// w1 and the partials (the fused sweeps without their fold), so the state
+// is unchanged.
 int vacgpu_batch_tune(vacgpu_batch* b, int* rows_out) {
   if (!b) return VACGPU_E_ARG;
   int rc;
   if (b->dirty && (rc = rebuild(b))) return rc;
+  if ((rc = sync_all(b))) return rc;
   hipEvent_t e0, e1;
   BHIP(b, hipEventCreate(&e0));
   BHIP(b, hipEventCreate(&e1));
   Work* tmp = nullptr;
   int tmp_cap = 0;
+  double* tpart = nullptr;
+  BHIP(b, hipMalloc(&tpart, sizeof(double) * 64));
   for (int c = 0; c < NCLASS; ++c) {
     // tiles of class-c instances per physics/tier key, for each candidate shape
     bool present = false;
@@ -965,9 +1380,9 @@ int vacgpu_batch_tune(vacgpu_batch* b, int* rows_out) {
           constexpr int ND = 2;
           BDISPATCH(sg.first.mhd, sg.first.f32, {
             hipLaunchKernelGGL((kb_pred<ND, MHD, R, 0>), dim3(n), blk, 0, b->stream, b->d_inst, b->d_bs, W, 0);
-            hipLaunchKernelGGL((kb_corr<ND, MHD, R, 0>), dim3(n), blk, 0, b->stream, b->d_inst, b->d_bs, W, 0, b->part, b->pstride);
+            hipLaunchKernelGGL((kb_corr<ND, MHD, R, 0>), dim3(n), blk, 0, b->stream, b->d_inst, b->d_bs, W, 0, tpart, (long long)0);
             hipLaunchKernelGGL((kb_pred<ND, MHD, R, 1>), dim3(n), blk, 0, b->stream, b->d_inst, b->d_bs, W, 0);
-            hipLaunchKernelGGL((kb_corr<ND, MHD, R, 1>), dim3(n), blk, 0, b->stream, b->d_inst, b->d_bs, W, 0, b->part, b->pstride);
+            hipLaunchKernelGGL((kb_corr<ND, MHD, R, 1>), dim3(n), blk, 0, b->stream, b->d_inst, b->d_bs, W, 0, tpart, (long long)0);
           });
         }
         BHIP(b, hipEventRecord(e1, b->stream));
@@ -993,7 +1408,7 @@ int vacgpu_batch_tune(vacgpu_batch* b, int* rows_out) {
       for (int i = 0; i < (int)b->inst.size(); ++i) {
         const HostInst& h = b->inst[i];
         if (!h.used || h.klass != c || !h.fused) continue;
-        b->h_inst[i].seglen = sl;
+        b->hdev[i].seglen = sl;
         const Geo& g = h.dev.geo;
         const int strips = (g.n[0] + FS_STRIDE - 1) / FS_STRIDE;
         const int sblocks = ((g.n[1] + sl - 1) / sl + 3) / 4;
@@ -1001,7 +1416,7 @@ int vacgpu_batch_tune(vacgpu_batch* b, int* rows_out) {
         for (int sb = 0; sb < sblocks; ++sb)
           for (int st = 0; st < strips; ++st) lists[k].push_back(Work{i, st, sb, 0, 0});
       }
-      BHIP(b, hipMemcpyAsync(b->d_inst, b->h_inst, sizeof(InstDev) * b->inst.size(), hipMemcpyHostToDevice, b->stream));
+      BHIP(b, hipMemcpy(b->d_inst, b->hdev.data(), sizeof(InstDev) * b->inst.size(), hipMemcpyHostToDevice));
       size_t total = 0;
       for (auto& kv : lists) total += kv.second.size();
       if ((int)total > tmp_cap) {
@@ -1024,7 +1439,7 @@ int vacgpu_batch_tune(vacgpu_batch* b, int* rows_out) {
           const int n = sg.second.second;
           const int lim = sg.first.lim;
           const dim3 fb(64, 4);
-#define KBT(DV, LV) hipLaunchKernelGGL((kb_fused<MHD, R, DV, LV>), dim3(n), fb, 0, b->stream, b->d_inst, b->d_bs, W, 0, b->part, b->pstride)
+#define KBT(DV, LV) hipLaunchKernelGGL((kb_fused<MHD, R, DV, LV>), dim3(n), fb, 0, b->stream, b->d_inst, b->d_bs, W, 0, b->d_fold, 0)
 #define KBTL(DV) do { if (lim == LIM_MINMOD) KBT(DV, LIM_MINMOD); else if (lim == LIM_WOODWARD) KBT(DV, LIM_WOODWARD); else KBT(DV, LIM_SUPERBEE); } while (0)
           BDISPATCH(sg.first.mhd, sg.first.f32, { KBTL(0); KBTL(1); });
 #undef KBTL
@@ -1042,6 +1457,7 @@ int vacgpu_batch_tune(vacgpu_batch* b, int* rows_out) {
     b->seglen[c] = best_sl;
   }
   if (tmp) BHIP(b, hipFree(tmp));
+  BHIP(b, hipFree(tpart));
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   b->dirty = true;
@@ -1068,7 +1484,6 @@ int vacgpu_batch_set_kernels(vacgpu_batch* b, int fused, const int* seglen) {
   if (seglen)
     for (int c = 0; c < NCLASS; ++c)
       if (seglen[c] < 4 || seglen[c] > 256) return bfail(b, VACGPU_E_ARG, "segment length must be in [4, 256]");
-  BHIP(b, hipStreamSynchronize(b->stream));
   b->use_fused = fused != 0;
   if (seglen)
     for (int c = 0; c < NCLASS; ++c) b->seglen[c] = seglen[c];
@@ -1092,6 +1507,8 @@ const char* vacgpu_batch_last_error(const vacgpu_batch* b) { return b ? b->err.c
 void vacgpu_batch_destroy(vacgpu_batch* b) {
   if (!b) return;
   if (b->stream) (void)hipStreamSynchronize(b->stream);
+  if (b->cstream) (void)hipStreamSynchronize(b->cstream);
+  (void)reap(b, true);
   for (auto& h : b->inst)
     if (h.used) {
       (void)hipFree(h.dev.w);
@@ -1099,24 +1516,37 @@ void vacgpu_batch_destroy(vacgpu_batch* b) {
       (void)hipFree(h.dev.w1);
       (void)hipFree(const_cast<double*>(h.dev.geo.dx[0]));
     }
-  for (auto& kv : b->lists) {
-    if (kv.second.all.d) (void)hipFree(kv.second.all.d);
+  for (auto& kv : b->lists)
     for (int c = 0; c < 2; ++c)
       if (kv.second.ph[c].d) (void)hipFree(kv.second.ph[c].d);
-  }
   for (auto& kv : b->pool) (void)hipFree(kv.second);
   if (b->gw4.d) (void)hipFree(b->gw4.d);
   if (b->gw6.d) (void)hipFree(b->gw6.d);
   if (b->d_ids) (void)hipFree(b->d_ids);
+  if (b->d_idsu) (void)hipFree(b->d_idsu);
   if (b->part) (void)hipFree(b->part);
   for (int p = 0; p < 2; ++p)
     if (b->graph[p]) (void)hipGraphExecDestroy(b->graph[p]);
   if (b->d_inst) (void)hipFree(b->d_inst);
   if (b->d_bs) (void)hipFree(b->d_bs);
-  if (b->h_bs) (void)hipHostFree(b->h_bs);
-  if (b->h_inst) (void)hipHostFree(b->h_inst);
-  if (b->stage) (void)hipHostFree(b->stage);
+  if (b->d_fold) (void)hipFree(b->d_fold);
+  for (int m = 0; m < NSNAP; ++m) {
+    if (b->h_snap[m]) (void)hipHostFree(b->h_snap[m]);
+    if (b->ev_snap[m]) (void)hipEventDestroy(b->ev_snap[m]);
+  }
+  for (auto& S : b->stages) {
+    if (S.p) (void)hipHostFree(S.p);
+    if (S.ev) (void)hipEventDestroy(S.ev);
+  }
+  for (int a = 0; a < 2; ++a) {
+    if (b->arena[a].h) (void)hipHostFree(b->arena[a].h);
+    if (b->arena[a].ev) (void)hipEventDestroy(b->arena[a].ev);
+  }
+  for (hipEvent_t e : b->evpool) (void)hipEventDestroy(e);
+  if (b->ev_admit) (void)hipEventDestroy(b->ev_admit);
   if (b->stream) (void)hipStreamDestroy(b->stream);
+  if (b->cstream) (void)hipStreamDestroy(b->cstream);
+  delete b->copier;
   delete b;
 }
 

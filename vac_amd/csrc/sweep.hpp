@@ -1304,21 +1304,48 @@ __device__ __forceinline__ double tvdlf_fy_body(const Geo& g, const Sel& s, cons
 
 #include "bulk32.hpp"
 
+// Cross-block Courant maximum of a launch without a memory fence (a
+// device-scope fence writes back and invalidates the whole L2 of the XCD,
+// once per block).  fold[0] counts finished blocks, fold[1] holds the
+// maximum as the bit pattern of a non-negative double (those order as the
+// values; a block maximum that is not above zero -- -inf for a block
+// without faces, -0, NaN -- enters as +0, which no use of the maximum tells
+// apart: dt*cm > 1 and cm > smalldouble are both false either way).  Each
+// block takes its maximum into fold[1] with an atomic max, then counts
+// itself; the count's address depends on the max's returned value (which
+// is never all ones), so the count is issued only after the max has been
+// performed.  Device-scope atomic read-modify-writes are performed at one
+// coherence point, so the block whose count completes the launch reads
+// every block's max with its exchange, and resets both words for the next
+// launch.  Thread 0 only; returns true in that last block, with *cm set.
+__device__ __forceinline__ bool fold_courant_last(unsigned long long* fold, double bm,
+                                                  unsigned long long nblocks, double* cm) {
+  const unsigned long long bits = bm > 0.0 ? (unsigned long long)__double_as_longlong(bm) : 0ull;
+  const unsigned long long old =
+      __hip_atomic_fetch_max(fold + 1, bits, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  const unsigned long long n = __hip_atomic_fetch_add(fold + (old == ~0ull ? 1 : 0), 1ull,
+                                                      __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  if (n != nblocks - 1) return false;
+  *cm = __longlong_as_double(
+      (long long)__hip_atomic_exchange(fold + 1, 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+  __hip_atomic_store(fold, 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  return true;
+}
+
 #ifndef VACGPU_FUSED_WAVES
 #define VACGPU_FUSED_WAVES 1
 #endif
 // blockDim = (64, 4): wave threadIdx.y takes segment blockIdx.y*4 + y of
-// strip blockIdx.x.  The block's Courant maximum goes to part[SLOT_C0]; the
-// last block to finish folds them and applies the dtcourant update of the
-// TVDLF corrector (src/vactvdlf.t:184-201) when courantpar > 0, then resets
-// the counter for the next launch.
+// strip blockIdx.x.  The last block to finish (fold_courant_last on fold[2],
+// zero between launches) applies the dtcourant update of the TVDLF
+// corrector (src/vactvdlf.t:184-201) when courantpar > 0 and, on the last
+// sweep of a device-loop step, advances it and t.
 template <bool MHD, class R, int D, int LIM>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(VACGPU_FUSED_WAVES))) void k_tvdlf_fused(
     Geo g, Sel s, BC bc, const double* __restrict__ w, double* __restrict__ wout,
-    Scal* __restrict__ sc, double* __restrict__ part, long long pstride, int seglen,
-    unsigned* __restrict__ ctr, double courantpar, int step_end) {
+    Scal* __restrict__ sc, int seglen, unsigned long long* __restrict__ fold, double courantpar,
+    int step_end) {
   __shared__ double sm[8];
-  __shared__ int last;
   const int seg = (int)(blockIdx.y * blockDim.y + threadIdx.y);
   const double dt = sc->dt, smallp = sc->smallp;
   double cmx;
@@ -1330,41 +1357,19 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(VACGPU_FUSE
   else
     cmx = tvdlf_fy_body<MHD, R, LIM>(g, s, bc, w, wout, dt, smallp, (int)blockIdx.x, seg, seglen);
   const double bm = block_reduce<true>(cmx, sm);
-  const unsigned nb = gridDim.x * gridDim.y;
-  const int tid = threadIdx.x + blockDim.x * threadIdx.y;
-  if (tid == 0) {
-    __hip_atomic_store(reinterpret_cast<unsigned long long*>(part + SLOT_C0 * pstride) +
-                           block_linear(),
-                       (unsigned long long)__double_as_longlong(bm), __ATOMIC_RELAXED,
-                       __HIP_MEMORY_SCOPE_AGENT);
-    __threadfence();
-    last = atomicAdd(ctr, 1u) == nb - 1;
-  }
-  __syncthreads();
-  if (!last) return;  // block-uniform
-  __threadfence();
-  double v = -INFINITY;
-  for (unsigned i = tid; i < nb; i += blockDim.x * blockDim.y) {
-    const unsigned long long u = __hip_atomic_load(
-        reinterpret_cast<unsigned long long*>(part + SLOT_C0 * pstride) + i, __ATOMIC_RELAXED,
-        __HIP_MEMORY_SCOPE_AGENT);
-    const double x = __longlong_as_double((long long)u);
-    v = v < x ? x : v;
-  }
-  const double cm = block_reduce<true>(v, sm);
-  if (tid == 0) {
-    if (courantpar > 0.0) {
-      if (dt * cm > 1.0) sc->nerror[2] += 1;
-      if (cm > kSmall) {
-        const double cc = courantpar / cm;
-        sc->dtcourant[D] = cc < sc->dtcourant[D] ? cc : sc->dtcourant[D];
-      }
+  if (threadIdx.x != 0 || threadIdx.y != 0) return;
+  double cm;
+  if (!fold_courant_last(fold, bm, (unsigned long long)gridDim.x * gridDim.y, &cm)) return;
+  if (courantpar > 0.0) {
+    if (dt * cm > 1.0) sc->nerror[2] += 1;
+    if (cm > kSmall) {
+      const double cc = courantpar / cm;
+      sc->dtcourant[D] = cc < sc->dtcourant[D] ? cc : sc->dtcourant[D];
     }
-    if (step_end) {  // the last sweep of a device-loop step: it and t advance (src/vac.t:97-98)
-      sc->it += 1;
-      sc->t = sc->t + dt;
-    }
-    atomicExch(ctr, 0u);
+  }
+  if (step_end) {  // the last sweep of a device-loop step: it and t advance (src/vac.t:97-98)
+    sc->it += 1;
+    sc->t = sc->t + dt;
   }
 }
 

@@ -72,7 +72,7 @@ struct vacgpu_grid {
   int ry_seglen = 16;  // k_ryu_ab_y (2D one-pass Ryu step)
   dim3 grid_ry;
   double* stage = nullptr;  // pinned staging of put_state / get_state
-  unsigned* ctr = nullptr;  // block counter of the fused sweeps (zero between launches)
+  unsigned long long* fold = nullptr;  // Courant fold words of the fused sweeps (zero between launches)
   // device loop: the last fused sweep of a step also advances it and t
   bool fold_step_end = false, step_end_done = false;
   size_t stage_bytes = 0;
@@ -589,7 +589,7 @@ int sweep_device(vacgpu_grid* g, const vacgpu_method* m, int d, int firstsweep, 
       const int lim = m->typelimiter[0];
       const int step_end = (g->fold_step_end && lastsweep) ? 1 : 0;
       if (step_end) g->step_end_done = true;
-#define FUSED_L(MV, RT, DV, LV) hipLaunchKernelGGL((k_tvdlf_fused<MV, RT, DV, LV>), g->grid_fs, g->block_fs, 0, g->stream, g->geo, s, g->bc, win, wout, g->sc, g->part, g->pstride, g->fs_seglen, g->ctr, m->courantpar, step_end)
+#define FUSED_L(MV, RT, DV, LV) hipLaunchKernelGGL((k_tvdlf_fused<MV, RT, DV, LV>), g->grid_fs, g->block_fs, 0, g->stream, g->geo, s, g->bc, win, wout, g->sc, g->fs_seglen, g->fold, m->courantpar, step_end)
 #define FUSED(MV, RT, DV) do { if (lim == LIM_MINMOD) FUSED_L(MV, RT, DV, LIM_MINMOD); else if (lim == LIM_WOODWARD) FUSED_L(MV, RT, DV, LIM_WOODWARD); else FUSED_L(MV, RT, DV, LIM_SUPERBEE); } while (0)
       if (mhd) {
         if (f32) { if (d == 0) FUSED(true, float, 0); else FUSED(true, float, 1); }
@@ -1372,8 +1372,8 @@ int vacgpu_create(const vacgpu_config* cfg, const double* dx0, const double* dx1
   if (e == hipSuccess) e = hipMalloc(&g->sc, sizeof(Scal));
   if (e == hipSuccess) e = hipHostMalloc(&g->sc_host, sizeof(Scal), hipHostMallocDefault);
   if (e == hipSuccess) e = hipMalloc(&g->part, sizeof(double) * NSLOT * g->pstride);
-  if (e == hipSuccess) e = hipMalloc(&g->ctr, sizeof(unsigned));
-  if (e == hipSuccess) e = hipMemset(g->ctr, 0, sizeof(unsigned));
+  if (e == hipSuccess) e = hipMalloc(&g->fold, 2 * sizeof(unsigned long long));
+  if (e == hipSuccess) e = hipMemset(g->fold, 0, 2 * sizeof(unsigned long long));
   if (e == hipSuccess) e = hipStreamCreateWithFlags(&g->stream, hipStreamNonBlocking);
   if (e != hipSuccess) {
     fprintf(stderr, "vacgpu_create: %s\n", hipGetErrorString(e));
@@ -1823,7 +1823,7 @@ void vacgpu_destroy(vacgpu_grid* g) {
   if (g->sc_host) (void)hipHostFree(g->sc_host);
   if (g->stage) (void)hipHostFree(g->stage);
   if (g->part) (void)hipFree(g->part);
-  if (g->ctr) (void)hipFree(g->ctr);
+  if (g->fold) (void)hipFree(g->fold);
   if (g->fixw) (void)hipFree(g->fixw);
   {
     auto& P = g->pr;

@@ -97,15 +97,18 @@ def run_batched(instances, steps=100, precision="fp64", budget=64 << 30, graph=T
             "tile_rows_per_class": rows, "fused_seglen_per_class": seglen, "graph": graph}
 
 
-def make_requests(rate, count, seed=7, lo=32, hi=1024):
+def make_requests(rate, count, seed=7, lo=32, hi=1024, quantum=1):
     """Open-loop Poisson arrivals: 50/50 HD or MHD, square side log-uniform in
-    [lo, hi], end time uniform in [0.05, 0.5], tier FP64 or FP32-bulk."""
+    [lo, hi] (rounded to a multiple of quantum), end time uniform in
+    [0.05, 0.5], tier FP64 or FP32-bulk."""
     rng = np.random.default_rng(seed)
     t = 0.0
     out = []
     for i in range(count):
         t += rng.exponential(1.0 / rate)
         n = log_uniform_int(rng, lo, hi)
+        if quantum > 1:
+            n = max(lo, int(round(n / quantum)) * quantum)
         physics = "mhd" if rng.uniform() < 0.5 else "hd"
         precision = "fp64" if rng.uniform() < 0.5 else "fp32bulk"
         t_end = rng.uniform(0.05, 0.5)
@@ -114,21 +117,65 @@ def make_requests(rate, count, seed=7, lo=32, hi=1024):
     return out
 
 
-def serve(requests, budget=32 << 30, capacity=1024, steps_per_poll=2, keep_states=False):
-    """Serve the requests (already prepared) in arrival order; returns metrics."""
+def prepare_shared(requests, cache=None):
+    """Initial states shared by requests of the same side and physics (the
+    state of the first such request's seed): bounds the set-up time of long
+    request streams.  Returns the cache for reuse."""
+    cache = {} if cache is None else cache
+    for r in requests:
+        key = (r.nx, r.ny, r.physics)
+        if key not in cache:
+            cache[key] = r.prepare()
+        src = cache[key]
+        r.w, r.dxs = src.w, src.dxs
+    return cache
+
+
+def serve(requests, budget=32 << 30, capacity=1024, steps_per_poll=2, keep_states=False,
+          graph=True):
+    """Serve the requests (already prepared) in arrival order; returns metrics.
+
+    The loop keeps one batch of steps queued on the device while the host
+    admits arrivals and reads the previous status snapshot: admission,
+    removal, status and result copies are asynchronous (vacgpu_batch), so
+    the host blocks only on the snapshot of the steps before the ones still
+    queued.  A request completes when its result (the final state) has
+    arrived in host memory; latency = completion - arrival.
+    """
     b = vacgpu.Batch(budget, capacity)
     b.set_tiles([4, 4, 4, 4])
+    b.graph(graph)
     queue = collections.deque()
     active = {}
+    fetching = {}
     latencies = []
     results = {}
     cell_updates = 0
     peak_active = 0
     nxt = 0
-    t_admit = t_step = t_poll = 0.0
+    inflight = 0
+    t_admit = t_wait = t_poll = 0.0
     polls = 0
     start = time.perf_counter()
-    while nxt < len(requests) or queue or active:
+
+    def take_snapshot(wait):
+        nonlocal inflight, t_wait, t_poll, polls
+        tw = time.perf_counter()
+        got = b.snapshot_wait(wait)
+        tp = time.perf_counter()
+        t_wait += tp - tw
+        if not got:
+            return
+        inflight -= 1
+        polls += 1
+        for slot in b.finished():
+            st = b.status(slot)
+            r = active.pop(slot)
+            fetching[b.fetch_start(slot)] = (r, st)
+            b.remove(slot)
+        t_poll += time.perf_counter() - tp
+
+    while nxt < len(requests) or queue or active or fetching or inflight:
         now = time.perf_counter() - start
         while nxt < len(requests) and requests[nxt].arrival <= now:
             queue.append(requests[nxt])
@@ -143,40 +190,46 @@ def serve(requests, budget=32 << 30, capacity=1024, steps_per_poll=2, keep_state
                 break
             queue.popleft()
             active[slot] = r
+        t_admit += time.perf_counter() - ta
         peak_active = max(peak_active, len(active))
-        if not active:
-            if nxt < len(requests):
-                time.sleep(max(0.0, min(0.001, requests[nxt].arrival - now)))
-            continue
-        ts = time.perf_counter()
-        t_admit += ts - ta
-        b.step(steps_per_poll)
-        b.sync()
-        tp = time.perf_counter()
-        t_step += tp - ts
-        polls += 1
-        done_at = tp - start
-        for slot in list(active):
-            st = b.status(slot)
-            if st["done"]:
-                r = active.pop(slot)
-                cell_updates += r.cells * st["it"]
-                if keep_states:
-                    results[id(r)] = (b.get_mesh(slot), st)
-                b.remove(slot)
-                latencies.append(done_at - r.arrival)
-        t_poll += time.perf_counter() - tp
+        if active:
+            b.step(steps_per_poll)
+            b.snapshot()
+            inflight += 1
+        if inflight >= 2 or (inflight and not active):
+            take_snapshot(True)
+        elif inflight:
+            take_snapshot(False)
+        tf = time.perf_counter()
+        for tk in list(fetching):
+            w = b.fetch_ready(tk, wait=False)
+            if w is None:
+                continue
+            r, st = fetching.pop(tk)
+            done_at = time.perf_counter() - start
+            cell_updates += r.cells * st["it"]
+            latencies.append(done_at - r.arrival)
+            if keep_states:
+                results[id(r)] = (w, st)
+        t_poll += time.perf_counter() - tf
+        if not active and not inflight and nxt < len(requests) and not queue:
+            if fetching:
+                continue
+            time.sleep(max(0.0, min(0.001, requests[nxt].arrival - (time.perf_counter() - start))))
     total = time.perf_counter() - start
     b.close()
     lat = np.array(latencies)
+    span = requests[-1].arrival - requests[0].arrival if len(requests) > 1 else 0.0
     out = {"requests": len(requests), "seconds": total,
+           "arrival_span_s": span,
            "p50_ms": float(np.percentile(lat, 50) * 1e3),
            "p99_ms": float(np.percentile(lat, 99) * 1e3),
            "mean_ms": float(lat.mean() * 1e3),
            "mcell_updates_per_s": cell_updates / total / 1e6,
+           "offered_mcell_updates_per_s": cell_updates / span / 1e6 if span > 0 else None,
            "peak_resident": peak_active, "polls": polls,
-           "host_s": {"admit": round(t_admit, 4), "step_and_sync": round(t_step, 4),
-                      "poll_and_remove": round(t_poll, 4)}}
+           "host_s": {"admit": round(t_admit, 4), "wait_for_device": round(t_wait, 4),
+                      "status_fetch_remove": round(t_poll, 4)}}
     if keep_states:
         out["results"] = results
     return out

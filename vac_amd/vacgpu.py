@@ -89,6 +89,13 @@ SIGNATURES = {
     "vacgpu_batch_sync": ([C.c_void_p], C.c_int),
     "vacgpu_batch_status": ([C.c_void_p, C.c_int, _IP, _DP, _DP, _IP, _IP], C.c_int),
     "vacgpu_batch_get_state": ([C.c_void_p, C.c_int, _DP, _IP], C.c_int),
+    "vacgpu_batch_snapshot": ([C.c_void_p], C.c_int),
+    "vacgpu_batch_snapshot_wait": ([C.c_void_p, C.c_int], C.c_int),
+    "vacgpu_batch_fetch_start": ([C.c_void_p, C.c_int, C.c_int, _IP], C.c_int),
+    "vacgpu_batch_finished": ([C.c_void_p, _IP, C.c_int], C.c_int),
+    "vacgpu_batch_add_mesh": ([C.c_void_p, C.POINTER(Config), C.POINTER(Method), _DP, _DP, _DP,
+                               C.c_int, C.c_double, C.c_int, C.c_double, C.c_int, _IP], C.c_int),
+    "vacgpu_batch_fetch_ready": ([C.c_void_p, C.c_int, _DP, _IP, C.c_int], C.c_int),
     "vacgpu_batch_graph": ([C.c_void_p, C.c_int], C.c_int),
     "vacgpu_batch_tune": ([C.c_void_p, _IP], C.c_int),
     "vacgpu_batch_set_tiles": ([C.c_void_p, _IP], C.c_int),
@@ -424,6 +431,7 @@ class Batch:
             raise VacGpuError(f"vacgpu_batch_create failed ({rc})")
         self.h = h
         self.meta = {}
+        self._tickets = {}
 
     def _check(self, rc, what):
         if rc < 0:
@@ -441,14 +449,15 @@ class Batch:
             precision="fp64", it=0, t=0.0, itmax=1 << 30, tmax=1e99, tmaxexact=False) -> int:
         cfg = make_config(nx, physics, gamma, bc=bc, precision=precision)
         m = method.to_c(physics)
-        w = np.ascontiguousarray(np.pad(np.asarray(w_mesh, dtype=np.float64),
-                                        [(0, 0), (2, 2), (2, 2)]))
+        w = np.ascontiguousarray(w_mesh, dtype=np.float64)  # the mesh; ghosts are filled on the device
+        if w.shape != (nvars(physics, 2), int(nx[1]), int(nx[0])):
+            raise ValueError(f"mesh state of shape {w.shape} for a {nx[0]}x{nx[1]} {physics} instance")
         dx0 = np.ascontiguousarray(dxs[0], dtype=np.float64)
         dx1 = np.ascontiguousarray(dxs[1], dtype=np.float64)
         slot = C.c_int()
-        rc = self.lib.vacgpu_batch_add(self.h, C.byref(cfg), C.byref(m), _ptr(dx0), _ptr(dx1),
-                                       _ptr(w), None, it, t, itmax, tmax, int(tmaxexact),
-                                       C.byref(slot))
+        rc = self.lib.vacgpu_batch_add_mesh(self.h, C.byref(cfg), C.byref(m), _ptr(dx0), _ptr(dx1),
+                                            _ptr(w), it, t, itmax, tmax, int(tmaxexact),
+                                            C.byref(slot))
         if rc == -4:
             raise Batch.BudgetFull(self.lib.vacgpu_batch_last_error(self.h).decode())
         self._check(rc, "batch_add")
@@ -480,6 +489,48 @@ class Batch:
         w = np.empty((nw, nx[1] + 4, nx[0] + 4))
         self._check(self.lib.vacgpu_batch_get_state(self.h, slot, _ptr(w), None), "get_state")
         return np.ascontiguousarray(w[:, 2:-2, 2:-2])
+
+    def snapshot(self):
+        """Enqueue a status snapshot after the steps enqueued so far (no wait)."""
+        self._check(self.lib.vacgpu_batch_snapshot(self.h), "batch_snapshot")
+
+    def snapshot_wait(self, wait: bool = True) -> bool:
+        """Make the oldest snapshot in flight current; False when none is in
+        flight or (wait=False) it has not arrived yet."""
+        rc = self.lib.vacgpu_batch_snapshot_wait(self.h, int(wait))
+        if rc < 0:
+            self._check(rc, "batch_snapshot_wait")
+        return rc == 1
+
+    def finished(self) -> list:
+        """Slots done in the current snapshot."""
+        n = self.lib.vacgpu_batch_finished(self.h, None, 0)
+        if n <= 0:
+            self._check(n, "batch_finished")
+            return []
+        out = np.zeros(n, dtype=np.int32)
+        n = self.lib.vacgpu_batch_finished(self.h, out.ctypes.data_as(_IP), n)
+        return [int(x) for x in out[:n]]
+
+    def fetch_start(self, slot: int) -> int:
+        """Enqueue a copy of the instance's mesh after the steps enqueued so
+        far; returns a ticket for fetch_ready.  The slot may be removed at once."""
+        nx, nw = self.meta[slot]
+        tk = C.c_int()
+        self._check(self.lib.vacgpu_batch_fetch_start(self.h, slot, 1, C.byref(tk)), "fetch_start")
+        self._tickets[tk.value] = np.empty((nw, nx[1], nx[0]))
+        return tk.value
+
+    def fetch_ready(self, ticket: int, wait: bool = False):
+        """The fetched mesh (nw, ny, nx), or None when (wait=False) it has not arrived."""
+        w = self._tickets[ticket]
+        rc = self.lib.vacgpu_batch_fetch_ready(self.h, ticket, _ptr(w), None, int(wait))
+        if rc < 0:
+            self._check(rc, "fetch_ready")
+        if rc == 0:
+            return None
+        del self._tickets[ticket]
+        return w
 
     def graph(self, enable: bool):
         self._check(self.lib.vacgpu_batch_graph(self.h, int(enable)), "batch_graph")
