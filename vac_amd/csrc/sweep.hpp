@@ -103,8 +103,42 @@ __device__ __forceinline__ CellIx cell_at(const Geo& g, int bx, int by, int bz) 
   return r;
 }
 
+// Tile of a single-grid launch, XCD-aware.  The hardware hands the blocks of
+// a launch to the eight XCDs in turn (block L to XCD L mod 8), each with its
+// own L2; tiles that share halo rows, planes or cache lines should run on
+// the same XCD at about the same time.  So XCD j takes the j-th contiguous
+// share of a tile order: block L, the (L / 8)-th block on its XCD, gets
+// order position start(L mod 8) + L / 8.  The order runs x fastest, then y
+// within a group of 8 tile rows, then z, then the groups: neighbours along
+// x are adjacent, along y 1 x-row apart (except across groups), along z one
+// group layer (8 x-rows) apart, so a tile's halos were loaded by recent
+// tiles of its XCD.  A bijection for any grid size, so every tile is
+// computed exactly once whatever the dispatch order really is.
+struct Tile {
+  int x, y, z;
+};
+__device__ __forceinline__ Tile xcd_tile() {
+  constexpr unsigned YG = 8;  // tile rows per group
+  const unsigned gx = gridDim.x, gy = gridDim.y, gz = gridDim.z;
+  const unsigned n = gx * gy * gz;
+  const unsigned L = blockIdx.x + gx * (blockIdx.y + gy * blockIdx.z);
+  const unsigned j = L & 7u, p = L >> 3, q = n >> 3, r = n & 7u;
+  unsigned t = j < r ? j * (q + 1) + p : r * (q + 1) + (j - r) * q + p;
+  const unsigned full = gy / YG, per = gx * YG * gz;
+  unsigned g = t / per, rows = YG;
+  if (g >= full) {  // the last, partial group
+    g = full;
+    rows = gy - full * YG;
+  }
+  t -= g * per;
+  const unsigned bx = t % gx;
+  t /= gx;
+  return Tile{(int)bx, (int)(g * YG + t % rows), (int)(t / rows)};
+}
+
 __device__ __forceinline__ CellIx cell_of(const Geo& g) {
-  return cell_at(g, (int)blockIdx.x, (int)blockIdx.y, (int)blockIdx.z);
+  const Tile b = xcd_tile();
+  return cell_at(g, b.x, b.y, b.z);
 }
 
 __device__ __forceinline__ long long block_linear() {
@@ -533,9 +567,9 @@ __global__ __launch_bounds__(256) void k_tvdlf_corr(Geo g, Sel s, const double* 
                                                    const Scal* __restrict__ sc,
                                                    double* __restrict__ part, long long pstride) {
   __shared__ double sm[8];
-  const double cmx = tvdlf_corr_body<ND, MHD, R, D>(g, s, w, w1, wout, sc->dt, sc->smallp,
-                                                    (int)blockIdx.x, (int)blockIdx.y,
-                                                    (int)blockIdx.z);
+  const Tile tb = xcd_tile();
+  const double cmx = tvdlf_corr_body<ND, MHD, R, D>(g, s, w, w1, wout, sc->dt, sc->smallp, tb.x,
+                                                    tb.y, tb.z);
   const double bm = block_reduce<true>(cmx, sm);
   if (threadIdx.x == 0 && threadIdx.y == 0) part[SLOT_C0 * pstride + block_linear()] = bm;
 }
@@ -565,6 +599,12 @@ constexpr int FS_STRIDE = 62;
 __device__ __forceinline__ int ghost_image(int c, int n, bool per_lo, bool per_hi) {
   const int lo = per_lo ? c + n : 2, hi = per_hi ? c - n : n + 1;
   return c < 2 ? lo : (c > n + 1 ? hi : c);
+}
+
+// whether mesh cell c (n >= 4) is the image of some ghost cell: the cheap
+// test that lets interior cells skip ghost_preimages
+__device__ __forceinline__ bool has_preimage(int c, int n, bool per_lo, bool per_hi) {
+  return (per_lo ? c >= n : c == 2) || (per_hi ? c <= 3 : c == n + 1);
 }
 
 // ghost cells whose image is mesh cell c (n >= 4), written to t; returns the count
@@ -612,6 +652,10 @@ __device__ __forceinline__ void store_cell(const Geo& g, const BC& bc, double* _
 #pragma unroll
   for (int iw = 0; iw < NW; ++iw) wout[iw * g.ps + idx] = out[iw];
   if (!ghosts) return;
+  bool edge = false;
+#pragma unroll
+  for (int k = 0; k < ND; ++k) edge |= has_preimage(c[k], g.n[k], bc.t[2 * k][0], bc.t[2 * k + 1][0]);
+  if (!edge) return;  // an interior cell
   int t[3][5], nt[3] = {1, 1, 1};
 #pragma unroll
   for (int k = 0; k < ND; ++k) {
@@ -631,8 +675,9 @@ __device__ __forceinline__ void store_cell(const Geo& g, const BC& bc, double* _
 }
 
 // predictor launch: one cell per thread; with ghosts set the ghost cells of
-// w1 that take a cell's value are written too (one boundary type per face)
-template <int ND, bool MHD, class R, int D, bool PRIM = false>
+// w1 that take a cell's value are written too (one boundary type per face).
+// LIM >= 0: one limiter type for every variable and no 'symm0' face.
+template <int ND, bool MHD, class R, int D, bool PRIM = false, int LIM = -1>
 __global__ __launch_bounds__(256) void k_tvdlf_pred(Geo g, Sel s, BC bc, const double* __restrict__ w,
                                                    double* __restrict__ w1,
                                                    const Scal* __restrict__ sc, int ghosts) {
@@ -640,7 +685,7 @@ __global__ __launch_bounds__(256) void k_tvdlf_pred(Geo g, Sel s, BC bc, const d
   const CellIx cl = cell_of(g);
   if (!cl.ok) return;
   double out[NW];
-  pred_point<ND, MHD, R, D, -1, 2, PRIM>(g, s, w, cl.idx, cl.c, sc->dt, sc->smallp, out);
+  pred_point<ND, MHD, R, D, LIM, 2, PRIM>(g, s, w, cl.idx, cl.c, sc->dt, sc->smallp, out);
   store_cell<ND, NW>(g, bc, w1, cl.idx, cl.c, out, ghosts);
 }
 
@@ -666,9 +711,10 @@ __global__ __launch_bounds__(256) void k_tvdlf_corrx(Geo g, Sel s, BC bc, const 
   __shared__ double sm[8];
   const double dt = sc->dt, smallp = sc->smallp;
   const int lane = (int)threadIdx.x;
-  const int c = 1 + (int)blockIdx.x * CX_STRIDE + lane;
-  const int y = 2 + (int)(blockIdx.y * BY + threadIdx.y);
-  const int z = (ND == 3) ? 2 + (int)blockIdx.z : 0;
+  const Tile tb = xcd_tile();
+  const int c = 1 + tb.x * CX_STRIDE + lane;
+  const int y = 2 + (int)(tb.y * BY + threadIdx.y);
+  const int z = (ND == 3) ? 2 + tb.z : 0;
   const bool rowok = y <= g.n[1] + 1;
   const bool useful = rowok && lane >= 1 && lane <= CX_STRIDE && c <= g.n[0] + 1;
   const bool face_ok = rowok && lane <= CX_STRIDE && c <= g.n[0] + 1;
@@ -752,10 +798,11 @@ __global__ __launch_bounds__(256) void k_tvdlf_corrm(Geo g, Sel s, BC bc, const 
   constexpr int NW = L::nw;
   __shared__ double sm[8];
   const double dt = sc->dt, smallp = sc->smallp;
-  const int x = 2 + (int)(blockIdx.x * BX + threadIdx.x);
-  const int seg = (int)(blockIdx.y * BY + threadIdx.y);
+  const Tile tb = xcd_tile();
+  const int x = 2 + (int)(tb.x * BX + threadIdx.x);
+  const int seg = (int)(tb.y * BY + threadIdx.y);
   const int s0 = 2 + seg * seglen;
-  const int t = (ND == 3) ? 2 + (int)blockIdx.z : 0;  // the coordinate that is neither x nor D
+  const int t = (ND == 3) ? 2 + tb.z : 0;  // the coordinate that is neither x nor D
   const int other = (D == 1) ? 2 : 1;
   const int nd_ = g.n[D];
   double cmx = -INFINITY;
@@ -1333,10 +1380,10 @@ __device__ __forceinline__ bool fold_courant_last(unsigned long long* fold, doub
 }
 
 #ifndef VACGPU_FUSED_WAVES
-#define VACGPU_FUSED_WAVES 1
+#define VACGPU_FUSED_WAVES 2  // at most 256 VGPRs: two waves per SIMD
 #endif
-// blockDim = (64, 4): wave threadIdx.y takes segment blockIdx.y*4 + y of
-// strip blockIdx.x.  The last block to finish (fold_courant_last on fold[2],
+// blockDim = (64, 4): wave threadIdx.y takes segment 4*tile.y + y of strip
+// tile.x (xcd_tile).  The last block to finish (fold_courant_last on fold[2],
 // zero between launches) applies the dtcourant update of the TVDLF
 // corrector (src/vactvdlf.t:184-201) when courantpar > 0 and, on the last
 // sweep of a device-loop step, advances it and t.
@@ -1346,16 +1393,17 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(VACGPU_FUSE
     Scal* __restrict__ sc, int seglen, unsigned long long* __restrict__ fold, double courantpar,
     int step_end) {
   __shared__ double sm[8];
-  const int seg = (int)(blockIdx.y * blockDim.y + threadIdx.y);
+  const Tile tb = xcd_tile();
+  const int seg = (int)(tb.y * blockDim.y + threadIdx.y);
   const double dt = sc->dt, smallp = sc->smallp;
   double cmx;
   if constexpr (sizeof(R) == 4)  // FP32-bulk: increments over FP64 bases
-    cmx = D == 0 ? bulk_fx_body<MHD, LIM>(g, s, bc, w, wout, dt, smallp, (int)blockIdx.x, seg, seglen)
-                 : bulk_fy_body<MHD, LIM>(g, s, bc, w, wout, dt, smallp, (int)blockIdx.x, seg, seglen);
+    cmx = D == 0 ? bulk_fx_body<MHD, LIM>(g, s, bc, w, wout, dt, smallp, tb.x, seg, seglen)
+                 : bulk_fy_body<MHD, LIM>(g, s, bc, w, wout, dt, smallp, tb.x, seg, seglen);
   else if constexpr (D == 0)
-    cmx = tvdlf_fx_body<MHD, R, LIM>(g, s, bc, w, wout, dt, smallp, (int)blockIdx.x, seg, seglen);
+    cmx = tvdlf_fx_body<MHD, R, LIM>(g, s, bc, w, wout, dt, smallp, tb.x, seg, seglen);
   else
-    cmx = tvdlf_fy_body<MHD, R, LIM>(g, s, bc, w, wout, dt, smallp, (int)blockIdx.x, seg, seglen);
+    cmx = tvdlf_fy_body<MHD, R, LIM>(g, s, bc, w, wout, dt, smallp, tb.x, seg, seglen);
   const double bm = block_reduce<true>(cmx, sm);
   if (threadIdx.x != 0 || threadIdx.y != 0) return;
   double cm;
@@ -1614,9 +1662,10 @@ __global__ __launch_bounds__(256) void k_tvdmu_x(Geo g, Sel s, BC bc, const doub
   __shared__ double sm[8];
   const double dt = sc->dt, smallp = sc->smallp;
   const int lane = (int)threadIdx.x;
-  const int c = 1 + (int)blockIdx.x * CX_STRIDE + lane;
-  const int y = 2 + (int)(blockIdx.y * BY + threadIdx.y);
-  const int z = (ND == 3) ? 2 + (int)blockIdx.z : 0;
+  const Tile tb = xcd_tile();
+  const int c = 1 + tb.x * CX_STRIDE + lane;
+  const int y = 2 + (int)(tb.y * BY + threadIdx.y);
+  const int z = (ND == 3) ? 2 + tb.z : 0;
   const bool rowok = y <= g.n[1] + 1;
   const bool useful = rowok && lane >= 1 && lane <= CX_STRIDE && c <= g.n[0] + 1;
   const bool face_ok = rowok && lane <= CX_STRIDE && c <= g.n[0] + 1;
@@ -1715,10 +1764,11 @@ __global__ __launch_bounds__(256) void k_tvdmu_m(Geo g, Sel s, BC bc, const doub
   constexpr int NW = L::nw;
   __shared__ double sm[8];
   const double dt = sc->dt, smallp = sc->smallp;
-  const int x = 2 + (int)(blockIdx.x * BX + threadIdx.x);
-  const int seg = (int)(blockIdx.y * BY + threadIdx.y);
+  const Tile tb = xcd_tile();
+  const int x = 2 + (int)(tb.x * BX + threadIdx.x);
+  const int seg = (int)(tb.y * BY + threadIdx.y);
   const int s0 = 2 + seg * seglen;
-  const int t = (ND == 3) ? 2 + (int)blockIdx.z : 0;  // the coordinate that is neither x nor D
+  const int t = (ND == 3) ? 2 + tb.z : 0;  // the coordinate that is neither x nor D
   const int other = (D == 1) ? 2 : 1;
   const int nd_ = g.n[D];
   double c0 = -INFINITY, c1 = -INFINITY;
@@ -1861,10 +1911,11 @@ __global__ __launch_bounds__(256) void k_tvd_x(Geo g, Sel s, BC bc, const double
   __shared__ double sm[8];
   const double dt = sc->dt, smallp = sc->smallp;
   const int lane = threadIdx.x;
-  const int x0 = 2 + (int)blockIdx.x * TX_STRIDE;
+  const Tile tb = xcd_tile();
+  const int x0 = 2 + tb.x * TX_STRIDE;
   const int J = x0 - 2 + lane;
-  const int y = 2 + (int)(blockIdx.y * BY + threadIdx.y);
-  const int z = (g.nd == 3) ? 2 + (int)blockIdx.z : 0;
+  const int y = 2 + (int)(tb.y * BY + threadIdx.y);
+  const int z = (g.nd == 3) ? 2 + tb.z : 0;
   const bool rowok = y <= g.n[1] + 1;
   const int yc = rowok ? y : g.n[1] + 1;
   const int Jc = J < g.n[0] + 2 ? J : g.n[0] + 2;
@@ -1957,11 +2008,12 @@ __global__ __launch_bounds__(256) void k_tvd_m(Geo g, Sel s, BC bc, const double
   constexpr int NW = L::nw;
   __shared__ double sm[8];
   const double dt = sc->dt, smallp = sc->smallp;
-  const int x = 2 + (int)(blockIdx.x * BX + threadIdx.x);
-  const int seg = (int)(blockIdx.y * BY + threadIdx.y);
+  const Tile tb = xcd_tile();
+  const int x = 2 + (int)(tb.x * BX + threadIdx.x);
+  const int seg = (int)(tb.y * BY + threadIdx.y);
   const int s0 = 2 + seg * seglen;
   // the transverse coordinate that is neither x nor d (3D only)
-  const int t = (g.nd == 3) ? 2 + (int)blockIdx.z : 0;
+  const int t = (g.nd == 3) ? 2 + tb.z : 0;
   const int nd_ = g.n[d];
   const bool active = x <= g.n[0] + 1 && s0 <= nd_ + 1;
   double c0 = -INFINITY, c1 = -INFINITY;
@@ -2085,11 +2137,12 @@ __global__ __launch_bounds__(256) void k_ryu_ab_y(Geo g, BC bc, const double* __
   constexpr int NW = L::nw;
   const int lane = (int)threadIdx.x;
   const int n0 = g.n[0], n1 = g.n[1];
-  int c = 1 + (int)blockIdx.x * RY_STRIDE + lane;
+  const Tile tb = xcd_tile();
+  int c = 1 + tb.x * RY_STRIDE + lane;
   const bool useful = lane >= 1 && lane <= RY_STRIDE && c <= n0 + 1;
   c = c < n0 + 2 ? c : n0 + 2;  // idle lanes: in-bounds, unused
   const int px = image_of(g, bc, 0, c);
-  const int s0 = 2 + ((int)blockIdx.y * (int)blockDim.y + (int)threadIdx.y) * seglen;
+  const int s0 = 2 + (tb.y * (int)blockDim.y + (int)threadIdx.y) * seglen;
   if (s0 > n1 + 1) return;  // wave-uniform
   const int s1 = (s0 + seglen - 1 < n1 + 1) ? s0 + seglen - 1 : n1 + 1;
   const double dt = sc->dt;

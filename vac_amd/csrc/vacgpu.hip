@@ -470,6 +470,35 @@ int roundtrip_wold(vacgpu_grid* g) {
   return launch_check(g, "k_roundtrip");
 }
 
+// Hancock predictor launch (w -> w1, ghost cells of w1 with ghosts set):
+// primitive-variable limiting (FP64 only, checked with the method), else
+// specialised for a limiter common to every variable
+int launch_pred(vacgpu_grid* g, const vacgpu_method* m, int d, const Sel& s, const double* win,
+                int ghosts) {
+  Timed t(g, KF_PRED);
+  if (m->useprimitive) {
+    DISPATCH(g, {
+      if constexpr (std::is_same<R, double>::value) {
+        if (d == 0) hipLaunchKernelGGL((k_tvdlf_pred<ND, MHD, R, 0, true>), g->grid, g->block, 0, g->stream, g->geo, s, g->bc, win, g->w1, g->sc, ghosts);
+        else if (d == 1) hipLaunchKernelGGL((k_tvdlf_pred<ND, MHD, R, 1, true>), g->grid, g->block, 0, g->stream, g->geo, s, g->bc, win, g->w1, g->sc, ghosts);
+        else hipLaunchKernelGGL((k_tvdlf_pred<ND, MHD, R, (ND == 3 ? 2 : 1), true>), g->grid, g->block, 0, g->stream, g->geo, s, g->bc, win, g->w1, g->sc, ghosts);
+      }
+    });
+  } else {
+    const int lim = common_limiter(g, m);
+#define PL(DV, LV) hipLaunchKernelGGL((k_tvdlf_pred<ND, MHD, R, DV, false, LV>), g->grid, g->block, 0, g->stream, g->geo, s, g->bc, win, g->w1, g->sc, ghosts)
+#define PLD(DV) do { if (lim == LIM_MINMOD) PL(DV, LIM_MINMOD); else if (lim == LIM_WOODWARD) PL(DV, LIM_WOODWARD); else if (lim == LIM_SUPERBEE) PL(DV, LIM_SUPERBEE); else PL(DV, -1); } while (0)
+    DISPATCH(g, {
+      if (d == 0) PLD(0);
+      else if (d == 1) PLD(1);
+      else PLD((ND == 3 ? 2 : 1));
+    });
+#undef PLD
+#undef PL
+  }
+  return launch_check(g, "k_tvdlf_pred");
+}
+
 // TVD-MUSCL corrector of sweep direction d (face-parallel: x one face per
 // lane, y / z marching): win -> wout with the predictor state g->w1 and the
 // limiter's base wl; nbc receives the block count of its Courant partials
@@ -610,25 +639,7 @@ int sweep_device(vacgpu_grid* g, const vacgpu_method* m, int d, int firstsweep, 
     // TVD-MUSCL: the Hancock predictor of TVDLF, then the face-parallel
     // corrector (central flux, Roe dissipation, Powell source for MHD)
     const int ghosts = (g->bc_uniform && g->use_fused) ? 1 : 0;
-    {
-      Timed t(g, KF_PRED);
-      if (m->useprimitive) {
-        // primitive-variable limiting (FP64 only, checked with the method)
-        DISPATCH(g, {
-          if constexpr (std::is_same<R, double>::value) {
-            if (d == 0) hipLaunchKernelGGL((k_tvdlf_pred<ND, MHD, R, 0, true>), g->grid, g->block, 0, g->stream, g->geo, s, g->bc, win, g->w1, g->sc, ghosts);
-            else if (d == 1) hipLaunchKernelGGL((k_tvdlf_pred<ND, MHD, R, 1, true>), g->grid, g->block, 0, g->stream, g->geo, s, g->bc, win, g->w1, g->sc, ghosts);
-            else hipLaunchKernelGGL((k_tvdlf_pred<ND, MHD, R, (ND == 3 ? 2 : 1), true>), g->grid, g->block, 0, g->stream, g->geo, s, g->bc, win, g->w1, g->sc, ghosts);
-          }
-        });
-      } else
-      DISPATCH(g, {
-        if (d == 0) hipLaunchKernelGGL((k_tvdlf_pred<ND, MHD, R, 0>), g->grid, g->block, 0, g->stream, g->geo, s, g->bc, win, g->w1, g->sc, ghosts);
-        else if (d == 1) hipLaunchKernelGGL((k_tvdlf_pred<ND, MHD, R, 1>), g->grid, g->block, 0, g->stream, g->geo, s, g->bc, win, g->w1, g->sc, ghosts);
-        else hipLaunchKernelGGL((k_tvdlf_pred<ND, MHD, R, (ND == 3 ? 2 : 1)>), g->grid, g->block, 0, g->stream, g->geo, s, g->bc, win, g->w1, g->sc, ghosts);
-      });
-      if ((rc = launch_check(g, "k_tvdlf_pred"))) return rc;
-    }
+    if ((rc = launch_pred(g, m, d, s, win, ghosts))) return rc;
     if (!ghosts && (rc = bound(g, g->w1))) return rc;
     long long nbc = 0;
     if ((rc = launch_tvdmu_corr(g, m, d, s, win, wout, wl, ghosts, nbc))) return rc;
@@ -647,25 +658,7 @@ int sweep_device(vacgpu_grid* g, const vacgpu_method* m, int d, int firstsweep, 
     long long corr_blocks = nb;
     // with one boundary type per face the kernels write their outputs' ghost cells
     const int ghosts = (g->bc_uniform && g->use_fused) ? 1 : 0;
-    {
-      Timed t(g, KF_PRED);
-      if (m->useprimitive) {
-        // primitive-variable limiting (FP64 only, checked with the method)
-        DISPATCH(g, {
-          if constexpr (std::is_same<R, double>::value) {
-            if (d == 0) hipLaunchKernelGGL((k_tvdlf_pred<ND, MHD, R, 0, true>), g->grid, g->block, 0, g->stream, g->geo, s, g->bc, win, g->w1, g->sc, ghosts);
-            else if (d == 1) hipLaunchKernelGGL((k_tvdlf_pred<ND, MHD, R, 1, true>), g->grid, g->block, 0, g->stream, g->geo, s, g->bc, win, g->w1, g->sc, ghosts);
-            else hipLaunchKernelGGL((k_tvdlf_pred<ND, MHD, R, (ND == 3 ? 2 : 1), true>), g->grid, g->block, 0, g->stream, g->geo, s, g->bc, win, g->w1, g->sc, ghosts);
-          }
-        });
-      } else
-      DISPATCH(g, {
-        if (d == 0) hipLaunchKernelGGL((k_tvdlf_pred<ND, MHD, R, 0>), g->grid, g->block, 0, g->stream, g->geo, s, g->bc, win, g->w1, g->sc, ghosts);
-        else if (d == 1) hipLaunchKernelGGL((k_tvdlf_pred<ND, MHD, R, 1>), g->grid, g->block, 0, g->stream, g->geo, s, g->bc, win, g->w1, g->sc, ghosts);
-        else hipLaunchKernelGGL((k_tvdlf_pred<ND, MHD, R, (ND == 3 ? 2 : 1)>), g->grid, g->block, 0, g->stream, g->geo, s, g->bc, win, g->w1, g->sc, ghosts);
-      });
-      if ((rc = launch_check(g, "k_tvdlf_pred"))) return rc;
-    }
+    if ((rc = launch_pred(g, m, d, s, win, ghosts))) return rc;
     if (!ghosts && (rc = bound(g, g->w1))) return rc;
     {
       // face-parallel corrector: x sweep one face per lane, y / z sweeps marching
