@@ -1752,8 +1752,11 @@ __global__ __launch_bounds__(256) void k_tvdmu_x(Geo g, Sel s, BC bc, const doub
 
 // y / z sweep: thread (x, segment) marches through cells s0..s1 along D,
 // carrying the face below (flux, Roe decomposition, phi) and a window of w and w1.
+// (the plain variant is held to 256 VGPRs, two waves per SIMD, at the cost
+// of a few spills; the entropy-fix and primitive variants would spill far
+// more and keep one wave)
 template <int ND, class R, int D, int LIM, bool MHD = true, bool ENT = false, bool PRIM = false>
-__global__ __launch_bounds__(256) void k_tvdmu_m(Geo g, Sel s, BC bc, const double* __restrict__ w,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu((ENT || PRIM) ? 1 : 2))) void k_tvdmu_m(Geo g, Sel s, BC bc, const double* __restrict__ w,
                                                 const double* __restrict__ w1,
                                                 double* __restrict__ wout,
                                                 const Scal* __restrict__ sc,
