@@ -358,3 +358,63 @@ def test_pipelined_slot_reuse_matches_alone():
         assert t == st["t"]
         assert np.array_equal(wa, wb), f"reused-slot instance {nx}x{ny} {ph} {prec} differs"
     b.close()
+
+
+TVDMU_MIX = [
+    # (nx, ny, physics, gamma, precision, seed, limiter)
+    (64, 48, "mhd", 5 / 3, "fp64", 21, "woodward"),
+    (40, 72, "hd", 1.4, "fp64", 22, "minmod"),
+    (96, 80, "mhd", 1.4, "fp64", 23, "superbee"),
+    (130, 66, "mhd", 5 / 3, "fp32bulk", 24, "woodward"),
+    (72, 40, "hd", 1.4, "fp32bulk", 25, "minmod"),
+]
+
+
+def test_tvdmu_instances_match_alone():
+    # TVD-MUSCL instances (predictor tiles, corrector strips and columns,
+    # their two-wave Courant fold) beside TVDLF instances in one batch: each
+    # bitwise the same grid alone, and against the oracle
+    b = vacgpu.Batch(4 << 30, 32)
+    b.graph(True)
+    entries = {}
+    for (nx, ny, ph, g, prec, seed, lim) in TVDMU_MIX:
+        w, dxs = state(nx, ny, ph, g, seed)
+        m = vacgpu.MethodSpec(typefull="tvdmu", limiter=lim, courantpar=0.8)
+        s = b.add(w, [nx, ny], ph, g, dxs, m, precision=prec, itmax=9 + seed % 3)
+        entries[s] = (w, dxs, nx, ny, ph, g, prec, m, 9 + seed % 3, lim)
+    for (nx, ny, ph, g, prec, seed) in MIX[:2]:
+        w, dxs = state(nx, ny, ph, g, seed)
+        s = b.add(w, [nx, ny], ph, g, dxs, METHOD, precision=prec, itmax=10)
+        entries[s] = (w, dxs, nx, ny, ph, g, prec, METHOD, 10, "minmod")
+    print("case TVD-MUSCL and TVDLF instances in one batch", flush=True)
+    for _ in range(13):
+        b.step(1)
+    b.sync()
+    for s, (w, dxs, nx, ny, ph, g, prec, m, itmax, lim) in entries.items():
+        st = b.status(s)
+        wb = b.get_mesh(s)
+        assert st["done"] and st["it"] == itmax, (s, st)
+        ga = vacgpu.Grid([nx, ny], ph, g, dxs, precision=prec)
+        ga.put_mesh(w)
+        ga.run(m, itmax, record_t=False)
+        wa = ga.get_mesh()
+        it, t = ga.get_time()
+        ga.close()
+        what = f"{m.typefull} {lim} {nx}x{ny} {ph} {prec}"
+        assert it == itmax and t == st["t"], what
+        assert np.array_equal(wa, wb), f"{what} differs from the grid alone"
+        assert_matches_oracle(wb, st["it"], st["t"], w, dxs, nx, ny, ph, g, prec, what,
+                              typefull=m.typefull, limiter=lim)
+    b.close()
+
+
+def test_tvdmu_admission_rules():
+    b = vacgpu.Batch(1 << 30, 4)
+    nx, ny, ph, g, prec, seed = MIX[0]
+    w, dxs = state(nx, ny, ph, g, seed)
+    with pytest.raises(vacgpu.VacGpuError, match="entropy"):
+        b.add(w, [nx, ny], ph, g, dxs, vacgpu.MethodSpec(typefull="tvdmu", limiter="woodward",
+                                                        typeentropy="harten"))
+    with pytest.raises(vacgpu.VacGpuError, match="'tvdlf' and 'tvdmu'"):
+        b.add(w, [nx, ny], ph, g, dxs, vacgpu.MethodSpec(typefull="tvd", limiter="woodward"))
+    b.close()

@@ -3,9 +3,9 @@
 // Every instance keeps its own state in HBM (state w, sweep output b,
 // predictor w1), its own device scalars (dt, t, it, dtcourant, smallp,
 // nerror, stop condition) and its own selectors.  One batched step runs the
-// TVDLF step (one x and one y sweep) of every active instance: each launch
-// walks a work list of (instance, tile) entries, so grids of any size share
-// a launch.  Instances are grouped by kernel instance (physics, arithmetic
+// TVDLF or TVD-MUSCL step (one x and one y sweep, src/vactvdlf.t:5-299) of
+// every active instance: each launch walks a work list of (instance, tile)
+// entries, so grids of any size share a launch.  Instances are grouped by kernel instance (physics, arithmetic
 // tier, limiter) and by tile shape, which the autotuner picks per size
 // class.  The sweep order of an instance follows its own step parity
 // (src/vac.t:310-324): its tiles sit in the x-first or the y-first list.
@@ -66,6 +66,7 @@ struct InstDev {
   double* w1;  // predictor
   long long nbound, slab0, slab1;
   int pbase2, ntiles2;  // sweep blocks (fused) or corrector tiles and their partials
+  int ntx, ntm;         // TVD-MUSCL: corrector tiles of the x and the y sweep (partials from pbase2)
   int mhd;
   int seglen;           // fused sweeps: rows per marching segment
 };
@@ -99,20 +100,16 @@ __device__ void batch_getdt_finish(BScal& S, const InstDev& I, const double* dtc
   }
 }
 
-// after the corrector of sweep k: the dtcourant update (src/vactvdlf.t:184-201)
-// from the instance's Courant maximum cm; after the last sweep also the end
-// of the step (src/vac.t:97-98) and the next dt; thread 0 only
-__device__ void batch_sweep_end(BScal& S, const InstDev& I, int k, double cm) {
+// direction of sweep k in the instance's current step (src/vac.t:310-324)
+__device__ __forceinline__ int sweep_dir(const BScal& S, const InstDev& I, int k) {
+  const bool fwd = (S.s.it / 2) * 2 == S.s.it;
+  return fwd ? k : I.geo.nd - 1 - k;
+}
+
+// after the last sweep of a step: the end of the step (src/vac.t:97-98) and
+// the next dt; thread 0 only
+__device__ void batch_step_tail(BScal& S, const InstDev& I, int k) {
   const int nd = I.geo.nd;
-  if (I.sel.courantpar > 0.0) {
-    const bool fwd = (S.s.it / 2) * 2 == S.s.it;
-    const int d = fwd ? k : nd - 1 - k;
-    if (S.s.dt * cm > 1.0) S.s.nerror[2] += 1;
-    if (cm > kSmall) {
-      double c = I.sel.courantpar / cm;
-      S.s.dtcourant[d] = c < S.s.dtcourant[d] ? c : S.s.dtcourant[d];
-    }
-  }
   if (k != nd - 1) return;
   S.s.it += 1;
   S.s.t = S.s.t + S.s.dt;
@@ -124,6 +121,39 @@ __device__ void batch_sweep_end(BScal& S, const InstDev& I, int k, double cm) {
   }
   if (lagged) batch_getdt_finish(S, I, dtc, cmd);
   else S.needdt = 1;
+}
+
+// after the TVDLF corrector of sweep k: the dtcourant update
+// (src/vactvdlf.t:184-201) from the instance's Courant maximum cm, then
+// batch_step_tail; thread 0 only
+__device__ void batch_sweep_end(BScal& S, const InstDev& I, int k, double cm) {
+  if (I.sel.courantpar > 0.0) {
+    const int d = sweep_dir(S, I, k);
+    if (S.s.dt * cm > 1.0) S.s.nerror[2] += 1;
+    if (cm > kSmall) {
+      double c = I.sel.courantpar / cm;
+      S.s.dtcourant[d] = c < S.s.dtcourant[d] ? c : S.s.dtcourant[d];
+    }
+  }
+  batch_step_tail(S, I, k);
+}
+
+// after the TVD-MUSCL corrector of sweep k: the dtcourant update of
+// tvdlimit2 for the two extreme waves' Courant numbers (src/vactvd.t:145-157),
+// then batch_step_tail; thread 0 only
+__device__ void batch_sweep_end_tvd(BScal& S, const InstDev& I, int k, double c0, double c1) {
+  if (I.sel.courantpar > 0.0) {
+    const int d = sweep_dir(S, I, k);
+    const double cm[2] = {c0, c1};
+    for (int q = 0; q < 2; ++q) {
+      if (cm[q] > 1.0) S.s.nerror[2] += 1;
+      if (cm[q] > kSmall) {
+        const double c = S.s.dt * I.sel.courantpar / cm[q];
+        S.s.dtcourant[d] = c < S.s.dtcourant[d] ? c : S.s.dtcourant[d];
+      }
+    }
+  }
+  batch_step_tail(S, I, k);
 }
 
 // block-uniform early exit for finished instances
@@ -233,6 +263,63 @@ __global__ void kb_fin_courant(const InstDev* __restrict__ inst, BScal* __restri
   if (threadIdx.x == 0) batch_sweep_end(S, I, k, cm);
 }
 
+// TVD-MUSCL corrector tiles of 2D instances (the single-grid bodies): x
+// sweep strips of 62 faces by 4 rows, y sweep 64 columns by 4 segments of
+// seglen rows; sweep k reads w (k = 0) or b with the predictor w1 and
+// writes b or w (mesh cells; the ghost cells follow in a pass), and the two
+// extreme waves' Courant numbers go to the partials
+template <bool MHD, class R, int LIM>
+__global__ __launch_bounds__(256) void kb_tvdmu_x(const InstDev* __restrict__ inst,
+                                                 const BScal* __restrict__ bs,
+                                                 const Work* __restrict__ work, int k,
+                                                 double* __restrict__ part, long long pstride) {
+  __shared__ double sm[8];
+  BATCH_PROLOGUE
+  const double* src = (k == 0) ? I.w : I.b;
+  double* dst = (k == 0) ? I.b : I.w;
+  double c0, c1;
+  tvdmu_x_body<2, R, LIM, MHD>(I.geo, I.sel, I.bc, src, I.w1, dst, 0, src, S.s.dt, S.s.smallp,
+                               Tile{wk.bx, wk.by, 0}, c0, c1);
+  const double b0 = block_reduce<true>(c0, sm);
+  if (threadIdx.x == 0 && threadIdx.y == 0) part[SLOT_C0 * pstride + wk.pidx] = b0;
+  const double b1 = block_reduce<true>(c1, sm);
+  if (threadIdx.x == 0 && threadIdx.y == 0) part[SLOT_C1 * pstride + wk.pidx] = b1;
+}
+
+template <bool MHD, class R, int LIM>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void kb_tvdmu_m(
+    const InstDev* __restrict__ inst, const BScal* __restrict__ bs, const Work* __restrict__ work,
+    int k, double* __restrict__ part, long long pstride) {
+  __shared__ double sm[8];
+  BATCH_PROLOGUE
+  const double* src = (k == 0) ? I.w : I.b;
+  double* dst = (k == 0) ? I.b : I.w;
+  double c0, c1;
+  tvdmu_m_body<2, R, 1, LIM, MHD>(I.geo, I.sel, I.bc, src, I.w1, dst, I.seglen, 0, src, S.s.dt,
+                                  S.s.smallp, Tile{wk.bx, wk.by, 0}, c0, c1);
+  const double b0 = block_reduce<true>(c0, sm);
+  if (threadIdx.x == 0 && threadIdx.y == 0) part[SLOT_C0 * pstride + wk.pidx] = b0;
+  const double b1 = block_reduce<true>(c1, sm);
+  if (threadIdx.x == 0 && threadIdx.y == 0) part[SLOT_C1 * pstride + wk.pidx] = b1;
+}
+
+// TVD-MUSCL instances: one block per listed instance folds the two partial
+// slots of sweep k and runs batch_sweep_end_tvd
+__global__ void kb_fin_courant_tvd(const InstDev* __restrict__ inst, BScal* __restrict__ bs,
+                                   const int* __restrict__ ids, const double* __restrict__ part,
+                                   long long pstride, int k) {
+  __shared__ double sm[8];
+  const int i = ids[blockIdx.x];
+  BScal& S = bs[i];
+  if (S.done) return;
+  const InstDev& I = inst[i];
+  const int n = sweep_dir(S, I, k) == 0 ? I.ntx : I.ntm;
+  const double c0 = fold_range(part + SLOT_C0 * pstride + I.pbase2, n, true, sm);
+  __syncthreads();
+  const double c1 = fold_range(part + SLOT_C1 * pstride + I.pbase2, n, true, sm);
+  if (threadIdx.x == 0) batch_sweep_end_tvd(S, I, k, c0, c1);
+}
+
 // The next dt of instances whose sweeps left a dtcourant unset (a Courant
 // maximum at or below smalldouble): the cell-centre maxima over the mesh
 // (src/vac.t:691-700) in one block per instance, then batch_getdt_finish.
@@ -294,14 +381,20 @@ enum { NCLASS = 4, NSNAP = 3 };
 const int kTileRows[] = {1, 2, 4, 8};
 const int kSegLens[] = {8, 16, 32, 64};
 
-// kernel instance of a work list: physics, tier, and for the fused sweeps
-// the limiter type (lim < 0: predictor / corrector tiles)
+// kinds of work list: fused TVDLF sweep blocks; TVDLF predictor + corrector
+// tiles; TVD-MUSCL predictor tiles, x-corrector strips, y-corrector columns
+enum { KIND_FUSED = 0, KIND_TILE = 1, KIND_MUPRED = 2, KIND_MUX = 3, KIND_MUM = 4 };
+constexpr int MU_SEGLEN = 16;  // rows per marching segment of the TVD-MUSCL y corrector
+
+// kernel instance of a work list: physics, tier, kind and (fused sweeps and
+// TVD-MUSCL correctors) the limiter type
 struct Key {
-  int mhd, f32, lim;
+  int mhd, f32, lim, kind;
   bool operator<(const Key& o) const {
     if (mhd != o.mhd) return mhd < o.mhd;
     if (f32 != o.f32) return f32 < o.f32;
-    return lim < o.lim;
+    if (lim != o.lim) return lim < o.lim;
+    return kind < o.kind;
   }
 };
 
@@ -319,7 +412,8 @@ struct HostInst {
   double t0 = 0.0;
   dim3 tiles;           // predictor / corrector tile grid of an unfused instance
   bool fused = false;   // sweeps by kb_fused (else predictor + corrector tiles)
-  int lim = -1;         // the common limiter type of a fused instance
+  int lim = -1;         // the common limiter type of a fused or TVD-MUSCL instance
+  bool tvdmu = false;   // TVD-MUSCL (else TVDLF)
 };
 
 struct Launch {          // one work list on the device
@@ -477,8 +571,9 @@ struct vacgpu_batch {
   std::map<std::pair<Key, int>, Lists> lists;
   struct { GWork* d = nullptr; int n = 0, cap = 0; } gw4, gw6;  // ghost work per nw
   int* d_ids = nullptr;    // every joined instance
-  int* d_idsu = nullptr;   // joined instances swept by predictor + corrector tiles
-  int nids = 0, nidsu = 0, ids_cap = 0;
+  int* d_idsu = nullptr;   // joined TVDLF instances swept by predictor + corrector tiles
+  int* d_idst = nullptr;   // joined TVD-MUSCL instances
+  int nids = 0, nidsu = 0, nidst = 0, ids_cap = 0;
   std::multimap<size_t, double*> pool;  // reusable device blocks by size
   hipStream_t stream = nullptr;         // steps
   hipStream_t cstream = nullptr;        // admissions and result copies
@@ -670,7 +765,7 @@ int rebuild(vacgpu_batch* b) {
   }
   std::map<std::pair<Key, int>, std::vector<Work>> ph0, ph1;
   std::vector<GWork> g4, g6;
-  std::vector<int> ids, idsu;
+  std::vector<int> ids, idsu, idst;
   int pbase2 = 0;
   int maxid = -1;
   for (int i = 0; i < (int)b->inst.size(); ++i) {
@@ -683,10 +778,39 @@ int rebuild(vacgpu_batch* b) {
     }
     const Geo& g = h.dev.geo;
     const bool fused = h.fused && b->use_fused;
-    const Key kr{h.cfg.physics == VACGPU_MHD, h.cfg.precision == VACGPU_FP32BULK, -1};
+    const Key kr{h.cfg.physics == VACGPU_MHD, h.cfg.precision == VACGPU_FP32BULK, -1, KIND_TILE};
     auto& phl = h.phase_class ? ph1 : ph0;
     h.dev.pbase2 = pbase2;
-    if (fused) {
+    if (h.tvdmu) {
+      // predictor tiles (64 x rows), x-corrector strips (62 faces x 4 rows),
+      // y-corrector columns (64 x 4 segments of MU_SEGLEN rows)
+      const int rows = b->rows[h.klass];
+      const Key kp{kr.mhd, kr.f32, -1, KIND_MUPRED};
+      auto& vp = phl[std::make_pair(kp, rows)];
+      const int px = (g.n[0] + BX - 1) / BX, py = (g.n[1] + rows - 1) / rows;
+      for (int by = 0; by < py; ++by)
+        for (int bx = 0; bx < px; ++bx) vp.push_back(Work{i, bx, by, 0, 0});
+      const Key kx{kr.mhd, kr.f32, h.lim, KIND_MUX};
+      auto& vx = phl[std::make_pair(kx, BY)];
+      const int xx = (g.n[0] + CX_STRIDE - 1) / CX_STRIDE, xy = (g.n[1] + BY - 1) / BY;
+      int t = 0;
+      for (int by = 0; by < xy; ++by)
+        for (int bx = 0; bx < xx; ++bx, ++t) vx.push_back(Work{i, bx, by, 0, pbase2 + t});
+      h.dev.ntx = t;
+      const Key km{kr.mhd, kr.f32, h.lim, KIND_MUM};
+      auto& vm = phl[std::make_pair(km, MU_SEGLEN)];
+      const int mx = (g.n[0] + BX - 1) / BX;
+      const int my = ((g.n[1] + MU_SEGLEN - 1) / MU_SEGLEN + BY - 1) / BY;
+      t = 0;
+      for (int by = 0; by < my; ++by)
+        for (int bx = 0; bx < mx; ++bx, ++t) vm.push_back(Work{i, bx, by, 0, pbase2 + t});
+      h.dev.ntm = t;
+      h.dev.seglen = MU_SEGLEN;
+      h.dev.ntiles2 = h.dev.ntx > h.dev.ntm ? h.dev.ntx : h.dev.ntm;
+      for (long long q = 0; q < h.dev.nbound; q += 256) (kr.mhd ? g6 : g4).push_back(GWork{i, (int)q});
+      idst.push_back(i);
+      pbase2 += h.dev.ntiles2;
+    } else if (fused) {
       // sweep blocks: strips of 62 x cells, blocks of four segments
       const int sl = b->seglen[h.klass];
       const int strips = (g.n[0] + FS_STRIDE - 1) / FS_STRIDE;
@@ -694,7 +818,7 @@ int rebuild(vacgpu_batch* b) {
       const int sblocks = (segs + 3) / 4;
       h.dev.seglen = sl;
       h.dev.ntiles2 = strips * sblocks;
-      const Key kf{kr.mhd, kr.f32, h.lim};
+      const Key kf{kr.mhd, kr.f32, h.lim, KIND_FUSED};
       auto& v = phl[std::make_pair(kf, sl)];
       for (int sb = 0; sb < sblocks; ++sb)
         for (int st = 0; st < strips; ++st) v.push_back(Work{i, st, sb, 0, 0});
@@ -722,7 +846,7 @@ int rebuild(vacgpu_batch* b) {
     if (b->part) old.frees.push_back(b->part);
     b->part = nullptr;
     b->pstride = (long long)pbase2 * 2 + 64;
-    BHIP(b, hipMalloc(&b->part, sizeof(double) * b->pstride));
+    BHIP(b, hipMalloc(&b->part, sizeof(double) * 2 * b->pstride));  // slots C0, C1
   }
   std::set<std::pair<Key, int>> keys;
   for (auto& kv : ph0) keys.insert(kv.first);
@@ -754,14 +878,17 @@ int rebuild(vacgpu_batch* b) {
   if ((int)ids.size() > b->ids_cap) {
     if (b->d_ids) old.frees.push_back(b->d_ids);
     if (b->d_idsu) old.frees.push_back(b->d_idsu);
-    b->d_ids = b->d_idsu = nullptr;
+    if (b->d_idst) old.frees.push_back(b->d_idst);
+    b->d_ids = b->d_idsu = b->d_idst = nullptr;
     b->ids_cap = (int)ids.size() * 2 + 16;
     BHIP(b, hipMalloc(&b->d_ids, sizeof(int) * b->ids_cap));
     BHIP(b, hipMalloc(&b->d_idsu, sizeof(int) * b->ids_cap));
+    BHIP(b, hipMalloc(&b->d_idst, sizeof(int) * b->ids_cap));
   }
   b->nids = (int)ids.size();
   b->nidsu = (int)idsu.size();
-  total += sizeof(int) * (ids.size() + idsu.size()) + 512;
+  b->nidst = (int)idst.size();
+  total += sizeof(int) * (ids.size() + idsu.size() + idst.size()) + 768;
   // uploads through the arena
   vacgpu_batch::Arena& A = b->arena[b->arena_cur];
   b->arena_cur ^= 1;
@@ -792,6 +919,7 @@ int rebuild(vacgpu_batch* b) {
   if ((rc = up(b->gw6.d, g6.data(), sizeof(GWork) * g6.size()))) return rc;
   if ((rc = up(b->d_ids, ids.data(), sizeof(int) * ids.size()))) return rc;
   if ((rc = up(b->d_idsu, idsu.data(), sizeof(int) * idsu.size()))) return rc;
+  if ((rc = up(b->d_idst, idst.data(), sizeof(int) * idst.size()))) return rc;
   BHIP(b, hipEventRecord(A.ev, b->stream));
   A.recorded = true;
   for (int p = 0; p < 2; ++p)
@@ -833,45 +961,56 @@ int enqueue_step(vacgpu_batch* b, int parity) {
   if ((rc = launch_err(b, "kb_getdt_rare"))) return rc;
   // sweeps: phase class c has even it when (c + parity) is even
   for (int k = 0; k < ND; ++k) {
-    for (int pass = 0; pass < 2; ++pass) {  // 0 predictor, 1 corrector
+    for (int pass = 0; pass < 2; ++pass) {  // 0 predictors, 1 correctors
       for (auto& kv : b->lists) {
         const auto& L = kv.second;
-        const dim3 blk(BX, kv.first.second);
-        const int lim = kv.first.first.lim;
+        const Key& key = kv.first.first;
+        const int shape = kv.first.second;
+        const int lim = key.lim;
         for (int c = 0; c < 2; ++c) {
           const Launch& W = L.ph[c];
           if (!W.n) continue;
           const bool fwd = ((c + parity) % 2) == 0;
           const int D = fwd ? k : ND - 1 - k;
-          if (lim >= 0) {  // fused sweep: one launch, in the corrector pass
+          const dim3 fb(64, 4);
+          if (key.kind == KIND_FUSED) {  // one launch, in the corrector pass
             if (pass == 1) {
-              const dim3 fb(64, 4);
 #define KBF(DV, LV) hipLaunchKernelGGL((kb_fused<MHD, R, DV, LV>), dim3(W.n), fb, 0, b->stream, b->d_inst, b->d_bs, W.d, k, b->d_fold, 1)
 #define KBFL(DV) do { if (lim == LIM_MINMOD) KBF(DV, LIM_MINMOD); else if (lim == LIM_WOODWARD) KBF(DV, LIM_WOODWARD); else KBF(DV, LIM_SUPERBEE); } while (0)
-              BDISPATCH(kv.first.first.mhd, kv.first.first.f32, {
+              BDISPATCH(key.mhd, key.f32, {
                 if (D == 0) KBFL(0); else KBFL(1);
               });
 #undef KBFL
 #undef KBF
             }
-            continue;
+          } else if (key.kind == KIND_TILE || key.kind == KIND_MUPRED) {
+            const dim3 blk(BX, shape);
+            BDISPATCH(key.mhd, key.f32, {
+              if (pass == 0) {
+                if (D == 0) hipLaunchKernelGGL((kb_pred<ND, MHD, R, 0>), dim3(W.n), blk, 0, b->stream, b->d_inst, b->d_bs, W.d, k);
+                else hipLaunchKernelGGL((kb_pred<ND, MHD, R, 1>), dim3(W.n), blk, 0, b->stream, b->d_inst, b->d_bs, W.d, k);
+              } else if (key.kind == KIND_TILE) {
+                if (D == 0) hipLaunchKernelGGL((kb_corr<ND, MHD, R, 0>), dim3(W.n), blk, 0, b->stream, b->d_inst, b->d_bs, W.d, k, b->part, b->pstride);
+                else hipLaunchKernelGGL((kb_corr<ND, MHD, R, 1>), dim3(W.n), blk, 0, b->stream, b->d_inst, b->d_bs, W.d, k, b->part, b->pstride);
+              }
+            });
+          } else if (pass == 1 && ((key.kind == KIND_MUX && D == 0) || (key.kind == KIND_MUM && D == 1))) {
+#define KMU(KN, LV) hipLaunchKernelGGL((KN<MHD, R, LV>), dim3(W.n), fb, 0, b->stream, b->d_inst, b->d_bs, W.d, k, b->part, b->pstride)
+#define KMUL(KN) do { if (lim == LIM_MINMOD) KMU(KN, LIM_MINMOD); else if (lim == LIM_WOODWARD) KMU(KN, LIM_WOODWARD); else KMU(KN, LIM_SUPERBEE); } while (0)
+            BDISPATCH(key.mhd, key.f32, {
+              if (key.kind == KIND_MUX) KMUL(kb_tvdmu_x); else KMUL(kb_tvdmu_m);
+            });
+#undef KMUL
+#undef KMU
           }
-          BDISPATCH(kv.first.first.mhd, kv.first.first.f32, {
-            if (pass == 0) {
-              if (D == 0) hipLaunchKernelGGL((kb_pred<ND, MHD, R, 0>), dim3(W.n), blk, 0, b->stream, b->d_inst, b->d_bs, W.d, k);
-              else hipLaunchKernelGGL((kb_pred<ND, MHD, R, 1>), dim3(W.n), blk, 0, b->stream, b->d_inst, b->d_bs, W.d, k);
-            } else {
-              if (D == 0) hipLaunchKernelGGL((kb_corr<ND, MHD, R, 0>), dim3(W.n), blk, 0, b->stream, b->d_inst, b->d_bs, W.d, k, b->part, b->pstride);
-              else hipLaunchKernelGGL((kb_corr<ND, MHD, R, 1>), dim3(W.n), blk, 0, b->stream, b->d_inst, b->d_bs, W.d, k, b->part, b->pstride);
-            }
-          });
         }
       }
-      if ((rc = launch_err(b, pass ? "kb_corr" : "kb_pred"))) return rc;
+      if ((rc = launch_err(b, pass ? "batch correctors" : "batch predictors"))) return rc;
       if (pass == 0) {
         if ((rc = bound_all(b, 2))) return rc;
       } else {
         if (b->nidsu) hipLaunchKernelGGL(kb_fin_courant, dim3(b->nidsu), dim3(256), 0, b->stream, b->d_inst, b->d_bs, b->d_idsu, b->part, b->pstride, k);
+        if (b->nidst) hipLaunchKernelGGL(kb_fin_courant_tvd, dim3(b->nidst), dim3(256), 0, b->stream, b->d_inst, b->d_bs, b->d_idst, b->part, b->pstride, k);
         if ((rc = launch_err(b, "kb_fin_courant"))) return rc;
         if ((rc = bound_all(b, k == 0 ? 1 : 0))) return rc;
       }
@@ -954,7 +1093,8 @@ int batch_add(vacgpu_batch* b, const vacgpu_config* cfg, const vacgpu_method* m,
               int it, double t, int itmax, double tmax, int tmaxexact, int* slot) {
   if (!b || !cfg || !m || !dx0 || !dx1 || !w || !slot) return VACGPU_E_ARG;
   if (cfg->ndim != 2) return bfail(b, VACGPU_E_ARG, "the batch engine runs 2D grids");
-  if (m->typefull != VACGPU_TVDLF) return bfail(b, VACGPU_E_ARG, "the batch engine runs typefull='tvdlf'");
+  if (m->typefull != VACGPU_TVDLF && m->typefull != VACGPU_TVDMU)
+    return bfail(b, VACGPU_E_ARG, "the batch engine runs typefull='tvdlf' and 'tvdmu'");
   if (m->dimsplit_xy) return bfail(b, VACGPU_E_ARG, "the batch engine runs typedimsplit='xyyx'");
   if (m->typelimited != VACGPU_LIM_ORIGINAL) return bfail(b, VACGPU_E_ARG, "the batch engine runs typelimited='original'");
   if (m->useprimitive) return bfail(b, VACGPU_E_ARG, "the batch engine limits the conserved variables (useprimitive=F)");
@@ -963,6 +1103,17 @@ int batch_add(vacgpu_batch* b, const vacgpu_config* cfg, const vacgpu_method* m,
     return bfail(b, VACGPU_E_ARG, "the batch engine's time step has no dtcantgrow / slowsteps");
   if (m->courantpar <= 0.0) return bfail(b, VACGPU_E_ARG, "the batch engine needs courantpar > 0");
   if (vacgpu_supported(cfg, m) != 0) return bfail(b, VACGPU_E_ARG, "unsupported selectors (see vacgpu_supported)");
+  const int nwv = cfg->physics == VACGPU_MHD ? 6 : 4;
+  if (m->typefull == VACGPU_TVDMU) {
+    for (int iw = 0; iw < nwv; ++iw) {
+      if (m->typeentropy[iw] != VACGPU_ENT_NUL)
+        return bfail(b, VACGPU_E_ARG, "the batch engine's TVD-MUSCL has no entropy fix");
+      if (m->typelimiter[iw] != m->typelimiter[0])
+        return bfail(b, VACGPU_E_ARG, "the batch engine's TVD-MUSCL takes one limiter for every wave");
+    }
+    if (m->typelimiter[0] != LIM_MINMOD && m->typelimiter[0] != LIM_WOODWARD && m->typelimiter[0] != LIM_SUPERBEE)
+      return bfail(b, VACGPU_E_ARG, "the batch engine's TVD-MUSCL takes minmod, woodward or superbee");
+  }
   for (int f = 0; f < 4; ++f)
     for (int iw = 0; iw < 8; ++iw)
       if (cfg->typeB[f][iw] != VACGPU_CONT && cfg->typeB[f][iw] != VACGPU_PERIODIC)
@@ -995,6 +1146,7 @@ int batch_add(vacgpu_batch* b, const vacgpu_config* cfg, const vacgpu_method* m,
   I.sel.divbfix = m->divbfix;
   I.sel.divbwave = m->divbwave;
   I.sel.sourceunsplit = m->sourceunsplit;
+  I.sel.srcmul = 1;  // split sweeps: each carries its own share of the source
   I.sel.courantpar = m->courantpar;
   I.sel.gamma = cfg->eqpar[0];
   // fused sweeps: one boundary type per face for all variables, one limiter,
@@ -1005,8 +1157,9 @@ int batch_add(vacgpu_batch* b, const vacgpu_config* cfg, const vacgpu_method* m,
       for (int iw = 1; iw < nw; ++iw) ok &= I.bc.t[f][iw] == I.bc.t[f][0];
     for (int iw = 1; iw < nw; ++iw) ok &= m->typelimiter[iw] == m->typelimiter[0];
     if (I.mhd) ok &= m->divbfix && m->sourceunsplit;
-    h.fused = ok;
-    h.lim = ok ? m->typelimiter[0] : -1;
+    h.tvdmu = m->typefull == VACGPU_TVDMU;
+    h.fused = ok && !h.tvdmu;
+    h.lim = (ok || h.tvdmu) ? m->typelimiter[0] : -1;
   }
   long long slab[3];
   I.nbound = ghost_slabs(I.geo, slab);
@@ -1341,7 +1494,7 @@ int vacgpu_batch_tune(vacgpu_batch* b, int* rows_out) {
   for (int c = 0; c < NCLASS; ++c) {
     // tiles of class-c instances per physics/tier key, for each candidate shape
     bool present = false;
-    for (const auto& h : b->inst) present |= (h.used && h.klass == c && !(h.fused && b->use_fused));
+    for (const auto& h : b->inst) present |= (h.used && h.klass == c && !h.tvdmu && !(h.fused && b->use_fused));
     if (!present) continue;
     float best = 1e30f;
     int best_rows = b->rows[c];
@@ -1349,8 +1502,8 @@ int vacgpu_batch_tune(vacgpu_batch* b, int* rows_out) {
       std::map<Key, std::vector<Work>> lists;
       for (int i = 0; i < (int)b->inst.size(); ++i) {
         const HostInst& h = b->inst[i];
-        if (!h.used || h.klass != c) continue;
-        Key k{h.cfg.physics == VACGPU_MHD, h.cfg.precision == VACGPU_FP32BULK};
+        if (!h.used || h.klass != c || h.tvdmu) continue;
+        Key k{h.cfg.physics == VACGPU_MHD, h.cfg.precision == VACGPU_FP32BULK, -1, KIND_TILE};
         const Geo& g = h.dev.geo;
         const int tx = (g.n[0] + BX - 1) / BX, ty = (g.n[1] + r - 1) / r;
         for (int by = 0; by < ty; ++by)
@@ -1412,7 +1565,7 @@ int vacgpu_batch_tune(vacgpu_batch* b, int* rows_out) {
         const Geo& g = h.dev.geo;
         const int strips = (g.n[0] + FS_STRIDE - 1) / FS_STRIDE;
         const int sblocks = ((g.n[1] + sl - 1) / sl + 3) / 4;
-        Key k{h.cfg.physics == VACGPU_MHD, h.cfg.precision == VACGPU_FP32BULK, h.lim};
+        Key k{h.cfg.physics == VACGPU_MHD, h.cfg.precision == VACGPU_FP32BULK, h.lim, KIND_FUSED};
         for (int sb = 0; sb < sblocks; ++sb)
           for (int st = 0; st < strips; ++st) lists[k].push_back(Work{i, st, sb, 0, 0});
       }
@@ -1524,6 +1677,7 @@ void vacgpu_batch_destroy(vacgpu_batch* b) {
   if (b->gw6.d) (void)hipFree(b->gw6.d);
   if (b->d_ids) (void)hipFree(b->d_ids);
   if (b->d_idsu) (void)hipFree(b->d_idsu);
+  if (b->d_idst) (void)hipFree(b->d_idst);
   if (b->part) (void)hipFree(b->part);
   for (int p = 0; p < 2; ++p)
     if (b->graph[p]) (void)hipGraphExecDestroy(b->graph[p]);

@@ -1651,18 +1651,15 @@ __device__ __forceinline__ void mu_face(const Sel& s, const double* u0, const do
 // x sweep: lane L owns face c+1/2 of cell c = x0 - 1 + L and updates c for
 // L in [1, 62]; the left face's flux and dissipation term come from lane L-1.
 template <int ND, class R, int LIM, bool MHD = true, bool ENT = false, bool PRIM = false>
-__global__ __launch_bounds__(256) void k_tvdmu_x(Geo g, Sel s, BC bc, const double* __restrict__ w,
-                                                const double* __restrict__ w1,
-                                                double* __restrict__ wout,
-                                                const Scal* __restrict__ sc,
-                                                double* __restrict__ part, long long pstride,
-                                                int ghosts, const double* __restrict__ wl) {
+__device__ __forceinline__ void tvdmu_x_body(const Geo& g, const Sel& s, const BC& bc,
+                                             const double* __restrict__ w,
+                                             const double* __restrict__ w1,
+                                             double* __restrict__ wout, int ghosts,
+                                             const double* __restrict__ wl, double dt,
+                                             double smallp, Tile tb, double& c0o, double& c1o) {
   using L = Layout<ND, MHD>;
   constexpr int NW = L::nw;
-  __shared__ double sm[8];
-  const double dt = sc->dt, smallp = sc->smallp;
   const int lane = (int)threadIdx.x;
-  const Tile tb = xcd_tile();
   const int c = 1 + tb.x * CX_STRIDE + lane;
   const int y = 2 + (int)(tb.y * BY + threadIdx.y);
   const int z = (ND == 3) ? 2 + tb.z : 0;
@@ -1744,6 +1741,20 @@ __global__ __launch_bounds__(256) void k_tvdmu_x(Geo g, Sel s, BC bc, const doub
     }
   }
   if (useful) store_cell<ND, NW>(g, bc, wout, idx, c3, out, ghosts);
+  c0o = c0;
+  c1o = c1;
+}
+
+template <int ND, class R, int LIM, bool MHD = true, bool ENT = false, bool PRIM = false>
+__global__ __launch_bounds__(256) void k_tvdmu_x(Geo g, Sel s, BC bc, const double* __restrict__ w,
+                                                const double* __restrict__ w1,
+                                                double* __restrict__ wout,
+                                                const Scal* __restrict__ sc,
+                                                double* __restrict__ part, long long pstride,
+                                                int ghosts, const double* __restrict__ wl) {
+  __shared__ double sm[8];
+  double c0, c1;
+  tvdmu_x_body<ND, R, LIM, MHD, ENT, PRIM>(g, s, bc, w, w1, wout, ghosts, wl, sc->dt, sc->smallp, xcd_tile(), c0, c1);
   const double b0 = block_reduce<true>(c0, sm);
   if (threadIdx.x == 0 && threadIdx.y == 0) part[SLOT_C0 * pstride + block_linear()] = b0;
   const double b1 = block_reduce<true>(c1, sm);
@@ -1752,22 +1763,15 @@ __global__ __launch_bounds__(256) void k_tvdmu_x(Geo g, Sel s, BC bc, const doub
 
 // y / z sweep: thread (x, segment) marches through cells s0..s1 along D,
 // carrying the face below (flux, Roe decomposition, phi) and a window of w and w1.
-// (the plain variant is held to 256 VGPRs, two waves per SIMD, at the cost
-// of a few spills; the entropy-fix and primitive variants would spill far
-// more and keep one wave)
 template <int ND, class R, int D, int LIM, bool MHD = true, bool ENT = false, bool PRIM = false>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu((ENT || PRIM) ? 1 : 2))) void k_tvdmu_m(Geo g, Sel s, BC bc, const double* __restrict__ w,
-                                                const double* __restrict__ w1,
-                                                double* __restrict__ wout,
-                                                const Scal* __restrict__ sc,
-                                                double* __restrict__ part, long long pstride,
-                                                int seglen, int ghosts,
-                                                const double* __restrict__ wl) {
+__device__ __forceinline__ void tvdmu_m_body(const Geo& g, const Sel& s, const BC& bc,
+                                             const double* __restrict__ w,
+                                             const double* __restrict__ w1,
+                                             double* __restrict__ wout, int seglen, int ghosts,
+                                             const double* __restrict__ wl, double dt,
+                                             double smallp, Tile tb, double& c0o, double& c1o) {
   using L = Layout<ND, MHD>;
   constexpr int NW = L::nw;
-  __shared__ double sm[8];
-  const double dt = sc->dt, smallp = sc->smallp;
-  const Tile tb = xcd_tile();
   const int x = 2 + (int)(tb.x * BX + threadIdx.x);
   const int seg = (int)(tb.y * BY + threadIdx.y);
   const int s0 = 2 + seg * seglen;
@@ -1891,6 +1895,24 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu((ENT || PRI
       nfp = nff;
     }
   }
+  c0o = c0;
+  c1o = c1;
+}
+
+// (the plain variant is held to 256 VGPRs, two waves per SIMD, at the cost
+// of a few spills; the entropy-fix and primitive variants would spill far
+// more and keep one wave)
+template <int ND, class R, int D, int LIM, bool MHD = true, bool ENT = false, bool PRIM = false>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu((ENT || PRIM) ? 1 : 2))) void k_tvdmu_m(Geo g, Sel s, BC bc, const double* __restrict__ w,
+                                                const double* __restrict__ w1,
+                                                double* __restrict__ wout,
+                                                const Scal* __restrict__ sc,
+                                                double* __restrict__ part, long long pstride,
+                                                int seglen, int ghosts,
+                                                const double* __restrict__ wl) {
+  __shared__ double sm[8];
+  double c0, c1;
+  tvdmu_m_body<ND, R, D, LIM, MHD, ENT, PRIM>(g, s, bc, w, w1, wout, seglen, ghosts, wl, sc->dt, sc->smallp, xcd_tile(), c0, c1);
   const double b0 = block_reduce<true>(c0, sm);
   if (threadIdx.x == 0 && threadIdx.y == 0) part[SLOT_C0 * pstride + block_linear()] = b0;
   const double b1 = block_reduce<true>(c1, sm);
