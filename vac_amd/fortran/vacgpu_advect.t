@@ -52,6 +52,7 @@ integer:: iw, code, idim
 real(c_double), target :: dtc(3), sp
 logical:: hostsync, fetch
 logical, save :: probed = .false.
+integer(8) :: clk, clkrate
 !-----------------------------------------------------------------------------
 
 if(ix^L/=ixG^L|.or.|.or.) call die('vacgpu advect: subgrids are not supported')
@@ -142,6 +143,12 @@ end select
 hostsync = residmin>zero .or. residmax<bigdouble .or. sourcesplit .or. typefilter1/='nul'
 
 if(firstsweep)then
+   ! with the trace on, the wall clock at the start of every step: the
+   ! difference of two such lines is one step of VAC's loop, advect included
+   if(vacgpu_trace)then
+      call system_clock(clk,clkrate)
+      write(*,'(a,i8,a,f18.6)')'vacgpu: step at it=',it,' wall_s=',dble(clk)/dble(clkrate)
+   endif
    if(.not.probed)then
       probed=.true.
       call vacgpu_probe_source(iws,w,w1)
