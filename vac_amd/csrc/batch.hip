@@ -157,8 +157,11 @@ __device__ void batch_sweep_end_tvd(BScal& S, const InstDev& I, int k, double c0
 }
 
 // block-uniform early exit for finished instances
-#define BATCH_PROLOGUE                        \
-  const Work wk = work[blockIdx.x];           \
+// A work list holds each instance's tiles together, neighbours adjacent;
+// block L takes item xcd_order(L), so every XCD works through a contiguous
+// stretch of the list and the halo rows its tiles share stay in its L2.
+#define BATCH_PROLOGUE                                     \
+  const Work wk = work[xcd_order(blockIdx.x, gridDim.x)];  \
   const InstDev& I = inst[wk.inst];           \
   const BScal& S = bs[wk.inst];               \
   if (S.done) return;
@@ -231,7 +234,7 @@ template <int NW>
 __global__ __launch_bounds__(256) void kb_bound(const InstDev* __restrict__ inst,
                                                const BScal* __restrict__ bs,
                                                const GWork* __restrict__ work, int which) {
-  const GWork gw = work[blockIdx.x];
+  const GWork gw = work[xcd_order(blockIdx.x, gridDim.x)];
   const InstDev& I = inst[gw.inst];
   if (bs[gw.inst].done) return;
   const long long t = (long long)gw.start + threadIdx.x;

@@ -117,13 +117,20 @@ __device__ __forceinline__ CellIx cell_at(const Geo& g, int bx, int by, int bz) 
 struct Tile {
   int x, y, z;
 };
+// Position of block L of an n-block launch in an XCD-contiguous order: XCD
+// j (the blocks L with L mod 8 = j, in turn) takes the j-th contiguous share
+// of the order (n/8 positions, one more for the first n mod 8 XCDs).  A
+// bijection of [0, n).
+__device__ __forceinline__ unsigned xcd_order(unsigned L, unsigned n) {
+  const unsigned j = L & 7u, p = L >> 3, q = n >> 3, r = n & 7u;
+  return j < r ? j * (q + 1) + p : r * (q + 1) + (j - r) * q + p;
+}
 __device__ __forceinline__ Tile xcd_tile() {
   constexpr unsigned YG = 8;  // tile rows per group
   const unsigned gx = gridDim.x, gy = gridDim.y, gz = gridDim.z;
   const unsigned n = gx * gy * gz;
   const unsigned L = blockIdx.x + gx * (blockIdx.y + gy * blockIdx.z);
-  const unsigned j = L & 7u, p = L >> 3, q = n >> 3, r = n & 7u;
-  unsigned t = j < r ? j * (q + 1) + p : r * (q + 1) + (j - r) * q + p;
+  unsigned t = xcd_order(L, n);
   const unsigned full = gy / YG, per = gx * YG * gz;
   unsigned g = t / per, rows = YG;
   if (g >= full) {  // the last, partial group
