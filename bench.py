@@ -129,7 +129,8 @@ def cpu_baseline(w0, coords):
     for config 1 (Orszag-Tang 128^2, TVDLF + minmod, 100 steps).  VAC is
     serial (no threads), so N processes is its multi-core form.  Advance time
     only ("Finish Advance" - "Start Advance", src/vac.t:52,115).  Falls back to
-    the C restatement on one core if the executables are missing."""
+    the C restatement on one core if the executables are missing.  Config 3
+    (3D TVDLF) is timed the same way on a 128^3 sample of its state."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     from run_ref import have_ref, make_ini, problem_par
     from vac_amd.initial import orszag_tang
@@ -149,6 +150,22 @@ def cpu_baseline(w0, coords):
                            courantpar=0.8)
         one1 = _ref_runs("vacO3_mhd22_132", ini1, par1, cores[:1])[0]
         many1 = _ref_runs("vacO3_mhd22_132", ini1, par1, cores)
+        cfg3 = None
+        if have_ref("vacO3_mhd33_132"):
+            # config 3 (3D MHD TVDLF + minmod, courantpar 0.6) on a 128^3 sample
+            # of its random-Fourier state: 2 steps per process
+            from vac_amd.initial import random_fourier
+
+            w3, c3 = random_fourier((128, 128, 128), SEED + 3, GAMMA)
+            ini3 = make_ini(w3, c3, "mhd", GAMMA, name="bench")
+            par3 = problem_par(8, "rho m1 m2 m3 e b1 b2 b3", 2, typefull="tvdlf",
+                               limiter="minmod", courantpar=0.6, typeB=["periodic"] * 6)
+            one3 = _ref_runs("vacO3_mhd33_132", ini3, par3, cores[:1])[0]
+            many3 = _ref_runs("vacO3_mhd33_132", ini3, par3, cores)
+            cfg3 = {"single_core": round(128 ** 3 * 2 / one3 / 1e6, 3),
+                    "aggregate": round(n * 128 ** 3 * 2 / max(many3) / 1e6, 3),
+                    "cores": n, "unit": "Mcell-updates/s",
+                    "sample": "128^3 of config 3's random-Fourier state, 2 steps per process"}
         return {"value": round(n * cells / max(many) / 1e6, 4), "unit": "Mcell-updates/s",
                 "cores": n, "kind": "reference",
                 "sample": (f"VAC 4.52 (flang -O3 -march=x86-64-v3), {n} independent processes "
@@ -160,7 +177,8 @@ def cpu_baseline(w0, coords):
                 "config1_ot128_tvdlf_100": {
                     "single_core": round(128 * 128 * 100 / one1 / 1e6, 3),
                     "aggregate": round(n * 128 * 128 * 100 / max(many1) / 1e6, 3),
-                    "cores": n, "unit": "Mcell-updates/s"}}
+                    "cores": n, "unit": "Mcell-updates/s"},
+                "config3_mhd3d_tvdlf": cfg3}
     import vacref
     from vac_amd.grid import widths_1d
 

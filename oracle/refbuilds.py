@@ -46,4 +46,6 @@ TIMING_FLAGS = "-O3 -march=x86-64-v3"
 TIMING = {
     "vacO3_mhd22_132": ("22", "mhd", "132,132"),
     "vacO3_mhd22_2052": ("22", "mhd", "2052,2052"),
+    # config 3's 3D TVDLF, timed on a 128^3 sample of it
+    "vacO3_mhd33_132": ("33", "mhd", "132,132,132"),
 }
