@@ -16,8 +16,9 @@
 # With the fifth argument "gpu" the same VAC host is linked with the GPU
 # drop-in instead of its own advect (vac_amd/fortran/vacgpu_advect.t, which
 # calls libvacgpu through vac_amd/fortran/vacgpu_mod.f90): the CPU advect of
-# vac.t is renamed advect_cpu and left unused, and VAC's savefile is renamed
-# savefile_cpu, which the drop-in's savefile calls after fetching the state.  This integration executable
+# vac.t is renamed advect_cpu and left unused, VAC's savefile is renamed
+# savefile_cpu, which the drop-in's savefile calls after fetching the state,
+# and the main loop's wold=w goes through the drop-in's vacgpu_store_wold.  This integration executable
 # is what the drop-in test runs against the plain reference build.
 #
 # The recipe is this script, not the reference's setvac/Makefile:
@@ -88,6 +89,10 @@ if [ "$MODE" = gpu ]; then
   # the drop-in's savefile fetches the device state, then calls VAC's
   perl -pi -e 's/^subroutine savefile\(ifile,w\)/subroutine savefile_cpu(ifile,w)/' vacio.t
   grep -q '^subroutine savefile_cpu(ifile,w)' vacio.t || { echo "savefile not found in vacio.t" >&2; exit 1; }
+  # the main loop's copy of w into wold (src/vac.t:85) goes through the
+  # drop-in, which skips it while the host's w lags the device state
+  perl -pi -e 's/^(\s*)wold\(ixG\^S,1:nw\)=w\(ixG\^S,1:nw\)\s*$/$1call vacgpu_store_wold(w)\n/' vac.t
+  grep -q 'call vacgpu_store_wold(w)' vac.t || { echo "wold copy not found in vac.t" >&2; exit 1; }
   $FC $FFLAGS -c vacgpu_mod.f90 -o vacgpu_mod.o
   MODS="$MODS vacgpu_advect"
   if [ "$POISSON" = poisson ]; then

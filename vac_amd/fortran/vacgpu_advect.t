@@ -8,6 +8,9 @@
 ! typelimiter, courantpar, divbfix, typeB, ...  A selector the GPU path does
 ! not implement stops the run through die(); nothing falls back to the CPU.
 !
+! The build also routes the main loop's wold=w (src/vac.t:85) through
+! vacgpu_store_wold below, which copies only while the host's w is current.
+!
 ! State transfer: the state stays on the device between steps.  VAC's w is
 ! copied there at the first sweep of the run (and after the host changed it)
 ! and fetched back only where VAC reads it on the host:
@@ -216,6 +219,28 @@ double precision:: w(ixG^T,nw)
 
 if(.not.vacgpu_host_current) call vacgpu_dropin_get(w)
 call savefile_cpu(ifile,w)
+
+return
+end
+
+!=============================================================================
+subroutine vacgpu_store_wold(w)
+
+! The main loop's wold=w (src/vac.t:85), which the drop-in build routes
+! here.  On the host only the residual (src/vac.t:90-101) reads wold; the
+! device keeps its own copy of the step's start for the limiters and the
+! source steps.  Between steps the host's w lags the device state unless a
+! host routine made it current (a save, the residual, split sources, a
+! filter), so the copy is made only then -- for the residual it always is.
+
+use vacgpu_mod
+
+include 'vacdef.f'
+
+double precision:: w(ixG^T,nw)
+!-----------------------------------------------------------------------------
+
+if(vacgpu_host_current) wold(ixG^S,1:nw)=w(ixG^S,1:nw)
 
 return
 end
