@@ -2391,12 +2391,24 @@ __global__ __launch_bounds__(256) void k_cmaxdx(Geo g, const double* __restrict_
   if (threadIdx.x == 0 && threadIdx.y == 0) part[(SLOT_DX + d) * pstride + block_linear()] = b;
 }
 
+// Block-wide max / min of p[0..n).  Each thread keeps four running values
+// and so four loads in flight (a one-block fold is bound by load latency);
+// max and min do not depend on the order of the values.
 __device__ double fold(const double* p, long long n, bool is_max, double* sm) {
-  double v = is_max ? -INFINITY : INFINITY;
-  for (long long i = threadIdx.x; i < n; i += blockDim.x) {
-    double u = p[i];
-    v = is_max ? (v < u ? u : v) : (u < v ? u : v);
+  auto pick = [is_max](double v, double u) { return is_max ? (v < u ? u : v) : (u < v ? u : v); };
+  const double id = is_max ? -INFINITY : INFINITY;
+  double v0 = id, v1 = id, v2 = id, v3 = id;
+  const long long st = blockDim.x;
+  long long i = threadIdx.x;
+  for (; i + 3 * st < n; i += 4 * st) {
+    const double a = p[i], b = p[i + st], c = p[i + 2 * st], e = p[i + 3 * st];
+    v0 = pick(v0, a);
+    v1 = pick(v1, b);
+    v2 = pick(v2, c);
+    v3 = pick(v3, e);
   }
+  for (; i < n; i += st) v0 = pick(v0, p[i]);
+  const double v = pick(pick(v0, v1), pick(v2, v3));
   return is_max ? block_reduce<true>(v, sm) : block_reduce<false>(v, sm);
 }
 
@@ -2530,8 +2542,10 @@ __global__ __launch_bounds__(1024) void k_getdt_lagged(Geo g, const double* __re
 }
 
 // dtcourant update of the TVDLF corrector (src/vactvdlf.t:184-201)
+// step_end: the last sweep of a device-loop step also advances it and t
+// (src/vac.t:97-98); what follows in the step reads dt only
 __global__ void k_fin_courant_tvdlf(Scal* sc, const double* part, long long pstride, long long nb,
-                                    int d, double courantpar) {
+                                    int d, double courantpar, int step_end) {
   __shared__ double sm[16];
   const double cm = fold(part + SLOT_C0 * pstride, nb, true, sm);
   if (threadIdx.x == 0) {
@@ -2540,12 +2554,16 @@ __global__ void k_fin_courant_tvdlf(Scal* sc, const double* part, long long pstr
       double c = courantpar / cm;
       sc->dtcourant[d] = c < sc->dtcourant[d] ? c : sc->dtcourant[d];
     }
+    if (step_end) {
+      sc->it += 1;
+      sc->t = sc->t + sc->dt;
+    }
   }
 }
 
 // dtcourant update of tvdlimit2 for the two extreme waves (src/vactvd.t:145-157)
 __global__ void k_fin_courant_tvd(Scal* sc, const double* part, long long pstride, long long nb,
-                                  int d, double courantpar) {
+                                  int d, double courantpar, int step_end) {
   __shared__ double sm[16];
   for (int slot = SLOT_C0; slot <= SLOT_C1; ++slot) {
     const double cm = fold(part + slot * pstride, nb, true, sm);
@@ -2557,6 +2575,10 @@ __global__ void k_fin_courant_tvd(Scal* sc, const double* part, long long pstrid
       }
     }
     __syncthreads();
+  }
+  if (step_end && threadIdx.x == 0) {
+    sc->it += 1;
+    sc->t = sc->t + sc->dt;
   }
 }
 

@@ -645,8 +645,10 @@ int sweep_device(vacgpu_grid* g, const vacgpu_method* m, int d, int firstsweep, 
     if ((rc = launch_tvdmu_corr(g, m, d, s, win, wout, wl, ghosts, nbc))) return rc;
     if (courant) {
       Timed t(g, KF_RED);
+      const int step_end = (g->fold_step_end && lastsweep) ? 1 : 0;
+      if (step_end) g->step_end_done = true;
       hipLaunchKernelGGL(k_fin_courant_tvd, dim3(1), dim3(1024), 0, g->stream, g->sc, g->part,
-                         g->pstride, nbc, d, m->courantpar);
+                         g->pstride, nbc, d, m->courantpar, step_end);
       if ((rc = launch_check(g, "k_fin_courant_tvd"))) return rc;
     }
     if (rt_wold && (rc = roundtrip_wold(g))) return rc;
@@ -704,8 +706,10 @@ int sweep_device(vacgpu_grid* g, const vacgpu_method* m, int d, int firstsweep, 
     }
     if (courant) {
       Timed t(g, KF_RED);
+      const int step_end = (g->fold_step_end && lastsweep) ? 1 : 0;
+      if (step_end) g->step_end_done = true;
       hipLaunchKernelGGL(k_fin_courant_tvdlf, dim3(1), dim3(1024), 0, g->stream, g->sc, g->part,
-                         g->pstride, corr_blocks, d, m->courantpar);
+                         g->pstride, corr_blocks, d, m->courantpar, step_end);
       if ((rc = launch_check(g, "k_fin_courant_tvdlf"))) return rc;
     }
     if (rt_wold && (rc = roundtrip_wold(g))) return rc;
@@ -871,8 +875,11 @@ int sweep_device(vacgpu_grid* g, const vacgpu_method* m, int d, int firstsweep, 
     }
     if (courant) {
       Timed t(g, KF_RED);
+      // (the Ryu step that may follow reads dt only)
+      const int step_end = (g->fold_step_end && lastsweep) ? 1 : 0;
+      if (step_end) g->step_end_done = true;
       hipLaunchKernelGGL(k_fin_courant_tvd, dim3(1), dim3(1024), 0, g->stream, g->sc, g->part,
-                         g->pstride, nbt, d, m->courantpar);
+                         g->pstride, nbt, d, m->courantpar, step_end);
       if ((rc = launch_check(g, "k_fin_courant_tvd"))) return rc;
     }
     const bool ryu_folded = srcmode == 2;
@@ -985,7 +992,7 @@ int sweep_unsplit_device(vacgpu_grid* g, const vacgpu_method* m) {
       if (m->courantpar > 0.0) {
         Timed t(g, KF_RED);
         hipLaunchKernelGGL(k_fin_courant_tvd, dim3(1), dim3(1024), 0, g->stream, g->sc, g->part,
-                           g->pstride, nbc, d, m->courantpar);
+                           g->pstride, nbc, d, m->courantpar, 0);
         if ((rc = launch_check(g, "k_fin_courant_tvd"))) return rc;
       }
       if (rt_wold && (rc = roundtrip_wold(g))) return rc;
@@ -1032,7 +1039,7 @@ int sweep_unsplit_device(vacgpu_grid* g, const vacgpu_method* m) {
     if (m->courantpar > 0.0) {
       Timed t(g, KF_RED);
       hipLaunchKernelGGL(k_fin_courant_tvdlf, dim3(1), dim3(1024), 0, g->stream, g->sc, g->part,
-                         g->pstride, nbc, d, m->courantpar);
+                         g->pstride, nbc, d, m->courantpar, 0);
       if ((rc = launch_check(g, "k_fin_courant_tvdlf"))) return rc;
     }
     if (rt_wold && (rc = roundtrip_wold(g))) return rc;
