@@ -224,15 +224,32 @@ __device__ __forceinline__ float face32(const Sel& s, const float* b, const floa
   return cm;
 }
 
-// The predictor of one row at column px (image coordinates), with the
-// inputs loaded from HBM: base U, rounded base b, FP64 pressures, increment
-// dP and slopes l.
+// The predictor of one row at column px (image coordinates): base U, rounded
+// base b, FP64 pressures, increment dP and slopes l.  make() loads its
+// inputs from HBM; build() takes them from registers (uc, its neighbours um
+// and up along D, the rounded difference bd of the B component across D
+// and the two widths).
 template <bool MHD, int D, int LIM>
 struct BulkRow {
   static constexpr int NW = Layout<2, MHD>::nw;
   double U[NW];
   float dP[NW], l[NW];
   double ptot, pth;
+  __device__ __forceinline__ void build(const Sel& s, const double* uc, const double* um,
+                                        const double* up, float bd, double dx0, double dx1,
+                                        double dt, double smallp) {
+    float b[NW];
+#pragma unroll
+    for (int iw = 0; iw < NW; ++iw) {
+      U[iw] = uc[iw];
+      b[iw] = float(U[iw]);
+    }
+    float bdiff[2] = {0.f, 0.f};
+    if constexpr (MHD) bdiff[1 - D] = bd;
+    const double dxk[2] = {dx0, dx1};
+    base_press<2, MHD>(U, s.gamma, ptot, pth);
+    pred32<MHD, D, LIM>(s, U, um, up, bdiff, dxk, ptot, pth, dt, smallp, b, dP, l);
+  }
   __device__ __forceinline__ void make(const Geo& g, const FusedGeo& f, const Sel& s,
                                        const double* __restrict__ w, int px, int py, double dt,
                                        double smallp) {
@@ -369,7 +386,9 @@ __device__ __forceinline__ double bulk_fx_body(const Geo& g, const Sel& s, const
 }
 
 // y sweep, FP32-bulk: a marching step carries row j's base, its face-left
-// increment, rounded predictor state and pressures, and the face j-1/2 flux
+// increment, rounded predictor state and pressures, and the face j-1/2 flux.
+// The raw rows j, j+1, j+2 of w at column px stay in registers, so a step
+// loads one new row (and B_x beside the next one), one step ahead.
 template <bool MHD, int LIM>
 __device__ __forceinline__ double bulk_fy_body(const Geo& g, const Sel& s, const BC& bc,
                                                const double* __restrict__ w,
@@ -390,12 +409,28 @@ __device__ __forceinline__ double bulk_fy_body(const Geo& g, const Sel& s, const
   const int s1 = (s0 + seglen - 1 < f.n1 + 1) ? s0 + seglen - 1 : f.n1 + 1;
   const double* dx0 = g.dx[0];
   const double* dx1 = g.dx[1];
-  // row r's predictor ('cont' ghost rows take zero corrector slopes)
+  const double dxp = dx0[px];
+  // row r's predictor gathered from HBM ('cont' ghost rows take zero
+  // corrector slopes)
   auto make = [&](int r, BulkRow<MHD, 1, LIM>& R) {
     R.make(g, f, s, w, px, ghost_image(r, f.n1, f.pylo, f.pyhi), dt, smallp);
     const bool zs = (r < 2 && !f.pylo) || (r > f.n1 + 1 && !f.pyhi);
 #pragma unroll
     for (int iw = 0; iw < NW; ++iw) R.l[iw] = zs ? 0.f : R.l[iw];
+  };
+  auto load_row = [&](int r, double* u) {
+    const unsigned o = f.at(px, r);
+#pragma unroll
+    for (int iw = 0; iw < NW; ++iw) u[iw] = ldp(w, g.ps, iw, o);
+  };
+  // B_x at (px -/+ 1, r), for the predictor's divergence
+  auto load_bx = [&](int r, double* bx) {
+    bx[0] = bx[1] = 0.0;
+    if constexpr (MHD) {
+      const unsigned o = f.at(px, r);
+      bx[0] = ldp(w, g.ps, L::b(0), o - 8u);
+      bx[1] = ldp(w, g.ps, L::b(0), o + 8u);
+    }
   };
   // carried row state (the lower row of the next face)
   double U0[NW], pt0, ph0, Fpm, byU_m = 0.0;
@@ -435,12 +470,34 @@ __device__ __forceinline__ double bulk_fy_body(const Geo& g, const Sel& s, const
     if (useful) cmx = cmx < cv ? cv : cmx;
     carry(R);
   }
+  // window: W0 = row j, W1 = row j+1, W2 = row j+2, bx beside row j+1 and
+  // the widths dy0 = dy(j), dy1 = dy(j+1), all loaded one marching step
+  // ahead, so no load is consumed in the step that issues it (a load waited
+  // for would also wait for the prefetch: loads complete in order)
+  double W0[NW], W1[NW], W2[NW], W3[NW] = {}, bx[2] = {0.0, 0.0}, bxn[2] = {0.0, 0.0};
+  load_row(s0, W0);
+  load_row(s0 + 1, W1);
+  load_row(s0 + 2, W2);
+  if (s0 + 1 <= f.n1 + 1) load_bx(s0 + 1, bx);
+  double dy0 = dx1[s0], dy1 = dx1[s0 + 1];
+  const double dxc = dx0[c];
   for (int j = s0; j <= s1; ++j) {
+    if (j + 1 <= s1 && j + 2 <= f.n1 + 1) {  // the next step's row is a mesh row
+      load_row(j + 3, W3);
+      load_bx(j + 2, bxn);
+    }
+    const double dy2 = dx1[j + 2];
     BulkRow<MHD, 1, LIM> R;  // row j+1
-    make(j + 1, R);
+    if (j + 1 <= f.n1 + 1) {
+      float bd = 0.f;
+      if constexpr (MHD) bd = float(bx[1] - bx[0]);
+      R.build(s, W1, W0, W2, bd, dxp, dy1, dt, smallp);
+    } else {
+      make(j + 1, R);
+    }
     float Fn[NW];
     double Fnm;
-    const double cv = face(R, dx1[j] + dx1[j + 1], Fn, Fnm);
+    const double cv = face(R, dy0 + dy1, Fn, Fnm);
     if (useful) cmx = cmx < cv ? cv : cmx;
     float inc[NW];
 #pragma unroll
@@ -448,22 +505,33 @@ __device__ __forceinline__ double bulk_fy_body(const Geo& g, const Sel& s, const
     if constexpr (MHD) {
       const double bxp = lane_next(U0[L::b(0)]), bxm = lane_prev(U0[L::b(0)]);
       const float dxp_ = lane_next(dPbx0), dxm_ = lane_prev(dPbx0);
-      const float gx = pdiv(0.5f * (float(bxp - bxm) + (dxp_ - dxm_)), float(dx0[c]));
-      const float gy = pdiv(0.5f * (float(R.U[L::b(1)] - byU_m) + (R.dP[L::b(1)] - byd_m)), float(dx1[j]));
+      const float gx = pdiv(0.5f * (float(bxp - bxm) + (dxp_ - dxm_)), float(dxc));
+      const float gy = pdiv(0.5f * (float(R.U[L::b(1)] - byU_m) + (R.dP[L::b(1)] - byd_m)), float(dy0));
       powell32<ND>(sP0, (gx + gy) * float(dt / ND), inc);
       byU_m = U0[L::b(1)];
       byd_m = dPby0;
     }
-    const double qdx = pdiv(dt, dx1[j]);
+    const double qdx = pdiv(dt, dy0);
     double out[NW];
 #pragma unroll
     for (int iw = 0; iw < NW; ++iw) out[iw] = U0[iw] - qdx * double(Fn[iw] - Fp[iw]) + double(inc[iw]);
     out[L::m(1)] = U0[L::m(1)] - qdx * (Fnm - Fpm) + double(inc[L::m(1)]);
+    prefetch_done();
     if (useful) store_with_ghosts<NW>(g, f, wout, c, j, out);
 #pragma unroll
     for (int iw = 0; iw < NW; ++iw) Fp[iw] = Fn[iw];
     Fpm = Fnm;
     carry(R);
+#pragma unroll
+    for (int iw = 0; iw < NW; ++iw) {
+      W0[iw] = W1[iw];
+      W1[iw] = W2[iw];
+      W2[iw] = W3[iw];
+    }
+    bx[0] = bxn[0];
+    bx[1] = bxn[1];
+    dy0 = dy1;
+    dy1 = dy2;
   }
   return cmx;
 }

@@ -67,8 +67,8 @@ struct vacgpu_grid {
   // 'symm0' faces: the variables with zero flux through each face
   unsigned char nf[6] = {0, 0, 0, 0, 0, 0};
   bool nfany = false;
-  int fs_seglen = 16;
-  dim3 grid_fs, block_fs;
+  int fs_seglen[2] = {16, 16};  // fused TVDLF sweeps, per direction (x, y)
+  dim3 grid_fs[2], block_fs;
   int ry_seglen = 16;  // k_ryu_ab_y (2D one-pass Ryu step)
   dim3 grid_ry;
   double* stage = nullptr;  // pinned staging of put_state / get_state
@@ -392,37 +392,39 @@ int pick_seglen(long long strips, int rows, long long slots, double prologue) {
 
 // launch shape of the fused TVDLF sweeps: strips of 62 x cells, waves of
 // one segment of seglen rows each, four waves per block.  seglen 0 picks the
-// length by pick_seglen (a prologue of two predictor rows, about 1.5 rows of
-// work) over the fewer resident blocks of the x and y kernels, or about 4096
-// waves when the occupancy is unknown.
+// length of each direction's launch by pick_seglen (a prologue of two
+// predictor rows, about 1.5 rows of work) over that kernel's resident
+// blocks (the x and y kernels may hold different numbers of waves), or about
+// 4096 waves when the occupancy is unknown.
 void set_fused_shape(vacgpu_grid* g, int seglen) {
   const int strips = (g->geo.n[0] + FS_STRIDE - 1) / FS_STRIDE;
-  if (seglen <= 0) {
-    const void* fx = nullptr;
-    const void* fy = nullptr;
-    const bool mhd = g->cfg.physics == VACGPU_MHD, f32 = g->cfg.precision == VACGPU_FP32BULK;
+  const void* fk[2] = {nullptr, nullptr};
+  const bool mhd = g->cfg.physics == VACGPU_MHD, f32 = g->cfg.precision == VACGPU_FP32BULK;
 #define FSLOT(MV, RT)                                                     \
   do {                                                                    \
-    fx = reinterpret_cast<const void*>(&k_tvdlf_fused<MV, RT, 0, 0>);     \
-    fy = reinterpret_cast<const void*>(&k_tvdlf_fused<MV, RT, 1, 0>);     \
+    fk[0] = reinterpret_cast<const void*>(&k_tvdlf_fused<MV, RT, 0, 0>);  \
+    fk[1] = reinterpret_cast<const void*>(&k_tvdlf_fused<MV, RT, 1, 0>);  \
   } while (0)
-    if (mhd) { if (f32) FSLOT(true, float); else FSLOT(true, double); }
-    else { if (f32) FSLOT(false, float); else FSLOT(false, double); }
+  if (mhd) { if (f32) FSLOT(true, float); else FSLOT(true, double); }
+  else { if (f32) FSLOT(false, float); else FSLOT(false, double); }
 #undef FSLOT
-    const long long a = resident_blocks(fx), b = resident_blocks(fy);
-    const long long slots = a < b ? a : b;
-    if (slots > 0) {
-      seglen = pick_seglen(strips, g->geo.n[1], slots, 1.5);
-    } else {
-      const int want = (4096 + strips - 1) / strips;
-      seglen = (g->geo.n[1] + want - 1) / want;
-    }
-  }
-  seglen = seglen < 4 ? 4 : (seglen > 256 ? 256 : seglen);
-  g->fs_seglen = seglen;
-  const int segs = (g->geo.n[1] + seglen - 1) / seglen;
   g->block_fs = dim3(64, 4, 1);
-  g->grid_fs = dim3(strips, (segs + 3) / 4, 1);
+  for (int d = 0; d < 2; ++d) {
+    int sl = seglen;
+    if (sl <= 0) {
+      const long long slots = resident_blocks(fk[d]);
+      if (slots > 0) {
+        sl = pick_seglen(strips, g->geo.n[1], slots, 1.5);
+      } else {
+        const int want = (4096 + strips - 1) / strips;
+        sl = (g->geo.n[1] + want - 1) / want;
+      }
+    }
+    sl = sl < 4 ? 4 : (sl > 256 ? 256 : sl);
+    g->fs_seglen[d] = sl;
+    const int segs = (g->geo.n[1] + sl - 1) / sl;
+    g->grid_fs[d] = dim3(strips, (segs + 3) / 4, 1);
+  }
 }
 
 // launch shape of k_ryu_ab_y (2D): strips of 62 x cells, four segments per
@@ -611,14 +613,13 @@ int sweep_device(vacgpu_grid* g, const vacgpu_method* m, int d, int firstsweep, 
                      g->use_fused && one_lim && (!mhd || src) && !dirty &&
                      m->typelimited == VACGPU_LIM_ORIGINAL && !m->useprimitive;
   if (fused) {
-    const long long nbf = (long long)g->grid_fs.x * g->grid_fs.y;
     {
       Timed t(g, KF_FUSED);
       const bool f32 = g->cfg.precision == VACGPU_FP32BULK;
       const int lim = m->typelimiter[0];
       const int step_end = (g->fold_step_end && lastsweep) ? 1 : 0;
       if (step_end) g->step_end_done = true;
-#define FUSED_L(MV, RT, DV, LV) hipLaunchKernelGGL((k_tvdlf_fused<MV, RT, DV, LV>), g->grid_fs, g->block_fs, 0, g->stream, g->geo, s, g->bc, win, wout, g->sc, g->fs_seglen, g->fold, m->courantpar, step_end)
+#define FUSED_L(MV, RT, DV, LV) hipLaunchKernelGGL((k_tvdlf_fused<MV, RT, DV, LV>), g->grid_fs[DV], g->block_fs, 0, g->stream, g->geo, s, g->bc, win, wout, g->sc, g->fs_seglen[DV], g->fold, m->courantpar, step_end)
 #define FUSED(MV, RT, DV) do { if (lim == LIM_MINMOD) FUSED_L(MV, RT, DV, LIM_MINMOD); else if (lim == LIM_WOODWARD) FUSED_L(MV, RT, DV, LIM_WOODWARD); else FUSED_L(MV, RT, DV, LIM_SUPERBEE); } while (0)
       if (mhd) {
         if (f32) { if (d == 0) FUSED(true, float, 0); else FUSED(true, float, 1); }
@@ -631,7 +632,6 @@ int sweep_device(vacgpu_grid* g, const vacgpu_method* m, int d, int firstsweep, 
 #undef FUSED_L
       if ((rc = launch_check(g, "k_tvdlf_fused"))) return rc;
     }
-    (void)nbf;
     // the fused kernel has folded the Courant maximum and filled wout's ghost cells
     g->cur = outi;
     return 0;
