@@ -16,6 +16,18 @@ using namespace vacgpu;
 namespace {
 
 constexpr int BX = 64, BY = 4;      // cell kernels: 64 lanes along x, 4 rows
+
+#ifndef VACGPU_NT
+#define VACGPU_NT 1
+#endif
+// A sweep's output (and the predictor's w1) is read again only by a later
+// launch, long after the 4 MB L2 of an XCD has turned over, so its stores
+// are nontemporal: they do not displace the input rows and planes that
+// neighbouring tiles re-read from the L2.  The stored values are the same.
+__device__ __forceinline__ void st_out(double* p, double v) {
+  if constexpr (VACGPU_NT) __builtin_nontemporal_store(v, p);
+  else *p = v;
+}
 constexpr int NSLOT = 8;            // partial-reduction slots
 enum { SLOT_C0 = 0, SLOT_C1 = 1, SLOT_PMIN = 2, SLOT_PMAX = 3, SLOT_DX = 4 };
 enum { KF_TVD = 0, KF_PRED = 1, KF_CORR = 2, KF_RYU = 3, KF_BOUND = 4, KF_RED = 5, KF_FUSED = 6,
@@ -657,7 +669,7 @@ __device__ __forceinline__ void store_cell(const Geo& g, const BC& bc, double* _
                                            long long idx, const int* c, const double* out,
                                            int ghosts) {
 #pragma unroll
-  for (int iw = 0; iw < NW; ++iw) wout[iw * g.ps + idx] = out[iw];
+  for (int iw = 0; iw < NW; ++iw) st_out(wout + iw * g.ps + idx, out[iw]);
   if (!ghosts) return;
   bool edge = false;
 #pragma unroll
@@ -677,7 +689,7 @@ __device__ __forceinline__ void store_cell(const Geo& g, const BC& bc, double* _
         long long o = g.off + t[0][a] + t[1][b] * g.s[1];
         if (ND == 3) o += t[2][e] * g.s[2];
 #pragma unroll
-        for (int iw = 0; iw < NW; ++iw) wout[iw * g.ps + o] = out[iw];
+        for (int iw = 0; iw < NW; ++iw) st_out(wout + iw * g.ps + o, out[iw]);
       }
 }
 
@@ -1051,7 +1063,7 @@ __device__ __forceinline__ double ldp(const double* __restrict__ w, long long ps
   return *reinterpret_cast<const double*>(reinterpret_cast<const char*>(w + iw * ps) + o);
 }
 __device__ __forceinline__ void stp(double* __restrict__ w, long long ps, int iw, unsigned o, double v) {
-  *reinterpret_cast<double*>(reinterpret_cast<char*>(w + iw * ps) + o) = v;
+  st_out(reinterpret_cast<double*>(reinterpret_cast<char*>(w + iw * ps) + o), v);
 }
 
 struct FusedGeo {  // per-wave constants of a fused sweep
