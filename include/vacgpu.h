@@ -174,6 +174,14 @@ int vacgpu_set_kernels(vacgpu_grid* g, int fused, int seglen);
 int vacgpu_check_quotients(const double* a, const double* b, long long n, double* q_kernel,
                            double* q_ieee);
 
+/* Self-test of the square root the device kernels use (sound and
+ * magnetosonic speeds, Roe-mean sqrt(rho), eigenvector weights; no
+ * counterpart in VAC, whose SQRT is the compiler's): computes r_kernel[i]
+ * with that root and r_ieee[i] = sqrt(x[i]) with the compiler's correctly
+ * rounded square root, on the current device.  The two agree bit for bit
+ * for every x.  Host arrays of n doubles. */
+int vacgpu_check_roots(const double* x, long long n, double* r_kernel, double* r_ieee);
+
 /* Asynchronous snapshots (no counterpart in VAC, whose savefile,
  * src/vacio.t:1008-1047, writes the host copy of w and so needs the state
  * downloaded first).  vacgpu_snapshot_start copies the current state (ixG,

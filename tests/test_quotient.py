@@ -1,4 +1,5 @@
-"""The quotient of the device kernels against the compiler's IEEE division.
+"""The quotient and the square root of the device kernels against the
+compiler's IEEE division and sqrt.
 
 Every division by a positive quantity on the hot path (densities, cell
 widths, squared sound speeds, the Roe-mean sqrt(rho)) goes through one
@@ -8,6 +9,10 @@ the same double as a correctly rounded division over the operand range a
 physical state produces, so FP64 results stay bitwise equal to VAC's.  The
 operands cover wide exponent ranges, significands with all bits set,
 quotients at and next to exact values, and zero numerators of both signs.
+
+Every square root takes the compiler's own sequence without its scaling and
+special-value steps wherever those do nothing, and sqrt() elsewhere: it must
+give the same double for every input, special values included.
 """
 import ctypes as C
 
@@ -61,3 +66,48 @@ def test_quotient_matches_ieee_division():
                   & ((a == 0) | (q >= tiny)) & np.isfinite(q))
     bad = np.nonzero(qk.view(np.uint64) != qi.view(np.uint64))[0]
     assert bad.size == 0, [(a[i], b[i], qk[i], qi[i]) for i in bad[:5]]
+
+
+def root_operands(n, seed):
+    rng = np.random.default_rng(seed)
+    x = np.exp2(rng.uniform(-1000.0, 1000.0, n)) * rng.uniform(1.0, 2.0, n)
+    k = n // 8
+    # significands with every bit set, and powers of two
+    x[:k] = np.nextafter(np.exp2(rng.integers(-900, 900, k).astype(float) + 1.0), 0.0)
+    x[k:2 * k] = np.exp2(rng.integers(-1000, 1000, k).astype(float))
+    # perfect squares and their neighbours (roots at and next to exact values)
+    r = rng.uniform(1.0, 2.0**26, k).round() * np.exp2(rng.integers(-300, 300, k).astype(float))
+    x[2 * k:3 * k] = r * r
+    x[3 * k:4 * k] = np.nextafter(r * r, np.inf)
+    x[4 * k:5 * k] = np.nextafter(r * r, -np.inf)
+    # the magnitudes of the sweep kernels
+    x[5 * k:6 * k] = rng.uniform(1e-6, 1e3, k)
+    # around the scaling threshold 2^-767, subnormals and the special values
+    t = 2.0**-767
+    m = 6 * k
+    for v in (t, np.nextafter(t, 0.0), np.nextafter(t, 1.0), 0.0, -0.0, np.inf, -np.inf, np.nan,
+              -1.0, -t, 5e-324, 2.2e-308, np.finfo(np.float64).max, np.finfo(np.float64).tiny):
+        x[m] = v
+        m += 1
+    x[m:m + 4096] = np.exp2(rng.uniform(-1074.0, -760.0, 4096))
+    return np.ascontiguousarray(x)
+
+
+def test_root_matches_ieee_sqrt():
+    lib = vacgpu.load_library()
+    n = 1 << 22
+    x = root_operands(n, 20261204)
+    rk = np.empty(n)
+    ri = np.empty(n)
+    p = lambda v: v.ctypes.data_as(C.POINTER(C.c_double))
+    print("case square-root self-test", n, flush=True)
+    assert lib.vacgpu_check_roots(p(x), n, p(rk), p(ri)) == 0
+    # the compiler's square root is correctly rounded: numpy's root (NaN for x < 0)
+    with np.errstate(invalid="ignore"):
+        ref = np.sqrt(x)
+    fin = ~np.isnan(ref)
+    assert np.array_equal(ri[fin].view(np.uint64), ref[fin].view(np.uint64))
+    assert np.all(np.isnan(ri[~fin]))
+    same = (rk.view(np.uint64) == ri.view(np.uint64)) | (np.isnan(rk) & np.isnan(ri))
+    bad = np.nonzero(~same)[0]
+    assert bad.size == 0, [(x[i], rk[i], ri[i]) for i in bad[:5]]

@@ -1255,6 +1255,15 @@ __global__ void k_check_quotients(const double* __restrict__ a, const double* __
   qi[i] = a[i] / b[i];
 }
 
+// psqrt, the square root of the sweep kernels, beside the compiler's sqrt
+__global__ void k_check_roots(const double* __restrict__ x, long long n, double* __restrict__ rk,
+                              double* __restrict__ ri) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  rk[i] = psqrt(x[i]);
+  ri[i] = sqrt(x[i]);
+}
+
 extern "C" {
 
 int vacgpu_supported(const vacgpu_config* cfg, const vacgpu_method* m) {
@@ -1707,6 +1716,24 @@ int vacgpu_check_quotients(const double* a, const double* b, long long n, double
   }
   if (e == hipSuccess) e = hipMemcpy(q_kernel, d + 2 * n, bytes, hipMemcpyDeviceToHost);
   if (e == hipSuccess) e = hipMemcpy(q_ieee, d + 3 * n, bytes, hipMemcpyDeviceToHost);
+  (void)hipFree(d);
+  return e == hipSuccess ? 0 : VACGPU_E_HIP;
+}
+
+int vacgpu_check_roots(const double* x, long long n, double* r_kernel, double* r_ieee) {
+  if (!x || !r_kernel || !r_ieee || n < 0) return VACGPU_E_ARG;
+  if (n == 0) return 0;
+  const size_t bytes = sizeof(double) * (size_t)n;
+  double* d = nullptr;
+  if (hipMalloc(&d, 3 * bytes) != hipSuccess) return VACGPU_E_HIP;
+  hipError_t e = hipMemcpy(d, x, bytes, hipMemcpyHostToDevice);
+  if (e == hipSuccess) {
+    hipLaunchKernelGGL(k_check_roots, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, nullptr, d,
+                       n, d + n, d + 2 * n);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipMemcpy(r_kernel, d + n, bytes, hipMemcpyDeviceToHost);
+  if (e == hipSuccess) e = hipMemcpy(r_ieee, d + 2 * n, bytes, hipMemcpyDeviceToHost);
   (void)hipFree(d);
   return e == hipSuccess ? 0 : VACGPU_E_HIP;
 }

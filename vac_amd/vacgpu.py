@@ -69,6 +69,7 @@ SIGNATURES = {
     "vacgpu_kernel_time": ([C.c_void_p, C.c_int, _DP, C.POINTER(C.c_long)], C.c_int),
     "vacgpu_set_kernels": ([C.c_void_p, C.c_int, C.c_int], C.c_int),
     "vacgpu_check_quotients": ([_DP, _DP, C.c_longlong, _DP, _DP], C.c_int),
+    "vacgpu_check_roots": ([_DP, C.c_longlong, _DP, _DP], C.c_int),
     "vacgpu_snapshot_start": ([C.c_void_p, C.c_int], C.c_int),
     "vacgpu_snapshot_fetch": ([C.c_void_p, C.c_int, _DP, _IP, _IP, _DP], C.c_int),
     "vacgpu_snapshot_fetch_mesh": ([C.c_void_p, C.c_int, _DP, _IP, _DP], C.c_int),
