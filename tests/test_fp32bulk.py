@@ -9,7 +9,9 @@ pressure of a state is the FP64 base pressure plus the FP32 change of the
 pressure formula, the pressure part of the normal-momentum flux and its
 difference across a cell stay FP64, and the Powell source is an FP32
 increment accumulated in FP64.  The other kernels keep the flux of the
-normal momentum, the face states and the source terms in FP64.
+normal momentum, the face states and the source terms in FP64; on 3D grids
+the TVDLF predictor stores w1 as its FP32 increment over the sweep's input,
+which the corrector adds back in FP64.
 
 Tolerance (stated by the project): relative L1 error <= 1e-5 per conserved
 variable against the FP64 oracle after the stated number of steps.  The L1
@@ -37,6 +39,9 @@ CASES = [
     ("rf128-tvdmu-woodward", lambda g: random_fourier((128, 128), 5, g), "mhd", 5 / 3, "tvdmu", "woodward", 40, "periodic"),
     ("hd-rf160x128-tvd-woodward", lambda g: random_fourier((160, 128), 10, g, physics="hd"), "hd", 1.4, "tvd", "woodward", 40, "periodic"),
     ("hd-rf128-tvdlf-minmod-cont", lambda g: random_fourier((128, 128), 9, g, physics="hd"), "hd", 5 / 3, "tvdlf", "minmod", 80, "cont"),
+    # 3D: w1 stored as its FP32 increment over the sweep's input
+    ("rf3d-26x20x16-tvdlf-woodward-cont", lambda g: random_fourier((26, 20, 16), 11, g), "mhd", 1.4, "tvdlf", "woodward", 20, "cont"),
+    ("hd-rf3d-28x18x22-tvdlf-superbee", lambda g: random_fourier((28, 18, 22), 12, g, physics="hd"), "hd", 5 / 3, "tvdlf", "superbee", 20, "periodic"),
 ]
 
 

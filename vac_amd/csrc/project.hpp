@@ -240,7 +240,7 @@ __global__ __launch_bounds__(256) void k_cg_matvec(Geo g, SBC sb, RDX rx,
   double pz = 0.0;
   if (cl.ok) {
     const double zc = first ? zo[cl.idx] : rhs[cl.idx] + bet * zo[cl.idx];
-    zn[cl.idx] = zc;
+    st_out(zn + cl.idx, zc);
     double qy = 0.0;
 #pragma unroll
     for (int d = 0; d < ND; ++d) {
@@ -264,7 +264,7 @@ __global__ __launch_bounds__(256) void k_cg_matvec(Geo g, SBC sb, RDX rx,
       const double qdd = pdiv_y(0.5 * (qdp - qdm), dxd[cd], rd[cd]);
       qy = qy + qdd;
     }
-    az[cl.idx] = qy;
+    st_out(az + cl.idx, qy);
     pz = zc * qy;
   }
   const double bs = block_sum(pz, sm);
@@ -303,7 +303,7 @@ __global__ __launch_bounds__(256) void k_cg_update(Geo g, double* __restrict__ p
   const double alf = cg->alf;
   double sq = 0.0, mx = -INFINITY;
   if (cl.ok) {
-    phi[cl.idx] = phi[cl.idx] + alf * z[cl.idx];
+    st_out(phi + cl.idx, phi[cl.idx] + alf * z[cl.idx]);
     const double r = rhs[cl.idx] - alf * az[cl.idx];
     rhs[cl.idx] = r;
     sq = r * r;

@@ -24,7 +24,8 @@ constexpr int BX = 64, BY = 4;      // cell kernels: 64 lanes along x, 4 rows
 // launch, long after the 4 MB L2 of an XCD has turned over, so its stores
 // are nontemporal: they do not displace the input rows and planes that
 // neighbouring tiles re-read from the L2.  The stored values are the same.
-__device__ __forceinline__ void st_out(double* p, double v) {
+template <class T>
+__device__ __forceinline__ void st_out(T* p, T v) {
   if constexpr (VACGPU_NT) __builtin_nontemporal_store(v, p);
   else *p = v;
 }
@@ -457,7 +458,7 @@ __device__ __forceinline__ void tvdlf_pred_body(const Geo& g, const Sel& s,
   double out[NW];
   pred_point<ND, MHD, R, D>(g, s, w, cl.idx, cl.c, dt, smallp, out);
 #pragma unroll
-  for (int iw = 0; iw < NW; ++iw) w1[iw * g.ps + cl.idx] = out[iw];
+  for (int iw = 0; iw < NW; ++iw) st_out(w1 + iw * g.ps + cl.idx, out[iw]);
 }
 
 
@@ -574,7 +575,7 @@ __device__ __forceinline__ double tvdlf_corr_body(const Geo& g, const Sel& s,
       }
     }
 #pragma unroll
-    for (int iw = 0; iw < NW; ++iw) wout[iw * g.ps + cl.idx] = out[iw];
+    for (int iw = 0; iw < NW; ++iw) st_out(wout + iw * g.ps + cl.idx, out[iw]);
   }
   return cmx;
 }
@@ -664,9 +665,9 @@ __device__ __forceinline__ T lane_prev(T v) { return dpp_shift<0x138>(v); }
 // store out at mesh cell c (index idx) and, when ghosts is set, at every
 // ghost cell whose image it is (one boundary type per face for all
 // variables, n >= 4): the sweep then leaves the output's ghost cells filled
-template <int ND, int NW>
-__device__ __forceinline__ void store_cell(const Geo& g, const BC& bc, double* __restrict__ wout,
-                                           long long idx, const int* c, const double* out,
+template <int ND, int NW, class T = double>
+__device__ __forceinline__ void store_cell(const Geo& g, const BC& bc, T* __restrict__ wout,
+                                           long long idx, const int* c, const T* out,
                                            int ghosts) {
 #pragma unroll
   for (int iw = 0; iw < NW; ++iw) st_out(wout + iw * g.ps + idx, out[iw]);
@@ -696,7 +697,11 @@ __device__ __forceinline__ void store_cell(const Geo& g, const BC& bc, double* _
 // predictor launch: one cell per thread; with ghosts set the ghost cells of
 // w1 that take a cell's value are written too (one boundary type per face).
 // LIM >= 0: one limiter type for every variable and no 'symm0' face.
-template <int ND, bool MHD, class R, int D, bool PRIM = false, int LIM = -1>
+// DW (FP32-bulk, copy-type faces): w1 is stored as its FP32 increment over
+// the sweep's input, w1 - w, in the first half of the w1 buffer (planes of
+// ps floats); the corrector adds it back to w in FP64.  A ghost cell's
+// increment is its image's, as its w is its image's.
+template <int ND, bool MHD, class R, int D, bool PRIM = false, int LIM = -1, bool DW = false>
 __global__ __launch_bounds__(256) void k_tvdlf_pred(Geo g, Sel s, BC bc, const double* __restrict__ w,
                                                    double* __restrict__ w1,
                                                    const Scal* __restrict__ sc, int ghosts) {
@@ -705,7 +710,27 @@ __global__ __launch_bounds__(256) void k_tvdlf_pred(Geo g, Sel s, BC bc, const d
   if (!cl.ok) return;
   double out[NW];
   pred_point<ND, MHD, R, D, LIM, 2, PRIM>(g, s, w, cl.idx, cl.c, sc->dt, sc->smallp, out);
-  store_cell<ND, NW>(g, bc, w1, cl.idx, cl.c, out, ghosts);
+  if constexpr (DW) {
+    float dw[NW];
+#pragma unroll
+    for (int iw = 0; iw < NW; ++iw) dw[iw] = float(out[iw] - w[iw * g.ps + cl.idx]);
+    store_cell<ND, NW, float>(g, bc, reinterpret_cast<float*>(w1), cl.idx, cl.c, dw, ghosts);
+  } else {
+    store_cell<ND, NW>(g, bc, w1, cl.idx, cl.c, out, ghosts);
+  }
+}
+
+// w1 at idx in the increment form: w + the stored FP32 increment
+template <int NW>
+__device__ __forceinline__ void load_dw(const double* __restrict__ w, const double* __restrict__ w1,
+                                        long long ps, long long idx, double* u) {
+  const float* d = reinterpret_cast<const float*>(w1);
+#pragma unroll
+  for (int iw = 0; iw < NW; ++iw) u[iw] = w[iw * ps + idx] + double(d[iw * ps + idx]);
+}
+__device__ __forceinline__ double ld_dw(const double* __restrict__ w, const double* __restrict__ w1,
+                                        long long ps, int iw, long long idx) {
+  return w[iw * ps + idx] + double(reinterpret_cast<const float*>(w1)[iw * ps + idx]);
 }
 
 
@@ -717,7 +742,7 @@ __global__ __launch_bounds__(256) void k_tvdlf_pred(Geo g, Sel s, BC bc, const d
 // face and of a cell update is tvdlf_corr_body's.
 constexpr int CX_STRIDE = 62;
 
-template <int ND, bool MHD, class R, int LIM, bool PRIM = false>
+template <int ND, bool MHD, class R, int LIM, bool PRIM = false, bool DW = false>
 __global__ __launch_bounds__(256) void k_tvdlf_corrx(Geo g, Sel s, BC bc, const double* __restrict__ w,
                                                     const double* __restrict__ w1,
                                                     double* __restrict__ wout,
@@ -760,6 +785,17 @@ __global__ __launch_bounds__(256) void k_tvdlf_corrx(Geo g, Sel s, BC bc, const 
     load<NW>(w1, g.ps, idx + 1, u1);
     roundtrips<ND, MHD>(u0, s.rtu, s.gamma, smallp);
     roundtrips<ND, MHD>(u1, s.rtu, s.gamma, smallp);
+  } else if constexpr (DW) {
+    // increment form (the limiter's base is w): w1 at this lane's cell, at
+    // the next cell from the next lane
+#pragma unroll
+    for (int iw = 0; iw < NW; ++iw) {
+      const double* p = w + iw * g.ps + idx;
+      l[iw] = slope<R, LIM>(s.lim[iw], p[-1], p[0], p[1]);
+    }
+    load_dw<NW>(w, w1, g.ps, idx, u0);
+#pragma unroll
+    for (int iw = 0; iw < NW; ++iw) u1[iw] = lane_next(u0[iw]);
   } else {
 #pragma unroll
     for (int iw = 0; iw < NW; ++iw) {
@@ -785,7 +821,25 @@ __global__ __launch_bounds__(256) void k_tvdlf_corrx(Geo g, Sel s, BC bc, const 
   if constexpr (MHD) {
     if (s.sourceunsplit && s.divbfix) {
       const int c3[3] = {cc, yc, z};
-      const double dv = divb_at<ND>(g, w1, idx, c3);
+      double dv;
+      if constexpr (DW) {
+        // divb_at's arithmetic on w1: B_x of the x neighbours from the next
+        // and previous lanes (lanes 1..62 store), the rest loaded
+        using LB = Layout<ND, true>;
+        const double bxp = lane_next(u0[LB::b(0)]), bxm = lane_prev(u0[LB::b(0)]);
+        dv = 0.0;
+        const double g0 = pdiv(0.5 * (bxp - bxm), g.dx[0][cc]);
+        dv = dv + g0;
+#pragma unroll
+        for (int d = 1; d < ND; ++d) {
+          const double bp = ld_dw(w, w1, g.ps, LB::b(d), idx + g.s[d]);
+          const double bm = ld_dw(w, w1, g.ps, LB::b(d), idx - g.s[d]);
+          const double gq = pdiv(0.5 * (bp - bm), g.dx[d][c3[d]]);
+          dv = dv + gq;
+        }
+      } else {
+        dv = divb_at<ND>(g, w1, idx, c3);
+      }
       if constexpr (PRIM) {
         double us[NW];
 #pragma unroll
@@ -805,7 +859,7 @@ __global__ __launch_bounds__(256) void k_tvdlf_corrx(Geo g, Sel s, BC bc, const 
   if (threadIdx.x == 0 && threadIdx.y == 0) part[SLOT_C0 * pstride + block_linear()] = bm;
 }
 
-template <int ND, bool MHD, class R, int D, int LIM, bool PRIM = false>
+template <int ND, bool MHD, class R, int D, int LIM, bool PRIM = false, bool DW = false>
 __global__ __launch_bounds__(256) void k_tvdlf_corrm(Geo g, Sel s, BC bc, const double* __restrict__ w,
                                                     const double* __restrict__ w1,
                                                     double* __restrict__ wout,
@@ -847,17 +901,27 @@ __global__ __launch_bounds__(256) void k_tvdlf_corrm(Geo g, Sel s, BC bc, const 
       load<NW>(w1, g.ps, at, v);
       if constexpr (PRIM) roundtrips<ND, MHD>(v, s.rtu, s.gamma, smallp);
     };
+    // increment form (DW, the limiter's base is w): w1 at J is the window's
+    // w at J plus the stored increment
+    auto dw1 = [&](long long at, const double* wj, double* v) {
+      const float* d = reinterpret_cast<const float*>(w1);
+#pragma unroll
+      for (int iw = 0; iw < NW; ++iw) v[iw] = wj[iw] + double(d[iw * g.ps + at]);
+    };
     double wa[NW], wb[NW], wc[NW], u0[NW], lc[NW], Fp[NW];
+    [[maybe_unused]] double bdm = 0.0;  // B_D of w1 one cell below (DW)
     lwin(base + (long long)(s0 - 2) * sd, wa);
     lwin(base + (long long)(s0 - 1) * sd, wb);
     lwin(base + (long long)s0 * sd, wc);
-    lw1(base + (long long)(s0 - 1) * sd, u0);
+    if constexpr (DW) dw1(base + (long long)(s0 - 1) * sd, wb, u0);
+    else lw1(base + (long long)(s0 - 1) * sd, u0);
 #pragma unroll
     for (int iw = 0; iw < NW; ++iw) lc[iw] = slope<R, LIM>(s.lim[iw], wa[iw], wb[iw], wc[iw]);
     // face s0-1/2
     {
       double u1[NW], ln[NW], wd[NW];
-      lw1(base + (long long)s0 * sd, u1);
+      if constexpr (DW) dw1(base + (long long)s0 * sd, wc, u1);
+      else lw1(base + (long long)s0 * sd, u1);
       lwin(base + (long long)(s0 + 1) * sd, wd);
 #pragma unroll
       for (int iw = 0; iw < NW; ++iw) ln[iw] = slope<R, LIM>(s.lim[iw], wb[iw], wc[iw], wd[iw]);
@@ -865,6 +929,7 @@ __global__ __launch_bounds__(256) void k_tvdlf_corrm(Geo g, Sel s, BC bc, const 
       corr_face<ND, MHD, R, D, PRIM>(s, u0, u1, lc, ln, dxd[s0 - 1] + dxd[s0], smallp, Fp, cv);
       if constexpr (LIM < 0) zero_flux<NW>(Fp, noflux(s, g, D, s0 - 1));
       cmx = cmx < cv ? cv : cmx;
+      if constexpr (DW && MHD) bdm = u0[Layout<ND, true>::b(D)];
 #pragma unroll
       for (int iw = 0; iw < NW; ++iw) {
         u0[iw] = u1[iw];
@@ -877,7 +942,8 @@ __global__ __launch_bounds__(256) void k_tvdlf_corrm(Geo g, Sel s, BC bc, const 
     // now: u0 = w1(i), lc = slope(i), wb = w(i), wc = w(i+1)
     for (int i = s0; i <= s1; ++i) {
       double u1[NW], ln[NW], wd[NW], F[NW], cv;
-      lw1(base + (long long)(i + 1) * sd, u1);
+      if constexpr (DW) dw1(base + (long long)(i + 1) * sd, wc, u1);
+      else lw1(base + (long long)(i + 1) * sd, u1);
       lwin(base + (long long)(i + 2) * sd, wd);
 #pragma unroll
       for (int iw = 0; iw < NW; ++iw) ln[iw] = slope<R, LIM>(s.lim[iw], wb[iw], wc[iw], wd[iw]);
@@ -898,7 +964,21 @@ __global__ __launch_bounds__(256) void k_tvdlf_corrm(Geo g, Sel s, BC bc, const 
       if constexpr (MHD) {
         if (s.sourceunsplit && s.divbfix) {
           c3[D] = i;
-          const double dv = divb_at<ND>(g, w1, idx, c3);
+          double dv;
+          if constexpr (DW) {
+            // divb_at's arithmetic on w1: B_D below and above from the window
+            using LB = Layout<ND, true>;
+            dv = 0.0;
+#pragma unroll
+            for (int d = 0; d < ND; ++d) {
+              const double bp = d == D ? u1[LB::b(d)] : ld_dw(w, w1, g.ps, LB::b(d), idx + g.s[d]);
+              const double bm = d == D ? bdm : ld_dw(w, w1, g.ps, LB::b(d), idx - g.s[d]);
+              const double gq = pdiv(0.5 * (bp - bm), g.dx[d][c3[d]]);
+              dv = dv + gq;
+            }
+          } else {
+            dv = divb_at<ND>(g, w1, idx, c3);
+          }
           if constexpr (PRIM) {
             double us[NW];
 #pragma unroll
@@ -912,6 +992,7 @@ __global__ __launch_bounds__(256) void k_tvdlf_corrm(Geo g, Sel s, BC bc, const 
       }
       c3[D] = i;
       store_cell<ND, NW>(g, bc, wout, idx, c3, out, ghosts);
+      if constexpr (DW && MHD) bdm = u0[Layout<ND, true>::b(D)];
 #pragma unroll
       for (int iw = 0; iw < NW; ++iw) {
         Fp[iw] = F[iw];
@@ -1457,7 +1538,7 @@ __global__ __launch_bounds__(256) void k_tvd_halfsrc(Geo g, const double* __rest
   for (int iw = 0; iw < NW; ++iw) out[iw] = u[iw];
   powell<ND>(u, divb_at<ND>(g, w, cl.idx, cl.c), sc->dt / 2, out);
 #pragma unroll
-  for (int iw = 0; iw < NW; ++iw) w1[iw * g.ps + cl.idx] = out[iw];
+  for (int iw = 0; iw < NW; ++iw) st_out(w1 + iw * g.ps + cl.idx, out[iw]);
 }
 
 // Sources folded into the TVD sweeps (2D, one boundary type per face for
@@ -2148,7 +2229,7 @@ __global__ __launch_bounds__(256) void k_tvd_m(Geo g, Sel s, BC bc, const double
       tvd_dissipate<ND, R, MHD>(fm.rt, phim, f0.rt, phi0, d, s, R(d0 + dm), R(dp + d0), d0, out,
                                 nfm, nf0);
 #pragma unroll
-      for (int iw = 0; iw < NW; ++iw) wout[iw * g.ps + idx] = out[iw];
+      for (int iw = 0; iw < NW; ++iw) st_out(wout + iw * g.ps + idx, out[iw]);
       if (i == s1) break;
       // shift the window
 #pragma unroll

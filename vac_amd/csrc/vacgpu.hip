@@ -476,9 +476,23 @@ int roundtrip_wold(vacgpu_grid* g) {
 // primitive-variable limiting (FP64 only, checked with the method), else
 // specialised for a limiter common to every variable
 int launch_pred(vacgpu_grid* g, const vacgpu_method* m, int d, const Sel& s, const double* win,
-                int ghosts) {
+                int ghosts, bool dw = false) {
   Timed t(g, KF_PRED);
-  if (m->useprimitive) {
+  if (dw) {
+    // FP32-bulk 3D: w1 as its FP32 increment over win
+    const int lim = common_limiter(g, m);
+#define PLW(DV, LV) hipLaunchKernelGGL((k_tvdlf_pred<ND, MHD, R, DV, false, LV, true>), g->grid, g->block, 0, g->stream, g->geo, s, g->bc, win, g->w1, g->sc, ghosts)
+#define PLWD(DV) do { if (lim == LIM_MINMOD) PLW(DV, LIM_MINMOD); else if (lim == LIM_WOODWARD) PLW(DV, LIM_WOODWARD); else PLW(DV, LIM_SUPERBEE); } while (0)
+    DISPATCH(g, {
+      if constexpr (ND == 3 && std::is_same<R, float>::value) {
+        if (d == 0) PLWD(0);
+        else if (d == 1) PLWD(1);
+        else PLWD(2);
+      }
+    });
+#undef PLWD
+#undef PLW
+  } else if (m->useprimitive) {
     DISPATCH(g, {
       if constexpr (std::is_same<R, double>::value) {
         if (d == 0) hipLaunchKernelGGL((k_tvdlf_pred<ND, MHD, R, 0, true>), g->grid, g->block, 0, g->stream, g->geo, s, g->bc, win, g->w1, g->sc, ghosts);
@@ -660,25 +674,38 @@ int sweep_device(vacgpu_grid* g, const vacgpu_method* m, int d, int firstsweep, 
     long long corr_blocks = nb;
     // with one boundary type per face the kernels write their outputs' ghost cells
     const int ghosts = (g->bc_uniform && g->use_fused) ? 1 : 0;
-    if ((rc = launch_pred(g, m, d, s, win, ghosts))) return rc;
+    const int lim = common_limiter(g, m);
+    // FP32-bulk on 3D grids with copy-type faces: the predictor stores w1 as
+    // its FP32 increment over win and the corrector adds it back to win in
+    // FP64 (the limiter's base must be win: typelimited 'original'), which
+    // moves 4 instead of 8 bytes per w1 variable each way
+    const bool dw = g->cfg.precision == VACGPU_FP32BULK && g->cfg.ndim == 3 && ghosts &&
+                    lim >= 0 && m->typelimited == VACGPU_LIM_ORIGINAL && !m->useprimitive;
+    if ((rc = launch_pred(g, m, d, s, win, ghosts, dw))) return rc;
     if (!ghosts && (rc = bound(g, g->w1))) return rc;
     {
       // face-parallel corrector: x sweep one face per lane, y / z sweeps marching
       Timed t(g, KF_CORR);
-      const int lim = common_limiter(g, m);
       long long nbc = 0;
 #define LIMSW(X) do { if (lim == LIM_MINMOD) X(LIM_MINMOD); else if (lim == LIM_WOODWARD) X(LIM_WOODWARD); else if (lim == LIM_SUPERBEE) X(LIM_SUPERBEE); else X(-1); } while (0)
+#define LIMSW3(X) do { if (lim == LIM_MINMOD) X(LIM_MINMOD); else if (lim == LIM_WOODWARD) X(LIM_WOODWARD); else X(LIM_SUPERBEE); } while (0)
       if (d == 0) {
         const dim3 gr((g->geo.n[0] + CX_STRIDE - 1) / CX_STRIDE, g->grid.y, g->grid.z);
         nbc = (long long)gr.x * gr.y * gr.z;
 #define CX(LV) hipLaunchKernelGGL((k_tvdlf_corrx<ND, MHD, R, LV>), gr, g->block, 0, g->stream, g->geo, s, g->bc, win, g->w1, wout, g->sc, g->part, g->pstride, ghosts, wl)
-        if (m->useprimitive)
+#define CXW(LV) hipLaunchKernelGGL((k_tvdlf_corrx<ND, MHD, R, LV, false, true>), gr, g->block, 0, g->stream, g->geo, s, g->bc, win, g->w1, wout, g->sc, g->part, g->pstride, ghosts, wl)
+        if (dw)
+          DISPATCH(g, {
+            if constexpr (ND == 3 && std::is_same<R, float>::value) LIMSW3(CXW);
+          });
+        else if (m->useprimitive)
           DISPATCH(g, {
             if constexpr (std::is_same<R, double>::value)
               hipLaunchKernelGGL((k_tvdlf_corrx<ND, MHD, R, -1, true>), gr, g->block, 0, g->stream, g->geo, s, g->bc, win, g->w1, wout, g->sc, g->part, g->pstride, ghosts, wl);
           });
         else
           DISPATCH(g, { LIMSW(CX); });
+#undef CXW
 #undef CX
       } else {
         const dim3 gr = g->grid_tm[d];
@@ -687,7 +714,15 @@ int sweep_device(vacgpu_grid* g, const vacgpu_method* m, int d, int firstsweep, 
 #define CM1(LV) CM(1, LV)
 #define CM2(LV) CM((ND == 3 ? 2 : 1), LV)
 #define CMP(DV) hipLaunchKernelGGL((k_tvdlf_corrm<ND, MHD, R, DV, -1, true>), gr, g->block, 0, g->stream, g->geo, s, g->bc, win, g->w1, wout, g->sc, g->part, g->pstride, g->seglen, ghosts, wl)
-        if (m->useprimitive)
+#define CMW1(LV) hipLaunchKernelGGL((k_tvdlf_corrm<ND, MHD, R, 1, LV, false, true>), gr, g->block, 0, g->stream, g->geo, s, g->bc, win, g->w1, wout, g->sc, g->part, g->pstride, g->seglen, ghosts, wl)
+#define CMW2(LV) hipLaunchKernelGGL((k_tvdlf_corrm<ND, MHD, R, 2, LV, false, true>), gr, g->block, 0, g->stream, g->geo, s, g->bc, win, g->w1, wout, g->sc, g->part, g->pstride, g->seglen, ghosts, wl)
+        if (dw)
+          DISPATCH(g, {
+            if constexpr (ND == 3 && std::is_same<R, float>::value) {
+              if (d == 1) LIMSW3(CMW1); else LIMSW3(CMW2);
+            }
+          });
+        else if (m->useprimitive)
           DISPATCH(g, {
             if constexpr (std::is_same<R, double>::value) {
               if (d == 1) CMP(1); else CMP((ND == 3 ? 2 : 1));
@@ -695,11 +730,14 @@ int sweep_device(vacgpu_grid* g, const vacgpu_method* m, int d, int firstsweep, 
           });
         else
           DISPATCH(g, { if (d == 1) LIMSW(CM1); else LIMSW(CM2); });
+#undef CMW2
+#undef CMW1
 #undef CMP
 #undef CM2
 #undef CM1
 #undef CM
       }
+#undef LIMSW3
 #undef LIMSW
       if ((rc = launch_check(g, "k_tvdlf_corr"))) return rc;
       corr_blocks = nbc;
