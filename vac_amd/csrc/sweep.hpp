@@ -2490,6 +2490,28 @@ __global__ __launch_bounds__(256) void k_cmaxdx(Geo g, const double* __restrict_
   if (threadIdx.x == 0 && threadIdx.y == 0) part[(SLOT_DX + d) * pstride + block_linear()] = b;
 }
 
+// k_cmaxdx for every dimension without a lagged dtcourant in one pass over
+// w (the first getdt of a run): the same per-cell values and block partials
+template <int ND, bool MHD>
+__global__ __launch_bounds__(256) void k_cmaxdx_all(Geo g, const double* __restrict__ w, double gamma,
+                                                   const Scal* __restrict__ sc,
+                                                   double* __restrict__ part, long long pstride) {
+  using L = Layout<ND, MHD>;
+  __shared__ double sm[8];
+  const CellIx cl = cell_at(g, (int)blockIdx.x, (int)blockIdx.y, (int)blockIdx.z);
+  double u[L::nw];
+  if (cl.ok) load<L::nw>(w, g.ps, cl.idx, u);
+  const double smallp = sc->smallp;
+#pragma unroll
+  for (int d = 0; d < ND; ++d) {
+    if (sc->dtcourant[d] < kBig) continue;  // uniform
+    const double v = cl.ok ? pdiv(cmax<ND, MHD, double>(u, d, gamma, smallp), g.dx[d][cl.c[d]])
+                           : -INFINITY;
+    const double b = block_reduce<true>(v, sm);
+    if (threadIdx.x == 0 && threadIdx.y == 0) part[(SLOT_DX + d) * pstride + block_linear()] = b;
+  }
+}
+
 // Block-wide max / min of p[0..n).  Each thread keeps four running values
 // and so four loads in flight (a one-block fold is bound by load latency);
 // max and min do not depend on the order of the values.

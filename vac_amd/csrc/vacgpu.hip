@@ -350,9 +350,8 @@ int getdt_device(vacgpu_grid* g, const vacgpu_method* m, double dtpar, int tmaxe
       hipLaunchKernelGGL(k_fin_smallp, dim3(1), dim3(1024), 0, g->stream, g->sc, g->part,
                          g->pstride, nb, m->smallpcoeff, (int)mhd);
       if (dtpar <= 0.0)
-        for (int d = 0; d < ND; ++d)
-          hipLaunchKernelGGL((k_cmaxdx<ND, MHD>), g->grid, g->block, 0, g->stream, g->geo, w,
-                             g->cfg.eqpar[0], g->sc, g->part, g->pstride, d);
+        hipLaunchKernelGGL((k_cmaxdx_all<ND, MHD>), g->grid, g->block, 0, g->stream, g->geo, w,
+                           g->cfg.eqpar[0], g->sc, g->part, g->pstride);
     });
     hipLaunchKernelGGL(k_fin_getdt, dim3(1), dim3(1024), 0, g->stream, g->sc, g->part, g->pstride,
                        nb, g->cfg.ndim, m->courantpar, dtpar, tmaxexact);
