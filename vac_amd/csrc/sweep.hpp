@@ -2029,12 +2029,43 @@ constexpr int TX_STRIDE = 61;  // cells updated per 64-lane wavefront along x
 
 // x sweep: lane L of a wavefront owns face J = x0 - 2 + L (between cells J and
 // J+1) and updates cell J for L in [2, 62].
+// Courant maxima of the TVD sweeps (tvdlimit2's two extreme waves): each
+// block takes its two maxima into one of CS_N words per wave family and
+// sweep direction (rows 2d and 2d + 1 of the word array, words 128 bytes
+// apart, chosen by block number) with a device-scope atomic max on the bit
+// pattern (non-negative doubles order as their bit patterns; a maximum not
+// above zero, -inf or NaN enters as +0, which no use of the maximum tells
+// apart: cm > 1 and cm > smalldouble are false either way).  A fold then
+// reads 2 x CS_N words instead of one partial per block, and clears them.
+constexpr int CS_N = 64, CS_PAD = 16;  // words per row, u64 per 128-byte line
+__device__ __forceinline__ void cs_max(unsigned long long* __restrict__ cs, int row, double v) {
+  const unsigned long long bits = v > 0.0 ? (unsigned long long)__double_as_longlong(v) : 0ull;
+  __hip_atomic_fetch_max(cs + ((long long)row * CS_N + (block_linear() & (CS_N - 1))) * CS_PAD, bits,
+                         __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+// The maxima of the rows k < nrows with bit k of rowmask set, one wave per
+// row (CS_N is the wave size): wave k reads and clears its row's words, one
+// per lane, and leaves the maximum in sm[k]; a one-dimensional block of at
+// least nrows waves, all threads calling.
+__device__ __forceinline__ void cs_fold_rows(unsigned long long* __restrict__ cs, int nrows,
+                                             unsigned rowmask, double* sm) {
+  static_assert(CS_N == 64, "one word per lane of a wave");
+  const int wave = (int)threadIdx.x >> 6, lane = (int)threadIdx.x & 63;
+  if (wave < nrows && ((rowmask >> wave) & 1u)) {  // wave-uniform
+    double v = __longlong_as_double((long long)__hip_atomic_exchange(
+        cs + ((long long)wave * CS_N + lane) * CS_PAD, 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+    v = wave_max(v);
+    if (lane == 0) sm[wave] = v;
+  }
+  __syncthreads();
+}
+
 template <int ND, class R, int LIM = -1, int SRC = 0, bool MHD = true, bool ENT = false>
 __global__ __launch_bounds__(256) void k_tvd_x(Geo g, Sel s, BC bc, const double* __restrict__ wct,
                                               const double* __restrict__ win,
                                               double* __restrict__ wout,
                                               const Scal* __restrict__ sc,
-                                              double* __restrict__ part, long long pstride,
+                                              unsigned long long* __restrict__ cs,
                                               const double* __restrict__ wold,
                                               double* __restrict__ wt, int ghosts) {
   using L = Layout<ND, MHD>;
@@ -2116,9 +2147,9 @@ __global__ __launch_bounds__(256) void k_tvd_x(Geo g, Sel s, BC bc, const double
     if constexpr (SRC == 2) ryu_first_half<ND>(g, bc, wold, wt, idx, c, dt, out, ghosts);
   }
   const double b0 = block_reduce<true>(c0, sm);
-  if (threadIdx.x == 0 && threadIdx.y == 0) part[SLOT_C0 * pstride + block_linear()] = b0;
+  if (threadIdx.x == 0 && threadIdx.y == 0) cs_max(cs, 0, b0);
   const double b1 = block_reduce<true>(c1, sm);
-  if (threadIdx.x == 0 && threadIdx.y == 0) part[SLOT_C1 * pstride + block_linear()] = b1;
+  if (threadIdx.x == 0 && threadIdx.y == 0) cs_max(cs, 0 + 1, b1);
 }
 
 // y / z sweep: thread (x, segment) marches through cells s0..s1 along d.
@@ -2127,7 +2158,7 @@ __global__ __launch_bounds__(256) void k_tvd_m(Geo g, Sel s, BC bc, const double
                                               const double* __restrict__ win,
                                               double* __restrict__ wout,
                                               const Scal* __restrict__ sc,
-                                              double* __restrict__ part, long long pstride,
+                                              unsigned long long* __restrict__ cs,
                                               int seglen, const double* __restrict__ wold,
                                               double* __restrict__ wt, int ghosts) {
   // (ghost cells of the output are left to a boundary pass: writing them here
@@ -2248,9 +2279,9 @@ __global__ __launch_bounds__(256) void k_tvd_m(Geo g, Sel s, BC bc, const double
     }
   }
   const double b0 = block_reduce<true>(c0, sm);
-  if (threadIdx.x == 0 && threadIdx.y == 0) part[SLOT_C0 * pstride + block_linear()] = b0;
+  if (threadIdx.x == 0 && threadIdx.y == 0) cs_max(cs, 2 * D, b0);
   const double b1 = block_reduce<true>(c1, sm);
-  if (threadIdx.x == 0 && threadIdx.y == 0) part[SLOT_C1 * pstride + block_linear()] = b1;
+  if (threadIdx.x == 0 && threadIdx.y == 0) cs_max(cs, 2 * D + 1, b1);
 }
 
 // k_ryu_ab on 2D grids as a march along y: lanes cover a strip of 62 x
@@ -2608,12 +2639,47 @@ __global__ void k_fin_getdt(Scal* sc, const double* part, long long pstride, lon
 // missing (a pressure floor that stays negative, a sweep whose Courant
 // maximum was below smalldouble), the block computes it itself over the mesh
 // -- slow, but exactly what the multi-block passes compute.
+// tvdlimit2's dtcourant update (src/vactvd.t:145-157) from one Courant
+// maximum of a TVD sweep in direction d (thread 0)
+__device__ __forceinline__ void tvd_courant_apply(Scal* sc, int d, double courantpar, double cm) {
+  if (cm > 1.0) sc->nerror[2] += 1;
+  if (cm > kSmall) {
+    double c = sc->dt * courantpar / cm;
+    sc->dtcourant[d] = c < sc->dtcourant[d] ? c : sc->dtcourant[d];
+  }
+}
+
+// With pend_mask set, the TVD sweeps of the previous device-loop step left
+// their Courant maxima in the words cs (directions in pend_mask): they are
+// folded first, as k_fin_courant_tvd_cs would have after each sweep, and
+// with pend_step_end the previous step's it and t advance; nothing between
+// those sweeps and this launch reads dtcourant, nerror, it or t.
 template <int ND, bool MHD>
 __global__ __launch_bounds__(1024) void k_getdt_lagged(Geo g, const double* __restrict__ w, double gamma,
                                                       Scal* sc, double smallpcoeff, double courantpar,
-                                                      double dtpar, int tmaxexact) {
+                                                      double dtpar, int tmaxexact,
+                                                      unsigned long long* cs, int pend_mask,
+                                                      int pend_step_end) {
   using L = Layout<ND, MHD>;
   __shared__ double sm[16];
+  if (pend_mask) {  // uniform
+    unsigned rows = 0u;
+    for (int d = 0; d < ND; ++d)
+      if ((pend_mask >> d) & 1) rows |= 3u << (2 * d);
+    cs_fold_rows(cs, 2 * ND, rows, sm);
+    if (threadIdx.x == 0) {
+      for (int d = 0; d < ND; ++d) {
+        if (!((pend_mask >> d) & 1)) continue;
+        tvd_courant_apply(sc, d, courantpar, sm[2 * d]);
+        tvd_courant_apply(sc, d, courantpar, sm[2 * d + 1]);
+      }
+    }
+    if (threadIdx.x == 0 && pend_step_end) {
+      sc->it += 1;
+      sc->t = sc->t + sc->dt;
+    }
+    __syncthreads();
+  }
   const long long ncell = (long long)g.n[0] * g.n[1] * (ND == 3 ? g.n[2] : 1);
   auto cell = [&](long long t, int* c) {
     c[0] = 2 + (int)(t % g.n[0]);
@@ -2700,6 +2766,23 @@ __global__ void k_fin_courant_tvd(Scal* sc, const double* part, long long pstrid
   if (step_end && threadIdx.x == 0) {
     sc->it += 1;
     sc->t = sc->t + sc->dt;
+  }
+}
+
+// k_fin_courant_tvd for a TVD sweep's Courant words (cs_max): both extreme
+// waves of direction d in VAC's order, and on the last sweep of a
+// device-loop step the step's it and t
+__global__ __launch_bounds__(384) void k_fin_courant_tvd_cs(Scal* sc, unsigned long long* cs, int d,
+                                                           double courantpar, int step_end) {
+  __shared__ double sm[6];
+  cs_fold_rows(cs, 6, 3u << (2 * d), sm);
+  if (threadIdx.x == 0) {
+    tvd_courant_apply(sc, d, courantpar, sm[2 * d]);
+    tvd_courant_apply(sc, d, courantpar, sm[2 * d + 1]);
+    if (step_end) {
+      sc->it += 1;
+      sc->t = sc->t + sc->dt;
+    }
   }
 }
 

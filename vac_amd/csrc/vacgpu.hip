@@ -73,6 +73,13 @@ struct vacgpu_grid {
   dim3 grid_ry;
   double* stage = nullptr;  // pinned staging of put_state / get_state
   unsigned long long* fold = nullptr;  // Courant fold words of the fused sweeps (zero between launches)
+  // Courant words of the TVD sweeps (cs_max; 6 rows of CS_N, zero between folds)
+  unsigned long long* cslot = nullptr;
+  // device loop without a per-step host sync: the TVD sweeps' Courant folds
+  // (and the step's it / t) are left to the next step's k_getdt_lagged
+  bool defer_folds = false, pend_step_end = false;
+  int pend_mask = 0;
+  double pend_courantpar = 0.0;
   // device loop: the last fused sweep of a step also advances it and t
   bool fold_step_end = false, step_end_done = false;
   size_t stage_bytes = 0;
@@ -444,9 +451,24 @@ int getdt_lagged(vacgpu_grid* g, const vacgpu_method* m, double dtpar, int tmaxe
     (void)sizeof(R);
     hipLaunchKernelGGL((k_getdt_lagged<ND, MHD>), dim3(1), dim3(1024), 0, g->stream, g->geo,
                        g->buf[g->cur], g->cfg.eqpar[0], g->sc, m->smallpcoeff, m->courantpar, dtpar,
-                       tmaxexact);
+                       tmaxexact, g->cslot, g->pend_mask, g->pend_step_end ? 1 : 0);
   });
+  g->pend_mask = 0;
+  g->pend_step_end = false;
   return launch_check(g, "k_getdt_lagged");
+}
+
+// folds that a device loop left pending when it stopped before the next
+// step's getdt (an error in a step): applied as the sweeps would have
+void flush_folds(vacgpu_grid* g) {
+  for (int d = 0; d < 3; ++d) {
+    if (!((g->pend_mask >> d) & 1)) continue;
+    const bool last = (g->pend_mask >> (d + 1)) == 0;
+    hipLaunchKernelGGL(k_fin_courant_tvd_cs, dim3(1), dim3(384), 0, g->stream, g->sc, g->cslot, d,
+                       g->pend_courantpar, (last && g->pend_step_end) ? 1 : 0);
+  }
+  g->pend_mask = 0;
+  g->pend_step_end = false;
 }
 
 // the limiter type shared by every variable / characteristic wave, or -1
@@ -794,8 +816,8 @@ int sweep_device(vacgpu_grid* g, const vacgpu_method* m, int d, int firstsweep, 
         // entropy fixes: limiter types read at run time, source steps unfolded
         const dim3 gr = d == 0 ? g->grid_tx : g->grid_tm[d];
         nbt = (long long)gr.x * gr.y * gr.z;
-#define EX(NDV, RT, MH) hipLaunchKernelGGL((k_tvd_x<NDV, RT, -1, 0, MH, true>), gr, g->block, 0, g->stream, g->geo, s, g->bc, wct, win, wout, g->sc, g->part, g->pstride, wold, g->w1, ghosts)
-#define EM(NDV, RT, DV, MH) hipLaunchKernelGGL((k_tvd_m<NDV, RT, DV, -1, 0, MH, true>), gr, g->block, 0, g->stream, g->geo, s, g->bc, wct, win, wout, g->sc, g->part, g->pstride, g->seglen, wold, g->w1, ghosts)
+#define EX(NDV, RT, MH) hipLaunchKernelGGL((k_tvd_x<NDV, RT, -1, 0, MH, true>), gr, g->block, 0, g->stream, g->geo, s, g->bc, wct, win, wout, g->sc, g->cslot, wold, g->w1, ghosts)
+#define EM(NDV, RT, DV, MH) hipLaunchKernelGGL((k_tvd_m<NDV, RT, DV, -1, 0, MH, true>), gr, g->block, 0, g->stream, g->geo, s, g->bc, wct, win, wout, g->sc, g->cslot, g->seglen, wold, g->w1, ghosts)
 #define EPH(X2, X3) do { if (g->cfg.ndim == 2) { if (f32) X2(float); else X2(double); } else { if (f32) X3(float); else X3(double); } } while (0)
 #define EX2M(RT) EX(2, RT, true)
 #define EX3M(RT) EX(3, RT, true)
@@ -822,8 +844,8 @@ int sweep_device(vacgpu_grid* g, const vacgpu_method* m, int d, int firstsweep, 
         // HD: the HD Roe solver, no source steps
         const dim3 gr = d == 0 ? g->grid_tx : g->grid_tm[d];
         nbt = (long long)gr.x * gr.y * gr.z;
-#define HX(NDV, RT, LV) hipLaunchKernelGGL((k_tvd_x<NDV, RT, LV, 0, false>), gr, g->block, 0, g->stream, g->geo, s, g->bc, wct, win, wout, g->sc, g->part, g->pstride, wold, g->w1, ghosts)
-#define HM(NDV, RT, DV, LV) hipLaunchKernelGGL((k_tvd_m<NDV, RT, DV, LV, 0, false>), gr, g->block, 0, g->stream, g->geo, s, g->bc, wct, win, wout, g->sc, g->part, g->pstride, g->seglen, wold, g->w1, ghosts)
+#define HX(NDV, RT, LV) hipLaunchKernelGGL((k_tvd_x<NDV, RT, LV, 0, false>), gr, g->block, 0, g->stream, g->geo, s, g->bc, wct, win, wout, g->sc, g->cslot, wold, g->w1, ghosts)
+#define HM(NDV, RT, DV, LV) hipLaunchKernelGGL((k_tvd_m<NDV, RT, DV, LV, 0, false>), gr, g->block, 0, g->stream, g->geo, s, g->bc, wct, win, wout, g->sc, g->cslot, g->seglen, wold, g->w1, ghosts)
 #define HX2D(LV) HX(2, double, LV)
 #define HX2F(LV) HX(2, float, LV)
 #define HX3D(LV) HX(3, double, LV)
@@ -855,7 +877,7 @@ int sweep_device(vacgpu_grid* g, const vacgpu_method* m, int d, int firstsweep, 
       } else if (d == 0) {
         const dim3 gr = g->grid_tx;
         nbt = (long long)gr.x * gr.y * gr.z;
-#define TVDX_L(NDV, RT, LV, SV) hipLaunchKernelGGL((k_tvd_x<NDV, RT, LV, SV>), gr, g->block, 0, g->stream, g->geo, s, g->bc, wct, win, wout, g->sc, g->part, g->pstride, wold, g->w1, ghosts)
+#define TVDX_L(NDV, RT, LV, SV) hipLaunchKernelGGL((k_tvd_x<NDV, RT, LV, SV>), gr, g->block, 0, g->stream, g->geo, s, g->bc, wct, win, wout, g->sc, g->cslot, wold, g->w1, ghosts)
 #define X2D(LV) TVDX_L(2, double, LV, 0)
 #define X2F(LV) TVDX_L(2, float, LV, 0)
 #define X3D(LV) TVDX_L(3, double, LV, 0)
@@ -881,7 +903,7 @@ int sweep_device(vacgpu_grid* g, const vacgpu_method* m, int d, int firstsweep, 
       } else {
         const dim3 gr = g->grid_tm[d];
         nbt = (long long)gr.x * gr.y * gr.z;
-#define TVDM_L(NDV, RT, DV, LV, SV) hipLaunchKernelGGL((k_tvd_m<NDV, RT, DV, LV, SV>), gr, g->block, 0, g->stream, g->geo, s, g->bc, wct, win, wout, g->sc, g->part, g->pstride, g->seglen, wold, g->w1, ghosts)
+#define TVDM_L(NDV, RT, DV, LV, SV) hipLaunchKernelGGL((k_tvd_m<NDV, RT, DV, LV, SV>), gr, g->block, 0, g->stream, g->geo, s, g->bc, wct, win, wout, g->sc, g->cslot, g->seglen, wold, g->w1, ghosts)
 #define M21D(LV) TVDM_L(2, double, 1, LV, 0)
 #define M21F(LV) TVDM_L(2, float, 1, LV, 0)
 #define M21D1(LV) TVDM_L(2, double, 1, LV, 1)
@@ -911,14 +933,22 @@ int sweep_device(vacgpu_grid* g, const vacgpu_method* m, int d, int firstsweep, 
       if ((rc = launch_check(g, "k_tvd_sweep"))) return rc;
     }
     if (courant) {
-      Timed t(g, KF_RED);
       // (the Ryu step that may follow reads dt only)
       const int step_end = (g->fold_step_end && lastsweep) ? 1 : 0;
       if (step_end) g->step_end_done = true;
-      hipLaunchKernelGGL(k_fin_courant_tvd, dim3(1), dim3(1024), 0, g->stream, g->sc, g->part,
-                         g->pstride, nbt, d, m->courantpar, step_end);
-      if ((rc = launch_check(g, "k_fin_courant_tvd"))) return rc;
+      if (g->defer_folds) {
+        // the next step's k_getdt_lagged folds the words and ends this step
+        g->pend_mask |= 1 << d;
+        g->pend_courantpar = m->courantpar;
+        if (step_end) g->pend_step_end = true;
+      } else {
+        Timed t(g, KF_RED);
+        hipLaunchKernelGGL(k_fin_courant_tvd_cs, dim3(1), dim3(384), 0, g->stream, g->sc, g->cslot, d,
+                           m->courantpar, step_end);
+        if ((rc = launch_check(g, "k_fin_courant_tvd_cs"))) return rc;
+      }
     }
+    (void)nbt;
     const bool ryu_folded = srcmode == 2;
     if (lastsweep && src && ghosts && !ryu_folded && !firstsweep) {
       // both Ryu halves in one pass, into the buffer of this sweep's input
@@ -1420,6 +1450,8 @@ int vacgpu_create(const vacgpu_config* cfg, const double* dx0, const double* dx1
   if (e == hipSuccess) e = hipMalloc(&g->part, sizeof(double) * NSLOT * g->pstride);
   if (e == hipSuccess) e = hipMalloc(&g->fold, 2 * sizeof(unsigned long long));
   if (e == hipSuccess) e = hipMemset(g->fold, 0, 2 * sizeof(unsigned long long));
+  if (e == hipSuccess) e = hipMalloc(&g->cslot, 6 * CS_N * CS_PAD * sizeof(unsigned long long));
+  if (e == hipSuccess) e = hipMemset(g->cslot, 0, 6 * CS_N * CS_PAD * sizeof(unsigned long long));
   if (e == hipSuccess) e = hipStreamCreateWithFlags(&g->stream, hipStreamNonBlocking);
   if (e != hipSuccess) {
     fprintf(stderr, "vacgpu_create: %s\n", hipGetErrorString(e));
@@ -1663,6 +1695,16 @@ int vacgpu_run(vacgpu_grid* g, const vacgpu_method* m, int nsteps, int itmax, do
   const int itmin = g->sc_host->itmin;
   double t = g->sc_host->t;
   const bool sync_each = (tmax < kBig) || t_hist;
+  // without a host sync per step, the TVD sweeps' Courant folds wait for
+  // the next step's getdt; whatever way the loop ends, none is left behind
+  struct Defer {
+    vacgpu_grid* g;
+    ~Defer() {
+      g->defer_folds = false;
+      if (g->pend_mask) flush_folds(g);
+    }
+  } defer{g};
+  g->defer_folds = !sync_each;
   int done = 0;
   for (;;) {
     if (t_hist) t_hist[done] = t;
@@ -1888,6 +1930,7 @@ void vacgpu_destroy(vacgpu_grid* g) {
   if (g->stage) (void)hipHostFree(g->stage);
   if (g->part) (void)hipFree(g->part);
   if (g->fold) (void)hipFree(g->fold);
+  if (g->cslot) (void)hipFree(g->cslot);
   if (g->fixw) (void)hipFree(g->fixw);
   {
     auto& P = g->pr;
