@@ -242,23 +242,25 @@ def test_courant_violation_counted_like_vac(typefull, limiter):
 
 
 @pytest.mark.parametrize("case", [
-    ("mhd", (96, 80), "periodic", "woodward", 0.8, 12),
-    ("mhd", (52, 44), "periodic", "minmod", 2.0, 4),
-    ("hd", (60, 48), "cont", "superbee", 0.8, 10),
-    ("mhd", (20, 16, 12), "periodic", "woodward", 0.8, 6),
-], ids=["mhd2d", "mhd2d-cfl-violation", "hd2d-cont", "mhd3d"])
+    ("mhd", (96, 80), "periodic", "woodward", 0.8, 12, "tvd"),
+    ("mhd", (52, 44), "periodic", "minmod", 2.0, 4, "tvd"),
+    ("hd", (60, 48), "cont", "superbee", 0.8, 10, "tvd"),
+    ("mhd", (20, 16, 12), "periodic", "woodward", 0.8, 6, "tvd"),
+    ("mhd", (70, 58), "periodic", "woodward", 0.8, 10, "tvdmu"),
+    ("mhd", (52, 44), "cont", "minmod", 2.0, 4, "tvdmu"),
+], ids=["mhd2d", "mhd2d-cfl-violation", "hd2d-cont", "mhd3d", "tvdmu-mhd2d", "tvdmu-cfl-violation"])
 def test_tvd_device_loop_without_host_sync_matches_oracle(case):
     # with neither a t history nor a tmax the device loop does not read the
-    # scalars between steps, and each TVD sweep's Courant fold (with the
+    # scalars between steps, and each TVD / TVD-MUSCL sweep's Courant fold (with the
     # step's it / t) is left to the next step's getdt: the state, it, t and
     # the error counts must still be VAC's (courantpar 2 makes every fold
     # count a violation)
-    physics, nx, bc, limiter, cp, steps = case
+    physics, nx, bc, limiter, cp, steps, typefull = case
     g = 5.0 / 3.0 if physics == "mhd" else 1.4
     w0, coords = random_fourier(nx, 31 + steps, g, physics=physics)
     dxs = [widths_1d(c) for c in coords]
-    method = vacgpu.MethodSpec(typefull="tvd", limiter=limiter, courantpar=cp)
-    print(f"case tvd deferred folds {physics} {nx} {bc} {limiter} cfl {cp}", flush=True)
+    method = vacgpu.MethodSpec(typefull=typefull, limiter=limiter, courantpar=cp)
+    print(f"case {typefull} deferred folds {physics} {nx} {bc} {limiter} cfl {cp}", flush=True)
     grid = vacgpu.Grid(list(nx), physics, g, dxs, bc=bc)
     grid.put_mesh(w0)
     n, _ = grid.run(method, steps)
@@ -266,7 +268,7 @@ def test_tvd_device_loop_without_host_sync_matches_oracle(case):
     sc = grid.scalars()
     it, t = grid.get_time()
     grid.close()
-    p = vacref.make_params(len(nx), physics, list(nx), g, typefull="tvd", limiter=limiter,
+    p = vacref.make_params(len(nx), physics, list(nx), g, typefull=typefull, limiter=limiter,
                            courantpar=cp, bc=bc)
     wr, tr, st = vacref.run(p, w0, dxs, steps)
     assert n == steps and it == steps

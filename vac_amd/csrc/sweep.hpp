@@ -1851,20 +1851,51 @@ __device__ __forceinline__ void tvdmu_x_body(const Geo& g, const Sel& s, const B
   c1o = c1;
 }
 
+// Courant maxima of the TVD sweeps (tvdlimit2's two extreme waves): each
+// block takes its two maxima into one of CS_N words per wave family and
+// sweep direction (rows 2d and 2d + 1 of the word array, words 128 bytes
+// apart, chosen by block number) with a device-scope atomic max on the bit
+// pattern (non-negative doubles order as their bit patterns; a maximum not
+// above zero, -inf or NaN enters as +0, which no use of the maximum tells
+// apart: cm > 1 and cm > smalldouble are false either way).  A fold then
+// reads 2 x CS_N words instead of one partial per block, and clears them.
+constexpr int CS_N = 64, CS_PAD = 16;  // words per row, u64 per 128-byte line
+__device__ __forceinline__ void cs_max(unsigned long long* __restrict__ cs, int row, double v) {
+  const unsigned long long bits = v > 0.0 ? (unsigned long long)__double_as_longlong(v) : 0ull;
+  __hip_atomic_fetch_max(cs + ((long long)row * CS_N + (block_linear() & (CS_N - 1))) * CS_PAD, bits,
+                         __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+// The maxima of the rows k < nrows with bit k of rowmask set, one wave per
+// row (CS_N is the wave size): wave k reads and clears its row's words, one
+// per lane, and leaves the maximum in sm[k]; a one-dimensional block of at
+// least nrows waves, all threads calling.
+__device__ __forceinline__ void cs_fold_rows(unsigned long long* __restrict__ cs, int nrows,
+                                             unsigned rowmask, double* sm) {
+  static_assert(CS_N == 64, "one word per lane of a wave");
+  const int wave = (int)threadIdx.x >> 6, lane = (int)threadIdx.x & 63;
+  if (wave < nrows && ((rowmask >> wave) & 1u)) {  // wave-uniform
+    double v = __longlong_as_double((long long)__hip_atomic_exchange(
+        cs + ((long long)wave * CS_N + lane) * CS_PAD, 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+    v = wave_max(v);
+    if (lane == 0) sm[wave] = v;
+  }
+  __syncthreads();
+}
+
 template <int ND, class R, int LIM, bool MHD = true, bool ENT = false, bool PRIM = false>
 __global__ __launch_bounds__(256) void k_tvdmu_x(Geo g, Sel s, BC bc, const double* __restrict__ w,
                                                 const double* __restrict__ w1,
                                                 double* __restrict__ wout,
                                                 const Scal* __restrict__ sc,
-                                                double* __restrict__ part, long long pstride,
+                                                unsigned long long* __restrict__ cs,
                                                 int ghosts, const double* __restrict__ wl) {
   __shared__ double sm[8];
   double c0, c1;
   tvdmu_x_body<ND, R, LIM, MHD, ENT, PRIM>(g, s, bc, w, w1, wout, ghosts, wl, sc->dt, sc->smallp, xcd_tile(), c0, c1);
   const double b0 = block_reduce<true>(c0, sm);
-  if (threadIdx.x == 0 && threadIdx.y == 0) part[SLOT_C0 * pstride + block_linear()] = b0;
+  if (threadIdx.x == 0 && threadIdx.y == 0) cs_max(cs, 0, b0);
   const double b1 = block_reduce<true>(c1, sm);
-  if (threadIdx.x == 0 && threadIdx.y == 0) part[SLOT_C1 * pstride + block_linear()] = b1;
+  if (threadIdx.x == 0 && threadIdx.y == 0) cs_max(cs, 0 + 1, b1);
 }
 
 // y / z sweep: thread (x, segment) marches through cells s0..s1 along D,
@@ -2013,52 +2044,22 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu((ENT || PRI
                                                 const double* __restrict__ w1,
                                                 double* __restrict__ wout,
                                                 const Scal* __restrict__ sc,
-                                                double* __restrict__ part, long long pstride,
+                                                unsigned long long* __restrict__ cs,
                                                 int seglen, int ghosts,
                                                 const double* __restrict__ wl) {
   __shared__ double sm[8];
   double c0, c1;
   tvdmu_m_body<ND, R, D, LIM, MHD, ENT, PRIM>(g, s, bc, w, w1, wout, seglen, ghosts, wl, sc->dt, sc->smallp, xcd_tile(), c0, c1);
   const double b0 = block_reduce<true>(c0, sm);
-  if (threadIdx.x == 0 && threadIdx.y == 0) part[SLOT_C0 * pstride + block_linear()] = b0;
+  if (threadIdx.x == 0 && threadIdx.y == 0) cs_max(cs, 2 * D, b0);
   const double b1 = block_reduce<true>(c1, sm);
-  if (threadIdx.x == 0 && threadIdx.y == 0) part[SLOT_C1 * pstride + block_linear()] = b1;
+  if (threadIdx.x == 0 && threadIdx.y == 0) cs_max(cs, 2 * D + 1, b1);
 }
 
 constexpr int TX_STRIDE = 61;  // cells updated per 64-lane wavefront along x
 
 // x sweep: lane L of a wavefront owns face J = x0 - 2 + L (between cells J and
 // J+1) and updates cell J for L in [2, 62].
-// Courant maxima of the TVD sweeps (tvdlimit2's two extreme waves): each
-// block takes its two maxima into one of CS_N words per wave family and
-// sweep direction (rows 2d and 2d + 1 of the word array, words 128 bytes
-// apart, chosen by block number) with a device-scope atomic max on the bit
-// pattern (non-negative doubles order as their bit patterns; a maximum not
-// above zero, -inf or NaN enters as +0, which no use of the maximum tells
-// apart: cm > 1 and cm > smalldouble are false either way).  A fold then
-// reads 2 x CS_N words instead of one partial per block, and clears them.
-constexpr int CS_N = 64, CS_PAD = 16;  // words per row, u64 per 128-byte line
-__device__ __forceinline__ void cs_max(unsigned long long* __restrict__ cs, int row, double v) {
-  const unsigned long long bits = v > 0.0 ? (unsigned long long)__double_as_longlong(v) : 0ull;
-  __hip_atomic_fetch_max(cs + ((long long)row * CS_N + (block_linear() & (CS_N - 1))) * CS_PAD, bits,
-                         __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-// The maxima of the rows k < nrows with bit k of rowmask set, one wave per
-// row (CS_N is the wave size): wave k reads and clears its row's words, one
-// per lane, and leaves the maximum in sm[k]; a one-dimensional block of at
-// least nrows waves, all threads calling.
-__device__ __forceinline__ void cs_fold_rows(unsigned long long* __restrict__ cs, int nrows,
-                                             unsigned rowmask, double* sm) {
-  static_assert(CS_N == 64, "one word per lane of a wave");
-  const int wave = (int)threadIdx.x >> 6, lane = (int)threadIdx.x & 63;
-  if (wave < nrows && ((rowmask >> wave) & 1u)) {  // wave-uniform
-    double v = __longlong_as_double((long long)__hip_atomic_exchange(
-        cs + ((long long)wave * CS_N + lane) * CS_PAD, 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
-    v = wave_max(v);
-    if (lane == 0) sm[wave] = v;
-  }
-  __syncthreads();
-}
 
 template <int ND, class R, int LIM = -1, int SRC = 0, bool MHD = true, bool ENT = false>
 __global__ __launch_bounds__(256) void k_tvd_x(Geo g, Sel s, BC bc, const double* __restrict__ wct,
@@ -2748,26 +2749,6 @@ __global__ void k_fin_courant_tvdlf(Scal* sc, const double* part, long long pstr
   }
 }
 
-// dtcourant update of tvdlimit2 for the two extreme waves (src/vactvd.t:145-157)
-__global__ void k_fin_courant_tvd(Scal* sc, const double* part, long long pstride, long long nb,
-                                  int d, double courantpar, int step_end) {
-  __shared__ double sm[16];
-  for (int slot = SLOT_C0; slot <= SLOT_C1; ++slot) {
-    const double cm = fold(part + slot * pstride, nb, true, sm);
-    if (threadIdx.x == 0) {
-      if (cm > 1.0) sc->nerror[2] += 1;
-      if (cm > kSmall) {
-        double c = sc->dt * courantpar / cm;
-        sc->dtcourant[d] = c < sc->dtcourant[d] ? c : sc->dtcourant[d];
-      }
-    }
-    __syncthreads();
-  }
-  if (step_end && threadIdx.x == 0) {
-    sc->it += 1;
-    sc->t = sc->t + sc->dt;
-  }
-}
 
 // k_fin_courant_tvd for a TVD sweep's Courant words (cs_max): both extreme
 // waves of direction d in VAC's order, and on the last sweep of a

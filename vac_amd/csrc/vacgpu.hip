@@ -555,9 +555,9 @@ int launch_tvdmu_corr(vacgpu_grid* g, const vacgpu_method* m, int d, const Sel& 
     if (d == 0) {
       const dim3 gr((g->geo.n[0] + CX_STRIDE - 1) / CX_STRIDE, g->grid.y, g->grid.z);
       nbc = (long long)gr.x * gr.y * gr.z;
-  #define MUX(NDV, RT, LV) do { if (ent) { if (mhd) MUXE(NDV, RT, true); else MUXE(NDV, RT, false); } else if (mhd) hipLaunchKernelGGL((k_tvdmu_x<NDV, RT, LV, true>), gr, g->block, 0, g->stream, g->geo, s, g->bc, win, g->w1, wout, g->sc, g->part, g->pstride, ghosts, wl); \
-                             else hipLaunchKernelGGL((k_tvdmu_x<NDV, RT, LV, false>), gr, g->block, 0, g->stream, g->geo, s, g->bc, win, g->w1, wout, g->sc, g->part, g->pstride, ghosts, wl); } while (0)
-  #define MUXE(NDV, RT, MH) hipLaunchKernelGGL((k_tvdmu_x<NDV, RT, -1, MH, true>), gr, g->block, 0, g->stream, g->geo, s, g->bc, win, g->w1, wout, g->sc, g->part, g->pstride, ghosts, wl)
+  #define MUX(NDV, RT, LV) do { if (ent) { if (mhd) MUXE(NDV, RT, true); else MUXE(NDV, RT, false); } else if (mhd) hipLaunchKernelGGL((k_tvdmu_x<NDV, RT, LV, true>), gr, g->block, 0, g->stream, g->geo, s, g->bc, win, g->w1, wout, g->sc, g->cslot, ghosts, wl); \
+                             else hipLaunchKernelGGL((k_tvdmu_x<NDV, RT, LV, false>), gr, g->block, 0, g->stream, g->geo, s, g->bc, win, g->w1, wout, g->sc, g->cslot, ghosts, wl); } while (0)
+  #define MUXE(NDV, RT, MH) hipLaunchKernelGGL((k_tvdmu_x<NDV, RT, -1, MH, true>), gr, g->block, 0, g->stream, g->geo, s, g->bc, win, g->w1, wout, g->sc, g->cslot, ghosts, wl)
   #define X2D(LV) MUX(2, double, LV)
   #define X2F(LV) MUX(2, float, LV)
   #define X3D(LV) MUX(3, double, LV)
@@ -565,8 +565,8 @@ int launch_tvdmu_corr(vacgpu_grid* g, const vacgpu_method* m, int d, const Sel& 
       if (m->useprimitive) {
         DISPATCH(g, {
           if constexpr (std::is_same<R, double>::value) {
-            if (ent) hipLaunchKernelGGL((k_tvdmu_x<ND, R, -1, MHD, true, true>), gr, g->block, 0, g->stream, g->geo, s, g->bc, win, g->w1, wout, g->sc, g->part, g->pstride, ghosts, wl);
-            else hipLaunchKernelGGL((k_tvdmu_x<ND, R, -1, MHD, false, true>), gr, g->block, 0, g->stream, g->geo, s, g->bc, win, g->w1, wout, g->sc, g->part, g->pstride, ghosts, wl);
+            if (ent) hipLaunchKernelGGL((k_tvdmu_x<ND, R, -1, MHD, true, true>), gr, g->block, 0, g->stream, g->geo, s, g->bc, win, g->w1, wout, g->sc, g->cslot, ghosts, wl);
+            else hipLaunchKernelGGL((k_tvdmu_x<ND, R, -1, MHD, false, true>), gr, g->block, 0, g->stream, g->geo, s, g->bc, win, g->w1, wout, g->sc, g->cslot, ghosts, wl);
           }
         });
       } else if (g->cfg.ndim == 2) { if (f32) LIMSW(X2F); else LIMSW(X2D); }
@@ -580,9 +580,9 @@ int launch_tvdmu_corr(vacgpu_grid* g, const vacgpu_method* m, int d, const Sel& 
     } else {
       const dim3 gr = g->grid_tm[d];
       nbc = (long long)gr.x * gr.y * gr.z;
-  #define MUME(NDV, RT, DV, MH) hipLaunchKernelGGL((k_tvdmu_m<NDV, RT, DV, -1, MH, true>), gr, g->block, 0, g->stream, g->geo, s, g->bc, win, g->w1, wout, g->sc, g->part, g->pstride, g->seglen, ghosts, wl)
-  #define MUM(NDV, RT, DV, LV) do { if (ent) { if (mhd) MUME(NDV, RT, DV, true); else MUME(NDV, RT, DV, false); } else if (mhd) hipLaunchKernelGGL((k_tvdmu_m<NDV, RT, DV, LV, true>), gr, g->block, 0, g->stream, g->geo, s, g->bc, win, g->w1, wout, g->sc, g->part, g->pstride, g->seglen, ghosts, wl); \
-                                 else hipLaunchKernelGGL((k_tvdmu_m<NDV, RT, DV, LV, false>), gr, g->block, 0, g->stream, g->geo, s, g->bc, win, g->w1, wout, g->sc, g->part, g->pstride, g->seglen, ghosts, wl); } while (0)
+  #define MUME(NDV, RT, DV, MH) hipLaunchKernelGGL((k_tvdmu_m<NDV, RT, DV, -1, MH, true>), gr, g->block, 0, g->stream, g->geo, s, g->bc, win, g->w1, wout, g->sc, g->cslot, g->seglen, ghosts, wl)
+  #define MUM(NDV, RT, DV, LV) do { if (ent) { if (mhd) MUME(NDV, RT, DV, true); else MUME(NDV, RT, DV, false); } else if (mhd) hipLaunchKernelGGL((k_tvdmu_m<NDV, RT, DV, LV, true>), gr, g->block, 0, g->stream, g->geo, s, g->bc, win, g->w1, wout, g->sc, g->cslot, g->seglen, ghosts, wl); \
+                                 else hipLaunchKernelGGL((k_tvdmu_m<NDV, RT, DV, LV, false>), gr, g->block, 0, g->stream, g->geo, s, g->bc, win, g->w1, wout, g->sc, g->cslot, g->seglen, ghosts, wl); } while (0)
   #define M21D(LV) MUM(2, double, 1, LV)
   #define M21F(LV) MUM(2, float, 1, LV)
   #define M31D(LV) MUM(3, double, 1, LV)
@@ -590,8 +590,8 @@ int launch_tvdmu_corr(vacgpu_grid* g, const vacgpu_method* m, int d, const Sel& 
   #define M32D(LV) MUM(3, double, 2, LV)
   #define M32F(LV) MUM(3, float, 2, LV)
       if (m->useprimitive) {
-  #define MUP(DV) do { if (ent) hipLaunchKernelGGL((k_tvdmu_m<ND, R, DV, -1, MHD, true, true>), gr, g->block, 0, g->stream, g->geo, s, g->bc, win, g->w1, wout, g->sc, g->part, g->pstride, g->seglen, ghosts, wl); \
-                   else hipLaunchKernelGGL((k_tvdmu_m<ND, R, DV, -1, MHD, false, true>), gr, g->block, 0, g->stream, g->geo, s, g->bc, win, g->w1, wout, g->sc, g->part, g->pstride, g->seglen, ghosts, wl); } while (0)
+  #define MUP(DV) do { if (ent) hipLaunchKernelGGL((k_tvdmu_m<ND, R, DV, -1, MHD, true, true>), gr, g->block, 0, g->stream, g->geo, s, g->bc, win, g->w1, wout, g->sc, g->cslot, g->seglen, ghosts, wl); \
+                   else hipLaunchKernelGGL((k_tvdmu_m<ND, R, DV, -1, MHD, false, true>), gr, g->block, 0, g->stream, g->geo, s, g->bc, win, g->w1, wout, g->sc, g->cslot, g->seglen, ghosts, wl); } while (0)
         DISPATCH(g, {
           if constexpr (std::is_same<R, double>::value) {
             if (d == 1) MUP(1); else MUP((ND == 3 ? 2 : 1));
@@ -679,13 +679,22 @@ int sweep_device(vacgpu_grid* g, const vacgpu_method* m, int d, int firstsweep, 
     long long nbc = 0;
     if ((rc = launch_tvdmu_corr(g, m, d, s, win, wout, wl, ghosts, nbc))) return rc;
     if (courant) {
-      Timed t(g, KF_RED);
+      // (tvdlimit2's check; the Courant words of the TVD sweeps, folded now or
+      // by the next step's getdt)
       const int step_end = (g->fold_step_end && lastsweep) ? 1 : 0;
       if (step_end) g->step_end_done = true;
-      hipLaunchKernelGGL(k_fin_courant_tvd, dim3(1), dim3(1024), 0, g->stream, g->sc, g->part,
-                         g->pstride, nbc, d, m->courantpar, step_end);
-      if ((rc = launch_check(g, "k_fin_courant_tvd"))) return rc;
+      if (g->defer_folds) {
+        g->pend_mask |= 1 << d;
+        g->pend_courantpar = m->courantpar;
+        if (step_end) g->pend_step_end = true;
+      } else {
+        Timed t(g, KF_RED);
+        hipLaunchKernelGGL(k_fin_courant_tvd_cs, dim3(1), dim3(384), 0, g->stream, g->sc, g->cslot, d,
+                           m->courantpar, step_end);
+        if ((rc = launch_check(g, "k_fin_courant_tvd_cs"))) return rc;
+      }
     }
+    (void)nbc;
     if (rt_wold && (rc = roundtrip_wold(g))) return rc;
     if (ghosts) {
       g->cur = outi;
@@ -1058,9 +1067,9 @@ int sweep_unsplit_device(vacgpu_grid* g, const vacgpu_method* m) {
       if ((rc = launch_tvdmu_corr(g, m, d, sd, base, dst, wl, gh, nbc))) return rc;
       if (m->courantpar > 0.0) {
         Timed t(g, KF_RED);
-        hipLaunchKernelGGL(k_fin_courant_tvd, dim3(1), dim3(1024), 0, g->stream, g->sc, g->part,
-                           g->pstride, nbc, d, m->courantpar, 0);
-        if ((rc = launch_check(g, "k_fin_courant_tvd"))) return rc;
+        hipLaunchKernelGGL(k_fin_courant_tvd_cs, dim3(1), dim3(384), 0, g->stream, g->sc, g->cslot, d,
+                           m->courantpar, 0);
+        if ((rc = launch_check(g, "k_fin_courant_tvd_cs"))) return rc;
       }
       if (rt_wold && (rc = roundtrip_wold(g))) return rc;
       continue;
